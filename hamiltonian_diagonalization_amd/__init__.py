@@ -1,0 +1,416 @@
+"""hamiltonian_diagonalization_amd -- MI355X exact-diagonalization hot path.
+
+Python plumbing over the C ABI of ``lib/libhdiag_hip.so`` (declared in
+``include/hdiag.h``).  The classes mirror the reference's interfaces
+(paths relative to the reference repository):
+
+  FockBasis        FermionFockBasis         src/basis/fermion_fock_basis.hpp:45-74
+  TermTables       Quartic/QuadraticTermTables  src/hamiltonians/term_tables_base.hpp:44-174
+  Hamiltonian      SpinlessFermionHamiltonian<T> + HamiltonianBase<T,B>
+                   src/hamiltonians/spinless_fermion_hamiltonian.hpp:52-436,
+                   src/hamiltonians/hamiltonian_base.hpp:70-931
+  Hamiltonian.spmv MatrixVectorFunction::Simple  src/hamiltonians/matrix_vector_routines.hpp:94-104
+  Hamiltonian.lanczos  ArpackWrapper::DiagonalizeSymmetricMatrix
+                   src/utilities/wrappers/arpack_wrapper.hpp:595-937
+
+All compute runs in the HIP library; there is no CPU fallback.  Importing the
+package does not touch the GPU; the first call that needs the library loads it
+and raises ``HDError`` when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libhdiag_hip.so")
+
+HD_OK, HD_ERR_ARG, HD_ERR_HIP, HD_ERR_NOMEM, HD_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+HD_ERR_NOT_CONVERGED, HD_ERR_EMPTY = 5, 6
+SECTOR_NONE, SECTOR_LZ, SECTOR_K2D, SECTOR_K1D = 0, 1, 2, 3
+LAYOUT_SPHERE, LAYOUT_OFL = 0, 1
+F64, F32 = 0, 1
+STORAGE_FULL, STORAGE_UPPER_REF = 0, 1
+START_RANDOM, START_USER = 0, 1
+ZERO_TOL = 1e-14  # _SPARSE_ZERO_TOL_ (src/hamiltonians/hamiltonian_data.hpp:54)
+
+
+class HDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hdiag error {code}: {msg}")
+        self.code = code
+
+
+class _Sector(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("a", ctypes.c_int32), ("b", ctypes.c_int32),
+                ("c", ctypes.c_int32), ("d", ctypes.c_int32)]
+
+
+class _HamInfo(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int64), ("nnz", ctypes.c_int64), ("storage", ctypes.c_int32),
+                ("is_complex", ctypes.c_int32), ("has_f32", ctypes.c_int32),
+                ("has_f64", ctypes.c_int32), ("n_trimmed", ctypes.c_int64),
+                ("max_row", ctypes.c_int64), ("t_count_ms", ctypes.c_double),
+                ("t_scan_ms", ctypes.c_double), ("t_fill_ms", ctypes.c_double),
+                ("t_sort_ms", ctypes.c_double), ("device_bytes", ctypes.c_size_t)]
+
+
+class _LanczosOpts(ctypes.Structure):
+    _fields_ = [("nev", ctypes.c_int32), ("ncv", ctypes.c_int32),
+                ("max_restarts", ctypes.c_int32), ("tol", ctypes.c_double),
+                ("start", ctypes.c_int32), ("seed", ctypes.c_uint64),
+                ("d_start", ctypes.c_void_p), ("precision", ctypes.c_int32),
+                ("f32_switch_tol", ctypes.c_double), ("want_vectors", ctypes.c_int32)]
+
+
+class _LanczosStats(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int32), ("nconv", ctypes.c_int32),
+                ("restarts", ctypes.c_int32), ("matvecs", ctypes.c_int32),
+                ("matvecs_f32", ctypes.c_int32), ("matvecs_f64", ctypes.c_int32),
+                ("residual", ctypes.c_double * 64), ("time_total_ms", ctypes.c_double),
+                ("time_spmv_ms", ctypes.c_double), ("time_ortho_ms", ctypes.c_double)]
+
+
+_lib = None
+
+# (name, restype, argtypes) of every C entry point the package binds
+_P, _I32, _I64, _U32, _U64, _D = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64,
+                                  ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double)
+_PP = ctypes.POINTER(ctypes.c_void_p)
+SIGNATURES = {
+    "hd_last_error": (ctypes.c_char_p, []),
+    "hd_version": (ctypes.c_int, []),
+    "hd_malloc": (ctypes.c_int, [_PP, ctypes.c_size_t]),
+    "hd_free": (ctypes.c_int, [_P]),
+    "hd_memcpy_h2d": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "hd_memcpy_d2h": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "hd_memset": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_size_t]),
+    "hd_synchronize": (ctypes.c_int, []),
+    "hd_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "hd_basis_create": (ctypes.c_int, [ctypes.POINTER(_Sector), _U32, _U32, _PP]),
+    "hd_basis_destroy": (ctypes.c_int, [_P]),
+    "hd_basis_dim": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
+    "hd_basis_copy_states": (ctypes.c_int, [_P, _I64, _I64, _P]),
+    "hd_basis_device_states": (ctypes.c_int, [_P, _PP]),
+    "hd_basis_find_index": (ctypes.c_int, [_P, _U64, _P, _I64, _P]),
+    "hd_sphere_ktable": (ctypes.c_int, [_I32, _P]),
+    "hd_sphere_vtable": (ctypes.c_int, [_I32, _P, _I32, _P]),
+    "hd_tables_create": (ctypes.c_int, [_I32, _U32, _P, _P, _I32, _P, _PP]),
+    "hd_tables_destroy": (ctypes.c_int, [_P]),
+    "hd_ham_build": (ctypes.c_int, [_P, _P, _I32, _D, _I32, _PP]),
+    "hd_ham_destroy": (ctypes.c_int, [_P]),
+    "hd_ham_info": (ctypes.c_int, [_P, ctypes.POINTER(_HamInfo)]),
+    "hd_ham_copy_crs": (ctypes.c_int, [_P, _I64, _I64, _I32, _P, _P, _P]),
+    "hd_spmv": (ctypes.c_int, [_P, _I32, _P, _P]),
+    "hd_spmv_host": (ctypes.c_int, [_P, _P, _P]),
+    "hd_spmv_set_lanes": (ctypes.c_int, [_P, _I32]),
+    "hd_spmv_get_lanes": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
+    "hd_lanczos": (ctypes.c_int, [_P, ctypes.POINTER(_LanczosOpts), _P,
+                                  ctypes.POINTER(_LanczosStats)]),
+    "hd_lanczos_vectors": (ctypes.c_int, [_P, _I32, _P, _P]),
+    "hd_full_diagonalize": (ctypes.c_int, [_P, _I32, _P]),
+}
+
+
+def lib():
+    """Load libhdiag_hip.so (raises HDError when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HDError(-1, f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                              "(make -C hamiltonian_diagonalization_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != HD_OK:
+        msg = lib().hd_last_error()
+        raise HDError(code, msg.decode() if msg else "")
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+# ------------------------------------------------------------------ device memory
+
+class DeviceBuffer:
+    """A raw HBM allocation (hd_malloc); freed on deletion."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib().hd_malloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        _check(lib().hd_memcpy_h2d(b.ptr, _ptr(a), a.nbytes))
+        return b
+
+    def to_array(self, dtype, count=None):
+        dtype = np.dtype(dtype)
+        count = self.nbytes // dtype.itemsize if count is None else count
+        out = np.empty(count, dtype=dtype)
+        _check(lib().hd_memcpy_d2h(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(lib().hd_memcpy_h2d(self.ptr, _ptr(a), a.nbytes))
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and _lib is not None:
+            _lib.hd_free(self.ptr)
+            self.ptr = None
+
+
+# --------------------------------------------------------------------- sectors
+
+class Sector:
+    """Descriptor replacing the reference's sector predicate functions."""
+
+    def __init__(self, kind, a=0, b=0, c=0, d=0):
+        self.s = _Sector(kind, a, b, c, d)
+
+    @classmethod
+    def none(cls):
+        return cls(SECTOR_NONE)
+
+    @classmethod
+    def lz(cls, lz_tot, max_lz):
+        return cls(SECTOR_LZ, lz_tot, max_lz)
+
+    @classmethod
+    def k2d(cls, kx, ky, dim_x, dim_y):
+        return cls(SECTOR_K2D, kx, ky, dim_x, dim_y)
+
+    @classmethod
+    def k1d(cls, ky, dim_y):
+        return cls(SECTOR_K1D, ky, dim_y)
+
+
+class FockBasis:
+    """FermionFockBasis::GenerateFockSpace on the device."""
+
+    def __init__(self, sector: Sector, n_particles: int, n_orbitals: int):
+        h = ctypes.c_void_p()
+        _check(lib().hd_basis_create(ctypes.byref(sector.s), n_particles, n_orbitals,
+                                     ctypes.byref(h)))
+        self.h = h
+        self.sector = sector
+        self.n_particles = n_particles
+        self.n_orbitals = n_orbitals
+        d = ctypes.c_int64()
+        _check(lib().hd_basis_dim(h, ctypes.byref(d)))
+        self.dim = d.value
+
+    def states(self, first=0, count=None) -> np.ndarray:
+        count = self.dim - first if count is None else count
+        out = np.empty(count, dtype=np.uint64)
+        _check(lib().hd_basis_copy_states(self.h, first, count, _ptr(out)))
+        return out
+
+    def find_index(self, states: np.ndarray, offset: int = 0) -> np.ndarray:
+        """FindFockStateIndex for many states (UINT64_MAX when absent)."""
+        st = np.ascontiguousarray(states, dtype=np.uint64)
+        din = DeviceBuffer.from_array(st)
+        dout = DeviceBuffer(max(st.nbytes, 8))
+        _check(lib().hd_basis_find_index(self.h, offset, din.ptr, len(st), dout.ptr))
+        return dout.to_array(np.uint64, len(st))
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.hd_basis_destroy(self.h)
+            self.h = None
+
+
+class TermTables:
+    """Quartic (+ optional diagonal quadratic) term tables on the device."""
+
+    def __init__(self, layout: int, kmax: int, ktable: np.ndarray, vtable: np.ndarray,
+                 eps: Optional[np.ndarray] = None):
+        self.is_complex = np.iscomplexobj(vtable)
+        kt = np.ascontiguousarray(ktable, dtype=np.uint16)
+        vt = np.ascontiguousarray(vtable, dtype=np.complex128 if self.is_complex else np.float64)
+        vtd = vt.view(np.float64)
+        ep = None
+        if eps is not None:
+            ep = np.ascontiguousarray(eps, dtype=np.complex128 if self.is_complex else np.float64)
+            ep = ep.view(np.float64)
+        h = ctypes.c_void_p()
+        _check(lib().hd_tables_create(layout, kmax, _ptr(kt), _ptr(vtd), int(self.is_complex),
+                                      _ptr(ep), ctypes.byref(h)))
+        self.h = h
+        self.layout, self.kmax = layout, kmax
+        self.ktable, self.vtable, self.eps = kt, vt, eps
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.hd_tables_destroy(self.h)
+            self.h = None
+
+
+def sphere_ktable(max_lz: int) -> np.ndarray:
+    kt = np.zeros((max_lz + 1) ** 3, dtype=np.uint16)
+    _check(lib().hd_sphere_ktable(max_lz, _ptr(kt)))
+    return kt
+
+
+def sphere_vtable(max_lz: int, pseudopotentials: Sequence[float]) -> np.ndarray:
+    vm = np.ascontiguousarray(pseudopotentials, dtype=np.float64)
+    vt = np.zeros((max_lz + 1) ** 3, dtype=np.float64)
+    _check(lib().hd_sphere_vtable(max_lz, _ptr(vm), len(vm), _ptr(vt)))
+    return vt
+
+
+class LanczosResult:
+    def __init__(self, evals, stats: _LanczosStats, nev):
+        self.evals = evals
+        self.converged = bool(stats.converged)
+        self.nconv = stats.nconv
+        self.restarts = stats.restarts
+        self.matvecs = stats.matvecs
+        self.matvecs_f32 = stats.matvecs_f32
+        self.matvecs_f64 = stats.matvecs_f64
+        self.residuals = np.array(stats.residual[:nev])
+        self.time_ms = stats.time_total_ms
+        self.time_spmv_ms = stats.time_spmv_ms
+        self.time_ortho_ms = stats.time_ortho_ms
+
+    def __repr__(self):
+        return (f"LanczosResult(evals={self.evals}, matvecs={self.matvecs} "
+                f"(f32 {self.matvecs_f32}), restarts={self.restarts}, "
+                f"max_resid={self.residuals.max():.2e}, {self.time_ms:.1f} ms)")
+
+
+class Hamiltonian:
+    """SpinlessFermionHamiltonian build + CRS + SpMV + Lanczos on the device."""
+
+    def __init__(self, basis: FockBasis, tables: TermTables, storage=STORAGE_FULL,
+                 zero_tol=ZERO_TOL, values=(F64, F32)):
+        mask = (1 if F64 in values else 0) | (2 if F32 in values else 0)
+        h = ctypes.c_void_p()
+        _check(lib().hd_ham_build(basis.h, tables.h, storage, zero_tol, mask, ctypes.byref(h)))
+        self.h = h
+        self.basis, self.tables = basis, tables
+        self.storage = storage
+        self.is_complex = tables.is_complex
+        self.dim = basis.dim
+
+    def info(self) -> dict:
+        i = _HamInfo()
+        _check(lib().hd_ham_info(self.h, ctypes.byref(i)))
+        return {f: getattr(i, f) for f, _ in _HamInfo._fields_}
+
+    @property
+    def nnz(self):
+        return self.info()["nnz"]
+
+    def crs(self, row0=0, row1=None, precision=F64):
+        """(rowptr int64 relative to row0, col int32, values) host copies."""
+        row1 = self.dim if row1 is None else row1
+        rp = np.zeros(row1 - row0 + 1, dtype=np.int64)
+        _check(lib().hd_ham_copy_crs(self.h, row0, row1, precision, _ptr(rp), None, None))
+        n = int(rp[-1])
+        col = np.zeros(max(n, 1), dtype=np.int32)
+        if precision == F64:
+            vdt = np.complex128 if self.is_complex else np.float64
+        else:
+            vdt = np.complex64 if self.is_complex else np.float32
+        val = np.zeros(max(n, 1), dtype=vdt)
+        _check(lib().hd_ham_copy_crs(self.h, row0, row1, precision, _ptr(rp), _ptr(col), _ptr(val)))
+        return rp, col[:n], val[:n]
+
+    def vec_dtype(self, precision=F64):
+        if precision == F64:
+            return np.complex128 if self.is_complex else np.float64
+        return np.complex64 if self.is_complex else np.float32
+
+    def spmv(self, x: np.ndarray, precision=F64) -> np.ndarray:
+        """y = H x (host arrays in, host array out; device SpMV in between)."""
+        dt = self.vec_dtype(precision)
+        xx = np.ascontiguousarray(x, dtype=dt)
+        dx = DeviceBuffer.from_array(xx)
+        dy = DeviceBuffer(xx.nbytes)
+        _check(lib().hd_spmv(self.h, precision, dx.ptr, dy.ptr))
+        _check(lib().hd_synchronize())
+        return dy.to_array(dt, self.dim)
+
+    def spmv_device(self, dx: DeviceBuffer, dy: DeviceBuffer, precision=F64):
+        _check(lib().hd_spmv(self.h, precision, dx.ptr, dy.ptr))
+
+    def simple(self, x: np.ndarray) -> np.ndarray:
+        """MatrixVectorFunction::Simple drop-in over host buffers (FP64)."""
+        dt = self.vec_dtype(F64)
+        xx = np.ascontiguousarray(x, dtype=dt)
+        y = np.zeros_like(xx)
+        _check(lib().hd_spmv_host(self.h, _ptr(xx), _ptr(y)))
+        return y
+
+    def set_lanes(self, lanes: int):
+        _check(lib().hd_spmv_set_lanes(self.h, lanes))
+
+    def lanes(self) -> int:
+        v = ctypes.c_int32()
+        _check(lib().hd_spmv_get_lanes(self.h, ctypes.byref(v)))
+        return v.value
+
+    def lanczos(self, nev=5, tol=1e-10, ncv=0, precision=F64, seed=0, start=None,
+                f32_switch_tol=0.0, max_restarts=0, raise_on_fail=True) -> LanczosResult:
+        o = _LanczosOpts()
+        o.nev, o.ncv, o.max_restarts, o.tol = nev, ncv, max_restarts, tol
+        o.seed, o.precision, o.f32_switch_tol, o.want_vectors = seed, precision, f32_switch_tol, 1
+        dstart = None
+        if start is None:
+            o.start = START_RANDOM
+        elif isinstance(start, DeviceBuffer):
+            o.start, o.d_start = START_USER, start.ptr
+        else:
+            dstart = DeviceBuffer.from_array(np.ascontiguousarray(start, dtype=self.vec_dtype(F64)))
+            o.start, o.d_start = START_USER, dstart.ptr
+        nev_eff = min(nev, self.dim)
+        ev = np.zeros(nev_eff, dtype=np.float64)
+        st = _LanczosStats()
+        code = lib().hd_lanczos(self.h, ctypes.byref(o), _ptr(ev), ctypes.byref(st))
+        if code != HD_OK and (raise_on_fail or code != HD_ERR_NOT_CONVERGED):
+            _check(code)
+        return LanczosResult(ev, st, nev_eff)
+
+    def eigenvector(self, i: int) -> np.ndarray:
+        out = np.zeros(self.dim, dtype=self.vec_dtype(F64))
+        _check(lib().hd_lanczos_vectors(self.h, i, None, _ptr(out)))
+        return out
+
+    def full_diagonalize(self, nev: int) -> np.ndarray:
+        ev = np.zeros(min(nev, self.dim), dtype=np.float64)
+        _check(lib().hd_full_diagonalize(self.h, nev, _ptr(ev)))
+        return ev
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.hd_ham_destroy(self.h)
+            self.h = None
+
+
+def header_functions():
+    """Function names declared in include/hdiag.h."""
+    import re
+    path = os.path.join(os.path.dirname(HERE), "include", "hdiag.h")
+    txt = open(path).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:int|char\s*\*|const char\s*\*)\s*\**\s*(hd_\w+)\s*\(",
+                                 txt, re.M)))

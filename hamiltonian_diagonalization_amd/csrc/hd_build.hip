@@ -1,0 +1,861 @@
+// hd_build.hip -- term tables and the Hamiltonian build into CRS.
+//
+// Reference: SpinlessFermionHamiltonian<T>::Add_CdC_Terms / Add_CdCdCC_Terms
+// (src/hamiltonians/spinless_fermion_hamiltonian.hpp:143-436).  For each row
+// (in-state) the reference walks k4 over occupied orbitals, k3 over the other
+// occupied orbitals, k2 over the remaining candidates, takes k1 from the table,
+// evaluates the Jordan-Wigner sign (fermions.hpp:52-144), binary-searches the
+// out-state and accumulates sign*V into a hash map keyed (row, col), in that
+// order, then trims |v| < 1e-14 and sorts into CRS.
+//
+// Pair structure used here.  With a table that conserves the pair (sphere
+// L_z and lattice momentum both do), every term is either diagonal
+// ({k1,k2} = {k3,k4}) or a two-particle hop {a,b} -> {c,d}.  A hop is reached by
+// exactly four terms, and in the reference's loop order they are
+//   (k4,k3,k2) = (a,b,c), (a,b,d), (b,a,c), (b,a,d)    (a<b, c<d)
+// with signs s, -s, -s, s where s is the sign of the first.  So the stored
+// value is s * P(a,b,c) with P = (((+V1) - V2) - V3) + V4 (terms with
+// |V| <= tol skipped), bit-identical to the reference's += sequence because
+// negation is exact in IEEE arithmetic.  P depends only on the orbitals and is
+// tabulated once on the host; the kernels only add the sign (masked popcounts)
+// and the column (combinatorial rank of the out-state).  Diagonal entries are
+// state dependent and are summed per row in the reference's exact order.
+// Tables that break the pair structure are rejected (HD_ERR_UNSUPPORTED).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+
+#include "hd_internal.hpp"
+
+namespace hd {
+
+using cd = std::complex<double>;
+
+// ---------------------------------------------------------- table helpers
+
+static int get_k1(const hd_tables* t, int k2, int k3, int k4)
+{
+    const int K = t->K;
+    if (t->layout == HD_LAYOUT_SPHERE) {
+        const int v = t->ktab[((size_t)k4 * K + k3) * K + k2];
+        return v < K ? v : -1;  // GetK1 returns no k1 when k1 >= kMax
+    }
+    int t1 = k4, t2 = k3;
+    if (k4 > k3) { t1 = k3; t2 = k4; }
+    const int v = t->ktab[((size_t)t1 * K + t2 - (t1 * (t1 + 1)) / 2) * K + k2];
+    return v < K ? v : -1;  // a folded-table k1 >= kMax cannot be a valid orbital
+}
+
+static cd get_v(const hd_tables* t, int k1, int k2, int k3, int k4)
+{
+    const int K = t->K;
+    size_t i;
+    if (t->layout == HD_LAYOUT_SPHERE) {
+        i = ((size_t)k4 * K + k3) * K + k2;
+    } else {
+        int t1 = k4, t2 = k3, t3 = k2;
+        if (k4 > k3) { t1 = k3; t2 = k4; t3 = k1; }
+        i = ((size_t)t1 * K + t2 - (t1 * (t1 + 1)) / 2) * K + t3;
+    }
+    if (t->is_complex) return cd(t->vtab[2 * i], t->vtab[2 * i + 1]);
+    return cd(t->vtab[i], 0.0);
+}
+
+static double absv(const cd& v, bool cplx) { return cplx ? std::abs(v) : std::fabs(v.real()); }
+
+// Device-side derived tables for one build.
+struct Derived {
+    int K = 0, npairs = 0, cplx = 0, has_eps = 0;
+    std::vector<uint8_t> kpair;  // npairs*K: d | 0x80 if trimmed; 0xFF none
+    std::vector<double> pval;    // npairs*K (x2 complex)
+    std::vector<double> dterm;   // K*K*2 (x2 complex): diagonal terms of ordered (k4,k3)
+    std::vector<uint8_t> dflag;  // K*K: bit0 first term present, bit1 second
+    std::vector<double> eps;     // K (x2 complex)
+    std::vector<uint8_t> epsflag;
+};
+
+static int derive_tables(const hd_tables* t, const hd_basis* b, double tol, Derived& D)
+{
+    const int K = t->K;
+    const bool cplx = t->is_complex != 0;
+    D.K = K;
+    D.cplx = cplx;
+    D.npairs = K * (K - 1) / 2;
+    const int w = cplx ? 2 : 1;
+    D.kpair.assign((size_t)D.npairs * K, 0xFF);
+    D.pval.assign((size_t)D.npairs * K * w, 0.0);
+    D.dterm.assign((size_t)K * K * 2 * w, 0.0);
+    D.dflag.assign((size_t)K * K, 0);
+    auto put = [&](std::vector<double>& v, size_t i, const cd& x) {
+        if (cplx) { v[2 * i] = x.real(); v[2 * i + 1] = x.imag(); }
+        else v[i] = x.real();
+    };
+    auto unsupported = [&](int k1, int k2, int k3, int k4) {
+        return fail(HD_ERR_UNSUPPORTED,
+                    "term table does not conserve the pair structure at (k1,k2,k3,k4) = (" +
+                        std::to_string(k1) + "," + std::to_string(k2) + "," +
+                        std::to_string(k3) + "," + std::to_string(k4) + ")");
+    };
+    for (int a = 0; a < K; ++a)
+        for (int bb = a + 1; bb < K; ++bb) {
+            const int pair = a * (2 * K - a - 1) / 2 + (bb - a - 1);
+            for (int c = 0; c < K; ++c) {
+                if (c == a || c == bb) continue;
+                const int d = get_k1(t, c, bb, a);
+                if (d < 0) {
+                    if (get_k1(t, c, a, bb) >= 0) return unsupported(get_k1(t, c, a, bb), c, a, bb);
+                    continue;
+                }
+                if (d == a || d == bb) return unsupported(d, c, bb, a);
+                if (d == c) continue;  // c+_k c+_k vanishes (fermions.hpp:70)
+                if (get_k1(t, d, bb, a) != c || get_k1(t, c, a, bb) != d || get_k1(t, d, a, bb) != c)
+                    return unsupported(d, c, bb, a);
+                if (c > d) continue;  // the hop {a,b}->{c,d} is tabulated at c < d
+                if (!sector_weight_delta_ok(b->rt, b->sector, a, bb, c, d)) continue;
+                const cd terms[4] = {get_v(t, d, c, bb, a), -get_v(t, c, d, bb, a),
+                                     -get_v(t, d, c, a, bb), get_v(t, c, d, a, bb)};
+                cd acc(0.0, 0.0);
+                bool any = false;
+                for (const cd& x : terms)
+                    if (absv(x, cplx) > tol) {
+                        acc += x;
+                        any = true;
+                    }
+                if (!any) continue;
+                const size_t i = (size_t)pair * K + c;
+                D.kpair[i] = (uint8_t)(d | (absv(acc, cplx) < tol ? 0x80 : 0));
+                put(D.pval, i, acc);
+            }
+        }
+    // diagonal terms for each ordered (k4, k3): k2 runs over {k3, k4} ascending
+    for (int k4 = 0; k4 < K; ++k4)
+        for (int k3 = 0; k3 < K; ++k3) {
+            if (k3 == k4) continue;
+            const int order[2] = {std::min(k3, k4), std::max(k3, k4)};
+            for (int q = 0; q < 2; ++q) {
+                const int k2 = order[q];
+                const int k1 = get_k1(t, k2, k3, k4);
+                if (k1 < 0 || k1 == k2) continue;
+                if (k1 != k3 && k1 != k4) return unsupported(k1, k2, k3, k4);
+                // c+_{k4} c+_{k3} c_{k3} c_{k4} = +n n ; c+_{k3} c+_{k4} c_{k3} c_{k4} = -n n
+                const double sign = (k1 == k4) ? 1.0 : -1.0;
+                const cd el = sign * get_v(t, k1, k2, k3, k4);
+                if (absv(el, cplx) > tol) {
+                    put(D.dterm, ((size_t)k4 * K + k3) * 2 + q, el);
+                    D.dflag[(size_t)k4 * K + k3] |= (uint8_t)(1 << q);
+                }
+            }
+        }
+    D.has_eps = t->eps.empty() ? 0 : 1;
+    D.eps.assign((size_t)K * w, 0.0);
+    D.epsflag.assign(K, 0);
+    if (D.has_eps)
+        for (int k = 0; k < K; ++k) {
+            const cd e = cplx ? cd(t->eps[2 * k], t->eps[2 * k + 1]) : cd(t->eps[k], 0.0);
+            if (absv(e, cplx) > tol) {
+                put(D.eps, k, e);
+                D.epsflag[k] = 1;
+            }
+        }
+    return HD_OK;
+}
+
+// ---------------------------------------------------------- device kernels
+
+template <bool CPLX> struct VT;
+template <> struct VT<false> { using T = double; using F = float; };
+template <> struct VT<true> { using T = double2; using F = float2; };
+
+HD_DEV double vabs(double x) { return fabs(x); }
+HD_DEV double vabs(double2 x) { return hypot(x.x, x.y); }
+HD_DEV double vadd(double a, double b) { return a + b; }
+HD_DEV double2 vadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+HD_DEV double vneg(double a) { return -a; }
+HD_DEV double2 vneg(double2 a) { return make_double2(-a.x, -a.y); }
+template <typename T> HD_DEV T vzero();
+template <> HD_DEV double vzero<double>() { return 0.0; }
+template <> HD_DEV double2 vzero<double2>() { return make_double2(0.0, 0.0); }
+HD_DEV float to_f32(double v) { return (float)v; }
+HD_DEV float2 to_f32(double2 v) { return make_float2((float)v.x, (float)v.y); }
+
+struct BuildArgs {
+    int N, Norb, K, upper, has_eps;
+    double tol;
+    uint64_t norb_mask;
+    const uint8_t* kpair;   // global copy (kernels stage it into LDS)
+    const void* pval;
+    const void* dterm;
+    const uint8_t* dflag;
+    const void* eps;
+    const uint8_t* epsflag;
+    int kpair_bytes;
+};
+
+// Parity of the sign of c+_d c+_c c_b c_a |s> (k4 = a acts first), the first of
+// the four terms of the hop; fermions.hpp:87-143 restated with popcounts.
+HD_DEV int hop_parity(uint64_t s, int a, int b, int c, int d)
+{
+    const uint64_t ba = 1ull << a, bb = 1ull << b, bc = 1ull << c, bd = 1ull << d;
+    int p = __popcll(s & (ba - 1));
+    const uint64_t s1 = s ^ ba;
+    p += __popcll(s1 & (bb - 1));
+    const uint64_t s2 = s1 ^ bb;
+    p += __popcll(s2 & (bc - 1));
+    const uint64_t s3 = s2 | bc;
+    p += __popcll(s3 & (bd - 1));
+    return p & 1;
+}
+
+// Walk the two-particle hops of state s: occupied a<b, unoccupied c (< d),
+// d from the pair table.  f(a,b,c,d,table_index,out,trimmed).
+template <bool UPPER, typename F>
+HD_DEV void for_each_hop(uint64_t s, int K, uint64_t norb_mask, const uint8_t* kp, F&& f)
+{
+    const uint64_t unocc = ~s & norb_mask;
+    for (uint64_t ta = s; ta; ta &= ta - 1) {
+        const int a = __builtin_ctzll(ta);
+        const int pbase = a * (2 * K - a - 1) / 2 - a - 1;
+        for (uint64_t tb = ta & (ta - 1); tb; tb &= tb - 1) {
+            const int b = __builtin_ctzll(tb);
+            const int rowi = (pbase + b) * K;
+            for (uint64_t tc = unocc; tc; tc &= tc - 1) {
+                const int c = __builtin_ctzll(tc);
+                const int e = kp[rowi + c];
+                if (e == 0xFF) continue;
+                const int d = e & 0x3F;
+                if ((s >> d) & 1ull) continue;  // Pauli: d already occupied
+                const uint64_t out = (s ^ (1ull << a) ^ (1ull << b)) | (1ull << c) | (1ull << d);
+                if (UPPER && out < s) continue;
+                f(a, b, c, d, rowi + c, out, (e & 0x80) != 0);
+            }
+        }
+    }
+}
+
+template <bool CPLX>
+__global__ void __launch_bounds__(256)
+k_count(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A, int32_t* __restrict__ counts,
+        typename VT<CPLX>::T* __restrict__ diag, uint8_t* __restrict__ dkeep,
+        unsigned long long* __restrict__ ntrim)
+{
+    using T = typename VT<CPLX>::T;
+    extern __shared__ int32_t lds[];
+    const int K = A.K;
+    // stage kpair, dterm, dflag, eps, epsflag in LDS
+    uint8_t* s_kp = reinterpret_cast<uint8_t*>(lds);
+    const int kpw = (A.kpair_bytes + 3) / 4;
+    T* s_dt = reinterpret_cast<T*>(lds + ((kpw + 3) & ~3));
+    uint8_t* s_df = reinterpret_cast<uint8_t*>(s_dt + 2 * K * K);
+    T* s_eps = reinterpret_cast<T*>(s_df + ((K * K + 15) & ~15));
+    uint8_t* s_ef = reinterpret_cast<uint8_t*>(s_eps + K);
+    block_copy_words(lds, reinterpret_cast<const int32_t*>(A.kpair), kpw);
+    for (int i = threadIdx.x; i < 2 * K * K; i += blockDim.x)
+        s_dt[i] = reinterpret_cast<const T*>(A.dterm)[i];
+    for (int i = threadIdx.x; i < K * K; i += blockDim.x) s_df[i] = A.dflag[i];
+    for (int i = threadIdx.x; i < K; i += blockDim.x) {
+        s_eps[i] = reinterpret_cast<const T*>(A.eps)[i];
+        s_ef[i] = A.epsflag[i];
+    }
+    __syncthreads();
+    unsigned long long trimmed = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < dim; row += stride) {
+        const uint64_t s = states[row];
+        // diagonal: quadratic terms first (Add_CdC_Terms runs before
+        // Add_CdCdCC_Terms), then the quartic terms in (k4, k3, k2) order
+        T dsum = vzero<T>();
+        bool any = false;
+        if (A.has_eps)
+            for (uint64_t t = s; t; t &= t - 1) {
+                const int k = __builtin_ctzll(t);
+                if (s_ef[k]) {
+                    dsum = vadd(dsum, s_eps[k]);
+                    any = true;
+                }
+            }
+        for (uint64_t t4 = s; t4; t4 &= t4 - 1) {
+            const int k4 = __builtin_ctzll(t4);
+            for (uint64_t t3 = s ^ (1ull << k4); t3; t3 &= t3 - 1) {
+                const int k3 = __builtin_ctzll(t3);
+                const int idx = k4 * K + k3;
+                const int f = s_df[idx];
+                if (f & 1) dsum = vadd(dsum, s_dt[2 * idx]);
+                if (f & 2) dsum = vadd(dsum, s_dt[2 * idx + 1]);
+                any |= (f != 0);
+            }
+        }
+        const bool keep = any && !(vabs(dsum) < A.tol);
+        if (any && !keep) ++trimmed;
+        int cnt = keep ? 1 : 0;
+        auto body = [&](int, int, int, int, int, uint64_t, bool tr) {
+            if (tr) ++trimmed;
+            else ++cnt;
+        };
+        if (A.upper) for_each_hop<true>(s, K, A.norb_mask, s_kp, body);
+        else for_each_hop<false>(s, K, A.norb_mask, s_kp, body);
+        counts[row] = cnt;
+        diag[row] = dsum;
+        dkeep[row] = keep ? 1 : 0;
+    }
+    if (trimmed) atomicAdd(ntrim, trimmed);
+}
+
+template <bool CPLX, bool UPPER, bool LZ>
+__global__ void __launch_bounds__(1024)
+k_fill(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A,
+       const int32_t* __restrict__ rank_blob, int rank_words, const int64_t* __restrict__ rowptr,
+       const typename VT<CPLX>::T* __restrict__ diag, const uint8_t* __restrict__ dkeep,
+       int32_t* __restrict__ col_out, typename VT<CPLX>::T* __restrict__ val_out,
+       unsigned int* __restrict__ err)
+{
+    using T = typename VT<CPLX>::T;
+    extern __shared__ int32_t lds[];
+    block_copy_words(lds, rank_blob, rank_words);
+    uint8_t* s_kp = reinterpret_cast<uint8_t*>(lds + rank_words);
+    block_copy_words(lds + rank_words, reinterpret_cast<const int32_t*>(A.kpair),
+                     (A.kpair_bytes + 3) / 4);
+    __syncthreads();
+    const RankView rv = rank_view(lds);
+    const T* P = reinterpret_cast<const T*>(A.pval);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < dim; row += stride) {
+        const uint64_t s = states[row];
+        int64_t pos = rowptr[row];
+        const int64_t end = rowptr[row + 1];
+        if (dkeep[row] && pos < end) {
+            col_out[pos] = (int32_t)row;
+            val_out[pos] = diag[row];
+            ++pos;
+        }
+        auto body = [&](int a, int b, int c, int d, int ti, uint64_t out, bool tr) {
+            if (tr) return;
+            if (pos >= end) {  // more entries than the count pass saw: never write past the row
+                atomicOr(err, 4u);
+                return;
+            }
+            uint64_t r;
+            if (!rank_state<LZ>(rv, out, r)) {
+                atomicOr(err, 1u);
+                r = 0;
+            }
+            T v = P[ti];
+            if (hop_parity(s, a, b, c, d)) v = vneg(v);
+            col_out[pos] = (int32_t)r;
+            val_out[pos] = v;
+            ++pos;
+        };
+        for_each_hop<UPPER>(s, A.K, A.norb_mask, s_kp, body);
+        if (pos != end) atomicOr(err, 2u);
+    }
+}
+
+// Per-row sort by column: one wavefront per row, bitonic network in LDS over
+// CAP slots, then the permuted values are gathered from the unsorted copy and
+// written in the final precisions.
+template <bool CPLX, int CAP>
+__global__ void __launch_bounds__(64)
+k_sort(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col_in,
+       const typename VT<CPLX>::T* __restrict__ val_in, int32_t* __restrict__ col_out,
+       typename VT<CPLX>::T* __restrict__ val64, typename VT<CPLX>::F* __restrict__ val32)
+{
+    __shared__ int32_t key[CAP];
+    __shared__ int16_t idx[CAP];
+    const int lane = threadIdx.x;
+    for (int64_t row = blockIdx.x; row < dim; row += gridDim.x) {
+        const int64_t start = rowptr[row];
+        const int L = (int)(rowptr[row + 1] - start);
+        int n = 1;
+        while (n < L) n <<= 1;
+        for (int q = lane; q < n; q += 64) {
+            key[q] = q < L ? col_in[start + q] : 0x7FFFFFFF;
+            idx[q] = (int16_t)q;
+        }
+        __syncthreads();
+        for (int k = 2; k <= n; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = lane; i < n; i += 64) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const int ki = key[i], kl = key[l];
+                        const bool up = (i & k) == 0;
+                        if ((ki > kl) == up) {
+                            key[i] = kl;
+                            key[l] = ki;
+                            const int16_t t = idx[i];
+                            idx[i] = idx[l];
+                            idx[l] = t;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int q = lane; q < L; q += 64) {
+            col_out[start + q] = key[q];
+            const typename VT<CPLX>::T v = val_in[start + idx[q]];
+            if (val64) val64[start + q] = v;
+            if (val32) val32[start + q] = to_f32(v);
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ scan kernels
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+
+__global__ void __launch_bounds__(kScanBlock)
+k_scan_partial(const int32_t* __restrict__ counts, int64_t n, long long* __restrict__ bsum,
+               int* __restrict__ bmax)
+{
+    __shared__ long long ss[kScanBlock];
+    __shared__ int sm[kScanBlock];
+    const int64_t base = (int64_t)blockIdx.x * kScanBlock * kScanItems;
+    long long s = 0;
+    int m = 0;
+    for (int i = 0; i < kScanItems; ++i) {
+        const int64_t j = base + (int64_t)i * kScanBlock + threadIdx.x;
+        if (j < n) {
+            s += counts[j];
+            m = max(m, counts[j]);
+        }
+    }
+    ss[threadIdx.x] = s;
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = kScanBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            ss[threadIdx.x] += ss[threadIdx.x + o];
+            sm[threadIdx.x] = max(sm[threadIdx.x], sm[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bsum[blockIdx.x] = ss[0];
+        bmax[blockIdx.x] = sm[0];
+    }
+}
+
+__global__ void __launch_bounds__(1024)
+k_scan_top(long long* __restrict__ bsum, const int* __restrict__ bmax, int nb,
+           long long* __restrict__ total, int* __restrict__ gmax)
+{
+    // serial exclusive scan of block sums in chunks of 1024
+    __shared__ long long buf[1024];
+    __shared__ int mb[1024];
+    __shared__ long long carry;
+    if (threadIdx.x == 0) carry = 0;
+    int m = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + threadIdx.x;
+        const long long v = i < nb ? bsum[i] : 0;
+        if (i < nb) m = max(m, bmax[i]);
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const long long t = (int)threadIdx.x >= o ? buf[threadIdx.x - o] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) bsum[i] = carry + buf[threadIdx.x] - v;  // exclusive
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    mb[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) mb[threadIdx.x] = max(mb[threadIdx.x], mb[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *total = carry;
+        *gmax = mb[0];
+    }
+}
+
+__global__ void __launch_bounds__(kScanBlock)
+k_scan_final(const int32_t* __restrict__ counts, int64_t n, const long long* __restrict__ bsum,
+             int64_t* __restrict__ rowptr)
+{
+    __shared__ long long ss[kScanBlock];
+    const int64_t base = (int64_t)blockIdx.x * kScanBlock * kScanItems;
+    // each thread owns kScanItems consecutive items
+    long long loc[kScanItems];
+    long long s = 0;
+    for (int i = 0; i < kScanItems; ++i) {
+        const int64_t j = base + (int64_t)threadIdx.x * kScanItems + i;
+        loc[i] = j < n ? counts[j] : 0;
+        s += loc[i];
+    }
+    ss[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < kScanBlock; o <<= 1) {
+        const long long t = (int)threadIdx.x >= o ? ss[threadIdx.x - o] : 0;
+        __syncthreads();
+        ss[threadIdx.x] += t;
+        __syncthreads();
+    }
+    long long run = bsum[blockIdx.x] + ss[threadIdx.x] - s;
+    for (int i = 0; i < kScanItems; ++i) {
+        const int64_t j = base + (int64_t)threadIdx.x * kScanItems + i;
+        if (j < n) rowptr[j] = run;
+        run += loc[i];
+        if (j == n - 1) rowptr[n] = run;
+    }
+}
+
+// --------------------------------------------------------------- launchers
+
+static int set_lds(const void* fn, size_t lds)
+{
+    if (lds > 160 * 1024)
+        return fail(HD_ERR_UNSUPPORTED, "build tables need " + std::to_string(lds) +
+                                            " bytes of LDS (limit 160 KiB)");
+    if (lds > 64 * 1024) HD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    return HD_OK;
+}
+
+struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    Events() { hipEventCreate(&e[0]); hipEventCreate(&e[1]); }
+    ~Events() { hipEventDestroy(e[0]); hipEventDestroy(e[1]); }
+    void start(hipStream_t s) { hipEventRecord(e[0], s); }
+    double stop(hipStream_t s)
+    {
+        hipEventRecord(e[1], s);
+        hipEventSynchronize(e[1]);
+        float ms = 0;
+        hipEventElapsedTime(&ms, e[0], e[1]);
+        return ms;
+    }
+};
+
+template <bool CPLX, bool UPPER, bool LZ>
+static int launch_fill(int grid, size_t lds, hipStream_t st, const hd_basis* b, const BuildArgs& A,
+                       int rank_words, const int64_t* rowptr, const typename VT<CPLX>::T* diag,
+                       const uint8_t* dkeep, int32_t* ctmp, typename VT<CPLX>::T* vtmp,
+                       unsigned int* err)
+{
+    HD_CHECK(set_lds((const void*)k_fill<CPLX, UPPER, LZ>, lds));
+    hipLaunchKernelGGL((k_fill<CPLX, UPPER, LZ>), dim3(grid), dim3(1024), lds, st, b->d_states, b->dim, A,
+                       b->d_rank, rank_words, rowptr, diag, dkeep, ctmp, vtmp, err);
+    return HD_OK;
+}
+
+template <bool CPLX>
+static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, double tol,
+                     int value_mask)
+{
+    using T = typename VT<CPLX>::T;
+    using F = typename VT<CPLX>::F;
+    const hipStream_t st = h->stream;
+    const int64_t dim = b->dim;
+    const int K = D.K;
+    int n_cu = 256;
+    HD_CHECK(device_props(&n_cu, nullptr));
+    // upload derived tables
+    uint8_t *d_kpair = nullptr, *d_dflag = nullptr, *d_epsflag = nullptr, *d_dkeep = nullptr;
+    T *d_pval = nullptr, *d_dterm = nullptr, *d_eps = nullptr, *d_diag = nullptr, *d_vtmp = nullptr;
+    int32_t *d_counts = nullptr, *d_ctmp = nullptr;
+    long long* d_bsum = nullptr;
+    int* d_bmax = nullptr;
+    unsigned long long* d_misc = nullptr;  // [0] trimmed, [1] total, [2] max, [3] error flags
+    int status = HD_OK;
+    const int kp_bytes = (int)D.kpair.size();
+    const size_t kp_alloc = ((size_t)kp_bytes + 3) & ~(size_t)3;
+    auto cleanup = [&]() {
+        dfree(d_kpair); dfree(d_dflag); dfree(d_epsflag); dfree(d_dkeep); dfree(d_pval);
+        dfree(d_dterm); dfree(d_eps); dfree(d_diag); dfree(d_vtmp); dfree(d_counts);
+        dfree(d_ctmp); dfree(d_bsum); dfree(d_bmax); dfree(d_misc);
+    };
+#define HD_TRY(x)                  \
+    do {                           \
+        status = (x);              \
+        if (status != HD_OK) {     \
+            cleanup();             \
+            return status;         \
+        }                          \
+    } while (0)
+#define HD_TRYHIP(x)                                                                      \
+    do {                                                                                  \
+        hipError_t _e = (x);                                                              \
+        if (_e != hipSuccess) {                                                           \
+            cleanup();                                                                    \
+            return fail(HD_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(_e));     \
+        }                                                                                 \
+    } while (0)
+    HD_TRY(dalloc(reinterpret_cast<void**>(&d_kpair), kp_alloc, "pair table"));
+    HD_TRY(dalloc_t(&d_pval, D.pval.size() / (CPLX ? 2 : 1), "pair values"));
+    HD_TRY(dalloc_t(&d_dterm, (size_t)2 * K * K, "diagonal terms"));
+    HD_TRY(dalloc_t(&d_dflag, (size_t)K * K, "diagonal flags"));
+    HD_TRY(dalloc_t(&d_eps, (size_t)K, "quadratic terms"));
+    HD_TRY(dalloc_t(&d_epsflag, (size_t)K, "quadratic flags"));
+    HD_TRY(dalloc_t(&d_misc, 4, "counters"));
+    HD_TRYHIP(hipMemsetAsync(d_kpair, 0xFF, kp_alloc, st));
+    HD_TRYHIP(hipMemcpyAsync(d_kpair, D.kpair.data(), kp_bytes, hipMemcpyHostToDevice, st));
+    HD_TRYHIP(hipMemcpyAsync(d_pval, D.pval.data(), D.pval.size() * 8, hipMemcpyHostToDevice, st));
+    HD_TRYHIP(hipMemcpyAsync(d_dterm, D.dterm.data(), D.dterm.size() * 8, hipMemcpyHostToDevice, st));
+    HD_TRYHIP(hipMemcpyAsync(d_dflag, D.dflag.data(), D.dflag.size(), hipMemcpyHostToDevice, st));
+    HD_TRYHIP(hipMemcpyAsync(d_eps, D.eps.data(), D.eps.size() * 8, hipMemcpyHostToDevice, st));
+    HD_TRYHIP(hipMemcpyAsync(d_epsflag, D.epsflag.data(), D.epsflag.size(), hipMemcpyHostToDevice, st));
+    HD_TRYHIP(hipMemsetAsync(d_misc, 0, 4 * sizeof(unsigned long long), st));
+
+    BuildArgs A;
+    A.N = b->N;
+    A.Norb = b->Norb;
+    A.K = K;
+    A.upper = h->storage == HD_STORAGE_UPPER_REF;
+    A.has_eps = D.has_eps;
+    A.tol = tol;
+    A.norb_mask = b->Norb >= 64 ? ~0ull : ((1ull << b->Norb) - 1ull);
+    A.kpair = d_kpair;
+    A.pval = d_pval;
+    A.dterm = d_dterm;
+    A.dflag = d_dflag;
+    A.eps = d_eps;
+    A.epsflag = d_epsflag;
+    A.kpair_bytes = kp_bytes;
+
+    // ---- count
+    HD_TRY(dalloc_t(&d_counts, (size_t)dim, "row counts"));
+    HD_TRY(dalloc_t(&d_diag, (size_t)dim, "diagonal"));
+    HD_TRY(dalloc_t(&d_dkeep, (size_t)dim, "diagonal flags"));
+    HD_TRY(dalloc_t(&h->d_rowptr, (size_t)dim + 1, "row pointers"));
+    h->bytes += ((size_t)dim + 1) * 8;
+    Events ev;
+    {
+        const size_t lds = (((kp_alloc / 4) + 3) & ~(size_t)3) * 4 + sizeof(T) * 2 * K * K +
+                           (((size_t)K * K + 15) & ~(size_t)15) + sizeof(T) * K + K + 16;
+        HD_TRY(set_lds((const void*)k_count<CPLX>, lds));
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((dim + 255) / 256, (int64_t)n_cu * 8));
+        ev.start(st);
+        hipLaunchKernelGGL(k_count<CPLX>, dim3(grid), dim3(256), lds, st, b->d_states, dim, A,
+                           d_counts, d_diag, d_dkeep, d_misc);
+        HD_TRYHIP(hipGetLastError());
+        h->t_count = ev.stop(st);
+    }
+    // ---- scan
+    unsigned long long host_misc[4] = {0, 0, 0, 0};
+    {
+        const int64_t per = (int64_t)kScanBlock * kScanItems;
+        const int nb = (int)((dim + per - 1) / per);
+        HD_TRY(dalloc_t(&d_bsum, (size_t)nb, "scan partials"));
+        HD_TRY(dalloc_t(&d_bmax, (size_t)nb, "scan maxima"));
+        ev.start(st);
+        hipLaunchKernelGGL(k_scan_partial, dim3(nb), dim3(kScanBlock), 0, st, d_counts, dim, d_bsum, d_bmax);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, d_bsum, d_bmax, nb,
+                           reinterpret_cast<long long*>(d_misc + 1), reinterpret_cast<int*>(d_misc + 2));
+        hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanBlock), 0, st, d_counts, dim, d_bsum,
+                           h->d_rowptr);
+        HD_TRYHIP(hipGetLastError());
+        h->t_scan = ev.stop(st);
+        HD_TRYHIP(hipMemcpyAsync(host_misc, d_misc, sizeof(host_misc), hipMemcpyDeviceToHost, st));
+        HD_TRYHIP(hipStreamSynchronize(st));
+    }
+    h->nnz = (int64_t)host_misc[1];
+    h->n_trimmed = (int64_t)host_misc[0];
+    h->max_row = (int64_t)(int)host_misc[2];
+    dfree(d_counts);
+    d_counts = nullptr;
+    if (h->max_row > 4096) {
+        cleanup();
+        return fail(HD_ERR_UNSUPPORTED, "row of " + std::to_string(h->max_row) +
+                                            " entries exceeds the 4096-entry row sort");
+    }
+    // ---- fill
+    const size_t nnz = (size_t)h->nnz;
+    HD_TRY(dalloc_t(&d_ctmp, nnz, "unsorted columns"));
+    HD_TRY(dalloc_t(&d_vtmp, nnz, "unsorted values"));
+    {
+        const int rank_words = (int)b->rt.blob.size();
+        const size_t lds = (size_t)rank_words * 4 + kp_alloc;
+        const bool lz = b->rt.is_lz != 0;
+        const int per_cu = lds > 80 * 1024 ? 1 : (lds > 40 * 1024 ? 2 : 4);
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((dim + 1023) / 1024, (int64_t)n_cu * per_cu));
+        unsigned int* d_err = reinterpret_cast<unsigned int*>(d_misc + 3);
+        ev.start(st);
+        if (A.upper && lz)
+            HD_TRY((launch_fill<CPLX, true, true>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_err)));
+        else if (A.upper)
+            HD_TRY((launch_fill<CPLX, true, false>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_err)));
+        else if (lz)
+            HD_TRY((launch_fill<CPLX, false, true>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_err)));
+        else
+            HD_TRY((launch_fill<CPLX, false, false>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_err)));
+        HD_TRYHIP(hipGetLastError());
+        h->t_fill = ev.stop(st);
+        HD_TRYHIP(hipMemcpyAsync(host_misc, d_misc, sizeof(host_misc), hipMemcpyDeviceToHost, st));
+        HD_TRYHIP(hipStreamSynchronize(st));
+        if (host_misc[3] != 0) {
+            cleanup();
+            return fail(HD_ERR_UNSUPPORTED, "fill pass inconsistency (flags " +
+                                                std::to_string(host_misc[3]) + ")");
+        }
+    }
+    dfree(d_diag); d_diag = nullptr;
+    dfree(d_dkeep); d_dkeep = nullptr;
+    // ---- sort into the final arrays
+    HD_TRY(dalloc_t(&h->d_col, nnz, "columns"));
+    h->bytes += nnz * 4;
+    if (value_mask & 1) {
+        HD_TRY(dalloc(&h->d_val64, nnz * sizeof(T), "FP64 values"));
+        h->bytes += nnz * sizeof(T);
+    }
+    if (value_mask & 2) {
+        HD_TRY(dalloc(&h->d_val32, nnz * sizeof(F), "FP32 values"));
+        h->bytes += nnz * sizeof(F);
+    }
+    {
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(dim, (int64_t)n_cu * 32));
+        ev.start(st);
+        T* v64 = reinterpret_cast<T*>(h->d_val64);
+        F* v32 = reinterpret_cast<F*>(h->d_val32);
+        if (h->max_row <= 512)
+            hipLaunchKernelGGL((k_sort<CPLX, 512>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, h->d_col, v64, v32);
+        else if (h->max_row <= 1024)
+            hipLaunchKernelGGL((k_sort<CPLX, 1024>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, h->d_col, v64, v32);
+        else if (h->max_row <= 2048)
+            hipLaunchKernelGGL((k_sort<CPLX, 2048>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, h->d_col, v64, v32);
+        else
+            hipLaunchKernelGGL((k_sort<CPLX, 4096>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, h->d_col, v64, v32);
+        HD_TRYHIP(hipGetLastError());
+        h->t_sort = ev.stop(st);
+    }
+    HD_TRYHIP(hipStreamSynchronize(st));
+    cleanup();
+#undef HD_TRY
+#undef HD_TRYHIP
+    return HD_OK;
+}
+
+}  // namespace hd
+
+using namespace hd;
+
+extern "C" {
+
+int hd_tables_create(int32_t layout, uint32_t kmax, const uint16_t* h_ktable, const double* h_vtable,
+                     int32_t is_complex, const double* h_eps, hd_tables** out)
+{
+    if (!out || !h_ktable || !h_vtable) return fail(HD_ERR_ARG, "hd_tables_create: null argument");
+    *out = nullptr;
+    if (kmax == 0 || kmax > 64) return fail(HD_ERR_ARG, "kMax must be in 1..64");
+    if (layout != HD_LAYOUT_SPHERE && layout != HD_LAYOUT_OFL)
+        return fail(HD_ERR_ARG, "unknown table layout");
+    const size_t K = kmax;
+    const size_t n = layout == HD_LAYOUT_SPHERE ? K * K * K : (K + 1) * K * K / 2;
+    hd_tables* t = new hd_tables();
+    t->layout = layout;
+    t->K = (int)kmax;
+    t->is_complex = is_complex ? 1 : 0;
+    t->ktab.assign(h_ktable, h_ktable + n);
+    const size_t w = t->is_complex ? 2 : 1;
+    t->vtab.assign(h_vtable, h_vtable + n * w);
+    if (h_eps) t->eps.assign(h_eps, h_eps + K * w);
+    *out = t;
+    return HD_OK;
+}
+
+int hd_tables_destroy(hd_tables* t)
+{
+    delete t;
+    return HD_OK;
+}
+
+int hd_ham_build(const hd_basis* b, const hd_tables* t, int32_t storage, double zero_tol,
+                 int32_t value_mask, hd_hamiltonian** out)
+{
+    if (!b || !t || !out) return fail(HD_ERR_ARG, "hd_ham_build: null argument");
+    *out = nullptr;
+    if (storage != HD_STORAGE_FULL && storage != HD_STORAGE_UPPER_REF)
+        return fail(HD_ERR_ARG, "unknown storage");
+    if (value_mask < 1 || value_mask > 3) return fail(HD_ERR_ARG, "value_mask must be 1, 2 or 3");
+    if (!(zero_tol >= 0.0)) return fail(HD_ERR_ARG, "zero_tol must be >= 0");
+    if (t->K != b->Norb)
+        return fail(HD_ERR_UNSUPPORTED, "kMax (" + std::to_string(t->K) + ") must equal the number of orbitals (" +
+                                            std::to_string(b->Norb) + ")");
+    Derived D;
+    HD_CHECK(derive_tables(t, b, zero_tol, D));
+    hd_hamiltonian* h = new hd_hamiltonian();
+    h->basis = b;
+    h->dim = b->dim;
+    h->storage = storage;
+    h->is_complex = t->is_complex;
+    hipGetDevice(&h->device);
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(HD_ERR_HIP, "stream creation failed");
+    }
+    const int st = t->is_complex ? run_build<true>(h, b, D, zero_tol, value_mask)
+                                 : run_build<false>(h, b, D, zero_tol, value_mask);
+    if (st != HD_OK) {
+        const std::string msg = hd_last_error();
+        hd_ham_destroy(h);
+        set_error(msg);
+        return st;
+    }
+    *out = h;
+    return HD_OK;
+}
+
+int hd_ham_destroy(hd_hamiltonian* h)
+{
+    if (!h) return HD_OK;
+    if (h->lz) lanczos_state_free(h->lz);
+    dfree(h->d_rowptr);
+    dfree(h->d_col);
+    dfree(h->d_val64);
+    dfree(h->d_val32);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return HD_OK;
+}
+
+int hd_ham_info(const hd_hamiltonian* h, hd_ham_info_t* info)
+{
+    if (!h || !info) return fail(HD_ERR_ARG, "hd_ham_info: null argument");
+    info->dim = h->dim;
+    info->nnz = h->nnz;
+    info->storage = h->storage;
+    info->is_complex = h->is_complex;
+    info->has_f32 = h->d_val32 != nullptr;
+    info->has_f64 = h->d_val64 != nullptr;
+    info->n_trimmed = h->n_trimmed;
+    info->max_row = h->max_row;
+    info->t_count_ms = h->t_count;
+    info->t_scan_ms = h->t_scan;
+    info->t_fill_ms = h->t_fill;
+    info->t_sort_ms = h->t_sort;
+    info->device_bytes = h->bytes;
+    return HD_OK;
+}
+
+int hd_ham_copy_crs(const hd_hamiltonian* h, int64_t row0, int64_t row1, int32_t precision,
+                    int64_t* h_rowptr, int32_t* h_col, void* h_val)
+{
+    if (!h || row0 < 0 || row1 > h->dim || row0 > row1 || !h_rowptr)
+        return fail(HD_ERR_ARG, "hd_ham_copy_crs: bad row range");
+    HD_HIP(hipMemcpy(h_rowptr, h->d_rowptr + row0, (size_t)(row1 - row0 + 1) * 8, hipMemcpyDeviceToHost));
+    const int64_t base = h_rowptr[0];
+    const int64_t n = h_rowptr[row1 - row0] - base;
+    for (int64_t i = 0; i <= row1 - row0; ++i) h_rowptr[i] -= base;
+    if (h_col && n > 0)
+        HD_HIP(hipMemcpy(h_col, h->d_col + base, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (h_val && n > 0) {
+        const void* src = precision == HD_F32 ? h->d_val32 : h->d_val64;
+        if (!src) return fail(HD_ERR_ARG, "requested value precision is not resident");
+        const size_t es = (precision == HD_F32 ? 4 : 8) * (h->is_complex ? 2 : 1);
+        HD_HIP(hipMemcpy(h_val, static_cast<const char*>(src) + base * es, (size_t)n * es,
+                         hipMemcpyDeviceToHost));
+    }
+    return HD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+// hd_internal.hpp -- shared definitions of libhdiag_hip (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/hdiag.h"
+
+namespace hd {
+
+// ------------------------------------------------------------------ errors
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define HD_HIP(expr)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return ::hd::fail(HD_ERR_HIP, std::string(#expr) + ": " +                 \
+                                              hipGetErrorString(_e));                  \
+    } while (0)
+
+#define HD_CHECK(expr)                      \
+    do {                                    \
+        int _s = (expr);                    \
+        if (_s != HD_OK) return _s;         \
+    } while (0)
+
+// Device allocation that reports HD_ERR_NOMEM with the size.
+int dalloc(void** p, size_t bytes, const char* what);
+template <typename T>
+int dalloc_t(T** p, size_t count, const char* what)
+{
+    return dalloc(reinterpret_cast<void**>(p), count * sizeof(T), what);
+}
+void dfree(void* p);
+
+// -------------------------------------------------------- sector rank table
+
+// Counting table of the sector: W[p][c][g] = number of subsets of orbitals
+// {0..p-1} with c elements whose weight is g.  Two families:
+//   LZ:     weight(k) = k (integer sum, the target is sum k = (lzTot+N*maxLz)/2),
+//           stored compactly per (p,c) over g in [gmin, gmax].
+//   finite: weight in a finite abelian group (Z_X x Z_Y for K2D, Z_Y for K1D,
+//           trivial for NONE), dense over g, subtraction through a table.
+// Layout of the device blob (int32 words), copied into LDS by the kernels:
+//   [0] N  [1] Norb  [2] G  [3] target  [4] is_lz  [5] tab_len
+//   off [(Norb+1)*(N+1)]    offset of (p,c) into tab, minus gmin
+//   gmm [(Norb+1)*(N+1)*2]  (gmin, gmax); gmax < gmin marks an empty row
+//   sub [G*Norb]            finite groups: g - weight(k)  (absent for LZ)
+//   tab [tab_len]           uint32 counts
+struct RankTable {
+    int N = 0, Norb = 0, G = 1, target = 0, is_lz = 0;
+    std::vector<int32_t> blob;
+    uint64_t dim = 0;
+    bool empty = true;
+    size_t bytes() const { return blob.size() * sizeof(int32_t); }
+};
+
+// Builds the table; returns HD_OK, HD_ERR_ARG or HD_ERR_UNSUPPORTED (counts
+// beyond 32 bits).  An empty sector gives rt.empty = true.
+int make_rank_table(const hd_sector& sec, int N, int Norb, RankTable& rt);
+// Group weight helpers on the host (used to test hop sector preservation).
+int sector_weight_delta_ok(const RankTable& rt, const hd_sector& sec, int a, int b, int c,
+                           int d);
+
+constexpr int kRankHeader = 6;
+
+// ------------------------------------------------------------ device side
+
+#define HD_DEV __device__ __forceinline__
+
+struct RankView {
+    const int32_t* off;
+    const int32_t* gmm;
+    const int32_t* sub;
+    const uint32_t* tab;
+    int N, Norb, G, target, is_lz;
+};
+
+HD_DEV RankView rank_view(const int32_t* blob)
+{
+    RankView v;
+    v.N = blob[0];
+    v.Norb = blob[1];
+    v.G = blob[2];
+    v.target = blob[3];
+    v.is_lz = blob[4];
+    const int npc = (v.Norb + 1) * (v.N + 1);
+    v.off = blob + kRankHeader;
+    v.gmm = v.off + npc;
+    v.sub = v.gmm + 2 * npc;
+    v.tab = reinterpret_cast<const uint32_t*>(v.sub + (v.is_lz ? 0 : v.G * v.Norb));
+    return v;
+}
+
+template <bool LZ>
+HD_DEV uint32_t rank_lookup(const RankView& v, int p, int c, int g)
+{
+    const int i = p * (v.N + 1) + c;
+    const int lo = v.gmm[2 * i], hi = v.gmm[2 * i + 1];
+    if (g < lo || g > hi) return 0u;
+    return v.tab[v.off[i] + g];
+}
+
+template <bool LZ>
+HD_DEV int group_sub(const RankView& v, int g, int k)
+{
+    if (LZ) return g - k;
+    return v.sub[g * v.Norb + k];
+}
+
+// Index of `s` within the sector in ascending order; returns false when s is
+// not a sector state (wrong particle number, bits beyond Norb or wrong weight).
+template <bool LZ>
+HD_DEV bool rank_state(const RankView& v, uint64_t s, uint64_t& r_out)
+{
+    if (__popcll(s) != v.N) return false;
+    if (v.Norb < 64 && (s >> v.Norb) != 0) return false;
+    uint64_t r = 0;
+    int c = v.N, g = v.target;
+    uint64_t t = s;
+    while (t) {
+        const int p = 63 - __clzll(t);
+        r += rank_lookup<LZ>(v, p, c, g);
+        g = group_sub<LZ>(v, g, p);
+        --c;
+        t ^= 1ull << p;
+    }
+    r_out = r;
+    return g == 0;
+}
+
+// Sector state of index r (ascending order); r < dim.
+template <bool LZ>
+HD_DEV uint64_t unrank_state(const RankView& v, uint64_t r)
+{
+    uint64_t s = 0;
+    int c = v.N, g = v.target;
+    for (int p = v.Norb - 1; p >= 0 && c > 0; --p) {
+        const uint64_t z = rank_lookup<LZ>(v, p, c, g);
+        if (r >= z) {
+            r -= z;
+            s |= 1ull << p;
+            g = group_sub<LZ>(v, g, p);
+            --c;
+        }
+    }
+    return s;
+}
+
+// Cooperative copy of n 32-bit words into LDS by the whole block.
+HD_DEV void block_copy_words(int32_t* dst, const int32_t* src, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+// complex helpers (interleaved re, im)
+HD_DEV double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+HD_DEV float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+
+int device_props(int* n_cu, size_t* max_lds);
+
+}  // namespace hd
+
+// ------------------------------------------------------------- handle types
+
+struct hd_basis {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hd_sector sector{};
+    int N = 0, Norb = 0;
+    int64_t dim = 0;
+    hd::RankTable rt;
+    int32_t* d_rank = nullptr;  // blob
+    uint64_t* d_states = nullptr;
+};
+
+struct hd_tables {
+    int layout = 0, K = 0, is_complex = 0;
+    std::vector<uint16_t> ktab;
+    std::vector<double> vtab;  // interleaved when complex
+    std::vector<double> eps;   // empty when no quadratic table
+};
+
+struct hd_lanczos_state;
+
+struct hd_hamiltonian {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    const hd_basis* basis = nullptr;
+    int64_t dim = 0, nnz = 0, n_trimmed = 0, max_row = 0;
+    int storage = HD_STORAGE_FULL;
+    int is_complex = 0;
+    int64_t* d_rowptr = nullptr;
+    int32_t* d_col = nullptr;
+    void* d_val64 = nullptr;  // double or double2
+    void* d_val32 = nullptr;  // float or float2
+    int lanes_per_row = 0;    // SpMV schedule (0 = pick on first use)
+    double t_count = 0, t_scan = 0, t_fill = 0, t_sort = 0;
+    size_t bytes = 0;
+    hd_lanczos_state* lz = nullptr;  // eigenvectors of the last Lanczos call
+};
+
+namespace hd {
+int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y,
+                hipStream_t stream);
+int spmv_autotune(hd_hamiltonian* h);
+void lanczos_state_free(hd_lanczos_state* s);
+}  // namespace hd

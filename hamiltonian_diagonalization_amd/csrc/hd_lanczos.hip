@@ -1,0 +1,728 @@
+// hd_lanczos.hip -- device-resident thick-restart Lanczos (FP64 recurrence).
+//
+// Reference: ArpackWrapper::DiagonalizeSymmetricMatrix (src/utilities/wrappers/arpack_wrapper.hpp:595-937)
+// drives PARPACK p?naupd/p?neupd by reverse communication, WHICH = 'SR',
+// NCV = min(N, 2*NEV+21), exact shifts, maxiter 2000 (arpack_wrapper.cpp:38-49),
+// with every mat-vec a host round trip through MatrixVectorFunction::Simple.
+// Here the Krylov basis, the operator and all vector work stay in HBM; the host
+// only sees the small projected matrix.  Method: thick-restart Lanczos (Wu and
+// Simon, SIAM J. Matrix Anal. Appl. 22 (2000) 602) with full
+// reorthogonalisation by classical Gram-Schmidt applied twice (Giraud et al.,
+// Numer. Math. 101 (2005) 87), the projected tridiagonal-plus-arrow matrix
+// diagonalised by cyclic Jacobi rotations on the host.
+//
+// Mixed precision (precision = HD_F32): the operator is applied with FP32
+// values and FP32 vectors (x rounded from the FP64 Lanczos vector, y widened
+// back) until every wanted Ritz residual is below f32_switch_tol; the sum of
+// those Ritz vectors then restarts the iteration with the FP64 operator, which
+// takes the residuals to tol.  All reductions are FP64 and deterministic
+// (per-block partials reduced in a fixed order).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+
+#include "hd_internal.hpp"
+
+struct hd_lanczos_state {
+    int64_t dim = 0;
+    int cplx = 0;
+    int nev = 0;
+    void* d_vec = nullptr;  // nev vectors, contiguous
+    std::vector<double> evals;
+};
+
+namespace hd {
+
+void lanczos_state_free(hd_lanczos_state* s)
+{
+    if (!s) return;
+    dfree(s->d_vec);
+    delete s;
+}
+
+// ------------------------------------------------------------ device side
+
+template <bool C> struct E;
+template <> struct E<false> { using T = double; using F = float; };
+template <> struct E<true> { using T = double2; using F = float2; };
+
+HD_DEV double conj_mul(double a, double b) { return a * b; }          // conj(a) b
+HD_DEV double2 conj_mul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+HD_DEV double mul(double a, double b) { return a * b; }
+HD_DEV double2 mul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+HD_DEV double sub(double a, double b) { return a - b; }
+HD_DEV double2 sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+HD_DEV double add2(double a, double b) { return a + b; }
+HD_DEV double2 add2(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+HD_DEV double norm2(double a) { return a * a; }
+HD_DEV double norm2(double2 a) { return a.x * a.x + a.y * a.y; }
+HD_DEV double scale(double a, double s) { return a * s; }
+HD_DEV double2 scale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+template <typename T> HD_DEV T zero();
+template <> HD_DEV double zero<double>() { return 0.0; }
+template <> HD_DEV double2 zero<double2>() { return make_double2(0.0, 0.0); }
+HD_DEV float narrow(double a) { return (float)a; }
+HD_DEV float2 narrow(double2 a) { return make_float2((float)a.x, (float)a.y); }
+HD_DEV double widen(float a) { return (double)a; }
+HD_DEV double2 widen(float2 a) { return make_double2((double)a.x, (double)a.y); }
+
+constexpr int kVB = 256;  // block size of the vector kernels
+constexpr int kNV = 8;    // vectors per dot-product pass
+
+// partial[j * nb + block] = sum_r conj(V_j[r]) w[r]  for j in [j0, j0+nv);
+// with want_norm, partial[(kNV) * nb + block] = sum |w|^2 (as .x).
+template <bool C>
+__global__ void __launch_bounds__(kVB)
+k_dots(int64_t dim, const typename E<C>::T* __restrict__ V, int64_t ld, int j0, int nv,
+       const typename E<C>::T* __restrict__ w, typename E<C>::T* __restrict__ partial, int want_norm)
+{
+    using T = typename E<C>::T;
+    __shared__ T red[kVB];
+    T acc[kNV];
+#pragma unroll
+    for (int q = 0; q < kNV; ++q) acc[q] = zero<T>();
+    double nrm = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kVB;
+    for (int64_t r = (int64_t)blockIdx.x * kVB + threadIdx.x; r < dim; r += stride) {
+        const T wr = w[r];
+        if (want_norm) nrm += norm2(wr);
+#pragma unroll
+        for (int q = 0; q < kNV; ++q)
+            if (q < nv) acc[q] = add2(acc[q], conj_mul(V[(int64_t)(j0 + q) * ld + r], wr));
+    }
+    const int nout = nv + (want_norm ? 1 : 0);
+    for (int q = 0; q < nout; ++q) {
+        T v;
+        if (q < nv) v = acc[q];
+        else v = zero<T>(), *reinterpret_cast<double*>(&v) = nrm;
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = kVB / 2; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] = add2(red[threadIdx.x], red[threadIdx.x + o]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[(int64_t)q * gridDim.x + blockIdx.x] = red[0];
+        __syncthreads();
+    }
+}
+
+// out[q] = sum_b partial[q * nb + b], b ascending (one thread per output).
+template <bool C>
+__global__ void k_reduce(const typename E<C>::T* __restrict__ partial, int nb, int nout,
+                         typename E<C>::T* __restrict__ out)
+{
+    using T = typename E<C>::T;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nout) return;
+    T s = zero<T>();
+    for (int b = 0; b < nb; ++b) s = add2(s, partial[(int64_t)q * nb + b]);
+    out[q] = s;
+}
+
+// w -= sum_{j<nv} V_j coef[j]; partial norm^2 of the result.
+template <bool C>
+__global__ void __launch_bounds__(kVB)
+k_project_out(int64_t dim, const typename E<C>::T* __restrict__ V, int64_t ld, int nv,
+              const typename E<C>::T* __restrict__ coef, typename E<C>::T* __restrict__ w,
+              double* __restrict__ partial)
+{
+    using T = typename E<C>::T;
+    __shared__ T sc[64];
+    __shared__ double red[kVB];
+    for (int i = threadIdx.x; i < nv; i += kVB) sc[i] = coef[i];
+    __syncthreads();
+    double nrm = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kVB;
+    for (int64_t r = (int64_t)blockIdx.x * kVB + threadIdx.x; r < dim; r += stride) {
+        T acc = w[r];
+        for (int j = 0; j < nv; ++j) acc = sub(acc, mul(V[(int64_t)j * ld + r], sc[j]));
+        w[r] = acc;
+        nrm += norm2(acc);
+    }
+    red[threadIdx.x] = nrm;
+    __syncthreads();
+    for (int o = kVB / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_reduce_real(const double* __restrict__ partial, int nb, double* __restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += partial[b];
+    *out = s;
+}
+
+template <bool C>
+__global__ void k_scale(int64_t dim, const typename E<C>::T* __restrict__ src, double s,
+                        typename E<C>::T* __restrict__ dst)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < dim; r += stride)
+        dst[r] = scale(src[r], s);
+}
+
+// out_l = sum_i V_i Y[i][l] for l in [l0, l0+nl), Y row-major m x ldy in LDS.
+template <bool C>
+__global__ void __launch_bounds__(kVB)
+k_combine(int64_t dim, const typename E<C>::T* __restrict__ V, int64_t ld, int m,
+          const double* __restrict__ Y, int ldy, int l0, int nl, typename E<C>::T* __restrict__ out)
+{
+    using T = typename E<C>::T;
+    __shared__ double sy[64 * 16];
+    for (int i = threadIdx.x; i < m * nl; i += kVB) sy[i] = Y[(i / nl) * ldy + l0 + (i % nl)];
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * kVB;
+    for (int64_t r = (int64_t)blockIdx.x * kVB + threadIdx.x; r < dim; r += stride) {
+        T acc[16];
+#pragma unroll
+        for (int l = 0; l < 16; ++l) acc[l] = zero<T>();
+        for (int i = 0; i < m; ++i) {
+            const T v = V[(int64_t)i * ld + r];
+#pragma unroll
+            for (int l = 0; l < 16; ++l)
+                if (l < nl) acc[l] = add2(acc[l], scale(v, sy[i * nl + l]));
+        }
+        for (int l = 0; l < nl; ++l) out[(int64_t)l * ld + r] = acc[l];
+    }
+}
+
+template <bool C>
+__global__ void k_accumulate(int64_t dim, const typename E<C>::T* __restrict__ src, typename E<C>::T* __restrict__ dst)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < dim; r += stride)
+        dst[r] = add2(dst[r], src[r]);
+}
+
+template <bool C>
+__global__ void k_to_f32(int64_t dim, const typename E<C>::T* __restrict__ src, typename E<C>::F* __restrict__ dst)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < dim; r += stride)
+        dst[r] = narrow(src[r]);
+}
+
+template <bool C>
+__global__ void k_to_f64(int64_t dim, const typename E<C>::F* __restrict__ src, typename E<C>::T* __restrict__ dst)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < dim; r += stride)
+        dst[r] = widen(src[r]);
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) counter-based generator.
+HD_DEV void philox(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+    for (int i = 0; i < 10; ++i) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// Standard normal entries (Box-Muller on Philox uniforms); element r uses
+// counter (r, stream).
+template <bool C>
+__global__ void k_randn(int64_t dim, uint64_t seed, uint32_t stream, typename E<C>::T* __restrict__ out)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < dim; r += stride) {
+        uint32_t c[4] = {(uint32_t)r, (uint32_t)((uint64_t)r >> 32), stream, 0x5EEDu};
+        philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const double u1 = ((double)c[0] + 1.0) * 2.3283064365386963e-10;  // (0,1]
+        const double u2 = (double)c[1] * 2.3283064365386963e-10;
+        const double rad = sqrt(-2.0 * log(u1));
+        const double z0 = rad * cos(6.283185307179586 * u2);
+        if constexpr (C) {
+            const double z1 = rad * sin(6.283185307179586 * u2);
+            out[r] = make_double2(z0, z1);
+        } else {
+            out[r] = z0;
+        }
+    }
+}
+
+// partial[b] = sum |y - theta x|^2
+template <bool C>
+__global__ void __launch_bounds__(kVB)
+k_resid(int64_t dim, const typename E<C>::T* __restrict__ y, const typename E<C>::T* __restrict__ x,
+        double theta, double* __restrict__ partial)
+{
+    __shared__ double red[kVB];
+    double s = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kVB;
+    for (int64_t r = (int64_t)blockIdx.x * kVB + threadIdx.x; r < dim; r += stride)
+        s += norm2(sub(y[r], scale(x[r], theta)));
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kVB / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// -------------------------------------------------------- host: Jacobi
+
+// Symmetric eigen-decomposition of the n x n row-major matrix A (destroyed)
+// by cyclic Jacobi rotations; eigenvalues ascending in w, eigenvectors as the
+// columns of Z (row-major n x n).
+void jacobi_eigh(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& Z)
+{
+    Z.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) Z[(size_t)i * n + i] = 1.0;
+    auto a = [&](int i, int j) -> double& { return A[(size_t)i * n + j]; };
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                tot += a(i, j) * a(i, j);
+                if (i != j) off += a(i, j) * a(i, j);
+            }
+        if (off <= 1e-34 * tot || off == 0.0) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = a(p, q);
+                if (std::fabs(apq) < 1e-300) continue;
+                const double theta = (a(q, q) - a(p, p)) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = a(k, p), akq = a(k, q);
+                    a(k, p) = c * akp - s * akq;
+                    a(k, q) = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = a(p, k), aqk = a(q, k);
+                    a(p, k) = c * apk - s * aqk;
+                    a(q, k) = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double zkp = Z[(size_t)k * n + p], zkq = Z[(size_t)k * n + q];
+                    Z[(size_t)k * n + p] = c * zkp - s * zkq;
+                    Z[(size_t)k * n + q] = s * zkp + c * zkq;
+                }
+            }
+    }
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int x, int y) { return a(x, x) < a(y, y); });
+    w.resize(n);
+    std::vector<double> Zs((size_t)n * n);
+    for (int c = 0; c < n; ++c) {
+        w[c] = a(idx[c], idx[c]);
+        for (int r = 0; r < n; ++r) Zs[(size_t)r * n + c] = Z[(size_t)r * n + idx[c]];
+    }
+    Z.swap(Zs);
+}
+
+// ------------------------------------------------------------ host driver
+
+template <bool C>
+struct Lanczos {
+    using T = typename E<C>::T;
+    using F = typename E<C>::F;
+    hd_hamiltonian* h;
+    hipStream_t st;
+    int64_t n;
+    int m;
+    int nb;  // blocks for reductions
+    T* V = nullptr;     // (m+1) x n
+    T* W = nullptr;     // n (operator output / work)
+    T* tmp = nullptr;   // up to m x n (restart combination)
+    F* x32 = nullptr;
+    F* y32 = nullptr;
+    T* partial = nullptr;
+    T* coef = nullptr;  // device coefficient vector (<= 64 + 1)
+    double* dpart = nullptr;
+    double* dscal = nullptr;
+    double* dY = nullptr;
+    double t_spmv = 0, t_ortho = 0;
+    int mv32 = 0, mv64 = 0;
+
+    ~Lanczos()
+    {
+        dfree(V); dfree(W); dfree(tmp); dfree(x32); dfree(y32); dfree(partial); dfree(coef);
+        dfree(dpart); dfree(dscal); dfree(dY);
+    }
+
+    int grid() const { return nb; }
+
+    int alloc(bool need32)
+    {
+        HD_CHECK(dalloc_t(&V, (size_t)(m + 1) * n, "Lanczos basis"));
+        HD_CHECK(dalloc_t(&W, (size_t)n, "Lanczos work vector"));
+        HD_CHECK(dalloc_t(&tmp, (size_t)m * n, "restart buffer"));
+        if (need32) {
+            HD_CHECK(dalloc_t(&x32, (size_t)n, "FP32 x"));
+            HD_CHECK(dalloc_t(&y32, (size_t)n, "FP32 y"));
+        }
+        HD_CHECK(dalloc_t(&partial, (size_t)(kNV + 1) * nb, "dot partials"));
+        HD_CHECK(dalloc_t(&coef, 80, "coefficients"));
+        HD_CHECK(dalloc_t(&dpart, (size_t)nb, "norm partials"));
+        HD_CHECK(dalloc_t(&dscal, 4, "scalars"));
+        HD_CHECK(dalloc_t(&dY, 64 * 64, "Ritz coefficients"));
+        return HD_OK;
+    }
+
+    T* vec(int j) { return V + (size_t)j * n; }
+
+    // W = H V_j in the requested precision
+    int apply(const T* x, T* y, int prec)
+    {
+        hipEvent_t e0, e1;
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        hipEventRecord(e0, st);
+        int s;
+        if (prec == HD_F32) {
+            hipLaunchKernelGGL(k_to_f32<C>, dim3(nb), dim3(kVB), 0, st, n, x, x32);
+            s = spmv_launch(h, HD_F32, x32, y32, st);
+            hipLaunchKernelGGL(k_to_f64<C>, dim3(nb), dim3(kVB), 0, st, n, y32, y);
+            ++mv32;
+        } else {
+            s = spmv_launch(h, HD_F64, x, y, st);
+            ++mv64;
+        }
+        hipEventRecord(e1, st);
+        hipEventSynchronize(e1);
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        t_spmv += ms;
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return s;
+    }
+
+    // coefficients c[0..nv) = V[0..nv)^H w (device, also copied to h_out),
+    // optionally |w|^2 into *nrm
+    int dots(int nv, const T* w, std::vector<T>& h_out)
+    {
+        h_out.resize(nv);
+        for (int j0 = 0; j0 < nv; j0 += kNV) {
+            const int cnt = std::min(kNV, nv - j0);
+            hipLaunchKernelGGL(k_dots<C>, dim3(nb), dim3(kVB), 0, st, n, V, n, j0, cnt, w, partial, 0);
+            hipLaunchKernelGGL(k_reduce<C>, dim3(1), dim3(64), 0, st, partial, nb, cnt, coef + j0);
+        }
+        HD_HIP(hipGetLastError());
+        HD_HIP(hipMemcpyAsync(h_out.data(), coef, sizeof(T) * nv, hipMemcpyDeviceToHost, st));
+        HD_HIP(hipStreamSynchronize(st));
+        return HD_OK;
+    }
+
+    // w -= V[0..nv) coef (coef on device); returns |w|^2
+    int project(int nv, T* w, double* nrm2)
+    {
+        hipLaunchKernelGGL(k_project_out<C>, dim3(nb), dim3(kVB), 0, st, n, V, n, nv, coef, w, dpart);
+        hipLaunchKernelGGL(k_reduce_real, dim3(1), dim3(64), 0, st, dpart, nb, dscal);
+        HD_HIP(hipGetLastError());
+        HD_HIP(hipMemcpyAsync(nrm2, dscal, sizeof(double), hipMemcpyDeviceToHost, st));
+        HD_HIP(hipStreamSynchronize(st));
+        return HD_OK;
+    }
+
+    int norm2_of(const T* w, double* out)
+    {
+        hipLaunchKernelGGL(k_dots<C>, dim3(nb), dim3(kVB), 0, st, n, V, n, 0, 0, w, partial, 1);
+        hipLaunchKernelGGL(k_reduce<C>, dim3(1), dim3(64), 0, st, partial, nb, 1, coef);
+        T v;
+        HD_HIP(hipMemcpyAsync(&v, coef, sizeof(T), hipMemcpyDeviceToHost, st));
+        HD_HIP(hipStreamSynchronize(st));
+        *out = *reinterpret_cast<double*>(&v);
+        return HD_OK;
+    }
+
+    // Full reorthogonalisation of w against V[0..nv): two CGS passes.
+    // Returns the summed coefficients and the final |w|.
+    int orthogonalize(int nv, T* w, std::vector<T>& c, double* beta)
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        std::vector<T> c2;
+        double nrm2 = 0.0;
+        HD_CHECK(dots(nv, w, c));
+        HD_CHECK(project(nv, w, &nrm2));
+        HD_CHECK(dots(nv, w, c2));
+        HD_CHECK(project(nv, w, &nrm2));
+        for (int j = 0; j < nv; ++j) c[j] = host_add(c[j], c2[j]);
+        *beta = std::sqrt(std::max(nrm2, 0.0));
+        t_ortho += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return HD_OK;
+    }
+
+    static double host_add(double a, double b) { return a + b; }
+    static double2 host_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+    static double host_re(double a) { return a; }
+    static double host_re(double2 a) { return a.x; }
+
+    int scale_into(const T* src, double s, T* dst)
+    {
+        hipLaunchKernelGGL(k_scale<C>, dim3(nb), dim3(kVB), 0, st, n, src, s, dst);
+        HD_HIP(hipGetLastError());
+        return HD_OK;
+    }
+
+    // out[0..kk) = V[0..mm) Y[:, 0..kk)  (Y row-major mm x ldy on the host)
+    int combine(int mm, const std::vector<double>& Y, int ldy, int kk, T* out)
+    {
+        HD_HIP(hipMemcpyAsync(dY, Y.data(), sizeof(double) * mm * ldy, hipMemcpyHostToDevice, st));
+        for (int l0 = 0; l0 < kk; l0 += 16) {
+            const int nl = std::min(16, kk - l0);
+            hipLaunchKernelGGL(k_combine<C>, dim3(nb), dim3(kVB), 0, st, n, V, n, mm, dY, ldy, l0, nl,
+                               out + (size_t)l0 * n);
+        }
+        HD_HIP(hipGetLastError());
+        HD_HIP(hipStreamSynchronize(st));
+        return HD_OK;
+    }
+
+    int random_start(uint64_t seed, uint32_t stream_id, T* out)
+    {
+        hipLaunchKernelGGL(k_randn<C>, dim3(nb), dim3(kVB), 0, st, n, seed, stream_id, out);
+        HD_HIP(hipGetLastError());
+        return HD_OK;
+    }
+
+    int residual_norm(const T* y, const T* x, double theta, double* out)
+    {
+        hipLaunchKernelGGL(k_resid<C>, dim3(nb), dim3(kVB), 0, st, n, y, x, theta, dpart);
+        hipLaunchKernelGGL(k_reduce_real, dim3(1), dim3(64), 0, st, dpart, nb, dscal);
+        HD_HIP(hipGetLastError());
+        HD_HIP(hipMemcpyAsync(out, dscal, sizeof(double), hipMemcpyDeviceToHost, st));
+        HD_HIP(hipStreamSynchronize(st));
+        *out = std::sqrt(std::max(*out, 0.0));
+        return HD_OK;
+    }
+};
+
+template <bool C>
+static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* evals, hd_lanczos_stats* stats)
+{
+    using T = typename E<C>::T;
+    const auto t_start = std::chrono::steady_clock::now();
+    Lanczos<C> L;
+    L.h = h;
+    L.st = h->stream;
+    L.n = h->dim;
+    const int nev = (int)std::min<int64_t>(o.nev, h->dim);
+    int m = o.ncv > 0 ? o.ncv : 2 * nev + 21;
+    m = (int)std::min<int64_t>(m, h->dim);
+    m = std::min(m, 64);
+    if (m < nev + 2) return fail(HD_ERR_ARG, "ncv must be at least nev + 2 (and dim > nev + 1)");
+    L.m = m;
+    int n_cu = 256;
+    HD_CHECK(device_props(&n_cu, nullptr));
+    L.nb = (int)std::max<int64_t>(1, std::min<int64_t>((h->dim + kVB - 1) / kVB, (int64_t)n_cu * 4));
+    const bool mixed = o.precision == HD_F32;
+    if (mixed && !h->d_val32) return fail(HD_ERR_ARG, "FP32 Lanczos needs FP32 values (value_mask & 2)");
+    if (!h->d_val64) return fail(HD_ERR_ARG, "Lanczos needs the FP64 values for the residual check");
+    HD_CHECK(L.alloc(mixed));
+    const double tol = o.tol > 0 ? o.tol : 1e-10;
+    const double sw_tol = o.f32_switch_tol > 0 ? o.f32_switch_tol : 1e-6;
+    const int max_restarts = o.max_restarts > 0 ? o.max_restarts : 2000;
+    int prec = mixed ? HD_F32 : HD_F64;
+    int kk_keep = nev + (m - nev) / 2;
+    kk_keep = std::min(std::max(kk_keep, nev + 1), m - 2 >= nev ? m - 2 : nev);
+    kk_keep = std::max(kk_keep, nev);
+
+    // start vector
+    if (o.start == HD_START_USER) {
+        if (!o.d_start) return fail(HD_ERR_ARG, "HD_START_USER needs d_start");
+        HD_HIP(hipMemcpyAsync(L.vec(0), o.d_start, sizeof(T) * L.n, hipMemcpyDeviceToDevice, L.st));
+    } else {
+        HD_CHECK(L.random_start(o.seed, 0u, L.vec(0)));
+    }
+    uint32_t rng_stream = 1;
+    double nrm;
+    HD_CHECK(L.norm2_of(L.vec(0), &nrm));
+    nrm = std::sqrt(nrm);
+    if (!(nrm > 0)) return fail(HD_ERR_ARG, "start vector is zero");
+    HD_CHECK(L.scale_into(L.vec(0), 1.0 / nrm, L.vec(0)));
+
+    std::vector<double> alpha(m, 0.0), beta(m, 0.0), theta, arrow;
+    std::vector<double> Tm, w, Y;
+    std::vector<T> c;
+    int k = 0;  // kept Ritz vectors at the head of V
+    int restarts = 0;
+    bool converged = false;
+    std::vector<double> res_est(m, 0.0);
+    double anorm = 0.0;
+    for (;;) {
+        for (int j = k; j < m; ++j) {
+            HD_CHECK(L.apply(L.vec(j), L.W, prec));
+            double b;
+            HD_CHECK(L.orthogonalize(j + 1, L.W, c, &b));
+            alpha[j] = Lanczos<C>::host_re(c[j]);
+            anorm = std::max(anorm, std::fabs(alpha[j]) + b);
+            if (b <= 1e-12 * std::max(anorm, 1e-300)) {
+                // invariant subspace: continue with a fresh random direction
+                HD_CHECK(L.random_start(o.seed, rng_stream++, L.W));
+                double bb;
+                std::vector<T> cc;
+                HD_CHECK(L.orthogonalize(j + 1, L.W, cc, &bb));
+                HD_CHECK(L.scale_into(L.W, 1.0 / bb, L.vec(j + 1)));
+                beta[j] = 0.0;
+            } else {
+                beta[j] = b;
+                HD_CHECK(L.scale_into(L.W, 1.0 / b, L.vec(j + 1)));
+            }
+        }
+        // projected matrix: diag(theta) + arrow at row k, tridiagonal below
+        Tm.assign((size_t)m * m, 0.0);
+        for (int i = 0; i < k; ++i) {
+            Tm[(size_t)i * m + i] = theta[i];
+            Tm[(size_t)i * m + k] = Tm[(size_t)k * m + i] = arrow[i];
+        }
+        for (int j = k; j < m; ++j) {
+            Tm[(size_t)j * m + j] = alpha[j];
+            if (j + 1 < m) Tm[(size_t)j * m + j + 1] = Tm[(size_t)(j + 1) * m + j] = beta[j];
+        }
+        jacobi_eigh(m, Tm, w, Y);
+        const double bm = beta[m - 1];
+        bool all = true;
+        for (int i = 0; i < nev; ++i) {
+            res_est[i] = std::fabs(bm * Y[(size_t)(m - 1) * m + i]) / std::max(1.0, std::fabs(w[i]));
+            const double lim = (prec == HD_F32) ? sw_tol : tol;
+            if (!(res_est[i] <= lim)) all = false;
+        }
+        ++restarts;
+        if (all && prec == HD_F32) {
+            // FP32 phase done: restart from the sum of the Ritz vectors with the FP64 operator
+            HD_CHECK(L.combine(m, Y, m, nev, L.tmp));
+            // V[0] = sum_i of the nev Ritz vectors
+            HD_HIP(hipMemcpyAsync(L.vec(0), L.tmp, sizeof(T) * L.n, hipMemcpyDeviceToDevice, L.st));
+            for (int i = 1; i < nev; ++i)
+                hipLaunchKernelGGL(k_accumulate<C>, dim3(L.nb), dim3(kVB), 0, L.st, L.n, L.tmp + (size_t)i * L.n,
+                                   L.vec(0));
+            HD_HIP(hipGetLastError());
+            HD_CHECK(L.norm2_of(L.vec(0), &nrm));
+            HD_CHECK(L.scale_into(L.vec(0), 1.0 / std::sqrt(nrm), L.vec(0)));
+            prec = HD_F64;
+            k = 0;
+            theta.clear();
+            arrow.clear();
+            if (restarts >= max_restarts) break;
+            continue;
+        }
+        if (all) {
+            converged = true;
+            break;
+        }
+        if (restarts >= max_restarts) break;
+        // thick restart: keep kk Ritz vectors, continue from the residual direction
+        const int kk = kk_keep;
+        HD_CHECK(L.combine(m, Y, m, kk, L.tmp));
+        HD_HIP(hipMemcpyAsync(L.vec(kk), L.vec(m), sizeof(T) * L.n, hipMemcpyDeviceToDevice, L.st));
+        HD_HIP(hipMemcpyAsync(L.vec(0), L.tmp, sizeof(T) * L.n * kk, hipMemcpyDeviceToDevice, L.st));
+        theta.assign(w.begin(), w.begin() + kk);
+        arrow.resize(kk);
+        for (int i = 0; i < kk; ++i) arrow[i] = bm * Y[(size_t)(m - 1) * m + i];
+        k = kk;
+    }
+    // Ritz vectors and FP64 residuals
+    hd_lanczos_state* S = h->lz;
+    if (!S) S = h->lz = new hd_lanczos_state();
+    dfree(S->d_vec);
+    S->d_vec = nullptr;
+    S->dim = L.n;
+    S->cplx = C;
+    S->nev = nev;
+    HD_CHECK(dalloc(&S->d_vec, sizeof(T) * (size_t)L.n * nev, "eigenvectors"));
+    HD_CHECK(L.combine(m, Y, m, nev, static_cast<T*>(S->d_vec)));
+    S->evals.assign(w.begin(), w.begin() + nev);
+    for (int i = 0; i < nev; ++i) evals[i] = w[i];
+    int nconv = 0;
+    if (stats) {
+        for (int i = 0; i < 64; ++i) stats->residual[i] = 0;
+    }
+    for (int i = 0; i < nev; ++i) {
+        T* xi = static_cast<T*>(S->d_vec) + (size_t)i * L.n;
+        HD_CHECK(L.apply(xi, L.W, HD_F64));
+        double r;
+        HD_CHECK(L.residual_norm(L.W, xi, w[i], &r));
+        r /= std::max(1.0, std::fabs(w[i]));
+        if (stats && i < 64) stats->residual[i] = r;
+        if (r <= tol) ++nconv;
+    }
+    if (stats) {
+        stats->converged = converged && nconv == nev;
+        stats->nconv = nconv;
+        stats->restarts = restarts;
+        stats->matvecs = L.mv32 + L.mv64;
+        stats->matvecs_f32 = L.mv32;
+        stats->matvecs_f64 = L.mv64;
+        stats->time_spmv_ms = L.t_spmv;
+        stats->time_ortho_ms = L.t_ortho;
+        stats->time_total_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    }
+    if (!converged) return fail(HD_ERR_NOT_CONVERGED, "Lanczos reached the restart limit before tol");
+    return HD_OK;
+}
+
+}  // namespace hd
+
+using namespace hd;
+
+extern "C" {
+
+int hd_lanczos(hd_hamiltonian* h, const hd_lanczos_opts* opts, double* h_evals, hd_lanczos_stats* stats)
+{
+    if (!h || !opts || !h_evals) return fail(HD_ERR_ARG, "hd_lanczos: null argument");
+    if (opts->nev <= 0) return fail(HD_ERR_ARG, "nev must be positive");
+    if (h->storage != HD_STORAGE_FULL) return fail(HD_ERR_ARG, "Lanczos needs full storage");
+    if (!h->lanes_per_row) HD_CHECK(spmv_autotune(h));
+    return h->is_complex ? run_lanczos<true>(h, *opts, h_evals, stats)
+                         : run_lanczos<false>(h, *opts, h_evals, stats);
+}
+
+int hd_lanczos_vectors(const hd_hamiltonian* h, int32_t i, void* d_out, void* h_out)
+{
+    if (!h || !h->lz || !h->lz->d_vec) return fail(HD_ERR_ARG, "no Lanczos result on this handle");
+    if (i < 0 || i >= h->lz->nev) return fail(HD_ERR_ARG, "eigenvector index out of range");
+    const size_t es = 8 * (h->lz->cplx ? 2 : 1);
+    const char* src = static_cast<const char*>(h->lz->d_vec) + (size_t)i * h->lz->dim * es;
+    if (d_out) HD_HIP(hipMemcpy(d_out, src, h->lz->dim * es, hipMemcpyDeviceToDevice));
+    else if (h_out) HD_HIP(hipMemcpy(h_out, src, h->lz->dim * es, hipMemcpyDeviceToHost));
+    else return fail(HD_ERR_ARG, "no output buffer");
+    return HD_OK;
+}
+
+int hd_full_diagonalize(hd_hamiltonian* h, int32_t nev, double* h_evals)
+{
+    if (!h || !h_evals || nev <= 0) return fail(HD_ERR_ARG, "hd_full_diagonalize: bad argument");
+    if (h->is_complex) return fail(HD_ERR_UNSUPPORTED, "dense fallback handles real matrices; use hd_lanczos");
+    if (h->dim > 4096) return fail(HD_ERR_ARG, "dense fallback limited to dim <= 4096");
+    const int64_t n = h->dim;
+    std::vector<int64_t> rp(n + 1);
+    std::vector<int32_t> col((size_t)h->nnz);
+    std::vector<double> val((size_t)h->nnz);
+    HD_CHECK(hd_ham_copy_crs(h, 0, n, HD_F64, rp.data(), col.data(), val.data()));
+    std::vector<double> A((size_t)n * n, 0.0), w, Z;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = rp[i]; j < rp[i + 1]; ++j) {
+            A[(size_t)i * n + col[j]] = val[j];
+            if (h->storage == HD_STORAGE_UPPER_REF && col[j] != i) A[(size_t)col[j] * n + i] = val[j];
+        }
+    jacobi_eigh((int)n, A, w, Z);
+    for (int i = 0; i < std::min<int64_t>(nev, n); ++i) h_evals[i] = w[i];
+    return HD_OK;
+}
+
+}  // extern "C"

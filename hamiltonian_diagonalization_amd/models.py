@@ -1,0 +1,109 @@
+"""Model inputs for the benchmark configurations (host side, no GPU).
+
+* Sphere (FQHE in the lowest Landau level): Coulomb pseudopotentials and the
+  term tables of SpherePseudopotentialModel::BuildTermTables
+  (src/fqhe_sphere/pseudopotential_model/pseudopotential_model.cpp:64-95).
+* Interacting flux-lattice-style model in one band: a synthetic,
+  momentum-conserving complex V in the folded table layout of
+  src/optical_flux_lattice/interacting_ofl_model/term_tables.cpp:95-175.
+"""
+from __future__ import annotations
+
+from decimal import Decimal, getcontext
+from fractions import Fraction
+from math import comb
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from . import (LAYOUT_OFL, LAYOUT_SPHERE, FockBasis, Sector, TermTables, sphere_ktable,
+               sphere_vtable)
+
+
+def coulomb_pseudopotential(two_s: int, m: int) -> float:
+    """LLL Coulomb pseudopotential V_m on the sphere of flux 2S (radius sqrt(S)):
+    V_m = (2/sqrt(S)) C(4S-2J, 2S-J) C(4S+2J+2, 2S+J+1) / C(4S+2, 2S+1)^2 with
+    J = 2S - m (Fano, Ortolani, Colombo, PRB 34 (1986) 2670), evaluated with
+    exact integers and rounded once to double.  Same formula as
+    src/fqhe_sphere/interaction_pseudopotentials/coulomb_pseudopotentials.cpp:61-67."""
+    J = two_s - m
+    r = Fraction(comb(2 * two_s - 2 * J, two_s - J) * comb(2 * two_s + 2 * J + 2, two_s + J + 1),
+                 comb(2 * two_s + 2, two_s + 1) ** 2)
+    getcontext().prec = 60
+    # 2/sqrt(S) = 2 sqrt(2/two_s)
+    v = 2 * (Decimal(r.numerator) / Decimal(r.denominator)) * (Decimal(2) / Decimal(two_s)).sqrt()
+    return float(v)
+
+
+def coulomb_list(two_s: int, overrides: Optional[Dict[int, float]] = None) -> np.ndarray:
+    """The V_m list for the sphere benchmarks: indices m = 0 .. 2S-2 (length
+    2S-1, within the reference's limit of 2S coefficients), Coulomb values on
+    odd m, zero on even m (spinless fermions see only odd m); `overrides`
+    replaces selected odd m (e.g. {1: V1, 3: V3})."""
+    v = np.zeros(two_s - 1, dtype=np.float64)
+    for m in range(1, two_s - 1, 2):
+        v[m] = coulomb_pseudopotential(two_s, m)
+    for m, val in (overrides or {}).items():
+        v[m] = val
+    return v
+
+
+def sphere_tables(two_s: int, pseudopotentials: Sequence[float]) -> TermTables:
+    """Quartic tables of the sphere model (kMax = 2S + 1, no quadratic terms)."""
+    kt = sphere_ktable(two_s)
+    vt = sphere_vtable(two_s, pseudopotentials)
+    return TermTables(LAYOUT_SPHERE, two_s + 1, kt, vt)
+
+
+def sphere_problem(n: int, two_s: int, lz: int = 0, pseudopotentials=None):
+    """(basis, tables) for N electrons at flux 2S in the 2*L_z = lz sector.
+    The reference labels sectors by lzTot = sum(2k - 2S) = -2 L_z."""
+    if pseudopotentials is None:
+        pseudopotentials = coulomb_list(two_s)
+    basis = FockBasis(Sector.lz(lz, two_s), n, two_s + 1)
+    return basis, sphere_tables(two_s, pseudopotentials)
+
+
+def ofl_tables(dim_x: int, dim_y: int, seed: int = 0, band_width: float = 0.1):
+    """Synthetic single-band flux-lattice tables on a dim_x x dim_y k grid
+    (k = kx*dim_y + ky):
+      eps_k ~ U(-band_width, band_width);
+      G(k1,k2,k3,k4) complex Gaussian (sigma = 1 per component) on every
+      momentum-conserving tuple k1 = k3 + k4 - k2 (mod grid);
+      V = antisymmetrised in (k1,k2) and (k3,k4), then Hermitised,
+      V_1234 = conj(V_4321).
+    Drawn from numpy's PCG64 generator with `seed`.  Returns
+    (ktable, vtable, eps) in the folded layout."""
+    rng = np.random.default_rng(seed)
+    X, Y = dim_x, dim_y
+    K = X * Y
+    eps = rng.uniform(-band_width, band_width, size=K)
+    k = np.arange(K)
+    kx, ky = k // Y, k % Y
+    # k1[k2, k3, k4]
+    k1 = (((kx[None, :, None] + kx[None, None, :] - kx[:, None, None]) % X) * Y
+          + (ky[None, :, None] + ky[None, None, :] - ky[:, None, None]) % Y)
+    G = rng.normal(size=(K, K, K)) + 1j * rng.normal(size=(K, K, K))  # indexed [k2, k3, k4]
+    i2, i3, i4 = np.meshgrid(k, k, k, indexing="ij")
+    i1 = k1
+    # A(1,2,3,4) = [G(1,2,3,4) - G(2,1,3,4) - G(1,2,4,3) + G(2,1,4,3)] / 4
+    A = (G[i2, i3, i4] - G[i1, i3, i4] - G[i2, i4, i3] + G[i1, i4, i3]) / 4
+    # V(1,2,3,4) = [A(1,2,3,4) + conj(A(4,3,2,1))] / 2, A(4,3,2,1) sits at [k3, k2, k1]
+    V = (A + np.conj(A[i3, i2, i1])) / 2
+    n = (K + 1) * K * K // 2
+    kt = np.zeros(n, dtype=np.uint16)
+    vt = np.zeros(n, dtype=np.complex128)
+    for t1 in range(K):
+        base = (t1 * K - t1 * (t1 + 1) // 2)
+        for t2 in range(t1, K):
+            idx = (base + t2) * K + np.arange(K)
+            kt[idx] = k1[:, t2, t1]
+            vt[idx] = V[:, t2, t1]
+    return kt, vt, eps.astype(np.complex128)
+
+
+def ofl_problem(dim_x: int, dim_y: int, n: int, kx: int = 0, ky: int = 0, seed: int = 0):
+    kt, vt, eps = ofl_tables(dim_x, dim_y, seed)
+    K = dim_x * dim_y
+    basis = FockBasis(Sector.k2d(kx, ky, dim_x, dim_y), n, K)
+    return basis, TermTables(LAYOUT_OFL, K, kt, vt, eps)

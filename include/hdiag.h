@@ -1,0 +1,236 @@
+/*
+ * hdiag.h -- C ABI of libhdiag_hip.so, the MI355X (gfx950) exact-diagonalization
+ * hot path: Fock basis, Hamiltonian build into CRS, SpMV and thick-restart
+ * Lanczos, plus the learned start-vector network.
+ *
+ * Conventions
+ *  - Every entry point returns an hd_status (0 = HD_OK) and never throws.
+ *    hd_last_error() returns a thread-local message for the last failure.
+ *  - Pointers named d_* are device (HBM) pointers; h_* are host pointers.
+ *  - Handles are opaque and bound to the device that was current when they
+ *    were created.  Calls are synchronous with respect to the host unless
+ *    stated otherwise; all device work runs on the handle's own stream.
+ *  - Types follow the reference (src/utilities/general/orbital_and_state_defs.hpp:36-58):
+ *    fock_t = uint64_t, kState_t = uint16_t, iSize_t = uint32_t.
+ *
+ * Each declaration names the reference interface it replaces
+ * (paths relative to the reference repository root).
+ */
+#ifndef HDIAG_H
+#define HDIAG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    HD_OK = 0,
+    HD_ERR_ARG = 1,           /* invalid argument (reference: mpi.m_exitFlag = true) */
+    HD_ERR_HIP = 2,           /* HIP runtime error */
+    HD_ERR_NOMEM = 3,         /* device allocation failed */
+    HD_ERR_UNSUPPORTED = 4,   /* valid input outside what the kernels implement */
+    HD_ERR_NOT_CONVERGED = 5, /* Lanczos hit its iteration limit (ARPACK INFO=1) */
+    HD_ERR_EMPTY = 6          /* zero-dimensional sector (GenerateFockSpace returns true) */
+} hd_status;
+
+/* Sector predicate descriptors; they replace the std::function<bool(fock_t)>
+ * constraint passed to FermionFockBasis::GenerateFockSpace
+ * (src/basis/fermion_fock_basis.cpp:241-354). */
+typedef enum {
+    HD_SECTOR_NONE = 0, /* [](fock_t){return true;}  (interacting_ofl_model.cpp:1010) */
+    HD_SECTOR_LZ = 1,   /* TestAngularMomentumSector(state, a=lzTot, b=maxLz)
+                           (angular_momentum_constraint.hpp:41-62) */
+    HD_SECTOR_K2D = 2,  /* TestLinearMomentumSector2(state, a=kx, b=ky, c=dimX, d=dimY)
+                           (linear_momentum_constraints_2D.cpp:38-79) */
+    HD_SECTOR_K1D = 3   /* TestLinearMomentumSector1(state, a=ky, b=dimY)
+                           (linear_momentum_constraints_2D.cpp:87-110) */
+} hd_sector_kind;
+
+typedef struct {
+    int32_t kind;
+    int32_t a, b, c, d;
+} hd_sector;
+
+/* Quartic table layouts (QuarticTermTables flavours). */
+typedef enum {
+    HD_LAYOUT_SPHERE = 0, /* index (k4*kMax+k3)*kMax+k2; k1 >= kMax means "no term"
+                             (src/fqhe_sphere/pseudopotential_model/term_tables.cpp:109-149) */
+    HD_LAYOUT_OFL = 1     /* folded on k3>=k4: (t1*kMax+t2-t1(t1+1)/2)*kMax+t3
+                             (src/optical_flux_lattice/interacting_ofl_model/term_tables.cpp:95-175) */
+} hd_layout;
+
+typedef enum { HD_F64 = 0, HD_F32 = 1 } hd_precision;
+
+typedef enum {
+    HD_STORAGE_FULL = 0,     /* both triangles, int64 row pointers (GPU SpMV layout) */
+    HD_STORAGE_UPPER_REF = 1 /* the reference's upper-triangle CRS: M[i][j] = <j|H|i>, j >= i
+                                (src/hamiltonians/spinless_fermion_hamiltonian.hpp:362,
+                                 src/utilities/data_structures/crs_sparse_matrix.hpp:127-133) */
+} hd_storage;
+
+typedef struct hd_basis hd_basis;
+typedef struct hd_tables hd_tables;
+typedef struct hd_hamiltonian hd_hamiltonian;
+
+/* ----------------------------------------------------------------- runtime */
+
+const char* hd_last_error(void);
+int hd_version(void);
+/* Device buffers for hosts without their own device allocator. */
+int hd_malloc(void** d_ptr, size_t bytes);
+int hd_free(void* d_ptr);
+int hd_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes);
+int hd_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes);
+int hd_memset(void* d_dst, int value, size_t bytes);
+int hd_synchronize(void);
+int hd_device_count(int* n);
+
+/* ------------------------------------------------------------------- basis */
+
+/* FermionFockBasis::GenerateFockSpace (src/basis/fermion_fock_basis.cpp:241-354):
+ * the sector basis in ascending numeric order, generated on the device by
+ * combinatorial unranking.  Returns HD_ERR_EMPTY (and no handle) for a
+ * zero-dimensional sector, mirroring the reference's `true` return. */
+int hd_basis_create(const hd_sector* sector, uint32_t n_particles, uint32_t n_orbitals,
+                    hd_basis** out);
+int hd_basis_destroy(hd_basis* b);
+/* FermionFockBasis::GetStoredDimension (fermion_fock_basis.cpp:222-225). */
+int hd_basis_dim(const hd_basis* b, int64_t* dim);
+/* FermionFockBasis::GetFockBasis (fermion_fock_basis.cpp:360-372): copies
+ * states [first, first+count) to host memory. */
+int hd_basis_copy_states(const hd_basis* b, int64_t first, int64_t count, uint64_t* h_out);
+/* Device pointer to the basis (dim uint64 states), owned by the handle. */
+int hd_basis_device_states(const hd_basis* b, const uint64_t** d_states);
+/* FermionFockBasis::FindFockStateIndex (fermion_fock_basis.cpp:134-161) for a
+ * batch of device states: d_index[i] = offset + index, or UINT64_MAX
+ * (_SEARCH_ERROR_CODE_) when the state is not in the sector. */
+int hd_basis_find_index(const hd_basis* b, uint64_t offset, const uint64_t* d_states,
+                        int64_t n, uint64_t* d_index);
+
+/* ------------------------------------------------------------------ tables */
+
+/* BaseGenerate4KTable (src/fqhe_sphere/pseudopotential_model/pseudopotential_model_base.hpp:97-139):
+ * h_ktable receives (maxLz+1)^3 entries. */
+int hd_sphere_ktable(int32_t max_lz, uint16_t* h_ktable);
+/* BaseTermsFromPseudopotentials (pseudopotential_model_base.hpp:150-229):
+ * V_{k1k2k3k4} = -sum_m V_m CG(S,m1;S,m2|J,M) CG(S,m3;S,m4|J,M), J = 2S - m,
+ * with exact-rational Clebsch-Gordan coefficients rounded once to double.
+ * h_vtable receives (maxLz+1)^3 doubles. */
+int hd_sphere_vtable(int32_t max_lz, const double* h_pseudopotentials, int32_t n_pseudo,
+                     double* h_vtable);
+
+/* QuarticTermTables / QuadraticTermTables (term_tables_base.hpp:44-174) moved
+ * to the device.  h_vtable holds doubles (interleaved re,im when is_complex);
+ * h_eps is the optional diagonal quadratic table E_k (kMax entries, interleaved
+ * when complex) or NULL when Add_CdC_Terms is not called.  The tables must
+ * conserve the pair structure (k1 = f(k2,k3,k4) only produces diagonal terms or
+ * two-particle hops), which both reference layouts do; other tables give
+ * HD_ERR_UNSUPPORTED. */
+int hd_tables_create(int32_t layout, uint32_t kmax, const uint16_t* h_ktable,
+                     const double* h_vtable, int32_t is_complex, const double* h_eps,
+                     hd_tables** out);
+int hd_tables_destroy(hd_tables* t);
+
+/* ------------------------------------------------------------------- build */
+
+typedef struct {
+    int64_t dim;
+    int64_t nnz;          /* stored entries (storage dependent) */
+    int32_t storage;      /* hd_storage */
+    int32_t is_complex;
+    int32_t has_f32;      /* an FP32 (complex FP32) value copy is resident */
+    int32_t has_f64;
+    int64_t n_trimmed;    /* entries removed by Trim(|v| < zero_tol) */
+    int64_t max_row;      /* longest row */
+    double t_count_ms, t_scan_ms, t_fill_ms, t_sort_ms; /* device time per stage */
+    size_t device_bytes;  /* HBM held by the handle */
+} hd_ham_info_t;
+
+/* SpinlessFermionHamiltonian<T>::Add_CdC_Terms + Add_CdCdCC_Terms
+ * (src/hamiltonians/spinless_fermion_hamiltonian.hpp:143-436) followed by
+ * MappedSparseMatrix::Trim and CrsSparseMatrix::Initialize(map)
+ * (mapped_sparse_matrix.hpp:423-462, crs_sparse_matrix.hpp:193-254), all on the
+ * device: count, exclusive scan, fill, per-row sort.  Values are accumulated
+ * in FP64 in the reference's term order; value_mask selects which value
+ * copies stay resident (1 = FP64, 2 = FP32, 3 = both). */
+int hd_ham_build(const hd_basis* b, const hd_tables* t, int32_t storage, double zero_tol,
+                 int32_t value_mask, hd_hamiltonian** out);
+int hd_ham_destroy(hd_hamiltonian* h);
+int hd_ham_info(const hd_hamiltonian* h, hd_ham_info_t* info);
+/* CrsSparseMatrix accessors (GetRowStarts/GetCols/GetData, crs_sparse_matrix.hpp:297-330)
+ * as host copies of rows [row0, row1): h_rowptr gets row1-row0+1 entries relative
+ * to the first row; precision selects the FP64 or FP32 value copy. */
+int hd_ham_copy_crs(const hd_hamiltonian* h, int64_t row0, int64_t row1, int32_t precision,
+                    int64_t* h_rowptr, int32_t* h_col, void* h_val);
+
+/* ---------------------------------------------------------------- mat-vec */
+
+/* MatrixVectorFunction<T,L1,L2>::Simple (src/hamiltonians/matrix_vector_routines.hpp:94-104):
+ * y = H x on device vectors.  precision = HD_F64: FP64 values, x, y (complex
+ * FP64 for complex tables).  HD_F32: FP32 values, x, y, FP32 accumulation.
+ * Full storage only.  The call is asynchronous on the handle's stream. */
+int hd_spmv(const hd_hamiltonian* h, int32_t precision, const void* d_x, void* d_y);
+/* Simple() drop-in over host buffers (FP64 / complex FP64): copies in, multiplies,
+ * copies out. */
+int hd_spmv_host(const hd_hamiltonian* h, const void* h_x, void* h_y);
+/* Rows per wavefront schedule of the SpMV (lanes per row 4..64, 0 = autotune). */
+int hd_spmv_set_lanes(hd_hamiltonian* h, int32_t lanes_per_row);
+int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes_per_row);
+
+/* ---------------------------------------------------------------- Lanczos */
+
+typedef enum {
+    HD_START_RANDOM = 0, /* normal(0,1) entries from a counter-based generator */
+    HD_START_USER = 1    /* caller's device vector (the reference's --use-initial file) */
+} hd_start;
+
+typedef struct {
+    int32_t nev;          /* eigenpairs wanted (clamped to dim, hamiltonian_base.hpp:605-623) */
+    int32_t ncv;          /* Krylov basis size; 0 = min(dim, 2*nev+21) (arpack_wrapper.hpp:702) */
+    int32_t max_restarts; /* 0 = 2000 (arpack_wrapper.cpp:44) */
+    double tol;           /* residual ||Hx - lx|| <= tol * max(1,|l|) */
+    int32_t start;        /* hd_start */
+    uint64_t seed;
+    const void* d_start;  /* HD_START_USER: dim FP64 (complex FP64) device vector */
+    int32_t precision;    /* HD_F64: FP64 operator throughout.  HD_F32: FP32 operator until
+                             the residual reaches f32_switch_tol, then FP64 refinement */
+    double f32_switch_tol;/* 0 = 1e-6 */
+    int32_t want_vectors; /* keep the eigenvectors on the device (hd_lanczos_vectors) */
+} hd_lanczos_opts;
+
+typedef struct {
+    int32_t converged;
+    int32_t nconv;
+    int32_t restarts;
+    int32_t matvecs;      /* all operator applications */
+    int32_t matvecs_f32;
+    int32_t matvecs_f64;
+    double residual[64];  /* ||H x - l x|| / max(1,|l|), FP64 operator, per eigenpair */
+    double time_total_ms;
+    double time_spmv_ms;
+    double time_ortho_ms;
+} hd_lanczos_stats;
+
+/* HamiltonianBase::LanczosDiagonalize -> ArpackWrapper::DiagonalizeSymmetricMatrix
+ * (src/hamiltonians/hamiltonian_base.hpp:769-931, src/utilities/wrappers/arpack_wrapper.hpp:595-937):
+ * the nev lowest eigenvalues (WHICH='SR'), ascending, by a device-resident
+ * thick-restart Lanczos with full reorthogonalisation in FP64.  Eigenvalues go
+ * to h_evals; with want_vectors the eigenvectors stay resident and are read with
+ * hd_lanczos_vectors. */
+int hd_lanczos(hd_hamiltonian* h, const hd_lanczos_opts* opts, double* h_evals,
+               hd_lanczos_stats* stats);
+/* Copies eigenvector i (dim FP64 / complex FP64) of the last hd_lanczos call to
+ * device memory d_out, or to host memory when d_out is NULL and h_out is set. */
+int hd_lanczos_vectors(const hd_hamiltonian* h, int32_t i, void* d_out, void* h_out);
+/* Dense fallback for tiny sectors (HamiltonianBase::FullDiagonalize with LAPACK
+ * dsyevr/zheevr, hamiltonian_base.hpp:634-758, lapack_wrapper.hpp:566-649):
+ * the full CRS is expanded on the host and diagonalised by Jacobi rotations. */
+int hd_full_diagonalize(hd_hamiltonian* h, int32_t nev, double* h_evals);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

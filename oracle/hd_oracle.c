@@ -1,0 +1,660 @@
+/*
+ * hd_oracle.c -- CPU restatement of the reference's Fock-basis, Hamiltonian
+ * build and symmetric mat-vec stages.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library and the timed CPU
+ * baseline.  The product never links or loads this file.
+ *
+ * Reference files restated (paths relative to the reference repository root):
+ *   src/utilities/mathematics/binary_number_tools.cpp:95-331   bit tools
+ *   src/utilities/mathematics/binomials.cpp:34-64              C(n,k)
+ *   src/fqhe_sphere/pseudopotential_model/angular_momentum_constraint.hpp:41-62
+ *   src/optical_flux_lattice/interacting_ofl_model/linear_momentum_constraints_2D.cpp:38-110
+ *   src/basis/fermion_fock_basis.cpp:134-161, 241-354          basis + lookup
+ *   src/utilities/algorithms/binary_search.hpp:50-81
+ *   src/utilities/mathematics/fermions.hpp:52-227              c+c+cc, c+c
+ *   src/fqhe_sphere/pseudopotential_model/term_tables.cpp:36-169     sphere tables
+ *   src/fqhe_sphere/pseudopotential_model/pseudopotential_model_base.hpp:62-139  k-table
+ *   src/optical_flux_lattice/interacting_ofl_model/term_tables.cpp:36-198 folded tables
+ *   src/hamiltonians/spinless_fermion_hamiltonian.hpp:143-436   the build loops
+ *   src/utilities/data_structures/mapped_sparse_matrix.hpp:355-462  Insert / Trim
+ *   src/utilities/data_structures/crs_sparse_matrix.hpp:193-254     map -> CRS
+ *   src/utilities/mathematics/sparse_linear_algebra.hpp:142-165     naive SpMV
+ *
+ * Pinning (what fixes each function independently of this file; tests/):
+ *   oc_binomial, oc_next_hamming, oc_generate_hamming, oc_index_hamming:
+ *       Python math.comb and itertools.combinations order (test_oracle_bits).
+ *   oc_test_*, oc_basis: brute force over all bitstrings with the predicate
+ *       written from its definition; sector dimensions from an independent
+ *       subset-sum count (8,512 / 7,764,392 / 1,001,402; test_oracle_basis).
+ *   oc_cdcdcc, oc_cdc: Jordan-Wigner Kronecker-product operator matrices for
+ *       Norb <= 8 (test_oracle_fermions).
+ *   oc_build: dense H assembled from the Jordan-Wigner matrices for Norb <= 12
+ *       (sphere and flux-lattice tables), Hermiticity, trace identity, and the
+ *       Laughlin zero mode of V1 at 2S = 3(N-1) (test_oracle_build).
+ *   oc_spmv_sym*: dense numpy product on the expanded matrix (test_oracle_build).
+ *   The reference itself cannot be compiled here (boost, ARPACK, sparsehash are
+ *   absent), so no function is pinned against reference binaries.
+ */
+#include "hd_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------- bit tools */
+
+/* C(n,k) exactly in 64 bits (binomials.cpp:34-64 restated; the reference's
+ * table is exact for every n <= 64 used on this path). */
+uint64_t oc_binomial(int n, int k)
+{
+    if (k < 0 || n < 0 || k > n) return 0;
+    if (k > n - k) k = n - k;
+    unsigned __int128 r = 1;
+    for (int i = 1; i <= k; ++i) r = r * (unsigned)(n - k + i) / (unsigned)i;
+    return (uint64_t)r;
+}
+
+int oc_popcount(uint64_t x) { return __builtin_popcountll(x); }
+
+/* NextHammingNumber64 (binary_number_tools.cpp:255-261). */
+uint64_t oc_next_hamming(uint64_t x)
+{
+    if (x == 0) return x;
+    uint64_t t = x & (0 - x);
+    return (((x ^ (x + t)) / t) >> 2) | (x + t);
+}
+
+/* GenerateHammingNumber (binary_number_tools.cpp:297-331): the index-th
+ * weight-`weight` number in ascending order. */
+uint64_t oc_generate_hamming(int weight, uint64_t index)
+{
+    if (index == 0) return (weight >= 64) ? ~0ull : ((1ull << weight) - 1ull);
+    uint64_t out = 0, cur = 0;
+    for (int p = weight - 1; p >= 0; --p) {
+        int m = p;
+        uint64_t inc;
+        do {
+            inc = oc_binomial(m, p);
+            cur += inc;
+            ++m;
+        } while (index >= cur);
+        cur -= inc;
+        out |= 1ull << (m - 1);
+    }
+    return out;
+}
+
+/* IndexHammingNumber64 (binary_number_tools.cpp:270-287). */
+uint64_t oc_index_hamming(int weight, uint64_t x)
+{
+    uint64_t out = 0, t = x;
+    for (int p = weight; p > 0; --p) {
+        int m = 63 - __builtin_clzll(t);
+        if (p <= m) out += oc_binomial(m, p);
+        t ^= 1ull << m;
+    }
+    return out;
+}
+
+/* ------------------------------------------------------- sector predicates */
+
+/* TestAngularMomentumSector (angular_momentum_constraint.hpp:41-62). */
+int oc_test_lz(uint64_t state, int lzTot, int maxLz)
+{
+    int16_t cum = 0; /* mState_t accumulator */
+    while (state) {
+        uint64_t r = state & (0 - state);
+        cum = (int16_t)(cum + 2 * (int)__builtin_ctzll(r) - maxLz);
+        state ^= r;
+    }
+    return cum == (int16_t)lzTot;
+}
+
+static int oc_mod(int a, int n) { int r = a % n; return r < 0 ? r + n : r; }
+
+/* TestLinearMomentumSector2 (linear_momentum_constraints_2D.cpp:38-79). */
+int oc_test_k2d(uint64_t state, int kx, int ky, int dimX, int dimY)
+{
+    int x = 0, y = 0;
+    while (state) {
+        uint64_t r = state & (0 - state);
+        int pos = __builtin_ctzll(r);
+        x += (int)floor((double)pos / dimY);
+        y += oc_mod(pos, dimY);
+        state ^= r;
+    }
+    x = oc_mod(x, dimX);
+    y = oc_mod(y, dimY);
+    return x == kx && y == ky;
+}
+
+/* TestLinearMomentumSector1 (linear_momentum_constraints_2D.cpp:89-110). */
+int oc_test_k1d(uint64_t state, int ky, int dimY)
+{
+    int y = 0;
+    while (state) {
+        uint64_t r = state & (0 - state);
+        y += oc_mod(__builtin_ctzll(r), dimY);
+        state ^= r;
+    }
+    return oc_mod(y, dimY) == ky;
+}
+
+int oc_sector_test(const oc_sector* s, uint64_t state)
+{
+    switch (s->kind) {
+    case OC_SECTOR_NONE: return 1;
+    case OC_SECTOR_LZ: return oc_test_lz(state, s->a, s->b);
+    case OC_SECTOR_K2D: return oc_test_k2d(state, s->a, s->b, s->c, s->d);
+    case OC_SECTOR_K1D: return oc_test_k1d(state, s->a, s->b);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------- basis */
+
+/* GenerateFockSpace, single rank (fermion_fock_basis.cpp:241-354): walk all
+ * C(Norb,N) weight-N numbers in ascending order from GenerateHammingNumber(N,0)
+ * with NextHammingNumber64 and keep those passing the predicate. */
+int64_t oc_basis(const oc_sector* s, int N, int Norb, uint64_t* out, int64_t cap)
+{
+    uint64_t full = oc_binomial(Norb, N);
+    uint64_t st = oc_generate_hamming(N, 0);
+    int64_t cnt = 0;
+    for (uint64_t i = 0; i < full; ++i) {
+        if (oc_sector_test(s, st)) {
+            if (cnt < cap) out[cnt] = st;
+            ++cnt;
+        }
+        st = oc_next_hamming(st);
+    }
+    return cnt;
+}
+
+/* BinarySearch (binary_search.hpp:50-81) as used by FindFockStateIndex. */
+int64_t oc_find_index(const uint64_t* a, int64_t dim, uint64_t v)
+{
+    if (dim <= 0) return -1;
+    int64_t lo = 0, hi = dim - 1;
+    if (v < a[lo] || v > a[hi]) return -1;
+    while (hi >= lo) {
+        int64_t mid = (hi + lo) / 2;
+        uint64_t m = a[mid];
+        if (v == m) return mid;
+        if (v > m) lo = mid + 1;
+        else {
+            if (hi == 0) break;
+            hi = mid - 1;
+        }
+    }
+    return -1;
+}
+
+/* --------------------------------------------------------- fermion algebra */
+
+/* Evaluate_CdCdCC (fermions.hpp:52-144). */
+uint64_t oc_cdcdcc(int* sign, uint64_t b1, uint64_t b2, uint64_t b3, uint64_t b4,
+                   uint64_t state, uint64_t highest)
+{
+    if (b3 > highest || b4 > highest || (state & b3) == 0 || (state & b4) == 0 ||
+        (b1 & b2) || (b3 & b4) ||
+        (!(b1 & b3) && !(b1 & b4) && (state & b1) != 0) ||
+        (!(b2 & b3) && !(b2 & b4) && (state & b2) != 0)) {
+        *sign = 0;
+        return 0;
+    }
+    int c = oc_popcount(state & (b4 - 1));
+    uint64_t out = state ^ b4;
+    c += oc_popcount(out & (b3 - 1));
+    out ^= b3;
+    c += oc_popcount(out & (b2 - 1));
+    out |= b2;
+    c += oc_popcount(out & (b1 - 1));
+    out |= b1;
+    *sign = (c & 1) ? -1 : 1;
+    return out;
+}
+
+/* Evaluate_CdC (fermions.hpp:155-227). */
+uint64_t oc_cdc(int* sign, uint64_t b1, uint64_t b2, uint64_t state, uint64_t highest)
+{
+    if (b2 > highest || (state & b2) == 0 || (!(b1 & b2) && (state & b1) != 0)) {
+        *sign = 0;
+        return 0;
+    }
+    if (b1 & b2) {
+        *sign = 1;
+        return state;
+    }
+    int c = oc_popcount(state & (b2 - 1));
+    uint64_t out = state ^ b2;
+    c += oc_popcount(out & (b1 - 1));
+    out |= b1;
+    *sign = (c & 1) ? -1 : 1;
+    return out;
+}
+
+/* ------------------------------------------------------------- term tables */
+
+/* BaseGenerate4KTable + FindM1 + ConvertStateLabel
+ * (pseudopotential_model_base.hpp:62-139): m4, m3, m2 descend from maxLz in
+ * steps of 2; k = (maxLz - m)/2; an out-of-range m1 stores the label of
+ * maxLz + 2, i.e. (uint16)-1. */
+void oc_sphere_ktable(int maxLz, uint16_t* kt)
+{
+    int64_t it = 0;
+    for (int m4 = maxLz; m4 >= -maxLz; m4 -= 2)
+        for (int m3 = maxLz; m3 >= -maxLz; m3 -= 2)
+            for (int m2 = maxLz; m2 >= -maxLz; m2 -= 2, ++it) {
+                int m1 = m3 + m4 - m2;
+                if (!(m1 <= maxLz && m1 >= -maxLz)) m1 = maxLz + 2;
+                kt[it] = (uint16_t)((-m1 + maxLz) / 2);
+            }
+}
+
+typedef struct {
+    int layout, K;
+    const uint16_t* kt;
+    const double* vt;
+    int cplx;
+} oc_tables;
+
+/* GetK1: sphere term_tables.cpp:109-124 (count 0 when k1 >= kMax);
+ * flux lattice term_tables.cpp:95-112 (folded on k3 >= k4, always one k1). */
+static int oc_get_k1(const oc_tables* t, int k2, int k3, int k4, int* k1)
+{
+    if (t->layout == OC_LAYOUT_SPHERE) {
+        int v = t->kt[((int64_t)k4 * t->K + k3) * t->K + k2];
+        *k1 = v;
+        return v < t->K;
+    }
+    int t1 = k4, t2 = k3;
+    if (k4 > k3) { t1 = k3; t2 = k4; }
+    *k1 = t->kt[((int64_t)t1 * t->K + t2 - (t1 * (t1 + 1)) / 2) * t->K + k2];
+    return 1;
+}
+
+/* GetVkkkk: sphere term_tables.cpp:137-149; flux lattice term_tables.cpp:150-175. */
+static int64_t oc_v_index(const oc_tables* t, int k1, int k2, int k3, int k4)
+{
+    if (t->layout == OC_LAYOUT_SPHERE) return ((int64_t)k4 * t->K + k3) * t->K + k2;
+    int t1 = k4, t2 = k3, t3 = k2;
+    if (k4 > k3) { t1 = k3; t2 = k4; t3 = k1; }
+    return ((int64_t)t1 * t->K + t2 - (t1 * (t1 + 1)) / 2) * t->K + t3;
+}
+
+/* ---------------------------------------------------- row accumulator (map) */
+
+/* MappedSparseMatrix::Insert(i,j) += el (mapped_sparse_matrix.hpp:355-370):
+ * a per-row open-addressed table keyed by column; values accumulate in
+ * generation order starting from 0.0. */
+typedef struct {
+    int64_t* keys;
+    int32_t* slot;  /* index into the entry arrays, -1 empty */
+    int64_t* ecol;
+    double complex* eval;
+    int n, cap, mask;
+} oc_rowacc;
+
+static void acc_init(oc_rowacc* a)
+{
+    a->cap = 1 << 14;
+    a->mask = (1 << 15) - 1;
+    a->keys = (int64_t*)malloc(sizeof(int64_t) * (a->mask + 1));
+    a->slot = (int32_t*)malloc(sizeof(int32_t) * (a->mask + 1));
+    a->ecol = (int64_t*)malloc(sizeof(int64_t) * a->cap);
+    a->eval = (double complex*)malloc(sizeof(double complex) * a->cap);
+    for (int i = 0; i <= a->mask; ++i) a->slot[i] = -1;
+    a->n = 0;
+}
+static void acc_free(oc_rowacc* a)
+{
+    free(a->keys); free(a->slot); free(a->ecol); free(a->eval);
+}
+static void acc_clear(oc_rowacc* a)
+{
+    for (int e = 0; e < a->n; ++e) {
+        /* clear only the used slots */
+        uint64_t h = ((uint64_t)a->ecol[e] * 0x9E3779B97F4A7C15ull) >> 49;
+        int s = (int)(h & a->mask);
+        while (a->slot[s] != -1) {
+            a->slot[s] = -1;
+            s = (s + 1) & a->mask;
+        }
+    }
+    a->n = 0;
+}
+static int acc_add(oc_rowacc* a, int64_t col, double complex v)
+{
+    uint64_t h = ((uint64_t)col * 0x9E3779B97F4A7C15ull) >> 49;
+    int s = (int)(h & a->mask);
+    while (a->slot[s] != -1) {
+        if (a->keys[s] == col) {
+            a->eval[a->slot[s]] += v;
+            return 0;
+        }
+        s = (s + 1) & a->mask;
+    }
+    if (a->n >= a->cap) return -1;
+    a->keys[s] = col;
+    a->slot[s] = a->n;
+    a->ecol[a->n] = col;
+    a->eval[a->n] = 0.0 + v; /* Insert() default-constructs 0, then += el */
+    a->n++;
+    return 0;
+}
+
+typedef struct { int64_t col; double complex v; } oc_entry;
+static int cmp_entry(const void* x, const void* y)
+{
+    int64_t a = ((const oc_entry*)x)->col, b = ((const oc_entry*)y)->col;
+    return (a > b) - (a < b);
+}
+
+/* ------------------------------------------------------------------- build */
+
+/* One row of Add_CdC_Terms + Add_CdCdCC_Terms
+ * (spinless_fermion_hamiltonian.hpp:143-257 and 281-403). */
+static int oc_build_row(oc_rowacc* acc, int N, int Norb, const uint64_t* basis, int64_t dim,
+                        int64_t row, const oc_tables* T, const double* eps, double tol,
+                        int full)
+{
+    (void)N;
+    const uint64_t in = basis[row];
+    const uint64_t highest = 1ull << (Norb - 1);
+    if (eps) {
+        /* Quadratic terms: k1 = GetK1(k2) = k2, so only diagonal elements. */
+        uint64_t t2 = in;
+        while (t2) {
+            uint64_t b2 = t2 & (0 - t2);
+            int k2 = oc_popcount(b2 - 1);
+            double complex el = T->cplx ? (eps[2 * k2] + I * eps[2 * k2 + 1]) : eps[k2];
+            if (cabs(el) > tol && acc_add(acc, row, el)) return -1;
+            t2 ^= b2;
+        }
+    }
+    const uint64_t mask = (1ull << Norb) - 1;
+    uint64_t t4 = in;
+    while (t4) {
+        uint64_t b4 = t4 & (0 - t4);
+        int k4 = oc_popcount(b4 - 1);
+        uint64_t t3 = in ^ b4;
+        while (t3) {
+            uint64_t b3 = t3 & (0 - t3);
+            int k3 = oc_popcount(b3 - 1);
+            if (k3 != k4) {
+                uint64_t t2 = mask ^ (t3 ^ b3);
+                while (t2) {
+                    uint64_t b2 = t2 & (0 - t2);
+                    int k2 = oc_popcount(b2 - 1);
+                    int k1;
+                    if (oc_get_k1(T, k2, k3, k4, &k1)) {
+                        uint64_t b1 = 1ull << k1;
+                        if ((!(b1 & in) || ((b1 & b3) || (b1 & b4))) &&
+                            (!(b2 & in) || ((b2 & b3) || (b2 & b4))) && !(b1 & b2)) {
+                            int sign;
+                            uint64_t out = oc_cdcdcc(&sign, b1, b2, b3, b4, in, highest);
+                            if (sign != 0 && (full || out >= in)) {
+                                int64_t idx = oc_find_index(basis, dim, out);
+                                if (idx >= 0) {
+                                    int64_t vi = oc_v_index(T, k1, k2, k3, k4);
+                                    double complex v = T->cplx
+                                        ? (T->vt[2 * vi] + I * T->vt[2 * vi + 1])
+                                        : T->vt[vi];
+                                    double complex el = (double)sign * v;
+                                    if (cabs(el) > tol && acc_add(acc, idx, el)) return -1;
+                                }
+                            }
+                        }
+                    }
+                    t2 ^= b2;
+                }
+            }
+            t3 ^= b3;
+        }
+        t4 ^= b4;
+    }
+    return 0;
+}
+
+typedef struct {
+    int N, Norb;
+    const uint64_t* basis;
+    int64_t dim;
+    oc_tables T;
+    const double* eps;
+    double tol;
+    int full;
+} oc_build_ctx;
+
+/* Trim (mapped_sparse_matrix.hpp:423-462, the erase-current form: an entry is
+ * dropped when |v| < tol) and sort by column (crs_sparse_matrix.hpp:193-254).
+ * Returns the number of kept entries written to (col, val) starting at off. */
+static int64_t oc_emit_row(oc_rowacc* acc, oc_entry* scratch, int cplx, double tol,
+                           int32_t* col, double* val, int64_t off, int64_t cap,
+                           int64_t* n_trim)
+{
+    int64_t k = 0;
+    for (int e = 0; e < acc->n; ++e) {
+        if (cabs(acc->eval[e]) < tol) {
+            if (n_trim) (*n_trim)++;
+            continue;
+        }
+        scratch[k].col = acc->ecol[e];
+        scratch[k].v = acc->eval[e];
+        ++k;
+    }
+    qsort(scratch, (size_t)k, sizeof(oc_entry), cmp_entry);
+    if (col) {
+        if (off + k > cap) return -1;
+        for (int64_t e = 0; e < k; ++e) {
+            col[off + e] = (int32_t)scratch[e].col;
+            if (cplx) {
+                val[2 * (off + e)] = creal(scratch[e].v);
+                val[2 * (off + e) + 1] = cimag(scratch[e].v);
+            } else {
+                val[off + e] = creal(scratch[e].v);
+            }
+        }
+    }
+    return k;
+}
+
+static int oc_table_ok(int layout, int kMax, int Norb)
+{
+    if (kMax < Norb) return 0;
+    if (layout != OC_LAYOUT_SPHERE && layout != OC_LAYOUT_OFL) return 0;
+    return 1;
+}
+
+int64_t oc_build(int N, int Norb, const uint64_t* basis, int64_t dim,
+                 int64_t row0, int64_t row1,
+                 int layout, int kMax, const uint16_t* ktab, const double* vtab,
+                 const double* eps, int is_complex, double tol, int full,
+                 int64_t* rowptr, int32_t* col, double* val, int64_t cap,
+                 int64_t* n_trimmed)
+{
+    if (!oc_table_ok(layout, kMax, Norb) || row0 < 0 || row1 > dim || row0 > row1) return -1;
+    oc_tables T = {layout, kMax, ktab, vtab, is_complex};
+    oc_rowacc acc;
+    acc_init(&acc);
+    oc_entry* scratch = (oc_entry*)malloc(sizeof(oc_entry) * acc.cap);
+    int64_t nnz = 0;
+    if (n_trimmed) *n_trimmed = 0;
+    if (rowptr) rowptr[0] = 0;
+    for (int64_t r = row0; r < row1; ++r) {
+        acc_clear(&acc);
+        if (oc_build_row(&acc, N, Norb, basis, dim, r, &T, eps, tol, full)) {
+            nnz = -1;
+            break;
+        }
+        int64_t k = oc_emit_row(&acc, scratch, is_complex, tol, rowptr ? col : NULL, val,
+                                nnz, cap, n_trimmed);
+        if (k < 0) { nnz = -1; break; }
+        nnz += k;
+        if (rowptr) rowptr[r - row0 + 1] = nnz;
+    }
+    free(scratch);
+    acc_free(&acc);
+    return nnz;
+}
+
+/* ------------------------------------------------------- multithreaded build */
+
+typedef struct {
+    oc_build_ctx* ctx;
+    int64_t r0, r1;
+    int64_t* counts; /* per row, relative to r0 in pass 1 */
+    int64_t* rowptr; int32_t* col; double* val;
+    int pass, err;
+} oc_build_job;
+
+static void* oc_build_worker(void* p)
+{
+    oc_build_job* j = (oc_build_job*)p;
+    oc_build_ctx* c = j->ctx;
+    oc_rowacc acc;
+    acc_init(&acc);
+    oc_entry* scratch = (oc_entry*)malloc(sizeof(oc_entry) * acc.cap);
+    for (int64_t r = j->r0; r < j->r1; ++r) {
+        acc_clear(&acc);
+        if (oc_build_row(&acc, c->N, c->Norb, c->basis, c->dim, r, &c->T, c->eps, c->tol,
+                         c->full)) { j->err = 1; break; }
+        if (j->pass == 0) {
+            j->counts[r] = oc_emit_row(&acc, scratch, c->T.cplx, c->tol, NULL, NULL, 0, 0, NULL);
+        } else {
+            oc_emit_row(&acc, scratch, c->T.cplx, c->tol, j->col, j->val, j->rowptr[r],
+                        j->rowptr[r + 1], NULL);
+        }
+    }
+    free(scratch);
+    acc_free(&acc);
+    return NULL;
+}
+
+int64_t oc_build_mt(int N, int Norb, const uint64_t* basis, int64_t dim,
+                    int layout, int kMax, const uint16_t* ktab, const double* vtab,
+                    const double* eps, int is_complex, double tol, int full,
+                    int64_t* rowptr, int32_t* col, double* val, int64_t cap,
+                    int nthreads)
+{
+    if (!oc_table_ok(layout, kMax, Norb)) return -1;
+    if (nthreads < 1) nthreads = 1;
+    oc_build_ctx ctx = {N, Norb, basis, dim, {layout, kMax, ktab, vtab, is_complex}, eps, tol, full};
+    int64_t* counts = (int64_t*)calloc((size_t)dim + 1, sizeof(int64_t));
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
+    oc_build_job* jobs = (oc_build_job*)calloc((size_t)nthreads, sizeof(oc_build_job));
+    int64_t nnz = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int t = 0; t < nthreads; ++t) {
+            jobs[t].ctx = &ctx;
+            /* interleaved blocks of 256 rows would balance better; contiguous
+             * blocks keep the code simple and the timing conservative */
+            jobs[t].r0 = dim * t / nthreads;
+            jobs[t].r1 = dim * (t + 1) / nthreads;
+            jobs[t].counts = counts;
+            jobs[t].rowptr = rowptr; jobs[t].col = col; jobs[t].val = val;
+            jobs[t].pass = pass;
+            jobs[t].err = 0;
+            pthread_create(&th[t], NULL, oc_build_worker, &jobs[t]);
+        }
+        int err = 0;
+        for (int t = 0; t < nthreads; ++t) {
+            pthread_join(th[t], NULL);
+            err |= jobs[t].err;
+        }
+        if (err) { nnz = -1; break; }
+        if (pass == 0) {
+            rowptr[0] = 0;
+            for (int64_t r = 0; r < dim; ++r) rowptr[r + 1] = rowptr[r] + counts[r];
+            nnz = rowptr[dim];
+            if (nnz > cap || !col) break;
+        }
+    }
+    free(jobs); free(th); free(counts);
+    return nnz;
+}
+
+/* -------------------------------------------------------------------- SpMV */
+
+/* SymmetricMatrixVectorMultiply, non-MKL branch with ALPHA=1, BETA=0
+ * (sparse_linear_algebra.hpp:142-165): y = U x + (U - D)^H x. */
+static void oc_spmv_rows(int64_t r0, int64_t r1, const int64_t* rp, const int32_t* col,
+                         const double* val, int cplx, const double* x, double* y)
+{
+    if (!cplx) {
+        for (int64_t i = r0; i < r1; ++i)
+            for (int64_t j = rp[i]; j < rp[i + 1]; ++j) {
+                int c = col[j];
+                double v = val[j];
+                y[i] += v * x[c];
+                if (i != c) y[c] += v * x[i];
+            }
+    } else {
+        const double complex* X = (const double complex*)x;
+        double complex* Y = (double complex*)y;
+        const double complex* V = (const double complex*)val;
+        for (int64_t i = r0; i < r1; ++i)
+            for (int64_t j = rp[i]; j < rp[i + 1]; ++j) {
+                int c = col[j];
+                double complex v = V[j];
+                Y[i] += v * X[c];
+                if (i != c) Y[c] += conj(v) * X[i];
+            }
+    }
+}
+
+void oc_spmv_sym(int64_t dim, const int64_t* rowptr, const int32_t* col,
+                 const double* val, int is_complex, const double* x, double* y)
+{
+    memset(y, 0, sizeof(double) * (size_t)dim * (is_complex ? 2 : 1));
+    oc_spmv_rows(0, dim, rowptr, col, val, is_complex, x, y);
+}
+
+typedef struct {
+    int64_t r0, r1, dim;
+    const int64_t* rp; const int32_t* col; const double* val;
+    int cplx; const double* x; double* y;
+} oc_spmv_job;
+
+static void* oc_spmv_worker(void* p)
+{
+    oc_spmv_job* j = (oc_spmv_job*)p;
+    memset(j->y, 0, sizeof(double) * (size_t)j->dim * (j->cplx ? 2 : 1));
+    oc_spmv_rows(j->r0, j->r1, j->rp, j->col, j->val, j->cplx, j->x, j->y);
+    return NULL;
+}
+
+void oc_spmv_sym_mt(int64_t dim, const int64_t* rowptr, const int32_t* col,
+                    const double* val, int is_complex, const double* x, double* y,
+                    int nthreads)
+{
+    if (nthreads <= 1) {
+        oc_spmv_sym(dim, rowptr, col, val, is_complex, x, y);
+        return;
+    }
+    size_t w = (size_t)dim * (is_complex ? 2 : 1);
+    double* priv = (double*)malloc(sizeof(double) * w * (size_t)nthreads);
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
+    oc_spmv_job* jobs = (oc_spmv_job*)malloc(sizeof(oc_spmv_job) * nthreads);
+    /* balance by nnz: each thread takes a contiguous block of ~nnz/nthreads */
+    int64_t nnz = rowptr[dim], r = 0;
+    for (int t = 0; t < nthreads; ++t) {
+        int64_t target = nnz * (t + 1) / nthreads;
+        int64_t r0 = r;
+        while (r < dim && rowptr[r] < target) ++r;
+        if (t == nthreads - 1) r = dim;
+        jobs[t] = (oc_spmv_job){r0, r, dim, rowptr, col, val, is_complex, x, priv + w * t};
+        pthread_create(&th[t], NULL, oc_spmv_worker, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    for (size_t i = 0; i < w; ++i) {
+        double s = 0.0;
+        for (int t = 0; t < nthreads; ++t) s += priv[w * t + i];
+        y[i] = s;
+    }
+    free(priv); free(th); free(jobs);
+}

@@ -1,0 +1,225 @@
+"""GPU parity: basis, CRS build, SpMV and Lanczos through the C ABI against the
+CPU oracle (bit-exact for integer data and FP64 matrix values)."""
+import sys
+
+import numpy as np
+import pytest
+
+import hamiltonian_diagonalization_amd as hd
+from hamiltonian_diagonalization_amd import models
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+SECTORS = [
+    ("lz N=8 2S=21 lz=0", hd.SECTOR_LZ, (0, 21), 8, 22),
+    ("lz N=4 2S=9 lz=2", hd.SECTOR_LZ, (2, 9), 4, 10),
+    ("lz N=5 2S=11 lz=-3", hd.SECTOR_LZ, (-3, 11), 5, 12),
+    ("k2d 3x4 N=4 (1,2)", hd.SECTOR_K2D, (1, 2, 3, 4), 4, 12),
+    ("k2d 6x5 N=10 (0,0)", hd.SECTOR_K2D, (0, 0, 6, 5), 10, 30),
+    ("k1d 4 N=4 ky=2", hd.SECTOR_K1D, (2, 4), 4, 12),
+    ("none N=3 Norb=9", hd.SECTOR_NONE, (), 3, 9),
+]
+
+
+@pytest.mark.parametrize("name,kind,args,n,norb", SECTORS, ids=[s[0] for s in SECTORS])
+def test_basis_matches_oracle(oracle, name, kind, args, n, norb):
+    say("basis", name)
+    want = oracle.basis(oracle.sector(kind, *args), n, norb)
+    b = hd.FockBasis(hd.Sector(kind, *args), n, norb)
+    assert b.dim == len(want)
+    got = b.states()
+    assert np.array_equal(got, want)
+    idx = b.find_index(got, offset=7)
+    assert np.array_equal(idx, np.arange(len(got), dtype=np.uint64) + 7)
+    # states outside the sector are not found
+    bad = np.array([0, (1 << n) - 1 if kind == hd.SECTOR_NONE else 1, 1 << 63], dtype=np.uint64)
+    res = b.find_index(bad)
+    for s, r in zip(bad, res):
+        inside = len(want) and np.searchsorted(want, s) < len(want) and want[np.searchsorted(want, s)] == s
+        assert (r != np.uint64(2**64 - 1)) == bool(inside)
+
+
+def test_basis_config2(oracle):
+    say("basis N=12 2S=33 lz=0 (7,764,392 states)")
+    b = hd.FockBasis(hd.Sector.lz(0, 33), 12, 34)
+    assert b.dim == 7764392
+    want = oracle.basis(oracle.sector(oracle.SECTOR_LZ, 0, 33), 12, 34)
+    assert np.array_equal(b.states(), want)
+
+
+def test_empty_sector():
+    say("empty sector")
+    with pytest.raises(hd.HDError) as e:
+        hd.FockBasis(hd.Sector.lz(1, 21), 8, 22)  # wrong parity
+    assert e.value.code == hd.HD_ERR_EMPTY
+
+
+def _random_sphere(two_s, seed):
+    rng = np.random.default_rng(seed)
+    K = two_s + 1
+    return hd.sphere_ktable(two_s), rng.normal(size=K ** 3)
+
+
+def _assert_crs_equal(h, ref, row0=0, row1=None):
+    rp, col, val = h.crs(row0, row1)
+    assert np.array_equal(rp, ref.rowptr)
+    assert np.array_equal(col, ref.col)
+    assert np.array_equal(val, ref.values()), np.max(np.abs(val - ref.values()))
+
+
+@pytest.mark.parametrize("two_s,n,lz,seed", [(7, 3, 1, 0), (9, 4, 0, 1), (11, 4, 2, 2)])
+def test_build_random_sphere_bit_exact(oracle, two_s, n, lz, seed):
+    say("build random sphere", two_s, n, lz)
+    kt, vt = _random_sphere(two_s, seed)
+    K = two_s + 1
+    b = hd.FockBasis(hd.Sector.lz(lz, two_s), n, K)
+    t = hd.TermTables(hd.LAYOUT_SPHERE, K, kt, vt)
+    bas = b.states()
+    for storage, full in ((hd.STORAGE_UPPER_REF, False), (hd.STORAGE_FULL, True)):
+        ref = oracle.build(n, K, bas, oracle.LAYOUT_SPHERE, K, kt, vt, full=full)
+        h = hd.Hamiltonian(b, t, storage=storage)
+        _assert_crs_equal(h, ref)
+        assert h.info()["n_trimmed"] == ref.n_trimmed
+
+
+def test_build_config1_bit_exact(oracle):
+    say("build N=8 2S=21 Coulomb")
+    b, t = models.sphere_problem(8, 21)
+    bas = b.states()
+    kt, vt = t.ktable, t.vtable
+    up = oracle.build(8, 22, bas, oracle.LAYOUT_SPHERE, 22, kt, vt, full=False)
+    h = hd.Hamiltonian(b, t, storage=hd.STORAGE_UPPER_REF)
+    _assert_crs_equal(h, up)
+    full = oracle.build(8, 22, bas, oracle.LAYOUT_SPHERE, 22, kt, vt, full=True)
+    hf = hd.Hamiltonian(b, t, storage=hd.STORAGE_FULL)
+    _assert_crs_equal(hf, full)
+    # FP32 copy is the rounded FP64 copy
+    _, _, v32 = hf.crs(precision=hd.F32)
+    assert np.array_equal(v32, full.values().astype(np.float32))
+    # the full matrix is symmetric to rounding and has the upper pattern mirrored
+    import scipy.sparse as sp
+    M = sp.csr_matrix((full.values(), full.col, full.rowptr), shape=(len(bas),) * 2)
+    assert abs(M - M.T).max() <= 1e-15 * abs(M).max()
+
+
+@pytest.mark.parametrize("X,Y,n,kx,ky,seed", [(3, 3, 3, 0, 0, 5), (3, 4, 5, 1, 2, 6), (6, 5, 4, 2, 3, 7)])
+def test_build_ofl_complex_bit_exact(oracle, X, Y, n, kx, ky, seed):
+    say("build ofl", X, Y, n)
+    kt, vt, eps = models.ofl_tables(X, Y, seed)
+    K = X * Y
+    b = hd.FockBasis(hd.Sector.k2d(kx, ky, X, Y), n, K)
+    t = hd.TermTables(hd.LAYOUT_OFL, K, kt, vt, eps)
+    bas = b.states()
+    for storage, full in ((hd.STORAGE_UPPER_REF, False), (hd.STORAGE_FULL, True)):
+        ref = oracle.build(n, K, bas, oracle.LAYOUT_OFL, K, kt, vt, eps=eps, is_complex=True, full=full)
+        h = hd.Hamiltonian(b, t, storage=storage)
+        _assert_crs_equal(h, ref)
+
+
+def test_build_config3_rows_bit_exact(oracle):
+    say("build flux lattice 6x5 N=10 (1,001,402 states), sampled rows")
+    b, t = models.ofl_problem(6, 5, 10, 0, 0, seed=0)
+    assert b.dim == 1001402
+    h = hd.Hamiltonian(b, t, storage=hd.STORAGE_FULL)
+    bas = b.states()
+    for r0 in (0, 500000, b.dim - 700):
+        ref = oracle.build(10, 30, bas, oracle.LAYOUT_OFL, 30, t.ktable, t.vtable, eps=t.eps,
+                           is_complex=True, full=True, row0=r0, row1=r0 + 700)
+        _assert_crs_equal(h, ref, r0, r0 + 700)
+
+
+def _dense_eigs(oracle, crs, dim, k):
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    full = crs
+    M = sp.csr_matrix((full.values(), full.col, full.rowptr), shape=(dim, dim))
+    if dim <= 3000:
+        return np.linalg.eigvalsh(M.toarray())[:k]
+    return np.sort(sla.eigsh(M, k=k, which="SA", tol=1e-14, ncv=60)[0])
+
+
+def test_spmv_config1(oracle):
+    say("spmv N=8 2S=21")
+    b, t = models.sphere_problem(8, 21)
+    bas = b.states()
+    up = oracle.build(8, 22, bas, oracle.LAYOUT_SPHERE, 22, t.ktable, t.vtable)
+    h = hd.Hamiltonian(b, t)
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=b.dim)
+    want = oracle.spmv_sym(up, x)
+    for lanes in (4, 8, 16, 32, 64):
+        h.set_lanes(lanes)
+        got = h.spmv(x)
+        assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+        g32 = h.spmv(x.astype(np.float32), precision=hd.F32)
+        assert np.max(np.abs(g32 - want)) <= 1e-5 * np.max(np.abs(want))
+    assert np.allclose(h.simple(x), want, rtol=0, atol=1e-13 * np.max(np.abs(want)))
+
+
+def test_spmv_complex(oracle):
+    say("spmv ofl 3x4 N=5 complex")
+    kt, vt, eps = models.ofl_tables(3, 4, 3)
+    b = hd.FockBasis(hd.Sector.k2d(1, 2, 3, 4), 5, 12)
+    t = hd.TermTables(hd.LAYOUT_OFL, 12, kt, vt, eps)
+    up = oracle.build(5, 12, b.states(), oracle.LAYOUT_OFL, 12, kt, vt, eps=eps, is_complex=True)
+    h = hd.Hamiltonian(b, t)
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=b.dim) + 1j * rng.normal(size=b.dim)
+    want = oracle.spmv_sym(up, x)
+    got = h.spmv(x)
+    assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+    g32 = h.spmv(x.astype(np.complex64), precision=hd.F32)
+    assert np.max(np.abs(g32 - want)) <= 1e-5 * np.max(np.abs(want))
+
+
+def test_lanczos_config1(oracle):
+    say("lanczos N=8 2S=21 nev=5")
+    b, t = models.sphere_problem(8, 21)
+    bas = b.states()
+    up = oracle.build(8, 22, bas, oracle.LAYOUT_SPHERE, 22, t.ktable, t.vtable)
+    full = oracle.build(8, 22, bas, oracle.LAYOUT_SPHERE, 22, t.ktable, t.vtable, full=True)
+    want = _dense_eigs(oracle, full, b.dim, 5)
+    h = hd.Hamiltonian(b, t)
+    r = h.lanczos(nev=5, tol=1e-12, seed=0)
+    say(r)
+    assert r.converged and np.all(r.residuals <= 1e-12)
+    assert np.max(np.abs(r.evals - want) / np.maximum(1, np.abs(want))) < 1e-10
+    # eigenvector residual through the oracle's naive SpMV
+    x0 = h.eigenvector(0)
+    res = oracle.spmv_sym(up, x0) - r.evals[0] * x0
+    assert np.linalg.norm(res) < 1e-11 * max(1, abs(r.evals[0]))
+    r32 = h.lanczos(nev=5, tol=1e-10, precision=hd.F32, seed=0)
+    say(r32)
+    assert r32.converged and np.all(r32.residuals <= 1e-10)
+    assert np.max(np.abs(r32.evals - want) / np.maximum(1, np.abs(want))) < 1e-10
+
+
+def test_lanczos_laughlin_zero_mode():
+    say("lanczos laughlin N=5 2S=12 V1 only")
+    b, t = models.sphere_problem(5, 12, 0, pseudopotentials=[0.0, 1.0])
+    h = hd.Hamiltonian(b, t)
+    r = h.lanczos(nev=3, tol=1e-11, seed=3)
+    assert r.converged
+    assert abs(r.evals[0]) < 1e-10 and r.evals[1] > 1e-3
+
+
+def test_lanczos_complex(oracle):
+    say("lanczos ofl 3x4 N=5 complex nev=6")
+    kt, vt, eps = models.ofl_tables(3, 4, 11)
+    b = hd.FockBasis(hd.Sector.k2d(0, 0, 3, 4), 5, 12)
+    t = hd.TermTables(hd.LAYOUT_OFL, 12, kt, vt, eps)
+    full = oracle.build(5, 12, b.states(), oracle.LAYOUT_OFL, 12, kt, vt, eps=eps, is_complex=True,
+                        full=True)
+    want = _dense_eigs(oracle, full, b.dim, 6)
+    h = hd.Hamiltonian(b, t)
+    for prec in (hd.F64, hd.F32):
+        r = h.lanczos(nev=6, tol=1e-10, precision=prec, seed=0)
+        say(prec, r)
+        assert r.converged
+        assert np.max(np.abs(r.evals - want) / np.maximum(1, np.abs(want))) < 1e-10
