@@ -32,7 +32,7 @@ HD_OK, HD_ERR_ARG, HD_ERR_HIP, HD_ERR_NOMEM, HD_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 HD_ERR_NOT_CONVERGED, HD_ERR_EMPTY = 5, 6
 SECTOR_NONE, SECTOR_LZ, SECTOR_K2D, SECTOR_K1D = 0, 1, 2, 3
 LAYOUT_SPHERE, LAYOUT_OFL = 0, 1
-F64, F32 = 0, 1
+F64, F32, CODED = 0, 1, 2
 STORAGE_FULL, STORAGE_UPPER_REF = 0, 1
 START_RANDOM, START_USER = 0, 1
 ZERO_TOL = 1e-14  # _SPARSE_ZERO_TOL_ (src/hamiltonians/hamiltonian_data.hpp:54)
@@ -104,8 +104,11 @@ SIGNATURES = {
     "hd_ham_destroy": (ctypes.c_int, [_P]),
     "hd_ham_info": (ctypes.c_int, [_P, ctypes.POINTER(_HamInfo)]),
     "hd_ham_copy_crs": (ctypes.c_int, [_P, _I64, _I64, _I32, _P, _P, _P]),
+    "hd_ham_device_crs": (ctypes.c_int, [_P, _PP, _PP, _PP, _PP]),
+    "hd_ham_device_coded": (ctypes.c_int, [_P, _PP, _PP, _PP, ctypes.POINTER(_I32)]),
     "hd_spmv": (ctypes.c_int, [_P, _I32, _P, _P]),
     "hd_spmv_host": (ctypes.c_int, [_P, _P, _P]),
+    "hd_spmv_time": (ctypes.c_int, [_P, _I32, _I32, ctypes.POINTER(_D)]),
     "hd_spmv_set_lanes": (ctypes.c_int, [_P, _I32]),
     "hd_spmv_get_lanes": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
     "hd_lanczos": (ctypes.c_int, [_P, ctypes.POINTER(_LanczosOpts), _P,
@@ -302,8 +305,10 @@ class Hamiltonian:
     """SpinlessFermionHamiltonian build + CRS + SpMV + Lanczos on the device."""
 
     def __init__(self, basis: FockBasis, tables: TermTables, storage=STORAGE_FULL,
-                 zero_tol=ZERO_TOL, values=(F64, F32)):
-        mask = (1 if F64 in values else 0) | (2 if F32 in values else 0)
+                 zero_tol=ZERO_TOL, values=None):
+        if values is None:
+            values = (F64, F32, CODED) if storage == STORAGE_FULL else (F64, F32)
+        mask = (1 if F64 in values else 0) | (2 if F32 in values else 0) | (4 if CODED in values else 0)
         h = ctypes.c_void_p()
         _check(lib().hd_ham_build(basis.h, tables.h, storage, zero_tol, mask, ctypes.byref(h)))
         self.h = h
@@ -336,8 +341,21 @@ class Hamiltonian:
         _check(lib().hd_ham_copy_crs(self.h, row0, row1, precision, _ptr(rp), _ptr(col), _ptr(val)))
         return rp, col[:n], val[:n]
 
+    def device_crs(self):
+        """Device pointers (rowptr, col, val64, val32) as integers (0 if absent)."""
+        p = [ctypes.c_void_p() for _ in range(4)]
+        _check(lib().hd_ham_device_crs(self.h, *[ctypes.byref(x) for x in p]))
+        return tuple(x.value or 0 for x in p)
+
+    def device_coded(self):
+        """Device pointers (code, diag, dict) as integers and the dictionary size."""
+        p = [ctypes.c_void_p() for _ in range(3)]
+        nd = ctypes.c_int32()
+        _check(lib().hd_ham_device_coded(self.h, *[ctypes.byref(x) for x in p], ctypes.byref(nd)))
+        return tuple(x.value or 0 for x in p) + (nd.value,)
+
     def vec_dtype(self, precision=F64):
-        if precision == F64:
+        if precision in (F64, CODED):
             return np.complex128 if self.is_complex else np.float64
         return np.complex64 if self.is_complex else np.float32
 
@@ -361,6 +379,22 @@ class Hamiltonian:
         y = np.zeros_like(xx)
         _check(lib().hd_spmv_host(self.h, _ptr(xx), _ptr(y)))
         return y
+
+    def spmv_time_ms(self, precision=F64, iters=10) -> float:
+        """Mean SpMV kernel time (HIP events on the library stream)."""
+        v = ctypes.c_double()
+        _check(lib().hd_spmv_time(self.h, precision, iters, ctypes.byref(v)))
+        return v.value
+
+    def spmv_bytes(self, precision=F64) -> int:
+        """Algorithmic HBM bytes of one SpMV: nnz*(4 + sizeof(value)) +
+        dim*(8 + sizeof(x) + sizeof(y)), x counted once."""
+        z = 2 if self.is_complex else 1
+        if precision == CODED:
+            # column + 16-bit code per entry; rowptr, diagonal, x, y per row
+            return self.nnz * 6 + self.dim * (8 + 3 * 8 * z)
+        vs = (8 if precision == F64 else 4) * z
+        return self.nnz * (4 + vs) + self.dim * (8 + 2 * vs)
 
     def set_lanes(self, lanes: int):
         _check(lib().hd_spmv_set_lanes(self.h, lanes))

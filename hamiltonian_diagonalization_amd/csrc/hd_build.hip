@@ -27,6 +27,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <map>
 
 #include "hd_internal.hpp"
 
@@ -75,7 +76,44 @@ struct Derived {
     std::vector<uint8_t> dflag;  // K*K: bit0 first term present, bit1 second
     std::vector<double> eps;     // K (x2 complex)
     std::vector<uint8_t> epsflag;
+    // coded format: remap[pair*K + c] -> index into dict (distinct values)
+    std::vector<int16_t> remap;
+    std::vector<double> dict;  // (x2 complex)
+    int ndict = 0;
+    bool codable = true;
 };
+
+// Dictionary of the distinct kept hop values (bitwise); the code of an entry is
+// its dictionary index, the sign of the hop goes into bit 15.
+static void build_dictionary(Derived& D)
+{
+    const int w = D.cplx ? 2 : 1;
+    D.remap.assign(D.kpair.size(), (int16_t)-1);
+    D.dict.clear();
+    std::map<std::pair<uint64_t, uint64_t>, int> seen;
+    for (size_t i = 0; i < D.kpair.size(); ++i) {
+        const uint8_t e = D.kpair[i];
+        if (e == 0xFF || (e & 0x80)) continue;
+        uint64_t re, im = 0;
+        std::memcpy(&re, &D.pval[i * w], 8);
+        if (D.cplx) std::memcpy(&im, &D.pval[i * w + 1], 8);
+        auto it = seen.find({re, im});
+        int code;
+        if (it == seen.end()) {
+            code = (int)seen.size();
+            seen[{re, im}] = code;
+            for (int q = 0; q < w; ++q) D.dict.push_back(D.pval[i * w + q]);
+        } else {
+            code = it->second;
+        }
+        if (code >= 0x7FFF) {
+            D.codable = false;
+            return;
+        }
+        D.remap[i] = (int16_t)code;
+    }
+    D.ndict = (int)seen.size();
+}
 
 static int derive_tables(const hd_tables* t, const hd_basis* b, double tol, Derived& D)
 {
@@ -160,6 +198,7 @@ static int derive_tables(const hd_tables* t, const hd_basis* b, double tol, Deri
                 D.epsflag[k] = 1;
             }
         }
+    build_dictionary(D);
     return HD_OK;
 }
 
@@ -192,6 +231,7 @@ struct BuildArgs {
     const void* eps;
     const uint8_t* epsflag;
     int kpair_bytes;
+    const int16_t* remap;  // coded format only (pair*K + c -> dictionary index)
 };
 
 // Parity of the sign of c+_d c+_c c_b c_a |s> (k4 = a acts first), the first of
@@ -309,7 +349,7 @@ k_fill(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A,
        const int32_t* __restrict__ rank_blob, int rank_words, const int64_t* __restrict__ rowptr,
        const typename VT<CPLX>::T* __restrict__ diag, const uint8_t* __restrict__ dkeep,
        int32_t* __restrict__ col_out, typename VT<CPLX>::T* __restrict__ val_out,
-       unsigned int* __restrict__ err)
+       uint16_t* __restrict__ code_out, unsigned int* __restrict__ err)
 {
     using T = typename VT<CPLX>::T;
     extern __shared__ int32_t lds[];
@@ -328,6 +368,7 @@ k_fill(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A,
         if (dkeep[row] && pos < end) {
             col_out[pos] = (int32_t)row;
             val_out[pos] = diag[row];
+            if (code_out) code_out[pos] = 0xFFFF;
             ++pos;
         }
         auto body = [&](int a, int b, int c, int d, int ti, uint64_t out, bool tr) {
@@ -342,9 +383,11 @@ k_fill(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A,
                 r = 0;
             }
             T v = P[ti];
-            if (hop_parity(s, a, b, c, d)) v = vneg(v);
+            const int par = hop_parity(s, a, b, c, d);
+            if (par) v = vneg(v);
             col_out[pos] = (int32_t)r;
             val_out[pos] = v;
+            if (code_out) code_out[pos] = (uint16_t)((uint16_t)A.remap[ti] | (par ? 0x8000u : 0u));
             ++pos;
         };
         for_each_hop<UPPER>(s, A.K, A.norb_mask, s_kp, body);
@@ -358,8 +401,9 @@ k_fill(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A,
 template <bool CPLX, int CAP>
 __global__ void __launch_bounds__(64)
 k_sort(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col_in,
-       const typename VT<CPLX>::T* __restrict__ val_in, int32_t* __restrict__ col_out,
-       typename VT<CPLX>::T* __restrict__ val64, typename VT<CPLX>::F* __restrict__ val32)
+       const typename VT<CPLX>::T* __restrict__ val_in, const uint16_t* __restrict__ code_in,
+       int32_t* __restrict__ col_out, typename VT<CPLX>::T* __restrict__ val64,
+       typename VT<CPLX>::F* __restrict__ val32, uint16_t* __restrict__ code_out)
 {
     __shared__ int32_t key[CAP];
     __shared__ int16_t idx[CAP];
@@ -398,6 +442,7 @@ k_sort(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restric
             const typename VT<CPLX>::T v = val_in[start + idx[q]];
             if (val64) val64[start + q] = v;
             if (val32) val32[start + q] = to_f32(v);
+            if (code_out) code_out[start + q] = code_in[start + idx[q]];
         }
         __syncthreads();
     }
@@ -541,11 +586,11 @@ template <bool CPLX, bool UPPER, bool LZ>
 static int launch_fill(int grid, size_t lds, hipStream_t st, const hd_basis* b, const BuildArgs& A,
                        int rank_words, const int64_t* rowptr, const typename VT<CPLX>::T* diag,
                        const uint8_t* dkeep, int32_t* ctmp, typename VT<CPLX>::T* vtmp,
-                       unsigned int* err)
+                       uint16_t* codetmp, unsigned int* err)
 {
     HD_CHECK(set_lds((const void*)k_fill<CPLX, UPPER, LZ>, lds));
     hipLaunchKernelGGL((k_fill<CPLX, UPPER, LZ>), dim3(grid), dim3(1024), lds, st, b->d_states, b->dim, A,
-                       b->d_rank, rank_words, rowptr, diag, dkeep, ctmp, vtmp, err);
+                       b->d_rank, rank_words, rowptr, diag, dkeep, ctmp, vtmp, codetmp, err);
     return HD_OK;
 }
 
@@ -564,6 +609,9 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
     uint8_t *d_kpair = nullptr, *d_dflag = nullptr, *d_epsflag = nullptr, *d_dkeep = nullptr;
     T *d_pval = nullptr, *d_dterm = nullptr, *d_eps = nullptr, *d_diag = nullptr, *d_vtmp = nullptr;
     int32_t *d_counts = nullptr, *d_ctmp = nullptr;
+    uint16_t* d_codetmp = nullptr;
+    int16_t* d_remap = nullptr;
+    const bool coded = (value_mask & 4) != 0;
     long long* d_bsum = nullptr;
     int* d_bmax = nullptr;
     unsigned long long* d_misc = nullptr;  // [0] trimmed, [1] total, [2] max, [3] error flags
@@ -573,7 +621,7 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
     auto cleanup = [&]() {
         dfree(d_kpair); dfree(d_dflag); dfree(d_epsflag); dfree(d_dkeep); dfree(d_pval);
         dfree(d_dterm); dfree(d_eps); dfree(d_diag); dfree(d_vtmp); dfree(d_counts);
-        dfree(d_ctmp); dfree(d_bsum); dfree(d_bmax); dfree(d_misc);
+        dfree(d_ctmp); dfree(d_bsum); dfree(d_bmax); dfree(d_misc); dfree(d_codetmp); dfree(d_remap);
     };
 #define HD_TRY(x)                  \
     do {                           \
@@ -622,6 +670,12 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
     A.eps = d_eps;
     A.epsflag = d_epsflag;
     A.kpair_bytes = kp_bytes;
+    A.remap = nullptr;
+    if (coded) {
+        HD_TRY(dalloc_t(&d_remap, D.remap.size(), "code remap"));
+        HD_TRYHIP(hipMemcpyAsync(d_remap, D.remap.data(), D.remap.size() * 2, hipMemcpyHostToDevice, st));
+        A.remap = d_remap;
+    }
 
     // ---- count
     HD_TRY(dalloc_t(&d_counts, (size_t)dim, "row counts"));
@@ -673,6 +727,7 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
     const size_t nnz = (size_t)h->nnz;
     HD_TRY(dalloc_t(&d_ctmp, nnz, "unsorted columns"));
     HD_TRY(dalloc_t(&d_vtmp, nnz, "unsorted values"));
+    if (coded) HD_TRY(dalloc_t(&d_codetmp, nnz, "unsorted codes"));
     {
         const int rank_words = (int)b->rt.blob.size();
         const size_t lds = (size_t)rank_words * 4 + kp_alloc;
@@ -682,13 +737,13 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
         unsigned int* d_err = reinterpret_cast<unsigned int*>(d_misc + 3);
         ev.start(st);
         if (A.upper && lz)
-            HD_TRY((launch_fill<CPLX, true, true>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_err)));
+            HD_TRY((launch_fill<CPLX, true, true>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_codetmp, d_err)));
         else if (A.upper)
-            HD_TRY((launch_fill<CPLX, true, false>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_err)));
+            HD_TRY((launch_fill<CPLX, true, false>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_codetmp, d_err)));
         else if (lz)
-            HD_TRY((launch_fill<CPLX, false, true>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_err)));
+            HD_TRY((launch_fill<CPLX, false, true>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_codetmp, d_err)));
         else
-            HD_TRY((launch_fill<CPLX, false, false>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_err)));
+            HD_TRY((launch_fill<CPLX, false, false>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_codetmp, d_err)));
         HD_TRYHIP(hipGetLastError());
         h->t_fill = ev.stop(st);
         HD_TRYHIP(hipMemcpyAsync(host_misc, d_misc, sizeof(host_misc), hipMemcpyDeviceToHost, st));
@@ -699,18 +754,38 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
                                                 std::to_string(host_misc[3]) + ")");
         }
     }
-    dfree(d_diag); d_diag = nullptr;
+    if (coded) {
+        h->d_diag = d_diag;  // the coded format reads the diagonal from here
+        h->bytes += (size_t)dim * sizeof(T);
+    } else {
+        dfree(d_diag);
+    }
+    d_diag = nullptr;
     dfree(d_dkeep); d_dkeep = nullptr;
     // ---- sort into the final arrays
-    HD_TRY(dalloc_t(&h->d_col, nnz, "columns"));
+    // every entry array is padded by 8 zeroed entries: the SpMV reads aligned
+    // groups of 4 that may run past the last row
+    HD_TRY(dalloc_t(&h->d_col, nnz + 8, "columns"));
+    HD_TRYHIP(hipMemsetAsync(h->d_col + nnz, 0, 8 * sizeof(int32_t), st));
     h->bytes += nnz * 4;
     if (value_mask & 1) {
-        HD_TRY(dalloc(&h->d_val64, nnz * sizeof(T), "FP64 values"));
+        HD_TRY(dalloc(&h->d_val64, (nnz + 8) * sizeof(T), "FP64 values"));
+        HD_TRYHIP(hipMemsetAsync(static_cast<T*>(h->d_val64) + nnz, 0, 8 * sizeof(T), st));
         h->bytes += nnz * sizeof(T);
     }
     if (value_mask & 2) {
-        HD_TRY(dalloc(&h->d_val32, nnz * sizeof(F), "FP32 values"));
+        HD_TRY(dalloc(&h->d_val32, (nnz + 8) * sizeof(F), "FP32 values"));
+        HD_TRYHIP(hipMemsetAsync(static_cast<F*>(h->d_val32) + nnz, 0, 8 * sizeof(F), st));
         h->bytes += nnz * sizeof(F);
+    }
+    if (coded) {
+        // padded by 8 entries so the SpMV's aligned 4-wide loads stay in bounds
+        HD_TRY(dalloc_t(&h->d_code, nnz + 8, "value codes"));
+        HD_TRY(dalloc_t(reinterpret_cast<T**>(&h->d_dict), (size_t)std::max(D.ndict, 1), "value dictionary"));
+        HD_TRYHIP(hipMemcpyAsync(h->d_dict, D.dict.data(), D.dict.size() * 8, hipMemcpyHostToDevice, st));
+        HD_TRYHIP(hipMemsetAsync(h->d_code + nnz, 0xFF, 16, st));
+        h->ndict = D.ndict;
+        h->bytes += (nnz + 8) * 2 + (size_t)D.ndict * sizeof(T);
     }
     {
         const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(dim, (int64_t)n_cu * 32));
@@ -718,13 +793,13 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
         T* v64 = reinterpret_cast<T*>(h->d_val64);
         F* v32 = reinterpret_cast<F*>(h->d_val32);
         if (h->max_row <= 512)
-            hipLaunchKernelGGL((k_sort<CPLX, 512>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, h->d_col, v64, v32);
+            hipLaunchKernelGGL((k_sort<CPLX, 512>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, d_codetmp, h->d_col, v64, v32, h->d_code);
         else if (h->max_row <= 1024)
-            hipLaunchKernelGGL((k_sort<CPLX, 1024>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, h->d_col, v64, v32);
+            hipLaunchKernelGGL((k_sort<CPLX, 1024>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, d_codetmp, h->d_col, v64, v32, h->d_code);
         else if (h->max_row <= 2048)
-            hipLaunchKernelGGL((k_sort<CPLX, 2048>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, h->d_col, v64, v32);
+            hipLaunchKernelGGL((k_sort<CPLX, 2048>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, d_codetmp, h->d_col, v64, v32, h->d_code);
         else
-            hipLaunchKernelGGL((k_sort<CPLX, 4096>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, h->d_col, v64, v32);
+            hipLaunchKernelGGL((k_sort<CPLX, 4096>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, d_codetmp, h->d_col, v64, v32, h->d_code);
         HD_TRYHIP(hipGetLastError());
         h->t_sort = ev.stop(st);
     }
@@ -776,13 +851,17 @@ int hd_ham_build(const hd_basis* b, const hd_tables* t, int32_t storage, double 
     *out = nullptr;
     if (storage != HD_STORAGE_FULL && storage != HD_STORAGE_UPPER_REF)
         return fail(HD_ERR_ARG, "unknown storage");
-    if (value_mask < 1 || value_mask > 3) return fail(HD_ERR_ARG, "value_mask must be 1, 2 or 3");
+    if (value_mask < 1 || value_mask > 7) return fail(HD_ERR_ARG, "value_mask must be in 1..7");
+    if ((value_mask & 4) && storage != HD_STORAGE_FULL)
+        return fail(HD_ERR_ARG, "the coded value format needs full storage");
     if (!(zero_tol >= 0.0)) return fail(HD_ERR_ARG, "zero_tol must be >= 0");
     if (t->K != b->Norb)
         return fail(HD_ERR_UNSUPPORTED, "kMax (" + std::to_string(t->K) + ") must equal the number of orbitals (" +
                                             std::to_string(b->Norb) + ")");
     Derived D;
     HD_CHECK(derive_tables(t, b, zero_tol, D));
+    if ((value_mask & 4) && !D.codable)
+        return fail(HD_ERR_UNSUPPORTED, "more than 32766 distinct values: the coded format does not apply");
     hd_hamiltonian* h = new hd_hamiltonian();
     h->basis = b;
     h->dim = b->dim;
@@ -813,6 +892,9 @@ int hd_ham_destroy(hd_hamiltonian* h)
     dfree(h->d_col);
     dfree(h->d_val64);
     dfree(h->d_val32);
+    dfree(h->d_code);
+    dfree(h->d_diag);
+    dfree(h->d_dict);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
     return HD_OK;
@@ -834,6 +916,28 @@ int hd_ham_info(const hd_hamiltonian* h, hd_ham_info_t* info)
     info->t_fill_ms = h->t_fill;
     info->t_sort_ms = h->t_sort;
     info->device_bytes = h->bytes;
+    return HD_OK;
+}
+
+int hd_ham_device_crs(const hd_hamiltonian* h, const int64_t** d_rowptr, const int32_t** d_col,
+                      const void** d_val64, const void** d_val32)
+{
+    if (!h) return fail(HD_ERR_ARG, "hd_ham_device_crs: null handle");
+    if (d_rowptr) *d_rowptr = h->d_rowptr;
+    if (d_col) *d_col = h->d_col;
+    if (d_val64) *d_val64 = h->d_val64;
+    if (d_val32) *d_val32 = h->d_val32;
+    return HD_OK;
+}
+
+int hd_ham_device_coded(const hd_hamiltonian* h, const uint16_t** d_code, const void** d_diag,
+                        const void** d_dict, int32_t* ndict)
+{
+    if (!h) return fail(HD_ERR_ARG, "hd_ham_device_coded: null handle");
+    if (d_code) *d_code = h->d_code;
+    if (d_diag) *d_diag = h->d_diag;
+    if (d_dict) *d_dict = h->d_dict;
+    if (ndict) *ndict = h->ndict;
     return HD_OK;
 }
 

@@ -201,6 +201,11 @@ struct hd_hamiltonian {
     int32_t* d_col = nullptr;
     void* d_val64 = nullptr;  // double or double2
     void* d_val32 = nullptr;  // float or float2
+    // coded values: entry = sign * dict[code & 0x7FFF], code 0xFFFF = diag[row]
+    uint16_t* d_code = nullptr;
+    void* d_diag = nullptr;   // double or double2, dim entries
+    void* d_dict = nullptr;   // double or double2, ndict entries
+    int ndict = 0;
     int lanes_per_row = 0;    // SpMV schedule (0 = pick on first use)
     double t_count = 0, t_scan = 0, t_fill = 0, t_sort = 0;
     size_t bytes = 0;

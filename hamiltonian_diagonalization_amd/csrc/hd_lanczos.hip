@@ -112,17 +112,25 @@ k_dots(int64_t dim, const typename E<C>::T* __restrict__ V, int64_t ld, int j0, 
     }
 }
 
-// out[q] = sum_b partial[q * nb + b], b ascending (one thread per output).
+// out[q] = sum_b partial[q * nb + b]: one block per output, strided partial
+// sums then a fixed tree, so the result does not depend on scheduling.
 template <bool C>
-__global__ void k_reduce(const typename E<C>::T* __restrict__ partial, int nb, int nout,
-                         typename E<C>::T* __restrict__ out)
+__global__ void __launch_bounds__(kVB)
+k_reduce(const typename E<C>::T* __restrict__ partial, int nb, int nout, typename E<C>::T* __restrict__ out)
 {
     using T = typename E<C>::T;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ T red[kVB];
+    const int q = blockIdx.x;
     if (q >= nout) return;
     T s = zero<T>();
-    for (int b = 0; b < nb; ++b) s = add2(s, partial[(int64_t)q * nb + b]);
-    out[q] = s;
+    for (int b = threadIdx.x; b < nb; b += kVB) s = add2(s, partial[(int64_t)q * nb + b]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kVB / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = add2(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = red[0];
 }
 
 // w -= sum_{j<nv} V_j coef[j]; partial norm^2 of the result.
@@ -154,12 +162,18 @@ k_project_out(int64_t dim, const typename E<C>::T* __restrict__ V, int64_t ld, i
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-__global__ void k_reduce_real(const double* __restrict__ partial, int nb, double* __restrict__ out)
+__global__ void __launch_bounds__(kVB) k_reduce_real(const double* __restrict__ partial, int nb, double* __restrict__ out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double red[kVB];
     double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += partial[b];
-    *out = s;
+    for (int b = threadIdx.x; b < nb; b += kVB) s += partial[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kVB / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
 }
 
 template <bool C>
@@ -194,6 +208,30 @@ k_combine(int64_t dim, const typename E<C>::T* __restrict__ V, int64_t ld, int m
         }
         for (int l = 0; l < nl; ++l) out[(int64_t)l * ld + r] = acc[l];
     }
+}
+
+// w -= a * p; partial[block] = sum conj(q) w (the updated w).
+template <bool C>
+__global__ void __launch_bounds__(kVB)
+k_axpy_dot(int64_t dim, const typename E<C>::T* __restrict__ p, double a, const typename E<C>::T* __restrict__ q,
+           typename E<C>::T* __restrict__ w, typename E<C>::T* __restrict__ partial)
+{
+    using T = typename E<C>::T;
+    __shared__ T red[kVB];
+    T s = zero<T>();
+    const int64_t stride = (int64_t)gridDim.x * kVB;
+    for (int64_t r = (int64_t)blockIdx.x * kVB + threadIdx.x; r < dim; r += stride) {
+        const T v = sub(w[r], scale(p[r], a));
+        w[r] = v;
+        s = add2(s, conj_mul(q[r], v));
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kVB / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = add2(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
 template <bool C>
@@ -355,7 +393,8 @@ struct Lanczos {
     double* dscal = nullptr;
     double* dY = nullptr;
     double t_spmv = 0, t_ortho = 0;
-    int mv32 = 0, mv64 = 0;
+    int mv32 = 0, mv64 = 0, second_passes = 0;
+    int prec64 = HD_F64;  // the FP64 operator: FP64 values or the coded format
 
     ~Lanczos()
     {
@@ -398,7 +437,7 @@ struct Lanczos {
             hipLaunchKernelGGL(k_to_f64<C>, dim3(nb), dim3(kVB), 0, st, n, y32, y);
             ++mv32;
         } else {
-            s = spmv_launch(h, HD_F64, x, y, st);
+            s = spmv_launch(h, prec, x, y, st);
             ++mv64;
         }
         hipEventRecord(e1, st);
@@ -413,17 +452,26 @@ struct Lanczos {
 
     // coefficients c[0..nv) = V[0..nv)^H w (device, also copied to h_out),
     // optionally |w|^2 into *nrm
-    int dots(int nv, const T* w, std::vector<T>& h_out)
+    int dots(int nv, const T* w, std::vector<T>& h_out, double* wnorm2 = nullptr)
     {
-        h_out.resize(nv);
+        h_out.resize(nv + 1);
         for (int j0 = 0; j0 < nv; j0 += kNV) {
             const int cnt = std::min(kNV, nv - j0);
-            hipLaunchKernelGGL(k_dots<C>, dim3(nb), dim3(kVB), 0, st, n, V, n, j0, cnt, w, partial, 0);
-            hipLaunchKernelGGL(k_reduce<C>, dim3(1), dim3(64), 0, st, partial, nb, cnt, coef + j0);
+            const int want = (wnorm2 && j0 == 0) ? 1 : 0;
+            hipLaunchKernelGGL(k_dots<C>, dim3(nb), dim3(kVB), 0, st, n, V, n, j0, cnt, w, partial, want);
+            hipLaunchKernelGGL(k_reduce<C>, dim3(cnt + want), dim3(kVB), 0, st, partial, nb, cnt + want,
+                               coef + (want ? 72 - cnt : j0));
+            if (want) {
+                // the norm lands in coef[72]; move the dots to their slots
+                HD_HIP(hipMemcpyAsync(coef + j0, coef + 72 - cnt, sizeof(T) * cnt, hipMemcpyDeviceToDevice, st));
+            }
         }
         HD_HIP(hipGetLastError());
         HD_HIP(hipMemcpyAsync(h_out.data(), coef, sizeof(T) * nv, hipMemcpyDeviceToHost, st));
+        if (wnorm2) HD_HIP(hipMemcpyAsync(&h_out[nv], coef + 72, sizeof(T), hipMemcpyDeviceToHost, st));
         HD_HIP(hipStreamSynchronize(st));
+        if (wnorm2) *wnorm2 = *reinterpret_cast<const double*>(&h_out[nv]);
+        h_out.resize(nv);
         return HD_OK;
     }
 
@@ -431,7 +479,7 @@ struct Lanczos {
     int project(int nv, T* w, double* nrm2)
     {
         hipLaunchKernelGGL(k_project_out<C>, dim3(nb), dim3(kVB), 0, st, n, V, n, nv, coef, w, dpart);
-        hipLaunchKernelGGL(k_reduce_real, dim3(1), dim3(64), 0, st, dpart, nb, dscal);
+        hipLaunchKernelGGL(k_reduce_real, dim3(1), dim3(kVB), 0, st, dpart, nb, dscal);
         HD_HIP(hipGetLastError());
         HD_HIP(hipMemcpyAsync(nrm2, dscal, sizeof(double), hipMemcpyDeviceToHost, st));
         HD_HIP(hipStreamSynchronize(st));
@@ -441,7 +489,7 @@ struct Lanczos {
     int norm2_of(const T* w, double* out)
     {
         hipLaunchKernelGGL(k_dots<C>, dim3(nb), dim3(kVB), 0, st, n, V, n, 0, 0, w, partial, 1);
-        hipLaunchKernelGGL(k_reduce<C>, dim3(1), dim3(64), 0, st, partial, nb, 1, coef);
+        hipLaunchKernelGGL(k_reduce<C>, dim3(1), dim3(kVB), 0, st, partial, nb, 1, coef);
         T v;
         HD_HIP(hipMemcpyAsync(&v, coef, sizeof(T), hipMemcpyDeviceToHost, st));
         HD_HIP(hipStreamSynchronize(st));
@@ -449,20 +497,47 @@ struct Lanczos {
         return HD_OK;
     }
 
-    // Full reorthogonalisation of w against V[0..nv): two CGS passes.
-    // Returns the summed coefficients and the final |w|.
-    int orthogonalize(int nv, T* w, std::vector<T>& c, double* beta)
+    // Full reorthogonalisation of w against V[0..nv) by classical Gram-Schmidt:
+    // one pass, and a second one only when the pass cancelled more than a
+    // factor `kappa` of the norm (the "twice is enough" test of Daniel, Gragg,
+    // Kaufman and Stewart, Math. Comp. 30 (1976) 772; one CGS pass leaves an
+    // orthogonality error of about eps * |w_before| / |w_after|), or always
+    // when `two_pass`.  Returns the summed coefficients and the final |w|.
+    int orthogonalize(int nv, T* w, std::vector<T>& c, double* beta, bool two_pass = false)
     {
         const auto t0 = std::chrono::steady_clock::now();
         std::vector<T> c2;
-        double nrm2 = 0.0;
-        HD_CHECK(dots(nv, w, c));
+        double before2 = 0.0, nrm2 = 0.0;
+        HD_CHECK(dots(nv, w, c, &before2));
         HD_CHECK(project(nv, w, &nrm2));
-        HD_CHECK(dots(nv, w, c2));
-        HD_CHECK(project(nv, w, &nrm2));
-        for (int j = 0; j < nv; ++j) c[j] = host_add(c[j], c2[j]);
+        const double kappa = 100.0;
+        if (two_pass || nrm2 * kappa * kappa < before2) {
+            HD_CHECK(dots(nv, w, c2));
+            HD_CHECK(project(nv, w, &nrm2));
+            for (int j = 0; j < nv; ++j) c[j] = host_add(c[j], c2[j]);
+            ++second_passes;
+        }
         *beta = std::sqrt(std::max(nrm2, 0.0));
         t_ortho += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return HD_OK;
+    }
+
+    // The local three-term part of a Lanczos step, done before the full
+    // reorthogonalisation so that its large coefficients never multiply the
+    // basis' own orthogonality error (which a single classical Gram-Schmidt
+    // pass would amplify by alpha/beta per step):
+    //   w -= beta_prev * v_prev;  a = <v_cur, w>;  w -= a * v_cur.
+    int local_step(const T* v_prev, double beta_prev, const T* v_cur, T* w, double* a)
+    {
+        hipLaunchKernelGGL(k_axpy_dot<C>, dim3(nb), dim3(kVB), 0, st, n, v_prev, beta_prev, v_cur, w, partial);
+        hipLaunchKernelGGL(k_reduce<C>, dim3(1), dim3(kVB), 0, st, partial, nb, 1, coef);
+        T v;
+        HD_HIP(hipGetLastError());
+        HD_HIP(hipMemcpyAsync(&v, coef, sizeof(T), hipMemcpyDeviceToHost, st));
+        HD_HIP(hipStreamSynchronize(st));
+        *a = host_re(v);
+        hipLaunchKernelGGL(k_axpy_dot<C>, dim3(nb), dim3(kVB), 0, st, n, v_cur, *a, v_cur, w, partial);
+        HD_HIP(hipGetLastError());
         return HD_OK;
     }
 
@@ -502,7 +577,7 @@ struct Lanczos {
     int residual_norm(const T* y, const T* x, double theta, double* out)
     {
         hipLaunchKernelGGL(k_resid<C>, dim3(nb), dim3(kVB), 0, st, n, y, x, theta, dpart);
-        hipLaunchKernelGGL(k_reduce_real, dim3(1), dim3(64), 0, st, dpart, nb, dscal);
+        hipLaunchKernelGGL(k_reduce_real, dim3(1), dim3(kVB), 0, st, dpart, nb, dscal);
         HD_HIP(hipGetLastError());
         HD_HIP(hipMemcpyAsync(out, dscal, sizeof(double), hipMemcpyDeviceToHost, st));
         HD_HIP(hipStreamSynchronize(st));
@@ -531,12 +606,13 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
     L.nb = (int)std::max<int64_t>(1, std::min<int64_t>((h->dim + kVB - 1) / kVB, (int64_t)n_cu * 4));
     const bool mixed = o.precision == HD_F32;
     if (mixed && !h->d_val32) return fail(HD_ERR_ARG, "FP32 Lanczos needs FP32 values (value_mask & 2)");
-    if (!h->d_val64) return fail(HD_ERR_ARG, "Lanczos needs the FP64 values for the residual check");
+    if (!h->d_val64 && !h->d_code) return fail(HD_ERR_ARG, "Lanczos needs FP64 or coded values for the FP64 operator");
+    L.prec64 = (o.precision == HD_CODED || !h->d_val64) ? HD_CODED : HD_F64;
     HD_CHECK(L.alloc(mixed));
     const double tol = o.tol > 0 ? o.tol : 1e-10;
     const double sw_tol = o.f32_switch_tol > 0 ? o.f32_switch_tol : 1e-6;
     const int max_restarts = o.max_restarts > 0 ? o.max_restarts : 2000;
-    int prec = mixed ? HD_F32 : HD_F64;
+    int prec = mixed ? HD_F32 : L.prec64;
     int kk_keep = nev + (m - nev) / 2;
     kk_keep = std::min(std::max(kk_keep, nev + 1), m - 2 >= nev ? m - 2 : nev);
     kk_keep = std::max(kk_keep, nev);
@@ -567,8 +643,17 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
         for (int j = k; j < m; ++j) {
             HD_CHECK(L.apply(L.vec(j), L.W, prec));
             double b;
-            HD_CHECK(L.orthogonalize(j + 1, L.W, c, &b));
-            alpha[j] = Lanczos<C>::host_re(c[j]);
+            if (j == k) {
+                // first vector of a cycle: its coupling to the kept Ritz
+                // vectors is the arrow, removed by two full CGS passes
+                HD_CHECK(L.orthogonalize(j + 1, L.W, c, &b, true));
+                alpha[j] = Lanczos<C>::host_re(c[j]);
+            } else {
+                double a;
+                HD_CHECK(L.local_step(L.vec(j - 1), beta[j - 1], L.vec(j), L.W, &a));
+                HD_CHECK(L.orthogonalize(j + 1, L.W, c, &b, false));
+                alpha[j] = a + Lanczos<C>::host_re(c[j]);
+            }
             anorm = std::max(anorm, std::fabs(alpha[j]) + b);
             if (b <= 1e-12 * std::max(anorm, 1e-300)) {
                 // invariant subspace: continue with a fresh random direction
@@ -613,7 +698,7 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
             HD_HIP(hipGetLastError());
             HD_CHECK(L.norm2_of(L.vec(0), &nrm));
             HD_CHECK(L.scale_into(L.vec(0), 1.0 / std::sqrt(nrm), L.vec(0)));
-            prec = HD_F64;
+            prec = L.prec64;
             k = 0;
             theta.clear();
             arrow.clear();
@@ -653,7 +738,7 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
     }
     for (int i = 0; i < nev; ++i) {
         T* xi = static_cast<T*>(S->d_vec) + (size_t)i * L.n;
-        HD_CHECK(L.apply(xi, L.W, HD_F64));
+        HD_CHECK(L.apply(xi, L.W, L.prec64));
         double r;
         HD_CHECK(L.residual_norm(L.W, xi, w[i], &r));
         r /= std::max(1.0, std::fabs(w[i]));

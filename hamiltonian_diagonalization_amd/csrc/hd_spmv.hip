@@ -6,10 +6,16 @@
 // :218-312).  The scattered y[col] += ... of the transposed half cannot run
 // race-free in parallel, so the device keeps both triangles and every row is a
 // pure gather: LPR lanes of a wavefront stream the row's column indices and
-// values (non-temporal, so the resident x vector keeps the caches), gather x,
-// and reduce with cross-lane shuffles.  Consecutive rows have near-identical
-// column sets, so each XCD is given a contiguous range of row blocks and its
-// private L2 sees the x entries those rows share.
+// values with 16-byte non-temporal loads (4 entries per lane, row start
+// aligned down to a multiple of 4 and masked, arrays padded by 8 entries),
+// gather x, and reduce with cross-lane shuffles.
+//
+// Coded format (HD_CODED): every off-diagonal value is sign * P[hop] from the
+// build's pair table, and the distinct P values are few (1,496 for the
+// N=12, 2S=33 sphere), so an entry is stored as int32 column + uint16 code
+// (bit 15 = sign, low 15 bits = dictionary index, 0xFFFF = the row's diagonal
+// from a separate FP64 array).  The FP64 dictionary lives in LDS.  That is the
+// exact FP64 operator in 6 bytes per entry instead of 12.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,12 +23,6 @@
 #include "hd_internal.hpp"
 
 namespace hd {
-
-template <typename V> struct Acc;
-template <> struct Acc<double> { using T = double; };
-template <> struct Acc<float> { using T = float; };
-template <> struct Acc<double2> { using T = double2; };
-template <> struct Acc<float2> { using T = float2; };
 
 HD_DEV double fmadd(double v, double x, double a) { return fma(v, x, a); }
 HD_DEV float fmadd(float v, float x, float a) { return fmaf(v, x, a); }
@@ -56,7 +56,45 @@ HD_DEV double add(double a, double b) { return a + b; }
 HD_DEV float add(float a, float b) { return a + b; }
 HD_DEV double2 add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 HD_DEV float2 add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+HD_DEV double negv(double a) { return -a; }
+HD_DEV double2 negv(double2 a) { return make_double2(-a.x, -a.y); }
 
+// 4 consecutive values starting at a 4-aligned index, non-temporal (clang
+// vector types: the non-temporal builtin does not take HIP's struct vectors).
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+typedef int nt_i4 __attribute__((ext_vector_type(4)));
+typedef unsigned int nt_u2 __attribute__((ext_vector_type(2)));
+template <typename V> struct Quad { V v[4]; };
+template <typename V> HD_DEV Quad<V> load4(const V* p);
+template <> HD_DEV Quad<float> load4<float>(const float* p)
+{
+    const nt_f4 q = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+    return {{q.x, q.y, q.z, q.w}};
+}
+template <> HD_DEV Quad<double> load4<double>(const double* p)
+{
+    const nt_d2 a = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
+    const nt_d2 b = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p) + 1);
+    return {{a.x, a.y, b.x, b.y}};
+}
+template <> HD_DEV Quad<float2> load4<float2>(const float2* p)
+{
+    const nt_f4 a = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+    const nt_f4 b = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p) + 1);
+    return {{make_float2(a.x, a.y), make_float2(a.z, a.w), make_float2(b.x, b.y), make_float2(b.z, b.w)}};
+}
+template <> HD_DEV Quad<double2> load4<double2>(const double2* p)
+{
+    Quad<double2> q;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const nt_d2 a = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p) + k);
+        q.v[k] = make_double2(a.x, a.y);
+    }
+    return q;
+}
+// scalar non-temporal loads (complex values as two component loads)
 template <typename T> HD_DEV T ntload(const T* p) { return __builtin_nontemporal_load(p); }
 template <> HD_DEV double2 ntload<double2>(const double2* p)
 {
@@ -69,6 +107,17 @@ template <> HD_DEV float2 ntload<float2>(const float2* p)
     return make_float2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
 }
 
+HD_DEV int4 load_col4(const int32_t* p)
+{
+    const nt_i4 c = __builtin_nontemporal_load(reinterpret_cast<const nt_i4*>(p));
+    return make_int4(c.x, c.y, c.z, c.w);
+}
+HD_DEV uint2 load_code4(const uint16_t* p)
+{
+    const nt_u2 c = __builtin_nontemporal_load(reinterpret_cast<const nt_u2*>(p));
+    return make_uint2(c.x, c.y);
+}
+
 constexpr int kSpmvBlock = 256;
 
 // Logical block index with the blocks of each XCD (dispatched round-robin,
@@ -79,64 +128,139 @@ HD_DEV int64_t xcd_block(int64_t b, int64_t nb)
     return x * q + (x < r ? x : r) + i;
 }
 
+// Persistent grid: the blocks on XCD x (block b runs on XCD b % 8) walk the
+// chunks [x*nc/8, (x+1)*nc/8) round-robin, so each XCD's L2 serves one
+// contiguous range of rows.
+struct ChunkWalk {
+    int64_t c, c1, step;
+    HD_DEV ChunkWalk(int64_t nchunks)
+    {
+        const int x = blockIdx.x & 7;
+        const int64_t nbx = ((int64_t)gridDim.x - x + 7) / 8;
+        c = (int64_t)x * nchunks / 8 + (blockIdx.x >> 3);
+        c1 = (int64_t)(x + 1) * nchunks / 8;
+        step = nbx;
+    }
+};
+
 template <typename V, int LPR>
 __global__ void __launch_bounds__(kSpmvBlock)
 k_spmv(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
        const V* __restrict__ val, const V* __restrict__ x, V* __restrict__ y)
 {
-    using A = typename Acc<V>::T;
+    // one block per chunk of RPB rows: blocks are dispatched in order, so the
+    // rows in flight at any time form a narrow window whose x entries overlap
     constexpr int RPB = kSpmvBlock / LPR;
-    const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
-    const int64_t row = lb * RPB + threadIdx.x / LPR;
     const int sub = threadIdx.x & (LPR - 1);
-    A acc0 = zero_of<A>(), acc1 = zero_of<A>();
+    const int64_t row = (int64_t)blockIdx.x * RPB + threadIdx.x / LPR;
+    V acc = zero_of<V>();
     if (row < dim) {
         const int64_t start = rowptr[row], end = rowptr[row + 1];
-        int64_t j = start + sub;
-        for (; j + LPR < end; j += 2 * LPR) {
-            const int c0 = ntload(col + j), c1 = ntload(col + j + LPR);
-            const V v0 = ntload(val + j), v1 = ntload(val + j + LPR);
-            acc0 = fmadd(v0, x[c0], acc0);
-            acc1 = fmadd(v1, x[c1], acc1);
-        }
-        if (j < end) acc0 = fmadd(ntload(val + j), x[ntload(col + j)], acc0);
+        for (int64_t j = start + sub; j < end; j += LPR)
+            acc = fmadd(ntload(val + j), x[ntload(col + j)], acc);
     }
-    A acc = add(acc0, acc1);
 #pragma unroll
     for (int m = LPR / 2; m > 0; m >>= 1) acc = add(acc, shfl_xor(acc, m));
     if (row < dim && sub == 0) y[row] = acc;
 }
 
 template <typename V, int LPR>
-static void launch_one(int64_t dim, const hd_hamiltonian* h, const void* val, const void* x, void* y,
-                       hipStream_t s)
+__global__ void __launch_bounds__(kSpmvBlock)
+k_spmv_coded(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+             const uint16_t* __restrict__ code, const V* __restrict__ diag, const V* __restrict__ dict,
+             int ndict, const V* __restrict__ x, V* __restrict__ y)
+{
+    extern __shared__ int32_t lds_raw[];
+    V* sdict = reinterpret_cast<V*>(lds_raw);
+    for (int i = threadIdx.x; i < ndict; i += kSpmvBlock) sdict[i] = dict[i];
+    __syncthreads();
+    constexpr int RPB = kSpmvBlock / LPR;
+    const int sub = threadIdx.x & (LPR - 1);
+    for (ChunkWalk w((dim + RPB - 1) / RPB); w.c < w.c1; w.c += w.step) {
+    const int64_t row = w.c * RPB + threadIdx.x / LPR;
+    V acc = zero_of<V>();
+    if (row < dim) {
+        const int64_t start = rowptr[row], end = rowptr[row + 1];
+        for (int64_t j = start + sub; j < end; j += LPR) {
+            const uint32_t q = ntload(code + j);
+            V v;
+            if (q == 0xFFFFu) v = diag[row];
+            else {
+                v = sdict[q & 0x7FFFu];
+                if (q & 0x8000u) v = negv(v);
+            }
+            acc = fmadd(v, x[ntload(col + j)], acc);
+        }
+    }
+#pragma unroll
+    for (int m = LPR / 2; m > 0; m >>= 1) acc = add(acc, shfl_xor(acc, m));
+    if (row < dim && sub == 0) y[row] = acc;
+    }
+}
+
+// Persistent grid of 8 blocks per CU, capped by the number of row chunks but
+// never below 8 so that every XCD's chunk range has a block walking it.
+static unsigned persistent_grid(int64_t nchunks)
+{
+    static int n_cu = 0;
+    if (!n_cu && device_props(&n_cu, nullptr) != HD_OK) n_cu = 256;
+    return (unsigned)std::max<int64_t>(8, std::min<int64_t>(nchunks, (int64_t)n_cu * 8));
+}
+
+template <typename V, int LPR>
+static void launch_one(const hd_hamiltonian* h, const void* val, const void* x, void* y, hipStream_t s)
 {
     constexpr int RPB = kSpmvBlock / LPR;
-    const int64_t nb = (dim + RPB - 1) / RPB;
-    hipLaunchKernelGGL((k_spmv<V, LPR>), dim3((unsigned)nb), dim3(kSpmvBlock), 0, s, dim, h->d_rowptr,
+    const unsigned grid = (unsigned)((h->dim + RPB - 1) / RPB);
+    hipLaunchKernelGGL((k_spmv<V, LPR>), dim3(grid), dim3(kSpmvBlock), 0, s, h->dim, h->d_rowptr,
                        h->d_col, static_cast<const V*>(val), static_cast<const V*>(x), static_cast<V*>(y));
 }
 
+template <typename V, int LPR>
+static void launch_coded(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
+{
+    constexpr int RPB = kSpmvBlock / LPR;
+    const unsigned grid = persistent_grid((h->dim + RPB - 1) / RPB);
+    const size_t lds = sizeof(V) * (size_t)std::max(h->ndict, 1);
+    hipLaunchKernelGGL((k_spmv_coded<V, LPR>), dim3(grid), dim3(kSpmvBlock), lds, s, h->dim,
+                       h->d_rowptr, h->d_col, h->d_code, static_cast<const V*>(h->d_diag),
+                       static_cast<const V*>(h->d_dict), h->ndict, static_cast<const V*>(x),
+                       static_cast<V*>(y));
+}
+
 template <typename V>
-static void launch_lpr(int lpr, int64_t dim, const hd_hamiltonian* h, const void* val, const void* x,
-                       void* y, hipStream_t s)
+static void launch_lpr(int lpr, const hd_hamiltonian* h, const void* val, const void* x, void* y,
+                       hipStream_t s)
 {
     switch (lpr) {
-    case 4: launch_one<V, 4>(dim, h, val, x, y, s); break;
-    case 8: launch_one<V, 8>(dim, h, val, x, y, s); break;
-    case 16: launch_one<V, 16>(dim, h, val, x, y, s); break;
-    case 32: launch_one<V, 32>(dim, h, val, x, y, s); break;
-    default: launch_one<V, 64>(dim, h, val, x, y, s); break;
+    case 4: launch_one<V, 4>(h, val, x, y, s); break;
+    case 8: launch_one<V, 8>(h, val, x, y, s); break;
+    case 16: launch_one<V, 16>(h, val, x, y, s); break;
+    case 32: launch_one<V, 32>(h, val, x, y, s); break;
+    default: launch_one<V, 64>(h, val, x, y, s); break;
+    }
+}
+
+template <typename V>
+static void launch_coded_lpr(int lpr, const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
+{
+    switch (lpr) {
+    case 4: launch_coded<V, 4>(h, x, y, s); break;
+    case 8: launch_coded<V, 8>(h, x, y, s); break;
+    case 16: launch_coded<V, 16>(h, x, y, s); break;
+    case 32: launch_coded<V, 32>(h, x, y, s); break;
+    default: launch_coded<V, 64>(h, x, y, s); break;
     }
 }
 
 static int pick_default_lanes(const hd_hamiltonian* h)
 {
+    // each lane takes 4 entries per step
     const double avg = h->dim ? (double)h->nnz / (double)h->dim : 0.0;
-    if (avg > 96) return 64;
-    if (avg > 48) return 32;
-    if (avg > 24) return 16;
-    if (avg > 12) return 8;
+    if (avg > 192) return 64;
+    if (avg > 96) return 32;
+    if (avg > 48) return 16;
+    if (avg > 24) return 8;
     return 4;
 }
 
@@ -144,26 +268,40 @@ int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y, 
 {
     if (h->storage != HD_STORAGE_FULL)
         return fail(HD_ERR_ARG, "SpMV needs full (both-triangle) storage");
-    const void* val = precision == HD_F32 ? h->d_val32 : h->d_val64;
-    if (!val) return fail(HD_ERR_ARG, "requested SpMV precision has no resident values");
     const int lpr = h->lanes_per_row ? h->lanes_per_row : pick_default_lanes(h);
     if (h->dim == 0) return HD_OK;
+    if (precision == HD_CODED) {
+        if (!h->d_code) return fail(HD_ERR_ARG, "no coded values resident (value_mask & 4)");
+        if (h->is_complex) launch_coded_lpr<double2>(lpr, h, x, y, stream);
+        else launch_coded_lpr<double>(lpr, h, x, y, stream);
+        HD_HIP(hipGetLastError());
+        return HD_OK;
+    }
+    const void* val = precision == HD_F32 ? h->d_val32 : h->d_val64;
+    if (!val) return fail(HD_ERR_ARG, "requested SpMV precision has no resident values");
     if (precision == HD_F32) {
-        if (h->is_complex) launch_lpr<float2>(lpr, h->dim, h, val, x, y, stream);
-        else launch_lpr<float>(lpr, h->dim, h, val, x, y, stream);
+        if (h->is_complex) launch_lpr<float2>(lpr, h, val, x, y, stream);
+        else launch_lpr<float>(lpr, h, val, x, y, stream);
     } else {
-        if (h->is_complex) launch_lpr<double2>(lpr, h->dim, h, val, x, y, stream);
-        else launch_lpr<double>(lpr, h->dim, h, val, x, y, stream);
+        if (h->is_complex) launch_lpr<double2>(lpr, h, val, x, y, stream);
+        else launch_lpr<double>(lpr, h, val, x, y, stream);
     }
     HD_HIP(hipGetLastError());
     return HD_OK;
 }
 
-// Times every lanes-per-row variant on the matrix and keeps the fastest
-// (FP32 values when resident, else FP64).
+static int default_precision(const hd_hamiltonian* h)
+{
+    if (h->d_code) return HD_CODED;
+    if (h->d_val32) return HD_F32;
+    return HD_F64;
+}
+
+// Times every lanes-per-row variant on the matrix (in its fastest resident
+// format) and keeps the fastest.
 int spmv_autotune(hd_hamiltonian* h)
 {
-    const int prec = h->d_val32 ? HD_F32 : HD_F64;
+    const int prec = default_precision(h);
     const size_t es = (prec == HD_F32 ? 4 : 8) * (h->is_complex ? 2 : 1);
     void *x = nullptr, *y = nullptr;
     HD_CHECK(dalloc(&x, (size_t)h->dim * es, "autotune x"));
@@ -180,8 +318,7 @@ int spmv_autotune(hd_hamiltonian* h)
     float best = 1e30f;
     int best_l = 64;
     for (int l : cand) {
-        h->lanes_per_row ? (void)0 : (void)0;
-        const_cast<hd_hamiltonian*>(h)->lanes_per_row = l;
+        h->lanes_per_row = l;
         spmv_launch(h, prec, x, y, h->stream);  // warm
         hipEventRecord(e0, h->stream);
         for (int r = 0; r < 3; ++r) spmv_launch(h, prec, x, y, h->stream);
@@ -219,17 +356,50 @@ int hd_spmv_host(const hd_hamiltonian* h, const void* h_x, void* h_y)
 {
     if (!h || !h_x || !h_y) return fail(HD_ERR_ARG, "hd_spmv_host: null argument");
     const size_t bytes = (size_t)h->dim * 8 * (h->is_complex ? 2 : 1);
+    const int prec = h->d_val64 ? HD_F64 : (h->d_code ? HD_CODED : -1);
+    if (prec < 0) return fail(HD_ERR_ARG, "Simple() needs FP64 or coded values");
     void *x = nullptr, *y = nullptr;
     HD_CHECK(dalloc(&x, bytes, "Simple() x"));
     int st = dalloc(&y, bytes, "Simple() y");
     if (st == HD_OK) {
         hipMemcpyAsync(x, h_x, bytes, hipMemcpyHostToDevice, h->stream);
-        st = spmv_launch(h, HD_F64, x, y, h->stream);
+        st = spmv_launch(h, prec, x, y, h->stream);
         if (st == HD_OK) {
             hipMemcpyAsync(h_y, y, bytes, hipMemcpyDeviceToHost, h->stream);
             if (hipStreamSynchronize(h->stream) != hipSuccess) st = fail(HD_ERR_HIP, "Simple() copy failed");
         }
     }
+    dfree(x);
+    dfree(y);
+    return st;
+}
+
+int hd_spmv_time(hd_hamiltonian* h, int32_t precision, int32_t iters, double* avg_ms)
+{
+    if (!h || !avg_ms || iters <= 0) return fail(HD_ERR_ARG, "hd_spmv_time: bad argument");
+    if (!h->lanes_per_row) HD_CHECK(spmv_autotune(h));
+    const size_t es = (precision == HD_F32 ? 4 : 8) * (h->is_complex ? 2 : 1);
+    void *x = nullptr, *y = nullptr;
+    HD_CHECK(dalloc(&x, (size_t)h->dim * es, "timing x"));
+    int st = dalloc(&y, (size_t)h->dim * es, "timing y");
+    if (st != HD_OK) {
+        dfree(x);
+        return st;
+    }
+    hipMemsetAsync(x, 0, (size_t)h->dim * es, h->stream);
+    st = spmv_launch(h, precision, x, y, h->stream);  // warm
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, h->stream);
+    for (int i = 0; i < iters && st == HD_OK; ++i) st = spmv_launch(h, precision, x, y, h->stream);
+    hipEventRecord(e1, h->stream);
+    hipEventSynchronize(e1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    *avg_ms = ms / iters;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
     dfree(x);
     dfree(y);
     return st;

@@ -62,7 +62,14 @@ typedef enum {
                              (src/optical_flux_lattice/interacting_ofl_model/term_tables.cpp:95-175) */
 } hd_layout;
 
-typedef enum { HD_F64 = 0, HD_F32 = 1 } hd_precision;
+/* Value formats of the device CRS / SpMV:
+ *   HD_F64    FP64 (complex FP64) values, FP64 vectors
+ *   HD_F32    FP32 (complex FP32) values, FP32 vectors, FP32 accumulation
+ *   HD_CODED  16-bit codes into an FP64 dictionary of the distinct
+ *             off-diagonal values (sign in bit 15, 0xFFFF = diagonal, kept in a
+ *             separate FP64 array): the exact FP64 operator in 6 bytes per
+ *             entry; FP64 vectors and accumulation. */
+typedef enum { HD_F64 = 0, HD_F32 = 1, HD_CODED = 2 } hd_precision;
 
 typedef enum {
     HD_STORAGE_FULL = 0,     /* both triangles, int64 row pointers (GPU SpMV layout) */
@@ -155,7 +162,8 @@ typedef struct {
  * (mapped_sparse_matrix.hpp:423-462, crs_sparse_matrix.hpp:193-254), all on the
  * device: count, exclusive scan, fill, per-row sort.  Values are accumulated
  * in FP64 in the reference's term order; value_mask selects which value
- * copies stay resident (1 = FP64, 2 = FP32, 3 = both). */
+ * copies stay resident (bit 0 = FP64, bit 1 = FP32, bit 2 = coded; full
+ * storage only for the coded copy). */
 int hd_ham_build(const hd_basis* b, const hd_tables* t, int32_t storage, double zero_tol,
                  int32_t value_mask, hd_hamiltonian** out);
 int hd_ham_destroy(hd_hamiltonian* h);
@@ -165,6 +173,17 @@ int hd_ham_info(const hd_hamiltonian* h, hd_ham_info_t* info);
  * to the first row; precision selects the FP64 or FP32 value copy. */
 int hd_ham_copy_crs(const hd_hamiltonian* h, int64_t row0, int64_t row1, int32_t precision,
                     int64_t* h_rowptr, int32_t* h_col, void* h_val);
+
+/* Zero-copy view of the device CRS (CrsSparseMatrix::GetRowStarts / GetCols /
+ * GetData, crs_sparse_matrix.hpp:297-330): int64 row pointers (dim+1), int32
+ * columns, FP64 and FP32 values (NULL when not resident).  Owned by the handle. */
+int hd_ham_device_crs(const hd_hamiltonian* h, const int64_t** d_rowptr, const int32_t** d_col,
+                      const void** d_val64, const void** d_val32);
+/* Zero-copy view of the coded values (no reference counterpart; the coded
+ * format is this library's): per-entry uint16 codes, the FP64 diagonal (dim)
+ * and the FP64 dictionary (ndict entries).  NULL when not resident. */
+int hd_ham_device_coded(const hd_hamiltonian* h, const uint16_t** d_code, const void** d_diag,
+                        const void** d_dict, int32_t* ndict);
 
 /* ---------------------------------------------------------------- mat-vec */
 
@@ -176,6 +195,10 @@ int hd_spmv(const hd_hamiltonian* h, int32_t precision, const void* d_x, void* d
 /* Simple() drop-in over host buffers (FP64 / complex FP64): copies in, multiplies,
  * copies out. */
 int hd_spmv_host(const hd_hamiltonian* h, const void* h_x, void* h_y);
+/* Measurement helper (no reference counterpart): runs `iters` SpMVs on zero
+ * vectors allocated by the library and reports the mean kernel time from HIP
+ * events recorded on the handle's stream. */
+int hd_spmv_time(hd_hamiltonian* h, int32_t precision, int32_t iters, double* avg_ms);
 /* Rows per wavefront schedule of the SpMV (lanes per row 4..64, 0 = autotune). */
 int hd_spmv_set_lanes(hd_hamiltonian* h, int32_t lanes_per_row);
 int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes_per_row);

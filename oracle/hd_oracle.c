@@ -614,6 +614,37 @@ void oc_spmv_sym(int64_t dim, const int64_t* rowptr, const int32_t* col,
     oc_spmv_rows(0, dim, rowptr, col, val, is_complex, x, y);
 }
 
+void oc_spmv_sym_block(int64_t row0, int64_t nrows, const int64_t* rowptr, const int32_t* col,
+                       const double* val, int is_complex, const double* x, double* y)
+{
+    /* rowptr/col/val hold rows [row0, row0+nrows) with rowptr relative to the
+     * first of them; the same loop as oc_spmv_rows with the row index shifted */
+    if (!is_complex) {
+        for (int64_t r = 0; r < nrows; ++r) {
+            const int64_t i = row0 + r;
+            for (int64_t j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+                int c = col[j];
+                double v = val[j];
+                y[i] += v * x[c];
+                if (i != c) y[c] += v * x[i];
+            }
+        }
+    } else {
+        const double complex* X = (const double complex*)x;
+        double complex* Y = (double complex*)y;
+        const double complex* V = (const double complex*)val;
+        for (int64_t r = 0; r < nrows; ++r) {
+            const int64_t i = row0 + r;
+            for (int64_t j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+                int c = col[j];
+                double complex v = V[j];
+                Y[i] += v * X[c];
+                if (i != c) Y[c] += conj(v) * X[i];
+            }
+        }
+    }
+}
+
 typedef struct {
     int64_t r0, r1, dim;
     const int64_t* rp; const int32_t* col; const double* val;

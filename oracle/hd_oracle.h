@@ -79,6 +79,11 @@ void oc_spmv_sym(int64_t dim, const int64_t* rowptr, const int32_t* col,
 void oc_spmv_sym_mt(int64_t dim, const int64_t* rowptr, const int32_t* col,
                     const double* val, int is_complex, const double* x, double* y,
                     int nthreads);
+/* Rows [row0, row0+nrows) of the naive symmetric product (rowptr relative to
+ * row0), accumulated into the full-length y: the per-row work of oc_spmv_sym
+ * on a row block, used to time a bounded sample of a large matrix. */
+void oc_spmv_sym_block(int64_t row0, int64_t nrows, const int64_t* rowptr, const int32_t* col,
+                       const double* val, int is_complex, const double* x, double* y);
 /* Row-parallel build over nthreads threads; the same outputs as oc_build
  * (row0 = 0, row1 = dim), used for the CPU baseline timing. */
 int64_t oc_build_mt(int N, int Norb, const uint64_t* basis, int64_t dim,

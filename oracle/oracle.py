@@ -88,6 +88,8 @@ def lib():
                                   P, P, P, i64, i32]
         L.oc_spmv_sym.restype = None
         L.oc_spmv_sym.argtypes = [i64, P, P, P, i32, P, P]
+        L.oc_spmv_sym_block.restype = None
+        L.oc_spmv_sym_block.argtypes = [i64, i64, P, P, P, i32, P, P]
         L.oc_spmv_sym_mt.restype = None
         L.oc_spmv_sym_mt.argtypes = [i64, P, P, P, i32, P, P, i32]
         _lib = L

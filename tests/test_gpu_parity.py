@@ -159,7 +159,14 @@ def test_spmv_config1(oracle):
         assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
         g32 = h.spmv(x.astype(np.float32), precision=hd.F32)
         assert np.max(np.abs(g32 - want)) <= 1e-5 * np.max(np.abs(want))
+        gc = h.spmv(x, precision=hd.CODED)
+        assert np.max(np.abs(gc - want)) <= 1e-13 * np.max(np.abs(want))
     assert np.allclose(h.simple(x), want, rtol=0, atol=1e-13 * np.max(np.abs(want)))
+    # the coded operator is the same FP64 matrix: identical products on unit vectors
+    for i in (0, 17, b.dim - 1):
+        e = np.zeros(b.dim)
+        e[i] = 1.0
+        assert np.array_equal(h.spmv(e, precision=hd.CODED), h.spmv(e))
 
 
 def test_spmv_complex(oracle):
@@ -176,6 +183,8 @@ def test_spmv_complex(oracle):
     assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
     g32 = h.spmv(x.astype(np.complex64), precision=hd.F32)
     assert np.max(np.abs(g32 - want)) <= 1e-5 * np.max(np.abs(want))
+    gc = h.spmv(x, precision=hd.CODED)
+    assert np.max(np.abs(gc - want)) <= 1e-13 * np.max(np.abs(want))
 
 
 def test_lanczos_config1(oracle):
@@ -198,6 +207,10 @@ def test_lanczos_config1(oracle):
     say(r32)
     assert r32.converged and np.all(r32.residuals <= 1e-10)
     assert np.max(np.abs(r32.evals - want) / np.maximum(1, np.abs(want))) < 1e-10
+    rc = h.lanczos(nev=5, tol=1e-12, precision=hd.CODED, seed=0)
+    say(rc)
+    assert rc.converged and np.all(rc.residuals <= 1e-12)
+    assert np.max(np.abs(rc.evals - want) / np.maximum(1, np.abs(want))) < 1e-10
 
 
 def test_lanczos_laughlin_zero_mode():
@@ -218,7 +231,7 @@ def test_lanczos_complex(oracle):
                         full=True)
     want = _dense_eigs(oracle, full, b.dim, 6)
     h = hd.Hamiltonian(b, t)
-    for prec in (hd.F64, hd.F32):
+    for prec in (hd.F64, hd.F32, hd.CODED):
         r = h.lanczos(nev=6, tol=1e-10, precision=prec, seed=0)
         say(prec, r)
         assert r.converged
