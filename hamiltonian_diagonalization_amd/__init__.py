@@ -115,6 +115,18 @@ SIGNATURES = {
                                   ctypes.POINTER(_LanczosStats)]),
     "hd_lanczos_vectors": (ctypes.c_int, [_P, _I32, _P, _P]),
     "hd_full_diagonalize": (ctypes.c_int, [_P, _I32, _P]),
+    "hd_mlp_create": (ctypes.c_int, [_I32, _I32, _U64, _PP]),
+    "hd_mlp_destroy": (ctypes.c_int, [_P]),
+    "hd_mlp_num_weights": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
+    "hd_mlp_get_weights": (ctypes.c_int, [_P, _P]),
+    "hd_mlp_set_weights": (ctypes.c_int, [_P, _P]),
+    "hd_mlp_forward": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
+    "hd_mlp_gradients": (ctypes.c_int, [_P, _P, _P, _P, _I64, ctypes.c_float,
+                                        ctypes.POINTER(ctypes.c_float), _P]),
+    "hd_mlp_train_step": (ctypes.c_int, [_P, _P, _P, _P, _I64, ctypes.c_float, ctypes.c_float,
+                                         ctypes.POINTER(ctypes.c_float)]),
+    "hd_mlp_start_vector": (ctypes.c_int, [_P, _P, _P, _P]),
+    "hd_mlp_gemm_check": (ctypes.c_int, [_I32, _I32, _I32, _P, _P, _P]),
 }
 
 

@@ -253,6 +253,39 @@ int hd_lanczos_vectors(const hd_hamiltonian* h, int32_t i, void* d_out, void* h_
  * the full CRS is expanded on the host and diagonalised by Jacobi rotations. */
 int hd_full_diagonalize(hd_hamiltonian* h, int32_t nev, double* h_evals);
 
+/* ------------------------------------------------------ learned start vector */
+
+/* No reference counterpart: the reference's warm start is an ARPACK initial
+ * vector read from a file (src/utilities/wrappers/arpack_wrapper.hpp:753-774).
+ * The amplitude network maps (occupations of n_bits orbitals, n_params
+ * Hamiltonian parameters) -> 256 -> 256 -> 256 (SiLU) -> (log|psi|, sign logit),
+ * BF16 weights and activations on the MFMA units with FP32 accumulation and an
+ * FP32 master copy trained with Adam.  Weights are exchanged as one flat FP32
+ * array: per layer the padded weight matrix (rows x cols: 256x64, 256x256,
+ * 256x256, 64x256) followed by its bias (rows). */
+typedef struct hd_mlp hd_mlp;
+int hd_mlp_create(int32_t n_bits, int32_t n_params, uint64_t seed, hd_mlp** out);
+int hd_mlp_destroy(hd_mlp* m);
+int hd_mlp_num_weights(const hd_mlp* m, int64_t* count);
+int hd_mlp_get_weights(hd_mlp* m, float* h_weights);
+int hd_mlp_set_weights(hd_mlp* m, const float* h_weights);
+/* Network outputs (n x 2: log|psi|, sign logit) for host states and per-sample
+ * parameters (n x n_params). */
+int hd_mlp_forward(hd_mlp* m, const uint64_t* h_states, const float* h_params, int64_t n, float* h_out);
+/* Loss and gradients (flat layout above) without an update, host inputs;
+ * targets are n x 3 (log|psi|, sign +-1, weight); lambda weights the sign term. */
+int hd_mlp_gradients(hd_mlp* m, const uint64_t* h_states, const float* h_params, const float* h_targets,
+                     int64_t n, float lambda, float* loss, float* h_grads);
+/* One Adam step on a device batch. */
+int hd_mlp_train_step(hd_mlp* m, const uint64_t* d_states, const float* d_params, const float* d_targets,
+                      int64_t n, float lambda, float lr, float* loss);
+/* The network's amplitudes over a whole sector basis as an FP64 Lanczos start
+ * vector (d_vec: dim entries; pass it as hd_lanczos_opts.d_start). */
+int hd_mlp_start_vector(hd_mlp* m, const hd_basis* b, const float* h_params, double* d_vec);
+/* The MFMA GEMM on host data: C (M x N, FP32) = A (M x K) . B (N x K)^T with
+ * BF16 bit patterns; M, N multiples of 64, K a multiple of 16. */
+int hd_mlp_gemm_check(int32_t M, int32_t N, int32_t K, const uint16_t* h_a, const uint16_t* h_b, float* h_c);
+
 #ifdef __cplusplus
 }
 #endif

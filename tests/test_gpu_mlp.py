@@ -1,0 +1,102 @@
+"""GPU: the BF16 MFMA GEMM, the network's forward and backward passes against
+the NumPy oracle, training, and the learned start vector."""
+import sys
+
+import numpy as np
+import pytest
+
+import hamiltonian_diagonalization_amd as hd
+from hamiltonian_diagonalization_amd import learned, models
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _bf16_bits(x):
+    return mo.bf16(x).view(np.uint32) >> 16
+
+
+@pytest.mark.parametrize("M,N,K", [(64, 64, 16), (128, 192, 64), (64, 256, 4096)])
+def test_mfma_gemm(M, N, K):
+    say("mfma gemm", M, N, K)
+    rng = np.random.default_rng(M + N + K)
+    A = mo.bf16(rng.normal(size=(M, K)).astype(np.float32))
+    B = mo.bf16(rng.normal(size=(N, K)).astype(np.float32))
+    C = learned.gemm_check(_bf16_bits(A).astype(np.uint16), _bf16_bits(B).astype(np.uint16))
+    want = A.astype(np.float64) @ B.astype(np.float64).T
+    assert np.max(np.abs(C - want)) <= 1e-5 * np.sqrt(K) * np.max(np.abs(want))
+
+
+def _data(rng, n, n_bits):
+    states = np.array([int(rng.integers(0, 1 << n_bits)) for _ in range(n)], dtype=np.uint64)
+    params = rng.random((n, 3)).astype(np.float32)
+    targets = np.stack([rng.normal(size=n), rng.choice([-1.0, 1.0], size=n),
+                        rng.random(n) + 0.5], 1).astype(np.float32)
+    return states, params, targets
+
+
+def test_forward_and_gradients_match_oracle():
+    say("mlp forward/backward vs oracle")
+    rng = np.random.default_rng(1)
+    n_bits = 34
+    net = learned.AmplitudeNet(n_bits, 3, seed=5)
+    w = net.get_weights()
+    states, params, targets = _data(rng, 300, n_bits)  # 300 pads to 320 rows
+    out = net.forward(states, params)
+    want, _, _ = mo.forward(w, states, params, n_bits, 3, bf=True)
+    assert np.max(np.abs(out - want[:, :2])) <= 2e-2 * max(1.0, np.max(np.abs(want[:, :2])))
+    loss, g = net.gradients(states, params, targets, lam=0.5)
+    lref, gref = mo.loss_and_grads(w, states, params, targets, n_bits, 3, lam=0.5, bf=True)
+    assert abs(loss - lref) <= 1e-3 * abs(lref)
+    rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+    say("gradient relative error", rel)
+    assert rel < 2e-2
+    # per layer too (no layer may hide behind a larger one)
+    for (gw, gb), (rw, rb) in zip(learned.unflatten(g), learned.unflatten(gref.astype(np.float32))):
+        assert np.linalg.norm(gw - rw) <= 5e-2 * np.linalg.norm(rw) + 1e-6
+        assert np.linalg.norm(gb - rb) <= 5e-2 * np.linalg.norm(rb) + 1e-6
+
+
+def test_weight_roundtrip_and_padding():
+    net = learned.AmplitudeNet(22, 3, seed=1)
+    w = net.get_weights()
+    layers = learned.unflatten(w)
+    assert np.all(layers[0][0][:, 25:] == 0) and np.all(layers[3][0][2:] == 0)
+    w2 = w.copy()
+    w2[:10] += 0.5
+    net.set_weights(w2)
+    assert np.array_equal(net.get_weights(), w2)
+
+
+def test_training_and_learned_start():
+    """Train on plain-solver ground states of a small sphere at a few
+    parameter points; the network's start vector must overlap the held-out
+    ground state better than before training, and Lanczos from it must reach
+    the same eigenvalues."""
+    say("mlp training on N=6 2S=15 ground states")
+    n, two_s = 6, 15
+    basis = hd.FockBasis(hd.Sector.lz(0, two_s), n, two_s + 1)
+    rng = np.random.default_rng(3)
+    pts = learned.sample_params(rng, 8)
+    data = learned.make_dataset(basis, two_s, pts, per_point=2048, seed=0, log=None)
+    net = learned.AmplitudeNet(two_s + 1, 3, seed=0)
+    held = learned.sample_params(np.random.default_rng(99), 1)[0]
+    h = learned.sphere_hamiltonian(basis, two_s, held)
+    ref = h.lanczos(nev=3, tol=1e-10, seed=0)
+    psi = h.eigenvector(0)
+    ov0 = learned.overlap(net, basis, held, psi)
+    losses = learned.train(net, data, steps=600, batch=1024, lr=2e-3, log=None)
+    say("loss", losses[0], "->", losses[-1])
+    assert losses[-1] < 0.3 * losses[0]
+    ov1 = learned.overlap(net, basis, held, psi)
+    say("overlap untrained", ov0, "trained", ov1)
+    assert ov1 > max(0.9, ov0)
+    start = net.start_vector(basis, held)
+    r = h.lanczos(nev=3, tol=1e-10, start=start)
+    assert r.converged
+    assert np.max(np.abs(r.evals - ref.evals)) < 1e-9
