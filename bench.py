@@ -151,11 +151,14 @@ def main():
         log(f"warmup {w}: {r}")
     hd.lib().hd_synchronize()
     res = None
+    live_ms, live_mv = 0.0, 0  # HIP-event time of every SpMV launched inside the timed steps
     t_all = time.perf_counter()
     for s in range(args.steps):
         t0 = time.perf_counter()
         res = ham.lanczos(nev=args.nev, tol=args.tol, precision=prec, seed=0)
         hd.lib().hd_synchronize()
+        live_ms += res.time_spmv_ms
+        live_mv += res.matvecs
         log(f"step {s}: {time.perf_counter() - t0:.3f} s {res}")
     total = time.perf_counter() - t_all
     step_s = total / max(args.steps, 1)
@@ -181,7 +184,8 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     key = "coded" if prec == hd.CODED else "f64"
-    achieved = sp_b[key] / (sp_ms[key] * 1e-3) / 1e9
+    spmv_live_ms = live_ms / max(live_mv, 1)
+    achieved = sp_b[key] / (spmv_live_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": round(step_s, 4),
@@ -225,6 +229,8 @@ def main():
             "traffic": None,
             "kernel": f"k_spmv_coded<double,{lanes}>" if prec == hd.CODED else f"k_spmv<double,{lanes}>",
             "bytes_per_launch": sp_b[key],
+            "avg_launch_ms": round(spmv_live_ms, 4),
+            "launches_timed": live_mv,
         },
         "cpu_baseline": cpu,
     }

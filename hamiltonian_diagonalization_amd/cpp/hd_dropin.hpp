@@ -1,0 +1,967 @@
+// hd_dropin.hpp -- C++ host classes with the reference's names and calling
+// conventions, implemented over the C ABI of libhdiag_hip.so (include/hdiag.h).
+//
+// A caller written against the reference's FermionFockBasis,
+// SpinlessFermionHamiltonian<T>, HamiltonianBase<T,B>, CrsSparseMatrix<T>,
+// MatrixVectorFunction<T,L1,L2> and ArpackWrapper keeps its call sequence:
+//
+//   FermionFockBasis basis;
+//   basis.GenerateFockSpace(N, Norb, AngularMomentumSector(lzTot, maxLz), mpi);
+//   ham.SetFockBasis(basis);
+//   ham.Initialize(N, Norb, utilities::_SPARSE_MAPPED_, mpi);
+//   ham.Add_CdC_Terms(&quadraticTables, 0, ham.m_data.m_fockSpaceDim, mpi);   // optional
+//   ham.Add_CdCdCC_Terms(&quarticTables, 0, ham.m_data.m_fockSpaceDim, mpi);
+//   MatrixVectorFunction<double, Quad, Quart> mv(ham, mpi);
+//   ham.LanczosDiagonalize(&quadraticTables, &quarticTables, mv, nev, mpi);
+//
+// Differences a caller sees (each forced by running on the device):
+//  - GenerateFockSpace takes the sector as a descriptor functor
+//    (AngularMomentumSector, LinearMomentumSector2D, LinearMomentumSector1D,
+//    NoConstraint).  They are callable like the reference's predicates
+//    (std::bind(&TestAngularMomentumSector, _1, lzTot, maxLz)), so they convert
+//    to std::function<bool(fock_t)>; an opaque std::function is rejected
+//    because a device basis cannot call a host predicate.
+//  - Add_CdC_Terms only records the quadratic table; Add_CdCdCC_Terms runs the
+//    whole build (count, scan, fill, sort) on the device over all rows, which
+//    is the only range the model callers use (pseudopotential_model.cpp:246-250).
+//  - The matrix lives in device memory in full (both triangles) storage.
+//    m_crsSparseMatrix presents the reference's upper-triangle layout
+//    (int32 row starts and columns, ascending columns, M[i][j] = <j|H|i>) and
+//    copies it from the device on first access.
+//  - LanczosDiagonalize runs the device thick-restart Lanczos instead of
+//    calling back into MatrixVectorFunction::Simple for every product; Simple
+//    stays available for callers that multiply host vectors themselves.
+//
+// Errors follow the reference: messages on stderr, mpi.m_exitFlag = true, and
+// MpiWrapper::ExitFlagTest() ends the process where the reference calls it.
+#ifndef HD_DROPIN_HPP
+#define HD_DROPIN_HPP
+
+#include <hdiag.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <iostream>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+// src/utilities/general/orbital_and_state_defs.hpp:36-58
+typedef uint64_t fock_t;
+typedef uint16_t kState_t;
+typedef int16_t mState_t;
+typedef uint32_t iSize_t;
+typedef int32_t crsIndex_t;
+typedef std::complex<double> dcmplx;
+#ifndef _SEARCH_ERROR_CODE_
+#define _SEARCH_ERROR_CODE_ UINT64_MAX
+#endif
+#ifndef _SPARSE_ZERO_TOL_
+#define _SPARSE_ZERO_TOL_ 1e-14
+#endif
+
+namespace utilities
+{
+#ifndef HD_DROPIN_REFERENCE_MPI
+    // Single-process stand-in for the reference's MpiWrapper
+    // (src/utilities/wrappers/mpi_wrapper.hpp): one host thread drives one GPU,
+    // so there is one rank and every stage owns all rows.  Inside the
+    // reference's build, define HD_DROPIN_REFERENCE_MPI and include its
+    // mpi_wrapper.hpp and sparse_matrix.hpp first; only m_id, m_nbrProcs,
+    // m_exitFlag and ExitFlagTest() are used.
+    struct MpiWrapper
+    {
+        int m_id = 0;
+        int m_nbrProcs = 1;
+        fock_t m_firstTask = 0;
+        fock_t m_lastTask = 0;
+        bool m_exitFlag = false;
+
+        // mpi_wrapper.cpp:327-338: stop the run once an error was flagged
+        void ExitFlagTest() const
+        {
+            if (m_exitFlag) {
+                std::cerr << "\n\tEXIT FLAG SET: TERMINATING" << std::endl;
+                std::exit(EXIT_FAILURE);
+            }
+        }
+    };
+
+    // src/utilities/data_structures/sparse_matrix.hpp:55
+    enum storageMethod_t { _SPARSE_MAPPED_, _SPARSE_CRS_, _DENSE_ };
+#endif
+
+    namespace detail
+    {
+        inline bool Ok(int status, const char* where)
+        {
+            if (status == HD_OK) return true;
+            std::cerr << "\n\tERROR in " << where << ": " << hd_last_error() << std::endl;
+            return false;
+        }
+        struct BasisDeleter { void operator()(hd_basis* b) const { hd_basis_destroy(b); } };
+        struct TablesDeleter { void operator()(hd_tables* t) const { hd_tables_destroy(t); } };
+        struct HamDeleter { void operator()(hd_hamiltonian* h) const { hd_ham_destroy(h); } };
+        template <typename T> struct IsComplex : std::false_type {};
+        template <> struct IsComplex<dcmplx> : std::true_type {};
+    }
+
+    // CrsSparseMatrix<T> (src/utilities/data_structures/crs_sparse_matrix.hpp:123-816):
+    // the reference's upper-triangle CRS of H^T.  The arrays are a host copy of
+    // a device build in that layout, made on first access.
+    template <typename T>
+    class CrsSparseMatrix
+    {
+        public:
+        CrsSparseMatrix() = default;
+
+        void Initialize(const int nbrRows, const int nbrCols, const long int nbrNonZerosEstimate)
+        {
+            Clear();
+            m_nbrRows = nbrRows;
+            m_nbrCols = nbrCols;
+            m_values.reserve((size_t)std::max<long int>(nbrNonZerosEstimate, 0));
+            m_cols.reserve((size_t)std::max<long int>(nbrNonZerosEstimate, 0));
+            m_rowStarts.assign((size_t)nbrRows + 1, 0);
+            m_materialized = true;
+        }
+
+        void Clear()
+        {
+            m_values.clear();
+            m_cols.clear();
+            m_rowStarts.clear();
+            m_nbrRows = m_nbrCols = 0;
+            m_materialized = false;
+            m_source.reset();
+            m_basis.reset();
+        }
+
+        // Records where the host copy comes from: the device basis and tables,
+        // built again in upper-triangle storage when an accessor is first used.
+        void AdoptDevice(std::shared_ptr<hd_basis> basis, std::shared_ptr<hd_tables> tables,
+                         const double zeroTol, const int dim)
+        {
+            Clear();
+            m_basis = std::move(basis);
+            m_source = std::move(tables);
+            m_zeroTol = zeroTol;
+            m_nbrRows = m_nbrCols = dim;
+        }
+
+        long int GetNbrNonZeros() const { Materialize(); return (long int)m_values.size(); }
+        int GetNbrRows() const { return m_nbrRows; }
+        const int* GetRowStarts() const { Materialize(); return m_rowStarts.data(); }
+        int* GetRowStarts() { Materialize(); return m_rowStarts.data(); }
+        const int* GetCols() const { Materialize(); return m_cols.data(); }
+        int* GetCols() { Materialize(); return m_cols.data(); }
+        const T* GetData() const { Materialize(); return m_values.data(); }
+        T* GetData() { Materialize(); return m_values.data(); }
+
+        // crs_sparse_matrix.hpp Value(i, j): the stored element or zero
+        T Value(const int i, const int j) const
+        {
+            Materialize();
+            const int* b = m_cols.data() + m_rowStarts[i];
+            const int* e = m_cols.data() + m_rowStarts[i + 1];
+            const int* p = std::lower_bound(b, e, j);
+            return (p != e && *p == j) ? m_values[(size_t)(p - m_cols.data())] : T(0.0);
+        }
+
+        // crs_sparse_matrix.hpp:643-660: row-major dense copy of the stored entries
+        void ConvertToArray(T* denseMatrix, const int nbrCols) const
+        {
+            Materialize();
+            std::fill(denseMatrix, denseMatrix + (size_t)m_nbrRows * nbrCols, T(0.0));
+            for (int i = 0; i < m_nbrRows; ++i)
+                for (int k = m_rowStarts[i]; k < m_rowStarts[i + 1]; ++k)
+                    denseMatrix[(size_t)i * nbrCols + m_cols[k]] = m_values[k];
+        }
+
+        void Print() const
+        {
+            Materialize();
+            for (int i = 0; i < m_nbrRows; ++i)
+                for (int k = m_rowStarts[i]; k < m_rowStarts[i + 1]; ++k)
+                    std::cout << "\t(" << i << "," << m_cols[k] << ")\t" << m_values[k] << "\n";
+        }
+
+        private:
+        void Materialize() const
+        {
+            if (m_materialized) return;
+            m_materialized = true;
+            if (!m_source) return;
+            hd_hamiltonian* raw = nullptr;
+            if (!detail::Ok(hd_ham_build(m_basis.get(), m_source.get(), HD_STORAGE_UPPER_REF, m_zeroTol, 1, &raw),
+                            "CrsSparseMatrix (upper-triangle build)"))
+                return;
+            std::unique_ptr<hd_hamiltonian, detail::HamDeleter> h(raw);
+            hd_ham_info_t info;
+            if (!detail::Ok(hd_ham_info(h.get(), &info), "CrsSparseMatrix")) return;
+            // crsIndex_t is int32 in the reference: nnz < 2^31 (crs_sparse_matrix.hpp:127-133)
+            if (info.nnz >= ((int64_t)1 << 31)) {
+                std::cerr << "\n\tERROR in CrsSparseMatrix: " << info.nnz
+                          << " non-zeros exceed the int32 row starts of the reference layout" << std::endl;
+                return;
+            }
+            std::vector<int64_t> rp((size_t)info.dim + 1);
+            m_cols.resize((size_t)info.nnz);
+            m_values.resize((size_t)info.nnz);
+            if (!detail::Ok(hd_ham_copy_crs(h.get(), 0, info.dim, HD_F64, rp.data(), m_cols.data(), m_values.data()),
+                            "CrsSparseMatrix"))
+                return;
+            m_rowStarts.assign(rp.begin(), rp.end());
+        }
+
+        mutable std::vector<T> m_values;
+        mutable std::vector<int> m_cols;
+        mutable std::vector<int> m_rowStarts;
+        mutable bool m_materialized = false;
+        int m_nbrRows = 0;
+        int m_nbrCols = 0;
+        double m_zeroTol = _SPARSE_ZERO_TOL_;
+        std::shared_ptr<hd_tables> m_source;
+        std::shared_ptr<hd_basis> m_basis;
+    };
+
+    namespace linearAlgebra
+    {
+        // Options of ArpackWrapper (src/utilities/wrappers/arpack_wrapper.hpp,
+        // defaults arpack_wrapper.cpp:38-49).  The iteration itself runs on the
+        // device (hd_lanczos); the initial/final vector files keep the
+        // reference's raw T[nodeDim] binary format (arpack_wrapper.hpp:753-774).
+        class ArpackWrapper
+        {
+            public:
+            void SetEigenvalueMagnitude(const char which) { m_which = which; }
+            void UseInitialVector() { m_provideInitial = true; }
+            void StoreFinalVector() { m_storeFinal = true; }
+            void SetInitialVectorFile(const std::string& name) { m_initialVectorFile = name; }
+            void SetFinalVectorFile(const std::string& name) { m_finalVectorFile = name; }
+            char GetUpperOrLower() const { return 'U'; }
+            // The reference leaves ARPACK's TOL uninitialised (arpack_wrapper.hpp:624);
+            // here it is explicit: ||Hx - lx|| <= tol * max(1, |l|).
+            void SetTolerance(const double tol) { m_tol = tol; }
+            void SetNbrKrylovVectors(const int ncv) { m_ncv = ncv; }
+            void SetMaxRestarts(const int n) { m_maxRestarts = n; }
+            void SetSeed(const uint64_t seed) { m_seed = seed; }
+            // HD_F64 (default), HD_CODED, or HD_F32 with FP64 refinement.
+            void SetPrecision(const int precision) { m_precision = precision; }
+            // A device start vector (e.g. the learned start of hd_mlp_start_vector);
+            // takes precedence over the initial-vector file.
+            void SetDeviceStartVector(const void* d_start) { m_deviceStart = d_start; }
+
+            char m_which = 'S';
+            bool m_provideInitial = false;
+            bool m_storeFinal = false;
+            std::string m_initialVectorFile;
+            std::string m_finalVectorFile;
+            double m_tol = 1e-12;
+            int m_ncv = 0;
+            int m_maxRestarts = 0;
+            uint64_t m_seed = 0;
+            int m_precision = HD_F64;
+            const void* m_deviceStart = nullptr;
+            hd_lanczos_stats m_lastStats{};
+        };
+    }
+}
+
+namespace diagonalization
+{
+    // Sector descriptors.  Each evaluates the same predicate as the reference
+    // function it names, so it can stand wherever that predicate was bound.
+
+    // TestAngularMomentumSector (angular_momentum_constraint.hpp:41-62)
+    struct AngularMomentumSector
+    {
+        int m_lzTot, m_maxLz;
+        AngularMomentumSector(const int lzTot, const int maxLz) : m_lzTot(lzTot), m_maxLz(maxLz) {}
+        bool operator()(const fock_t state) const
+        {
+            long s = 0;
+            for (fock_t b = state; b; b &= b - 1) s += 2 * __builtin_ctzll(b) - m_maxLz;
+            return s == m_lzTot;
+        }
+        hd_sector Descriptor() const { return hd_sector{HD_SECTOR_LZ, m_lzTot, m_maxLz, 0, 0}; }
+    };
+
+    // TestLinearMomentumSector2 (linear_momentum_constraints_2D.cpp:38-79), k = kx*dimY + ky
+    struct LinearMomentumSector2D
+    {
+        int m_kx, m_ky, m_dimX, m_dimY;
+        LinearMomentumSector2D(const int kx, const int ky, const int dimX, const int dimY)
+            : m_kx(kx), m_ky(ky), m_dimX(dimX), m_dimY(dimY) {}
+        bool operator()(const fock_t state) const
+        {
+            long sx = 0, sy = 0;
+            for (fock_t b = state; b; b &= b - 1) {
+                const int k = __builtin_ctzll(b);
+                sx += k / m_dimY;
+                sy += k % m_dimY;
+            }
+            return sx % m_dimX == m_kx && sy % m_dimY == m_ky;
+        }
+        hd_sector Descriptor() const { return hd_sector{HD_SECTOR_K2D, m_kx, m_ky, m_dimX, m_dimY}; }
+    };
+
+    // TestLinearMomentumSector1 (linear_momentum_constraints_2D.cpp:87-110)
+    struct LinearMomentumSector1D
+    {
+        int m_ky, m_dimY;
+        LinearMomentumSector1D(const int ky, const int dimY) : m_ky(ky), m_dimY(dimY) {}
+        bool operator()(const fock_t state) const
+        {
+            long s = 0;
+            for (fock_t b = state; b; b &= b - 1) s += __builtin_ctzll(b);
+            return s % m_dimY == m_ky;
+        }
+        hd_sector Descriptor() const { return hd_sector{HD_SECTOR_K1D, m_ky, m_dimY, 0, 0}; }
+    };
+
+    // [](const fock_t state){return true;} (pseudopotential_model.cpp:197)
+    struct NoConstraint
+    {
+        bool operator()(const fock_t) const { return true; }
+        hd_sector Descriptor() const { return hd_sector{HD_SECTOR_NONE, 0, 0, 0, 0}; }
+    };
+
+    // FermionFockBasis (src/basis/fermion_fock_basis.hpp, .cpp): m_basis is the
+    // host copy of the device basis, in the reference's ascending order.
+    class FermionFockBasis
+    {
+        public:
+        std::vector<fock_t> m_basis;
+        bool m_basisStored = false;
+        fock_t m_firstState = 0;
+
+        void Clear()
+        {
+            m_basis.clear();
+            m_basisStored = false;
+            m_firstState = 0;
+            m_device.reset();
+        }
+
+        // fermion_fock_basis.cpp:86-92: C(nbrOrbitals, nbrParticles), particles first
+        fock_t GetFullFockSpaceDimension(const iSize_t nbrParticles, const iSize_t nbrOrbitals) const
+        {
+            if (nbrParticles > nbrOrbitals) return 0;
+            unsigned __int128 c = 1;
+            for (iSize_t i = 0; i < nbrParticles; ++i) c = c * (nbrOrbitals - i) / (i + 1);
+            return (fock_t)c;
+        }
+
+        fock_t GetReducedFockSpaceDimension(const iSize_t, const iSize_t, const utilities::MpiWrapper&) const
+        {
+            return m_basis.size();
+        }
+
+        // fermion_fock_basis.cpp:134-161: offset + position of the state, or
+        // _SEARCH_ERROR_CODE_; the search is over the host copy, whose order is
+        // the device ranking's.
+        fock_t FindFockStateIndex(const fock_t offset, const fock_t state) const
+        {
+            if (!m_basisStored) {
+                std::cerr << "ERROR in FindFockStateIndex: BASIS WAS NOT STORED " << std::endl;
+                return _SEARCH_ERROR_CODE_;
+            }
+            auto it = std::lower_bound(m_basis.begin(), m_basis.end(), state);
+            if (it == m_basis.end() || *it != state) return _SEARCH_ERROR_CODE_;
+            return offset + (fock_t)(it - m_basis.begin());
+        }
+
+        fock_t GetFirstState(const fock_t firstNodeIndex, const iSize_t,
+                             std::vector<fock_t>::const_iterator& it_state) const
+        {
+            if (firstNodeIndex != m_firstState || m_basis.empty()) {
+                std::cerr << "\n\tERROR in GetFirstState: non-matching first state index." << std::endl;
+                std::exit(EXIT_FAILURE);
+            }
+            it_state = m_basis.begin();
+            return m_basis[0];
+        }
+
+        fock_t GetNextState(const fock_t, std::vector<fock_t>::const_iterator& it_state) const
+        {
+            ++it_state;
+            return *it_state;
+        }
+
+        // fermion_fock_basis.cpp:241-354: returns true when the sector is empty.
+        bool GenerateFockSpace(const iSize_t nbrParticles, const iSize_t nbrOrbitals,
+                               const std::function<bool(const fock_t state)>& Constraint,
+                               utilities::MpiWrapper& mpi)
+        {
+            hd_sector s;
+            if (auto* p = Constraint.target<AngularMomentumSector>()) s = p->Descriptor();
+            else if (auto* p2 = Constraint.target<LinearMomentumSector2D>()) s = p2->Descriptor();
+            else if (auto* p1 = Constraint.target<LinearMomentumSector1D>()) s = p1->Descriptor();
+            else if (Constraint.target<NoConstraint>()) s = NoConstraint().Descriptor();
+            else {
+                std::cerr << "\n\tERROR in GenerateFockSpace: the constraint must be a sector descriptor "
+                             "(AngularMomentumSector, LinearMomentumSector2D, LinearMomentumSector1D, "
+                             "NoConstraint); a device basis cannot evaluate a host predicate." << std::endl;
+                mpi.m_exitFlag = true;
+                mpi.ExitFlagTest();
+                return true;
+            }
+            return GenerateFockSpace(nbrParticles, nbrOrbitals, s, mpi);
+        }
+
+        bool GenerateFockSpace(const iSize_t nbrParticles, const iSize_t nbrOrbitals, const hd_sector& sector,
+                               utilities::MpiWrapper& mpi)
+        {
+            Clear();
+            hd_basis* raw = nullptr;
+            const int st = hd_basis_create(&sector, nbrParticles, nbrOrbitals, &raw);
+            if (st == HD_ERR_EMPTY) return true;
+            if (!utilities::detail::Ok(st, "GenerateFockSpace")) {
+                mpi.m_exitFlag = true;
+                mpi.ExitFlagTest();
+                return true;
+            }
+            m_device.reset(raw, utilities::detail::BasisDeleter());
+            int64_t dim = 0;
+            hd_basis_dim(raw, &dim);
+            m_basis.resize((size_t)dim);
+            if (!utilities::detail::Ok(hd_basis_copy_states(raw, 0, dim, m_basis.data()), "GenerateFockSpace")) {
+                mpi.m_exitFlag = true;
+                mpi.ExitFlagTest();
+            }
+            m_basisStored = true;
+            m_firstState = 0;
+            return dim == 0;
+        }
+
+        fock_t GetStoredDimension() const { return m_basis.size(); }
+
+        // fermion_fock_basis.cpp:360-377
+        void GetFockBasis(fock_t* buffer, const fock_t dim, utilities::MpiWrapper& mpi) const
+        {
+            if (m_basis.size() >= dim) {
+                std::memcpy(buffer, m_basis.data(), dim * sizeof(fock_t));
+            } else {
+                std::cerr << "\n\tERROR in GetFockBasis: Given copy buffer is longer than the allocated "
+                             "Fock space buffer dimension" << std::endl;
+                mpi.m_exitFlag = true;
+            }
+            mpi.ExitFlagTest();
+        }
+
+        // Device basis handle (shared by copies, as the reference copies the basis
+        // into the Hamiltonian: hamiltonian_base.hpp:597-600).
+        std::shared_ptr<hd_basis> Device() const { return m_device; }
+
+        private:
+        std::shared_ptr<hd_basis> m_device;
+    };
+
+    // HamiltonianData (src/hamiltonians/hamiltonian_data.hpp:58-85)
+    struct HamiltonianData
+    {
+        iSize_t m_nbrParticles = 0;
+        iSize_t m_nbrOrbitals = 0;
+        fock_t m_highestOrbital = 0;
+        utilities::storageMethod_t m_storageMethod = utilities::_SPARSE_CRS_;
+        bool m_matrixAllocated = false;
+        bool m_eigenvaluesCalculated = false;
+        bool m_eigenvectorsCalculated = false;
+        fock_t m_fockSpaceDim = 0;
+        fock_t m_nodeDim = 0;
+        iSize_t m_nbrEigenvalues = 0;
+
+        HamiltonianData() = default;
+        HamiltonianData(const iSize_t nbrParticles, const iSize_t nbrOrbitals,
+                        const utilities::storageMethod_t storageMethod)
+            : m_nbrParticles(nbrParticles), m_nbrOrbitals(nbrOrbitals),
+              m_highestOrbital(nbrOrbitals ? (fock_t)1 << (nbrOrbitals - 1) : 0),  // hamiltonian_data.cpp:67
+              m_storageMethod(storageMethod) {}
+        void Clear() { *this = HamiltonianData(); }
+        void SetDataDimensions(const fock_t fockSpaceDimension, const iSize_t, utilities::MpiWrapper&)
+        {
+            m_fockSpaceDim = fockSpaceDimension;
+            m_nodeDim = fockSpaceDimension;
+        }
+    };
+
+    template <typename T, class L1, class L2> class MatrixVectorFunction;
+
+    // HamiltonianBase<T,B> (src/hamiltonians/hamiltonian_base.hpp)
+    template <typename T, class B>
+    class HamiltonianBase
+    {
+        static_assert(std::is_same<T, double>::value || std::is_same<T, dcmplx>::value,
+                      "matrix elements are double or std::complex<double>");
+
+        public:
+        HamiltonianData m_data;
+        B m_fockBasis;
+        std::vector<double> m_eigenvalues;
+        std::vector<T> m_eigenvectors;        // nev x nodeDim, eigenvector i at [i*nodeDim]
+        utilities::CrsSparseMatrix<T> m_crsSparseMatrix;
+        utilities::linearAlgebra::ArpackWrapper m_arpackWrapper;
+
+        void SetFockBasis(const B& fockBasis) { m_fockBasis = fockBasis; }
+
+        // hamiltonian_base.hpp:605-623
+        void SetNbrEigenvalues(const iSize_t nbrEigenvalues, const utilities::MpiWrapper&)
+        {
+            if (nbrEigenvalues > m_data.m_fockSpaceDim) {
+                std::cerr << "\n\tWARNING in SpinlessFermionHamiltonian: nbrEigenvalues larger than Fock space dimension:"
+                          << std::endl << "\tSet number of eigenvalues to " << m_data.m_fockSpaceDim << std::endl;
+                m_data.m_nbrEigenvalues = (iSize_t)m_data.m_fockSpaceDim;
+            } else {
+                m_data.m_nbrEigenvalues = nbrEigenvalues;
+            }
+        }
+
+        // hamiltonian_base.hpp:378-449: true when every stored entry is diagonal
+        bool MatrixDiagonalCheck(const int, const utilities::MpiWrapper&)
+        {
+            if (!m_device) return false;
+            hd_ham_info_t info;
+            if (!utilities::detail::Ok(hd_ham_info(m_device.get(), &info), "MatrixDiagonalCheck")) return false;
+            if (info.nnz > info.dim) return false;
+            std::vector<int64_t> rp((size_t)info.dim + 1);
+            std::vector<int32_t> col((size_t)info.nnz);
+            std::vector<T> val((size_t)info.nnz);
+            if (!utilities::detail::Ok(hd_ham_copy_crs(m_device.get(), 0, info.dim, HD_F64, rp.data(), col.data(),
+                                                       val.data()), "MatrixDiagonalCheck"))
+                return false;
+            for (int64_t i = 0; i < info.dim; ++i)
+                for (int64_t k = rp[i]; k < rp[i + 1]; ++k)
+                    if (col[k] != i) return false;
+            m_diagonal.assign((size_t)info.dim, 0.0);
+            for (int64_t i = 0; i < info.dim; ++i)
+                for (int64_t k = rp[i]; k < rp[i + 1]; ++k) m_diagonal[i] = std::real(val[k]);
+            return true;
+        }
+
+        // hamiltonian_base.hpp:634-758 (dense diagonalisation of small sectors)
+        bool FullDiagonalize(iSize_t& nbrEigenvalues, utilities::MpiWrapper& mpi)
+        {
+            SetNbrEigenvalues(nbrEigenvalues, mpi);
+            nbrEigenvalues = m_data.m_nbrEigenvalues;
+            if (!m_device) {
+                std::cerr << "\n\tERROR in FullDiagonalize: the Hamiltonian was not built" << std::endl;
+                return false;
+            }
+            m_eigenvalues.assign(m_data.m_nbrEigenvalues, 0.0);
+            const bool ok = utilities::detail::Ok(
+                hd_full_diagonalize(m_device.get(), (int32_t)m_data.m_nbrEigenvalues, m_eigenvalues.data()),
+                "FullDiagonalize");
+            m_data.m_eigenvaluesCalculated = ok;
+            m_data.m_eigenvectorsCalculated = false;  // eigenvalues only on this path
+            DeallocateMatrixMemory();
+            return ok;
+        }
+
+        // hamiltonian_base.hpp:769-931
+        template <class L1, class L2>
+        bool LanczosDiagonalize(L1*, L2*, MatrixVectorFunction<T, L1, L2>&, iSize_t& nbrEigenvalues,
+                                utilities::MpiWrapper& mpi)
+        {
+            SetNbrEigenvalues(nbrEigenvalues, mpi);
+            nbrEigenvalues = m_data.m_nbrEigenvalues;
+            m_data.m_eigenvaluesCalculated = m_data.m_eigenvectorsCalculated = false;
+            if (!m_data.m_matrixAllocated || !m_device) {
+                std::cerr << "\n\tERROR in LanczosDiagonalize: the Hamiltonian was not built" << std::endl;
+                return false;
+            }
+            const iSize_t nev = m_data.m_nbrEigenvalues;
+            const fock_t dim = m_data.m_nodeDim;
+            m_eigenvalues.assign(nev, 0.0);
+            m_eigenvectors.assign((size_t)nev * dim, T(0.0));
+            if (MatrixDiagonalCheck(0, mpi)) {
+                // hamiltonian_base.hpp:803-872: sorted diagonal, unit vectors on each
+                // degenerate set, normalised by the set's size as in the reference
+                std::vector<double> sorted(m_diagonal);
+                std::sort(sorted.begin(), sorted.end());
+                for (iSize_t i = 0; i < nev; ++i) {
+                    m_eigenvalues[i] = sorted[i];
+                    double norm = 0.0;
+                    for (fock_t j = 0; j < dim; ++j)
+                        if (std::fabs(m_eigenvalues[i] - m_diagonal[j]) < _SPARSE_ZERO_TOL_) {
+                            m_eigenvectors[(size_t)i * dim + j] = T(1.0);
+                            norm += 1.0;
+                        }
+                    for (fock_t j = 0; j < dim; ++j) m_eigenvectors[(size_t)i * dim + j] /= norm;
+                }
+            } else {
+                hd_lanczos_opts o{};
+                o.nev = (int32_t)nev;
+                o.ncv = m_arpackWrapper.m_ncv;
+                o.max_restarts = m_arpackWrapper.m_maxRestarts;
+                o.tol = m_arpackWrapper.m_tol;
+                o.seed = m_arpackWrapper.m_seed;
+                o.precision = m_arpackWrapper.m_precision;
+                o.want_vectors = 1;
+                o.start = HD_START_RANDOM;
+                void* d_file = nullptr;
+                if (m_arpackWrapper.m_deviceStart) {
+                    o.start = HD_START_USER;
+                    o.d_start = m_arpackWrapper.m_deviceStart;
+                } else if (m_arpackWrapper.m_provideInitial) {
+                    // arpack_wrapper.hpp:753-774: raw T[nodeDim]; a missing file falls
+                    // back to the random start with a message
+                    std::vector<T> init(dim);
+                    std::ifstream f(m_arpackWrapper.m_initialVectorFile, std::ios::binary);
+                    if (!f.is_open()) {
+                        std::cerr << "\n\tERROR in DiagonalizeSymmetricMatrix: Cannot open initial vector file "
+                                  << m_arpackWrapper.m_initialVectorFile << std::endl
+                                  << "\n\tUsing random starting vector instead." << std::endl;
+                    } else {
+                        f.read(reinterpret_cast<char*>(init.data()), (std::streamsize)(dim * sizeof(T)));
+                        if (hd_malloc(&d_file, dim * sizeof(T)) == HD_OK &&
+                            hd_memcpy_h2d(d_file, init.data(), dim * sizeof(T)) == HD_OK) {
+                            o.start = HD_START_USER;
+                            o.d_start = d_file;
+                        }
+                    }
+                }
+                const int st = hd_lanczos(m_device.get(), &o, m_eigenvalues.data(), &m_arpackWrapper.m_lastStats);
+                if (d_file) hd_free(d_file);
+                if (st != HD_OK && st != HD_ERR_NOT_CONVERGED) {
+                    utilities::detail::Ok(st, "LanczosDiagonalize");
+                    return false;
+                }
+                if (st == HD_ERR_NOT_CONVERGED)
+                    std::cerr << "\n\tWARNING in LanczosDiagonalize: " << hd_last_error() << std::endl;
+                for (iSize_t i = 0; i < nev; ++i)
+                    if (!utilities::detail::Ok(hd_lanczos_vectors(m_device.get(), (int32_t)i, nullptr,
+                                                                  m_eigenvectors.data() + (size_t)i * dim),
+                                               "LanczosDiagonalize"))
+                        return false;
+                if (m_arpackWrapper.m_storeFinal && !m_arpackWrapper.m_finalVectorFile.empty()) {
+                    // the ground state in the same raw format, for a later --use-initial run
+                    std::ofstream f(m_arpackWrapper.m_finalVectorFile, std::ios::binary);
+                    f.write(reinterpret_cast<const char*>(m_eigenvectors.data()), (std::streamsize)(dim * sizeof(T)));
+                }
+                if (st == HD_ERR_NOT_CONVERGED) return false;
+            }
+            DeallocateMatrixMemory();
+            m_data.m_eigenvaluesCalculated = m_data.m_eigenvectorsCalculated = true;
+            return true;
+        }
+
+        // hamiltonian_base.hpp:1188-1200
+        void GetEigenvalues(double* buffer, const iSize_t nbrEigenvalues) const
+        {
+            if (nbrEigenvalues > m_eigenvalues.size())
+                std::cerr << "\n\t- ERROR in GetEigenvalues: incorrect number specified" << std::endl;
+            std::memcpy(buffer, m_eigenvalues.data(),
+                        sizeof(double) * std::min<size_t>(nbrEigenvalues, m_eigenvalues.size()));
+        }
+
+        fock_t GetEigenvectorNodeDim() const { return m_data.m_nodeDim; }
+
+        void PrintEigensystem() const
+        {
+            std::cout << "\n\tEIGENVALUES:\n";
+            for (double e : m_eigenvalues) std::printf("\t%.15g\n", e);
+        }
+
+        // Device handle of the full-storage matrix (null after DeallocateMatrixMemory).
+        hd_hamiltonian* Device() const { return m_device.get(); }
+
+        protected:
+        // hamiltonian_base.hpp:149-180: the matrix is released after diagonalisation
+        void DeallocateMatrixMemory()
+        {
+            m_device.reset();
+            m_data.m_matrixAllocated = false;
+        }
+
+        std::shared_ptr<hd_hamiltonian> m_device;
+        std::vector<double> m_diagonal;
+    };
+
+    // SpinlessFermionHamiltonian<T> (src/hamiltonians/spinless_fermion_hamiltonian.hpp)
+    template <typename T>
+    class SpinlessFermionHamiltonian : public HamiltonianBase<T, FermionFockBasis>
+    {
+        public:
+        // spinless_fermion_hamiltonian.hpp:59-105
+        void Initialize(const iSize_t nbrParticles, const iSize_t nbrOrbitals,
+                        const utilities::storageMethod_t storageMethod, utilities::MpiWrapper& mpi)
+        {
+            this->m_data = HamiltonianData(nbrParticles, nbrOrbitals, storageMethod);
+            if (0 >= (int)this->m_data.m_nbrParticles) {
+                std::cerr << "\n\tERROR: MUST SPECIFY nbr particles > 0" << std::endl;
+                mpi.m_exitFlag = true;
+            }
+            if (0 >= (int)this->m_data.m_nbrOrbitals) {
+                std::cerr << "\n\tERROR: MUST SPECIFY nbr orbitals > 0" << std::endl;
+                mpi.m_exitFlag = true;
+            }
+            if (this->m_data.m_nbrParticles >= 64) {
+                std::cerr << "\n\n\tERROR: CANNOT ADDRESS 64 PARTICLES OR MORE.\n" << std::endl;
+                mpi.m_exitFlag = true;
+            }
+            if (this->m_data.m_nbrParticles > this->m_data.m_nbrOrbitals) {
+                std::cerr << "\n\n\tERROR: NBR PARTICLES > NBR ORBITALS - NOT ALLOWED\n" << std::endl;
+                mpi.m_exitFlag = true;
+            }
+            mpi.ExitFlagTest();
+            this->m_data.SetDataDimensions(this->m_fockBasis.GetStoredDimension(), mpi.m_nbrProcs, mpi);
+            m_quadratic.clear();
+            m_haveQuadratic = false;
+        }
+
+        void Clear()
+        {
+            this->DeallocateMatrixMemory();
+            this->m_fockBasis.Clear();
+            this->m_data.Clear();
+            this->m_crsSparseMatrix.Clear();
+            this->m_eigenvalues.clear();
+            this->m_eigenvectors.clear();
+            m_quadratic.clear();
+            m_haveQuadratic = false;
+        }
+
+        // spinless_fermion_hamiltonian.hpp:143-257.  Quadratic terms are diagonal
+        // on both reference models (GetK1(k2) returns k1 = k2); their E_k enter the
+        // device build together with the quartic terms, quadratic terms first as
+        // in the reference.
+        template <class L>
+        void Add_CdC_Terms(L* termTables, const fock_t minRow, const fock_t maxRow, utilities::MpiWrapper& mpi)
+        {
+            if (!CheckRange(minRow, maxRow, "Add_CdC_Terms", mpi)) return;
+            const iSize_t kMax = termTables->GetDimension();
+            m_quadratic.assign(kMax * (utilities::detail::IsComplex<T>::value ? 2 : 1), 0.0);
+            for (iSize_t k = 0; k < kMax; ++k) {
+                kState_t buf[8];
+                iSize_t n = 0;
+                termTables->GetK1(buf, n, (kState_t)k);
+                if (n > 1 || (n == 1 && buf[0] != k)) {
+                    std::cerr << "\n\tERROR in Add_CdC_Terms: only diagonal quadratic terms (k1 == k2) are "
+                                 "supported on the device" << std::endl;
+                    mpi.m_exitFlag = true;
+                    return;
+                }
+                const T e = n ? termTables->GetEkk((kState_t)k, (kState_t)k) : T(0.0);
+                std::memcpy(&m_quadratic[k * (utilities::detail::IsComplex<T>::value ? 2 : 1)], &e, sizeof(T));
+            }
+            m_haveQuadratic = true;
+        }
+
+        // spinless_fermion_hamiltonian.hpp:281-436 plus the map -> CRS conversion
+        // (mapped_sparse_matrix.hpp:423-462, crs_sparse_matrix.hpp:193-254)
+        template <class L>
+        void Add_CdCdCC_Terms(L* termTables, const fock_t minRow, const fock_t maxRow, utilities::MpiWrapper& mpi)
+        {
+            if (!CheckRange(minRow, maxRow, "Add_CdCdCC_Terms", mpi)) return;
+            std::vector<kState_t>* kt = termTables->GetKTable();
+            std::vector<T>* vt = termTables->GetVTable();
+            const size_t n = vt->size();
+            // the table's layout follows from its size: kMax^3 (sphere,
+            // term_tables.cpp:36-44) or (kMax+1) kMax^2 / 2 (flux lattice fold)
+            int layout = -1;
+            uint32_t kMax = 0;
+            for (uint32_t K = 1; K <= 64 && layout < 0; ++K) {
+                if ((size_t)K * K * K == n && kt->size() == n) { layout = HD_LAYOUT_SPHERE; kMax = K; }
+                else if ((size_t)(K + 1) * K * K / 2 == n && kt->size() == n) { layout = HD_LAYOUT_OFL; kMax = K; }
+            }
+            if (layout < 0) {
+                std::cerr << "\n\tERROR in Add_CdCdCC_Terms: table size " << n
+                          << " matches neither the sphere nor the flux-lattice layout" << std::endl;
+                mpi.m_exitFlag = true;
+                return;
+            }
+            if (m_haveQuadratic && m_quadratic.size() != kMax * (utilities::detail::IsComplex<T>::value ? 2u : 1u)) {
+                std::cerr << "\n\tERROR in Add_CdCdCC_Terms: quadratic and quartic tables disagree on kMax" << std::endl;
+                mpi.m_exitFlag = true;
+                return;
+            }
+            hd_tables* rawT = nullptr;
+            if (!utilities::detail::Ok(hd_tables_create(layout, kMax, kt->data(),
+                                                        reinterpret_cast<const double*>(vt->data()),
+                                                        utilities::detail::IsComplex<T>::value ? 1 : 0,
+                                                        m_haveQuadratic ? m_quadratic.data() : nullptr, &rawT),
+                                       "Add_CdCdCC_Terms")) {
+                mpi.m_exitFlag = true;
+                return;
+            }
+            std::shared_ptr<hd_tables> tables(rawT, utilities::detail::TablesDeleter());
+            hd_hamiltonian* rawH = nullptr;
+            if (!utilities::detail::Ok(hd_ham_build(this->m_fockBasis.Device().get(), tables.get(), HD_STORAGE_FULL,
+                                                    _SPARSE_ZERO_TOL_, m_valueMask, &rawH),
+                                       "Add_CdCdCC_Terms")) {
+                mpi.m_exitFlag = true;
+                return;
+            }
+            this->m_device.reset(rawH, utilities::detail::HamDeleter());
+            this->m_data.m_matrixAllocated = true;
+            this->m_data.m_storageMethod = utilities::_SPARSE_CRS_;
+            this->m_crsSparseMatrix.AdoptDevice(this->m_fockBasis.Device(), tables, _SPARSE_ZERO_TOL_,
+                                                (int)this->m_data.m_fockSpaceDim);
+        }
+
+        // Resident value copies of the device build (hdiag.h: bit 0 FP64, bit 1
+        // FP32, bit 2 coded FP64); FP64 is always kept for Simple() on host vectors.
+        void SetValueFormats(const int mask) { m_valueMask = mask | 1; }
+
+        private:
+        bool CheckRange(const fock_t minRow, const fock_t maxRow, const char* where, utilities::MpiWrapper& mpi)
+        {
+            if (!this->m_fockBasis.Device()) {
+                std::cerr << "\n\tERROR in " << where << ": no device Fock basis (GenerateFockSpace first)" << std::endl;
+                mpi.m_exitFlag = true;
+                return false;
+            }
+            if (minRow != 0 || maxRow < this->m_data.m_fockSpaceDim) {
+                std::cerr << "\n\tERROR in " << where << ": the device build covers all rows [0, "
+                          << this->m_data.m_fockSpaceDim << "); got [" << minRow << ", " << maxRow << ")" << std::endl;
+                mpi.m_exitFlag = true;
+                return false;
+            }
+            return true;
+        }
+
+        std::vector<double> m_quadratic;
+        bool m_haveQuadratic = false;
+        int m_valueMask = 1 | 2;
+    };
+
+    // MatrixVectorFunction<T,L1,L2> (src/hamiltonians/matrix_vector_routines.hpp:40-135)
+    template <typename T, class L1, class L2>
+    class MatrixVectorFunction
+    {
+        public:
+        MatrixVectorFunction(SpinlessFermionHamiltonian<T>& hamiltonian, const utilities::MpiWrapper&)
+            : m_hamiltonian(&hamiltonian), UPLO(hamiltonian.m_arpackWrapper.GetUpperOrLower()) {}
+
+        // matrix_vector_routines.hpp:94-104: y = H x on host vectors of length nodeDim
+        void Simple(T* x, T* y, const utilities::MpiWrapper& mpi)
+        {
+            if (!utilities::detail::Ok(hd_spmv_host(m_hamiltonian->Device(), x, y), "MatrixVectorFunction::Simple"))
+                const_cast<utilities::MpiWrapper&>(mpi).m_exitFlag = true;
+        }
+
+        // The same product on device vectors (asynchronous on the matrix's stream).
+        int SimpleDevice(const T* d_x, T* d_y, const int precision = HD_F64)
+        {
+            return hd_spmv(m_hamiltonian->Device(), precision, d_x, d_y);
+        }
+
+        private:
+        SpinlessFermionHamiltonian<T>* m_hamiltonian;
+        char UPLO;
+    };
+
+    // Term tables with the reference's accessors (term_tables_base.hpp:44-174;
+    // sphere term_tables.cpp:36-169; flux-lattice term_tables.cpp:36-198), for
+    // callers that have no table classes of their own.
+    template <typename T>
+    class TermTables
+    {
+        public:
+        virtual ~TermTables() = default;
+        iSize_t GetMaxKCount() const { return 1; }
+        iSize_t GetDimension() const { return (iSize_t)m_vTable.size(); }
+        std::vector<kState_t>* GetKTable() { return &m_kTable; }
+        std::vector<T>* GetVTable() { return &m_vTable; }
+        kState_t GetKMax() const { return m_kMax; }
+
+        protected:
+        std::vector<T> m_vTable;
+        std::vector<kState_t> m_kTable;
+        kState_t m_kMax = 0;
+    };
+
+    // Sphere QuarticTermTables: index (k4*kMax+k3)*kMax+k2
+    class SphereQuarticTermTables : public TermTables<double>
+    {
+        public:
+        void Initialize(const kState_t kMax, const utilities::MpiWrapper&)
+        {
+            m_kMax = kMax;
+            m_kTable.assign((size_t)kMax * kMax * kMax, 0);
+            m_vTable.assign((size_t)kMax * kMax * kMax, 0.0);
+        }
+        // BaseGenerate4KTable + BaseTermsFromPseudopotentials
+        // (pseudopotential_model_base.hpp:97-229) on the C ABI
+        bool FromPseudopotentials(const int maxLz, const std::vector<double>& pseudopotentials,
+                                  const utilities::MpiWrapper& mpi)
+        {
+            Initialize((kState_t)(maxLz + 1), mpi);
+            return utilities::detail::Ok(hd_sphere_ktable(maxLz, m_kTable.data()), "QuarticTermTables") &&
+                   utilities::detail::Ok(hd_sphere_vtable(maxLz, pseudopotentials.data(),
+                                                          (int32_t)pseudopotentials.size(), m_vTable.data()),
+                                         "QuarticTermTables");
+        }
+        void GetK1(kState_t* buf, iSize_t& n, const kState_t k2, const kState_t k3, const kState_t k4) const
+        {
+            buf[0] = m_kTable[((size_t)k4 * m_kMax + k3) * m_kMax + k2];
+            n = buf[0] >= m_kMax ? 0 : 1;
+        }
+        double GetVkkkk(const kState_t, const kState_t k2, const kState_t k3, const kState_t k4) const
+        {
+            return m_vTable[((size_t)k4 * m_kMax + k3) * m_kMax + k2];
+        }
+        void SetK1(const kState_t k1, const kState_t k2, const kState_t k3, const kState_t k4)
+        {
+            m_kTable[((size_t)k4 * m_kMax + k3) * m_kMax + k2] = k1;
+        }
+        void SetVkkkk(const double v, const kState_t, const kState_t k2, const kState_t k3, const kState_t k4)
+        {
+            m_vTable[((size_t)k4 * m_kMax + k3) * m_kMax + k2] = v;
+        }
+    };
+
+    // Flux-lattice QuarticTermTables: folded on k4 <= k3
+    class OflQuarticTermTables : public TermTables<dcmplx>
+    {
+        public:
+        void Initialize(const kState_t kMax, const utilities::MpiWrapper&)
+        {
+            m_kMax = kMax;
+            m_kTable.assign((size_t)(kMax + 1) * kMax * kMax / 2, 0);
+            m_vTable.assign((size_t)(kMax + 1) * kMax * kMax / 2, dcmplx(0.0, 0.0));
+        }
+        size_t Index(const kState_t k2, const kState_t k3, const kState_t k4, const kState_t k1) const
+        {
+            int t1 = k4, t2 = k3, t3 = k2;
+            if (k4 > k3) { t1 = k3; t2 = k4; t3 = k1; }
+            return ((size_t)t1 * m_kMax + t2 - (size_t)(t1 * (t1 + 1)) / 2) * m_kMax + t3;
+        }
+        void SetK1(const kState_t k1, const kState_t k2, const kState_t k3, const kState_t k4)
+        {
+            m_kTable[Index(k2, k3, k4, k1)] = k1;
+        }
+        void SetVkkkk(const dcmplx v, const kState_t k1, const kState_t k2, const kState_t k3, const kState_t k4)
+        {
+            m_vTable[Index(k2, k3, k4, k1)] = v;
+        }
+    };
+
+    // Quadratic tables: E_k on the diagonal (GetK1(k2) gives k1 = k2)
+    template <typename T>
+    class DiagonalQuadraticTermTables : public TermTables<T>
+    {
+        public:
+        void Initialize(const kState_t kMax, const utilities::MpiWrapper&)
+        {
+            this->m_kMax = kMax;
+            this->m_kTable.resize(kMax);
+            for (kState_t k = 0; k < kMax; ++k) this->m_kTable[k] = k;
+            this->m_vTable.assign(kMax, T(0.0));
+        }
+        void GetK1(kState_t* buf, iSize_t& n, const kState_t k2) const { buf[0] = k2; n = 1; }
+        T GetEkk(const kState_t k1, const kState_t) const { return this->m_vTable[k1]; }
+        void SetEkk(const T e, const kState_t k1) { this->m_vTable[k1] = e; }
+    };
+}
+
+#endif

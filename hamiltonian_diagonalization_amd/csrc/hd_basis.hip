@@ -370,12 +370,15 @@ int hd_basis_create(const hd_sector* sector, uint32_t N, uint32_t Norb, hd_basis
         return fail(HD_ERR_EMPTY, "FOCK BASIS IS ZERO DIMENSIONAL");
     }
     b->dim = (int64_t)b->rt.dim;
-    hipGetDevice(&b->device);
+    if (hipGetDevice(&b->device) != hipSuccess) {
+        delete b;
+        return fail(HD_ERR_HIP, "hipGetDevice failed");
+    }
     st = HD_OK;
     auto cleanup = [&]() {
         dfree(b->d_rank);
         dfree(b->d_states);
-        if (b->stream) hipStreamDestroy(b->stream);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
         delete b;
     };
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -420,7 +423,7 @@ int hd_basis_destroy(hd_basis* b)
     if (!b) return HD_OK;
     dfree(b->d_rank);
     dfree(b->d_states);
-    if (b->stream) hipStreamDestroy(b->stream);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return HD_OK;
 }

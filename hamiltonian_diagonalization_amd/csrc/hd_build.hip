@@ -567,21 +567,6 @@ static int set_lds(const void* fn, size_t lds)
     return HD_OK;
 }
 
-struct Events {
-    hipEvent_t e[2] = {nullptr, nullptr};
-    Events() { hipEventCreate(&e[0]); hipEventCreate(&e[1]); }
-    ~Events() { hipEventDestroy(e[0]); hipEventDestroy(e[1]); }
-    void start(hipStream_t s) { hipEventRecord(e[0], s); }
-    double stop(hipStream_t s)
-    {
-        hipEventRecord(e[1], s);
-        hipEventSynchronize(e[1]);
-        float ms = 0;
-        hipEventElapsedTime(&ms, e[0], e[1]);
-        return ms;
-    }
-};
-
 template <bool CPLX, bool UPPER, bool LZ>
 static int launch_fill(int grid, size_t lds, hipStream_t st, const hd_basis* b, const BuildArgs& A,
                        int rank_words, const int64_t* rowptr, const typename VT<CPLX>::T* diag,
@@ -683,7 +668,7 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
     HD_TRY(dalloc_t(&d_dkeep, (size_t)dim, "diagonal flags"));
     HD_TRY(dalloc_t(&h->d_rowptr, (size_t)dim + 1, "row pointers"));
     h->bytes += ((size_t)dim + 1) * 8;
-    Events ev;
+    EventTimer ev;
     {
         const size_t lds = (((kp_alloc / 4) + 3) & ~(size_t)3) * 4 + sizeof(T) * 2 * K * K +
                            (((size_t)K * K + 15) & ~(size_t)15) + sizeof(T) * K + K + 16;
@@ -804,6 +789,7 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
         h->t_sort = ev.stop(st);
     }
     HD_TRYHIP(hipStreamSynchronize(st));
+    HD_TRY(ev.status("build stage timing"));
     cleanup();
 #undef HD_TRY
 #undef HD_TRYHIP
@@ -867,8 +853,8 @@ int hd_ham_build(const hd_basis* b, const hd_tables* t, int32_t storage, double 
     h->dim = b->dim;
     h->storage = storage;
     h->is_complex = t->is_complex;
-    hipGetDevice(&h->device);
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipGetDevice(&h->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return fail(HD_ERR_HIP, "stream creation failed");
     }
@@ -895,7 +881,7 @@ int hd_ham_destroy(hd_hamiltonian* h)
     dfree(h->d_code);
     dfree(h->d_diag);
     dfree(h->d_dict);
-    if (h->stream) hipStreamDestroy(h->stream);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return HD_OK;
 }

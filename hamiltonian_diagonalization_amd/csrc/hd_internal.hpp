@@ -31,6 +31,66 @@ int fail(int code, const std::string& msg);
         if (_s != HD_OK) return _s;         \
     } while (0)
 
+// A pair of HIP events timing work on one stream.  The first HIP error of the
+// sequence is kept (later calls are skipped) and reported by status(); the
+// events are released by the destructor.
+struct EventTimer {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t err = hipSuccess;
+    EventTimer()
+    {
+        note(hipEventCreate(&e0));
+        note(hipEventCreate(&e1));
+    }
+    ~EventTimer()
+    {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+    EventTimer(const EventTimer&) = delete;
+    EventTimer& operator=(const EventTimer&) = delete;
+    void note(hipError_t e)
+    {
+        if (err == hipSuccess) err = e;
+    }
+    void start(hipStream_t s)
+    {
+        if (err == hipSuccess) note(hipEventRecord(e0, s));
+    }
+    // Waits for the work recorded since start(); 0 ms after an error.
+    float stop(hipStream_t s)
+    {
+        float ms = 0.f;
+        if (err == hipSuccess) note(hipEventRecord(e1, s));
+        if (err == hipSuccess) note(hipEventSynchronize(e1));
+        if (err == hipSuccess) note(hipEventElapsedTime(&ms, e0, e1));
+        return err == hipSuccess ? ms : 0.f;
+    }
+    int status(const char* what) const
+    {
+        return err == hipSuccess ? HD_OK : fail(HD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(err));
+    }
+};
+
+// The first failure of a run of enqueued HIP calls (copies, fills, launches)
+// whose status is checked once, after the run: HD_ENQ(q, hipMemcpyAsync(...)).
+struct HipErrors {
+    hipError_t err = hipSuccess;
+    const char* what = "";
+    void operator()(hipError_t e, const char* w)
+    {
+        if (err == hipSuccess && e != hipSuccess) {
+            err = e;
+            what = w;
+        }
+    }
+    int status() const
+    {
+        return err == hipSuccess ? HD_OK : fail(HD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(err));
+    }
+};
+#define HD_ENQ(q, expr) (q)((expr), #expr)
+
 // Device allocation that reports HD_ERR_NOMEM with the size.
 int dalloc(void** p, size_t bytes, const char* what);
 template <typename T>

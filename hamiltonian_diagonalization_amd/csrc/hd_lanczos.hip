@@ -7,9 +7,14 @@
 // Here the Krylov basis, the operator and all vector work stay in HBM; the host
 // only sees the small projected matrix.  Method: thick-restart Lanczos (Wu and
 // Simon, SIAM J. Matrix Anal. Appl. 22 (2000) 602) with full
-// reorthogonalisation by classical Gram-Schmidt applied twice (Giraud et al.,
-// Numer. Math. 101 (2005) 87), the projected tridiagonal-plus-arrow matrix
-// diagonalised by cyclic Jacobi rotations on the host.
+// reorthogonalisation: each step first removes the three-term part locally
+// (w -= beta v_{j-1}; alpha = v_j.w; w -= alpha v_j), then one classical
+// Gram-Schmidt pass against the whole basis, with a second pass when the norm
+// drops by more than 100x (the DGKS criterion); the first step after a restart
+// (whose projection is the arrow, not tridiagonal) always takes two passes
+// (Giraud et al., Numer. Math. 101 (2005) 87).  The projected
+// tridiagonal-plus-arrow matrix is diagonalised by cyclic Jacobi rotations on
+// the host.
 //
 // Mixed precision (precision = HD_F32): the operator is applied with FP32
 // values and FP32 vectors (x rounded from the FP64 Lanczos vector, y widened
@@ -395,6 +400,7 @@ struct Lanczos {
     double t_spmv = 0, t_ortho = 0;
     int mv32 = 0, mv64 = 0, second_passes = 0;
     int prec64 = HD_F64;  // the FP64 operator: FP64 values or the coded format
+    EventTimer timer;     // brackets every operator application on st
 
     ~Lanczos()
     {
@@ -426,10 +432,7 @@ struct Lanczos {
     // W = H V_j in the requested precision
     int apply(const T* x, T* y, int prec)
     {
-        hipEvent_t e0, e1;
-        hipEventCreate(&e0);
-        hipEventCreate(&e1);
-        hipEventRecord(e0, st);
+        timer.start(st);
         int s;
         if (prec == HD_F32) {
             hipLaunchKernelGGL(k_to_f32<C>, dim3(nb), dim3(kVB), 0, st, n, x, x32);
@@ -440,13 +443,8 @@ struct Lanczos {
             s = spmv_launch(h, prec, x, y, st);
             ++mv64;
         }
-        hipEventRecord(e1, st);
-        hipEventSynchronize(e1);
-        float ms = 0;
-        hipEventElapsedTime(&ms, e0, e1);
-        t_spmv += ms;
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        t_spmv += timer.stop(st);
+        if (s == HD_OK) s = timer.status("SpMV timing");
         return s;
     }
 

@@ -438,6 +438,7 @@ int hd_mlp_create(int32_t n_bits, int32_t n_params, uint64_t seed, hd_mlp** out)
     }
     std::mt19937_64 rng(seed);
     int st = HD_OK;
+    HipErrors q;
     for (int l = 0; l < 4 && st == HD_OK; ++l) {
         const int R = m->rows[l], C = m->cols[l];
         const size_t n = (size_t)R * C;
@@ -458,13 +459,15 @@ int hd_mlp_create(int32_t n_bits, int32_t n_params, uint64_t seed, hd_mlp** out)
         for (float** p : {&m->mb[l], &m->vb[l]})
             if ((st = dalloc_t(p, R, "Adam moments")) != HD_OK) break;
         if (st != HD_OK) break;
-        hipMemcpyAsync(m->w[l], w.data(), n * 4, hipMemcpyHostToDevice, m->stream);
-        hipMemcpyAsync(m->b[l], bz.data(), (size_t)R * 4, hipMemcpyHostToDevice, m->stream);
-        hipMemsetAsync(m->mw[l], 0, n * 4, m->stream);
-        hipMemsetAsync(m->vw[l], 0, n * 4, m->stream);
-        hipMemsetAsync(m->mb[l], 0, (size_t)R * 4, m->stream);
-        hipMemsetAsync(m->vb[l], 0, (size_t)R * 4, m->stream);
+        // synchronous copies: w and bz are released at the end of this iteration
+        HD_ENQ(q, hipMemcpy(m->w[l], w.data(), n * 4, hipMemcpyHostToDevice));
+        HD_ENQ(q, hipMemcpy(m->b[l], bz.data(), (size_t)R * 4, hipMemcpyHostToDevice));
+        HD_ENQ(q, hipMemsetAsync(m->mw[l], 0, n * 4, m->stream));
+        HD_ENQ(q, hipMemsetAsync(m->vw[l], 0, n * 4, m->stream));
+        HD_ENQ(q, hipMemsetAsync(m->mb[l], 0, (size_t)R * 4, m->stream));
+        HD_ENQ(q, hipMemsetAsync(m->vb[l], 0, (size_t)R * 4, m->stream));
     }
+    if (st == HD_OK) st = q.status();
     if (st == HD_OK) st = mlp_refresh_bf16(m);
     if (st == HD_OK && hipStreamSynchronize(m->stream) != hipSuccess) st = fail(HD_ERR_HIP, "MLP init failed");
     if (st != HD_OK) {
@@ -487,7 +490,7 @@ int hd_mlp_destroy(hd_mlp* m)
             dfree(p);
     }
     dfree(m->st_buf); dfree(m->pr_buf); dfree(m->tg_buf);
-    if (m->stream) hipStreamDestroy(m->stream);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return HD_OK;
 }
@@ -593,7 +596,11 @@ int hd_mlp_start_vector(hd_mlp* m, const hd_basis* basis, const float* h_params,
     HD_CHECK(mlp_work(m, batch, false));
     float* dpar = nullptr;
     HD_CHECK(dalloc_t(&dpar, (size_t)std::max(m->n_params, 1), "MLP parameters"));
-    hipMemcpyAsync(dpar, h_params, sizeof(float) * m->n_params, hipMemcpyHostToDevice, m->stream);
+    if (hipMemcpy(dpar, h_params, sizeof(float) * m->n_params, hipMemcpyHostToDevice) != hipSuccess) {
+        dfree(dpar);
+        return fail(HD_ERR_HIP, "start vector: parameter upload failed");
+    }
+    HipErrors q;
     // pass 1: network outputs for all states, kept as log-amplitudes in d_vec
     // (as float pairs would double the memory; store psi after a shift pass)
     float* allout = nullptr;
@@ -609,14 +616,15 @@ int hd_mlp_start_vector(hd_mlp* m, const hd_basis* basis, const float* h_params,
         hipLaunchKernelGGL(k_featurize, dim3((unsigned)((npad + 255) / 256)), dim3(256), 0, m->stream, n, npad,
                            basis->d_states + b0, dpar, 0, m->n_bits, m->n_params, m->h[0], (__bf16*)nullptr);
         mlp_forward_core(m, npad, false);
-        hipMemcpy2DAsync(allout + 2 * b0, 2 * sizeof(float), m->out, kOutPad * sizeof(float), 2 * sizeof(float), n,
-                         hipMemcpyDeviceToDevice, m->stream);
+        HD_ENQ(q, hipMemcpy2DAsync(allout + 2 * b0, 2 * sizeof(float), m->out, kOutPad * sizeof(float),
+                                   2 * sizeof(float), n, hipMemcpyDeviceToDevice, m->stream));
         const int nb = 256;
         hipLaunchKernelGGL(k_max_logamp, dim3(nb), dim3(256), 0, m->stream, n, m->out, m->partial);
         std::vector<float> part(nb);
         if (hipMemcpyAsync(part.data(), m->partial, sizeof(float) * nb, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
             hipStreamSynchronize(m->stream) != hipSuccess)
             st = fail(HD_ERR_HIP, "start vector pass failed");
+        if (st == HD_OK) st = q.status();
         for (float p : part) gmax = std::max(gmax, p);
     }
     if (st == HD_OK) {
@@ -624,13 +632,14 @@ int hd_mlp_start_vector(hd_mlp* m, const hd_basis* basis, const float* h_params,
         // by viewing allout as an n x 2 array
         for (int64_t b0 = 0; b0 < basis->dim; b0 += batch) {
             const int64_t n = std::min(batch, basis->dim - b0);
-            hipMemcpy2DAsync(m->out, kOutPad * sizeof(float), allout + 2 * b0, 2 * sizeof(float), 2 * sizeof(float),
-                             n, hipMemcpyDeviceToDevice, m->stream);
+            HD_ENQ(q, hipMemcpy2DAsync(m->out, kOutPad * sizeof(float), allout + 2 * b0, 2 * sizeof(float),
+                                       2 * sizeof(float), n, hipMemcpyDeviceToDevice, m->stream));
             hipLaunchKernelGGL(k_to_amplitude, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, n, m->out,
                                gmax, d_vec + b0);
         }
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(m->stream) != hipSuccess)
             st = fail(HD_ERR_HIP, "start vector assembly failed");
+        if (st == HD_OK) st = q.status();
     }
     dfree(allout);
     dfree(dpar);
@@ -649,8 +658,12 @@ int hd_mlp_gemm_check(int32_t M, int32_t N, int32_t K, const uint16_t* h_a, cons
     int st = dalloc_t(&b, (size_t)N * K, "gemm B");
     if (st == HD_OK) st = dalloc_t(&c, (size_t)M * N, "gemm C");
     if (st == HD_OK) {
-        hipMemcpy(a, h_a, (size_t)M * K * 2, hipMemcpyHostToDevice);
-        hipMemcpy(b, h_b, (size_t)N * K * 2, hipMemcpyHostToDevice);
+        HipErrors q;
+        HD_ENQ(q, hipMemcpy(a, h_a, (size_t)M * K * 2, hipMemcpyHostToDevice));
+        HD_ENQ(q, hipMemcpy(b, h_b, (size_t)N * K * 2, hipMemcpyHostToDevice));
+        st = q.status();
+    }
+    if (st == HD_OK) {
         GemmArgs g = ga(M, N, K, a, K, b, K);
         g.C = c; g.ldc = N;
         gemm(0, EPI_STORE, g);

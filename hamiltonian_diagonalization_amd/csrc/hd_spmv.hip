@@ -5,10 +5,12 @@
 // applying y = U x + (U - D)^H x (src/utilities/mathematics/sparse_linear_algebra.hpp:142-165,
 // :218-312).  The scattered y[col] += ... of the transposed half cannot run
 // race-free in parallel, so the device keeps both triangles and every row is a
-// pure gather: LPR lanes of a wavefront stream the row's column indices and
-// values with 16-byte non-temporal loads (4 entries per lane, row start
-// aligned down to a multiple of 4 and masked, arrays padded by 8 entries),
-// gather x, and reduce with cross-lane shuffles.
+// pure gather: LPR lanes of a wavefront (4..64, autotuned per matrix) stream
+// the row's column indices and values with scalar non-temporal loads (the
+// stream is read once and must not evict x from L2/MALL), gather x, and reduce
+// with cross-lane shuffles.  Measured on MI355X the kernel is bound by the x
+// gathers, not by the stream: 16-byte vector loads and persistent grids were
+// slower than this one-block-per-chunk form for FP64 and FP32 values.
 //
 // Coded format (HD_CODED): every off-diagonal value is sign * P[hop] from the
 // build's pair table, and the distinct P values are few (1,496 for the
@@ -59,41 +61,6 @@ HD_DEV float2 add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y)
 HD_DEV double negv(double a) { return -a; }
 HD_DEV double2 negv(double2 a) { return make_double2(-a.x, -a.y); }
 
-// 4 consecutive values starting at a 4-aligned index, non-temporal (clang
-// vector types: the non-temporal builtin does not take HIP's struct vectors).
-typedef float nt_f4 __attribute__((ext_vector_type(4)));
-typedef double nt_d2 __attribute__((ext_vector_type(2)));
-typedef int nt_i4 __attribute__((ext_vector_type(4)));
-typedef unsigned int nt_u2 __attribute__((ext_vector_type(2)));
-template <typename V> struct Quad { V v[4]; };
-template <typename V> HD_DEV Quad<V> load4(const V* p);
-template <> HD_DEV Quad<float> load4<float>(const float* p)
-{
-    const nt_f4 q = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
-    return {{q.x, q.y, q.z, q.w}};
-}
-template <> HD_DEV Quad<double> load4<double>(const double* p)
-{
-    const nt_d2 a = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
-    const nt_d2 b = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p) + 1);
-    return {{a.x, a.y, b.x, b.y}};
-}
-template <> HD_DEV Quad<float2> load4<float2>(const float2* p)
-{
-    const nt_f4 a = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
-    const nt_f4 b = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p) + 1);
-    return {{make_float2(a.x, a.y), make_float2(a.z, a.w), make_float2(b.x, b.y), make_float2(b.z, b.w)}};
-}
-template <> HD_DEV Quad<double2> load4<double2>(const double2* p)
-{
-    Quad<double2> q;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const nt_d2 a = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p) + k);
-        q.v[k] = make_double2(a.x, a.y);
-    }
-    return q;
-}
 // scalar non-temporal loads (complex values as two component loads)
 template <typename T> HD_DEV T ntload(const T* p) { return __builtin_nontemporal_load(p); }
 template <> HD_DEV double2 ntload<double2>(const double2* p)
@@ -107,26 +74,7 @@ template <> HD_DEV float2 ntload<float2>(const float2* p)
     return make_float2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
 }
 
-HD_DEV int4 load_col4(const int32_t* p)
-{
-    const nt_i4 c = __builtin_nontemporal_load(reinterpret_cast<const nt_i4*>(p));
-    return make_int4(c.x, c.y, c.z, c.w);
-}
-HD_DEV uint2 load_code4(const uint16_t* p)
-{
-    const nt_u2 c = __builtin_nontemporal_load(reinterpret_cast<const nt_u2*>(p));
-    return make_uint2(c.x, c.y);
-}
-
 constexpr int kSpmvBlock = 256;
-
-// Logical block index with the blocks of each XCD (dispatched round-robin,
-// block b on XCD b % 8) mapped to one contiguous range.
-HD_DEV int64_t xcd_block(int64_t b, int64_t nb)
-{
-    const int64_t x = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
-    return x * q + (x < r ? x : r) + i;
-}
 
 // Persistent grid: the blocks on XCD x (block b runs on XCD b % 8) walk the
 // chunks [x*nc/8, (x+1)*nc/8) round-robin, so each XCD's L2 serves one
@@ -310,34 +258,30 @@ int spmv_autotune(hd_hamiltonian* h)
         dfree(x);
         return st;
     }
-    hipMemsetAsync(x, 0, (size_t)h->dim * es, h->stream);
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    HipErrors q;
+    HD_ENQ(q, hipMemsetAsync(x, 0, (size_t)h->dim * es, h->stream));
+    EventTimer timer;
     const int cand[5] = {4, 8, 16, 32, 64};
     float best = 1e30f;
     int best_l = 64;
     for (int l : cand) {
+        if ((st = q.status()) != HD_OK) break;
         h->lanes_per_row = l;
-        spmv_launch(h, prec, x, y, h->stream);  // warm
-        hipEventRecord(e0, h->stream);
-        for (int r = 0; r < 3; ++r) spmv_launch(h, prec, x, y, h->stream);
-        hipEventRecord(e1, h->stream);
-        hipEventSynchronize(e1);
-        float ms = 0;
-        hipEventElapsedTime(&ms, e0, e1);
+        if ((st = spmv_launch(h, prec, x, y, h->stream)) != HD_OK) break;  // warm
+        timer.start(h->stream);
+        for (int r = 0; r < 3 && st == HD_OK; ++r) st = spmv_launch(h, prec, x, y, h->stream);
+        const float ms = timer.stop(h->stream);
+        if (st == HD_OK) st = timer.status("SpMV autotune timing");
+        if (st != HD_OK) break;
         if (ms < best) {
             best = ms;
             best_l = l;
         }
     }
     h->lanes_per_row = best_l;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     dfree(x);
     dfree(y);
-    HD_HIP(hipGetLastError());
-    return HD_OK;
+    return st;
 }
 
 }  // namespace hd
@@ -362,11 +306,15 @@ int hd_spmv_host(const hd_hamiltonian* h, const void* h_x, void* h_y)
     HD_CHECK(dalloc(&x, bytes, "Simple() x"));
     int st = dalloc(&y, bytes, "Simple() y");
     if (st == HD_OK) {
-        hipMemcpyAsync(x, h_x, bytes, hipMemcpyHostToDevice, h->stream);
-        st = spmv_launch(h, prec, x, y, h->stream);
+        // copies and product in the order of the handle's stream, then one wait
+        HipErrors q;
+        HD_ENQ(q, hipMemcpyAsync(x, h_x, bytes, hipMemcpyHostToDevice, h->stream));
+        st = q.status();
+        if (st == HD_OK) st = spmv_launch(h, prec, x, y, h->stream);
         if (st == HD_OK) {
-            hipMemcpyAsync(h_y, y, bytes, hipMemcpyDeviceToHost, h->stream);
-            if (hipStreamSynchronize(h->stream) != hipSuccess) st = fail(HD_ERR_HIP, "Simple() copy failed");
+            HD_ENQ(q, hipMemcpyAsync(h_y, y, bytes, hipMemcpyDeviceToHost, h->stream));
+            HD_ENQ(q, hipStreamSynchronize(h->stream));
+            st = q.status();
         }
     }
     dfree(x);
@@ -386,20 +334,14 @@ int hd_spmv_time(hd_hamiltonian* h, int32_t precision, int32_t iters, double* av
         dfree(x);
         return st;
     }
-    hipMemsetAsync(x, 0, (size_t)h->dim * es, h->stream);
-    st = spmv_launch(h, precision, x, y, h->stream);  // warm
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventRecord(e0, h->stream);
+    if (hipMemsetAsync(x, 0, (size_t)h->dim * es, h->stream) != hipSuccess) st = fail(HD_ERR_HIP, "timing fill failed");
+    if (st == HD_OK) st = spmv_launch(h, precision, x, y, h->stream);  // warm
+    EventTimer timer;
+    timer.start(h->stream);
     for (int i = 0; i < iters && st == HD_OK; ++i) st = spmv_launch(h, precision, x, y, h->stream);
-    hipEventRecord(e1, h->stream);
-    hipEventSynchronize(e1);
-    float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
+    const float ms = timer.stop(h->stream);
+    if (st == HD_OK) st = timer.status("SpMV timing");
     *avg_ms = ms / iters;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     dfree(x);
     dfree(y);
     return st;

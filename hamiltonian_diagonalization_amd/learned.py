@@ -141,14 +141,17 @@ def sphere_hamiltonian(basis: FockBasis, two_s: int, v135) -> Hamiltonian:
     return Hamiltonian(basis, models.sphere_tables(two_s, vm), values=(F64,))
 
 
-def ground_state_targets(psi: np.ndarray):
+def ground_state_targets(psi: np.ndarray, floor: float = -30.0):
     """(log|psi| - max log|psi|, sign) with the global sign fixed so that the
-    largest-magnitude component is positive."""
+    largest-magnitude component is positive.  Log-amplitudes are clamped at
+    `floor` (e^-30 ~ 1e-13 of the largest component, irrelevant to a start
+    vector) so that vanishing components do not dominate the squared error."""
     i = int(np.argmax(np.abs(psi)))
     if psi[i] < 0:
         psi = -psi
     a = np.log(np.maximum(np.abs(psi), 1e-300))
-    return (a - a[i]).astype(np.float32), np.where(psi >= 0, 1.0, -1.0).astype(np.float32)
+    a = np.maximum(a - a[i], floor)
+    return a.astype(np.float32), np.where(psi >= 0, 1.0, -1.0).astype(np.float32)
 
 
 def make_dataset(basis: FockBasis, two_s: int, points: np.ndarray, per_point: int, seed: int = 0,

@@ -1,0 +1,134 @@
+"""Train the amplitude network on plain-solver ground states and evaluate the
+learned start vector against the random start.
+
+python -m hamiltonian_diagonalization_amd.train_start --n 12 --two-s 33 \
+    --train-points 128 --eval-points 4 --out <checkpoint.npz>
+
+Training points (V1, V3, V5) are uniform in [0.5,1.5] x [0,0.5] x [0,0.2]
+(seed 0), visited in greedy nearest-neighbour order so each ground-state solve
+warm-starts from the previous point's vector; held-out points use seed 1.
+For each held-out point the 5 lowest eigenpairs are solved to residual 1e-10
+from (a) the random-normal start (seed 0), (b) the network's start vector and
+(c) the network's start vector plus a random admixture, with the FP64
+operator; (d) the network start with the FP32-value mixed solve.  Results go to
+<checkpoint>.json.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import CODED, F32, F64, DeviceBuffer, FockBasis, Hamiltonian, Sector
+from . import learned, models
+
+
+def log(*a):
+    print("[train_start]", *a, flush=True)
+
+
+def nn_order(points: np.ndarray) -> np.ndarray:
+    scale = np.array([1.0, 2.0, 5.0])  # ranges normalised to 1
+    left = list(range(len(points)))
+    order = [left.pop(0)]
+    while left:
+        p = points[order[-1]] * scale
+        d = [np.sum((points[i] * scale - p) ** 2) for i in left]
+        order.append(left.pop(int(np.argmin(d))))
+    return np.array(order)
+
+
+def hamiltonian(basis, two_s, v):
+    vm = models.coulomb_list(two_s, {1: float(v[0]), 3: float(v[1]), 5: float(v[2])})
+    return Hamiltonian(basis, models.sphere_tables(two_s, vm), values=(F64, F32))
+
+
+def evaluate(net, basis, two_s, v, nev, tol, mix=0.3):
+    h = hamiltonian(basis, two_s, v)
+    out = {"params": [float(x) for x in v]}
+    plain = h.lanczos(nev=nev, tol=tol, seed=0)
+    psi0 = h.eigenvector(0)
+    out["plain"] = {"matvecs": plain.matvecs, "ms": plain.time_ms, "evals": plain.evals.tolist(),
+                    "max_resid": float(plain.residuals.max())}
+    t0 = time.perf_counter()
+    start = net.start_vector(basis, v)
+    t_net = (time.perf_counter() - t0) * 1e3
+    vec = start.to_array(np.float64, basis.dim)
+    out["overlap"] = float(abs(vec @ psi0) / (np.linalg.norm(vec) * np.linalg.norm(psi0)))
+    out["net_eval_ms"] = t_net
+    # diagnostics: |psi0|^2-weighted sign agreement (up to the global sign) and
+    # the overlap of the magnitudes alone
+    w = psi0 ** 2 / np.sum(psi0 ** 2)
+    agree = float(np.sum(w[np.sign(vec) == np.sign(psi0)]))
+    out["sign_agreement"] = max(agree, 1.0 - agree)
+    out["magnitude_overlap"] = float(np.abs(vec) @ np.abs(psi0) / (np.linalg.norm(vec) * np.linalg.norm(psi0)))
+    runs = {"learned": (vec, F64), "learned_mix": (None, F64), "learned_f32": (vec, F32)}
+    rnd = np.random.default_rng(0).normal(size=basis.dim)
+    for name, (s, prec) in runs.items():
+        if s is None:
+            s = vec / np.linalg.norm(vec) + mix * rnd / np.linalg.norm(rnd)
+        r = h.lanczos(nev=nev, tol=tol, precision=prec, start=s)
+        out[name] = {"matvecs": r.matvecs, "ms": r.time_ms + t_net, "evals": r.evals.tolist(),
+                     "max_resid": float(r.residuals.max()),
+                     "max_abs_dE_vs_plain": float(np.max(np.abs(r.evals - plain.evals)))}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=12)
+    ap.add_argument("--two-s", type=int, default=33)
+    ap.add_argument("--train-points", type=int, default=128)
+    ap.add_argument("--per-point", type=int, default=32768)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--eval-points", type=int, default=4)
+    ap.add_argument("--eval-only", action="store_true")
+    ap.add_argument("--nev", type=int, default=5)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--out", required=True)
+    args = ap.parse_args()
+
+    basis = FockBasis(Sector.lz(0, args.two_s), args.n, args.two_s + 1)
+    log(f"N={args.n} 2S={args.two_s} L_z=0 dim={basis.dim}")
+    report = {"n": args.n, "two_s": args.two_s, "dim": basis.dim}
+    if args.eval_only:
+        net = learned.AmplitudeNet.load(args.out)
+    else:
+        pts = learned.sample_params(np.random.default_rng(0), args.train_points)
+        pts = pts[nn_order(pts)]
+        t0 = time.perf_counter()
+        data = learned.make_dataset(basis, args.two_s, pts, args.per_point, seed=0, tol=1e-8, log=log)
+        t_data = time.perf_counter() - t0
+        log(f"dataset: {len(data.states)} samples in {t_data:.1f} s")
+        net = learned.AmplitudeNet(args.two_s + 1, 3, seed=0)
+        t0 = time.perf_counter()
+        losses = learned.train(net, data, steps=args.steps, batch=args.batch, lr=args.lr, log=log,
+                               log_every=max(200, args.steps // 20))
+        t_train = time.perf_counter() - t0
+        log(f"trained {args.steps} steps in {t_train:.1f} s; loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+        net.save(args.out)
+        report.update({"train_points": args.train_points, "per_point": args.per_point, "steps": args.steps,
+                       "batch": args.batch, "dataset_s": t_data, "train_s": t_train,
+                       "loss_first": losses[0], "loss_last": losses[-1],
+                       "loss_curve": [float(x) for x in losses[:: max(1, len(losses) // 50)]]})
+    ev = learned.sample_params(np.random.default_rng(1), args.eval_points)
+    report["eval"] = []
+    for i, v in enumerate(ev):
+        r = evaluate(net, basis, args.two_s, v, args.nev, args.tol)
+        log(f"held-out {i}: overlap {r['overlap']:.4f} (signs {r['sign_agreement']:.4f}, magnitudes "
+            f"{r['magnitude_overlap']:.4f}); matvecs plain {r['plain']['matvecs']} "
+            f"learned {r['learned']['matvecs']} mix {r['learned_mix']['matvecs']} "
+            f"f32 {r['learned_f32']['matvecs']}; ms plain {r['plain']['ms']:.0f} learned "
+            f"{r['learned']['ms']:.0f} mix {r['learned_mix']['ms']:.0f} f32 {r['learned_f32']['ms']:.0f}")
+        report["eval"].append(r)
+    path = args.out[:-4] + ".json" if args.out.endswith(".npz") else args.out + ".json"
+    with open(path, "w") as f:
+        json.dump(report, f, indent=1)
+    log(f"wrote {path}")
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,177 @@
+"""The C++ drop-in classes (hamiltonian_diagonalization_amd/cpp/hd_dropin.hpp)
+driven like the reference's model callers, through the command-line driver
+hd_diagonalize: Fock basis, upper-triangle CRS and eigenvalues compared with
+the oracle (bit-exact for the basis and the CRS)."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import hamiltonian_diagonalization_amd as hd
+from hamiltonian_diagonalization_amd import models
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CPP = os.path.join(ROOT, "hamiltonian_diagonalization_amd", "cpp")
+DRIVER = os.path.join(ROOT, "hamiltonian_diagonalization_amd", "lib", "hd_diagonalize")
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def run(args, check=True):
+    if not os.path.exists(DRIVER):
+        pytest.fail("hd_diagonalize is not built (run __graft_entry__.build())")
+    p = subprocess.run([DRIVER] + [str(a) for a in args], capture_output=True, text=True, timeout=300)
+    if check and p.returncode != 0:
+        raise AssertionError(f"hd_diagonalize failed ({p.returncode}):\n{p.stdout}\n{p.stderr}")
+    return p
+
+
+def last_json(p):
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
+def pseudo_arg(vm):
+    return ",".join("%.17g" % v for v in vm)
+
+
+# ------------------------------------------------------------------ CPU checks
+
+def test_header_compiles_standalone(tmp_path):
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "hd_dropin.hpp"\n'
+                   "using namespace diagonalization;\n"
+                   "int main() {\n"
+                   "  utilities::MpiWrapper mpi;\n"
+                   "  SpinlessFermionHamiltonian<double> a; SpinlessFermionHamiltonian<dcmplx> b;\n"
+                   "  std::function<bool(fock_t)> f = AngularMomentumSector(0, 21);\n"
+                   "  return f(0) ? 1 : 0; }\n")
+    p = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                        "-I" + os.path.join(ROOT, "include"), "-I" + CPP, str(src)],
+                       capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
+
+
+def test_sector_functors_match_oracle_predicate(oracle, tmp_path):
+    """The descriptor functors evaluate the reference predicates: compared with the
+    oracle's basis (which filters by the restated predicates) over all states."""
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "hd_dropin.hpp"\n#include <cstdio>\n'
+                   "using namespace diagonalization;\n"
+                   "int main() {\n"
+                   "  AngularMomentumSector lz(2, 9); LinearMomentumSector2D k2(1, 2, 3, 4);\n"
+                   "  LinearMomentumSector1D k1(2, 4);\n"
+                   "  for (fock_t s = 0; s < (1u << 12); ++s) {\n"
+                   "    if (__builtin_popcountll(s) == 4 && s < (1u << 10) && lz(s)) std::printf(\"L %llu\\n\", (unsigned long long)s);\n"
+                   "    if (__builtin_popcountll(s) == 4 && k2(s)) std::printf(\"K %llu\\n\", (unsigned long long)s);\n"
+                   "    if (__builtin_popcountll(s) == 4 && k1(s)) std::printf(\"O %llu\\n\", (unsigned long long)s);\n"
+                   "  }\n  return 0; }\n")
+    exe = tmp_path / "t"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"), "-I" + CPP, str(src),
+                    "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {t: np.array([int(l.split()[1]) for l in out if l.startswith(t + " ")], dtype=np.uint64)
+           for t in "LKO"}
+    assert np.array_equal(got["L"], oracle.basis(oracle.sector(oracle.SECTOR_LZ, 2, 9), 4, 10))
+    assert np.array_equal(got["K"], oracle.basis(oracle.sector(oracle.SECTOR_K2D, 1, 2, 3, 4), 4, 12))
+    assert np.array_equal(got["O"], oracle.basis(oracle.sector(oracle.SECTOR_K1D, 2, 4), 4, 12))
+
+
+def test_driver_usage_and_empty_sector():
+    p = run([], check=False)
+    assert p.returncode == 2
+    # opposite-parity sector: GenerateFockSpace reports an empty basis (no device work)
+    p = run(["--nbr-particles", 3, "--nbr-orbitals", 6, "--lz", 0, "--pseudopotentials", "0,1"])
+    assert last_json(p) == {"dim": 0, "empty": True}
+
+
+# ------------------------------------------------------------------ GPU parity
+
+@pytest.mark.gpu
+def test_dropin_sphere_config1(oracle, tmp_path):
+    say("drop-in sphere N=8 2S=21 lz=0 Coulomb: basis, upper CRS, 5 eigenvalues")
+    vm = models.coulomb_list(21)
+    pre = str(tmp_path / "crs")
+    p = run(["--nbr-particles", 8, "--nbr-orbitals", 22, "--lz", 0, "--pseudopotentials", pseudo_arg(vm),
+             "--nbr-eigenvalues", 5, "--tol", 1e-12, "--dump-crs", pre, "--dump-basis", str(tmp_path / "basis")])
+    r = last_json(p)
+    say(r)
+    bas = np.fromfile(tmp_path / "basis", dtype=np.uint64)
+    want_b = oracle.basis(oracle.sector(oracle.SECTOR_LZ, 0, 21), 8, 22)
+    assert r["dim"] == 8512 and np.array_equal(bas, want_b)
+    kt = hd.sphere_ktable(21)
+    vt = hd.sphere_vtable(21, vm)
+    up = oracle.build(8, 22, want_b, oracle.LAYOUT_SPHERE, 22, kt, vt, full=False)
+    assert np.array_equal(np.fromfile(pre + ".rowstarts", dtype=np.int32), up.rowptr.astype(np.int32))
+    assert np.array_equal(np.fromfile(pre + ".cols", dtype=np.int32), up.col)
+    assert np.array_equal(np.fromfile(pre + ".values", dtype=np.float64), up.values())
+    assert r["nnz_upper"] == len(up.col)
+    full = oracle.build(8, 22, want_b, oracle.LAYOUT_SPHERE, 22, kt, vt, full=True)
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    M = sp.csr_matrix((full.values(), full.col, full.rowptr), shape=(8512, 8512))
+    want = np.sort(sla.eigsh(M, k=5, which="SA", tol=1e-14, ncv=60)[0])
+    got = np.array(r["eigenvalues"])
+    assert r["ok"] and r["method"] == "lanczos"
+    assert np.max(np.abs(got - want) / np.maximum(1, np.abs(want))) < 1e-10
+    assert max(r["residuals"]) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_dropin_initial_vector_file(tmp_path):
+    """--final-vector writes the ground state in the raw T[nodeDim] format; reading it
+    back with --initial-vector (the reference's --use-initial) starts from it."""
+    say("drop-in initial/final vector files N=6 2S=15")
+    vm = models.coulomb_list(15)
+    base = ["--nbr-particles", 6, "--nbr-orbitals", 16, "--lz", 0, "--pseudopotentials", pseudo_arg(vm),
+            "--nbr-eigenvalues", 1, "--tol", 1e-11]
+    f = str(tmp_path / "gs.bin")
+    r0 = last_json(run(base + ["--final-vector", f]))
+    assert os.path.getsize(f) == 8 * r0["dim"]
+    r1 = last_json(run(base + ["--initial-vector", f]))
+    say(r0["matvecs"], "->", r1["matvecs"])
+    assert abs(r1["eigenvalues"][0] - r0["eigenvalues"][0]) < 1e-10 * max(1, abs(r0["eigenvalues"][0]))
+    assert r1["matvecs"] < r0["matvecs"]
+    # a missing file falls back to the random start, as in the reference
+    r2 = last_json(run(base + ["--initial-vector", str(tmp_path / "missing.bin")]))
+    assert r2["matvecs"] == r0["matvecs"]
+
+
+@pytest.mark.gpu
+def test_dropin_dense_path_small_sector(oracle):
+    """dim < 60 goes through FullDiagonalize, as BaseDiagonalize does."""
+    say("drop-in dense path N=3 2S=8 lz=4")
+    vm = models.coulomb_list(8)
+    r = last_json(run(["--nbr-particles", 3, "--nbr-orbitals", 9, "--lz", 4, "--pseudopotentials", pseudo_arg(vm),
+                       "--nbr-eigenvalues", 4]))
+    assert r["method"] == "full" and r["dim"] < 60
+    bas = oracle.basis(oracle.sector(oracle.SECTOR_LZ, 4, 8), 3, 9)
+    up = oracle.build(3, 9, bas, oracle.LAYOUT_SPHERE, 9, hd.sphere_ktable(8), hd.sphere_vtable(8, vm))
+    want = np.linalg.eigvalsh(up.to_dense(len(bas)))[:4]
+    assert np.max(np.abs(np.array(r["eigenvalues"]) - want)) < 1e-11
+
+
+@pytest.mark.gpu
+def test_dropin_flux_lattice_complex_tables(oracle, tmp_path):
+    say("drop-in flux lattice 3x4 N=5 complex tables from files")
+    kt, vt, eps = models.ofl_tables(3, 4, 11)
+    kt.tofile(tmp_path / "k.bin")
+    vt.astype(np.complex128).tofile(tmp_path / "v.bin")
+    eps.astype(np.complex128).tofile(tmp_path / "e.bin")
+    pre = str(tmp_path / "crs")
+    r = last_json(run(["--nbr-particles", 5, "--kmax", 12, "--complex", 1, "--ktable", tmp_path / "k.bin",
+                       "--vtable", tmp_path / "v.bin", "--eps", tmp_path / "e.bin", "--sector", "k2d:0,0,3,4",
+                       "--nbr-eigenvalues", 6, "--tol", 1e-11, "--dense-below", 60, "--dump-crs", pre]))
+    say(r)
+    bas = oracle.basis(oracle.sector(oracle.SECTOR_K2D, 0, 0, 3, 4), 5, 12)
+    up = oracle.build(5, 12, bas, oracle.LAYOUT_OFL, 12, kt, vt, eps=eps, is_complex=True, full=False)
+    assert np.array_equal(np.fromfile(pre + ".rowstarts", dtype=np.int32), up.rowptr.astype(np.int32))
+    assert np.array_equal(np.fromfile(pre + ".cols", dtype=np.int32), up.col)
+    assert np.array_equal(np.fromfile(pre + ".values", dtype=np.complex128), up.values())
+    want = np.linalg.eigvalsh(up.to_dense(len(bas)))[:6]
+    assert r["ok"] and np.max(np.abs(np.array(r["eigenvalues"]) - want) / np.maximum(1, np.abs(want))) < 1e-10
