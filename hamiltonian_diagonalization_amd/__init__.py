@@ -74,6 +74,19 @@ class _LanczosStats(ctypes.Structure):
                 ("time_spmv_ms", ctypes.c_double), ("time_ortho_ms", ctypes.c_double)]
 
 
+class _BatchOpts(ctypes.Structure):
+    _fields_ = [("nev", ctypes.c_int32), ("ncv", ctypes.c_int32), ("max_restarts", ctypes.c_int32),
+                ("tol", ctypes.c_double), ("seed", ctypes.c_uint64), ("dense_below", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32)]
+
+
+class _BatchStats(ctypes.Structure):
+    _fields_ = [("problems", ctypes.c_int32), ("lanczos_problems", ctypes.c_int32),
+                ("dense_problems", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("restarts", ctypes.c_int32), ("steps", ctypes.c_int32),
+                ("max_residual", ctypes.c_double), ("time_total_ms", ctypes.c_double)]
+
+
 _lib = None
 
 # (name, restype, argtypes) of every C entry point the package binds
@@ -109,6 +122,7 @@ SIGNATURES = {
     "hd_spmv": (ctypes.c_int, [_P, _I32, _P, _P]),
     "hd_spmv_host": (ctypes.c_int, [_P, _P, _P]),
     "hd_spmv_time": (ctypes.c_int, [_P, _I32, _I32, ctypes.POINTER(_D)]),
+    "hd_stream_copy": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _I32, ctypes.POINTER(_D)]),
     "hd_spmv_set_lanes": (ctypes.c_int, [_P, _I32]),
     "hd_spmv_get_lanes": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
     "hd_lanczos": (ctypes.c_int, [_P, ctypes.POINTER(_LanczosOpts), _P,
@@ -127,6 +141,13 @@ SIGNATURES = {
                                          ctypes.POINTER(ctypes.c_float)]),
     "hd_mlp_start_vector": (ctypes.c_int, [_P, _P, _P, _P]),
     "hd_mlp_gemm_check": (ctypes.c_int, [_I32, _I32, _I32, _P, _P, _P]),
+    "hd_batch_create": (ctypes.c_int, [_P, _I32, _P, _I32, _D, _P]),
+    "hd_batch_destroy": (ctypes.c_int, [_P]),
+    "hd_batch_info": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "hd_batch_copy_pattern": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "hd_batch_spmm": (ctypes.c_int, [_P, _P, _P]),
+    "hd_batch_spmm_time": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_D)]),
+    "hd_batch_solve": (ctypes.c_int, [_P, ctypes.POINTER(_BatchOpts), _P, _P, ctypes.POINTER(_BatchStats)]),
 }
 
 
@@ -460,3 +481,65 @@ def header_functions():
     txt = open(path).read()
     return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:int|char\s*\*|const char\s*\*)\s*\**\s*(hd_\w+)\s*\(",
                                  txt, re.M)))
+
+
+class BatchProblem:
+    """Sectors x parameter points solved together (hd_batch_*): one coded CRS
+    pattern per sector shared by all points, point-fastest vectors."""
+
+    def __init__(self, bases: Sequence[FockBasis], tables: Sequence[TermTables], zero_tol: float = ZERO_TOL):
+        self.bases, self.tables = list(bases), list(tables)
+        bp = (ctypes.c_void_p * len(self.bases))(*[b.h.value for b in self.bases])
+        tp = (ctypes.c_void_p * len(self.tables))(*[t.h.value for t in self.tables])
+        h = ctypes.c_void_p()
+        _check(lib().hd_batch_create(bp, len(self.bases), tp, len(self.tables), zero_tol, ctypes.byref(h)))
+        self.h = h
+        r, n, nd, pp = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().hd_batch_info(h, ctypes.byref(r), ctypes.byref(n), ctypes.byref(nd), ctypes.byref(pp)))
+        self.rows, self.nnz, self.ndict, self.points_padded = r.value, n.value, nd.value, pp.value
+        self.points = len(self.tables)
+        self.row0 = np.cumsum([0] + [b.dim for b in self.bases])[:-1]
+
+    def pattern(self):
+        """(rowptr, col, code, dict[ndict][Pp], diag[rows][Pp]) host copies."""
+        rp = np.zeros(self.rows + 1, dtype=np.int64)
+        col = np.zeros(self.nnz, dtype=np.int32)
+        code = np.zeros(self.nnz, dtype=np.uint16)
+        dic = np.zeros((max(self.ndict, 1), self.points_padded), dtype=np.float64)
+        dg = np.zeros((self.rows, self.points_padded), dtype=np.float64)
+        _check(lib().hd_batch_copy_pattern(self.h, _ptr(rp), _ptr(col), _ptr(code), _ptr(dic), _ptr(dg)))
+        return rp, col, code, dic[:self.ndict], dg
+
+    def spmm(self, x: np.ndarray) -> np.ndarray:
+        """Y[:, p] = H_p X[:, p] for host X of shape (rows, points)."""
+        xp = np.zeros((self.rows, self.points_padded), dtype=np.float64)
+        xp[:, :self.points] = x
+        dx, dy = DeviceBuffer.from_array(xp), DeviceBuffer(xp.nbytes)
+        _check(lib().hd_batch_spmm(self.h, dx.ptr, dy.ptr))
+        return dy.to_array(np.float64, xp.size).reshape(xp.shape)[:, :self.points]
+
+    def spmm_time_ms(self, iters: int = 5) -> float:
+        v = ctypes.c_double()
+        _check(lib().hd_batch_spmm_time(self.h, iters, ctypes.byref(v)))
+        return v.value
+
+    def solve(self, nev=3, tol=1e-10, ncv=0, seed=0, dense_below=0, use_graph=True, max_restarts=0,
+              raise_on_fail=True):
+        """(evals[sector][point][nev], residuals, stats dict)."""
+        o = _BatchOpts()
+        o.nev, o.ncv, o.max_restarts, o.tol, o.seed = nev, ncv, max_restarts, tol, seed
+        o.dense_below, o.use_graph = dense_below, 1 if use_graph else 0
+        S, P = len(self.bases), self.points
+        ev = np.zeros((S, P, nev), dtype=np.float64)
+        rs = np.zeros((S, P, nev), dtype=np.float64)
+        st = _BatchStats()
+        code = lib().hd_batch_solve(self.h, ctypes.byref(o), _ptr(ev), _ptr(rs), ctypes.byref(st))
+        stats = {f: getattr(st, f) for f, _ in _BatchStats._fields_}
+        if code != HD_OK and (raise_on_fail or code != HD_ERR_NOT_CONVERGED):
+            _check(code)
+        return ev, rs, stats
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.hd_batch_destroy(self.h)
+            self.h = None

@@ -1,0 +1,1212 @@
+// hd_batch.hip -- many symmetry sectors x many parameter points of one model,
+// diagonalised together by one batched thick-restart Lanczos.
+//
+// Reference: the sphere program loops over the L_z sectors, building and
+// diagonalising each one separately (src/fqhe_sphere/main/
+// fqhe_sphere_spinless_fermions_diagonalization.cpp:255-291 with
+// pseudopotential_model_base.hpp:235-285), and a parameter sweep reruns the
+// whole program per point.  Here every (sector, point) pair is one problem of
+// a single batch:
+//
+//  - The CRS pattern of a sector does not depend on the pseudopotentials, only
+//    the hop values do.  Each sector is built once in the coded format whose
+//    16-bit codes index the tuple of a hop's values over all points
+//    (build_batch_sector), and the sectors are concatenated into one
+//    block-diagonal CRS.  Per point there is a dictionary column and a
+//    diagonal, both FP64 and bit-identical to the single-point build.
+//  - Vectors are stored point-fastest, X[row][p] (Pp = points padded to 16,
+//    32 or 64), so the SpMM gathers PT consecutive doubles of x per entry: one
+//    cache line serves PT problems.  The dictionary of the block's PT points
+//    sits in LDS.
+//  - Every Lanczos vector operation is one launch over all problems; inner
+//    products reduce per 1024-row chunk (chunks never straddle sectors) and
+//    then per sector in a fixed order, so results are deterministic.
+//  - The recurrence coefficients stay on the device.  The projected matrices
+//    of all problems are diagonalised on the device (one wavefront per
+//    problem, cyclic Jacobi in LDS), which also decides convergence and forms
+//    the restart rotation.  A restart cycle is therefore a fixed sequence of
+//    launches; from the second cycle on it is captured once as a hipGraph and
+//    replayed, and the host reads one counter per cycle.
+//  - Sectors below the dense cutoff (the reference's 60) are diagonalised
+//    densely by the same Jacobi kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "hd_internal.hpp"
+
+struct hd_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int S = 0, P = 0, Pp = 0, K = 0, N = 0;
+    std::vector<int64_t> dim, row0;  // per sector
+    int64_t R = 0, nnz = 0;
+    int64_t* d_rowptr = nullptr;
+    int32_t* d_col = nullptr;
+    uint16_t* d_code = nullptr;
+    double* d_dict = nullptr;  // [ndict][Pp]
+    int ndict = 0;
+    double* d_diag = nullptr;  // [R][Pp]
+    int spmm_pt = 16;          // points per SpMM tile (autotuned)
+    int spmm_grid = 0;         // blocks per tile (autotuned)
+    size_t bytes = 0;
+};
+
+namespace hd {
+namespace {
+
+constexpr int kChunk = 1024;  // rows per reduction chunk
+constexpr int kVB = 256;      // threads of the chunked vector kernels
+constexpr int kMaxM = 64;     // Krylov basis limit: one lane per projected row
+
+// ------------------------------------------------------------------ build
+
+__global__ void k_rowptr_offset(const int64_t* __restrict__ src, int64_t n, int64_t add, int64_t* __restrict__ dst)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] + add;
+}
+
+__global__ void k_col_offset(const int32_t* __restrict__ src, int64_t n, int32_t add, int32_t* __restrict__ dst)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i] + add;
+}
+
+// Diagonal of every point in the reference's order (quadratic E_k first, then
+// the quartic (k4, k3) terms; hd_build.hip k_count), trimmed below tol.
+// dterm/eps hold 0.0 where the reference skips a term, so adding them is
+// bit-identical to skipping.  One thread per (row, point).
+__global__ void k_batch_diag(const uint64_t* __restrict__ states, int64_t dim, int K, int P, int Pp,
+                             const double* __restrict__ dterm, const double* __restrict__ eps, double tol,
+                             double* __restrict__ diag)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= dim * Pp) return;
+    const int p = (int)(i % Pp);
+    const int64_t row = i / Pp;
+    if (p >= P) {
+        diag[i] = 0.0;
+        return;
+    }
+    const uint64_t s = states[row];
+    double sum = 0.0;
+    if (eps)
+        for (uint64_t t = s; t; t &= t - 1) sum += eps[(size_t)__builtin_ctzll(t) * P + p];
+    for (uint64_t t4 = s; t4; t4 &= t4 - 1) {
+        const int k4 = __builtin_ctzll(t4);
+        for (uint64_t t3 = s ^ (1ull << k4); t3; t3 &= t3 - 1) {
+            const int k3 = __builtin_ctzll(t3);
+            const size_t idx = ((size_t)k4 * K + k3) * 2;
+            sum += dterm[idx * P + p];
+            sum += dterm[(idx + 1) * P + p];
+        }
+    }
+    diag[i] = fabs(sum) < tol ? 0.0 : sum;
+}
+
+// ------------------------------------------------------------------ SpMM
+
+// Y[r][p] = diag[r][p] X[r][p] + sum_e sign_e dict[idx_e][p] X[col_e][p] for the
+// PT points of tile blockIdx.y.  A wavefront holds 64/PT row slots of PT lanes;
+// the row's columns and codes are uniform within a slot.
+template <int PT>
+__global__ void __launch_bounds__(1024)
+k_bspmm(int64_t R, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+        const uint16_t* __restrict__ code, const double* __restrict__ dict, int ndict,
+        const double* __restrict__ diag, int Pp, const double* __restrict__ x, double* __restrict__ y)
+{
+    extern __shared__ double s_dict[];
+    const int p0 = blockIdx.y * PT;
+    for (int i = threadIdx.x; i < ndict * PT; i += blockDim.x)
+        s_dict[i] = dict[(size_t)(i / PT) * Pp + p0 + (i % PT)];
+    __syncthreads();
+    constexpr int RS = 64 / PT;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = lane / PT, pl = lane % PT;
+    const int64_t rows_per_iter = (int64_t)(blockDim.x >> 6) * RS;
+    const double* xp = x + p0 + pl;
+    for (int64_t r = (int64_t)blockIdx.x * rows_per_iter + wave * RS + slot; r < R;
+         r += (int64_t)gridDim.x * rows_per_iter) {
+        const int64_t e0 = rowptr[r], e1 = rowptr[r + 1];
+        double acc = 0.0;
+        int64_t e = e0;
+        for (; e + 4 <= e1; e += 4) {
+            int c[4];
+            unsigned cd[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                c[u] = col[e + u];
+                cd[u] = code[e + u];
+            }
+            double xv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xv[u] = xp[(int64_t)c[u] * Pp];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool dg = cd[u] == 0xFFFFu;  // the diagonal is added below
+                double v = s_dict[(dg ? 0u : (cd[u] & 0x7FFFu)) * PT + pl];
+                v = dg ? 0.0 : ((cd[u] & 0x8000u) ? -v : v);
+                acc = fma(v, xv[u], acc);
+            }
+        }
+        for (; e < e1; ++e) {
+            const unsigned cd = code[e];
+            const double xv = xp[(int64_t)col[e] * Pp];
+            const bool dg = cd == 0xFFFFu;
+            double v = s_dict[(dg ? 0u : (cd & 0x7FFFu)) * PT + pl];
+            v = dg ? 0.0 : ((cd & 0x8000u) ? -v : v);
+            acc = fma(v, xv, acc);
+        }
+        const size_t o = (size_t)r * Pp + p0 + pl;
+        y[o] = fma(diag[o], x[o], acc);
+    }
+}
+
+// ------------------------------------------------------ chunked vector work
+//
+// Block c works on rows [ck_r0[c], ck_r1[c]) of sector ck_sec[c]; thread t on
+// point p = t % Pp and rows r0 + t / Pp, stepping by kVB / Pp.  Coefficient
+// arrays are addressed coef[s*cs + i*ci + p].
+
+struct Chunks {
+    const int64_t* r0;
+    const int64_t* r1;
+    const int32_t* sec;
+};
+
+// partial[(c*PS + i)*Pp + p] = sum_rows V_i w for i < nv, and |w|^2 at i = nv
+// when want_norm.  Skipped entirely when *any == 0.
+__global__ void __launch_bounds__(kVB)
+k_bdots(int nv, const double* __restrict__ V0, int64_t vstride, const double* __restrict__ w, int want_norm,
+        Chunks ck, int Pp, int PS, double* __restrict__ partial, const int* __restrict__ any)
+{
+    if (any && *any == 0) return;
+    __shared__ double red[kVB];
+    const int c = blockIdx.x;
+    const int64_t r0 = ck.r0[c], r1 = ck.r1[c];
+    const int p = threadIdx.x % Pp, rl = threadIdx.x / Pp, RL = kVB / Pp;
+    const int nout = nv + want_norm;
+    for (int i0 = 0; i0 < nout; i0 += 8) {
+        const int cnt = min(8, nout - i0);
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int64_t r = r0 + rl; r < r1; r += RL) {
+            const size_t o = (size_t)r * Pp + p;
+            const double wv = w[o];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (u < cnt) {
+                    const int i = i0 + u;
+                    const double a = i < nv ? V0[(size_t)i * vstride + o] : wv;
+                    acc[u] = fma(a, wv, acc[u]);
+                }
+        }
+        for (int u = 0; u < cnt; ++u) {
+            red[threadIdx.x] = acc[u];
+            __syncthreads();
+            if (rl == 0) {
+                double s = 0.0;
+                for (int q = 0; q < RL; ++q) s += red[q * Pp + p];
+                partial[((size_t)c * PS + i0 + u) * Pp + p] = s;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// out[(s*PS + i)*Pp + p] = sum over the sector's chunks, in chunk order.
+__global__ void k_breduce(int nout, const double* __restrict__ partial, int PS, const int32_t* __restrict__ lsec,
+                          const int32_t* __restrict__ sec_c0, const int32_t* __restrict__ sec_c1, int Pp,
+                          double* __restrict__ out, const int* __restrict__ any)
+{
+    if (any && *any == 0) return;
+    const int s = lsec[blockIdx.x];
+    const int i = blockIdx.y;
+    const int p = threadIdx.x;
+    if (i >= nout || p >= Pp) return;
+    double sum = 0.0;
+    for (int c = sec_c0[s]; c < sec_c1[s]; ++c) sum += partial[((size_t)c * PS + i) * Pp + p];
+    out[((size_t)s * PS + i) * Pp + p] = sum;
+}
+
+// w -= sum_{i<nv} coef[s*cs + i*ci + p] * (mask ? mask[s*Pp+p] : 1) * V_i
+__global__ void __launch_bounds__(kVB)
+k_bupdate(int nv, const double* __restrict__ V0, int64_t vstride, const double* __restrict__ coef, int64_t cs,
+          int64_t ci, const double* __restrict__ mask, Chunks ck, int Pp, double* __restrict__ w,
+          const int* __restrict__ any)
+{
+    if (any && *any == 0) return;
+    const int c = blockIdx.x;
+    const int s = ck.sec[c];
+    const int64_t r0 = ck.r0[c], r1 = ck.r1[c];
+    const int p = threadIdx.x % Pp, rl = threadIdx.x / Pp, RL = kVB / Pp;
+    const double m = mask ? mask[(size_t)s * Pp + p] : 1.0;
+    for (int i0 = 0; i0 < nv; i0 += 8) {
+        const int cnt = min(8, nv - i0);
+        double cf[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) cf[u] = u < cnt ? m * coef[(size_t)s * cs + (size_t)(i0 + u) * ci + p] : 0.0;
+        for (int64_t r = r0 + rl; r < r1; r += RL) {
+            const size_t o = (size_t)r * Pp + p;
+            double v = w[o];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (u < cnt) v = fma(-cf[u], V0[(size_t)(i0 + u) * vstride + o], v);
+            w[o] = v;
+        }
+    }
+}
+
+// dst = src * scale[s*Pp + p]
+__global__ void __launch_bounds__(kVB)
+k_bscale(const double* __restrict__ src, const double* __restrict__ scale, Chunks ck, int Pp, double* __restrict__ dst)
+{
+    const int c = blockIdx.x;
+    const int s = ck.sec[c];
+    const int64_t r0 = ck.r0[c], r1 = ck.r1[c];
+    const int p = threadIdx.x % Pp, rl = threadIdx.x / Pp, RL = kVB / Pp;
+    const double f = scale[(size_t)s * Pp + p];
+    for (int64_t r = r0 + rl; r < r1; r += RL) dst[(size_t)r * Pp + p] = src[(size_t)r * Pp + p] * f;
+}
+
+// Normal(0,1) start entries from a counter hash (splitmix64 + Box-Muller);
+// points beyond P get zero vectors.
+HD_DEV uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(kVB)
+k_brandn(uint64_t seed, Chunks ck, int Pp, int P, double* __restrict__ v)
+{
+    const int c = blockIdx.x;
+    const int64_t r0 = ck.r0[c], r1 = ck.r1[c];
+    const int p = threadIdx.x % Pp, rl = threadIdx.x / Pp, RL = kVB / Pp;
+    for (int64_t r = r0 + rl; r < r1; r += RL) {
+        double z = 0.0;
+        if (p < P) {
+            const uint64_t h = mix64(seed ^ mix64(((uint64_t)r << 8) | (uint64_t)p));
+            const double u1 = ((h >> 11) + 1) * (1.0 / 9007199254740993.0);  // (0, 1]
+            const double u2 = (mix64(h) >> 11) * (1.0 / 9007199254740992.0);
+            z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+        }
+        v[(size_t)r * Pp + p] = z;
+    }
+}
+
+// In-place restart rotation: V_i <- sum_{l<m} V_l Y[l][i] for i < keep, then
+// V_keep <- V_m.  Y is [(l*keep + i)*S + s]*Pp + p.
+__global__ void __launch_bounds__(kVB)
+k_bcombine(int m, int keep, double* __restrict__ V0, int64_t vstride, const double* __restrict__ Y, int S, Chunks ck,
+           int Pp)
+{
+    const int c = blockIdx.x;
+    const int s = ck.sec[c];
+    const int64_t r0 = ck.r0[c], r1 = ck.r1[c];
+    const int p = threadIdx.x % Pp, rl = threadIdx.x / Pp, RL = kVB / Pp;
+    for (int64_t r = r0 + rl; r < r1; r += RL) {
+        const size_t o = (size_t)r * Pp + p;
+        double v[kMaxM];
+#pragma unroll
+        for (int l = 0; l < kMaxM; ++l) v[l] = l < m ? V0[(size_t)l * vstride + o] : 0.0;
+        for (int i = 0; i < keep; ++i) {
+            double a = 0.0;
+#pragma unroll
+            for (int l = 0; l < kMaxM; ++l)
+                if (l < m) a = fma(v[l], Y[(((size_t)l * keep + i) * S + s) * Pp + p], a);
+            V0[(size_t)i * vstride + o] = a;
+        }
+        V0[(size_t)keep * vstride + o] = V0[(size_t)m * vstride + o];
+    }
+}
+
+// ||H x - theta x||^2 partials for residual checks: partial[c*Pp + p], with
+// theta of problem (s, p) at theta[s*ts + p*tp]
+__global__ void __launch_bounds__(kVB)
+k_bresid(const double* __restrict__ hx, const double* __restrict__ x, const double* __restrict__ theta, int64_t ts,
+         int64_t tp, Chunks ck, int Pp, double* __restrict__ partial)
+{
+    __shared__ double red[kVB];
+    const int c = blockIdx.x;
+    const int s = ck.sec[c];
+    const int64_t r0 = ck.r0[c], r1 = ck.r1[c];
+    const int p = threadIdx.x % Pp, rl = threadIdx.x / Pp, RL = kVB / Pp;
+    const double th = theta[(size_t)s * ts + (size_t)p * tp];
+    double acc = 0.0;
+    for (int64_t r = r0 + rl; r < r1; r += RL) {
+        const size_t o = (size_t)r * Pp + p;
+        const double d = fma(-th, x[o], hx[o]);
+        acc = fma(d, d, acc);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (rl == 0) {
+        double sum = 0.0;
+        for (int q = 0; q < RL; ++q) sum += red[q * Pp + p];
+        partial[(size_t)c * Pp + p] = sum;
+    }
+}
+
+// ------------------------------------------------- per-problem small kernels
+
+struct Problems {
+    const int32_t* lsec;  // Lanczos sectors
+    int nls;              // their count
+    int Pp;
+};
+
+HD_DEV bool problem_of(const Problems& pr, int gid, int& s, int& p)
+{
+    if (gid >= pr.nls * pr.Pp) return false;
+    s = pr.lsec[gid / pr.Pp];
+    p = gid % pr.Pp;
+    return true;
+}
+
+__global__ void k_fill_d(double* __restrict__ a, int64_t n, double v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void k_fill_i(int* __restrict__ a, int64_t n, int v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// 1/||v|| for the random start (points beyond P keep a zero vector)
+__global__ void k_binv_norm(Problems pr, const double* __restrict__ nrm, int PS, int P, double* __restrict__ inv)
+{
+    int s, p;
+    if (!problem_of(pr, blockIdx.x * blockDim.x + threadIdx.x, s, p)) return;
+    const double n2 = nrm[(size_t)s * PS * pr.Pp + p];
+    inv[(size_t)s * pr.Pp + p] = (p < P && n2 > 0.0) ? 1.0 / sqrt(n2) : 0.0;
+}
+
+// After the first pass: second pass where the norm fell below 1e-4 of its value
+// before the projection (DGKS) or on the first step of a cycle; *any = OR.
+__global__ void k_bflag(Problems pr, int first, const double* __restrict__ c1, int PS, int jn,
+                        const double* __restrict__ n1, double* __restrict__ need2, int* __restrict__ any)
+{
+    __shared__ int sh;
+    if (threadIdx.x == 0) sh = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int g = threadIdx.x; g < pr.nls * pr.Pp; g += blockDim.x) {
+        int s, p;
+        problem_of(pr, g, s, p);
+        const double before = c1[((size_t)s * PS + jn) * pr.Pp + p];
+        const double after = n1[(size_t)s * PS * pr.Pp + p];
+        const int need = first || after < 1e-4 * before;
+        need2[(size_t)s * pr.Pp + p] = need ? 1.0 : 0.0;
+        mine |= need;
+    }
+    if (mine) atomicOr(&sh, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) *any = sh;
+}
+
+
+// Coefficients of step j: alpha_j, beta_j (layout [j][s][p]), the scale of
+// v_{j+1}, breakdown.  state: 0 alive, 1 broke down in this cycle (exact
+// invariant subspace of size nact), 2 finished (vectors frozen).
+__global__ void k_bfinish(Problems pr, int S, int j, int local, const double* __restrict__ a,
+                          const double* __restrict__ c1, const double* __restrict__ c2, int PS,
+                          const double* __restrict__ need2, const double* __restrict__ n1,
+                          const double* __restrict__ n2, double* __restrict__ alpha, double* __restrict__ beta,
+                          double* __restrict__ inv, double* __restrict__ hscale, int* __restrict__ state,
+                          int* __restrict__ nact)
+{
+    int s, p;
+    if (!problem_of(pr, blockIdx.x * blockDim.x + threadIdx.x, s, p)) return;
+    const int Pp = pr.Pp;
+    const size_t q = (size_t)s * Pp + p;
+    const bool two = need2[q] != 0.0;
+    double al = c1[((size_t)s * PS + j) * Pp + p];
+    if (local) al += a[(size_t)s * PS * Pp + p];
+    if (two) al += c2[((size_t)s * PS + j) * Pp + p];
+    const double nn = two ? n2[(size_t)s * PS * Pp + p] : n1[(size_t)s * PS * Pp + p];
+    const double be = sqrt(fmax(nn, 0.0));
+    const double hs = fmax(hscale[q], fabs(al) + be);
+    hscale[q] = hs;
+    const int st = state[q];
+    const bool brk = !(be > 1e-12 * hs);
+    if (st == 0 && brk) {
+        state[q] = 1;
+        nact[q] = j + 1;
+    }
+    const bool live = st == 0 && !brk;
+    const size_t jq = ((size_t)j * S + s) * Pp + p;
+    alpha[jq] = al;
+    beta[jq] = live ? be : 0.0;
+    inv[q] = live ? 1.0 / be : 0.0;
+}
+
+// Cyclic Jacobi on the symmetric n x n matrix A (LDS, row-major) by one
+// wavefront: A becomes diagonal, Z (LDS) collects the rotations, d the
+// eigenvalues (unsorted).  Rotation formulas of Rutishauser (Numer. Math. 9
+// (1966) 1): t = sgn(th)/(|th| + sqrt(th^2 + 1)), th = (a_qq - a_pp)/(2 a_pq).
+HD_DEV double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+HD_DEV void wave_jacobi(double* A, double* Z, double* d, int n)
+{
+    const int lane = threadIdx.x;
+    for (int i = lane; i < n * n; i += 64) Z[i] = (i / n == i % n) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = lane; i < n * n; i += 64) {
+            const double v = A[i];
+            tot += v * v;
+            if (i / n != i % n) off += v * v;
+        }
+        off = wave_sum(off);
+        tot = wave_sum(tot);
+        if (!(off > 1e-28 * tot)) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[p * n + q];
+                if (fabs(apq) <= 1e-300) continue;  // uniform: every lane read the same value
+                const double app = A[p * n + p], aqq = A[q * n + q];
+                const double th = (aqq - app) / (2.0 * apq);
+                const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                __syncthreads();
+                if (lane < n) {  // columns p, q
+                    const double akp = A[lane * n + p], akq = A[lane * n + q];
+                    A[lane * n + p] = c * akp - s * akq;
+                    A[lane * n + q] = s * akp + c * akq;
+                }
+                __syncthreads();
+                if (lane < n) {  // rows p, q
+                    const double apk = A[p * n + lane], aqk = A[q * n + lane];
+                    A[p * n + lane] = c * apk - s * aqk;
+                    A[q * n + lane] = s * apk + c * aqk;
+                    const double zkp = Z[lane * n + p], zkq = Z[lane * n + q];
+                    Z[lane * n + p] = c * zkp - s * zkq;
+                    Z[lane * n + q] = s * zkp + c * zkq;
+                }
+                __syncthreads();
+                if (lane == 0) {  // the rotated 2x2 block takes its exact values
+                    A[p * n + q] = 0.0;
+                    A[q * n + p] = 0.0;
+                    A[p * n + p] = app - t * apq;
+                    A[q * n + q] = aqq + t * apq;
+                }
+                __syncthreads();
+            }
+    }
+    for (int i = lane; i < n; i += 64) d[i] = A[i * n + i];
+    __syncthreads();
+}
+
+// Ascending order of d[0..n) into perm (lane 0), visible to the wavefront.
+HD_DEV void wave_sort(const double* d, int* perm, int n)
+{
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < n; ++i) perm[i] = i;
+        for (int i = 0; i < n; ++i) {
+            int b = i;
+            for (int k = i + 1; k < n; ++k)
+                if (d[perm[k]] < d[perm[b]]) b = k;
+            const int t = perm[i];
+            perm[i] = perm[b];
+            perm[b] = t;
+        }
+    }
+    __syncthreads();
+}
+
+// LDS of one Jacobi problem: A, Z (n x n), d (n) and perm (n ints)
+__host__ __device__ inline size_t jacobi_lds(int n)
+{
+    return (size_t)(2 * n * n + n) * sizeof(double) + (size_t)n * sizeof(int);
+}
+
+// One wavefront per Lanczos problem: diagonalise the projected matrix
+// (arrow of the kept Ritz values for i < k, tridiagonal from k), record the nev
+// lowest Ritz values and residual estimates |beta_{m-1} y_{m-1,i}|, decide
+// convergence, and write the restart rotation Y and the next cycle's kept
+// alpha_i = theta_i, arrow_i = beta_{m-1} y_{m-1,i}.
+__global__ void __launch_bounds__(64)
+k_brestart(Problems pr, int S, int m, int k, int keep, int nev, double tol, double* __restrict__ alpha,
+           const double* __restrict__ beta, double* __restrict__ arrow, int* __restrict__ state,
+           const int* __restrict__ nact, int* __restrict__ conv, double* __restrict__ evals,
+           double* __restrict__ resid, double* __restrict__ Y, int* __restrict__ unconverged)
+{
+    extern __shared__ double sm[];
+    int s, p;
+    if (!problem_of(pr, blockIdx.x, s, p)) return;
+    const int lane = threadIdx.x;
+    const int Pp = pr.Pp;
+    const size_t q = (size_t)s * Pp + p;
+    const int st = state[q];
+    auto yat = [&](int l, int i) -> double& { return Y[(((size_t)l * keep + i) * S + s) * Pp + p]; };
+    if (st == 2) {  // finished: keep the vectors as they are
+        for (int idx = lane; idx < m * keep; idx += 64) yat(idx / keep, idx % keep) = (idx / keep == idx % keep) ? 1.0 : 0.0;
+        return;
+    }
+    const int n = st == 1 ? nact[q] : m;
+    double* A = sm;
+    double* Z = A + n * n;
+    double* d = Z + n * n;
+    int* perm = reinterpret_cast<int*>(d + n);
+    for (int idx = lane; idx < n * n; idx += 64) A[idx] = 0.0;
+    __syncthreads();
+    if (lane == 0) {
+        for (int i = 0; i < n; ++i) A[i * n + i] = alpha[((size_t)i * S + s) * Pp + p];
+        for (int i = 0; i < k && k < n; ++i) {
+            const double ar = arrow[((size_t)i * S + s) * Pp + p];
+            A[i * n + k] = ar;
+            A[k * n + i] = ar;
+        }
+        for (int j = k; j + 1 < n; ++j) {
+            const double be = beta[((size_t)j * S + s) * Pp + p];
+            A[j * n + j + 1] = be;
+            A[(j + 1) * n + j] = be;
+        }
+    }
+    __syncthreads();
+    wave_jacobi(A, Z, d, n);
+    wave_sort(d, perm, n);
+    const double bm = st == 1 ? 0.0 : beta[((size_t)(m - 1) * S + s) * Pp + p];
+    if (lane == 0) {
+        bool ok = true;
+        for (int i = 0; i < nev; ++i) {
+            double th = NAN, r = NAN;
+            if (i < n) {
+                th = d[perm[i]];
+                r = fabs(bm * Z[(n - 1) * n + perm[i]]);
+                if (!(r <= tol * fmax(1.0, fabs(th)))) ok = false;
+            }
+            evals[q * nev + i] = th;
+            resid[q * nev + i] = r;
+        }
+        if (st == 1) {
+            ok = true;  // exact: the Krylov space is invariant
+            state[q] = 2;
+        }
+        conv[q] = ok ? 1 : 0;
+        if (!ok) atomicAdd(unconverged, 1);
+    }
+    for (int idx = lane; idx < m * keep; idx += 64) {
+        const int l = idx / keep, i = idx % keep;
+        yat(l, i) = (l < n && i < n) ? Z[l * n + perm[i]] : 0.0;
+    }
+    for (int i = lane; i < keep; i += 64) {
+        alpha[((size_t)i * S + s) * Pp + p] = i < n ? d[perm[i]] : 0.0;
+        arrow[((size_t)i * S + s) * Pp + p] = i < n ? bm * Z[(n - 1) * n + perm[i]] : 0.0;
+    }
+}
+
+// Dense path: one wavefront per (sector, point) of a small sector, the matrix
+// assembled from the coded CRS, the dictionary and the diagonal.
+__global__ void __launch_bounds__(64)
+k_bdense(const int32_t* __restrict__ dsec, const int64_t* __restrict__ row0, const int64_t* __restrict__ dims,
+         const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col, const uint16_t* __restrict__ code,
+         const double* __restrict__ dict, const double* __restrict__ diag, int Pp, int P, int nev,
+         double* __restrict__ evals)
+{
+    extern __shared__ double sm[];
+    const int s = dsec[blockIdx.x / Pp];
+    const int p = blockIdx.x % Pp;
+    if (p >= P) return;
+    const int n = (int)dims[s];
+    const int64_t r0 = row0[s];
+    const int lane = threadIdx.x;
+    double* A = sm;
+    double* Z = A + n * n;
+    double* d = Z + n * n;
+    int* perm = reinterpret_cast<int*>(d + n);
+    for (int idx = lane; idx < n * n; idx += 64) A[idx] = 0.0;
+    __syncthreads();
+    if (lane < n) {
+        const int64_t r = r0 + lane;
+        for (int64_t e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+            const unsigned cd = code[e];
+            if (cd == 0xFFFFu) continue;
+            const int j = (int)(col[e] - r0);
+            const double v = dict[(size_t)(cd & 0x7FFFu) * Pp + p];
+            A[lane * n + j] = (cd & 0x8000u) ? -v : v;
+        }
+        A[lane * n + lane] = diag[(size_t)r * Pp + p];
+    }
+    __syncthreads();
+    wave_jacobi(A, Z, d, n);
+    wave_sort(d, perm, n);
+    if (lane < nev) evals[((size_t)s * Pp + p) * nev + lane] = lane < n ? d[perm[lane]] : NAN;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- host side
+
+static int spmm_launch(const hd_batch* b, int pt, int grid, const double* x, double* y, hipStream_t st)
+{
+    const int tiles = b->Pp / pt;
+    const size_t lds = (size_t)std::max(b->ndict, 1) * pt * sizeof(double);
+    if (pt == 16)
+        hipLaunchKernelGGL(k_bspmm<16>, dim3(grid, tiles), dim3(1024), lds, st, b->R, b->d_rowptr, b->d_col,
+                           b->d_code, b->d_dict, b->ndict, b->d_diag, b->Pp, x, y);
+    else
+        hipLaunchKernelGGL(k_bspmm<8>, dim3(grid, tiles), dim3(1024), lds, st, b->R, b->d_rowptr, b->d_col,
+                           b->d_code, b->d_dict, b->ndict, b->d_diag, b->Pp, x, y);
+    HD_HIP(hipGetLastError());
+    return HD_OK;
+}
+
+static int spmm(const hd_batch* b, const double* x, double* y, hipStream_t st)
+{
+    return spmm_launch(b, b->spmm_pt, b->spmm_grid, x, y, st);
+}
+
+static int set_big_lds(const void* fn, size_t lds)
+{
+    if (lds > 160 * 1024) return fail(HD_ERR_UNSUPPORTED, "kernel needs " + std::to_string(lds) + " bytes of LDS");
+    if (lds > 64 * 1024) HD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    return HD_OK;
+}
+
+// Tile width and grid of the SpMM by timing on the given vectors (x is read,
+// y written).  PT = 16 when the dictionary tile fits in LDS, else 8.
+static int spmm_autotune(hd_batch* b, const double* x, double* y)
+{
+    int n_cu = 256;
+    HD_CHECK(device_props(&n_cu, nullptr));
+    const size_t nd = (size_t)std::max(b->ndict, 1);
+    float best = 1e30f;
+    int best_pt = 0, best_grid = 0;
+    EventTimer timer;
+    for (int pt : {16, 8}) {
+        const size_t lds = nd * pt * sizeof(double);
+        if (lds > 160 * 1024 || b->Pp % pt) continue;
+        HD_CHECK(set_big_lds(pt == 16 ? (const void*)k_bspmm<16> : (const void*)k_bspmm<8>, lds));
+        const int tiles = b->Pp / pt;
+        const int per_cu = lds > 80 * 1024 ? 1 : 2;
+        for (int mult : {1, 2}) {
+            const int64_t rows_per_block = 16 * (64 / pt);
+            const int grid = (int)std::max<int64_t>(
+                1, std::min<int64_t>((int64_t)n_cu * per_cu * mult / tiles, (b->R + rows_per_block - 1) / rows_per_block));
+            HD_CHECK(spmm_launch(b, pt, grid, x, y, b->stream));  // warm
+            timer.start(b->stream);
+            for (int r = 0; r < 3; ++r) HD_CHECK(spmm_launch(b, pt, grid, x, y, b->stream));
+            const float ms = timer.stop(b->stream);
+            HD_CHECK(timer.status("batched SpMM autotune"));
+            if (ms < best) {
+                best = ms;
+                best_pt = pt;
+                best_grid = grid;
+            }
+        }
+    }
+    if (!best_pt)
+        return fail(HD_ERR_UNSUPPORTED, "the value dictionary (" + std::to_string(b->ndict) +
+                                            " tuples) does not fit in LDS for 8 points per tile");
+    b->spmm_pt = best_pt;
+    b->spmm_grid = best_grid;
+    return HD_OK;
+}
+
+namespace {
+
+// Device buffers of one solve, released together.
+struct Work {
+    std::vector<void*> ptrs;
+    ~Work()
+    {
+        for (void* p : ptrs) dfree(p);
+    }
+    template <typename T>
+    int get(T** p, size_t count, const char* what)
+    {
+        HD_CHECK(dalloc_t(p, std::max<size_t>(count, 1), what));
+        ptrs.push_back(*p);
+        return HD_OK;
+    }
+};
+
+struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    ~Graph()
+    {
+        if (e) (void)hipGraphExecDestroy(e);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+}  // namespace
+
+}  // namespace hd
+
+using namespace hd;
+
+extern "C" {
+
+int hd_batch_create(const hd_basis* const* bases, int32_t n_sectors, const hd_tables* const* tables,
+                    int32_t n_points, double zero_tol, hd_batch** out)
+{
+    if (!bases || !tables || !out || n_sectors < 1 || n_points < 1 || n_points > 64)
+        return fail(HD_ERR_ARG, "hd_batch_create: 1..64 points and at least one sector");
+    *out = nullptr;
+    if (!(zero_tol >= 0.0)) return fail(HD_ERR_ARG, "zero_tol must be >= 0");
+    const hd_tables* t0 = tables[0];
+    for (int p = 0; p < n_points; ++p) {
+        const hd_tables* t = tables[p];
+        if (!t) return fail(HD_ERR_ARG, "null table");
+        if (t->is_complex) return fail(HD_ERR_UNSUPPORTED, "batched solves take real tables");
+        if (t->layout != t0->layout || t->K != t0->K || t->ktab != t0->ktab || t->eps.empty() != t0->eps.empty())
+            return fail(HD_ERR_ARG, "batch tables must share layout, kMax, k table and quadratic presence");
+    }
+    for (int s = 0; s < n_sectors; ++s) {
+        if (!bases[s]) return fail(HD_ERR_ARG, "null basis");
+        if (bases[s]->N != bases[0]->N || bases[s]->Norb != bases[0]->Norb || bases[s]->Norb != t0->K)
+            return fail(HD_ERR_ARG, "batch sectors must share N and the orbital count (= kMax)");
+    }
+    hd_batch* b = new hd_batch();
+    b->S = n_sectors;
+    b->P = n_points;
+    b->Pp = n_points <= 16 ? 16 : (n_points <= 32 ? 32 : 64);
+    b->K = t0->K;
+    b->N = bases[0]->N;
+    auto bail = [&](int st) {
+        const std::string msg = hd_last_error();
+        hd_batch_destroy(b);
+        set_error(msg);
+        return st;
+    };
+    if (hipGetDevice(&b->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(HD_ERR_HIP, "stream creation failed"));
+    // per-sector coded patterns with one shared tuple dictionary
+    BatchTables bt;
+    std::vector<hd_hamiltonian*> hs((size_t)n_sectors, nullptr);
+    auto free_hs = [&]() {
+        for (auto*& h : hs) {
+            hd_ham_destroy(h);
+            h = nullptr;
+        }
+    };
+    b->dim.resize((size_t)n_sectors);
+    b->row0.resize((size_t)n_sectors);
+    int64_t nnz = 0;
+    for (int s = 0; s < n_sectors; ++s) {
+        const int st = build_batch_sector(bases[s], tables, n_points, zero_tol, bt, &hs[(size_t)s]);
+        if (st != HD_OK) {
+            const std::string msg = hd_last_error();
+            free_hs();
+            set_error(msg);
+            return bail(st);
+        }
+        b->dim[(size_t)s] = bases[s]->dim;
+        b->row0[(size_t)s] = b->R;
+        b->R += bases[s]->dim;
+        nnz += hs[(size_t)s]->nnz;
+    }
+    b->nnz = nnz;
+    if (b->R >= ((int64_t)1 << 31)) {
+        free_hs();
+        return bail(fail(HD_ERR_UNSUPPORTED, "more than 2^31 rows in one batch"));
+    }
+    b->ndict = (int)bt.index.size();
+    const int Pp = b->Pp;
+    int st = HD_OK;
+    HipErrors q;
+    if (st == HD_OK) st = dalloc_t(&b->d_rowptr, (size_t)b->R + 1, "batch row pointers");
+    if (st == HD_OK) st = dalloc_t(&b->d_col, (size_t)nnz + 8, "batch columns");
+    if (st == HD_OK) st = dalloc_t(&b->d_code, (size_t)nnz + 8, "batch codes");
+    if (st == HD_OK) st = dalloc_t(&b->d_dict, (size_t)std::max(b->ndict, 1) * Pp, "batch dictionary");
+    if (st == HD_OK) st = dalloc_t(&b->d_diag, (size_t)b->R * Pp, "batch diagonals");
+    double *d_dterm = nullptr, *d_eps = nullptr;
+    if (st == HD_OK) st = dalloc_t(&d_dterm, std::max<size_t>(bt.dterm.size(), 1), "diagonal terms");
+    if (st == HD_OK && bt.has_eps) st = dalloc_t(&d_eps, bt.eps.size(), "quadratic terms");
+    if (st == HD_OK) {
+        // dictionary [ndict][Pp], zero for padded points
+        std::vector<double> dict((size_t)std::max(b->ndict, 1) * Pp, 0.0);
+        for (int i = 0; i < b->ndict; ++i)
+            for (int p = 0; p < n_points; ++p) dict[(size_t)i * Pp + p] = bt.dict[(size_t)i * n_points + p];
+        HD_ENQ(q, hipMemcpy(b->d_dict, dict.data(), dict.size() * 8, hipMemcpyHostToDevice));
+        HD_ENQ(q, hipMemcpy(d_dterm, bt.dterm.data(), bt.dterm.size() * 8, hipMemcpyHostToDevice));
+        if (bt.has_eps) HD_ENQ(q, hipMemcpy(d_eps, bt.eps.data(), bt.eps.size() * 8, hipMemcpyHostToDevice));
+        st = q.status();
+    }
+    int64_t e_off = 0;
+    for (int s = 0; s < n_sectors && st == HD_OK; ++s) {
+        const hd_hamiltonian* h = hs[(size_t)s];
+        const int64_t r0 = b->row0[(size_t)s], dim = b->dim[(size_t)s];
+        hipLaunchKernelGGL(k_rowptr_offset, dim3((unsigned)((dim + 1 + 255) / 256)), dim3(256), 0, b->stream,
+                           h->d_rowptr, dim + 1, e_off, b->d_rowptr + r0);
+        hipLaunchKernelGGL(k_col_offset, dim3((unsigned)std::min<int64_t>((h->nnz + 255) / 256 + 1, 65536)),
+                           dim3(256), 0, b->stream, h->d_col, h->nnz, (int32_t)r0, b->d_col + e_off);
+        HD_ENQ(q, hipMemcpyAsync(b->d_code + e_off, h->d_code, (size_t)h->nnz * 2, hipMemcpyDeviceToDevice,
+                                 b->stream));
+        hipLaunchKernelGGL(k_batch_diag, dim3((unsigned)((dim * Pp + 255) / 256)), dim3(256), 0, b->stream,
+                           bases[s]->d_states, dim, b->K, n_points, Pp, d_dterm, bt.has_eps ? d_eps : nullptr,
+                           zero_tol, b->d_diag + (size_t)r0 * Pp);
+        HD_ENQ(q, hipGetLastError());
+        e_off += h->nnz;
+        if (hipStreamSynchronize(b->stream) != hipSuccess) st = fail(HD_ERR_HIP, "batch assembly failed");
+        if (st == HD_OK) st = q.status();
+        hd_ham_destroy(hs[(size_t)s]);
+        hs[(size_t)s] = nullptr;
+    }
+    if (st == HD_OK) {
+        // padding entries: column 0, diagonal code (skipped by every kernel)
+        HD_ENQ(q, hipMemsetAsync(b->d_col + nnz, 0, 8 * sizeof(int32_t), b->stream));
+        HD_ENQ(q, hipMemsetAsync(b->d_code + nnz, 0xFF, 8 * sizeof(uint16_t), b->stream));
+        HD_ENQ(q, hipStreamSynchronize(b->stream));
+        st = q.status();
+    }
+    dfree(d_dterm);
+    dfree(d_eps);
+    free_hs();
+    if (st != HD_OK) return bail(st);
+    b->bytes = ((size_t)b->R + 1) * 8 + ((size_t)nnz + 8) * 6 + (size_t)b->R * Pp * 8 +
+               (size_t)std::max(b->ndict, 1) * Pp * 8;
+    *out = b;
+    return HD_OK;
+}
+
+int hd_batch_destroy(hd_batch* b)
+{
+    if (!b) return HD_OK;
+    dfree(b->d_rowptr);
+    dfree(b->d_col);
+    dfree(b->d_code);
+    dfree(b->d_dict);
+    dfree(b->d_diag);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return HD_OK;
+}
+
+int hd_batch_info(const hd_batch* b, int64_t* rows, int64_t* nnz, int32_t* ndict, int32_t* points_padded)
+{
+    if (!b) return fail(HD_ERR_ARG, "null batch");
+    if (rows) *rows = b->R;
+    if (nnz) *nnz = b->nnz;
+    if (ndict) *ndict = b->ndict;
+    if (points_padded) *points_padded = b->Pp;
+    return HD_OK;
+}
+
+int hd_batch_copy_pattern(const hd_batch* b, int64_t* h_rowptr, int32_t* h_col, uint16_t* h_code, double* h_dict,
+                          double* h_diag)
+{
+    if (!b) return fail(HD_ERR_ARG, "null batch");
+    HipErrors q;
+    if (h_rowptr) HD_ENQ(q, hipMemcpy(h_rowptr, b->d_rowptr, ((size_t)b->R + 1) * 8, hipMemcpyDeviceToHost));
+    if (h_col) HD_ENQ(q, hipMemcpy(h_col, b->d_col, (size_t)b->nnz * 4, hipMemcpyDeviceToHost));
+    if (h_code) HD_ENQ(q, hipMemcpy(h_code, b->d_code, (size_t)b->nnz * 2, hipMemcpyDeviceToHost));
+    if (h_dict)
+        HD_ENQ(q, hipMemcpy(h_dict, b->d_dict, (size_t)b->ndict * b->Pp * 8, hipMemcpyDeviceToHost));
+    if (h_diag) HD_ENQ(q, hipMemcpy(h_diag, b->d_diag, (size_t)b->R * b->Pp * 8, hipMemcpyDeviceToHost));
+    return q.status();
+}
+
+int hd_batch_spmm(hd_batch* b, const double* d_x, double* d_y)
+{
+    if (!b || !d_x || !d_y) return fail(HD_ERR_ARG, "hd_batch_spmm: null argument");
+    if (!b->spmm_grid) HD_CHECK(spmm_autotune(b, d_x, d_y));
+    HD_CHECK(spmm(b, d_x, d_y, b->stream));
+    HD_HIP(hipStreamSynchronize(b->stream));
+    return HD_OK;
+}
+
+int hd_batch_spmm_time(hd_batch* b, int32_t iters, double* avg_ms)
+{
+    if (!b || !avg_ms || iters <= 0) return fail(HD_ERR_ARG, "hd_batch_spmm_time: bad argument");
+    Work w;
+    double *x = nullptr, *y = nullptr;
+    HD_CHECK(w.get(&x, (size_t)b->R * b->Pp, "timing x"));
+    HD_CHECK(w.get(&y, (size_t)b->R * b->Pp, "timing y"));
+    const int64_t n = (int64_t)b->R * b->Pp;
+    hipLaunchKernelGGL(k_fill_d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, x, n, 1.0);
+    HD_HIP(hipGetLastError());
+    if (!b->spmm_grid) HD_CHECK(spmm_autotune(b, x, y));
+    EventTimer timer;
+    timer.start(b->stream);
+    for (int i = 0; i < iters; ++i) HD_CHECK(spmm(b, x, y, b->stream));
+    const float ms = timer.stop(b->stream);
+    HD_CHECK(timer.status("batched SpMM timing"));
+    *avg_ms = ms / iters;
+    return HD_OK;
+}
+
+int hd_batch_solve(hd_batch* b, const hd_batch_opts* o, double* h_evals, double* h_resid, hd_batch_stats* stats)
+{
+    if (!b || !o || !h_evals) return fail(HD_ERR_ARG, "hd_batch_solve: null argument");
+    const auto t_start = std::chrono::steady_clock::now();
+    const int nev = o->nev;
+    if (nev < 1) return fail(HD_ERR_ARG, "nev must be >= 1");
+    const int m = o->ncv > 0 ? o->ncv : 2 * nev + 21;
+    if (m > kMaxM || m < nev + 2) return fail(HD_ERR_ARG, "ncv must be in [nev+2, 64]");
+    const int keep = nev + (m - nev) / 2;
+    const int max_restarts = o->max_restarts > 0 ? o->max_restarts : 2000;
+    const double tol = o->tol > 0 ? o->tol : 1e-10;
+    const int dense_below = std::max(o->dense_below > 0 ? o->dense_below : 60, m + 1);
+    if (dense_below > 65) return fail(HD_ERR_ARG, "dense cutoff above 65 (the dense kernel holds 64 rows)");
+    const int S = b->S, P = b->P, Pp = b->Pp;
+    const hipStream_t st = b->stream;
+    hd_batch_stats sx{};
+    std::vector<int32_t> lsec, dsec;
+    for (int s = 0; s < S; ++s) (b->dim[(size_t)s] < dense_below ? dsec : lsec).push_back(s);
+    sx.problems = S * P;
+    sx.dense_problems = (int)dsec.size() * P;
+    sx.lanczos_problems = (int)lsec.size() * P;
+    std::vector<double> ev((size_t)S * Pp * nev, NAN), rs((size_t)S * Pp * nev, NAN);
+    Work w;
+    HipErrors q;
+    int64_t* d_row0 = nullptr;
+    int64_t* d_dims = nullptr;
+    HD_CHECK(w.get(&d_row0, (size_t)S, "sector rows"));
+    HD_CHECK(w.get(&d_dims, (size_t)S, "sector dims"));
+    HD_ENQ(q, hipMemcpy(d_row0, b->row0.data(), (size_t)S * 8, hipMemcpyHostToDevice));
+    HD_ENQ(q, hipMemcpy(d_dims, b->dim.data(), (size_t)S * 8, hipMemcpyHostToDevice));
+    HD_CHECK(q.status());
+    double* d_ev = nullptr;
+    double* d_rs = nullptr;
+    HD_CHECK(w.get(&d_ev, ev.size(), "eigenvalues"));
+    HD_CHECK(w.get(&d_rs, rs.size(), "residual estimates"));
+    // ---- dense path
+    if (!dsec.empty()) {
+        int32_t* d_dsec = nullptr;
+        HD_CHECK(w.get(&d_dsec, dsec.size(), "dense sectors"));
+        HD_ENQ(q, hipMemcpy(d_dsec, dsec.data(), dsec.size() * 4, hipMemcpyHostToDevice));
+        HD_CHECK(q.status());
+        int nmax = 1;
+        for (int s : dsec) nmax = std::max<int>(nmax, (int)b->dim[(size_t)s]);
+        const size_t lds = jacobi_lds(nmax);
+        HD_CHECK(set_big_lds((const void*)k_bdense, lds));
+        hipLaunchKernelGGL(k_bdense, dim3((unsigned)(dsec.size() * Pp)), dim3(64), lds, st, d_dsec, d_row0, d_dims,
+                           b->d_rowptr, b->d_col, b->d_code, b->d_dict, b->d_diag, Pp, P, nev, d_ev);
+        HD_HIP(hipGetLastError());
+        HD_HIP(hipStreamSynchronize(st));
+    }
+    // ---- batched Lanczos
+    if (!lsec.empty()) {
+        const int nls = (int)lsec.size();
+        // chunks of the Lanczos sectors
+        std::vector<int64_t> ck_r0, ck_r1;
+        std::vector<int32_t> ck_sec, sec_c0((size_t)S, 0), sec_c1((size_t)S, 0);
+        for (int s : lsec) {
+            sec_c0[(size_t)s] = (int)ck_r0.size();
+            for (int64_t r = 0; r < b->dim[(size_t)s]; r += kChunk) {
+                ck_r0.push_back(b->row0[(size_t)s] + r);
+                ck_r1.push_back(b->row0[(size_t)s] + std::min<int64_t>(r + kChunk, b->dim[(size_t)s]));
+                ck_sec.push_back(s);
+            }
+            sec_c1[(size_t)s] = (int)ck_r0.size();
+        }
+        const int nck = (int)ck_r0.size();
+        const int PS = m + 2;
+        const size_t vsz = (size_t)b->R * Pp;
+        const size_t prob = (size_t)S * Pp;
+        int64_t *d_ck_r0 = nullptr, *d_ck_r1 = nullptr;
+        int32_t *d_ck_sec = nullptr, *d_sec_c0 = nullptr, *d_sec_c1 = nullptr, *d_lsec = nullptr;
+        HD_CHECK(w.get(&d_ck_r0, (size_t)nck, "chunks"));
+        HD_CHECK(w.get(&d_ck_r1, (size_t)nck, "chunks"));
+        HD_CHECK(w.get(&d_ck_sec, (size_t)nck, "chunks"));
+        HD_CHECK(w.get(&d_sec_c0, (size_t)S, "sector chunks"));
+        HD_CHECK(w.get(&d_sec_c1, (size_t)S, "sector chunks"));
+        HD_CHECK(w.get(&d_lsec, (size_t)nls, "Lanczos sectors"));
+        HD_ENQ(q, hipMemcpy(d_ck_r0, ck_r0.data(), (size_t)nck * 8, hipMemcpyHostToDevice));
+        HD_ENQ(q, hipMemcpy(d_ck_r1, ck_r1.data(), (size_t)nck * 8, hipMemcpyHostToDevice));
+        HD_ENQ(q, hipMemcpy(d_ck_sec, ck_sec.data(), (size_t)nck * 4, hipMemcpyHostToDevice));
+        HD_ENQ(q, hipMemcpy(d_sec_c0, sec_c0.data(), (size_t)S * 4, hipMemcpyHostToDevice));
+        HD_ENQ(q, hipMemcpy(d_sec_c1, sec_c1.data(), (size_t)S * 4, hipMemcpyHostToDevice));
+        HD_ENQ(q, hipMemcpy(d_lsec, lsec.data(), (size_t)nls * 4, hipMemcpyHostToDevice));
+        HD_CHECK(q.status());
+        const Chunks ck{d_ck_r0, d_ck_r1, d_ck_sec};
+        const Problems pr{d_lsec, nls, Pp};
+        double *V = nullptr, *W = nullptr, *partial = nullptr, *C1 = nullptr, *C2 = nullptr, *A = nullptr,
+               *N1 = nullptr, *N2 = nullptr, *need2 = nullptr, *inv = nullptr, *hscale = nullptr, *alpha = nullptr,
+               *beta = nullptr, *arrow = nullptr, *Yr = nullptr;
+        int *state = nullptr, *nact = nullptr, *conv = nullptr, *d_any = nullptr, *d_unconv = nullptr;
+        HD_CHECK(w.get(&V, (size_t)(m + 1) * vsz, "Lanczos basis"));
+        HD_CHECK(w.get(&W, vsz, "Lanczos work vector"));
+        HD_CHECK(w.get(&partial, (size_t)nck * PS * Pp, "dot partials"));
+        for (double** p : {&C1, &C2, &A, &N1, &N2}) HD_CHECK(w.get(p, prob * PS, "dot results"));
+        for (double** p : {&need2, &inv, &hscale}) HD_CHECK(w.get(p, prob, "per-problem scalars"));
+        for (double** p : {&alpha, &beta, &arrow}) HD_CHECK(w.get(p, (size_t)m * prob, "projected matrices"));
+        HD_CHECK(w.get(&Yr, (size_t)m * keep * prob, "restart rotations"));
+        for (int** p : {&state, &nact, &conv}) HD_CHECK(w.get(p, prob, "problem state"));
+        HD_CHECK(w.get(&d_any, 1, "flag"));
+        HD_CHECK(w.get(&d_unconv, 1, "counter"));
+        auto fill_d = [&](double* a, size_t n, double v) {
+            hipLaunchKernelGGL(k_fill_d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, (int64_t)n, v);
+        };
+        auto fill_i = [&](int* a, size_t n, int v) {
+            hipLaunchKernelGGL(k_fill_i, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, (int64_t)n, v);
+        };
+        fill_d(V, (size_t)(m + 1) * vsz, 0.0);
+        fill_d(W, vsz, 0.0);
+        for (double* p : {alpha, beta, arrow}) fill_d(p, (size_t)m * prob, 0.0);
+        fill_d(hscale, prob, 0.0);
+        fill_d(need2, prob, 0.0);
+        fill_i(nact, prob, m);
+        fill_i(conv, prob, 0);
+        {
+            // padded points are finished from the start
+            std::vector<int> s0(prob, 2);
+            for (int s = 0; s < S; ++s)
+                for (int p = 0; p < P; ++p) s0[(size_t)s * Pp + p] = 0;
+            HD_ENQ(q, hipMemcpyAsync(state, s0.data(), prob * 4, hipMemcpyHostToDevice, st));
+            HD_ENQ(q, hipStreamSynchronize(st));
+        }
+        HD_ENQ(q, hipGetLastError());
+        HD_CHECK(q.status());
+        const int64_t vstride = (int64_t)vsz;
+        auto Vj = [&](int j) { return V + (size_t)j * vsz; };
+        const dim3 gp((unsigned)((nls * Pp + 255) / 256)), bp(256);
+        auto dots = [&](int nv, const double* base, const double* wv, int want_norm, double* out, const int* any) {
+            hipLaunchKernelGGL(k_bdots, dim3((unsigned)nck), dim3(kVB), 0, st, nv, base, vstride, wv, want_norm, ck,
+                               Pp, PS, partial, any);
+            hipLaunchKernelGGL(k_breduce, dim3((unsigned)nls, (unsigned)(nv + want_norm)), dim3(Pp), 0, st,
+                               nv + want_norm, partial, PS, d_lsec, d_sec_c0, d_sec_c1, Pp, out, any);
+        };
+        auto update = [&](int nv, const double* base, const double* coef, int64_t cs, int64_t ci, const double* mask,
+                          const int* any) {
+            hipLaunchKernelGGL(k_bupdate, dim3((unsigned)nck), dim3(kVB), 0, st, nv, base, vstride, coef, cs, ci,
+                               mask, ck, Pp, W, any);
+        };
+        // ---- start: V_0 = normalised random vector per problem
+        hipLaunchKernelGGL(k_brandn, dim3((unsigned)nck), dim3(kVB), 0, st, o->seed, ck, Pp, P, V);
+        dots(0, V, V, 1, N1, nullptr);
+        hipLaunchKernelGGL(k_binv_norm, gp, bp, 0, st, pr, N1, PS, P, inv);
+        hipLaunchKernelGGL(k_bscale, dim3((unsigned)nck), dim3(kVB), 0, st, V, inv, ck, Pp, V);
+        HD_HIP(hipGetLastError());
+        HD_CHECK(spmm_autotune(b, V, W));
+        const size_t rlds = jacobi_lds(m);
+        HD_CHECK(set_big_lds((const void*)k_brestart, rlds));
+        int spmm_err = HD_OK;
+        auto step = [&](int j, int k) {
+            if (spmm_err == HD_OK) spmm_err = spmm(b, Vj(j), W, st);
+            const int local = j > k;
+            if (local) {
+                update(1, Vj(j - 1), beta + (size_t)(j - 1) * prob, Pp, 0, nullptr, nullptr);
+                dots(1, Vj(j), W, 0, A, nullptr);
+                update(1, Vj(j), A, (int64_t)PS * Pp, 0, nullptr, nullptr);
+            }
+            dots(j + 1, V, W, 1, C1, nullptr);
+            update(j + 1, V, C1, (int64_t)PS * Pp, Pp, nullptr, nullptr);
+            dots(0, V, W, 1, N1, nullptr);
+            hipLaunchKernelGGL(k_bflag, dim3(1), dim3(1024), 0, st, pr, j == k ? 1 : 0, C1, PS, j + 1, N1, need2,
+                               d_any);
+            dots(j + 1, V, W, 0, C2, d_any);
+            update(j + 1, V, C2, (int64_t)PS * Pp, Pp, need2, d_any);
+            dots(0, V, W, 1, N2, d_any);
+            hipLaunchKernelGGL(k_bfinish, gp, bp, 0, st, pr, S, j, local, A, C1, C2, PS, need2, N1, N2, alpha, beta,
+                               inv, hscale, state, nact);
+            hipLaunchKernelGGL(k_bscale, dim3((unsigned)nck), dim3(kVB), 0, st, W, inv, ck, Pp, Vj(j + 1));
+        };
+        auto cycle = [&](int k) {
+            fill_i(d_unconv, 1, 0);
+            for (int j = k; j < m; ++j) step(j, k);
+            hipLaunchKernelGGL(k_brestart, dim3((unsigned)(nls * Pp)), dim3(64), rlds, st, pr, S, m, k, keep, nev,
+                               tol, alpha, beta, arrow, state, nact, conv, d_ev, d_rs, Yr, d_unconv);
+            hipLaunchKernelGGL(k_bcombine, dim3((unsigned)nck), dim3(kVB), 0, st, m, keep, V, vstride, Yr, S, ck, Pp);
+        };
+        int unconv = 0;
+        auto read_unconv = [&]() -> int {
+            HD_HIP(hipGetLastError());
+            HD_HIP(hipMemcpyAsync(&unconv, d_unconv, 4, hipMemcpyDeviceToHost, st));
+            HD_HIP(hipStreamSynchronize(st));
+            return spmm_err;
+        };
+        // first cycle from k = 0, eagerly
+        cycle(0);
+        HD_CHECK(read_unconv());
+        sx.steps = m;
+        int restarts = 0;
+        Graph graph;
+        while (unconv > 0 && restarts < max_restarts) {
+            if (o->use_graph) {
+                if (!graph.e) {
+                    HD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                    cycle(keep);
+                    const hipError_t ce = hipStreamEndCapture(st, &graph.g);
+                    if (ce != hipSuccess) return fail(HD_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+                    HD_CHECK(spmm_err);
+                    HD_HIP(hipGraphInstantiate(&graph.e, graph.g, nullptr, nullptr, 0));
+                }
+                HD_HIP(hipGraphLaunch(graph.e, st));
+            } else {
+                cycle(keep);
+            }
+            HD_CHECK(read_unconv());
+            ++restarts;
+            sx.steps += m - keep;
+        }
+        sx.restarts = restarts;
+        // true residuals of the nev Ritz vectors (V_0..V_{nev-1} after the last combine)
+        std::vector<double> r2(prob);
+        double* R2 = N2;
+        for (int i = 0; i < nev; ++i) {
+            HD_CHECK(spmm(b, Vj(i), W, st));
+            hipLaunchKernelGGL(k_bresid, dim3((unsigned)nck), dim3(kVB), 0, st, W, Vj(i), d_ev + i,
+                               (int64_t)Pp * nev, (int64_t)nev, ck, Pp, partial);
+            hipLaunchKernelGGL(k_breduce, dim3((unsigned)nls, 1u), dim3(Pp), 0, st, 1, partial, 1, d_lsec, d_sec_c0,
+                               d_sec_c1, Pp, R2, nullptr);
+            HD_HIP(hipGetLastError());
+            std::vector<double> tmp((size_t)S * Pp);
+            HD_HIP(hipMemcpyAsync(tmp.data(), R2, tmp.size() * 8, hipMemcpyDeviceToHost, st));
+            HD_HIP(hipStreamSynchronize(st));
+            for (size_t k = 0; k < tmp.size(); ++k) r2[k] = tmp[k];
+            std::vector<double> evh(ev.size());
+            HD_HIP(hipMemcpy(evh.data(), d_ev, evh.size() * 8, hipMemcpyDeviceToHost));
+            for (int s : lsec)
+                for (int p = 0; p < P; ++p) {
+                    const size_t qq = (size_t)s * Pp + p;
+                    const double th = evh[qq * nev + i];
+                    rs[qq * nev + i] = std::sqrt(std::max(r2[qq], 0.0)) / std::max(1.0, std::fabs(th));
+                }
+        }
+        std::vector<int> convh(prob);
+        HD_HIP(hipMemcpy(convh.data(), conv, prob * 4, hipMemcpyDeviceToHost));
+        for (int s : lsec)
+            for (int p = 0; p < P; ++p) {
+                const size_t qq = (size_t)s * Pp + p;
+                bool ok = convh[qq] != 0;
+                for (int i = 0; i < nev; ++i) {
+                    if (std::isnan(rs[qq * nev + i])) continue;
+                    sx.max_residual = std::max(sx.max_residual, rs[qq * nev + i]);
+                    if (!(rs[qq * nev + i] <= tol)) ok = false;
+                }
+                sx.converged += ok ? 1 : 0;
+            }
+    }
+    HD_HIP(hipMemcpy(ev.data(), d_ev, ev.size() * 8, hipMemcpyDeviceToHost));
+    // dense problems are exact
+    for (int s : dsec)
+        for (int p = 0; p < P; ++p) {
+            for (int i = 0; i < nev; ++i) rs[((size_t)s * Pp + p) * nev + i] = i < b->dim[(size_t)s] ? 0.0 : NAN;
+            ++sx.converged;
+        }
+    for (int s = 0; s < S; ++s)
+        for (int p = 0; p < P; ++p)
+            for (int i = 0; i < nev; ++i) {
+                const size_t src = ((size_t)s * Pp + p) * nev + i, dst = ((size_t)s * P + p) * nev + i;
+                h_evals[dst] = i < b->dim[(size_t)s] ? ev[src] : NAN;
+                if (h_resid) h_resid[dst] = i < b->dim[(size_t)s] ? rs[src] : NAN;
+            }
+    sx.time_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    if (stats) *stats = sx;
+    return sx.converged == sx.problems ? HD_OK
+                                       : fail(HD_ERR_NOT_CONVERGED, std::to_string(sx.problems - sx.converged) +
+                                                                        " batched problems did not converge");
+}
+
+}  // extern "C"

@@ -796,6 +796,98 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
     return HD_OK;
 }
 
+// One sector of a batch: tables[0..n) share their k table and differ in the
+// values.  The CRS pattern keeps every hop that survives the trim under at
+// least one table; the code of an entry indexes the tuple of its hop value
+// under all tables (0.0 where the hop is trimmed or absent), numbered in order
+// of first appearance across the sectors of the batch.
+int build_batch_sector(const hd_basis* b, const hd_tables* const* tables, int n, double tol, BatchTables& bt,
+                       hd_hamiltonian** out)
+{
+    *out = nullptr;
+    const hd_tables* t0 = tables[0];
+    std::vector<Derived> Ds((size_t)n);
+    for (int p = 0; p < n; ++p) HD_CHECK(derive_tables(tables[p], b, tol, Ds[(size_t)p]));
+    const int K = t0->K;
+    const size_t nk = Ds[0].kpair.size();
+    Derived M = Ds[0];  // diagonal flags of table 0 only mark where the pattern holds a diagonal entry
+    M.remap.assign(nk, (int16_t)-1);
+    std::vector<double> tuple((size_t)n);
+    std::vector<uint64_t> key((size_t)n);
+    for (size_t i = 0; i < nk; ++i) {
+        int d = -1;
+        bool kept = false;
+        for (int p = 0; p < n; ++p) {
+            const uint8_t e = Ds[(size_t)p].kpair[i];
+            if (e == 0xFF) {
+                tuple[(size_t)p] = 0.0;
+                continue;
+            }
+            if (d >= 0 && (e & 0x3F) != d)
+                return fail(HD_ERR_UNSUPPORTED, "batch tables disagree on the hop structure");
+            d = e & 0x3F;
+            const bool k = !(e & 0x80);
+            kept |= k;
+            tuple[(size_t)p] = k ? Ds[(size_t)p].pval[i] : 0.0;
+        }
+        if (d < 0) {
+            M.kpair[i] = 0xFF;
+            continue;
+        }
+        M.kpair[i] = (uint8_t)(kept ? d : (d | 0x80));
+        if (!kept) continue;
+        for (int p = 0; p < n; ++p) std::memcpy(&key[(size_t)p], &tuple[(size_t)p], 8);
+        auto it = bt.index.find(key);
+        int code;
+        if (it == bt.index.end()) {
+            code = (int)bt.index.size();
+            if (code >= 0x7FFF)
+                return fail(HD_ERR_UNSUPPORTED, "more than 32766 distinct hop-value tuples in the batch");
+            bt.index.emplace(key, code);
+            bt.dict.insert(bt.dict.end(), tuple.begin(), tuple.end());
+        } else {
+            code = it->second;
+        }
+        M.remap[i] = (int16_t)code;
+    }
+    M.codable = true;
+    if (bt.dterm.empty()) {
+        // diagonal terms and E_k of every table, 0.0 where the reference skips
+        // the term (|el| <= tol), point index fastest
+        bt.K = K;
+        bt.n = n;
+        bt.has_eps = Ds[0].has_eps;
+        bt.dterm.assign((size_t)K * K * 2 * n, 0.0);
+        bt.eps.assign((size_t)K * n, 0.0);
+        for (int p = 0; p < n; ++p) {
+            const Derived& D = Ds[(size_t)p];
+            for (size_t idx = 0; idx < (size_t)K * K; ++idx)
+                for (int q = 0; q < 2; ++q)
+                    if (D.dflag[idx] & (1 << q)) bt.dterm[(idx * 2 + q) * n + p] = D.dterm[idx * 2 + q];
+            for (int k = 0; k < K; ++k)
+                if (D.has_eps && D.epsflag[(size_t)k]) bt.eps[(size_t)k * n + p] = D.eps[(size_t)k];
+        }
+    }
+    hd_hamiltonian* h = new hd_hamiltonian();
+    h->basis = b;
+    h->dim = b->dim;
+    h->storage = HD_STORAGE_FULL;
+    if (hipGetDevice(&h->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(HD_ERR_HIP, "stream creation failed");
+    }
+    const int st = run_build<false>(h, b, M, tol, 4);
+    if (st != HD_OK) {
+        const std::string msg = hd_last_error();
+        hd_ham_destroy(h);
+        set_error(msg);
+        return st;
+    }
+    *out = h;
+    return HD_OK;
+}
+
 }  // namespace hd
 
 using namespace hd;

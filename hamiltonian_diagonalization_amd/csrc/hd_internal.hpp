@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -273,6 +274,19 @@ struct hd_hamiltonian {
 };
 
 namespace hd {
+// Hop-value tuples of a batch of tables (one value per table), shared by all
+// sectors of the batch, and the per-table diagonal terms.
+struct BatchTables {
+    int K = 0, n = 0, has_eps = 0;
+    std::map<std::vector<uint64_t>, int> index;  // tuple bits -> dictionary index
+    std::vector<double> dict;                    // tuple i at [i*n, (i+1)*n)
+    std::vector<double> dterm;                   // [((k4*K + k3)*2 + q)*n + p], 0.0 where skipped
+    std::vector<double> eps;                     // [k*n + p]
+};
+// Coded full-storage pattern of one sector for a batch of real tables
+// (hd_build.hip); the codes index bt's tuples.
+int build_batch_sector(const hd_basis* b, const hd_tables* const* tables, int n, double tol, BatchTables& bt,
+                       hd_hamiltonian** out);
 int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y,
                 hipStream_t stream);
 int spmv_autotune(hd_hamiltonian* h);

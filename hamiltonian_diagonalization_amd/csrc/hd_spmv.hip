@@ -247,6 +247,14 @@ static int default_precision(const hd_hamiltonian* h)
 
 // Times every lanes-per-row variant on the matrix (in its fastest resident
 // format) and keeps the fastest.
+// Plain streaming copy, 16 bytes per lane per iteration (the byte-counter
+// calibration kernel).
+__global__ void __launch_bounds__(256) k_stream_copy(const int4* __restrict__ src, int4* __restrict__ dst, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 int spmv_autotune(hd_hamiltonian* h)
 {
     const int prec = default_precision(h);
@@ -354,6 +362,27 @@ int hd_spmv_set_lanes(hd_hamiltonian* h, int32_t lanes)
     if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
         return fail(HD_ERR_ARG, "lanes per row must be 4, 8, 16, 32 or 64");
     h->lanes_per_row = lanes;
+    return HD_OK;
+}
+
+int hd_stream_copy(const void* d_src, void* d_dst, size_t bytes, int32_t iters, double* avg_ms)
+{
+    if (!d_src || !d_dst || !avg_ms || iters <= 0 || bytes == 0 || bytes % 16)
+        return fail(HD_ERR_ARG, "hd_stream_copy: bad argument");
+    const int64_t n = (int64_t)(bytes / 16);
+    int dev = 0;
+    HD_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HD_HIP(hipGetDeviceProperties(&prop, dev));
+    const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)prop.multiProcessorCount * 16);
+    EventTimer timer;
+    timer.start(nullptr);
+    for (int i = 0; i < iters; ++i)
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, nullptr, (const int4*)d_src, (int4*)d_dst, n);
+    HD_HIP(hipGetLastError());
+    const float ms = timer.stop(nullptr);
+    HD_CHECK(timer.status("copy timing"));
+    *avg_ms = ms / iters;
     return HD_OK;
 }
 

@@ -1,0 +1,45 @@
+"""A short fixed workload for hardware-counter runs of the SpMV kernels.
+
+python -m hamiltonian_diagonalization_amd.profile_spmv [--n 12 --two-s 33 --iters 3]
+
+Builds the N=12, 2S=33 L_z=0 Coulomb sphere Hamiltonian with FP64, FP32 and
+coded values, then launches each SpMV format `iters` times (after one warm-up
+launch) and the plain streaming copy kernel over 1 GiB `iters` times, which
+calibrates FETCH_SIZE / WRITE_SIZE against a known byte count.  Prints the
+algorithmic bytes of every launch so that the counter totals can be compared.
+"""
+import argparse
+import ctypes
+import json
+
+from . import CODED, F32, F64, STORAGE_FULL, DeviceBuffer, FockBasis, Hamiltonian, Sector, _check, lib
+from . import models
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=12)
+    ap.add_argument("--two-s", type=int, default=33)
+    ap.add_argument("--iters", type=int, default=3)
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    args = ap.parse_args()
+    basis = FockBasis(Sector.lz(0, args.two_s), args.n, args.two_s + 1)
+    ham = Hamiltonian(basis, models.sphere_tables(args.two_s, models.coulomb_list(args.two_s)),
+                      storage=STORAGE_FULL, values=(F64, F32, CODED))
+    ham.set_lanes(0)
+    out = {"dim": basis.dim, "nnz": ham.nnz, "lanes": ham.lanes(), "iters": args.iters, "spmv": {}}
+    for name, p in (("f64", F64), ("f32", F32), ("coded", CODED)):
+        ms = ham.spmv_time_ms(p, args.iters)
+        out["spmv"][name] = {"ms": ms, "alg_bytes": ham.spmv_bytes(p)}
+        print(f"[profile] spmv {name}: {ms:.3f} ms, {ham.spmv_bytes(p)} algorithmic bytes", flush=True)
+    src, dst = DeviceBuffer(args.copy_bytes), DeviceBuffer(args.copy_bytes)
+    v = ctypes.c_double()
+    _check(lib().hd_stream_copy(src.ptr, dst.ptr, args.copy_bytes, args.iters, ctypes.byref(v)))
+    out["copy"] = {"ms": v.value, "bytes_read": args.copy_bytes, "bytes_written": args.copy_bytes}
+    print(f"[profile] copy {args.copy_bytes} B: {v.value:.3f} ms "
+          f"({2 * args.copy_bytes / v.value / 1e6:.0f} GB/s read+write)", flush=True)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -199,6 +199,11 @@ int hd_spmv_host(const hd_hamiltonian* h, const void* h_x, void* h_y);
  * vectors allocated by the library and reports the mean kernel time from HIP
  * events recorded on the handle's stream. */
 int hd_spmv_time(hd_hamiltonian* h, int32_t precision, int32_t iters, double* avg_ms);
+/* Measurement helper (no reference counterpart): `iters` launches of a plain
+ * streaming copy kernel d_dst[i] = d_src[i] over `bytes` (a multiple of 16),
+ * the calibration reference for hardware byte counters; mean kernel time from
+ * HIP events on the default stream. */
+int hd_stream_copy(const void* d_src, void* d_dst, size_t bytes, int32_t iters, double* avg_ms);
 /* Rows per wavefront schedule of the SpMV (lanes per row 4..64, 0 = autotune). */
 int hd_spmv_set_lanes(hd_hamiltonian* h, int32_t lanes_per_row);
 int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes_per_row);
@@ -252,6 +257,59 @@ int hd_lanczos_vectors(const hd_hamiltonian* h, int32_t i, void* d_out, void* h_
  * dsyevr/zheevr, hamiltonian_base.hpp:634-758, lapack_wrapper.hpp:566-649):
  * the full CRS is expanded on the host and diagonalised by Jacobi rotations. */
 int hd_full_diagonalize(hd_hamiltonian* h, int32_t nev, double* h_evals);
+
+/* ------------------------------------------------ batched sectors x points */
+
+/* No single reference counterpart: the reference diagonalises one L_z sector
+ * per pass of its sector loop (src/fqhe_sphere/main/
+ * fqhe_sphere_spinless_fermions_diagonalization.cpp:255-291 ->
+ * pseudopotential_model_base.hpp:235-285) and one parameter point per program
+ * run.  A batch holds n_sectors bases (same N and orbital count) and n_points
+ * real term tables (same layout and k table, different values); every
+ * (sector, point) pair is one problem.  The CRS pattern of each sector is built
+ * once (coded format, codes shared by all points), with an FP64 dictionary and
+ * diagonal per point that are bit-identical to single builds. */
+typedef struct hd_batch hd_batch;
+int hd_batch_create(const hd_basis* const* bases, int32_t n_sectors, const hd_tables* const* tables,
+                    int32_t n_points, double zero_tol, hd_batch** out);
+int hd_batch_destroy(hd_batch* b);
+/* Rows (sum of the sector dimensions), stored entries, dictionary tuples and
+ * the padded point count Pp (16, 32 or 64) of the point-fastest vectors. */
+int hd_batch_info(const hd_batch* b, int64_t* rows, int64_t* nnz, int32_t* ndict, int32_t* points_padded);
+/* Host copies of the concatenated pattern: row pointers (rows+1), columns
+ * (global rows), codes (bit 15 sign, low 15 bits tuple index, 0xFFFF
+ * diagonal), the dictionary (ndict x Pp) and the diagonals (rows x Pp).  Any
+ * pointer may be NULL. */
+int hd_batch_copy_pattern(const hd_batch* b, int64_t* h_rowptr, int32_t* h_col, uint16_t* h_code, double* h_dict,
+                          double* h_diag);
+/* Y[r][p] = (H_p X)[r] for every point on device arrays of rows x Pp doubles. */
+int hd_batch_spmm(hd_batch* b, const double* d_x, double* d_y);
+/* Measurement helper: mean time of one batched SpMM over all points. */
+int hd_batch_spmm_time(hd_batch* b, int32_t iters, double* avg_ms);
+
+typedef struct {
+    int32_t nev;          /* eigenvalues per problem (sectors smaller than nev give NaN beyond their dimension) */
+    int32_t ncv;          /* Krylov basis size, 0 = 2*nev+21; at most 64 */
+    int32_t max_restarts; /* 0 = 2000 */
+    double tol;           /* residual ||Hx - lx|| <= tol * max(1,|l|); 0 = 1e-10 */
+    uint64_t seed;        /* random-normal start of every problem */
+    int32_t dense_below;  /* sectors below this dimension are diagonalised densely; 0 = 60
+                             (BaseDiagonalize, pseudopotential_model_base.hpp:243-252); at most 65 */
+    int32_t use_graph;    /* replay the restart cycle as a captured hipGraph */
+} hd_batch_opts;
+
+typedef struct {
+    int32_t problems, lanczos_problems, dense_problems, converged;
+    int32_t restarts;     /* restart cycles after the first */
+    int32_t steps;        /* Lanczos steps (operator applications per problem) */
+    double max_residual;  /* largest true residual over the Lanczos problems */
+    double time_total_ms;
+} hd_batch_stats;
+
+/* The nev lowest eigenvalues of every problem, h_evals[(s*n_points + p)*nev + i];
+ * h_resid (optional) the true residuals ||Hx - lx||/max(1,|l|) (0 on the
+ * dense path).  HD_ERR_NOT_CONVERGED when a problem misses tol. */
+int hd_batch_solve(hd_batch* b, const hd_batch_opts* opts, double* h_evals, double* h_resid, hd_batch_stats* stats);
 
 /* ------------------------------------------------------ learned start vector */
 
