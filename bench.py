@@ -98,6 +98,21 @@ def cpu_baseline(n, two_s, bas, tables, matvecs, sample_rows, threads):
     }
 
 
+def measured_traffic(kernel, n, two_s):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (calibrated against a plain copy kernel),
+    or None when that kernel and workload were not profiled."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"spmv_n{n}_2s{two_s}_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return int(f[2])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +201,8 @@ def main():
     key = "coded" if prec == hd.CODED else "f64"
     spmv_live_ms = live_ms / max(live_mv, 1)
     achieved = sp_b[key] / (spmv_live_ms * 1e-3) / 1e9
+    kernel = f"k_spmv_coded<double,{lanes}>" if prec == hd.CODED else f"k_spmv<double,{lanes}>"
+    traffic = measured_traffic(kernel, n, two_s)
     out = {
         "metric": METRIC,
         "value": round(step_s, 4),
@@ -226,8 +243,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": f"k_spmv_coded<double,{lanes}>" if prec == hd.CODED else f"k_spmv<double,{lanes}>",
+            "traffic": traffic,
+            "kernel": kernel,
             "bytes_per_launch": sp_b[key],
             "avg_launch_ms": round(spmv_live_ms, 4),
             "launches_timed": live_mv,
