@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--two-s", type=int, default=33)
     ap.add_argument("--nev", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--ncv", type=int, default=48,
+                    help="Krylov basis size (the reference's ARPACK default is 2*nev+21)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-compare", action="store_true",
                     help="skip the FP64-CRS and FP32-mixed comparison solves")
@@ -150,7 +152,7 @@ def main():
         f"(count {info['t_count_ms']:.1f} scan {info['t_scan_ms']:.1f} fill {info['t_fill_ms']:.1f} "
         f"sort {info['t_sort_ms']:.1f} ms)")
     ham.set_lanes(0)
-    lanes = ham.lanes()
+    lanes, spmv_threads = ham.lanes(), ham.threads()
     fmts = {"coded": hd.CODED, "f32": hd.F32, "f64": hd.F64}
     sp_ms = {k: ham.spmv_time_ms(p, 20) for k, p in fmts.items()}
     sp_b = {k: ham.spmv_bytes(p) for k, p in fmts.items()}
@@ -162,7 +164,7 @@ def main():
     prec = hd.CODED if sp_ms["coded"] < sp_ms["f64"] else hd.F64
     op_name = "coded_fp64" if prec == hd.CODED else "fp64_crs"
     for w in range(args.warmup):
-        r = ham.lanczos(nev=args.nev, tol=args.tol, precision=prec, seed=0)
+        r = ham.lanczos(nev=args.nev, tol=args.tol, precision=prec, seed=0, ncv=args.ncv)
         log(f"warmup {w}: {r}")
     hd.lib().hd_synchronize()
     res = None
@@ -170,7 +172,7 @@ def main():
     t_all = time.perf_counter()
     for s in range(args.steps):
         t0 = time.perf_counter()
-        res = ham.lanczos(nev=args.nev, tol=args.tol, precision=prec, seed=0)
+        res = ham.lanczos(nev=args.nev, tol=args.tol, precision=prec, seed=0, ncv=args.ncv)
         hd.lib().hd_synchronize()
         live_ms += res.time_spmv_ms
         live_mv += res.matvecs
@@ -182,7 +184,7 @@ def main():
     if not args.skip_compare:
         others = [("coded_fp64", hd.CODED)] if prec == hd.F64 else [("fp64_crs", hd.F64)]
         for name, p in others + [("f32_mixed", hd.F32)]:
-            r = ham.lanczos(nev=args.nev, tol=args.tol, precision=p, seed=0)
+            r = ham.lanczos(nev=args.nev, tol=args.tol, precision=p, seed=0, ncv=args.ncv)
             log(f"{name}: {r}")
             cmp[name] = r
     ref = cmp.get("fp64_crs") or cmp.get("coded_fp64")
@@ -201,7 +203,8 @@ def main():
     key = "coded" if prec == hd.CODED else "f64"
     spmv_live_ms = live_ms / max(live_mv, 1)
     achieved = sp_b[key] / (spmv_live_ms * 1e-3) / 1e9
-    kernel = f"k_spmv_coded<double,{lanes}>" if prec == hd.CODED else f"k_spmv<double,{lanes}>"
+    kernel = (f"k_spmv_coded<double,{lanes},{spmv_threads}>" if prec == hd.CODED
+              else f"k_spmv<double,{lanes},{spmv_threads}>")
     traffic = measured_traffic(kernel, n, two_s)
     out = {
         "metric": METRIC,
@@ -221,8 +224,9 @@ def main():
                         f"residual {args.tol:g}",
             "dim": basis.dim,
             "nnz_full": info["nnz"],
-            "ncv": 2 * args.nev + 21,
+            "ncv": args.ncv,
             "lanes_per_row": lanes,
+            "threads_per_block": spmv_threads,
             "operator": ("exact FP64 values in the coded CRS (int32 column + 16-bit dictionary code)"
                          if prec == hd.CODED else "FP64 CRS (int64 row pointers, int32 columns, FP64 values)"),
         },

@@ -63,7 +63,8 @@ class _LanczosOpts(ctypes.Structure):
                 ("max_restarts", ctypes.c_int32), ("tol", ctypes.c_double),
                 ("start", ctypes.c_int32), ("seed", ctypes.c_uint64),
                 ("d_start", ctypes.c_void_p), ("precision", ctypes.c_int32),
-                ("f32_switch_tol", ctypes.c_double), ("want_vectors", ctypes.c_int32)]
+                ("f32_switch_tol", ctypes.c_double), ("want_vectors", ctypes.c_int32),
+                ("block", ctypes.c_int32)]
 
 
 class _LanczosStats(ctypes.Structure):
@@ -122,6 +123,7 @@ SIGNATURES = {
     "hd_spmv": (ctypes.c_int, [_P, _I32, _P, _P]),
     "hd_spmv_host": (ctypes.c_int, [_P, _P, _P]),
     "hd_spmv_time": (ctypes.c_int, [_P, _I32, _I32, ctypes.POINTER(_D)]),
+    "hd_spmv_get_threads": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
     "hd_stream_copy": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _I32, ctypes.POINTER(_D)]),
     "hd_spmv_set_lanes": (ctypes.c_int, [_P, _I32]),
     "hd_spmv_get_lanes": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
@@ -141,6 +143,7 @@ SIGNATURES = {
                                          ctypes.POINTER(ctypes.c_float)]),
     "hd_mlp_start_vector": (ctypes.c_int, [_P, _P, _P, _P]),
     "hd_mlp_gemm_check": (ctypes.c_int, [_I32, _I32, _I32, _P, _P, _P]),
+    "hd_sym_eig": (ctypes.c_int, [_I32, _P, _P, _P]),
     "hd_batch_create": (ctypes.c_int, [_P, _I32, _P, _I32, _D, _P]),
     "hd_batch_destroy": (ctypes.c_int, [_P]),
     "hd_batch_info": (ctypes.c_int, [_P, _P, _P, _P, _P]),
@@ -437,11 +440,19 @@ class Hamiltonian:
         _check(lib().hd_spmv_get_lanes(self.h, ctypes.byref(v)))
         return v.value
 
+    def threads(self) -> int:
+        """Threads per block of the SpMV schedule in use."""
+        v = ctypes.c_int32()
+        _check(lib().hd_spmv_get_threads(self.h, ctypes.byref(v)))
+        return v.value
+
     def lanczos(self, nev=5, tol=1e-10, ncv=0, precision=F64, seed=0, start=None,
-                f32_switch_tol=0.0, max_restarts=0, raise_on_fail=True) -> LanczosResult:
+                f32_switch_tol=0.0, max_restarts=0, raise_on_fail=True, block=1) -> LanczosResult:
+        """block > 1 selects the block Lanczos (4, 8 or 16 vectors per SpMM)."""
         o = _LanczosOpts()
         o.nev, o.ncv, o.max_restarts, o.tol = nev, ncv, max_restarts, tol
         o.seed, o.precision, o.f32_switch_tol, o.want_vectors = seed, precision, f32_switch_tol, 1
+        o.block = block
         dstart = None
         if start is None:
             o.start = START_RANDOM
@@ -543,3 +554,14 @@ class BatchProblem:
         if getattr(self, "h", None) is not None and _lib is not None:
             _lib.hd_batch_destroy(self.h)
             self.h = None
+
+
+def sym_eig(a: np.ndarray):
+    """(eigenvalues ascending, eigenvectors as columns) of a symmetric matrix by
+    the library's host eigensolver (hd_sym_eig)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    w = np.zeros(n)
+    z = np.zeros((n, n))
+    _check(lib().hd_sym_eig(n, _ptr(a), _ptr(w), _ptr(z)))
+    return w, z

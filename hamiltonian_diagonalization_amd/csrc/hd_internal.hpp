@@ -249,7 +249,14 @@ struct hd_tables {
     std::vector<double> eps;   // empty when no quadratic table
 };
 
-struct hd_lanczos_state;
+// Eigenpairs of the last Lanczos call on a Hamiltonian handle.
+struct hd_lanczos_state {
+    int64_t dim = 0;
+    int cplx = 0;
+    int nev = 0;
+    void* d_vec = nullptr;  // nev vectors, contiguous
+    std::vector<double> evals;
+};
 
 struct hd_hamiltonian {
     int device = 0;
@@ -267,7 +274,8 @@ struct hd_hamiltonian {
     void* d_diag = nullptr;   // double or double2, dim entries
     void* d_dict = nullptr;   // double or double2, ndict entries
     int ndict = 0;
-    int lanes_per_row = 0;    // SpMV schedule (0 = pick on first use)
+    int lanes_per_row = 0;    // SpMV lanes per row (0 = pick on first use)
+    int spmv_sched = -1;      // SpMV schedule index (lanes, threads); -1 = default
     double t_count = 0, t_scan = 0, t_fill = 0, t_sort = 0;
     size_t bytes = 0;
     hd_lanczos_state* lz = nullptr;  // eigenvectors of the last Lanczos call
@@ -289,6 +297,13 @@ int build_batch_sector(const hd_basis* b, const hd_tables* const* tables, int n,
                        hd_hamiltonian** out);
 int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y,
                 hipStream_t stream);
+// Cyclic Jacobi on a symmetric row-major matrix (destroyed): ascending
+// eigenvalues w, eigenvectors as the columns of Z (hd_lanczos.hip).
+void jacobi_eigh(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& Z);
+// Householder + implicit QL, same contract (hd_eigsym.cpp); false if QL fails.
+bool sym_eig(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& Z);
+// Block thick-restart Lanczos for opts.block > 1 (hd_block.hip).
+int run_block_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* evals, hd_lanczos_stats* stats);
 int spmv_autotune(hd_hamiltonian* h);
 void lanczos_state_free(hd_lanczos_state* s);
 }  // namespace hd

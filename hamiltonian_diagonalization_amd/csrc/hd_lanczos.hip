@@ -31,13 +31,6 @@
 
 #include "hd_internal.hpp"
 
-struct hd_lanczos_state {
-    int64_t dim = 0;
-    int cplx = 0;
-    int nev = 0;
-    void* d_vec = nullptr;  // nev vectors, contiguous
-    std::vector<double> evals;
-};
 
 namespace hd {
 
@@ -79,6 +72,8 @@ HD_DEV double2 widen(float2 a) { return make_double2((double)a.x, (double)a.y); 
 
 constexpr int kVB = 256;  // block size of the vector kernels
 constexpr int kNV = 8;    // vectors per dot-product pass
+constexpr int kMaxBasis = 256;               // Krylov basis limit (ncv)
+constexpr int kNormSlot = kMaxBasis + 8;     // coefficient slot of the staged norm
 
 // partial[j * nb + block] = sum_r conj(V_j[r]) w[r]  for j in [j0, j0+nv);
 // with want_norm, partial[(kNV) * nb + block] = sum |w|^2 (as .x).
@@ -146,7 +141,7 @@ k_project_out(int64_t dim, const typename E<C>::T* __restrict__ V, int64_t ld, i
               double* __restrict__ partial)
 {
     using T = typename E<C>::T;
-    __shared__ T sc[64];
+    __shared__ T sc[kMaxBasis];
     __shared__ double red[kVB];
     for (int i = threadIdx.x; i < nv; i += kVB) sc[i] = coef[i];
     __syncthreads();
@@ -197,7 +192,7 @@ k_combine(int64_t dim, const typename E<C>::T* __restrict__ V, int64_t ld, int m
           const double* __restrict__ Y, int ldy, int l0, int nl, typename E<C>::T* __restrict__ out)
 {
     using T = typename E<C>::T;
-    __shared__ double sy[64 * 16];
+    __shared__ double sy[kMaxBasis * 16];
     for (int i = threadIdx.x; i < m * nl; i += kVB) sy[i] = Y[(i / nl) * ldy + l0 + (i % nl)];
     __syncthreads();
     const int64_t stride = (int64_t)gridDim.x * kVB;
@@ -393,7 +388,7 @@ struct Lanczos {
     F* x32 = nullptr;
     F* y32 = nullptr;
     T* partial = nullptr;
-    T* coef = nullptr;  // device coefficient vector (<= 64 + 1)
+    T* coef = nullptr;  // device coefficients (kMaxBasis) + staged norm at kNormSlot
     double* dpart = nullptr;
     double* dscal = nullptr;
     double* dY = nullptr;
@@ -420,10 +415,10 @@ struct Lanczos {
             HD_CHECK(dalloc_t(&y32, (size_t)n, "FP32 y"));
         }
         HD_CHECK(dalloc_t(&partial, (size_t)(kNV + 1) * nb, "dot partials"));
-        HD_CHECK(dalloc_t(&coef, 80, "coefficients"));
+        HD_CHECK(dalloc_t(&coef, kNormSlot + 8, "coefficients"));
         HD_CHECK(dalloc_t(&dpart, (size_t)nb, "norm partials"));
         HD_CHECK(dalloc_t(&dscal, 4, "scalars"));
-        HD_CHECK(dalloc_t(&dY, 64 * 64, "Ritz coefficients"));
+        HD_CHECK(dalloc_t(&dY, (size_t)kMaxBasis * kMaxBasis, "Ritz coefficients"));
         return HD_OK;
     }
 
@@ -458,15 +453,16 @@ struct Lanczos {
             const int want = (wnorm2 && j0 == 0) ? 1 : 0;
             hipLaunchKernelGGL(k_dots<C>, dim3(nb), dim3(kVB), 0, st, n, V, n, j0, cnt, w, partial, want);
             hipLaunchKernelGGL(k_reduce<C>, dim3(cnt + want), dim3(kVB), 0, st, partial, nb, cnt + want,
-                               coef + (want ? 72 - cnt : j0));
+                               coef + (want ? kNormSlot - cnt : j0));
             if (want) {
-                // the norm lands in coef[72]; move the dots to their slots
-                HD_HIP(hipMemcpyAsync(coef + j0, coef + 72 - cnt, sizeof(T) * cnt, hipMemcpyDeviceToDevice, st));
+                // the norm lands in coef[kNormSlot]; move the dots to their slots
+                HD_HIP(hipMemcpyAsync(coef + j0, coef + kNormSlot - cnt, sizeof(T) * cnt, hipMemcpyDeviceToDevice,
+                                      st));
             }
         }
         HD_HIP(hipGetLastError());
         HD_HIP(hipMemcpyAsync(h_out.data(), coef, sizeof(T) * nv, hipMemcpyDeviceToHost, st));
-        if (wnorm2) HD_HIP(hipMemcpyAsync(&h_out[nv], coef + 72, sizeof(T), hipMemcpyDeviceToHost, st));
+        if (wnorm2) HD_HIP(hipMemcpyAsync(&h_out[nv], coef + kNormSlot, sizeof(T), hipMemcpyDeviceToHost, st));
         HD_HIP(hipStreamSynchronize(st));
         if (wnorm2) *wnorm2 = *reinterpret_cast<const double*>(&h_out[nv]);
         h_out.resize(nv);
@@ -596,7 +592,7 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
     const int nev = (int)std::min<int64_t>(o.nev, h->dim);
     int m = o.ncv > 0 ? o.ncv : 2 * nev + 21;
     m = (int)std::min<int64_t>(m, h->dim);
-    m = std::min(m, 64);
+    m = std::min(m, kMaxBasis);
     if (m < nev + 2) return fail(HD_ERR_ARG, "ncv must be at least nev + 2 (and dim > nev + 1)");
     L.m = m;
     int n_cu = 256;
@@ -676,7 +672,7 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
             Tm[(size_t)j * m + j] = alpha[j];
             if (j + 1 < m) Tm[(size_t)j * m + j + 1] = Tm[(size_t)(j + 1) * m + j] = beta[j];
         }
-        jacobi_eigh(m, Tm, w, Y);
+        if (!sym_eig(m, Tm, w, Y)) return fail(HD_ERR_NOT_CONVERGED, "projected eigenproblem did not converge");
         const double bm = beta[m - 1];
         bool all = true;
         for (int i = 0; i < nev; ++i) {
@@ -771,6 +767,7 @@ int hd_lanczos(hd_hamiltonian* h, const hd_lanczos_opts* opts, double* h_evals, 
     if (opts->nev <= 0) return fail(HD_ERR_ARG, "nev must be positive");
     if (h->storage != HD_STORAGE_FULL) return fail(HD_ERR_ARG, "Lanczos needs full storage");
     if (!h->lanes_per_row) HD_CHECK(spmv_autotune(h));
+    if (opts->block > 1) return run_block_lanczos(h, *opts, h_evals, stats);
     return h->is_complex ? run_lanczos<true>(h, *opts, h_evals, stats)
                          : run_lanczos<false>(h, *opts, h_evals, stats);
 }

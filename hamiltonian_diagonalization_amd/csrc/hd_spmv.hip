@@ -5,12 +5,13 @@
 // applying y = U x + (U - D)^H x (src/utilities/mathematics/sparse_linear_algebra.hpp:142-165,
 // :218-312).  The scattered y[col] += ... of the transposed half cannot run
 // race-free in parallel, so the device keeps both triangles and every row is a
-// pure gather: LPR lanes of a wavefront (4..64, autotuned per matrix) stream
-// the row's column indices and values with scalar non-temporal loads (the
-// stream is read once and must not evict x from L2/MALL), gather x, and reduce
-// with cross-lane shuffles.  Measured on MI355X the kernel is bound by the x
-// gathers, not by the stream: 16-byte vector loads and persistent grids were
-// slower than this one-block-per-chunk form for FP64 and FP32 values.
+// pure gather: LPR lanes of a wavefront stream the row's column indices and
+// values with scalar non-temporal loads (the stream is read once and must not
+// evict x from L2/MALL), gather x, and reduce with cross-lane shuffles.  One
+// block handles one chunk of NT/LPR rows; (LPR, NT) is autotuned per matrix.
+// Measured on MI355X the kernel is bound by the L2-miss traffic of the x
+// gathers, not by the stream, so the block -> chunk order is chosen for L2
+// reuse (window_chunk); 16-byte vector loads and persistent grids were slower.
 //
 // Coded format (HD_CODED): every off-diagonal value is sign * P[hop] from the
 // build's pair table, and the distinct P values are few (1,496 for the
@@ -74,33 +75,32 @@ template <> HD_DEV float2 ntload<float2>(const float2* p)
     return make_float2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
 }
 
-constexpr int kSpmvBlock = 256;
+// Block -> chunk order.  Blocks are dispatched in index order and block b runs
+// on XCD b % 8.  Within each window of 8 * kWinG blocks, XCD x takes kWinG
+// consecutive chunks, so every XCD's L2 sees a contiguous band of rows whose
+// column sets overlap; the window keeps the rows in flight across the chip
+// narrow.  (Measured on the N=12, 2S=33 sphere: 9.1 ms against 10.6 ms for the
+// plain order and 12.6 ms for a persistent XCD-partitioned walk.)
+constexpr int64_t kWinG = 1024;
 
-// Persistent grid: the blocks on XCD x (block b runs on XCD b % 8) walk the
-// chunks [x*nc/8, (x+1)*nc/8) round-robin, so each XCD's L2 serves one
-// contiguous range of rows.
-struct ChunkWalk {
-    int64_t c, c1, step;
-    HD_DEV ChunkWalk(int64_t nchunks)
-    {
-        const int x = blockIdx.x & 7;
-        const int64_t nbx = ((int64_t)gridDim.x - x + 7) / 8;
-        c = (int64_t)x * nchunks / 8 + (blockIdx.x >> 3);
-        c1 = (int64_t)(x + 1) * nchunks / 8;
-        step = nbx;
-    }
-};
+HD_DEV int64_t window_chunk(int64_t b, int64_t nb)
+{
+    const int64_t W = 8 * kWinG;
+    const int64_t base = (b / W) * W;
+    if (base + W > nb) return b;  // the last partial window keeps the plain order
+    const int64_t o = b - base;
+    return base + (o & 7) * kWinG + (o >> 3);
+}
 
-template <typename V, int LPR>
-__global__ void __launch_bounds__(kSpmvBlock)
+// y = H x, one block per chunk of NT / LPR rows, LPR lanes per row.
+template <typename V, int LPR, int NT>
+__global__ void __launch_bounds__(NT)
 k_spmv(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
        const V* __restrict__ val, const V* __restrict__ x, V* __restrict__ y)
 {
-    // one block per chunk of RPB rows: blocks are dispatched in order, so the
-    // rows in flight at any time form a narrow window whose x entries overlap
-    constexpr int RPB = kSpmvBlock / LPR;
+    constexpr int RPB = NT / LPR;
     const int sub = threadIdx.x & (LPR - 1);
-    const int64_t row = (int64_t)blockIdx.x * RPB + threadIdx.x / LPR;
+    const int64_t row = window_chunk(blockIdx.x, gridDim.x) * RPB + threadIdx.x / LPR;
     V acc = zero_of<V>();
     if (row < dim) {
         const int64_t start = rowptr[row], end = rowptr[row + 1];
@@ -112,20 +112,20 @@ k_spmv(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restric
     if (row < dim && sub == 0) y[row] = acc;
 }
 
-template <typename V, int LPR>
-__global__ void __launch_bounds__(kSpmvBlock)
+// Coded values: the block copies the dictionary into LDS, then as k_spmv.
+template <typename V, int LPR, int NT>
+__global__ void __launch_bounds__(NT)
 k_spmv_coded(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
              const uint16_t* __restrict__ code, const V* __restrict__ diag, const V* __restrict__ dict,
              int ndict, const V* __restrict__ x, V* __restrict__ y)
 {
     extern __shared__ int32_t lds_raw[];
     V* sdict = reinterpret_cast<V*>(lds_raw);
-    for (int i = threadIdx.x; i < ndict; i += kSpmvBlock) sdict[i] = dict[i];
+    for (int i = threadIdx.x; i < ndict; i += NT) sdict[i] = dict[i];
     __syncthreads();
-    constexpr int RPB = kSpmvBlock / LPR;
+    constexpr int RPB = NT / LPR;
     const int sub = threadIdx.x & (LPR - 1);
-    for (ChunkWalk w((dim + RPB - 1) / RPB); w.c < w.c1; w.c += w.step) {
-    const int64_t row = w.c * RPB + threadIdx.x / LPR;
+    const int64_t row = window_chunk(blockIdx.x, gridDim.x) * RPB + threadIdx.x / LPR;
     V acc = zero_of<V>();
     if (row < dim) {
         const int64_t start = rowptr[row], end = rowptr[row + 1];
@@ -143,96 +143,102 @@ k_spmv_coded(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __r
 #pragma unroll
     for (int m = LPR / 2; m > 0; m >>= 1) acc = add(acc, shfl_xor(acc, m));
     if (row < dim && sub == 0) y[row] = acc;
-    }
 }
 
-// Persistent grid of 8 blocks per CU, capped by the number of row chunks but
-// never below 8 so that every XCD's chunk range has a block walking it.
-static unsigned persistent_grid(int64_t nchunks)
-{
-    static int n_cu = 0;
-    if (!n_cu && device_props(&n_cu, nullptr) != HD_OK) n_cu = 256;
-    return (unsigned)std::max<int64_t>(8, std::min<int64_t>(nchunks, (int64_t)n_cu * 8));
-}
+// Schedules (lanes per row, threads per block) the autotune chooses from.
+struct Sched {
+    int lpr, nt;
+};
+constexpr Sched kScheds[] = {{4, 256},  {8, 256},  {16, 256}, {16, 512}, {32, 256},
+                             {32, 512}, {32, 1024}, {64, 512}, {64, 1024}};
+constexpr int kNumScheds = (int)(sizeof(kScheds) / sizeof(kScheds[0]));
 
-template <typename V, int LPR>
+template <typename V, int LPR, int NT>
 static void launch_one(const hd_hamiltonian* h, const void* val, const void* x, void* y, hipStream_t s)
 {
-    constexpr int RPB = kSpmvBlock / LPR;
+    constexpr int RPB = NT / LPR;
     const unsigned grid = (unsigned)((h->dim + RPB - 1) / RPB);
-    hipLaunchKernelGGL((k_spmv<V, LPR>), dim3(grid), dim3(kSpmvBlock), 0, s, h->dim, h->d_rowptr,
-                       h->d_col, static_cast<const V*>(val), static_cast<const V*>(x), static_cast<V*>(y));
+    hipLaunchKernelGGL((k_spmv<V, LPR, NT>), dim3(grid), dim3(NT), 0, s, h->dim, h->d_rowptr, h->d_col,
+                       static_cast<const V*>(val), static_cast<const V*>(x), static_cast<V*>(y));
 }
 
-template <typename V, int LPR>
+template <typename V, int LPR, int NT>
 static void launch_coded(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
 {
-    constexpr int RPB = kSpmvBlock / LPR;
-    const unsigned grid = persistent_grid((h->dim + RPB - 1) / RPB);
+    constexpr int RPB = NT / LPR;
+    const unsigned grid = (unsigned)((h->dim + RPB - 1) / RPB);
     const size_t lds = sizeof(V) * (size_t)std::max(h->ndict, 1);
-    hipLaunchKernelGGL((k_spmv_coded<V, LPR>), dim3(grid), dim3(kSpmvBlock), lds, s, h->dim,
-                       h->d_rowptr, h->d_col, h->d_code, static_cast<const V*>(h->d_diag),
-                       static_cast<const V*>(h->d_dict), h->ndict, static_cast<const V*>(x),
-                       static_cast<V*>(y));
+    hipLaunchKernelGGL((k_spmv_coded<V, LPR, NT>), dim3(grid), dim3(NT), lds, s, h->dim, h->d_rowptr, h->d_col,
+                       h->d_code, static_cast<const V*>(h->d_diag), static_cast<const V*>(h->d_dict), h->ndict,
+                       static_cast<const V*>(x), static_cast<V*>(y));
 }
 
-template <typename V>
-static void launch_lpr(int lpr, const hd_hamiltonian* h, const void* val, const void* x, void* y,
-                       hipStream_t s)
+template <typename V, bool CODED>
+static void launch_sched(int si, const hd_hamiltonian* h, const void* val, const void* x, void* y, hipStream_t s)
 {
-    switch (lpr) {
-    case 4: launch_one<V, 4>(h, val, x, y, s); break;
-    case 8: launch_one<V, 8>(h, val, x, y, s); break;
-    case 16: launch_one<V, 16>(h, val, x, y, s); break;
-    case 32: launch_one<V, 32>(h, val, x, y, s); break;
-    default: launch_one<V, 64>(h, val, x, y, s); break;
+#define HD_SCHED(I, L, T)                                   \
+    case I:                                                 \
+        if constexpr (CODED) launch_coded<V, L, T>(h, x, y, s); \
+        else launch_one<V, L, T>(h, val, x, y, s);          \
+        break;
+    switch (si) {
+        HD_SCHED(0, 4, 256)
+        HD_SCHED(1, 8, 256)
+        HD_SCHED(2, 16, 256)
+        HD_SCHED(3, 16, 512)
+        HD_SCHED(4, 32, 256)
+        HD_SCHED(6, 32, 1024)
+        HD_SCHED(7, 64, 512)
+        HD_SCHED(8, 64, 1024)
+    default:  // 5: 32 lanes, 512 threads
+        if constexpr (CODED) launch_coded<V, 32, 512>(h, x, y, s);
+        else launch_one<V, 32, 512>(h, val, x, y, s);
+        break;
     }
+#undef HD_SCHED
 }
 
-template <typename V>
-static void launch_coded_lpr(int lpr, const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
+// schedule index for a lanes-per-row request: 512 threads from 16 lanes up
+static int sched_for_lanes(int lpr)
 {
-    switch (lpr) {
-    case 4: launch_coded<V, 4>(h, x, y, s); break;
-    case 8: launch_coded<V, 8>(h, x, y, s); break;
-    case 16: launch_coded<V, 16>(h, x, y, s); break;
-    case 32: launch_coded<V, 32>(h, x, y, s); break;
-    default: launch_coded<V, 64>(h, x, y, s); break;
-    }
+    const int nt = lpr >= 16 ? 512 : 256;
+    for (int i = 0; i < kNumScheds; ++i)
+        if (kScheds[i].lpr == lpr && kScheds[i].nt == nt) return i;
+    return 5;
 }
 
-static int pick_default_lanes(const hd_hamiltonian* h)
+static int pick_default_sched(const hd_hamiltonian* h)
 {
-    // each lane takes 4 entries per step
     const double avg = h->dim ? (double)h->nnz / (double)h->dim : 0.0;
-    if (avg > 192) return 64;
-    if (avg > 96) return 32;
-    if (avg > 48) return 16;
-    if (avg > 24) return 8;
-    return 4;
+    if (avg > 96) return 5;  // 32 lanes, 512 threads
+    if (avg > 48) return 3;
+    if (avg > 24) return 1;
+    return 0;
 }
 
 int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y, hipStream_t stream)
 {
     if (h->storage != HD_STORAGE_FULL)
         return fail(HD_ERR_ARG, "SpMV needs full (both-triangle) storage");
-    const int lpr = h->lanes_per_row ? h->lanes_per_row : pick_default_lanes(h);
+    const int si = h->spmv_sched >= 0 ? h->spmv_sched : pick_default_sched(h);
     if (h->dim == 0) return HD_OK;
     if (precision == HD_CODED) {
         if (!h->d_code) return fail(HD_ERR_ARG, "no coded values resident (value_mask & 4)");
-        if (h->is_complex) launch_coded_lpr<double2>(lpr, h, x, y, stream);
-        else launch_coded_lpr<double>(lpr, h, x, y, stream);
+        const size_t lds = (size_t)std::max(h->ndict, 1) * (h->is_complex ? 16 : 8);
+        if (lds > 64 * 1024) return fail(HD_ERR_UNSUPPORTED, "value dictionary larger than 64 KiB of LDS");
+        if (h->is_complex) launch_sched<double2, true>(si, h, nullptr, x, y, stream);
+        else launch_sched<double, true>(si, h, nullptr, x, y, stream);
         HD_HIP(hipGetLastError());
         return HD_OK;
     }
     const void* val = precision == HD_F32 ? h->d_val32 : h->d_val64;
     if (!val) return fail(HD_ERR_ARG, "requested SpMV precision has no resident values");
     if (precision == HD_F32) {
-        if (h->is_complex) launch_lpr<float2>(lpr, h, val, x, y, stream);
-        else launch_lpr<float>(lpr, h, val, x, y, stream);
+        if (h->is_complex) launch_sched<float2, false>(si, h, val, x, y, stream);
+        else launch_sched<float, false>(si, h, val, x, y, stream);
     } else {
-        if (h->is_complex) launch_lpr<double2>(lpr, h, val, x, y, stream);
-        else launch_lpr<double>(lpr, h, val, x, y, stream);
+        if (h->is_complex) launch_sched<double2, false>(si, h, val, x, y, stream);
+        else launch_sched<double, false>(si, h, val, x, y, stream);
     }
     HD_HIP(hipGetLastError());
     return HD_OK;
@@ -269,12 +275,11 @@ int spmv_autotune(hd_hamiltonian* h)
     HipErrors q;
     HD_ENQ(q, hipMemsetAsync(x, 0, (size_t)h->dim * es, h->stream));
     EventTimer timer;
-    const int cand[5] = {4, 8, 16, 32, 64};
     float best = 1e30f;
-    int best_l = 64;
-    for (int l : cand) {
+    int best_s = pick_default_sched(h);
+    for (int si = 0; si < kNumScheds; ++si) {
         if ((st = q.status()) != HD_OK) break;
-        h->lanes_per_row = l;
+        h->spmv_sched = si;
         if ((st = spmv_launch(h, prec, x, y, h->stream)) != HD_OK) break;  // warm
         timer.start(h->stream);
         for (int r = 0; r < 3 && st == HD_OK; ++r) st = spmv_launch(h, prec, x, y, h->stream);
@@ -283,10 +288,11 @@ int spmv_autotune(hd_hamiltonian* h)
         if (st != HD_OK) break;
         if (ms < best) {
             best = ms;
-            best_l = l;
+            best_s = si;
         }
     }
-    h->lanes_per_row = best_l;
+    h->spmv_sched = best_s;
+    h->lanes_per_row = kScheds[best_s].lpr;
     dfree(x);
     dfree(y);
     return st;
@@ -362,6 +368,7 @@ int hd_spmv_set_lanes(hd_hamiltonian* h, int32_t lanes)
     if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
         return fail(HD_ERR_ARG, "lanes per row must be 4, 8, 16, 32 or 64");
     h->lanes_per_row = lanes;
+    h->spmv_sched = sched_for_lanes(lanes);
     return HD_OK;
 }
 
@@ -389,7 +396,14 @@ int hd_stream_copy(const void* d_src, void* d_dst, size_t bytes, int32_t iters, 
 int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes)
 {
     if (!h || !lanes) return fail(HD_ERR_ARG, "null argument");
-    *lanes = h->lanes_per_row ? h->lanes_per_row : pick_default_lanes(h);
+    *lanes = kScheds[h->spmv_sched >= 0 ? h->spmv_sched : pick_default_sched(h)].lpr;
+    return HD_OK;
+}
+
+int hd_spmv_get_threads(const hd_hamiltonian* h, int32_t* threads)
+{
+    if (!h || !threads) return fail(HD_ERR_ARG, "null argument");
+    *threads = kScheds[h->spmv_sched >= 0 ? h->spmv_sched : pick_default_sched(h)].nt;
     return HD_OK;
 }
 

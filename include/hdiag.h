@@ -207,8 +207,17 @@ int hd_stream_copy(const void* d_src, void* d_dst, size_t bytes, int32_t iters, 
 /* Rows per wavefront schedule of the SpMV (lanes per row 4..64, 0 = autotune). */
 int hd_spmv_set_lanes(hd_hamiltonian* h, int32_t lanes_per_row);
 int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes_per_row);
+/* Threads per block of the SpMV schedule in use (256, 512 or 1024). */
+int hd_spmv_get_threads(const hd_hamiltonian* h, int32_t* threads);
 
 /* ---------------------------------------------------------------- Lanczos */
+
+/* The host eigensolver of the projected matrices (Householder tridiagonal
+ * reduction + implicit QL; the role of LAPACK dsyevr in the reference's dense
+ * path, lapack_wrapper.hpp:566-649): h_a is an n x n symmetric row-major
+ * matrix, h_w receives the eigenvalues ascending, h_z (optional) the
+ * eigenvectors as columns, row-major.  Runs on the CPU, no device needed. */
+int hd_sym_eig(int32_t n, const double* h_a, double* h_w, double* h_z);
 
 typedef enum {
     HD_START_RANDOM = 0, /* normal(0,1) entries from a counter-based generator */
@@ -227,6 +236,9 @@ typedef struct {
                              the residual reaches f32_switch_tol, then FP64 refinement */
     double f32_switch_tol;/* 0 = 1e-6 */
     int32_t want_vectors; /* keep the eigenvectors on the device (hd_lanczos_vectors) */
+    int32_t block;        /* 0 or 1: single-vector Lanczos.  4, 8 or 16: block Lanczos with
+                             that many vectors per SpMM (real matrices, FP64 or coded operator;
+                             ncv then counts columns, rounded up to whole blocks) */
 } hd_lanczos_opts;
 
 typedef struct {

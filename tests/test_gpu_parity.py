@@ -236,3 +236,33 @@ def test_lanczos_complex(oracle):
         say(prec, r)
         assert r.converged
         assert np.max(np.abs(r.evals - want) / np.maximum(1, np.abs(want))) < 1e-10
+
+
+@pytest.mark.parametrize("block", [4, 8, 16])
+def test_block_lanczos_config1(oracle, block):
+    say("block lanczos N=8 2S=21 nev=5 block", block)
+    b, t = models.sphere_problem(8, 21)
+    bas = b.states()
+    up = oracle.build(8, 22, bas, oracle.LAYOUT_SPHERE, 22, t.ktable, t.vtable)
+    full = oracle.build(8, 22, bas, oracle.LAYOUT_SPHERE, 22, t.ktable, t.vtable, full=True)
+    want = _dense_eigs(oracle, full, b.dim, 5)
+    h = hd.Hamiltonian(b, t, values=(hd.F64, hd.CODED))
+    for prec in (hd.CODED, hd.F64):
+        r = h.lanczos(nev=5, tol=1e-11, seed=0, block=block, precision=prec)
+        say(prec, r)
+        assert r.converged and np.all(r.residuals <= 1e-11)
+        assert np.max(np.abs(r.evals - want) / np.maximum(1, np.abs(want))) < 1e-10
+    x0 = h.eigenvector(0)
+    res = oracle.spmv_sym(up, x0) - r.evals[0] * x0
+    assert np.linalg.norm(res) < 1e-10 * max(1, abs(r.evals[0]))
+
+
+def test_block_lanczos_user_start():
+    say("block lanczos with a start vector in column 0 (N=6 2S=15)")
+    b, t = models.sphere_problem(6, 15)
+    h = hd.Hamiltonian(b, t, values=(hd.F64, hd.CODED))
+    r0 = h.lanczos(nev=1, tol=1e-11, seed=0)
+    r1 = h.lanczos(nev=1, tol=1e-11, seed=0, block=4, start=h.eigenvector(0))
+    say(r0, r1)
+    assert abs(r1.evals[0] - r0.evals[0]) < 1e-10 * max(1, abs(r0.evals[0]))
+    assert r1.matvecs <= 4 * 8
