@@ -113,6 +113,44 @@ def measured_traffic(kernel, n, two_s):
     return None
 
 
+def learned_start(args, hd, basis, ham, prec, plain):
+    """The committed amplitude network's start vector for this Hamiltonian's
+    (V1, V3, V5), then the same solve from it: FP64 operator and FP32-mixed.
+    Times include evaluating the network over the whole sector."""
+    from hamiltonian_diagonalization_amd import learned as lmod
+    from hamiltonian_diagonalization_amd import models
+    ckpt = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hamiltonian_diagonalization_amd", "checkpoints",
+                        f"start_n{args.n}_2s{args.two_s}.npz")
+    if not os.path.exists(ckpt):
+        return None
+    net = lmod.AmplitudeNet.load(ckpt)
+    vm = models.coulomb_list(args.two_s)
+    params = [vm[1], vm[3], vm[5]]
+    out = {"checkpoint": os.path.basename(ckpt), "params_v1_v3_v5": [float(v) for v in params],
+           "note": "the Coulomb point lies outside the training box V1 in [0.5,1.5], V3 in [0,0.5], V5 in [0,0.2]"}
+    hd.lib().hd_synchronize()
+    t0 = time.perf_counter()
+    start = net.start_vector(basis, params)
+    hd.lib().hd_synchronize()
+    t_net = time.perf_counter() - t0
+    v = start.to_array(np.float64, basis.dim)
+    psi0 = ham.eigenvector(0)  # ground state of the last converged solve
+    out["overlap_with_ground_state"] = float(abs(v @ psi0) / (np.linalg.norm(v) * np.linalg.norm(psi0)))
+    del v, psi0
+    out["network_s"] = round(t_net, 4)
+    for name, p in (("fp64", prec), ("f32_mixed", hd.F32)):
+        t0 = time.perf_counter()
+        r = ham.lanczos(nev=args.nev, tol=args.tol, precision=p, seed=0, ncv=args.ncv, start=start,
+                        raise_on_fail=False)
+        hd.lib().hd_synchronize()
+        dt = time.perf_counter() - t0 + t_net
+        out[name] = {"s": round(dt, 4), "matvecs": r.matvecs, "max_residual": float(r.residuals.max()),
+                     "max_abs_dE_vs_plain": float(np.max(np.abs(r.evals - plain.evals)))}
+        log(f"learned start, {name}: {dt:.3f} s {r}")
+    out["solve_s"] = out["fp64"]["s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +166,7 @@ def main():
     ap.add_argument("--skip-compare", action="store_true",
                     help="skip the FP64-CRS and FP32-mixed comparison solves")
     ap.add_argument("--cpu-rows", type=int, default=64000)
+    ap.add_argument("--skip-learned", action="store_true", help="skip the learned-start solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the sharded design is "
@@ -152,7 +191,6 @@ def main():
         f"(count {info['t_count_ms']:.1f} scan {info['t_scan_ms']:.1f} fill {info['t_fill_ms']:.1f} "
         f"sort {info['t_sort_ms']:.1f} ms)")
     ham.set_lanes(0)
-    lanes, spmv_threads = ham.lanes(), ham.threads()
     fmts = {"coded": hd.CODED, "f32": hd.F32, "f64": hd.F64}
     sp_ms = {k: ham.spmv_time_ms(p, 20) for k, p in fmts.items()}
     sp_b = {k: ham.spmv_bytes(p) for k, p in fmts.items()}
@@ -187,6 +225,7 @@ def main():
             r = ham.lanczos(nev=args.nev, tol=args.tol, precision=p, seed=0, ncv=args.ncv)
             log(f"{name}: {r}")
             cmp[name] = r
+    learned = learned_start(args, hd, basis, ham, prec, res) if not args.skip_learned else None
     ref = cmp.get("fp64_crs") or cmp.get("coded_fp64")
     err = None
     if ref is not None:
@@ -203,6 +242,7 @@ def main():
     key = "coded" if prec == hd.CODED else "f64"
     spmv_live_ms = live_ms / max(live_mv, 1)
     achieved = sp_b[key] / (spmv_live_ms * 1e-3) / 1e9
+    lanes, spmv_threads = ham.schedule(prec)
     kernel = (f"k_spmv_coded<double,{lanes},{spmv_threads}>" if prec == hd.CODED
               else f"k_spmv<double,{lanes},{spmv_threads}>")
     traffic = measured_traffic(kernel, n, two_s)
@@ -232,7 +272,8 @@ def main():
         },
         "solve_s": {op_name: round(step_s, 4)},
         "matvecs": {op_name: res.matvecs},
-        "learned_start_s": None,
+        "learned_start_s": learned["solve_s"] if learned else None,
+        "learned_start": learned,
         "max_residual": float(np.max(res.residuals)),
         "eigenvalues": [float(e) for e in res.evals],
         "abs_err_between_fp64_formats": err,

@@ -123,7 +123,7 @@ SIGNATURES = {
     "hd_spmv": (ctypes.c_int, [_P, _I32, _P, _P]),
     "hd_spmv_host": (ctypes.c_int, [_P, _P, _P]),
     "hd_spmv_time": (ctypes.c_int, [_P, _I32, _I32, ctypes.POINTER(_D)]),
-    "hd_spmv_get_threads": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
+    "hd_spmv_get_schedule": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "hd_stream_copy": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _I32, ctypes.POINTER(_D)]),
     "hd_spmv_set_lanes": (ctypes.c_int, [_P, _I32]),
     "hd_spmv_get_lanes": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
@@ -440,11 +440,11 @@ class Hamiltonian:
         _check(lib().hd_spmv_get_lanes(self.h, ctypes.byref(v)))
         return v.value
 
-    def threads(self) -> int:
-        """Threads per block of the SpMV schedule in use."""
-        v = ctypes.c_int32()
-        _check(lib().hd_spmv_get_threads(self.h, ctypes.byref(v)))
-        return v.value
+    def schedule(self, precision=F64):
+        """(lanes per row, threads per block) of the SpMV kernel for one value format."""
+        lanes, threads = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().hd_spmv_get_schedule(self.h, precision, ctypes.byref(lanes), ctypes.byref(threads)))
+        return lanes.value, threads.value
 
     def lanczos(self, nev=5, tol=1e-10, ncv=0, precision=F64, seed=0, start=None,
                 f32_switch_tol=0.0, max_restarts=0, raise_on_fail=True, block=1) -> LanczosResult:

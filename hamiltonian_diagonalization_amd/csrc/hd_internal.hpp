@@ -275,7 +275,7 @@ struct hd_hamiltonian {
     void* d_dict = nullptr;   // double or double2, ndict entries
     int ndict = 0;
     int lanes_per_row = 0;    // SpMV lanes per row (0 = pick on first use)
-    int spmv_sched = -1;      // SpMV schedule index (lanes, threads); -1 = default
+    int spmv_sched[3] = {-1, -1, -1};  // SpMV schedule per precision (hd_precision); -1 = default
     double t_count = 0, t_scan = 0, t_fill = 0, t_sort = 0;
     size_t bytes = 0;
     hd_lanczos_state* lz = nullptr;  // eigenvectors of the last Lanczos call

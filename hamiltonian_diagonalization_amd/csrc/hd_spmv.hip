@@ -220,7 +220,8 @@ int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y, 
 {
     if (h->storage != HD_STORAGE_FULL)
         return fail(HD_ERR_ARG, "SpMV needs full (both-triangle) storage");
-    const int si = h->spmv_sched >= 0 ? h->spmv_sched : pick_default_sched(h);
+    if (precision < 0 || precision > 2) return fail(HD_ERR_ARG, "unknown SpMV precision");
+    const int si = h->spmv_sched[precision] >= 0 ? h->spmv_sched[precision] : pick_default_sched(h);
     if (h->dim == 0) return HD_OK;
     if (precision == HD_CODED) {
         if (!h->d_code) return fail(HD_ERR_ARG, "no coded values resident (value_mask & 4)");
@@ -263,8 +264,8 @@ __global__ void __launch_bounds__(256) k_stream_copy(const int4* __restrict__ sr
 
 int spmv_autotune(hd_hamiltonian* h)
 {
-    const int prec = default_precision(h);
-    const size_t es = (prec == HD_F32 ? 4 : 8) * (h->is_complex ? 2 : 1);
+    // one schedule per resident value format, each timed on zero vectors
+    const size_t es = 8 * (h->is_complex ? 2 : 1);
     void *x = nullptr, *y = nullptr;
     HD_CHECK(dalloc(&x, (size_t)h->dim * es, "autotune x"));
     int st = dalloc(&y, (size_t)h->dim * es, "autotune y");
@@ -274,25 +275,29 @@ int spmv_autotune(hd_hamiltonian* h)
     }
     HipErrors q;
     HD_ENQ(q, hipMemsetAsync(x, 0, (size_t)h->dim * es, h->stream));
+    st = q.status();
     EventTimer timer;
-    float best = 1e30f;
-    int best_s = pick_default_sched(h);
-    for (int si = 0; si < kNumScheds; ++si) {
-        if ((st = q.status()) != HD_OK) break;
-        h->spmv_sched = si;
-        if ((st = spmv_launch(h, prec, x, y, h->stream)) != HD_OK) break;  // warm
-        timer.start(h->stream);
-        for (int r = 0; r < 3 && st == HD_OK; ++r) st = spmv_launch(h, prec, x, y, h->stream);
-        const float ms = timer.stop(h->stream);
-        if (st == HD_OK) st = timer.status("SpMV autotune timing");
-        if (st != HD_OK) break;
-        if (ms < best) {
-            best = ms;
-            best_s = si;
+    const bool have[3] = {h->d_val64 != nullptr, h->d_val32 != nullptr, h->d_code != nullptr};
+    for (int prec = 0; prec < 3 && st == HD_OK; ++prec) {
+        if (!have[prec]) continue;
+        float best = 1e30f;
+        int best_s = pick_default_sched(h);
+        for (int si = 0; si < kNumScheds && st == HD_OK; ++si) {
+            h->spmv_sched[prec] = si;
+            if ((st = spmv_launch(h, prec, x, y, h->stream)) != HD_OK) break;  // warm
+            timer.start(h->stream);
+            for (int r = 0; r < 3 && st == HD_OK; ++r) st = spmv_launch(h, prec, x, y, h->stream);
+            const float ms = timer.stop(h->stream);
+            if (st == HD_OK) st = timer.status("SpMV autotune timing");
+            if (st == HD_OK && ms < best) {
+                best = ms;
+                best_s = si;
+            }
         }
+        h->spmv_sched[prec] = best_s;
     }
-    h->spmv_sched = best_s;
-    h->lanes_per_row = kScheds[best_s].lpr;
+    h->lanes_per_row = kScheds[h->spmv_sched[default_precision(h)] >= 0 ? h->spmv_sched[default_precision(h)]
+                                                                        : pick_default_sched(h)].lpr;
     dfree(x);
     dfree(y);
     return st;
@@ -368,7 +373,7 @@ int hd_spmv_set_lanes(hd_hamiltonian* h, int32_t lanes)
     if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
         return fail(HD_ERR_ARG, "lanes per row must be 4, 8, 16, 32 or 64");
     h->lanes_per_row = lanes;
-    h->spmv_sched = sched_for_lanes(lanes);
+    for (int& si : h->spmv_sched) si = sched_for_lanes(lanes);
     return HD_OK;
 }
 
@@ -396,14 +401,17 @@ int hd_stream_copy(const void* d_src, void* d_dst, size_t bytes, int32_t iters, 
 int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes)
 {
     if (!h || !lanes) return fail(HD_ERR_ARG, "null argument");
-    *lanes = kScheds[h->spmv_sched >= 0 ? h->spmv_sched : pick_default_sched(h)].lpr;
+    const int si = h->spmv_sched[default_precision(h)];
+    *lanes = kScheds[si >= 0 ? si : pick_default_sched(h)].lpr;
     return HD_OK;
 }
 
-int hd_spmv_get_threads(const hd_hamiltonian* h, int32_t* threads)
+int hd_spmv_get_schedule(const hd_hamiltonian* h, int32_t precision, int32_t* lanes, int32_t* threads)
 {
-    if (!h || !threads) return fail(HD_ERR_ARG, "null argument");
-    *threads = kScheds[h->spmv_sched >= 0 ? h->spmv_sched : pick_default_sched(h)].nt;
+    if (!h || !lanes || !threads || precision < 0 || precision > 2) return fail(HD_ERR_ARG, "bad argument");
+    const int si = h->spmv_sched[precision] >= 0 ? h->spmv_sched[precision] : pick_default_sched(h);
+    *lanes = kScheds[si].lpr;
+    *threads = kScheds[si].nt;
     return HD_OK;
 }
 

@@ -207,8 +207,9 @@ int hd_stream_copy(const void* d_src, void* d_dst, size_t bytes, int32_t iters, 
 /* Rows per wavefront schedule of the SpMV (lanes per row 4..64, 0 = autotune). */
 int hd_spmv_set_lanes(hd_hamiltonian* h, int32_t lanes_per_row);
 int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes_per_row);
-/* Threads per block of the SpMV schedule in use (256, 512 or 1024). */
-int hd_spmv_get_threads(const hd_hamiltonian* h, int32_t* threads);
+/* The SpMV schedule of one value format: lanes per row and threads per block
+ * (each format is autotuned separately; no reference counterpart). */
+int hd_spmv_get_schedule(const hd_hamiltonian* h, int32_t precision, int32_t* lanes_per_row, int32_t* threads);
 
 /* ---------------------------------------------------------------- Lanczos */
 
