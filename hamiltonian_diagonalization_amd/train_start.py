@@ -72,6 +72,11 @@ def evaluate(net, basis, two_s, v, nev, tol, mix=0.3):
         out[name] = {"matvecs": r.matvecs, "ms": r.time_ms + t_net, "evals": r.evals.tolist(),
                      "max_resid": float(r.residuals.max()),
                      "max_abs_dE_vs_plain": float(np.max(np.abs(r.evals - plain.evals)))}
+    # the ground state alone, where a start close to psi0 can pay off
+    p1 = h.lanczos(nev=1, tol=tol, seed=0)
+    l1 = h.lanczos(nev=1, tol=tol, start=vec)
+    out["ground_only"] = {"plain_matvecs": p1.matvecs, "plain_ms": p1.time_ms, "learned_matvecs": l1.matvecs,
+                          "learned_ms": l1.time_ms + t_net, "abs_dE": float(abs(l1.evals[0] - p1.evals[0]))}
     return out
 
 
@@ -88,7 +93,8 @@ def main():
     ap.add_argument("--eval-only", action="store_true")
     ap.add_argument("--nev", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--out", required=True, help="checkpoint to write (or read with --eval-only)")
+    ap.add_argument("--report", default=None, help="JSON report path (default: next to the checkpoint)")
     args = ap.parse_args()
 
     basis = FockBasis(Sector.lz(0, args.two_s), args.n, args.two_s + 1)
@@ -118,13 +124,30 @@ def main():
     report["eval"] = []
     for i, v in enumerate(ev):
         r = evaluate(net, basis, args.two_s, v, args.nev, args.tol)
+        g = r["ground_only"]
+        log(f"held-out {i}: ground state only: matvecs plain {g['plain_matvecs']} learned {g['learned_matvecs']}")
         log(f"held-out {i}: overlap {r['overlap']:.4f} (signs {r['sign_agreement']:.4f}, magnitudes "
             f"{r['magnitude_overlap']:.4f}); matvecs plain {r['plain']['matvecs']} "
             f"learned {r['learned']['matvecs']} mix {r['learned_mix']['matvecs']} "
             f"f32 {r['learned_f32']['matvecs']}; ms plain {r['plain']['ms']:.0f} learned "
             f"{r['learned']['ms']:.0f} mix {r['learned_mix']['ms']:.0f} f32 {r['learned_f32']['ms']:.0f}")
         report["eval"].append(r)
-    path = args.out[:-4] + ".json" if args.out.endswith(".npz") else args.out + ".json"
+    ev = report["eval"]
+    report["summary"] = {
+        "held_out_points": len(ev),
+        "mean_overlap": float(np.mean([r["overlap"] for r in ev])),
+        "nev_matvecs_plain_mean": float(np.mean([r["plain"]["matvecs"] for r in ev])),
+        "nev_matvecs_learned_mean": float(np.mean([r["learned"]["matvecs"] for r in ev])),
+        "nev_ms_plain_mean": float(np.mean([r["plain"]["ms"] for r in ev])),
+        "nev_ms_learned_mean": float(np.mean([r["learned"]["ms"] for r in ev])),
+        "ground_matvecs_plain_mean": float(np.mean([r["ground_only"]["plain_matvecs"] for r in ev])),
+        "ground_matvecs_learned_mean": float(np.mean([r["ground_only"]["learned_matvecs"] for r in ev])),
+        "ground_ms_plain_mean": float(np.mean([r["ground_only"]["plain_ms"] for r in ev])),
+        "ground_ms_learned_mean": float(np.mean([r["ground_only"]["learned_ms"] for r in ev])),
+        "max_abs_dE": float(max(max(r["learned"]["max_abs_dE_vs_plain"], r["ground_only"]["abs_dE"]) for r in ev)),
+    }
+    log("summary", report["summary"])
+    path = args.report or (args.out[:-4] + ".json" if args.out.endswith(".npz") else args.out + ".json")
     with open(path, "w") as f:
         json.dump(report, f, indent=1)
     log(f"wrote {path}")
