@@ -15,6 +15,13 @@
 //          --precision f64|coded|f32  --dense-below D (60; the flux-lattice caller uses 1000)
 //          --dump-crs PREFIX (writes PREFIX.rowstarts/.cols/.values of the upper-triangle CRS)
 //          --dump-basis FILE  --initial-vector FILE  --final-vector FILE
+// reference file options (general_options.hpp:57-78): --file-format 0|1 (text|binary)
+//          --out-path DIR (./output/)  --out-file-name NAME (hd)
+//          --hamiltonian-file 1   writes OUT/NAME[_sector_S]_hamiltonian_0.dat (CRS format)
+//          --eigenvalues-file 1 / --eigenvectors-file 1   writes OUT/NAME[_sector_S]_eigensystem.dat
+//          --store-terms 1 / --retrieve-terms 1   term tables in OUT as
+//              quadratic|quartic_coefficient_table_L_<NORB>.dat (sphere) or
+//              ..._kx_<X>_ky_<Y>.dat (--layout ofl with --dim-x X --dim-y Y)
 // Prints one JSON line with the dimension, the CRS size, the eigenvalues and
 // the solver statistics.
 #include "hd_dropin.hpp"
@@ -66,17 +73,56 @@ namespace
         f.write(reinterpret_cast<const char*>(p), (std::streamsize)(n * sizeof(V)));
     }
 
-    // Term tables given as raw arrays (the layout is inferred from the size by
-    // SpinlessFermionHamiltonian::Add_CdCdCC_Terms).
-    template <typename T>
-    struct RawQuarticTables : public TermTables<T>
+    bool Flag(std::map<std::string, std::string>& args, const char* name)
     {
-        void Set(std::vector<kState_t> k, std::vector<T> v)
-        {
-            this->m_kTable = std::move(k);
-            this->m_vTable = std::move(v);
+        return args.count(name) && args[name] != "0";
+    }
+
+    io::fileFormat_t FileFormat(std::map<std::string, std::string>& args)
+    {
+        return (args.count("file-format") && args["file-format"] == "1") ? io::_BINARY_ : io::_TEXT_;
+    }
+
+    std::string OutPath(std::map<std::string, std::string>& args)
+    {
+        return args.count("out-path") ? args["out-path"] : "./output/";
+    }
+
+    // MakeBaseFileName (pseudopotential_model_data.cpp:397-417)
+    std::string BaseFileName(std::map<std::string, std::string>& args)
+    {
+        std::string name = OutPath(args) + "/" + (args.count("out-file-name") ? args["out-file-name"] : "hd");
+        if (args.count("lz")) name += "_sector_" + args["lz"];
+        else if (args.count("sector") && args["sector"].rfind("lz:", 0) == 0)
+            name += "_sector_" + args["sector"].substr(3, args["sector"].find(',') - 3);
+        return name;
+    }
+
+    // TermsToFile / TermsFromFile names (pseudopotential_model.cpp:100-137,
+    // interacting_ofl_model.cpp:624-680)
+    std::string TermsFileName(std::map<std::string, std::string>& args, const char* kind, const iSize_t norb)
+    {
+        std::string tail = args.count("dim-x") && args.count("dim-y")
+                               ? "_kx_" + args["dim-x"] + "_ky_" + args["dim-y"] + ".dat"
+                               : "_L_" + std::to_string(norb) + ".dat";
+        return OutPath(args) + "/" + kind + "_coefficient_table" + tail;
+    }
+
+    // Quartic tables from raw arrays: the layout's class, with the sizes checked
+    template <class Quart, typename T>
+    bool FillRaw(Quart& quart, const iSize_t kmax, std::vector<kState_t>& kt, std::vector<T>& vt,
+                 utilities::MpiWrapper& mpi)
+    {
+        quart.Initialize((kState_t)kmax, mpi);
+        if (kt.size() != quart.GetKTable()->size() || vt.size() != quart.GetVTable()->size()) {
+            std::fprintf(stderr, "--ktable/--vtable sizes (%zu, %zu) do not match kmax %u (%zu entries)\n",
+                         kt.size(), vt.size(), kmax, quart.GetVTable()->size());
+            return false;
         }
-    };
+        *quart.GetKTable() = std::move(kt);
+        *quart.GetVTable() = std::move(vt);
+        return true;
+    }
 
     template <typename T, class Quad, class Quart>
     int Run(SpinlessFermionHamiltonian<T>& ham, Quad* quad, Quart* quart, bool haveQuad,
@@ -100,6 +146,11 @@ namespace
         mpi.ExitFlagTest();
 
         long int nnzUpper = -1;
+        const io::fileFormat_t format = FileFormat(args);
+        if (Flag(args, "hamiltonian-file")) {
+            ham.HamiltonianToFile(BaseFileName(args) + "_hamiltonian_0.dat", format, mpi);
+            mpi.ExitFlagTest();
+        }
         if (args.count("dump-crs")) {
             const auto& crs = ham.m_crsSparseMatrix;
             nnzUpper = crs.GetNbrNonZeros();
@@ -131,6 +182,11 @@ namespace
             method = "lanczos";
             MatrixVectorFunction<T, Quad, Quart> mv(ham, mpi);
             ok = ham.LanczosDiagonalize(quad, quart, mv, nev, mpi);
+        }
+        if (ok && (Flag(args, "eigenvalues-file") || Flag(args, "eigenvectors-file"))) {
+            ham.EigensystemToFile(BaseFileName(args) + "_eigensystem.dat", Flag(args, "eigenvalues-file"),
+                                  Flag(args, "eigenvectors-file"), format, mpi);
+            mpi.ExitFlagTest();
         }
         const hd_lanczos_stats& st = ham.m_arpackWrapper.m_lastStats;
         std::printf("{\"dim\": %llu, \"nnz_upper\": %ld, \"method\": \"%s\", \"ok\": %s, \"eigenvalues\": [",
@@ -178,51 +234,86 @@ int main(int argc, char** argv)
         return 2;
     }
     const iSize_t N = (iSize_t)std::stoul(args["nbr-particles"]);
+    const io::fileFormat_t format = FileFormat(args);
+    const bool store = Flag(args, "store-terms"), retrieve = Flag(args, "retrieve-terms");
+    const bool ofl = args.count("layout") && args["layout"] == "ofl";
 
-    if (args.count("pseudopotentials")) {
-        // sphere model: tables from V_m (pseudopotential_model_base.hpp:97-229)
+    if (!ofl && (args.count("pseudopotentials") || (retrieve && args.count("nbr-orbitals")))) {
+        // sphere model: tables from V_m (pseudopotential_model_base.hpp:97-229) or from files
         const iSize_t norb = (iSize_t)std::stoul(args["nbr-orbitals"]);
         SphereQuarticTermTables quart;
-        if (!quart.FromPseudopotentials((int)norb - 1, ParseList(args["pseudopotentials"]), mpi)) return 1;
+        DiagonalQuadraticTermTables<double> quad;
+        if (retrieve) {
+            quad.FromFile(TermsFileName(args, "quadratic", norb), format, mpi);
+            quart.FromFile(TermsFileName(args, "quartic", norb), format, mpi);
+            if (quart.GetKMax() != norb) {
+                std::fprintf(stderr, "quartic table has kMax %u, expected %u\n", quart.GetKMax(), norb);
+                return 2;
+            }
+        } else {
+            if (!quart.FromPseudopotentials((int)norb - 1, ParseList(args["pseudopotentials"]), mpi)) return 1;
+            quad.Initialize((kState_t)norb, mpi);
+        }
+        if (store) {
+            quad.ToFile(TermsFileName(args, "quadratic", norb), format, mpi);
+            quart.ToFile(TermsFileName(args, "quartic", norb), format, mpi);
+        }
         FermionFockBasis basis;
         if (!MakeBasis(basis, args, mpi, N, norb)) {
             std::printf("{\"dim\": 0, \"empty\": true}\n");
             return 0;
         }
         SpinlessFermionHamiltonian<double> ham;
-        DiagonalQuadraticTermTables<double> quad;
-        return Run(ham, &quad, &quart, false, args, mpi, basis);
+        // the sphere model adds no quadratic terms unless they were read
+        return Run(ham, &quad, &quart, retrieve, args, mpi, basis);
     }
 
     const iSize_t kmax = (iSize_t)std::stoul(args["kmax"]);
     const bool cplx = args.count("complex") && args["complex"] == "1";
     std::vector<kState_t> kt;
-    if (!ReadRaw(args["ktable"], kt)) { std::fprintf(stderr, "cannot read --ktable\n"); return 2; }
+    if (!retrieve && !ReadRaw(args["ktable"], kt)) { std::fprintf(stderr, "cannot read --ktable\n"); return 2; }
     FermionFockBasis basis;
     if (!MakeBasis(basis, args, mpi, N, kmax)) {
         std::printf("{\"dim\": 0, \"empty\": true}\n");
         return 0;
     }
-    if (cplx) {
-        std::vector<dcmplx> vt, eps;
-        if (!ReadRaw(args["vtable"], vt)) { std::fprintf(stderr, "cannot read --vtable\n"); return 2; }
-        RawQuarticTables<dcmplx> quart;
-        quart.Set(std::move(kt), std::move(vt));
-        DiagonalQuadraticTermTables<dcmplx> quad;
+    auto tables = [&](auto& quart, auto& quad, auto& vt, auto& eps) -> int {
+        if (retrieve) {
+            quad.FromFile(TermsFileName(args, "quadratic", kmax), format, mpi);
+            quart.FromFile(TermsFileName(args, "quartic", kmax), format, mpi);
+            return quart.GetKMax() == kmax ? 1 : -1;
+        }
+        if (!ReadRaw(args["vtable"], vt)) { std::fprintf(stderr, "cannot read --vtable\n"); return -1; }
+        if (!FillRaw(quart, kmax, kt, vt, mpi)) return -1;
         quad.Initialize((kState_t)kmax, mpi);
         const bool haveQuad = args.count("eps") && ReadRaw(args["eps"], eps);
         for (iSize_t k = 0; haveQuad && k < kmax; ++k) quad.SetEkk(eps.at(k), (kState_t)k);
+        return haveQuad ? 1 : 0;
+    };
+    auto finish = [&](auto& ham, auto& quart, auto& quad, int haveQuad) -> int {
+        if (haveQuad < 0) return 2;
+        if (store) {
+            quad.ToFile(TermsFileName(args, "quadratic", kmax), format, mpi);
+            quart.ToFile(TermsFileName(args, "quartic", kmax), format, mpi);
+        }
+        return Run(ham, &quad, &quart, haveQuad > 0, args, mpi, basis);
+    };
+    if (cplx) {
+        std::vector<dcmplx> vt, eps;
+        OflQuarticTermTables quart;
+        DiagonalQuadraticTermTables<dcmplx> quad;
         SpinlessFermionHamiltonian<dcmplx> ham;
-        return Run(ham, &quad, &quart, haveQuad, args, mpi, basis);
+        return finish(ham, quart, quad, tables(quart, quad, vt, eps));
     }
     std::vector<double> vt, eps;
-    if (!ReadRaw(args["vtable"], vt)) { std::fprintf(stderr, "cannot read --vtable\n"); return 2; }
-    RawQuarticTables<double> quart;
-    quart.Set(std::move(kt), std::move(vt));
     DiagonalQuadraticTermTables<double> quad;
-    quad.Initialize((kState_t)kmax, mpi);
-    const bool haveQuad = args.count("eps") && ReadRaw(args["eps"], eps);
-    for (iSize_t k = 0; haveQuad && k < kmax; ++k) quad.SetEkk(eps.at(k), (kState_t)k);
     SpinlessFermionHamiltonian<double> ham;
-    return Run(ham, &quad, &quart, haveQuad, args, mpi, basis);
+    if (ofl) {
+        // a real table in the flux-lattice fold is stored as complex by the
+        // reference's OFL classes; the driver keeps it real
+        std::fprintf(stderr, "--layout ofl needs --complex 1\n");
+        return 2;
+    }
+    SphereQuarticTermTables quart;
+    return finish(ham, quart, quad, tables(quart, quad, vt, eps));
 }

@@ -48,8 +48,10 @@
 #include <cstring>
 #include <fstream>
 #include <functional>
+#include <iomanip>
 #include <iostream>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -66,6 +68,15 @@ typedef std::complex<double> dcmplx;
 #endif
 #ifndef _SPARSE_ZERO_TOL_
 #define _SPARSE_ZERO_TOL_ 1e-14
+#endif
+
+#ifndef HD_DROPIN_REFERENCE_MPI
+// src/utilities/wrappers/io_wrapper.hpp:21-27
+namespace io
+{
+    enum fileFormat_t { _TEXT_ = 0, _BINARY_ = 1 };
+    enum io_t { _IN_, _OUT_ };
+}
 #endif
 
 namespace utilities
@@ -97,6 +108,50 @@ namespace utilities
 
     // src/utilities/data_structures/sparse_matrix.hpp:55
     enum storageMethod_t { _SPARSE_MAPPED_, _SPARSE_CRS_, _DENSE_ };
+
+    // io_wrapper.hpp:35-70: open a text or binary stream; on failure the
+    // message goes to stderr and mpi.m_exitFlag is set.
+    template <typename F>
+    void GenFileStream(F& stream, const std::string fileName, const io::fileFormat_t format, MpiWrapper& mpi)
+    {
+        constexpr bool in = std::is_same<F, std::ifstream>::value;
+        std::ios::openmode mode = in ? std::ios::in : std::ios::out;
+        if (format == io::_BINARY_) mode |= std::ios::binary;
+        else if (format != io::_TEXT_) {
+            std::cerr << "Unknown file format " << format << std::endl;
+            mpi.m_exitFlag = true;
+            return;
+        }
+        stream.open(fileName.c_str(), mode);
+        if (!stream.is_open()) {
+            std::cerr << "Could not open file " << fileName << std::endl;
+            mpi.m_exitFlag = true;
+        }
+    }
+
+    // io_wrapper.hpp:77-119: text fields are printed with 15 significant
+    // digits, a complex number as "re<TAB>im".
+    template <typename T>
+    std::string ToStream(const T variable)
+    {
+        std::stringstream ss;
+        ss << std::setprecision(15) << variable;
+        return ss.str();
+    }
+    inline std::string ToStream(const dcmplx variable)
+    {
+        std::stringstream ss;
+        ss << std::setprecision(15) << variable.real() << "\t" << std::setprecision(15) << variable.imag();
+        return ss.str();
+    }
+    inline void FromStream(std::ifstream& stream, double& value) { stream >> value; }
+    inline void FromStream(std::ifstream& stream, unsigned short& value) { stream >> value; }
+    inline void FromStream(std::ifstream& stream, dcmplx& value)
+    {
+        double re = 0.0, im = 0.0;
+        stream >> re >> im;
+        value = dcmplx(re, im);
+    }
 #endif
 
     namespace detail
@@ -192,6 +247,85 @@ namespace utilities
             for (int i = 0; i < m_nbrRows; ++i)
                 for (int k = m_rowStarts[i]; k < m_rowStarts[i + 1]; ++k)
                     std::cout << "\t(" << i << "," << m_cols[k] << ")\t" << m_values[k] << "\n";
+        }
+
+        // crs_sparse_matrix.hpp:662-700.  Binary: uint32 nnz, uint32 rows, int32
+        // row starts (rows + 1), int32 columns (nnz), T values (nnz).  Text: nnz,
+        // rows, the first `rows` row starts one per line, then "row col value"
+        // per entry.
+        void ToFile(const std::string fileName, const io::fileFormat_t format, MpiWrapper& mpi) const
+        {
+            Materialize();
+            std::ofstream f_out;
+            GenFileStream(f_out, fileName, format, mpi);
+            mpi.ExitFlagTest();
+            const unsigned int nnz = (unsigned int)m_values.size();
+            const unsigned int rows = m_rowStarts.empty() ? 0u : (unsigned int)(m_rowStarts.size() - 1);
+            if (format == io::_TEXT_) {
+                f_out << nnz << "\n" << rows << "\n";
+                for (unsigned int i = 0; i < rows; ++i) f_out << m_rowStarts[i] << "\n";
+                for (unsigned int i = 0; i < rows; ++i)
+                    for (int j = m_rowStarts[i]; j < m_rowStarts[i + 1]; ++j)
+                        f_out << i << " " << m_cols[j] << " " << ToStream(m_values[j]) << "\n";
+            } else {
+                f_out.write(reinterpret_cast<const char*>(&nnz), sizeof(unsigned int));
+                f_out.write(reinterpret_cast<const char*>(&rows), sizeof(unsigned int));
+                const int zero = 0;
+                const int* rs = m_rowStarts.empty() ? &zero : m_rowStarts.data();
+                f_out.write(reinterpret_cast<const char*>(rs), ((std::streamsize)rows + 1) * sizeof(int));
+                f_out.write(reinterpret_cast<const char*>(m_cols.data()), (std::streamsize)nnz * sizeof(int));
+                f_out.write(reinterpret_cast<const char*>(m_values.data()), (std::streamsize)nnz * sizeof(T));
+            }
+        }
+
+        // crs_sparse_matrix.hpp:705-751.  The text layout omits the final row
+        // start; it is nnz, which the header line holds (the reference leaves it
+        // zero and so drops the last row's entries on reading).  A short or
+        // inconsistent file is an error here.
+        void FromFile(const std::string fileName, const io::fileFormat_t format, MpiWrapper& mpi)
+        {
+            std::ifstream f_in;
+            GenFileStream(f_in, fileName, format, mpi);
+            mpi.ExitFlagTest();
+            Clear();
+            unsigned int nnz = 0, rows = 0;
+            bool ok = true;
+            if (format == io::_TEXT_) {
+                ok = (bool)(f_in >> nnz >> rows);
+                m_values.resize(nnz);
+                m_cols.resize(nnz);
+                m_rowStarts.assign((size_t)rows + 1, 0);
+                for (unsigned int i = 0; ok && i < rows; ++i) ok = (bool)(f_in >> m_rowStarts[i]);
+                m_rowStarts[rows] = (int)nnz;
+                for (unsigned int i = 0; ok && i < rows; ++i) {
+                    if (m_rowStarts[i] > m_rowStarts[i + 1] || m_rowStarts[i] < 0) { ok = false; break; }
+                    for (int j = m_rowStarts[i]; ok && j < m_rowStarts[i + 1]; ++j) {
+                        int row = 0;
+                        ok = (bool)(f_in >> row >> m_cols[j]);
+                        FromStream(f_in, m_values[j]);
+                        ok = ok && (bool)f_in && row == (int)i;
+                    }
+                }
+            } else {
+                f_in.read(reinterpret_cast<char*>(&nnz), sizeof(unsigned int));
+                f_in.read(reinterpret_cast<char*>(&rows), sizeof(unsigned int));
+                m_values.resize(nnz);
+                m_cols.resize(nnz);
+                m_rowStarts.assign((size_t)rows + 1, 0);
+                f_in.read(reinterpret_cast<char*>(m_rowStarts.data()), ((std::streamsize)rows + 1) * sizeof(int));
+                f_in.read(reinterpret_cast<char*>(m_cols.data()), (std::streamsize)nnz * sizeof(int));
+                f_in.read(reinterpret_cast<char*>(m_values.data()), (std::streamsize)nnz * sizeof(T));
+                ok = (bool)f_in && m_rowStarts[rows] == (int)nnz;
+            }
+            if (!ok) {
+                std::cerr << "\n\tERROR in CrsSparseMatrix::FromFile: " << fileName << " is truncated or inconsistent"
+                          << std::endl;
+                mpi.m_exitFlag = true;
+                Clear();
+                return;
+            }
+            m_nbrRows = m_nbrCols = (int)rows;
+            m_materialized = true;
         }
 
         private:
@@ -665,6 +799,121 @@ namespace diagonalization
 
         fock_t GetEigenvectorNodeDim() const { return m_data.m_nodeDim; }
 
+        // hamiltonian_base.hpp:1123-1153: the upper-triangle CRS in the
+        // CrsSparseMatrix file format
+        void HamiltonianToFile(const std::string fileName, const io::fileFormat_t format, utilities::MpiWrapper& mpi)
+        {
+            if (!m_data.m_matrixAllocated) {
+                std::cerr << "\n\t- Hamiltonain not allocated - failed to srite to file" << std::endl;
+                return;
+            }
+            m_crsSparseMatrix.ToFile(fileName, format, mpi);
+        }
+
+        // hamiltonian_base.hpp:1204-1298.  Binary: uint32 nev, double
+        // eigenvalues[nev], then (with vectors) the dimension in 4 bytes and
+        // T vectors[nev][dim].  Text: "##  NBR EIGENVALUES", nev,
+        // "##  EIGENVALUES", one value per line (15 digits), then
+        // "##  FOCK SPACE DIMENSION", dim, "##  EIGENVECTORS", one entry per line.
+        // Writing vectors implies writing values.
+        void EigensystemToFile(const std::string fileName, bool writeValues, const bool writeVectors,
+                               const io::fileFormat_t format, utilities::MpiWrapper& mpi)
+        {
+            if (writeVectors) writeValues = true;
+            if (!m_data.m_eigenvaluesCalculated && writeValues) {
+                std::cerr << "ERROR eigenvalues cannot be written to file as they are not calculated" << std::endl;
+                mpi.m_exitFlag = true;
+                return;
+            }
+            if (!m_data.m_eigenvectorsCalculated && writeVectors) {
+                std::cerr << "ERROR eigenvectors cannot be written to file as they are not calculated" << std::endl;
+                mpi.m_exitFlag = true;
+                return;
+            }
+            std::ofstream f_out;
+            utilities::GenFileStream(f_out, fileName, format, mpi);
+            if (mpi.m_exitFlag) return;
+            const iSize_t nev = m_data.m_nbrEigenvalues;
+            const fock_t dim = m_data.m_fockSpaceDim;
+            if (writeValues) {
+                if (format == io::_TEXT_) {
+                    f_out << "##  NBR EIGENVALUES\n" << nev << "\n" << "##  EIGENVALUES\n";
+                    for (iSize_t i = 0; i < nev; ++i) f_out << std::setprecision(15) << m_eigenvalues[i] << "\n";
+                } else {
+                    f_out.write(reinterpret_cast<const char*>(&nev), sizeof(unsigned int));
+                    f_out.write(reinterpret_cast<const char*>(m_eigenvalues.data()), (std::streamsize)nev * sizeof(double));
+                }
+            }
+            if (writeVectors) {
+                if (format == io::_TEXT_) {
+                    f_out << "##  FOCK SPACE DIMENSION\n" << dim << "\n" << "##  EIGENVECTORS\n";
+                    for (size_t i = 0; i < (size_t)nev * dim; ++i) f_out << utilities::ToStream(m_eigenvectors[i]) << "\n";
+                } else {
+                    const unsigned int dim32 = (unsigned int)dim;
+                    f_out.write(reinterpret_cast<const char*>(&dim32), sizeof(unsigned int));
+                    f_out.write(reinterpret_cast<const char*>(m_eigenvectors.data()),
+                                (std::streamsize)((size_t)nev * dim * sizeof(T)));
+                }
+            }
+        }
+
+        // hamiltonian_base.hpp:1304-1432.  Reading vectors implies reading
+        // values.  The vectors are flagged as present only when they were read
+        // (the reference flags them whenever the file opened), and a truncated
+        // file sets the exit flag.
+        void EigensystemFromFile(const std::string fileName, bool readValues, const bool readVectors,
+                                 const io::fileFormat_t format, utilities::MpiWrapper& mpi)
+        {
+            if (readVectors) readValues = true;
+            std::ifstream f_in;
+            utilities::GenFileStream(f_in, fileName, format, mpi);
+            if (mpi.m_exitFlag) return;
+            bool ok = true;
+            if (readValues) {
+                iSize_t nev = 0;
+                if (format == io::_TEXT_) {
+                    std::string line;
+                    while (std::getline(f_in, line) && line != "##  NBR EIGENVALUES") {}
+                    f_in >> nev;
+                    while (std::getline(f_in, line) && line != "##  EIGENVALUES") {}
+                    m_eigenvalues.assign(nev, 0.0);
+                    for (iSize_t i = 0; i < nev; ++i) f_in >> m_eigenvalues[i];
+                } else {
+                    f_in.read(reinterpret_cast<char*>(&nev), sizeof(unsigned int));
+                    m_eigenvalues.assign(f_in ? nev : 0, 0.0);
+                    f_in.read(reinterpret_cast<char*>(m_eigenvalues.data()), (std::streamsize)m_eigenvalues.size() * sizeof(double));
+                }
+                ok = (bool)f_in;
+                m_data.m_nbrEigenvalues = nev;
+                m_data.m_eigenvaluesCalculated = ok;
+            }
+            if (readVectors && ok) {
+                fock_t dim = 0;
+                if (format == io::_TEXT_) {
+                    std::string line;
+                    while (std::getline(f_in, line) && line != "##  FOCK SPACE DIMENSION") {}
+                    f_in >> dim;
+                    while (std::getline(f_in, line) && line != "##  EIGENVECTORS") {}
+                    m_eigenvectors.assign((size_t)m_data.m_nbrEigenvalues * dim, T(0.0));
+                    for (auto& v : m_eigenvectors) utilities::FromStream(f_in, v);
+                } else {
+                    unsigned int dim32 = 0;
+                    f_in.read(reinterpret_cast<char*>(&dim32), sizeof(unsigned int));
+                    dim = dim32;
+                    m_eigenvectors.assign((size_t)m_data.m_nbrEigenvalues * dim, T(0.0));
+                    f_in.read(reinterpret_cast<char*>(m_eigenvectors.data()),
+                              (std::streamsize)(m_eigenvectors.size() * sizeof(T)));
+                }
+                ok = (bool)f_in;
+                m_data.m_fockSpaceDim = m_data.m_nodeDim = dim;
+                m_data.m_eigenvectorsCalculated = ok;
+            }
+            if (!ok) {
+                std::cerr << "\n\tERROR in EigensystemFromFile: " << fileName << " is truncated" << std::endl;
+                mpi.m_exitFlag = true;
+            }
+        }
+
         void PrintEigensystem() const
         {
             std::cout << "\n\tEIGENVALUES:\n";
@@ -860,36 +1109,186 @@ namespace diagonalization
         char UPLO;
     };
 
-    // Term tables with the reference's accessors (term_tables_base.hpp:44-174;
-    // sphere term_tables.cpp:36-169; flux-lattice term_tables.cpp:36-198), for
-    // callers that have no table classes of their own.
+    // Term tables with the reference's accessors and file formats
+    // (term_tables_base.hpp:44-345; sphere term_tables.cpp:36-169; flux-lattice
+    // term_tables.cpp:36-221), for callers that have no table classes of their own.
     template <typename T>
     class TermTables
     {
         public:
+        TermTables() : m_fileHeader("# File contains term table data in array format") {}
         virtual ~TermTables() = default;
+
+        // term_tables_base.hpp:77-93: both arrays get CalculateDim(kMax) entries
+        virtual void Initialize(const kState_t kMax, const utilities::MpiWrapper&)
+        {
+            m_kMax = kMax;
+            const iSize_t dim = CalculateDim(kMax);
+            m_kTable.assign(dim, 0);
+            m_vTable.assign(dim, T(0.0));
+        }
+        void Clear()
+        {
+            m_kTable.clear();
+            m_vTable.clear();
+        }
+
+        virtual void GetK1(kState_t*, iSize_t&, const kState_t, const kState_t, const kState_t) const {}
+        virtual void SetK1(const kState_t, const kState_t, const kState_t, const kState_t) {}
+        virtual T GetVkkkk(const kState_t, const kState_t, const kState_t, const kState_t) const { return T(0.0); }
+        virtual void SetVkkkk(const T, const kState_t, const kState_t, const kState_t, const kState_t) {}
+        virtual void GetK1(kState_t*, iSize_t&, const kState_t) const {}
+        virtual T GetEkk(const kState_t, const kState_t) const { return T(0.0); }
+        virtual void ToFile(const std::string fileName, const io::fileFormat_t format,
+                            utilities::MpiWrapper& mpi) const = 0;
+        virtual void FromFile(const std::string fileName, const io::fileFormat_t format,
+                              utilities::MpiWrapper& mpi) = 0;
+
         iSize_t GetMaxKCount() const { return 1; }
         iSize_t GetDimension() const { return (iSize_t)m_vTable.size(); }
         std::vector<kState_t>* GetKTable() { return &m_kTable; }
         std::vector<T>* GetVTable() { return &m_vTable; }
         kState_t GetKMax() const { return m_kMax; }
 
+        // term_tables_base.hpp:161-169
+        void SetCoefficient(const T coefficient)
+        {
+            for (auto& v : m_vTable) v *= coefficient;
+        }
+
         protected:
+        virtual iSize_t CalculateDim(const kState_t kMax) const = 0;
+
+        // term_tables_base.hpp:179-247.  Binary: uint32 header length, the
+        // header, uint32 label count (2 or 4), kMax in a 4-byte field, then the
+        // k table (4 labels only, uint16 x dim) and the value table (T x dim).
+        // The reference writes that field from its 2-byte m_kMax, so its upper
+        // two bytes are whatever follows m_kMax in memory; they are written as
+        // zero here and ignored on reading.  Text: the header line, the label
+        // count, kMax, then "k1 k1 value" per k (2 labels) or "k1 k2 k3 k4
+        // value" over k4, k3, k2 with k1 from GetK1 (4 labels), tab separated.
+        void ToFileBase(const std::string fileName, const io::fileFormat_t format, const iSize_t nbrLabels,
+                        utilities::MpiWrapper& mpi) const
+        {
+            std::ofstream f_out;
+            utilities::GenFileStream(f_out, fileName, format, mpi);
+            if (!mpi.m_exitFlag) {
+                if (format == io::_BINARY_) {
+                    const iSize_t strSize = (iSize_t)m_fileHeader.size();
+                    f_out.write(reinterpret_cast<const char*>(&strSize), sizeof(iSize_t));
+                    f_out.write(m_fileHeader.data(), strSize);
+                    f_out.write(reinterpret_cast<const char*>(&nbrLabels), sizeof(iSize_t));
+                    const iSize_t kMaxField = m_kMax;
+                    f_out.write(reinterpret_cast<const char*>(&kMaxField), sizeof(iSize_t));
+                    if (nbrLabels == 4)
+                        f_out.write(reinterpret_cast<const char*>(m_kTable.data()),
+                                    (std::streamsize)(m_kTable.size() * sizeof(kState_t)));
+                    if (nbrLabels == 2 || nbrLabels == 4)
+                        f_out.write(reinterpret_cast<const char*>(m_vTable.data()),
+                                    (std::streamsize)(m_vTable.size() * sizeof(T)));
+                } else {
+                    f_out << m_fileHeader << "\n" << nbrLabels << "\n" << m_kMax << "\n";
+                    if (nbrLabels == 2) {
+                        for (kState_t k1 = 0; k1 < m_kMax; ++k1)
+                            f_out << k1 << "\t" << k1 << "\t" << utilities::ToStream(m_vTable[k1]) << "\n";
+                    } else if (nbrLabels == 4) {
+                        for (kState_t k4 = 0; k4 < m_kMax; ++k4)
+                            for (kState_t k3 = 0; k3 < m_kMax; ++k3)
+                                for (kState_t k2 = 0; k2 < m_kMax; ++k2) {
+                                    kState_t k1 = 0;
+                                    iSize_t n = 0;
+                                    this->GetK1(&k1, n, k2, k3, k4);
+                                    f_out << k1 << "\t" << k2 << "\t" << k3 << "\t" << k4 << "\t"
+                                          << utilities::ToStream(this->GetVkkkk(k1, k2, k3, k4)) << "\n";
+                                }
+                    }
+                }
+            }
+            mpi.ExitFlagTest();
+        }
+
+        // term_tables_base.hpp:252-345: the header must match; a mismatch is
+        // reported and sets the exit flag, as in the reference.  A truncated
+        // file is reported the same way.
+        void FromFileBase(const std::string fileName, const io::fileFormat_t format, utilities::MpiWrapper& mpi)
+        {
+            std::ifstream f_in;
+            utilities::GenFileStream(f_in, fileName, format, mpi);
+            if (!mpi.m_exitFlag) {
+                iSize_t nbrLabels = 0;
+                std::string fileHeader;
+                bool ok = true;
+                if (format == io::_BINARY_) {
+                    iSize_t strSize = 0;
+                    f_in.read(reinterpret_cast<char*>(&strSize), sizeof(iSize_t));
+                    if (!f_in || strSize > 4096) strSize = 0;
+                    fileHeader.resize(strSize);
+                    if (strSize) f_in.read(&fileHeader[0], strSize);
+                    if (fileHeader != m_fileHeader) return BadHeader(fileHeader, mpi);
+                    iSize_t kMaxField = 0;
+                    f_in.read(reinterpret_cast<char*>(&nbrLabels), sizeof(iSize_t));
+                    f_in.read(reinterpret_cast<char*>(&kMaxField), sizeof(iSize_t));
+                    this->Initialize((kState_t)(kMaxField & 0xFFFFu), mpi);
+                    if (nbrLabels == 4)
+                        f_in.read(reinterpret_cast<char*>(m_kTable.data()),
+                                  (std::streamsize)(m_kTable.size() * sizeof(kState_t)));
+                    if (nbrLabels == 2 || nbrLabels == 4)
+                        f_in.read(reinterpret_cast<char*>(m_vTable.data()),
+                                  (std::streamsize)(m_vTable.size() * sizeof(T)));
+                    ok = (bool)f_in;
+                } else {
+                    std::getline(f_in, fileHeader);
+                    if (fileHeader != m_fileHeader) return BadHeader(fileHeader, mpi);
+                    kState_t kMax = 0;
+                    f_in >> nbrLabels >> kMax;
+                    this->Initialize(kMax, mpi);
+                    if (nbrLabels == 2) {
+                        for (iSize_t i = 0; i < m_vTable.size(); ++i) {
+                            kState_t k1, k2;
+                            f_in >> k1 >> k2;
+                            utilities::FromStream(f_in, m_vTable[i]);
+                        }
+                    } else if (nbrLabels == 4) {
+                        for (kState_t k4 = 0; k4 < m_kMax; ++k4)
+                            for (kState_t k3 = 0; k3 < m_kMax; ++k3)
+                                for (kState_t k2 = 0; k2 < m_kMax; ++k2) {
+                                    kState_t k1, dummy;
+                                    T v;
+                                    f_in >> k1 >> dummy >> dummy >> dummy;
+                                    utilities::FromStream(f_in, v);
+                                    this->SetK1(k1, k2, k3, k4);
+                                    this->SetVkkkk(v, k1, k2, k3, k4);
+                                }
+                    }
+                    ok = (bool)f_in;
+                }
+                if (!ok) {
+                    std::cerr << "\n\tFile read in error - " << fileName << " is truncated" << std::endl;
+                    mpi.m_exitFlag = true;
+                }
+            }
+            mpi.ExitFlagTest();
+        }
+
         std::vector<T> m_vTable;
         std::vector<kState_t> m_kTable;
         kState_t m_kMax = 0;
+        std::string m_fileHeader;
+
+        private:
+        void BadHeader(const std::string& fileHeader, utilities::MpiWrapper& mpi)
+        {
+            std::cerr << "\n\tFile read in error - wrong file header " << fileHeader << std::endl;
+            mpi.m_exitFlag = true;
+            mpi.ExitFlagTest();
+        }
     };
 
-    // Sphere QuarticTermTables: index (k4*kMax+k3)*kMax+k2
+    // Sphere QuarticTermTables (fqhe_sphere term_tables.cpp:96-187): index
+    // (k4*kMax+k3)*kMax+k2, kMax^3 entries
     class SphereQuarticTermTables : public TermTables<double>
     {
         public:
-        void Initialize(const kState_t kMax, const utilities::MpiWrapper&)
-        {
-            m_kMax = kMax;
-            m_kTable.assign((size_t)kMax * kMax * kMax, 0);
-            m_vTable.assign((size_t)kMax * kMax * kMax, 0.0);
-        }
         // BaseGenerate4KTable + BaseTermsFromPseudopotentials
         // (pseudopotential_model_base.hpp:97-229) on the C ABI
         bool FromPseudopotentials(const int maxLz, const std::vector<double>& pseudopotentials,
@@ -901,66 +1300,102 @@ namespace diagonalization
                                                           (int32_t)pseudopotentials.size(), m_vTable.data()),
                                          "QuarticTermTables");
         }
-        void GetK1(kState_t* buf, iSize_t& n, const kState_t k2, const kState_t k3, const kState_t k4) const
+        void GetK1(kState_t* buf, iSize_t& n, const kState_t k2, const kState_t k3, const kState_t k4) const override
         {
             buf[0] = m_kTable[((size_t)k4 * m_kMax + k3) * m_kMax + k2];
             n = buf[0] >= m_kMax ? 0 : 1;
         }
-        double GetVkkkk(const kState_t, const kState_t k2, const kState_t k3, const kState_t k4) const
-        {
-            return m_vTable[((size_t)k4 * m_kMax + k3) * m_kMax + k2];
-        }
-        void SetK1(const kState_t k1, const kState_t k2, const kState_t k3, const kState_t k4)
+        void SetK1(const kState_t k1, const kState_t k2, const kState_t k3, const kState_t k4) override
         {
             m_kTable[((size_t)k4 * m_kMax + k3) * m_kMax + k2] = k1;
         }
-        void SetVkkkk(const double v, const kState_t, const kState_t k2, const kState_t k3, const kState_t k4)
+        double GetVkkkk(const kState_t, const kState_t k2, const kState_t k3, const kState_t k4) const override
+        {
+            if (k2 >= m_kMax || k3 >= m_kMax || k4 >= m_kMax)
+                std::cerr << "\n\tERROR in GetVkkkk: k index out of range" << std::endl;
+            return m_vTable[((size_t)k4 * m_kMax + k3) * m_kMax + k2];
+        }
+        void SetVkkkk(const double v, const kState_t, const kState_t k2, const kState_t k3, const kState_t k4) override
         {
             m_vTable[((size_t)k4 * m_kMax + k3) * m_kMax + k2] = v;
         }
+        void ToFile(const std::string fileName, const io::fileFormat_t format, utilities::MpiWrapper& mpi) const override
+        {
+            ToFileBase(fileName, format, 4, mpi);
+        }
+        void FromFile(const std::string fileName, const io::fileFormat_t format, utilities::MpiWrapper& mpi) override
+        {
+            FromFileBase(fileName, format, mpi);
+        }
+
+        protected:
+        iSize_t CalculateDim(const kState_t kMax) const override { return (iSize_t)kMax * kMax * kMax; }
     };
 
-    // Flux-lattice QuarticTermTables: folded on k4 <= k3
+    // Flux-lattice QuarticTermTables (optical_flux_lattice term_tables.cpp:95-221):
+    // folded on k4 <= k3, (kMax+1) kMax^2 / 2 entries; with k4 > k3 the entry of
+    // (k3, k4) is used and k1 takes the place of k2 in the value index.
     class OflQuarticTermTables : public TermTables<dcmplx>
     {
         public:
-        void Initialize(const kState_t kMax, const utilities::MpiWrapper&)
+        void GetK1(kState_t* buf, iSize_t& n, const kState_t k2, const kState_t k3, const kState_t k4) const override
         {
-            m_kMax = kMax;
-            m_kTable.assign((size_t)(kMax + 1) * kMax * kMax / 2, 0);
-            m_vTable.assign((size_t)(kMax + 1) * kMax * kMax / 2, dcmplx(0.0, 0.0));
+            n = 1;
+            buf[0] = m_kTable[Fold(k3, k4) * m_kMax + k2];
         }
-        size_t Index(const kState_t k2, const kState_t k3, const kState_t k4, const kState_t k1) const
+        void SetK1(const kState_t k1, const kState_t k2, const kState_t k3, const kState_t k4) override
         {
-            int t1 = k4, t2 = k3, t3 = k2;
-            if (k4 > k3) { t1 = k3; t2 = k4; t3 = k1; }
-            return ((size_t)t1 * m_kMax + t2 - (size_t)(t1 * (t1 + 1)) / 2) * m_kMax + t3;
+            m_kTable[Fold(k3, k4) * m_kMax + k2] = k1;
         }
-        void SetK1(const kState_t k1, const kState_t k2, const kState_t k3, const kState_t k4)
+        dcmplx GetVkkkk(const kState_t k1, const kState_t k2, const kState_t k3, const kState_t k4) const override
         {
-            m_kTable[Index(k2, k3, k4, k1)] = k1;
+            return m_vTable[Fold(k3, k4) * m_kMax + (k4 > k3 ? k1 : k2)];
         }
-        void SetVkkkk(const dcmplx v, const kState_t k1, const kState_t k2, const kState_t k3, const kState_t k4)
+        void SetVkkkk(const dcmplx v, const kState_t k1, const kState_t k2, const kState_t k3, const kState_t k4) override
         {
-            m_vTable[Index(k2, k3, k4, k1)] = v;
+            m_vTable[Fold(k3, k4) * m_kMax + (k4 > k3 ? k1 : k2)] = v;
+        }
+        void ToFile(const std::string fileName, const io::fileFormat_t format, utilities::MpiWrapper& mpi) const override
+        {
+            ToFileBase(fileName, format, 4, mpi);
+        }
+        void FromFile(const std::string fileName, const io::fileFormat_t format, utilities::MpiWrapper& mpi) override
+        {
+            FromFileBase(fileName, format, mpi);
+        }
+
+        protected:
+        iSize_t CalculateDim(const kState_t kMax) const override { return (iSize_t)(kMax + 1) * kMax * kMax / 2; }
+
+        private:
+        // row of the folded (min, max) pair: t1*kMax + t2 - t1(t1+1)/2
+        size_t Fold(const kState_t k3, const kState_t k4) const
+        {
+            const size_t t1 = std::min(k3, k4), t2 = std::max(k3, k4);
+            return t1 * m_kMax + t2 - (t1 * (t1 + 1)) / 2;
         }
     };
 
-    // Quadratic tables: E_k on the diagonal (GetK1(k2) gives k1 = k2)
+    // QuadraticTermTables of both models (term_tables.cpp:36-88): E_k on the
+    // diagonal, GetK1(k2) gives k1 = k2, kMax entries, written with 2 labels
     template <typename T>
     class DiagonalQuadraticTermTables : public TermTables<T>
     {
         public:
-        void Initialize(const kState_t kMax, const utilities::MpiWrapper&)
-        {
-            this->m_kMax = kMax;
-            this->m_kTable.resize(kMax);
-            for (kState_t k = 0; k < kMax; ++k) this->m_kTable[k] = k;
-            this->m_vTable.assign(kMax, T(0.0));
-        }
-        void GetK1(kState_t* buf, iSize_t& n, const kState_t k2) const { buf[0] = k2; n = 1; }
-        T GetEkk(const kState_t k1, const kState_t) const { return this->m_vTable[k1]; }
+        void GetK1(kState_t* buf, iSize_t& n, const kState_t k2) const override { buf[0] = k2; n = 1; }
+        T GetEkk(const kState_t k1, const kState_t) const override { return this->m_vTable[k1]; }
         void SetEkk(const T e, const kState_t k1) { this->m_vTable[k1] = e; }
+        void ToFile(const std::string fileName, const io::fileFormat_t format, utilities::MpiWrapper& mpi) const override
+        {
+            this->ToFileBase(fileName, format, 2, mpi);
+        }
+        void FromFile(const std::string fileName, const io::fileFormat_t format, utilities::MpiWrapper& mpi) override
+        {
+            this->FromFileBase(fileName, format, mpi);
+        }
+
+        protected:
+        iSize_t CalculateDim(const kState_t kMax) const override { return kMax; }
     };
 }
 

@@ -90,6 +90,15 @@ class AmplitudeNet:
         _check(lib().hd_mlp_start_vector(self.h, basis.h, _ptr(pr), buf.ptr))
         return buf
 
+    def write_initial_vector(self, basis: FockBasis, params: Sequence[float], path: str) -> np.ndarray:
+        """The start vector for `params` as the raw float64[dim] file the
+        reference's ArpackWrapper reads with --use-initial
+        (arpack_wrapper.hpp:753-774); returns the vector."""
+        from . import fileio
+        v = self.start_vector(basis, params).to_array(np.float64, basis.dim)
+        fileio.write_initial_vector(path, v)
+        return v
+
     def save(self, path: str):
         np.savez(path, weights=self.get_weights(), n_bits=self.n_bits, n_params=self.n_params)
 

@@ -100,3 +100,24 @@ def test_training_and_learned_start():
     r = h.lanczos(nev=3, tol=1e-10, start=start)
     assert r.converged
     assert np.max(np.abs(r.evals - ref.evals)) < 1e-9
+    # the same start vector as the reference's --use-initial file, read by the
+    # C++ drop-in driver: same ground state, fewer products than a random start
+    import json
+    import os
+    import subprocess
+    import tempfile
+    from hamiltonian_diagonalization_amd import fileio
+    driver = os.path.join(os.path.dirname(hd.__file__), "lib", "hd_diagonalize")
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "initial_vector.bin")
+        v = net.write_initial_vector(basis, held, f)
+        assert np.array_equal(fileio.read_initial_vector(f), v) and len(v) == basis.dim
+        vm = models.coulomb_list(two_s, {1: float(held[0]), 3: float(held[1]), 5: float(held[2])})
+        base = [driver, "--nbr-particles", str(n), "--nbr-orbitals", str(two_s + 1), "--lz", "0",
+                "--pseudopotentials", ",".join("%.17g" % x for x in vm), "--nbr-eigenvalues", "1", "--tol", "1e-10"]
+        out = [json.loads(subprocess.run(base + extra, capture_output=True, text=True, check=True,
+                                         timeout=300).stdout.strip().splitlines()[-1])
+               for extra in ([], ["--initial-vector", f])]
+    say("drop-in matvecs random start", out[0]["matvecs"], "learned start", out[1]["matvecs"])
+    assert abs(out[1]["eigenvalues"][0] - ref.evals[0]) < 1e-9
+    assert out[1]["matvecs"] < out[0]["matvecs"]
