@@ -108,7 +108,7 @@ def measured_traffic(kernel, n, two_s):
         return None
     for line in open(path):
         f = line.split()
-        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1].replace(",true>", ">") == kernel:
             return int(f[2])
     return None
 

@@ -800,7 +800,7 @@ int hd_full_diagonalize(hd_hamiltonian* h, int32_t nev, double* h_evals)
             A[(size_t)i * n + col[j]] = val[j];
             if (h->storage == HD_STORAGE_UPPER_REF && col[j] != i) A[(size_t)col[j] * n + i] = val[j];
         }
-    jacobi_eigh((int)n, A, w, Z);
+    if (!sym_eig((int)n, A, w, Z)) return fail(HD_ERR_NOT_CONVERGED, "dense eigenproblem did not converge");
     for (int i = 0; i < std::min<int64_t>(nev, n); ++i) h_evals[i] = w[i];
     return HD_OK;
 }

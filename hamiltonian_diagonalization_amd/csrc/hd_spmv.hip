@@ -112,17 +112,26 @@ k_spmv(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restric
     if (row < dim && sub == 0) y[row] = acc;
 }
 
-// Coded values: the block copies the dictionary into LDS, then as k_spmv.
-template <typename V, int LPR, int NT>
+// Coded values: the block copies the dictionary into LDS, then as k_spmv.  A
+// dictionary above kLdsDictBytes (random complex couplings give tens of
+// thousands of distinct values) is read in place instead: at most 32767
+// entries, 512 KiB complex, it stays resident in each XCD's 4 MiB L2.
+constexpr size_t kLdsDictBytes = 64 * 1024;
+
+template <typename V, int LPR, int NT, bool LDS_DICT>
 __global__ void __launch_bounds__(NT)
 k_spmv_coded(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
              const uint16_t* __restrict__ code, const V* __restrict__ diag, const V* __restrict__ dict,
              int ndict, const V* __restrict__ x, V* __restrict__ y)
 {
     extern __shared__ int32_t lds_raw[];
-    V* sdict = reinterpret_cast<V*>(lds_raw);
-    for (int i = threadIdx.x; i < ndict; i += NT) sdict[i] = dict[i];
-    __syncthreads();
+    const V* sdict = dict;
+    if constexpr (LDS_DICT) {
+        V* t = reinterpret_cast<V*>(lds_raw);
+        for (int i = threadIdx.x; i < ndict; i += NT) t[i] = dict[i];
+        __syncthreads();
+        sdict = t;
+    }
     constexpr int RPB = NT / LPR;
     const int sub = threadIdx.x & (LPR - 1);
     const int64_t row = window_chunk(blockIdx.x, gridDim.x) * RPB + threadIdx.x / LPR;
@@ -168,9 +177,16 @@ static void launch_coded(const hd_hamiltonian* h, const void* x, void* y, hipStr
     constexpr int RPB = NT / LPR;
     const unsigned grid = (unsigned)((h->dim + RPB - 1) / RPB);
     const size_t lds = sizeof(V) * (size_t)std::max(h->ndict, 1);
-    hipLaunchKernelGGL((k_spmv_coded<V, LPR, NT>), dim3(grid), dim3(NT), lds, s, h->dim, h->d_rowptr, h->d_col,
-                       h->d_code, static_cast<const V*>(h->d_diag), static_cast<const V*>(h->d_dict), h->ndict,
-                       static_cast<const V*>(x), static_cast<V*>(y));
+    if (lds <= kLdsDictBytes)
+        hipLaunchKernelGGL((k_spmv_coded<V, LPR, NT, true>), dim3(grid), dim3(NT), lds, s, h->dim, h->d_rowptr,
+                           h->d_col, h->d_code, static_cast<const V*>(h->d_diag),
+                           static_cast<const V*>(h->d_dict), h->ndict, static_cast<const V*>(x),
+                           static_cast<V*>(y));
+    else
+        hipLaunchKernelGGL((k_spmv_coded<V, LPR, NT, false>), dim3(grid), dim3(NT), 0, s, h->dim, h->d_rowptr,
+                           h->d_col, h->d_code, static_cast<const V*>(h->d_diag),
+                           static_cast<const V*>(h->d_dict), h->ndict, static_cast<const V*>(x),
+                           static_cast<V*>(y));
 }
 
 template <typename V, bool CODED>
@@ -225,8 +241,7 @@ int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y, 
     if (h->dim == 0) return HD_OK;
     if (precision == HD_CODED) {
         if (!h->d_code) return fail(HD_ERR_ARG, "no coded values resident (value_mask & 4)");
-        const size_t lds = (size_t)std::max(h->ndict, 1) * (h->is_complex ? 16 : 8);
-        if (lds > 64 * 1024) return fail(HD_ERR_UNSUPPORTED, "value dictionary larger than 64 KiB of LDS");
+        if (h->ndict > 0x7FFF) return fail(HD_ERR_UNSUPPORTED, "value dictionary above 32767 entries");
         if (h->is_complex) launch_sched<double2, true>(si, h, nullptr, x, y, stream);
         else launch_sched<double, true>(si, h, nullptr, x, y, stream);
         HD_HIP(hipGetLastError());

@@ -187,6 +187,30 @@ def test_spmv_complex(oracle):
     assert np.max(np.abs(gc - want)) <= 1e-13 * np.max(np.abs(want))
 
 
+def test_spmv_complex_large_dictionary(oracle):
+    # 30 momenta with random complex couplings: far more distinct hop values
+    # than fit the LDS copy of the dictionary, so the coded kernel reads it in place
+    say("spmv ofl 6x5 N=4 complex, dictionary above the LDS limit")
+    kt, vt, eps = models.ofl_tables(6, 5, 9)
+    b = hd.FockBasis(hd.Sector.k2d(2, 1, 6, 5), 4, 30)
+    t = hd.TermTables(hd.LAYOUT_OFL, 30, kt, vt, eps)
+    up = oracle.build(4, 30, b.states(), oracle.LAYOUT_OFL, 30, kt, vt, eps=eps, is_complex=True)
+    h = hd.Hamiltonian(b, t, values=(hd.F64, hd.CODED))
+    ndict = h.device_coded()[3]
+    assert ndict * 16 > 64 * 1024
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=b.dim) + 1j * rng.normal(size=b.dim)
+    want = oracle.spmv_sym(up, x)
+    for lanes in (4, 32):
+        h.set_lanes(lanes)
+        gc = h.spmv(x, precision=hd.CODED)
+        assert np.max(np.abs(gc - want)) <= 1e-13 * np.max(np.abs(want))
+    for i in (0, b.dim // 2, b.dim - 1):
+        e = np.zeros(b.dim, dtype=np.complex128)
+        e[i] = 1.0
+        assert np.array_equal(h.spmv(e, precision=hd.CODED), h.spmv(e))
+
+
 def test_lanczos_config1(oracle):
     say("lanczos N=8 2S=21 nev=5")
     b, t = models.sphere_problem(8, 21)
