@@ -190,17 +190,22 @@ def main():
     log(f"nnz={info['nnz']} max_row={info['max_row']} build {t_build:.2f} s "
         f"(count {info['t_count_ms']:.1f} scan {info['t_scan_ms']:.1f} fill {info['t_fill_ms']:.1f} "
         f"sort {info['t_sort_ms']:.1f} ms)")
+    t0 = time.perf_counter()
+    tile = ham.tile()
+    t_tile = time.perf_counter() - t0
+    log(f"tiled layout: {t_tile:.3f} s, {tile['lines'] / max(tile['entries'], 1):.4f} staged lines per entry")
     ham.set_lanes(0)
-    fmts = {"coded": hd.CODED, "f32": hd.F32, "f64": hd.F64}
+    fmts = {"coded": hd.CODED, "f32": hd.F32, "f64": hd.F64, "tiled": hd.TILED}
     sp_ms = {k: ham.spmv_time_ms(p, 20) for k, p in fmts.items()}
     sp_b = {k: ham.spmv_bytes(p) for k, p in fmts.items()}
     for k in fmts:
         log(f"SpMV {k}: {sp_ms[k]:.3f} ms, {sp_b[k] / sp_ms[k] / 1e6:.0f} GB/s")
 
-    # timed steps: the exact FP64 operator in whichever of its two formats (FP64
-    # CRS or coded) has the faster SpMV, random-normal start, seed 0
-    prec = hd.CODED if sp_ms["coded"] < sp_ms["f64"] else hd.F64
-    op_name = "coded_fp64" if prec == hd.CODED else "fp64_crs"
+    # timed steps: the exact FP64 operator in whichever of its three formats
+    # (FP64 CRS, coded, tiled) has the fastest SpMV, random-normal start, seed 0
+    exact = {"f64": (hd.F64, "fp64_crs"), "coded": (hd.CODED, "coded_fp64"), "tiled": (hd.TILED, "tiled_fp64")}
+    key = min(exact, key=lambda k: sp_ms[k])
+    prec, op_name = exact[key]
     for w in range(args.warmup):
         r = ham.lanczos(nev=args.nev, tol=args.tol, precision=prec, seed=0, ncv=args.ncv)
         log(f"warmup {w}: {r}")
@@ -220,17 +225,17 @@ def main():
 
     cmp = {}
     if not args.skip_compare:
-        others = [("coded_fp64", hd.CODED)] if prec == hd.F64 else [("fp64_crs", hd.F64)]
+        others = [(nm, p) for k, (p, nm) in exact.items() if k != key]
         for name, p in others + [("f32_mixed", hd.F32)]:
             r = ham.lanczos(nev=args.nev, tol=args.tol, precision=p, seed=0, ncv=args.ncv)
             log(f"{name}: {r}")
             cmp[name] = r
     learned = learned_start(args, hd, basis, ham, prec, res) if not args.skip_learned else None
-    ref = cmp.get("fp64_crs") or cmp.get("coded_fp64")
     err = None
-    if ref is not None:
-        err = [float(abs(a - b)) for a, b in zip(res.evals, ref.evals)]
-        log(f"|E_steps - E_other_fp64_format| per eigenvalue: {err}")
+    refs = [cmp[nm] for k, (p, nm) in exact.items() if k != key and nm in cmp]
+    if refs:
+        err = [float(max(abs(a - r.evals[i]) for r in refs)) for i, a in enumerate(res.evals)]
+        log(f"|E_steps - E_other_fp64_formats| per eigenvalue: {err}")
 
     cpu = None
     if not args.skip_cpu:
@@ -239,12 +244,11 @@ def main():
         cpu = cpu_baseline(n, two_s, basis.states(), tables, res.matvecs, args.cpu_rows, threads)
         log(f"cpu baseline: {cpu}")
 
-    key = "coded" if prec == hd.CODED else "f64"
     spmv_live_ms = live_ms / max(live_mv, 1)
     achieved = sp_b[key] / (spmv_live_ms * 1e-3) / 1e9
     lanes, spmv_threads = ham.schedule(prec)
-    kernel = (f"k_spmv_coded<double,{lanes},{spmv_threads}>" if prec == hd.CODED
-              else f"k_spmv<double,{lanes},{spmv_threads}>")
+    kernel = {"coded": f"k_spmv_coded<double,{lanes},{spmv_threads}>", "f64": f"k_spmv<double,{lanes},{spmv_threads}>",
+              "tiled": "k_spmv_tiled"}[key]
     traffic = measured_traffic(kernel, n, two_s)
     out = {
         "metric": METRIC,
@@ -267,8 +271,11 @@ def main():
             "ncv": args.ncv,
             "lanes_per_row": lanes,
             "threads_per_block": spmv_threads,
-            "operator": ("exact FP64 values in the coded CRS (int32 column + 16-bit dictionary code)"
-                         if prec == hd.CODED else "FP64 CRS (int64 row pointers, int32 columns, FP64 values)"),
+            "operator": {
+                "coded": "exact FP64 values in the coded CRS (int32 column + 16-bit dictionary code)",
+                "f64": "FP64 CRS (int64 row pointers, int32 columns, FP64 values)",
+                "tiled": "exact FP64 values, tiled coded layout (64-row groups, x lines staged in LDS, "
+                         "4-byte entries)"}[key],
         },
         "solve_s": {op_name: round(step_s, 4)},
         "matvecs": {op_name: res.matvecs},
@@ -279,6 +286,7 @@ def main():
         "abs_err_between_fp64_formats": err,
         "basis_s": round(t_basis, 4),
         "build_s": round(t_build, 3),
+        "tile_s": round(t_tile, 3),
         "build_ms": {k: round(info[k], 2) for k in ("t_count_ms", "t_scan_ms", "t_fill_ms", "t_sort_ms")},
         "spmv_ms": {k: round(v, 4) for k, v in sp_ms.items()},
         "spmv_gbs": {k: round(sp_b[k] / sp_ms[k] / 1e6, 1) for k in fmts},

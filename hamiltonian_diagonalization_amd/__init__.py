@@ -32,7 +32,7 @@ HD_OK, HD_ERR_ARG, HD_ERR_HIP, HD_ERR_NOMEM, HD_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 HD_ERR_NOT_CONVERGED, HD_ERR_EMPTY = 5, 6
 SECTOR_NONE, SECTOR_LZ, SECTOR_K2D, SECTOR_K1D = 0, 1, 2, 3
 LAYOUT_SPHERE, LAYOUT_OFL = 0, 1
-F64, F32, CODED = 0, 1, 2
+F64, F32, CODED, TILED = 0, 1, 2, 3
 STORAGE_FULL, STORAGE_UPPER_REF = 0, 1
 START_RANDOM, START_USER = 0, 1
 ZERO_TOL = 1e-14  # _SPARSE_ZERO_TOL_ (src/hamiltonians/hamiltonian_data.hpp:54)
@@ -56,6 +56,11 @@ class _HamInfo(ctypes.Structure):
                 ("max_row", ctypes.c_int64), ("t_count_ms", ctypes.c_double),
                 ("t_scan_ms", ctypes.c_double), ("t_fill_ms", ctypes.c_double),
                 ("t_sort_ms", ctypes.c_double), ("device_bytes", ctypes.c_size_t)]
+
+
+class _TileInfo(ctypes.Structure):
+    _fields_ = [("groups", ctypes.c_int64), ("entries", ctypes.c_int64), ("lines", ctypes.c_int64),
+                ("offsets", ctypes.c_int64), ("spmv_bytes", ctypes.c_int64), ("build_ms", ctypes.c_double)]
 
 
 class _LanczosOpts(ctypes.Structure):
@@ -120,6 +125,9 @@ SIGNATURES = {
     "hd_ham_copy_crs": (ctypes.c_int, [_P, _I64, _I64, _I32, _P, _P, _P]),
     "hd_ham_device_crs": (ctypes.c_int, [_P, _PP, _PP, _PP, _PP]),
     "hd_ham_device_coded": (ctypes.c_int, [_P, _PP, _PP, _PP, ctypes.POINTER(_I32)]),
+    "hd_ham_tile": (ctypes.c_int, [_P]),
+    "hd_ham_tile_info": (ctypes.c_int, [_P, ctypes.POINTER(_TileInfo)]),
+    "hd_ham_device_tiled": (ctypes.c_int, [_P, _PP, _PP, _PP, _PP, _PP]),
     "hd_spmv": (ctypes.c_int, [_P, _I32, _P, _P]),
     "hd_spmv_host": (ctypes.c_int, [_P, _P, _P]),
     "hd_spmv_time": (ctypes.c_int, [_P, _I32, _I32, ctypes.POINTER(_D)]),
@@ -390,8 +398,24 @@ class Hamiltonian:
         _check(lib().hd_ham_device_coded(self.h, *[ctypes.byref(x) for x in p], ctypes.byref(nd)))
         return tuple(x.value or 0 for x in p) + (nd.value,)
 
+    def tile(self) -> dict:
+        """Builds the tiled layout of the coded operator (precision TILED)."""
+        _check(lib().hd_ham_tile(self.h))
+        return self.tile_info()
+
+    def tile_info(self) -> dict:
+        i = _TileInfo()
+        _check(lib().hd_ham_tile_info(self.h, ctypes.byref(i)))
+        return {f: getattr(i, f) for f, _ in _TileInfo._fields_}
+
+    def device_tiled(self):
+        """Device pointers (bases, entries, lines, offsets, diagonal) of the tiled layout."""
+        p = [ctypes.c_void_p() for _ in range(5)]
+        _check(lib().hd_ham_device_tiled(self.h, *[ctypes.byref(x) for x in p]))
+        return tuple(x.value or 0 for x in p)
+
     def vec_dtype(self, precision=F64):
-        if precision in (F64, CODED):
+        if precision in (F64, CODED, TILED):
             return np.complex128 if self.is_complex else np.float64
         return np.complex64 if self.is_complex else np.float32
 
@@ -426,6 +450,8 @@ class Hamiltonian:
         """Algorithmic HBM bytes of one SpMV: nnz*(4 + sizeof(value)) +
         dim*(8 + sizeof(x) + sizeof(y)), x counted once."""
         z = 2 if self.is_complex else 1
+        if precision == TILED:
+            return self.tile_info()["spmv_bytes"]
         if precision == CODED:
             # column + 16-bit code per entry; rowptr, diagonal, x, y per row
             return self.nnz * 6 + self.dim * (8 + 3 * 8 * z)

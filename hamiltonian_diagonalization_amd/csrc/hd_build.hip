@@ -973,6 +973,7 @@ int hd_ham_destroy(hd_hamiltonian* h)
     dfree(h->d_code);
     dfree(h->d_diag);
     dfree(h->d_dict);
+    tiled_free(h);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return HD_OK;

@@ -275,7 +275,15 @@ struct hd_hamiltonian {
     void* d_dict = nullptr;   // double or double2, ndict entries
     int ndict = 0;
     int lanes_per_row = 0;    // SpMV lanes per row (0 = pick on first use)
-    int spmv_sched[3] = {-1, -1, -1};  // SpMV schedule per precision (hd_precision); -1 = default
+    int spmv_sched[4] = {-1, -1, -1, -1};  // SpMV schedule per precision (hd_precision); -1 = default
+    // tiled coded layout (hd_tiled.hip): 64-row groups, x lines staged in LDS
+    int64_t tg_groups = 0, tg_entries = 0, tg_lines = 0, tg_offs = 0;
+    int64_t* d_tg_base = nullptr;  // entry, line and offset-table bases, 3 x (groups + 1)
+    uint32_t* d_tg_entries = nullptr;
+    uint32_t* d_tg_lines = nullptr;
+    uint32_t* d_tg_offs = nullptr;
+    double* d_tg_diag = nullptr;
+    double t_tile = 0;
     double t_count = 0, t_scan = 0, t_fill = 0, t_sort = 0;
     size_t bytes = 0;
     hd_lanczos_state* lz = nullptr;  // eigenvectors of the last Lanczos call
@@ -297,6 +305,11 @@ int build_batch_sector(const hd_basis* b, const hd_tables* const* tables, int n,
                        hd_hamiltonian** out);
 int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y,
                 hipStream_t stream);
+// Tiled coded layout (hd_tiled.hip).
+int tiled_build(hd_hamiltonian* h);
+void tiled_free(hd_hamiltonian* h);
+int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s);
+size_t tiled_bytes(const hd_hamiltonian* h);
 // Cyclic Jacobi on a symmetric row-major matrix (destroyed): ascending
 // eigenvalues w, eigenvectors as the columns of Z (hd_lanczos.hip).
 void jacobi_eigh(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& Z);

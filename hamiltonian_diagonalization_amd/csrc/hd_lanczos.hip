@@ -600,8 +600,12 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
     L.nb = (int)std::max<int64_t>(1, std::min<int64_t>((h->dim + kVB - 1) / kVB, (int64_t)n_cu * 4));
     const bool mixed = o.precision == HD_F32;
     if (mixed && !h->d_val32) return fail(HD_ERR_ARG, "FP32 Lanczos needs FP32 values (value_mask & 2)");
-    if (!h->d_val64 && !h->d_code) return fail(HD_ERR_ARG, "Lanczos needs FP64 or coded values for the FP64 operator");
-    L.prec64 = (o.precision == HD_CODED || !h->d_val64) ? HD_CODED : HD_F64;
+    if (!h->d_val64 && !h->d_code && !h->d_tg_base)
+        return fail(HD_ERR_ARG, "Lanczos needs FP64, coded or tiled values for the FP64 operator");
+    if (o.precision == HD_TILED && !h->d_tg_base) return fail(HD_ERR_ARG, "no tiled layout (hd_ham_tile)");
+    L.prec64 = o.precision == HD_TILED ? HD_TILED
+               : (o.precision == HD_CODED || !h->d_val64) ? (h->d_code ? HD_CODED : HD_TILED)
+                                                          : HD_F64;
     HD_CHECK(L.alloc(mixed));
     const double tol = o.tol > 0 ? o.tol : 1e-10;
     const double sw_tol = o.f32_switch_tol > 0 ? o.f32_switch_tol : 1e-6;

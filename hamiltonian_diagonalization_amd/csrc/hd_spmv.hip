@@ -236,7 +236,8 @@ int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y, 
 {
     if (h->storage != HD_STORAGE_FULL)
         return fail(HD_ERR_ARG, "SpMV needs full (both-triangle) storage");
-    if (precision < 0 || precision > 2) return fail(HD_ERR_ARG, "unknown SpMV precision");
+    if (precision < 0 || precision > 3) return fail(HD_ERR_ARG, "unknown SpMV precision");
+    if (precision == HD_TILED) return tiled_launch(h, x, y, stream);
     const int si = h->spmv_sched[precision] >= 0 ? h->spmv_sched[precision] : pick_default_sched(h);
     if (h->dim == 0) return HD_OK;
     if (precision == HD_CODED) {
@@ -334,8 +335,8 @@ int hd_spmv_host(const hd_hamiltonian* h, const void* h_x, void* h_y)
 {
     if (!h || !h_x || !h_y) return fail(HD_ERR_ARG, "hd_spmv_host: null argument");
     const size_t bytes = (size_t)h->dim * 8 * (h->is_complex ? 2 : 1);
-    const int prec = h->d_val64 ? HD_F64 : (h->d_code ? HD_CODED : -1);
-    if (prec < 0) return fail(HD_ERR_ARG, "Simple() needs FP64 or coded values");
+    const int prec = h->d_val64 ? HD_F64 : h->d_code ? HD_CODED : h->d_tg_base ? HD_TILED : -1;
+    if (prec < 0) return fail(HD_ERR_ARG, "Simple() needs FP64, coded or tiled values");
     void *x = nullptr, *y = nullptr;
     HD_CHECK(dalloc(&x, bytes, "Simple() x"));
     int st = dalloc(&y, bytes, "Simple() y");
@@ -423,7 +424,12 @@ int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes)
 
 int hd_spmv_get_schedule(const hd_hamiltonian* h, int32_t precision, int32_t* lanes, int32_t* threads)
 {
-    if (!h || !lanes || !threads || precision < 0 || precision > 2) return fail(HD_ERR_ARG, "bad argument");
+    if (!h || !lanes || !threads || precision < 0 || precision > 3) return fail(HD_ERR_ARG, "bad argument");
+    if (precision == HD_TILED) {  // one schedule: 16 lanes per row, 1024 threads per 64-row group
+        *lanes = 16;
+        *threads = 1024;
+        return HD_OK;
+    }
     const int si = h->spmv_sched[precision] >= 0 ? h->spmv_sched[precision] : pick_default_sched(h);
     *lanes = kScheds[si].lpr;
     *threads = kScheds[si].nt;

@@ -1,0 +1,497 @@
+// hd_tiled.hip -- the coded operator re-laid out for x reuse through LDS.
+//
+// The CRS SpMV is bound by its x gathers: every 8-byte gather pulls a whole
+// cache line from L2 into L1, and a row's ~330 columns lie in ~265 distinct
+// 128-byte lines.  Consecutive rows are numerically adjacent states that differ
+// in their low orbitals, and one hop applied to each of them lands on nearby
+// ranks, so 64 consecutive rows together touch only 0.06-0.08 distinct lines
+// per stored entry (N=12, 2S=33 sphere).  The tiled layout makes that reuse
+// explicit:
+//
+//   group    = 64 consecutive rows (one 1024-thread block, 16 lanes per row)
+//   lines    = the sorted distinct x lines (16 doubles) the group's off-diagonal
+//              entries read, split into chunks of 256 lines (32 KiB of LDS)
+//   entry    = uint32: bits 24-31 line slot within the chunk, 20-23 word
+//              within the line, 0-19 value code (bit 19 sign, bits 0-18
+//              dictionary index into the coded format's FP64 dictionary)
+//   offsets  = per (chunk, row) start of the entries, chunk-major, plus the
+//              total: the entries of (chunk c, row r) are contiguous and in the
+//              row's column order
+//   diagonal = FP64 per row, applied after the off-diagonal sum
+//
+// The SpMV block stages each chunk's lines into LDS with coalesced 16-byte
+// loads (the next chunk's lines are in flight while the current one is
+// processed), then every lane walks its row's entries of that chunk reading x
+// and the dictionary from LDS.  HBM carries 4 bytes per entry plus the line lists
+// and offsets; the operator is the coded format's exact FP64 matrix.
+//
+// The layout is built on the device from the coded CRS, one block per group:
+// an LDS hash set collects the group's lines, a bitonic sort orders them,
+// per-(chunk, row) counts give the offsets, and each entry is written to its
+// slot.  Every write is bounds-checked against the group's counts.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "hd_internal.hpp"
+
+namespace hd {
+
+namespace {
+
+constexpr int kTG = 64;           // rows per group
+constexpr int kTC = 256;          // lines per chunk
+constexpr int kLineWords = 16;    // doubles per 128-byte line
+constexpr int kHashSlots = 4096;  // LDS hash set of the group's lines
+constexpr int kMaxLines = 3072;   // distinct lines per group (load factor 0.75)
+constexpr int kMaxChunks = kMaxLines / kTC;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr int kBuildThreads = 512;
+constexpr int kTPR = 16;                   // SpMV lanes per row
+constexpr int kU = 8;                      // entries each lane has in flight
+constexpr int kSpmvThreads = kTG * kTPR;
+
+__device__ __forceinline__ uint32_t line_hash(uint32_t line)
+{
+    return (line * 2654435761u) >> 20;  // 12 bits
+}
+
+// Inserts `line`; returns 1 when it was new, 0 when present, -1 when the
+// probe sequence found no room (bounded by the table size).
+__device__ __forceinline__ int hash_insert(uint32_t* keys, uint32_t line)
+{
+    uint32_t h = line_hash(line);
+    for (int probe = 0; probe < kHashSlots; ++probe) {
+        const uint32_t old = atomicCAS(&keys[h], kEmpty, line);
+        if (old == kEmpty) return 1;
+        if (old == line) return 0;
+        h = (h + 1) & (kHashSlots - 1);
+    }
+    return -1;
+}
+
+__device__ __forceinline__ int hash_find(const uint32_t* keys, uint32_t line)
+{
+    uint32_t h = line_hash(line);
+    for (int probe = 0; probe < kHashSlots; ++probe) {
+        const uint32_t k = keys[h];
+        if (k == line) return (int)h;
+        if (k == kEmpty) return -1;
+        h = (h + 1) & (kHashSlots - 1);
+    }
+    return -1;
+}
+
+// Pass 1: distinct lines and off-diagonal entries per group; flags groups
+// whose lines overflow the hash set.
+__global__ void __launch_bounds__(kBuildThreads)
+k_tile_count(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+             const uint16_t* __restrict__ code, int64_t* __restrict__ n_lines, int64_t* __restrict__ n_entries,
+             int* __restrict__ overflow)
+{
+    __shared__ uint32_t keys[kHashSlots];
+    __shared__ int s_lines, s_entries, s_over;
+    const int64_t g = blockIdx.x;
+    const int64_t r0 = g * kTG, r1 = min(r0 + kTG, dim);
+    for (int i = threadIdx.x; i < kHashSlots; i += kBuildThreads) keys[i] = kEmpty;
+    if (threadIdx.x == 0) s_lines = s_entries = s_over = 0;
+    __syncthreads();
+    const int64_t e0 = rowptr[r0], e1 = rowptr[r1];
+    int my_lines = 0, my_entries = 0, over = 0;
+    for (int64_t j = e0 + threadIdx.x; j < e1; j += kBuildThreads) {
+        if (code[j] == 0xFFFFu) continue;
+        ++my_entries;
+        const int r = hash_insert(keys, (uint32_t)col[j] >> 4);
+        if (r > 0) ++my_lines;
+        else if (r < 0) over = 1;
+    }
+    atomicAdd(&s_lines, my_lines);
+    atomicAdd(&s_entries, my_entries);
+    if (over) atomicOr(&s_over, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        n_lines[g] = s_lines;
+        n_entries[g] = s_entries;
+        if (s_over || s_lines > kMaxLines) atomicOr(overflow, 1);
+    }
+}
+
+// Pass 2: sorted line list, (chunk, row) offsets, entries and the diagonal.
+__global__ void __launch_bounds__(kBuildThreads)
+k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+            const uint16_t* __restrict__ code, const double* __restrict__ diag, const int64_t* __restrict__ base,
+            int64_t groups, uint32_t* __restrict__ entries, uint32_t* __restrict__ lines,
+            uint32_t* __restrict__ offs, double* __restrict__ tdiag, int* __restrict__ error)
+{
+    __shared__ uint32_t keys[kHashSlots];
+    __shared__ uint32_t sorted[kHashSlots];
+    __shared__ uint16_t rank_of[kHashSlots];
+    __shared__ int counts[kMaxChunks * kTG];
+    __shared__ int cum[kMaxChunks * kTG];
+    __shared__ uint32_t offl[kMaxChunks * kTG + 1];
+    __shared__ int64_t rp[kTG + 1];
+    __shared__ int64_t dpos[kTG];
+    __shared__ int s_n;
+    const int64_t g = blockIdx.x;
+    const int64_t r0 = g * kTG, r1 = min(r0 + kTG, dim);
+    const int nrows = (int)(r1 - r0);
+    const int64_t* ebase = base;
+    const int64_t* lbase = base + (groups + 1);
+    const int64_t* obase = base + 2 * (groups + 1);
+    for (int i = threadIdx.x; i < kHashSlots; i += kBuildThreads) {
+        keys[i] = kEmpty;
+        sorted[i] = kEmpty;
+    }
+    for (int i = threadIdx.x; i < kMaxChunks * kTG; i += kBuildThreads) counts[i] = 0;
+    for (int i = threadIdx.x; i <= nrows; i += kBuildThreads) rp[i] = rowptr[r0 + i];
+    for (int i = threadIdx.x; i < kTG; i += kBuildThreads) dpos[i] = -1;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const int64_t e0 = rp[0], e1 = rp[nrows];
+    // the group's lines, and each row's diagonal position
+    for (int64_t j = e0 + threadIdx.x; j < e1; j += kBuildThreads) {
+        if (code[j] == 0xFFFFu) {
+            int r = 0;
+            while (r + 1 < nrows && rp[r + 1] <= j) ++r;
+            dpos[r] = j;
+            continue;
+        }
+        (void)hash_insert(keys, (uint32_t)col[j] >> 4);
+    }
+    __syncthreads();
+    // compact, then bitonic sort (ascending; kEmpty pads the tail)
+    for (int i = threadIdx.x; i < kHashSlots; i += kBuildThreads) {
+        const uint32_t k = keys[i];
+        if (k != kEmpty) sorted[atomicAdd(&s_n, 1)] = k;
+    }
+    __syncthreads();
+    const int L = s_n;
+    int P = 1;
+    while (P < L) P <<= 1;
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int jj = k >> 1; jj > 0; jj >>= 1) {
+            for (int i = threadIdx.x; i < P; i += kBuildThreads) {
+                const int ixj = i ^ jj;
+                if (ixj > i) {
+                    const uint32_t a = sorted[i], b = sorted[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        sorted[i] = b;
+                        sorted[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int64_t lb = lbase[g];
+    const int64_t lend = lbase[g + 1];
+    for (int i = threadIdx.x; i < L; i += kBuildThreads) {
+        const int slot = hash_find(keys, sorted[i]);
+        if (slot >= 0) rank_of[slot] = (uint16_t)i;
+        if (lb + i < lend) lines[lb + i] = sorted[i];
+        else atomicOr(error, 1);
+    }
+    if (threadIdx.x == 0 && L != lend - lb) atomicOr(error, 2);
+    __syncthreads();
+    // entries per (chunk, row)
+    for (int64_t j = e0 + threadIdx.x; j < e1; j += kBuildThreads) {
+        if (code[j] == 0xFFFFu) continue;
+        int r = 0;
+        while (r + 1 < nrows && rp[r + 1] <= j) ++r;
+        const int slot = hash_find(keys, (uint32_t)col[j] >> 4);
+        const int c = slot >= 0 ? rank_of[slot] / kTC : 0;
+        if (slot < 0 || c >= kMaxChunks) atomicOr(error, 4);
+        else atomicAdd(&counts[c * kTG + r], 1);
+    }
+    __syncthreads();
+    const int nch = (L + kTC - 1) / kTC;
+    if (threadIdx.x == 0) {
+        // chunk-major exclusive scan into the offset table; cum = entries of
+        // the row in earlier chunks
+        const int64_t ob = obase[g];
+        const int64_t oend = obase[g + 1];
+        int run = 0;
+        int idx = 0;
+        for (int c = 0; c < nch; ++c)
+            for (int r = 0; r < kTG; ++r, ++idx) {
+                offl[idx] = (uint32_t)run;
+                run += counts[c * kTG + r];
+            }
+        offl[idx] = (uint32_t)run;
+        if (ob + idx + 1 != oend || run != ebase[g + 1] - ebase[g]) atomicOr(error, 8);
+    }
+    __syncthreads();
+    {
+        const int64_t ob = obase[g];
+        const int64_t n_off = obase[g + 1] - ob;
+        for (int i = threadIdx.x; i <= nch * kTG && i < n_off; i += kBuildThreads) offs[ob + i] = offl[i];
+    }
+    for (int r = threadIdx.x; r < kTG; r += kBuildThreads) {
+        int s = 0;
+        for (int c = 0; c < nch; ++c) {
+            cum[c * kTG + r] = s;
+            s += counts[c * kTG + r];
+        }
+    }
+    __syncthreads();
+    // entries: offset of (chunk, row) + index among the row's entries in the chunk
+    const int64_t eb = ebase[g];
+    const int64_t en = ebase[g + 1] - eb;
+    for (int64_t j = e0 + threadIdx.x; j < e1; j += kBuildThreads) {
+        const uint32_t q = code[j];
+        if (q == 0xFFFFu) continue;
+        int r = 0;
+        while (r + 1 < nrows && rp[r + 1] <= j) ++r;
+        const uint32_t cj = (uint32_t)col[j];
+        const int slot = hash_find(keys, cj >> 4);
+        if (slot < 0) continue;  // flagged above
+        const int rank = rank_of[slot];
+        const int c = rank / kTC;
+        if (c >= nch) continue;
+        const int64_t i_off = (j - rp[r]) - (dpos[r] >= 0 && dpos[r] < j ? 1 : 0);
+        const int64_t pos = (int64_t)offl[c * kTG + r] + (i_off - cum[c * kTG + r]);
+        if (pos < 0 || pos >= en) {
+            atomicOr(error, 16);
+            continue;
+        }
+        const uint32_t code20 = ((q >> 15) << 19) | (q & 0x7FFFu);
+        entries[eb + pos] = ((uint32_t)(rank % kTC) << 24) | ((cj & 15u) << 20) | code20;
+    }
+    for (int r = threadIdx.x; r < nrows; r += kBuildThreads)
+        tdiag[r0 + r] = dpos[r] >= 0 ? diag[r0 + r] : 0.0;
+}
+
+// y = H x on the tiled layout: one block per group, kTPR lanes per row.  The
+// group's line ids are staged in LDS once; the x lines of chunk c+1 are loaded
+// into registers while chunk c is processed (double buffering through the
+// register file), and each lane prefetches kU entries at a time.
+__global__ void __launch_bounds__(kSpmvThreads)
+k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, const uint32_t* __restrict__ entries,
+             const uint32_t* __restrict__ lines, const uint32_t* __restrict__ offs, const double* __restrict__ tdiag,
+             const double* __restrict__ dict, int ndict, int x_aligned, const double* __restrict__ x,
+             double* __restrict__ y)
+{
+    constexpr int PER = (kTC * 8) / kSpmvThreads;  // 16-byte pieces per thread per chunk
+    static_assert(PER * kSpmvThreads == kTC * 8, "pieces must divide evenly");
+    __shared__ double xs[kTC * kLineWords];
+    __shared__ uint32_t ids[kMaxLines];
+    extern __shared__ double sdict[];
+    for (int i = threadIdx.x; i < ndict; i += kSpmvThreads) sdict[i] = dict[i];
+    const int64_t g = blockIdx.x;
+    const int64_t lb = base[groups + 1 + g], le = base[groups + 1 + g + 1];
+    const uint32_t* off = offs + base[2 * (groups + 1) + g];
+    const uint32_t* ent = entries + base[g];
+    const int r = threadIdx.x / kTPR, sub = threadIdx.x % kTPR;
+    const int64_t row = g * kTG + r;
+    const int nl = (int)min(le - lb, (int64_t)kMaxLines);
+    const int nch = (nl + kTC - 1) / kTC;
+    for (int i = threadIdx.x; i < nl; i += kSpmvThreads) ids[i] = lines[lb + i];
+    __syncthreads();
+    double2 pre[PER];
+    auto fetch = [&](int c) {
+        const int n_here = min(kTC, nl - c * kTC);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = threadIdx.x + k * kSpmvThreads;
+            pre[k] = make_double2(0.0, 0.0);
+            if (i < n_here * 8) {
+                const int64_t xi = (int64_t)ids[c * kTC + (i >> 3)] * kLineWords + (i & 7) * 2;
+                if (x_aligned && xi + 1 < dim) pre[k] = *reinterpret_cast<const double2*>(x + xi);
+                else pre[k] = make_double2(xi < dim ? x[xi] : 0.0, xi + 1 < dim ? x[xi + 1] : 0.0);
+            }
+        }
+    };
+    if (nch > 0) fetch(0);
+    double acc = 0.0;
+    for (int c = 0; c < nch; ++c) {
+        __syncthreads();  // chunk c-1 consumed
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = threadIdx.x + k * kSpmvThreads;
+            *reinterpret_cast<double2*>(&xs[(i >> 3) * kLineWords + (i & 7) * 2]) = pre[k];
+        }
+        __syncthreads();
+        if (c + 1 < nch) fetch(c + 1);
+        const uint32_t jb = off[c * kTG + r], je = off[c * kTG + r + 1];
+        for (uint32_t j0 = jb + sub; j0 < je; j0 += kTPR * kU) {
+            uint32_t e[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const uint32_t j = j0 + u * kTPR;
+                e[u] = j < je ? ent[j] : kEmpty;
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (e[u] == kEmpty) continue;
+                double v = sdict[e[u] & 0x7FFFFu];
+                if (e[u] & 0x80000u) v = -v;
+                acc = fma(v, xs[(e[u] >> 24) * kLineWords + ((e[u] >> 20) & 15u)], acc);
+            }
+        }
+    }
+#pragma unroll
+    for (int m = kTPR / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m, 64);
+    if (row < dim && sub == 0) y[row] = fma(tdiag[row], x[row], acc);
+}
+
+}  // namespace
+
+int tiled_build(hd_hamiltonian* h)
+{
+    if (!h->d_code) return fail(HD_ERR_ARG, "the tiled layout is built from the coded values (value_mask & 4)");
+    if (h->is_complex) return fail(HD_ERR_UNSUPPORTED, "the tiled layout is implemented for real matrices");
+    if (h->storage != HD_STORAGE_FULL) return fail(HD_ERR_ARG, "the tiled layout needs full storage");
+    if (h->ndict > (1 << 19)) return fail(HD_ERR_UNSUPPORTED, "dictionary too large for the tiled layout");
+    if ((size_t)h->ndict * 8 + sizeof(double) * kTC * kLineWords + sizeof(uint32_t) * kMaxLines > 96 * 1024)
+        return fail(HD_ERR_UNSUPPORTED, "dictionary too large for LDS next to the line buffer");
+    tiled_free(h);
+    if (h->dim == 0) return HD_OK;
+    const hipStream_t st = h->stream;
+    const int64_t G = (h->dim + kTG - 1) / kTG;
+    EventTimer tm;
+    tm.start(st);
+    int64_t *d_nl = nullptr, *d_ne = nullptr;
+    int* d_flag = nullptr;
+    int rc = dalloc_t(&d_nl, (size_t)G, "tile line counts");
+    if (rc == HD_OK) rc = dalloc_t(&d_ne, (size_t)G, "tile entry counts");
+    if (rc == HD_OK) rc = dalloc_t(&d_flag, 2, "tile flags");
+    HipErrors q;
+    std::vector<int64_t> nl(G), ne(G);
+    int flags[2] = {0, 0};
+    if (rc == HD_OK) {
+        HD_ENQ(q, hipMemsetAsync(d_flag, 0, 2 * sizeof(int), st));
+        hipLaunchKernelGGL(k_tile_count, dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim, h->d_rowptr,
+                           h->d_col, h->d_code, d_nl, d_ne, d_flag);
+        HD_ENQ(q, hipGetLastError());
+        HD_ENQ(q, hipMemcpyAsync(nl.data(), d_nl, G * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        HD_ENQ(q, hipMemcpyAsync(ne.data(), d_ne, G * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        HD_ENQ(q, hipMemcpyAsync(flags, d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        HD_ENQ(q, hipStreamSynchronize(st));
+        rc = q.status();
+    }
+    if (rc == HD_OK && flags[0])
+        rc = fail(HD_ERR_UNSUPPORTED, "a row group reads more than 3072 distinct x lines");
+    std::vector<int64_t> base(3 * (size_t)(G + 1));
+    if (rc == HD_OK) {
+        int64_t e = 0, l = 0, o = 0;
+        for (int64_t g = 0; g < G; ++g) {
+            base[g] = e;
+            base[G + 1 + g] = l;
+            base[2 * (G + 1) + g] = o;
+            e += ne[g];
+            l += nl[g];
+            o += ((nl[g] + kTC - 1) / kTC) * kTG + 1;
+        }
+        base[G] = e;
+        base[2 * G + 1] = l;
+        base[3 * G + 2] = o;
+        h->tg_groups = G;
+        h->tg_entries = e;
+        h->tg_lines = l;
+        h->tg_offs = o;
+    }
+    if (rc == HD_OK) rc = dalloc_t(&h->d_tg_base, base.size(), "tile bases");
+    if (rc == HD_OK) rc = dalloc_t(&h->d_tg_entries, (size_t)std::max<int64_t>(h->tg_entries, 1), "tile entries");
+    if (rc == HD_OK) rc = dalloc_t(&h->d_tg_lines, (size_t)std::max<int64_t>(h->tg_lines, 1), "tile lines");
+    if (rc == HD_OK) rc = dalloc_t(&h->d_tg_offs, (size_t)std::max<int64_t>(h->tg_offs, 1), "tile offsets");
+    if (rc == HD_OK) rc = dalloc_t(&h->d_tg_diag, (size_t)h->dim, "tile diagonal");
+    if (rc == HD_OK) {
+        HD_ENQ(q, hipMemcpyAsync(h->d_tg_base, base.data(), base.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        HD_ENQ(q, hipMemsetAsync(d_flag + 1, 0, sizeof(int), st));
+        hipLaunchKernelGGL(k_tile_fill, dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim, h->d_rowptr, h->d_col,
+                           h->d_code, static_cast<const double*>(h->d_diag), h->d_tg_base, G, h->d_tg_entries,
+                           h->d_tg_lines, h->d_tg_offs, h->d_tg_diag, d_flag + 1);
+        HD_ENQ(q, hipGetLastError());
+        HD_ENQ(q, hipMemcpyAsync(flags + 1, d_flag + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        HD_ENQ(q, hipStreamSynchronize(st));
+        rc = q.status();
+        if (rc == HD_OK && flags[1])
+            rc = fail(HD_ERR_HIP, "tiled layout build found inconsistent counts (flags " + std::to_string(flags[1]) +
+                                      ")");
+    }
+    h->t_tile = tm.stop(st);
+    if (rc == HD_OK) rc = tm.status("tiled layout timing");
+    dfree(d_nl);
+    dfree(d_ne);
+    dfree(d_flag);
+    if (rc != HD_OK) {
+        tiled_free(h);
+        return rc;
+    }
+    h->bytes += (size_t)h->tg_entries * 4 + (size_t)h->tg_lines * 4 + (size_t)h->tg_offs * 4 + base.size() * 8 +
+                (size_t)h->dim * 8;
+    return HD_OK;
+}
+
+void tiled_free(hd_hamiltonian* h)
+{
+    dfree(h->d_tg_base);
+    dfree(h->d_tg_entries);
+    dfree(h->d_tg_lines);
+    dfree(h->d_tg_offs);
+    dfree(h->d_tg_diag);
+    h->d_tg_base = nullptr;
+    h->d_tg_entries = nullptr;
+    h->d_tg_lines = nullptr;
+    h->d_tg_offs = nullptr;
+    h->d_tg_diag = nullptr;
+    h->tg_groups = h->tg_entries = h->tg_lines = h->tg_offs = 0;
+}
+
+int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
+{
+    if (!h->d_tg_base) return fail(HD_ERR_ARG, "no tiled layout (hd_ham_tile)");
+    if (h->dim == 0) return HD_OK;
+    const size_t lds = sizeof(double) * (size_t)std::max(h->ndict, 1);
+    hipLaunchKernelGGL(k_spmv_tiled, dim3((unsigned)h->tg_groups), dim3(kSpmvThreads), lds, s, h->dim, h->tg_groups,
+                       h->d_tg_base, h->d_tg_entries, h->d_tg_lines, h->d_tg_offs, h->d_tg_diag,
+                       static_cast<const double*>(h->d_dict), h->ndict, ((uintptr_t)x & 15) == 0 ? 1 : 0,
+                       static_cast<const double*>(x), static_cast<double*>(y));
+    HD_HIP(hipGetLastError());
+    return HD_OK;
+}
+
+// HBM bytes one tiled SpMV must move: entries, offsets, line lists, group
+// bases, the dictionary, the diagonal, x once and y.
+size_t tiled_bytes(const hd_hamiltonian* h)
+{
+    return (size_t)h->tg_entries * 4 + (size_t)h->tg_offs * 4 + (size_t)h->tg_lines * 4 +
+           (size_t)(3 * (h->tg_groups + 1)) * 8 + (size_t)h->ndict * 8 + (size_t)h->dim * 8 * 3;
+}
+
+}  // namespace hd
+
+using namespace hd;
+
+extern "C" int hd_ham_tile(hd_hamiltonian* h)
+{
+    if (!h) return fail(HD_ERR_ARG, "hd_ham_tile: null handle");
+    return tiled_build(h);
+}
+
+extern "C" int hd_ham_tile_info(const hd_hamiltonian* h, hd_tile_info_t* info)
+{
+    if (!h || !info) return fail(HD_ERR_ARG, "hd_ham_tile_info: null argument");
+    if (!h->d_tg_base && h->dim > 0) return fail(HD_ERR_ARG, "no tiled layout (hd_ham_tile)");
+    info->groups = h->tg_groups;
+    info->entries = h->tg_entries;
+    info->lines = h->tg_lines;
+    info->offsets = h->tg_offs;
+    info->spmv_bytes = (int64_t)tiled_bytes(h);
+    info->build_ms = h->t_tile;
+    return HD_OK;
+}
+
+extern "C" int hd_ham_device_tiled(const hd_hamiltonian* h, const int64_t** d_base, const uint32_t** d_entries,
+                                   const uint32_t** d_lines, const uint32_t** d_offsets, const double** d_diag)
+{
+    if (!h) return fail(HD_ERR_ARG, "hd_ham_device_tiled: null handle");
+    if (d_base) *d_base = h->d_tg_base;
+    if (d_entries) *d_entries = h->d_tg_entries;
+    if (d_lines) *d_lines = h->d_tg_lines;
+    if (d_offsets) *d_offsets = h->d_tg_offs;
+    if (d_diag) *d_diag = h->d_tg_diag;
+    return HD_OK;
+}

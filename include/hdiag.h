@@ -68,8 +68,11 @@ typedef enum {
  *   HD_CODED  16-bit codes into an FP64 dictionary of the distinct
  *             off-diagonal values (sign in bit 15, 0xFFFF = diagonal, kept in a
  *             separate FP64 array): the exact FP64 operator in 6 bytes per
- *             entry; FP64 vectors and accumulation. */
-typedef enum { HD_F64 = 0, HD_F32 = 1, HD_CODED = 2 } hd_precision;
+ *             entry; FP64 vectors and accumulation.
+ *   HD_TILED  the coded operator re-laid out in 64-row groups whose x lines
+ *             are staged through LDS (hd_ham_tile): 4 bytes per entry, the
+ *             same exact FP64 matrix; real matrices. */
+typedef enum { HD_F64 = 0, HD_F32 = 1, HD_CODED = 2, HD_TILED = 3 } hd_precision;
 
 typedef enum {
     HD_STORAGE_FULL = 0,     /* both triangles, int64 row pointers (GPU SpMV layout) */
@@ -207,6 +210,28 @@ int hd_stream_copy(const void* d_src, void* d_dst, size_t bytes, int32_t iters, 
 /* Rows per wavefront schedule of the SpMV (lanes per row 4..64, 0 = autotune). */
 int hd_spmv_set_lanes(hd_hamiltonian* h, int32_t lanes_per_row);
 int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes_per_row);
+/* Tiled layout of the coded operator (no reference counterpart; it replaces
+ * the column gathers of SymmetricMatrixVectorMultiply,
+ * src/utilities/mathematics/sparse_linear_algebra.hpp:218-312, for the
+ * product path): built on the device from the coded values, after which
+ * precision HD_TILED is available to hd_spmv and hd_lanczos.  Fails with
+ * HD_ERR_UNSUPPORTED for complex matrices or when a 64-row group reads more
+ * than 3072 distinct 128-byte lines of x. */
+int hd_ham_tile(hd_hamiltonian* h);
+typedef struct {
+    int64_t groups;         /* 64-row groups */
+    int64_t entries;        /* off-diagonal entries (4 bytes each) */
+    int64_t lines;          /* staged x lines, summed over groups */
+    int64_t offsets;        /* (chunk, row) offset table entries */
+    int64_t spmv_bytes;     /* algorithmic HBM bytes of one product */
+    double build_ms;
+} hd_tile_info_t;
+int hd_ham_tile_info(const hd_hamiltonian* h, hd_tile_info_t* info);
+/* Zero-copy view of the tiled layout: group bases (3 x (groups+1) int64:
+ * entries, lines, offset table), entries, line lists, offset tables and the
+ * FP64 diagonal.  Owned by the handle. */
+int hd_ham_device_tiled(const hd_hamiltonian* h, const int64_t** d_base, const uint32_t** d_entries,
+                        const uint32_t** d_lines, const uint32_t** d_offsets, const double** d_diag);
 /* The SpMV schedule of one value format: lanes per row and threads per block
  * (each format is autotuned separately; no reference counterpart). */
 int hd_spmv_get_schedule(const hd_hamiltonian* h, int32_t precision, int32_t* lanes_per_row, int32_t* threads);

@@ -211,6 +211,57 @@ def test_spmv_complex_large_dictionary(oracle):
         assert np.array_equal(h.spmv(e, precision=hd.CODED), h.spmv(e))
 
 
+@pytest.mark.parametrize("n,two_s,lz", [(4, 11, 0), (5, 13, 3), (6, 15, 0), (6, 15, 24)])
+def test_tiled_columns_exact(oracle, n, two_s, lz):
+    """Every column H e_i through the tiled layout equals the oracle's matrix
+    column bit for bit (one nonzero term per output), so the layout holds
+    exactly the coded operator's entries, signs and diagonal."""
+    say("tiled columns", n, two_s, lz)
+    vm = models.coulomb_list(two_s, {1: 1.0, 3: 0.37, 5: 0.11})
+    b = hd.FockBasis(hd.Sector.lz(lz, two_s), n, two_s + 1)
+    t = models.sphere_tables(two_s, vm)
+    K = two_s + 1
+    full = oracle.build(n, K, b.states(), oracle.LAYOUT_SPHERE, K, t.ktable, t.vtable, full=True)
+    h = hd.Hamiltonian(b, t, values=(hd.CODED,))
+    ti = h.tile()
+    assert ti["groups"] == (b.dim + 63) // 64
+    assert ti["entries"] == len(full.col) - int(np.sum(full.col == np.repeat(np.arange(b.dim), np.diff(full.rowptr))))
+    M = full.to_dense(b.dim, mirror=False)
+    cols = range(b.dim) if b.dim <= 1500 else np.random.default_rng(0).choice(b.dim, 600, replace=False)
+    for i in cols:
+        e = np.zeros(b.dim)
+        e[i] = 1.0
+        assert np.array_equal(h.spmv(e, precision=hd.TILED), M[:, i]), i
+
+
+def test_tiled_spmv_and_lanczos_config1(oracle):
+    say("tiled spmv + lanczos N=8 2S=21")
+    b, t = models.sphere_problem(8, 21)
+    up = oracle.build(8, 22, b.states(), oracle.LAYOUT_SPHERE, 22, t.ktable, t.vtable)
+    h = hd.Hamiltonian(b, t, values=(hd.F64, hd.CODED))
+    ti = h.tile()
+    say(ti)
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=b.dim)
+    want = oracle.spmv_sym(up, x)
+    got = h.spmv(x, precision=hd.TILED)
+    assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+    r64 = h.lanczos(nev=5, tol=1e-12, seed=0)
+    rt = h.lanczos(nev=5, tol=1e-12, seed=0, precision=hd.TILED)
+    assert rt.converged and np.all(rt.residuals <= 1e-12)
+    assert np.max(np.abs(rt.evals - r64.evals)) < 1e-11
+    # the tiled product is deterministic
+    assert np.array_equal(h.spmv(x, precision=hd.TILED), got)
+
+
+def test_tiled_rejects_complex():
+    kt, vt, eps = models.ofl_tables(3, 3, 1)
+    b = hd.FockBasis(hd.Sector.k2d(0, 0, 3, 3), 3, 9)
+    h = hd.Hamiltonian(b, hd.TermTables(hd.LAYOUT_OFL, 9, kt, vt, eps), values=(hd.CODED,))
+    with pytest.raises(hd.HDError):
+        h.tile()
+
+
 def test_lanczos_config1(oracle):
     say("lanczos N=8 2S=21 nev=5")
     b, t = models.sphere_problem(8, 21)
