@@ -60,7 +60,7 @@ class _HamInfo(ctypes.Structure):
 
 class _TileInfo(ctypes.Structure):
     _fields_ = [("groups", ctypes.c_int64), ("entries", ctypes.c_int64), ("lines", ctypes.c_int64),
-                ("offsets", ctypes.c_int64), ("spmv_bytes", ctypes.c_int64), ("build_ms", ctypes.c_double)]
+                ("chunks", ctypes.c_int64), ("spmv_bytes", ctypes.c_int64), ("build_ms", ctypes.c_double)]
 
 
 class _LanczosOpts(ctypes.Structure):

@@ -14,9 +14,11 @@
 //   entry    = uint32: bits 24-31 line slot within the chunk, 20-23 word
 //              within the line, 0-19 value code (bit 19 sign, bits 0-18
 //              dictionary index into the coded format's FP64 dictionary)
-//   offsets  = per (chunk, row) start of the entries, chunk-major, plus the
-//              total: the entries of (chunk c, row r) are contiguous and in the
-//              row's column order
+//   chunks   = 256 consecutive lines; per chunk the 64 row starts and the
+//              chunk end (the entries of (chunk c, row r) are
+//              contiguous and in the row's column order; each chunk's block
+//              is padded to a multiple of 4 entries), then the chunk line
+//              starts: 66 words per chunk plus one
 //   diagonal = FP64 per row, applied after the off-diagonal sum
 //
 // The SpMV block stages each chunk's lines into LDS with coalesced 16-byte
@@ -27,8 +29,9 @@
 //
 // The layout is built on the device from the coded CRS, one block per group:
 // an LDS hash set collects the group's lines, a bitonic sort orders them,
-// per-(chunk, row) counts give the offsets, and each entry is written to its
-// slot.  Every write is bounds-checked against the group's counts.
+// per-(chunk, row) counts give the row starts, and each entry is written to
+// its slot.  Every write is
+// bounds-checked against the group's counts.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,10 +49,10 @@ constexpr int kLineWords = 16;    // doubles per 128-byte line
 constexpr int kHashSlots = 4096;  // LDS hash set of the group's lines
 constexpr int kMaxLines = 3072;   // distinct lines per group (load factor 0.75)
 constexpr int kMaxChunks = kMaxLines / kTC;
+constexpr int kU = 8;             // entries each SpMV lane has in flight
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kBuildThreads = 512;
 constexpr int kTPR = 16;                   // SpMV lanes per row
-constexpr int kU = 8;                      // entries each lane has in flight
 constexpr int kSpmvThreads = kTG * kTPR;
 
 __device__ __forceinline__ uint32_t line_hash(uint32_t line)
@@ -57,15 +60,14 @@ __device__ __forceinline__ uint32_t line_hash(uint32_t line)
     return (line * 2654435761u) >> 20;  // 12 bits
 }
 
-// Inserts `line`; returns 1 when it was new, 0 when present, -1 when the
-// probe sequence found no room (bounded by the table size).
+// Inserts `line` (if absent) and returns its slot, or -1 when the probe
+// sequence found no room (bounded by the table size).
 __device__ __forceinline__ int hash_insert(uint32_t* keys, uint32_t line)
 {
     uint32_t h = line_hash(line);
     for (int probe = 0; probe < kHashSlots; ++probe) {
         const uint32_t old = atomicCAS(&keys[h], kEmpty, line);
-        if (old == kEmpty) return 1;
-        if (old == line) return 0;
+        if (old == kEmpty || old == line) return (int)h;
         h = (h + 1) & (kHashSlots - 1);
     }
     return -1;
@@ -83,224 +85,265 @@ __device__ __forceinline__ int hash_find(const uint32_t* keys, uint32_t line)
     return -1;
 }
 
-// Pass 1: distinct lines and off-diagonal entries per group; flags groups
-// whose lines overflow the hash set.
-__global__ void __launch_bounds__(kBuildThreads)
-k_tile_count(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-             const uint16_t* __restrict__ code, int64_t* __restrict__ n_lines, int64_t* __restrict__ n_entries,
-             int* __restrict__ overflow)
+// The plan of one row group, shared by both build passes: the sorted distinct
+// lines, their chunks (kTC consecutive lines each), entry counts per (chunk,
+// row), the per-chunk row starts T (chunk blocks padded to 4 entries) and the
+// row's entries in earlier chunks (cum).
+struct TilePlanLds {
+    uint32_t keys[kHashSlots];
+    uint32_t sorted[kHashSlots];
+    uint16_t rank_of[kHashSlots];
+    uint8_t chunk_of[kHashSlots];
+    uint32_t counts[kMaxChunks * kTG];
+    uint32_t cum[kMaxChunks * kTG];
+    uint32_t T[kMaxChunks * (kTG + 1)];
+    uint32_t ls[kMaxChunks + 1];
+    uint32_t cbase[kMaxChunks + 1];
+    int64_t rp[kTG + 1];
+    int64_t dpos[kTG];
+    int n_lines, n_chunks, flags;
+};
+
+__device__ int row_of(const TilePlanLds& S, int nrows, int64_t j)
 {
-    __shared__ uint32_t keys[kHashSlots];
-    __shared__ int s_lines, s_entries, s_over;
-    const int64_t g = blockIdx.x;
-    const int64_t r0 = g * kTG, r1 = min(r0 + kTG, dim);
-    for (int i = threadIdx.x; i < kHashSlots; i += kBuildThreads) keys[i] = kEmpty;
-    if (threadIdx.x == 0) s_lines = s_entries = s_over = 0;
-    __syncthreads();
-    const int64_t e0 = rowptr[r0], e1 = rowptr[r1];
-    int my_lines = 0, my_entries = 0, over = 0;
-    for (int64_t j = e0 + threadIdx.x; j < e1; j += kBuildThreads) {
-        if (code[j] == 0xFFFFu) continue;
-        ++my_entries;
-        const int r = hash_insert(keys, (uint32_t)col[j] >> 4);
-        if (r > 0) ++my_lines;
-        else if (r < 0) over = 1;
-    }
-    atomicAdd(&s_lines, my_lines);
-    atomicAdd(&s_entries, my_entries);
-    if (over) atomicOr(&s_over, 1);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        n_lines[g] = s_lines;
-        n_entries[g] = s_entries;
-        if (s_over || s_lines > kMaxLines) atomicOr(overflow, 1);
-    }
+    int r = 0;
+    while (r + 1 < nrows && S.rp[r + 1] <= j) ++r;
+    return r;
 }
 
-// Pass 2: sorted line list, (chunk, row) offsets, entries and the diagonal.
-__global__ void __launch_bounds__(kBuildThreads)
-k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-            const uint16_t* __restrict__ code, const double* __restrict__ diag, const int64_t* __restrict__ base,
-            int64_t groups, uint32_t* __restrict__ entries, uint32_t* __restrict__ lines,
-            uint32_t* __restrict__ offs, double* __restrict__ tdiag, int* __restrict__ error)
+// Builds the plan in LDS; S.flags != 0 when the group does not fit.
+__device__ void tile_plan(TilePlanLds& S, int64_t dim, const int64_t* __restrict__ rowptr,
+                          const int32_t* __restrict__ col, const uint16_t* __restrict__ code, int64_t g)
 {
-    __shared__ uint32_t keys[kHashSlots];
-    __shared__ uint32_t sorted[kHashSlots];
-    __shared__ uint16_t rank_of[kHashSlots];
-    __shared__ int counts[kMaxChunks * kTG];
-    __shared__ int cum[kMaxChunks * kTG];
-    __shared__ uint32_t offl[kMaxChunks * kTG + 1];
-    __shared__ int64_t rp[kTG + 1];
-    __shared__ int64_t dpos[kTG];
-    __shared__ int s_n;
-    const int64_t g = blockIdx.x;
+    const int t = threadIdx.x;
     const int64_t r0 = g * kTG, r1 = min(r0 + kTG, dim);
     const int nrows = (int)(r1 - r0);
-    const int64_t* ebase = base;
-    const int64_t* lbase = base + (groups + 1);
-    const int64_t* obase = base + 2 * (groups + 1);
-    for (int i = threadIdx.x; i < kHashSlots; i += kBuildThreads) {
-        keys[i] = kEmpty;
-        sorted[i] = kEmpty;
+    for (int i = t; i < kHashSlots; i += kBuildThreads) {
+        S.keys[i] = kEmpty;
+        S.sorted[i] = kEmpty;
     }
-    for (int i = threadIdx.x; i < kMaxChunks * kTG; i += kBuildThreads) counts[i] = 0;
-    for (int i = threadIdx.x; i <= nrows; i += kBuildThreads) rp[i] = rowptr[r0 + i];
-    for (int i = threadIdx.x; i < kTG; i += kBuildThreads) dpos[i] = -1;
-    if (threadIdx.x == 0) s_n = 0;
+    for (int i = t; i < kMaxChunks * kTG; i += kBuildThreads) S.counts[i] = 0;
+    for (int i = t; i <= nrows; i += kBuildThreads) S.rp[i] = rowptr[r0 + i];
+    for (int i = t; i < kTG; i += kBuildThreads) S.dpos[i] = -1;
+    if (t == 0) S.n_lines = S.n_chunks = S.flags = 0;
     __syncthreads();
-    const int64_t e0 = rp[0], e1 = rp[nrows];
-    // the group's lines, and each row's diagonal position
-    for (int64_t j = e0 + threadIdx.x; j < e1; j += kBuildThreads) {
+    const int64_t e0 = S.rp[0], e1 = S.rp[nrows];
+    for (int64_t j = e0 + t; j < e1; j += kBuildThreads) {
         if (code[j] == 0xFFFFu) {
-            int r = 0;
-            while (r + 1 < nrows && rp[r + 1] <= j) ++r;
-            dpos[r] = j;
+            S.dpos[row_of(S, nrows, j)] = j;
             continue;
         }
-        (void)hash_insert(keys, (uint32_t)col[j] >> 4);
+        if (hash_insert(S.keys, (uint32_t)col[j] >> 4) < 0) atomicOr(&S.flags, 1);
     }
     __syncthreads();
-    // compact, then bitonic sort (ascending; kEmpty pads the tail)
-    for (int i = threadIdx.x; i < kHashSlots; i += kBuildThreads) {
-        const uint32_t k = keys[i];
-        if (k != kEmpty) sorted[atomicAdd(&s_n, 1)] = k;
-    }
+    for (int i = t; i < kHashSlots; i += kBuildThreads)
+        if (S.keys[i] != kEmpty) S.sorted[atomicAdd(&S.n_lines, 1)] = S.keys[i];
     __syncthreads();
-    const int L = s_n;
+    const int L = S.n_lines;
+    if (L > kMaxLines) {
+        if (t == 0) S.flags |= 2;
+        __syncthreads();
+        return;
+    }
     int P = 1;
     while (P < L) P <<= 1;
-    for (int k = 2; k <= P; k <<= 1) {
+    for (int k = 2; k <= P; k <<= 1)
         for (int jj = k >> 1; jj > 0; jj >>= 1) {
-            for (int i = threadIdx.x; i < P; i += kBuildThreads) {
+            for (int i = t; i < P; i += kBuildThreads) {
                 const int ixj = i ^ jj;
                 if (ixj > i) {
-                    const uint32_t a = sorted[i], b = sorted[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((a > b) == up) {
-                        sorted[i] = b;
-                        sorted[ixj] = a;
+                    const uint32_t a = S.sorted[i], b = S.sorted[ixj];
+                    if ((a > b) == ((i & k) == 0)) {
+                        S.sorted[i] = b;
+                        S.sorted[ixj] = a;
                     }
                 }
             }
             __syncthreads();
         }
+    // ranks, and the chunks: 256 consecutive lines each
+    for (int i = t; i < L; i += kBuildThreads) {
+        const int h = hash_find(S.keys, S.sorted[i]);
+        S.rank_of[h] = (uint16_t)i;
+        S.chunk_of[i] = (uint8_t)(i / kTC);
     }
-    const int64_t lb = lbase[g];
-    const int64_t lend = lbase[g + 1];
-    for (int i = threadIdx.x; i < L; i += kBuildThreads) {
-        const int slot = hash_find(keys, sorted[i]);
-        if (slot >= 0) rank_of[slot] = (uint16_t)i;
-        if (lb + i < lend) lines[lb + i] = sorted[i];
-        else atomicOr(error, 1);
-    }
-    if (threadIdx.x == 0 && L != lend - lb) atomicOr(error, 2);
+    const int nch = (L + kTC - 1) / kTC;
+    for (int c = t; c <= nch; c += kBuildThreads) S.ls[c] = (uint32_t)min(c * kTC, L);
+    if (t == 0) S.n_chunks = nch;
     __syncthreads();
     // entries per (chunk, row)
-    for (int64_t j = e0 + threadIdx.x; j < e1; j += kBuildThreads) {
+    for (int64_t j = e0 + t; j < e1; j += kBuildThreads) {
         if (code[j] == 0xFFFFu) continue;
-        int r = 0;
-        while (r + 1 < nrows && rp[r + 1] <= j) ++r;
-        const int slot = hash_find(keys, (uint32_t)col[j] >> 4);
-        const int c = slot >= 0 ? rank_of[slot] / kTC : 0;
-        if (slot < 0 || c >= kMaxChunks) atomicOr(error, 4);
-        else atomicAdd(&counts[c * kTG + r], 1);
+        const int h = hash_find(S.keys, (uint32_t)col[j] >> 4);
+        if (h < 0) {
+            atomicOr(&S.flags, 8);
+            continue;
+        }
+        atomicAdd(&S.counts[S.chunk_of[S.rank_of[h]] * kTG + row_of(S, nrows, j)], 1u);
     }
     __syncthreads();
-    const int nch = (L + kTC - 1) / kTC;
-    if (threadIdx.x == 0) {
-        // chunk-major exclusive scan into the offset table; cum = entries of
-        // the row in earlier chunks
-        const int64_t ob = obase[g];
-        const int64_t oend = obase[g + 1];
-        int run = 0;
-        int idx = 0;
-        for (int c = 0; c < nch; ++c)
-            for (int r = 0; r < kTG; ++r, ++idx) {
-                offl[idx] = (uint32_t)run;
-                run += counts[c * kTG + r];
-            }
-        offl[idx] = (uint32_t)run;
-        if (ob + idx + 1 != oend || run != ebase[g + 1] - ebase[g]) atomicOr(error, 8);
+    // T: per chunk, the row starts; chunk blocks start on 4-entry boundaries
+    if (t < nch) {
+        uint32_t run = 0;
+        for (int r = 0; r < kTG; ++r) {
+            S.T[t * (kTG + 1) + r] = run;
+            run += S.counts[t * kTG + r];
+        }
+        S.T[t * (kTG + 1) + kTG] = run;  // chunk size, made absolute below
     }
-    __syncthreads();
-    {
-        const int64_t ob = obase[g];
-        const int64_t n_off = obase[g + 1] - ob;
-        for (int i = threadIdx.x; i <= nch * kTG && i < n_off; i += kBuildThreads) offs[ob + i] = offl[i];
-    }
-    for (int r = threadIdx.x; r < kTG; r += kBuildThreads) {
-        int s = 0;
+    if (t >= 64 && t < 64 + kTG) {
+        const int r = t - 64;
+        uint32_t run = 0;
         for (int c = 0; c < nch; ++c) {
-            cum[c * kTG + r] = s;
-            s += counts[c * kTG + r];
+            S.cum[c * kTG + r] = run;
+            run += S.counts[c * kTG + r];
         }
     }
     __syncthreads();
-    // entries: offset of (chunk, row) + index among the row's entries in the chunk
-    const int64_t eb = ebase[g];
-    const int64_t en = ebase[g + 1] - eb;
-    for (int64_t j = e0 + threadIdx.x; j < e1; j += kBuildThreads) {
+    if (t == 0) {
+        uint32_t b = 0;
+        for (int c = 0; c < nch; ++c) {
+            S.cbase[c] = b;
+            b += (S.T[c * (kTG + 1) + kTG] + 3u) & ~3u;
+        }
+        S.cbase[nch] = b;
+    }
+    __syncthreads();
+    for (int i = t; i < nch * (kTG + 1); i += kBuildThreads) S.T[i] += S.cbase[i / (kTG + 1)];
+    __syncthreads();
+}
+
+// Pass 1: per group, the distinct lines, the chunks and the padded entries.
+__global__ void __launch_bounds__(kBuildThreads)
+k_tile_count(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+             const uint16_t* __restrict__ code, int64_t* __restrict__ n_lines, int64_t* __restrict__ n_entries,
+             int64_t* __restrict__ n_chunks, int* __restrict__ overflow)
+{
+    __shared__ TilePlanLds S;
+    const int64_t g = blockIdx.x;
+    tile_plan(S, dim, rowptr, col, code, g);
+    if (threadIdx.x == 0) {
+        n_lines[g] = S.n_lines;
+        n_chunks[g] = S.n_chunks;
+        n_entries[g] = S.flags ? 0 : S.cbase[S.n_chunks];
+        if (S.flags) atomicOr(overflow, S.flags);
+    }
+}
+
+// Pass 2: the line list, the chunk table (T per chunk, then the chunk line
+// starts), the entries and the diagonal.
+__global__ void __launch_bounds__(kBuildThreads)
+k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+            const uint16_t* __restrict__ code, const double* __restrict__ diag, const int64_t* __restrict__ base,
+            int64_t groups, uint32_t* __restrict__ entries, uint32_t* __restrict__ lines,
+            uint32_t* __restrict__ meta, double* __restrict__ tdiag, int* __restrict__ error)
+{
+    __shared__ TilePlanLds S;
+    const int t = threadIdx.x;
+    const int64_t g = blockIdx.x;
+    tile_plan(S, dim, rowptr, col, code, g);
+    const int64_t r0 = g * kTG, r1 = min(r0 + kTG, dim);
+    const int nrows = (int)(r1 - r0);
+    const int64_t eb = base[g], en = base[g + 1] - eb;
+    const int64_t lb = base[groups + 1 + g], ln = base[groups + 1 + g + 1] - lb;
+    const int64_t mb = base[2 * (groups + 1) + g], mn = base[2 * (groups + 1) + g + 1] - mb;
+    const int L = S.n_lines, nch = S.n_chunks;
+    if (S.flags || L != ln || (int64_t)nch * (kTG + 2) + 1 != mn || (int64_t)S.cbase[nch] != en) {
+        if (t == 0) atomicOr(error, 16 | S.flags);
+        return;
+    }
+    for (int i = t; i < L; i += kBuildThreads) lines[lb + i] = S.sorted[i];
+    for (int i = t; i < nch * (kTG + 1); i += kBuildThreads) meta[mb + i] = S.T[i];
+    for (int i = t; i <= nch; i += kBuildThreads) meta[mb + nch * (kTG + 1) + i] = S.ls[i];
+    // padding entries never read by a row: kEmpty
+    for (int c = 0; c < nch; ++c) {
+        const uint32_t a = S.T[c * (kTG + 1) + kTG], b = S.cbase[c + 1];
+        if (t < (int)(b - a)) entries[eb + a + t] = kEmpty;
+    }
+    const int64_t e0 = S.rp[0], e1 = S.rp[nrows];
+    for (int64_t j = e0 + t; j < e1; j += kBuildThreads) {
         const uint32_t q = code[j];
         if (q == 0xFFFFu) continue;
-        int r = 0;
-        while (r + 1 < nrows && rp[r + 1] <= j) ++r;
+        const int r = row_of(S, nrows, j);
         const uint32_t cj = (uint32_t)col[j];
-        const int slot = hash_find(keys, cj >> 4);
-        if (slot < 0) continue;  // flagged above
-        const int rank = rank_of[slot];
-        const int c = rank / kTC;
-        if (c >= nch) continue;
-        const int64_t i_off = (j - rp[r]) - (dpos[r] >= 0 && dpos[r] < j ? 1 : 0);
-        const int64_t pos = (int64_t)offl[c * kTG + r] + (i_off - cum[c * kTG + r]);
-        if (pos < 0 || pos >= en) {
-            atomicOr(error, 16);
+        const int h = hash_find(S.keys, cj >> 4);
+        if (h < 0) continue;  // flagged by the plan
+        const int rank = S.rank_of[h];
+        const int c = S.chunk_of[rank];
+        const int64_t i_off = (j - S.rp[r]) - (S.dpos[r] >= 0 && S.dpos[r] < j ? 1 : 0);
+        const int64_t pos = (int64_t)S.T[c * (kTG + 1) + r] + (i_off - S.cum[c * kTG + r]);
+        if (pos < S.T[c * (kTG + 1) + r] || pos >= S.T[c * (kTG + 1) + r + 1] || pos >= en) {
+            atomicOr(error, 32);
             continue;
         }
         const uint32_t code20 = ((q >> 15) << 19) | (q & 0x7FFFu);
-        entries[eb + pos] = ((uint32_t)(rank % kTC) << 24) | ((cj & 15u) << 20) | code20;
+        entries[eb + pos] = ((uint32_t)(rank - (int)S.ls[c]) << 24) | ((cj & 15u) << 20) | code20;
     }
-    for (int r = threadIdx.x; r < nrows; r += kBuildThreads)
-        tdiag[r0 + r] = dpos[r] >= 0 ? diag[r0 + r] : 0.0;
+    for (int r = t; r < nrows; r += kBuildThreads) tdiag[r0 + r] = S.dpos[r] >= 0 ? diag[r0 + r] : 0.0;
 }
 
 // y = H x on the tiled layout: one block per group, kTPR lanes per row.  The
-// group's line ids are staged in LDS once; the x lines of chunk c+1 are loaded
-// into registers while chunk c is processed (double buffering through the
-// register file), and each lane prefetches kU entries at a time.
+// group's line ids and chunk table are staged in LDS once.  While chunk c is
+// processed, chunk c+1's x lines (block-wide) and each lane's first kU entries
+// of chunk c+1 are already in flight into registers (double buffering through
+// the register file); only rows with more than kTPR * kU entries in a chunk
+// issue further loads inside it.
 __global__ void __launch_bounds__(kSpmvThreads)
 k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, const uint32_t* __restrict__ entries,
-             const uint32_t* __restrict__ lines, const uint32_t* __restrict__ offs, const double* __restrict__ tdiag,
+             const uint32_t* __restrict__ lines, const uint32_t* __restrict__ meta, const double* __restrict__ tdiag,
              const double* __restrict__ dict, int ndict, int x_aligned, const double* __restrict__ x,
              double* __restrict__ y)
 {
-    constexpr int PER = (kTC * 8) / kSpmvThreads;  // 16-byte pieces per thread per chunk
+    constexpr int PER = (kTC * 8) / kSpmvThreads;  // 16-byte x pieces per thread per chunk
     static_assert(PER * kSpmvThreads == kTC * 8, "pieces must divide evenly");
+    constexpr int kMeta = kMaxChunks * (kTG + 2) + 1;
     __shared__ double xs[kTC * kLineWords];
     __shared__ uint32_t ids[kMaxLines];
+    __shared__ uint32_t sm[kMeta];
     extern __shared__ double sdict[];
     for (int i = threadIdx.x; i < ndict; i += kSpmvThreads) sdict[i] = dict[i];
     const int64_t g = blockIdx.x;
     const int64_t lb = base[groups + 1 + g], le = base[groups + 1 + g + 1];
-    const uint32_t* off = offs + base[2 * (groups + 1) + g];
+    const int64_t mb = base[2 * (groups + 1) + g], me = base[2 * (groups + 1) + g + 1];
+    const int nmeta = (int)min(me - mb, (int64_t)kMeta);
+    const int nch = (nmeta - 1) / (kTG + 2);
     const uint32_t* ent = entries + base[g];
     const int r = threadIdx.x / kTPR, sub = threadIdx.x % kTPR;
     const int64_t row = g * kTG + r;
     const int nl = (int)min(le - lb, (int64_t)kMaxLines);
-    const int nch = (nl + kTC - 1) / kTC;
     for (int i = threadIdx.x; i < nl; i += kSpmvThreads) ids[i] = lines[lb + i];
+    for (int i = threadIdx.x; i < nmeta; i += kSpmvThreads) sm[i] = meta[mb + i];
     __syncthreads();
+    const uint32_t* T = sm;
+    const uint32_t* LS = sm + nch * (kTG + 1);
     double2 pre[PER];
+    uint32_t e[kU];
     auto fetch = [&](int c) {
-        const int n_here = min(kTC, nl - c * kTC);
+        const int l0 = (int)LS[c], n_here = (int)LS[c + 1] - l0;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int i = threadIdx.x + k * kSpmvThreads;
             pre[k] = make_double2(0.0, 0.0);
             if (i < n_here * 8) {
-                const int64_t xi = (int64_t)ids[c * kTC + (i >> 3)] * kLineWords + (i & 7) * 2;
+                const int64_t xi = (int64_t)ids[l0 + (i >> 3)] * kLineWords + (i & 7) * 2;
                 if (x_aligned && xi + 1 < dim) pre[k] = *reinterpret_cast<const double2*>(x + xi);
                 else pre[k] = make_double2(xi < dim ? x[xi] : 0.0, xi + 1 < dim ? x[xi + 1] : 0.0);
             }
+        }
+        const uint32_t j0 = T[c * (kTG + 1) + r] + sub, je = T[c * (kTG + 1) + r + 1];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint32_t j = j0 + u * kTPR;
+            e[u] = j < je ? ent[j] : kEmpty;
+        }
+    };
+    auto consume = [&](const uint32_t (&b)[kU], double& a) {
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (b[u] == kEmpty) continue;
+            double v = sdict[b[u] & 0x7FFFFu];
+            if (b[u] & 0x80000u) v = -v;
+            a = fma(v, xs[(b[u] >> 24) * kLineWords + ((b[u] >> 20) & 15u)], a);
         }
     };
     if (nch > 0) fetch(0);
@@ -312,23 +355,21 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
             const int i = threadIdx.x + k * kSpmvThreads;
             *reinterpret_cast<double2*>(&xs[(i >> 3) * kLineWords + (i & 7) * 2]) = pre[k];
         }
+        uint32_t cur[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) cur[u] = e[u];
         __syncthreads();
         if (c + 1 < nch) fetch(c + 1);
-        const uint32_t jb = off[c * kTG + r], je = off[c * kTG + r + 1];
-        for (uint32_t j0 = jb + sub; j0 < je; j0 += kTPR * kU) {
-            uint32_t e[kU];
+        consume(cur, acc);
+        // rows with more than kTPR * kU entries in this chunk
+        const uint32_t je = T[c * (kTG + 1) + r + 1];
+        for (uint32_t j0 = T[c * (kTG + 1) + r] + sub + kTPR * kU; j0 < je; j0 += kTPR * kU) {
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
                 const uint32_t j = j0 + u * kTPR;
-                e[u] = j < je ? ent[j] : kEmpty;
+                cur[u] = j < je ? ent[j] : kEmpty;
             }
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                if (e[u] == kEmpty) continue;
-                double v = sdict[e[u] & 0x7FFFFu];
-                if (e[u] & 0x80000u) v = -v;
-                acc = fma(v, xs[(e[u] >> 24) * kLineWords + ((e[u] >> 20) & 15u)], acc);
-            }
+            consume(cur, acc);
         }
     }
 #pragma unroll
@@ -344,7 +385,9 @@ int tiled_build(hd_hamiltonian* h)
     if (h->is_complex) return fail(HD_ERR_UNSUPPORTED, "the tiled layout is implemented for real matrices");
     if (h->storage != HD_STORAGE_FULL) return fail(HD_ERR_ARG, "the tiled layout needs full storage");
     if (h->ndict > (1 << 19)) return fail(HD_ERR_UNSUPPORTED, "dictionary too large for the tiled layout");
-    if ((size_t)h->ndict * 8 + sizeof(double) * kTC * kLineWords + sizeof(uint32_t) * kMaxLines > 96 * 1024)
+    if ((size_t)h->ndict * 8 + sizeof(double) * kTC * kLineWords +
+            sizeof(uint32_t) * (kMaxLines + kMaxChunks * (kTG + 2) + 1) >
+        80 * 1024)
         return fail(HD_ERR_UNSUPPORTED, "dictionary too large for LDS next to the line buffer");
     tiled_free(h);
     if (h->dim == 0) return HD_OK;
@@ -352,27 +395,30 @@ int tiled_build(hd_hamiltonian* h)
     const int64_t G = (h->dim + kTG - 1) / kTG;
     EventTimer tm;
     tm.start(st);
-    int64_t *d_nl = nullptr, *d_ne = nullptr;
+    int64_t *d_nl = nullptr, *d_ne = nullptr, *d_nc = nullptr;
     int* d_flag = nullptr;
     int rc = dalloc_t(&d_nl, (size_t)G, "tile line counts");
     if (rc == HD_OK) rc = dalloc_t(&d_ne, (size_t)G, "tile entry counts");
+    if (rc == HD_OK) rc = dalloc_t(&d_nc, (size_t)G, "tile chunk counts");
     if (rc == HD_OK) rc = dalloc_t(&d_flag, 2, "tile flags");
     HipErrors q;
-    std::vector<int64_t> nl(G), ne(G);
+    std::vector<int64_t> nl(G), ne(G), nc(G);
     int flags[2] = {0, 0};
     if (rc == HD_OK) {
         HD_ENQ(q, hipMemsetAsync(d_flag, 0, 2 * sizeof(int), st));
         hipLaunchKernelGGL(k_tile_count, dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim, h->d_rowptr,
-                           h->d_col, h->d_code, d_nl, d_ne, d_flag);
+                           h->d_col, h->d_code, d_nl, d_ne, d_nc, d_flag);
         HD_ENQ(q, hipGetLastError());
         HD_ENQ(q, hipMemcpyAsync(nl.data(), d_nl, G * sizeof(int64_t), hipMemcpyDeviceToHost, st));
         HD_ENQ(q, hipMemcpyAsync(ne.data(), d_ne, G * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        HD_ENQ(q, hipMemcpyAsync(nc.data(), d_nc, G * sizeof(int64_t), hipMemcpyDeviceToHost, st));
         HD_ENQ(q, hipMemcpyAsync(flags, d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
         HD_ENQ(q, hipStreamSynchronize(st));
         rc = q.status();
     }
     if (rc == HD_OK && flags[0])
-        rc = fail(HD_ERR_UNSUPPORTED, "a row group reads more than 3072 distinct x lines");
+        rc = fail(HD_ERR_UNSUPPORTED, "a row group does not fit the tiled layout (more than 3072 distinct x lines "
+                                      "or 32 chunks; flags " + std::to_string(flags[0]) + ")");
     std::vector<int64_t> base(3 * (size_t)(G + 1));
     if (rc == HD_OK) {
         int64_t e = 0, l = 0, o = 0;
@@ -380,9 +426,9 @@ int tiled_build(hd_hamiltonian* h)
             base[g] = e;
             base[G + 1 + g] = l;
             base[2 * (G + 1) + g] = o;
-            e += ne[g];
+            e += ne[g];  // a multiple of 4: every group starts 16-byte aligned
             l += nl[g];
-            o += ((nl[g] + kTC - 1) / kTC) * kTG + 1;
+            o += nc[g] * (kTG + 2) + 1;
         }
         base[G] = e;
         base[2 * G + 1] = l;
@@ -395,7 +441,7 @@ int tiled_build(hd_hamiltonian* h)
     if (rc == HD_OK) rc = dalloc_t(&h->d_tg_base, base.size(), "tile bases");
     if (rc == HD_OK) rc = dalloc_t(&h->d_tg_entries, (size_t)std::max<int64_t>(h->tg_entries, 1), "tile entries");
     if (rc == HD_OK) rc = dalloc_t(&h->d_tg_lines, (size_t)std::max<int64_t>(h->tg_lines, 1), "tile lines");
-    if (rc == HD_OK) rc = dalloc_t(&h->d_tg_offs, (size_t)std::max<int64_t>(h->tg_offs, 1), "tile offsets");
+    if (rc == HD_OK) rc = dalloc_t(&h->d_tg_offs, (size_t)std::max<int64_t>(h->tg_offs, 1), "tile chunk tables");
     if (rc == HD_OK) rc = dalloc_t(&h->d_tg_diag, (size_t)h->dim, "tile diagonal");
     if (rc == HD_OK) {
         HD_ENQ(q, hipMemcpyAsync(h->d_tg_base, base.data(), base.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
@@ -415,6 +461,7 @@ int tiled_build(hd_hamiltonian* h)
     if (rc == HD_OK) rc = tm.status("tiled layout timing");
     dfree(d_nl);
     dfree(d_ne);
+    dfree(d_nc);
     dfree(d_flag);
     if (rc != HD_OK) {
         tiled_free(h);
@@ -478,7 +525,7 @@ extern "C" int hd_ham_tile_info(const hd_hamiltonian* h, hd_tile_info_t* info)
     info->groups = h->tg_groups;
     info->entries = h->tg_entries;
     info->lines = h->tg_lines;
-    info->offsets = h->tg_offs;
+    info->chunks = (h->tg_offs - h->tg_groups) / (kTG + 2);
     info->spmv_bytes = (int64_t)tiled_bytes(h);
     info->build_ms = h->t_tile;
     return HD_OK;

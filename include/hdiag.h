@@ -220,9 +220,10 @@ int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes_per_row);
 int hd_ham_tile(hd_hamiltonian* h);
 typedef struct {
     int64_t groups;         /* 64-row groups */
-    int64_t entries;        /* off-diagonal entries (4 bytes each) */
+    int64_t entries;        /* stored entries (4 bytes each): the off-diagonal
+                               entries plus at most 3 padding entries per chunk */
     int64_t lines;          /* staged x lines, summed over groups */
-    int64_t offsets;        /* (chunk, row) offset table entries */
+    int64_t chunks;         /* chunks (at most 256 lines and 4096 entries each) */
     int64_t spmv_bytes;     /* algorithmic HBM bytes of one product */
     double build_ms;
 } hd_tile_info_t;
