@@ -60,7 +60,8 @@ class _HamInfo(ctypes.Structure):
 
 class _TileInfo(ctypes.Structure):
     _fields_ = [("groups", ctypes.c_int64), ("entries", ctypes.c_int64), ("lines", ctypes.c_int64),
-                ("chunks", ctypes.c_int64), ("spmv_bytes", ctypes.c_int64), ("build_ms", ctypes.c_double)]
+                ("chunks", ctypes.c_int64), ("rows_per_group", ctypes.c_int64), ("spmv_bytes", ctypes.c_int64),
+                ("build_ms", ctypes.c_double)]
 
 
 class _LanczosOpts(ctypes.Structure):
@@ -125,7 +126,7 @@ SIGNATURES = {
     "hd_ham_copy_crs": (ctypes.c_int, [_P, _I64, _I64, _I32, _P, _P, _P]),
     "hd_ham_device_crs": (ctypes.c_int, [_P, _PP, _PP, _PP, _PP]),
     "hd_ham_device_coded": (ctypes.c_int, [_P, _PP, _PP, _PP, ctypes.POINTER(_I32)]),
-    "hd_ham_tile": (ctypes.c_int, [_P]),
+    "hd_ham_tile": (ctypes.c_int, [_P, _I32]),
     "hd_ham_tile_info": (ctypes.c_int, [_P, ctypes.POINTER(_TileInfo)]),
     "hd_ham_device_tiled": (ctypes.c_int, [_P, _PP, _PP, _PP, _PP, _PP]),
     "hd_spmv": (ctypes.c_int, [_P, _I32, _P, _P]),
@@ -398,9 +399,10 @@ class Hamiltonian:
         _check(lib().hd_ham_device_coded(self.h, *[ctypes.byref(x) for x in p], ctypes.byref(nd)))
         return tuple(x.value or 0 for x in p) + (nd.value,)
 
-    def tile(self) -> dict:
-        """Builds the tiled layout of the coded operator (precision TILED)."""
-        _check(lib().hd_ham_tile(self.h))
+    def tile(self, rows_per_group: int = 0) -> dict:
+        """Builds the tiled layout of the coded operator (precision TILED);
+        rows_per_group 128 or 64, 0 = 128 with a fallback to 64."""
+        _check(lib().hd_ham_tile(self.h, rows_per_group))
         return self.tile_info()
 
     def tile_info(self) -> dict:

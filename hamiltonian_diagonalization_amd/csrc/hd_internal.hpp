@@ -276,8 +276,9 @@ struct hd_hamiltonian {
     int ndict = 0;
     int lanes_per_row = 0;    // SpMV lanes per row (0 = pick on first use)
     int spmv_sched[4] = {-1, -1, -1, -1};  // SpMV schedule per precision (hd_precision); -1 = default
-    // tiled coded layout (hd_tiled.hip): 64-row groups, x lines staged in LDS
+    // tiled coded layout (hd_tiled.hip): row groups, x lines staged in LDS
     int64_t tg_groups = 0, tg_entries = 0, tg_lines = 0, tg_offs = 0;
+    int tg_rows = 0;  // rows per group (128, or 64 for denser matrices)
     int64_t* d_tg_base = nullptr;  // entry, line and offset-table bases, 3 x (groups + 1)
     uint32_t* d_tg_entries = nullptr;
     uint32_t* d_tg_lines = nullptr;
@@ -306,7 +307,7 @@ int build_batch_sector(const hd_basis* b, const hd_tables* const* tables, int n,
 int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y,
                 hipStream_t stream);
 // Tiled coded layout (hd_tiled.hip).
-int tiled_build(hd_hamiltonian* h);
+int tiled_build(hd_hamiltonian* h, int rows_per_group);
 void tiled_free(hd_hamiltonian* h);
 int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s);
 size_t tiled_bytes(const hd_hamiltonian* h);

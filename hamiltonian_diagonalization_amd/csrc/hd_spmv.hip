@@ -425,8 +425,8 @@ int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes)
 int hd_spmv_get_schedule(const hd_hamiltonian* h, int32_t precision, int32_t* lanes, int32_t* threads)
 {
     if (!h || !lanes || !threads || precision < 0 || precision > 3) return fail(HD_ERR_ARG, "bad argument");
-    if (precision == HD_TILED) {  // one schedule: 16 lanes per row, 1024 threads per 64-row group
-        *lanes = 16;
+    if (precision == HD_TILED) {  // one schedule: 1024 threads per row group
+        *lanes = h->tg_rows ? 1024 / h->tg_rows : 0;
         *threads = 1024;
         return HD_OK;
     }

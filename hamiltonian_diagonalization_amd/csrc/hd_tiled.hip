@@ -43,21 +43,19 @@ namespace hd {
 
 namespace {
 
-constexpr int kTG = 64;           // rows per group
-constexpr int kTC = 256;          // lines per chunk
-constexpr int kLineWords = 16;    // doubles per 128-byte line
-constexpr int kHashSlots = 4096;  // LDS hash set of the group's lines
-constexpr int kMaxLines = 3072;   // distinct lines per group (load factor 0.75)
+constexpr int kTC = 256;           // lines per chunk
+constexpr int kLineWords = 16;     // doubles per 128-byte line
+constexpr int kHashSlots = 8192;   // LDS hash set of a group's lines
+constexpr int kMaxLines = 5120;    // distinct lines per group (load factor 0.625)
 constexpr int kMaxChunks = kMaxLines / kTC;
-constexpr int kU = 8;             // entries each SpMV lane has in flight
+constexpr int kU = 8;              // entries each SpMV lane has in flight
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kBuildThreads = 512;
-constexpr int kTPR = 16;                   // SpMV lanes per row
-constexpr int kSpmvThreads = kTG * kTPR;
+constexpr int kSpmvThreads = 1024;  // one block per group: kSpmvThreads / TG lanes per row
 
 __device__ __forceinline__ uint32_t line_hash(uint32_t line)
 {
-    return (line * 2654435761u) >> 20;  // 12 bits
+    return (line * 2654435761u) >> 19;  // 13 bits
 }
 
 // Inserts `line` (if absent) and returns its slot, or -1 when the probe
@@ -89,42 +87,50 @@ __device__ __forceinline__ int hash_find(const uint32_t* keys, uint32_t line)
 // lines, their chunks (kTC consecutive lines each), entry counts per (chunk,
 // row), the per-chunk row starts T (chunk blocks padded to 4 entries) and the
 // row's entries in earlier chunks (cum).
+template <int TG>
 struct TilePlanLds {
     uint32_t keys[kHashSlots];
     uint32_t sorted[kHashSlots];
     uint16_t rank_of[kHashSlots];
     uint8_t chunk_of[kHashSlots];
-    uint32_t counts[kMaxChunks * kTG];
-    uint32_t cum[kMaxChunks * kTG];
-    uint32_t T[kMaxChunks * (kTG + 1)];
+    uint32_t counts[kMaxChunks * TG];
+    uint32_t cum[kMaxChunks * TG];
+    uint32_t T[kMaxChunks * (TG + 1)];
     uint32_t ls[kMaxChunks + 1];
     uint32_t cbase[kMaxChunks + 1];
-    int64_t rp[kTG + 1];
-    int64_t dpos[kTG];
+    int64_t rp[TG + 1];
+    int64_t dpos[TG];
     int n_lines, n_chunks, flags;
 };
 
-__device__ int row_of(const TilePlanLds& S, int nrows, int64_t j)
+// Row (within the group) of entry j: the last r with rp[r] <= j.
+template <int TG>
+__device__ int row_of(const TilePlanLds<TG>& S, int nrows, int64_t j)
 {
-    int r = 0;
-    while (r + 1 < nrows && S.rp[r + 1] <= j) ++r;
-    return r;
+    int lo = 0, hi = nrows - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (S.rp[mid] <= j) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 // Builds the plan in LDS; S.flags != 0 when the group does not fit.
-__device__ void tile_plan(TilePlanLds& S, int64_t dim, const int64_t* __restrict__ rowptr,
+template <int TG>
+__device__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const int64_t* __restrict__ rowptr,
                           const int32_t* __restrict__ col, const uint16_t* __restrict__ code, int64_t g)
 {
     const int t = threadIdx.x;
-    const int64_t r0 = g * kTG, r1 = min(r0 + kTG, dim);
+    const int64_t r0 = g * TG, r1 = min(r0 + TG, dim);
     const int nrows = (int)(r1 - r0);
     for (int i = t; i < kHashSlots; i += kBuildThreads) {
         S.keys[i] = kEmpty;
         S.sorted[i] = kEmpty;
     }
-    for (int i = t; i < kMaxChunks * kTG; i += kBuildThreads) S.counts[i] = 0;
+    for (int i = t; i < kMaxChunks * TG; i += kBuildThreads) S.counts[i] = 0;
     for (int i = t; i <= nrows; i += kBuildThreads) S.rp[i] = rowptr[r0 + i];
-    for (int i = t; i < kTG; i += kBuildThreads) S.dpos[i] = -1;
+    for (int i = t; i < TG; i += kBuildThreads) S.dpos[i] = -1;
     if (t == 0) S.n_lines = S.n_chunks = S.flags = 0;
     __syncthreads();
     const int64_t e0 = S.rp[0], e1 = S.rp[nrows];
@@ -179,24 +185,24 @@ __device__ void tile_plan(TilePlanLds& S, int64_t dim, const int64_t* __restrict
             atomicOr(&S.flags, 8);
             continue;
         }
-        atomicAdd(&S.counts[S.chunk_of[S.rank_of[h]] * kTG + row_of(S, nrows, j)], 1u);
+        atomicAdd(&S.counts[S.chunk_of[S.rank_of[h]] * TG + row_of(S, nrows, j)], 1u);
     }
     __syncthreads();
     // T: per chunk, the row starts; chunk blocks start on 4-entry boundaries
     if (t < nch) {
         uint32_t run = 0;
-        for (int r = 0; r < kTG; ++r) {
-            S.T[t * (kTG + 1) + r] = run;
-            run += S.counts[t * kTG + r];
+        for (int r = 0; r < TG; ++r) {
+            S.T[t * (TG + 1) + r] = run;
+            run += S.counts[t * TG + r];
         }
-        S.T[t * (kTG + 1) + kTG] = run;  // chunk size, made absolute below
+        S.T[t * (TG + 1) + TG] = run;  // chunk size, made absolute below
     }
-    if (t >= 64 && t < 64 + kTG) {
+    if (t >= 64 && t < 64 + TG) {
         const int r = t - 64;
         uint32_t run = 0;
         for (int c = 0; c < nch; ++c) {
-            S.cum[c * kTG + r] = run;
-            run += S.counts[c * kTG + r];
+            S.cum[c * TG + r] = run;
+            run += S.counts[c * TG + r];
         }
     }
     __syncthreads();
@@ -204,22 +210,23 @@ __device__ void tile_plan(TilePlanLds& S, int64_t dim, const int64_t* __restrict
         uint32_t b = 0;
         for (int c = 0; c < nch; ++c) {
             S.cbase[c] = b;
-            b += (S.T[c * (kTG + 1) + kTG] + 3u) & ~3u;
+            b += (S.T[c * (TG + 1) + TG] + 3u) & ~3u;
         }
         S.cbase[nch] = b;
     }
     __syncthreads();
-    for (int i = t; i < nch * (kTG + 1); i += kBuildThreads) S.T[i] += S.cbase[i / (kTG + 1)];
+    for (int i = t; i < nch * (TG + 1); i += kBuildThreads) S.T[i] += S.cbase[i / (TG + 1)];
     __syncthreads();
 }
 
 // Pass 1: per group, the distinct lines, the chunks and the padded entries.
+template <int TG>
 __global__ void __launch_bounds__(kBuildThreads)
 k_tile_count(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
              const uint16_t* __restrict__ code, int64_t* __restrict__ n_lines, int64_t* __restrict__ n_entries,
              int64_t* __restrict__ n_chunks, int* __restrict__ overflow)
 {
-    __shared__ TilePlanLds S;
+    __shared__ TilePlanLds<TG> S;
     const int64_t g = blockIdx.x;
     tile_plan(S, dim, rowptr, col, code, g);
     if (threadIdx.x == 0) {
@@ -232,32 +239,33 @@ k_tile_count(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __r
 
 // Pass 2: the line list, the chunk table (T per chunk, then the chunk line
 // starts), the entries and the diagonal.
+template <int TG>
 __global__ void __launch_bounds__(kBuildThreads)
 k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
             const uint16_t* __restrict__ code, const double* __restrict__ diag, const int64_t* __restrict__ base,
             int64_t groups, uint32_t* __restrict__ entries, uint32_t* __restrict__ lines,
             uint32_t* __restrict__ meta, double* __restrict__ tdiag, int* __restrict__ error)
 {
-    __shared__ TilePlanLds S;
+    __shared__ TilePlanLds<TG> S;
     const int t = threadIdx.x;
     const int64_t g = blockIdx.x;
     tile_plan(S, dim, rowptr, col, code, g);
-    const int64_t r0 = g * kTG, r1 = min(r0 + kTG, dim);
+    const int64_t r0 = g * TG, r1 = min(r0 + TG, dim);
     const int nrows = (int)(r1 - r0);
     const int64_t eb = base[g], en = base[g + 1] - eb;
     const int64_t lb = base[groups + 1 + g], ln = base[groups + 1 + g + 1] - lb;
     const int64_t mb = base[2 * (groups + 1) + g], mn = base[2 * (groups + 1) + g + 1] - mb;
     const int L = S.n_lines, nch = S.n_chunks;
-    if (S.flags || L != ln || (int64_t)nch * (kTG + 2) + 1 != mn || (int64_t)S.cbase[nch] != en) {
+    if (S.flags || L != ln || (int64_t)nch * (TG + 2) + 1 != mn || (int64_t)S.cbase[nch] != en) {
         if (t == 0) atomicOr(error, 16 | S.flags);
         return;
     }
     for (int i = t; i < L; i += kBuildThreads) lines[lb + i] = S.sorted[i];
-    for (int i = t; i < nch * (kTG + 1); i += kBuildThreads) meta[mb + i] = S.T[i];
-    for (int i = t; i <= nch; i += kBuildThreads) meta[mb + nch * (kTG + 1) + i] = S.ls[i];
+    for (int i = t; i < nch * (TG + 1); i += kBuildThreads) meta[mb + i] = S.T[i];
+    for (int i = t; i <= nch; i += kBuildThreads) meta[mb + nch * (TG + 1) + i] = S.ls[i];
     // padding entries never read by a row: kEmpty
     for (int c = 0; c < nch; ++c) {
-        const uint32_t a = S.T[c * (kTG + 1) + kTG], b = S.cbase[c + 1];
+        const uint32_t a = S.T[c * (TG + 1) + TG], b = S.cbase[c + 1];
         if (t < (int)(b - a)) entries[eb + a + t] = kEmpty;
     }
     const int64_t e0 = S.rp[0], e1 = S.rp[nrows];
@@ -271,8 +279,8 @@ k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __re
         const int rank = S.rank_of[h];
         const int c = S.chunk_of[rank];
         const int64_t i_off = (j - S.rp[r]) - (S.dpos[r] >= 0 && S.dpos[r] < j ? 1 : 0);
-        const int64_t pos = (int64_t)S.T[c * (kTG + 1) + r] + (i_off - S.cum[c * kTG + r]);
-        if (pos < S.T[c * (kTG + 1) + r] || pos >= S.T[c * (kTG + 1) + r + 1] || pos >= en) {
+        const int64_t pos = (int64_t)S.T[c * (TG + 1) + r] + (i_off - S.cum[c * TG + r]);
+        if (pos < S.T[c * (TG + 1) + r] || pos >= S.T[c * (TG + 1) + r + 1] || pos >= en) {
             atomicOr(error, 32);
             continue;
         }
@@ -282,12 +290,13 @@ k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __re
     for (int r = t; r < nrows; r += kBuildThreads) tdiag[r0 + r] = S.dpos[r] >= 0 ? diag[r0 + r] : 0.0;
 }
 
-// y = H x on the tiled layout: one block per group, kTPR lanes per row.  The
+// y = H x on the tiled layout: one block of kSpmvThreads per group of TG rows.  The
 // group's line ids and chunk table are staged in LDS once.  While chunk c is
 // processed, chunk c+1's x lines (block-wide) and each lane's first kU entries
 // of chunk c+1 are already in flight into registers (double buffering through
 // the register file); only rows with more than kTPR * kU entries in a chunk
 // issue further loads inside it.
+template <int TG>
 __global__ void __launch_bounds__(kSpmvThreads)
 k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, const uint32_t* __restrict__ entries,
              const uint32_t* __restrict__ lines, const uint32_t* __restrict__ meta, const double* __restrict__ tdiag,
@@ -296,7 +305,8 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
 {
     constexpr int PER = (kTC * 8) / kSpmvThreads;  // 16-byte x pieces per thread per chunk
     static_assert(PER * kSpmvThreads == kTC * 8, "pieces must divide evenly");
-    constexpr int kMeta = kMaxChunks * (kTG + 2) + 1;
+    constexpr int kMeta = kMaxChunks * (TG + 2) + 1;
+    constexpr int kTPR = kSpmvThreads / TG;
     __shared__ double xs[kTC * kLineWords];
     __shared__ uint32_t ids[kMaxLines];
     __shared__ uint32_t sm[kMeta];
@@ -306,16 +316,16 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
     const int64_t lb = base[groups + 1 + g], le = base[groups + 1 + g + 1];
     const int64_t mb = base[2 * (groups + 1) + g], me = base[2 * (groups + 1) + g + 1];
     const int nmeta = (int)min(me - mb, (int64_t)kMeta);
-    const int nch = (nmeta - 1) / (kTG + 2);
+    const int nch = (nmeta - 1) / (TG + 2);
     const uint32_t* ent = entries + base[g];
     const int r = threadIdx.x / kTPR, sub = threadIdx.x % kTPR;
-    const int64_t row = g * kTG + r;
+    const int64_t row = g * TG + r;
     const int nl = (int)min(le - lb, (int64_t)kMaxLines);
     for (int i = threadIdx.x; i < nl; i += kSpmvThreads) ids[i] = lines[lb + i];
     for (int i = threadIdx.x; i < nmeta; i += kSpmvThreads) sm[i] = meta[mb + i];
     __syncthreads();
     const uint32_t* T = sm;
-    const uint32_t* LS = sm + nch * (kTG + 1);
+    const uint32_t* LS = sm + nch * (TG + 1);
     double2 pre[PER];
     uint32_t e[kU];
     auto fetch = [&](int c) {
@@ -330,7 +340,7 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
                 else pre[k] = make_double2(xi < dim ? x[xi] : 0.0, xi + 1 < dim ? x[xi + 1] : 0.0);
             }
         }
-        const uint32_t j0 = T[c * (kTG + 1) + r] + sub, je = T[c * (kTG + 1) + r + 1];
+        const uint32_t j0 = T[c * (TG + 1) + r] + sub, je = T[c * (TG + 1) + r + 1];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const uint32_t j = j0 + u * kTPR;
@@ -362,8 +372,8 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
         if (c + 1 < nch) fetch(c + 1);
         consume(cur, acc);
         // rows with more than kTPR * kU entries in this chunk
-        const uint32_t je = T[c * (kTG + 1) + r + 1];
-        for (uint32_t j0 = T[c * (kTG + 1) + r] + sub + kTPR * kU; j0 < je; j0 += kTPR * kU) {
+        const uint32_t je = T[c * (TG + 1) + r + 1];
+        for (uint32_t j0 = T[c * (TG + 1) + r] + sub + kTPR * kU; j0 < je; j0 += kTPR * kU) {
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
                 const uint32_t j = j0 + u * kTPR;
@@ -379,20 +389,13 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
 
 }  // namespace
 
-int tiled_build(hd_hamiltonian* h)
+// Builds the layout with TG-row groups; HD_ERR_UNSUPPORTED when a group reads
+// more than kMaxLines distinct lines.
+template <int TG>
+static int tiled_build_tg(hd_hamiltonian* h)
 {
-    if (!h->d_code) return fail(HD_ERR_ARG, "the tiled layout is built from the coded values (value_mask & 4)");
-    if (h->is_complex) return fail(HD_ERR_UNSUPPORTED, "the tiled layout is implemented for real matrices");
-    if (h->storage != HD_STORAGE_FULL) return fail(HD_ERR_ARG, "the tiled layout needs full storage");
-    if (h->ndict > (1 << 19)) return fail(HD_ERR_UNSUPPORTED, "dictionary too large for the tiled layout");
-    if ((size_t)h->ndict * 8 + sizeof(double) * kTC * kLineWords +
-            sizeof(uint32_t) * (kMaxLines + kMaxChunks * (kTG + 2) + 1) >
-        80 * 1024)
-        return fail(HD_ERR_UNSUPPORTED, "dictionary too large for LDS next to the line buffer");
-    tiled_free(h);
-    if (h->dim == 0) return HD_OK;
     const hipStream_t st = h->stream;
-    const int64_t G = (h->dim + kTG - 1) / kTG;
+    const int64_t G = (h->dim + TG - 1) / TG;
     EventTimer tm;
     tm.start(st);
     int64_t *d_nl = nullptr, *d_ne = nullptr, *d_nc = nullptr;
@@ -406,7 +409,7 @@ int tiled_build(hd_hamiltonian* h)
     int flags[2] = {0, 0};
     if (rc == HD_OK) {
         HD_ENQ(q, hipMemsetAsync(d_flag, 0, 2 * sizeof(int), st));
-        hipLaunchKernelGGL(k_tile_count, dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim, h->d_rowptr,
+        hipLaunchKernelGGL(k_tile_count<TG>, dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim, h->d_rowptr,
                            h->d_col, h->d_code, d_nl, d_ne, d_nc, d_flag);
         HD_ENQ(q, hipGetLastError());
         HD_ENQ(q, hipMemcpyAsync(nl.data(), d_nl, G * sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -417,8 +420,9 @@ int tiled_build(hd_hamiltonian* h)
         rc = q.status();
     }
     if (rc == HD_OK && flags[0])
-        rc = fail(HD_ERR_UNSUPPORTED, "a row group does not fit the tiled layout (more than 3072 distinct x lines "
-                                      "or 32 chunks; flags " + std::to_string(flags[0]) + ")");
+        rc = fail(HD_ERR_UNSUPPORTED, "a " + std::to_string(TG) + "-row group reads more than " +
+                                          std::to_string(kMaxLines) + " distinct x lines (flags " +
+                                          std::to_string(flags[0]) + ")");
     std::vector<int64_t> base(3 * (size_t)(G + 1));
     if (rc == HD_OK) {
         int64_t e = 0, l = 0, o = 0;
@@ -428,7 +432,7 @@ int tiled_build(hd_hamiltonian* h)
             base[2 * (G + 1) + g] = o;
             e += ne[g];  // a multiple of 4: every group starts 16-byte aligned
             l += nl[g];
-            o += nc[g] * (kTG + 2) + 1;
+            o += nc[g] * (TG + 2) + 1;
         }
         base[G] = e;
         base[2 * G + 1] = l;
@@ -446,7 +450,7 @@ int tiled_build(hd_hamiltonian* h)
     if (rc == HD_OK) {
         HD_ENQ(q, hipMemcpyAsync(h->d_tg_base, base.data(), base.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
         HD_ENQ(q, hipMemsetAsync(d_flag + 1, 0, sizeof(int), st));
-        hipLaunchKernelGGL(k_tile_fill, dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim, h->d_rowptr, h->d_col,
+        hipLaunchKernelGGL(k_tile_fill<TG>, dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim, h->d_rowptr, h->d_col,
                            h->d_code, static_cast<const double*>(h->d_diag), h->d_tg_base, G, h->d_tg_entries,
                            h->d_tg_lines, h->d_tg_offs, h->d_tg_diag, d_flag + 1);
         HD_ENQ(q, hipGetLastError());
@@ -467,9 +471,31 @@ int tiled_build(hd_hamiltonian* h)
         tiled_free(h);
         return rc;
     }
+    h->tg_rows = TG;
     h->bytes += (size_t)h->tg_entries * 4 + (size_t)h->tg_lines * 4 + (size_t)h->tg_offs * 4 + base.size() * 8 +
                 (size_t)h->dim * 8;
     return HD_OK;
+}
+
+int tiled_build(hd_hamiltonian* h, int rows_per_group)
+{
+    if (rows_per_group != 0 && rows_per_group != 64 && rows_per_group != 128)
+        return fail(HD_ERR_ARG, "rows per group must be 0 (automatic), 64 or 128");
+    if (!h->d_code) return fail(HD_ERR_ARG, "the tiled layout is built from the coded values (value_mask & 4)");
+    if (h->is_complex) return fail(HD_ERR_UNSUPPORTED, "the tiled layout is implemented for real matrices");
+    if (h->storage != HD_STORAGE_FULL) return fail(HD_ERR_ARG, "the tiled layout needs full storage");
+    if (h->ndict > (1 << 19)) return fail(HD_ERR_UNSUPPORTED, "dictionary too large for the tiled layout");
+    if ((size_t)h->ndict * 8 + sizeof(double) * kTC * kLineWords +
+            sizeof(uint32_t) * (kMaxLines + kMaxChunks * (128 + 2) + 1) >
+        80 * 1024)
+        return fail(HD_ERR_UNSUPPORTED, "dictionary too large for LDS next to the line buffer");
+    tiled_free(h);
+    if (h->dim == 0) return HD_OK;
+    if (rows_per_group == 64) return tiled_build_tg<64>(h);
+    // 128-row groups stage fewer lines per entry; denser matrices fall back to 64
+    const int rc = tiled_build_tg<128>(h);
+    if (rc != HD_ERR_UNSUPPORTED || rows_per_group == 128) return rc;
+    return tiled_build_tg<64>(h);
 }
 
 void tiled_free(hd_hamiltonian* h)
@@ -485,6 +511,7 @@ void tiled_free(hd_hamiltonian* h)
     h->d_tg_offs = nullptr;
     h->d_tg_diag = nullptr;
     h->tg_groups = h->tg_entries = h->tg_lines = h->tg_offs = 0;
+    h->tg_rows = 0;
 }
 
 int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
@@ -492,10 +519,17 @@ int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
     if (!h->d_tg_base) return fail(HD_ERR_ARG, "no tiled layout (hd_ham_tile)");
     if (h->dim == 0) return HD_OK;
     const size_t lds = sizeof(double) * (size_t)std::max(h->ndict, 1);
-    hipLaunchKernelGGL(k_spmv_tiled, dim3((unsigned)h->tg_groups), dim3(kSpmvThreads), lds, s, h->dim, h->tg_groups,
-                       h->d_tg_base, h->d_tg_entries, h->d_tg_lines, h->d_tg_offs, h->d_tg_diag,
-                       static_cast<const double*>(h->d_dict), h->ndict, ((uintptr_t)x & 15) == 0 ? 1 : 0,
-                       static_cast<const double*>(x), static_cast<double*>(y));
+    const int aligned = ((uintptr_t)x & 15) == 0 ? 1 : 0;
+    if (h->tg_rows == 128)
+        hipLaunchKernelGGL(k_spmv_tiled<128>, dim3((unsigned)h->tg_groups), dim3(kSpmvThreads), lds, s, h->dim,
+                           h->tg_groups, h->d_tg_base, h->d_tg_entries, h->d_tg_lines, h->d_tg_offs, h->d_tg_diag,
+                           static_cast<const double*>(h->d_dict), h->ndict, aligned, static_cast<const double*>(x),
+                           static_cast<double*>(y));
+    else
+        hipLaunchKernelGGL(k_spmv_tiled<64>, dim3((unsigned)h->tg_groups), dim3(kSpmvThreads), lds, s, h->dim,
+                           h->tg_groups, h->d_tg_base, h->d_tg_entries, h->d_tg_lines, h->d_tg_offs, h->d_tg_diag,
+                           static_cast<const double*>(h->d_dict), h->ndict, aligned, static_cast<const double*>(x),
+                           static_cast<double*>(y));
     HD_HIP(hipGetLastError());
     return HD_OK;
 }
@@ -512,10 +546,10 @@ size_t tiled_bytes(const hd_hamiltonian* h)
 
 using namespace hd;
 
-extern "C" int hd_ham_tile(hd_hamiltonian* h)
+extern "C" int hd_ham_tile(hd_hamiltonian* h, int32_t rows_per_group)
 {
     if (!h) return fail(HD_ERR_ARG, "hd_ham_tile: null handle");
-    return tiled_build(h);
+    return tiled_build(h, rows_per_group);
 }
 
 extern "C" int hd_ham_tile_info(const hd_hamiltonian* h, hd_tile_info_t* info)
@@ -525,7 +559,8 @@ extern "C" int hd_ham_tile_info(const hd_hamiltonian* h, hd_tile_info_t* info)
     info->groups = h->tg_groups;
     info->entries = h->tg_entries;
     info->lines = h->tg_lines;
-    info->chunks = (h->tg_offs - h->tg_groups) / (kTG + 2);
+    info->chunks = h->tg_rows ? (h->tg_offs - h->tg_groups) / (h->tg_rows + 2) : 0;
+    info->rows_per_group = h->tg_rows;
     info->spmv_bytes = (int64_t)tiled_bytes(h);
     info->build_ms = h->t_tile;
     return HD_OK;

@@ -3,8 +3,8 @@
 python -m hamiltonian_diagonalization_amd.profile_spmv [--n 12 --two-s 33 --iters 3]
 
 Builds the N=12, 2S=33 L_z=0 Coulomb sphere Hamiltonian with FP64, FP32 and
-coded values, then launches each SpMV format `iters` times (after one warm-up
-launch) and the plain streaming copy kernel over 1 GiB `iters` times, which
+coded values and the tiled layout, then launches each SpMV format `iters` times
+(after one warm-up launch) and the plain streaming copy kernel over 1 GiB `iters` times, which
 calibrates FETCH_SIZE / WRITE_SIZE against a known byte count.  Prints the
 algorithmic bytes of every launch so that the counter totals can be compared.
 """
@@ -12,7 +12,7 @@ import argparse
 import ctypes
 import json
 
-from . import CODED, F32, F64, STORAGE_FULL, DeviceBuffer, FockBasis, Hamiltonian, Sector, _check, lib
+from . import CODED, F32, F64, STORAGE_FULL, TILED, DeviceBuffer, FockBasis, Hamiltonian, Sector, _check, lib
 from . import models
 
 
@@ -26,9 +26,10 @@ def main():
     basis = FockBasis(Sector.lz(0, args.two_s), args.n, args.two_s + 1)
     ham = Hamiltonian(basis, models.sphere_tables(args.two_s, models.coulomb_list(args.two_s)),
                       storage=STORAGE_FULL, values=(F64, F32, CODED))
+    tile = ham.tile()
     ham.set_lanes(0)
-    out = {"dim": basis.dim, "nnz": ham.nnz, "lanes": ham.lanes(), "iters": args.iters, "spmv": {}}
-    for name, p in (("f64", F64), ("f32", F32), ("coded", CODED)):
+    out = {"dim": basis.dim, "nnz": ham.nnz, "lanes": ham.lanes(), "iters": args.iters, "tile": tile, "spmv": {}}
+    for name, p in (("f64", F64), ("f32", F32), ("coded", CODED), ("tiled", TILED)):
         ms = ham.spmv_time_ms(p, args.iters)
         out["spmv"][name] = {"ms": ms, "alg_bytes": ham.spmv_bytes(p)}
         print(f"[profile] spmv {name}: {ms:.3f} ms, {ham.spmv_bytes(p)} algorithmic bytes", flush=True)

@@ -69,7 +69,7 @@ typedef enum {
  *             off-diagonal values (sign in bit 15, 0xFFFF = diagonal, kept in a
  *             separate FP64 array): the exact FP64 operator in 6 bytes per
  *             entry; FP64 vectors and accumulation.
- *   HD_TILED  the coded operator re-laid out in 64-row groups whose x lines
+ *   HD_TILED  the coded operator re-laid out in row groups whose x lines
  *             are staged through LDS (hd_ham_tile): 4 bytes per entry, the
  *             same exact FP64 matrix; real matrices. */
 typedef enum { HD_F64 = 0, HD_F32 = 1, HD_CODED = 2, HD_TILED = 3 } hd_precision;
@@ -215,15 +215,17 @@ int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes_per_row);
  * src/utilities/mathematics/sparse_linear_algebra.hpp:218-312, for the
  * product path): built on the device from the coded values, after which
  * precision HD_TILED is available to hd_spmv and hd_lanczos.  Fails with
- * HD_ERR_UNSUPPORTED for complex matrices or when a 64-row group reads more
- * than 3072 distinct 128-byte lines of x. */
-int hd_ham_tile(hd_hamiltonian* h);
+ * HD_ERR_UNSUPPORTED for complex matrices or when a group reads more than
+ * 5120 distinct 128-byte lines of x.  rows_per_group: 128 or 64, or 0 for
+ * 128 with a fallback to 64. */
+int hd_ham_tile(hd_hamiltonian* h, int32_t rows_per_group);
 typedef struct {
-    int64_t groups;         /* 64-row groups */
+    int64_t groups;         /* row groups */
     int64_t entries;        /* stored entries (4 bytes each): the off-diagonal
                                entries plus at most 3 padding entries per chunk */
     int64_t lines;          /* staged x lines, summed over groups */
-    int64_t chunks;         /* chunks (at most 256 lines and 4096 entries each) */
+    int64_t chunks;         /* chunks of 256 lines */
+    int64_t rows_per_group; /* 128, or 64 when a 128-row group reads too many lines */
     int64_t spmv_bytes;     /* algorithmic HBM bytes of one product */
     double build_ms;
 } hd_tile_info_t;

@@ -211,20 +211,21 @@ def test_spmv_complex_large_dictionary(oracle):
         assert np.array_equal(h.spmv(e, precision=hd.CODED), h.spmv(e))
 
 
+@pytest.mark.parametrize("rows", [64, 128])
 @pytest.mark.parametrize("n,two_s,lz", [(4, 11, 0), (5, 13, 3), (6, 15, 0), (6, 15, 24)])
-def test_tiled_columns_exact(oracle, n, two_s, lz):
+def test_tiled_columns_exact(oracle, n, two_s, lz, rows):
     """Every column H e_i through the tiled layout equals the oracle's matrix
     column bit for bit (one nonzero term per output), so the layout holds
     exactly the coded operator's entries, signs and diagonal."""
-    say("tiled columns", n, two_s, lz)
+    say("tiled columns", n, two_s, lz, rows)
     vm = models.coulomb_list(two_s, {1: 1.0, 3: 0.37, 5: 0.11})
     b = hd.FockBasis(hd.Sector.lz(lz, two_s), n, two_s + 1)
     t = models.sphere_tables(two_s, vm)
     K = two_s + 1
     full = oracle.build(n, K, b.states(), oracle.LAYOUT_SPHERE, K, t.ktable, t.vtable, full=True)
     h = hd.Hamiltonian(b, t, values=(hd.CODED,))
-    ti = h.tile()
-    assert ti["groups"] == (b.dim + 63) // 64
+    ti = h.tile(rows)
+    assert ti["rows_per_group"] == rows and ti["groups"] == (b.dim + rows - 1) // rows
     offdiag = len(full.col) - int(np.sum(full.col == np.repeat(np.arange(b.dim), np.diff(full.rowptr))))
     assert offdiag <= ti["entries"] <= offdiag + 3 * ti["chunks"] and ti["entries"] % 4 == 0
     M = full.to_dense(b.dim, mirror=False)
@@ -240,13 +241,17 @@ def test_tiled_spmv_and_lanczos_config1(oracle):
     b, t = models.sphere_problem(8, 21)
     up = oracle.build(8, 22, b.states(), oracle.LAYOUT_SPHERE, 22, t.ktable, t.vtable)
     h = hd.Hamiltonian(b, t, values=(hd.F64, hd.CODED))
+    h64 = hd.Hamiltonian(b, t, values=(hd.CODED,))
+    h64.tile(64)
     ti = h.tile()
     say(ti)
+    assert ti["rows_per_group"] == 128
     rng = np.random.default_rng(4)
     x = rng.normal(size=b.dim)
     want = oracle.spmv_sym(up, x)
     got = h.spmv(x, precision=hd.TILED)
     assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+    assert np.max(np.abs(h64.spmv(x, precision=hd.TILED) - want)) <= 1e-13 * np.max(np.abs(want))
     r64 = h.lanczos(nev=5, tol=1e-12, seed=0)
     rt = h.lanczos(nev=5, tol=1e-12, seed=0, precision=hd.TILED)
     assert rt.converged and np.all(rt.residuals <= 1e-12)
