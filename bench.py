@@ -248,7 +248,7 @@ def main():
     achieved = sp_b[key] / (spmv_live_ms * 1e-3) / 1e9
     lanes, spmv_threads = ham.schedule(prec)
     kernel = {"coded": f"k_spmv_coded<double,{lanes},{spmv_threads}>", "f64": f"k_spmv<double,{lanes},{spmv_threads}>",
-              "tiled": "k_spmv_tiled"}[key]
+              "tiled": f"k_spmv_tiled<{tile['rows_per_group']}>"}[key]
     traffic = measured_traffic(kernel, n, two_s)
     out = {
         "metric": METRIC,
