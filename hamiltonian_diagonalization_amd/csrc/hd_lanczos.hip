@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 
 #include "hd_internal.hpp"
@@ -71,7 +72,7 @@ HD_DEV double widen(float a) { return (double)a; }
 HD_DEV double2 widen(float2 a) { return make_double2((double)a.x, (double)a.y); }
 
 constexpr int kVB = 256;  // block size of the vector kernels
-constexpr int kNV = 8;    // vectors per dot-product pass
+constexpr int kNV = 16;   // vectors per dot-product pass
 constexpr int kMaxBasis = 256;               // Krylov basis limit (ncv)
 constexpr int kNormSlot = kMaxBasis + 8;     // coefficient slot of the staged norm
 
@@ -393,7 +394,7 @@ struct Lanczos {
     double* dscal = nullptr;
     double* dY = nullptr;
     double t_spmv = 0, t_ortho = 0;
-    int mv32 = 0, mv64 = 0, second_passes = 0;
+    int mv32 = 0, mv64 = 0, second_passes = 0, skipped_projections = 0;
     int prec64 = HD_F64;  // the FP64 operator: FP64 values or the coded format
     EventTimer timer;     // brackets every operator application on st
 
@@ -497,12 +498,30 @@ struct Lanczos {
     // Kaufman and Stewart, Math. Comp. 30 (1976) 772; one CGS pass leaves an
     // orthogonality error of about eps * |w_before| / |w_after|), or always
     // when `two_pass`.  Returns the summed coefficients and the final |w|.
+    //
+    // After the local step, w's components along the basis are at rounding
+    // level.  When every one of them is below kSkipTol * |w| the projection
+    // is skipped and the coefficients are returned as zero (the recurrence then
+    // holds for the w actually kept): each new vector is measured against the
+    // whole basis, so the basis stays orthogonal to kSkipTol throughout.
+    static constexpr double kSkipTol = 1e-12;
     int orthogonalize(int nv, T* w, std::vector<T>& c, double* beta, bool two_pass = false)
     {
         const auto t0 = std::chrono::steady_clock::now();
         std::vector<T> c2;
         double before2 = 0.0, nrm2 = 0.0;
         HD_CHECK(dots(nv, w, c, &before2));
+        if (!two_pass) {
+            double cmax2 = 0.0;
+            for (int j = 0; j < nv; ++j) cmax2 = std::max(cmax2, host_abs2(c[j]));
+            if (cmax2 <= kSkipTol * kSkipTol * before2) {
+                c.assign(nv, host_zero());
+                *beta = std::sqrt(std::max(before2, 0.0));
+                ++skipped_projections;
+                t_ortho += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                return HD_OK;
+            }
+        }
         HD_CHECK(project(nv, w, &nrm2));
         const double kappa = 100.0;
         if (two_pass || nrm2 * kappa * kappa < before2) {
@@ -535,6 +554,9 @@ struct Lanczos {
         return HD_OK;
     }
 
+    static double host_abs2(double a) { return a * a; }
+    static double host_abs2(double2 a) { return a.x * a.x + a.y * a.y; }
+    static T host_zero() { T z; std::memset(&z, 0, sizeof(T)); return z; }
     static double host_add(double a, double b) { return a + b; }
     static double2 host_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
     static double host_re(double a) { return a; }
