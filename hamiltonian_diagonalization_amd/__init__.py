@@ -129,6 +129,12 @@ SIGNATURES = {
     "hd_ham_tile": (ctypes.c_int, [_P, _I32]),
     "hd_ham_tile_info": (ctypes.c_int, [_P, ctypes.POINTER(_TileInfo)]),
     "hd_ham_device_tiled": (ctypes.c_int, [_P, _PP, _PP, _PP, _PP, _PP]),
+    "hd_obs_density": (ctypes.c_int, [_P, _I32, _P, _P]),
+    "hd_obs_density_density": (ctypes.c_int, [_P, _I32, _P, _P]),
+    "hd_obs_translational_dd": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P, _P]),
+    "hd_obs_rotational_dd": (ctypes.c_int, [_P, _I32, _P, _P, _P]),
+    "hd_obs_participation_ratio": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(_D)]),
+    "hd_obs_most_probable": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P]),
     "hd_spmv": (ctypes.c_int, [_P, _I32, _P, _P]),
     "hd_spmv_host": (ctypes.c_int, [_P, _P, _P]),
     "hd_spmv_time": (ctypes.c_int, [_P, _I32, _I32, ctypes.POINTER(_D)]),
@@ -280,6 +286,70 @@ class FockBasis:
         dout = DeviceBuffer(max(st.nbytes, 8))
         _check(lib().hd_basis_find_index(self.h, offset, din.ptr, len(st), dout.ptr))
         return dout.to_array(np.uint64, len(st))
+
+    # ---- observables of a vector over this basis (struct Observables,
+    # src/observables/fermion_observables.hpp:38-626), computed on the device
+
+    def _dvec(self, vec):
+        if isinstance(vec, DeviceBuffer):
+            raise TypeError("pass (DeviceBuffer, is_complex) as a tuple, or a numpy array")
+        if isinstance(vec, tuple):
+            return vec[0], int(vec[1])
+        v = np.ascontiguousarray(vec)
+        if len(v) != self.dim:
+            raise ValueError("vector length differs from the basis dimension")
+        cplx = np.iscomplexobj(v)
+        return DeviceBuffer.from_array(v.astype(np.complex128 if cplx else np.float64)), int(cplx)
+
+    def density(self, vec) -> np.ndarray:
+        """DensityFunction: <c+_k c_k> for every orbital k."""
+        d, c = self._dvec(vec)
+        out = np.zeros(self.n_orbitals)
+        _check(lib().hd_obs_density(self.h, c, d.ptr, _ptr(out)))
+        return out
+
+    def density_density(self, vec) -> np.ndarray:
+        """DensityDensityFunction: <n_k n_q> as an Norb x Norb array."""
+        d, c = self._dvec(vec)
+        out = np.zeros(self.n_orbitals ** 2)
+        _check(lib().hd_obs_density_density(self.h, c, d.ptr, _ptr(out)))
+        return out.reshape(self.n_orbitals, self.n_orbitals)
+
+    def translational_dd(self, vec, add: np.ndarray, sub: np.ndarray) -> np.ndarray:
+        """TranslationalDensityDensityFunction with StateAddition / StateSubtraction
+        tables add[k, q] = k + q, sub[k, q] = k - q (shape Norb x nq)."""
+        d, c = self._dvec(vec)
+        a = np.ascontiguousarray(add, dtype=np.uint16)
+        b = np.ascontiguousarray(sub, dtype=np.uint16)
+        nq = a.shape[1]
+        out = np.zeros(nq)
+        _check(lib().hd_obs_translational_dd(self.h, c, d.ptr, nq, _ptr(a), _ptr(b), _ptr(out)))
+        return out
+
+    def rotational_dd(self, vec, inv_r60: np.ndarray) -> np.ndarray:
+        """RotationalDensityDensityFunction: R(m), m = 0..5, with InverseR60 as a
+        table inv_r60[k, alpha] (shape Norb x 6)."""
+        d, c = self._dvec(vec)
+        t = np.ascontiguousarray(inv_r60, dtype=np.uint16)
+        out = np.zeros(12)
+        _check(lib().hd_obs_rotational_dd(self.h, c, d.ptr, _ptr(t), _ptr(out)))
+        return out[0::2] + 1j * out[1::2]
+
+    def participation_ratio(self, vec) -> float:
+        """GetParticipationRatio: 1 / sum |psi|^4."""
+        d, c = self._dvec(vec)
+        v = ctypes.c_double()
+        _check(lib().hd_obs_participation_ratio(self.h, c, d.ptr, ctypes.byref(v)))
+        return v.value
+
+    def most_probable(self, vec, n: int):
+        """GetMostProbable for one vector: (states, amplitudes) of the n largest |psi|."""
+        d, c = self._dvec(vec)
+        n = min(n, self.dim)
+        st = np.zeros(n, dtype=np.uint64)
+        amp = np.zeros(n, dtype=np.complex128 if c else np.float64)
+        _check(lib().hd_obs_most_probable(self.h, c, d.ptr, n, _ptr(st), _ptr(amp)))
+        return st, amp
 
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib is not None:

@@ -19,6 +19,9 @@
 //          --out-path DIR (./output/)  --out-file-name NAME (hd)
 //          --hamiltonian-file 1   writes OUT/NAME[_sector_S]_hamiltonian_0.dat (CRS format)
 //          --eigenvalues-file 1 / --eigenvectors-file 1   writes OUT/NAME[_sector_S]_eigensystem.dat
+//          --observables 1   eigenvector 0's density, participation ratio, 5 most probable
+//              states (Observables, fermion_observables.hpp), plus the translational and
+//              rotational density-density functions when --dim-x/--dim-y are given
 //          --store-terms 1 / --retrieve-terms 1   term tables in OUT as
 //              quadratic|quartic_coefficient_table_L_<NORB>.dat (sphere) or
 //              ..._kx_<X>_ky_<Y>.dat (--layout ofl with --dim-x X --dim-y Y)
@@ -124,6 +127,10 @@ namespace
         return true;
     }
 
+    template <typename T>
+    std::string ObservablesJson(SpinlessFermionHamiltonian<T>& ham, std::map<std::string, std::string>& args,
+                                utilities::MpiWrapper& mpi);
+
     template <typename T, class Quad, class Quart>
     int Run(SpinlessFermionHamiltonian<T>& ham, Quad* quad, Quart* quart, bool haveQuad,
             std::map<std::string, std::string>& args, utilities::MpiWrapper& mpi, const FermionFockBasis& basis)
@@ -188,6 +195,8 @@ namespace
                                   Flag(args, "eigenvectors-file"), format, mpi);
             mpi.ExitFlagTest();
         }
+        std::string obsJson;
+        if (ok && Flag(args, "observables") && ham.m_data.m_eigenvectorsCalculated) obsJson = ObservablesJson(ham, args, mpi);
         const hd_lanczos_stats& st = ham.m_arpackWrapper.m_lastStats;
         std::printf("{\"dim\": %llu, \"nnz_upper\": %ld, \"method\": \"%s\", \"ok\": %s, \"eigenvalues\": [",
                     (unsigned long long)ham.m_data.m_fockSpaceDim, nnzUpper, method.c_str(), ok ? "true" : "false");
@@ -196,8 +205,80 @@ namespace
         std::printf("], \"matvecs\": %d, \"restarts\": %d, \"residuals\": [", st.matvecs, st.restarts);
         for (iSize_t i = 0; method == "lanczos" && i < nev && i < 64; ++i)
             std::printf("%s%.3e", i ? ", " : "", st.residual[i]);
-        std::printf("]}\n");
+        std::printf("]%s}\n", obsJson.c_str());
         return ok ? 0 : 1;
+    }
+
+    std::string JsonList(const double* v, size_t n)
+    {
+        std::string o = "[";
+        char buf[64];
+        for (size_t i = 0; i < n; ++i) {
+            std::snprintf(buf, sizeof buf, "%s%.17g", i ? ", " : "", v[i]);
+            o += buf;
+        }
+        return o + "]";
+    }
+
+    // StateAddition / StateSubtraction / InverseR60 of the k grid
+    // (linear_momentum_constraints_2D.cpp:128-264), k = kx*dimY + ky
+    int Mod(const int a, const int m) { return ((a % m) + m) % m; }
+
+    template <typename T>
+    std::string ObservablesJson(SpinlessFermionHamiltonian<T>& ham, std::map<std::string, std::string>& args,
+                                utilities::MpiWrapper& mpi)
+    {
+        Observables obs;
+        const iSize_t norb = ham.m_data.m_nbrOrbitals;
+        std::vector<double> dens(norb);
+        obs.DensityFunction(ham, 0, dens.data(), mpi);
+        const double pr = obs.GetParticipationRatio(ham, 0, mpi);
+        const iSize_t nmp = 5;
+        const iSize_t nev = ham.m_data.m_nbrEigenvalues;
+        std::vector<fock_t> mp((size_t)nmp * nev);
+        std::vector<dcmplx> amp((size_t)nmp * nev);
+        obs.GetMostProbable(ham, nmp, mp.data(), amp.data(), mpi);
+        std::string o = ", \"density\": " + JsonList(dens.data(), norb);
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "%.17g", pr);
+        o += std::string(", \"participation_ratio\": ") + buf + ", \"most_probable\": [";
+        for (iSize_t i = 0; i < std::min<iSize_t>(nmp, (iSize_t)ham.m_data.m_fockSpaceDim); ++i)
+            o += (i ? ", " : "") + std::to_string(mp[i]);
+        o += "]";
+        if (args.count("dim-x") && args.count("dim-y")) {
+            const int dx = std::stoi(args["dim-x"]), dy = std::stoi(args["dim-y"]);
+            auto add = [dx, dy](const kState_t a, const kState_t b) {
+                return (kState_t)(Mod(a / dy + b / dy, dx) * dy + Mod(a % dy + b % dy, dy));
+            };
+            auto sub = [dx, dy](const kState_t a, const kState_t b) {
+                return (kState_t)(Mod(a / dy - b / dy, dx) * dy + Mod(a % dy - b % dy, dy));
+            };
+            auto inv = [dx, dy](const kState_t k, const int r) {
+                const int kx = k / dy, ky = k % dy;
+                int x = kx, y = ky;
+                switch (Mod(r, 6)) {
+                case 1: x = -ky; y = kx + ky; break;
+                case 2: x = -kx - ky; y = kx; break;
+                case 3: x = -kx; y = -ky; break;
+                case 4: x = ky; y = -kx - ky; break;
+                case 5: x = kx + ky; y = -kx; break;
+                default: break;
+                }
+                return (kState_t)(Mod(x, dx) * dy + Mod(y, dy));
+            };
+            std::vector<double> tdd(norb);
+            obs.TranslationalDensityDensityFunction(ham, 0, add, sub, tdd, mpi);
+            dcmplx rdd[6];
+            obs.RotationalDensityDensityFunction(ham, 0, inv, inv, rdd, mpi);
+            double rr[12];
+            for (int m = 0; m < 6; ++m) {
+                rr[2 * m] = rdd[m].real();
+                rr[2 * m + 1] = rdd[m].imag();
+            }
+            o += ", \"translational\": " + JsonList(tdd.data(), norb) + ", \"rotational\": " + JsonList(rr, 12);
+        }
+        mpi.ExitFlagTest();
+        return o;
     }
 
     bool MakeBasis(FermionFockBasis& basis, std::map<std::string, std::string>& args, utilities::MpiWrapper& mpi,

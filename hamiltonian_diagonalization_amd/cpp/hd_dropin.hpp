@@ -1109,6 +1109,147 @@ namespace diagonalization
         char UPLO;
     };
 
+    // Observables (src/observables/fermion_observables.hpp:38-626) of the
+    // Hamiltonian's eigenvectors, computed on the device (hd_obs_*): the
+    // eigenvector is uploaded, the basis is the Hamiltonian's device basis.
+    // Differences a caller sees: an eigenvector index equal to the number of
+    // eigenvalues is rejected (the reference's test lets it through and reads
+    // past the vectors), and GetMostProbable returns real eigenvectors'
+    // amplitudes as (value, 0) where the reference reinterprets the doubles.
+    struct Observables
+    {
+        template <class H>
+        void DensityFunction(const H& hamiltonian, const iSize_t eigenvectorIndex, double* probabilityList,
+                             utilities::MpiWrapper& mpi)
+        {
+            Device<H> d;
+            if (!d.Load(hamiltonian, eigenvectorIndex, "DensityFunction", mpi)) return;
+            Check(hd_obs_density(d.basis, d.cplx, d.vec, probabilityList), mpi);
+        }
+
+        template <class H>
+        void DensityDensityFunction(const H& hamiltonian, const iSize_t eigenvectorIndex, double* probabilityList,
+                                    utilities::MpiWrapper& mpi)
+        {
+            Device<H> d;
+            if (!d.Load(hamiltonian, eigenvectorIndex, "DensityDensityFunction", mpi)) return;
+            Check(hd_obs_density_density(d.basis, d.cplx, d.vec, probabilityList), mpi);
+        }
+
+        template <class H>
+        void TranslationalDensityDensityFunction(const H& hamiltonian, const iSize_t eigenvectorIndex,
+                                                 std::function<kState_t(const kState_t, const kState_t)> StateAddition,
+                                                 std::function<kState_t(const kState_t, const kState_t)> StateSubtraction,
+                                                 std::vector<double>& functionList, utilities::MpiWrapper& mpi)
+        {
+            Device<H> d;
+            if (!d.Load(hamiltonian, eigenvectorIndex, "TranslationalDensityDensityFunction", mpi)) return;
+            const iSize_t norb = hamiltonian.m_data.m_nbrOrbitals;
+            const iSize_t nq = (iSize_t)functionList.size();
+            std::vector<kState_t> add((size_t)norb * nq), sub((size_t)norb * nq);
+            for (iSize_t k = 0; k < norb; ++k)
+                for (iSize_t q = 0; q < nq; ++q) {
+                    add[(size_t)k * nq + q] = StateAddition((kState_t)k, (kState_t)q);
+                    sub[(size_t)k * nq + q] = StateSubtraction((kState_t)k, (kState_t)q);
+                }
+            Check(hd_obs_translational_dd(d.basis, d.cplx, d.vec, (int32_t)nq, add.data(), sub.data(),
+                                          functionList.data()), mpi);
+        }
+
+        template <class H>
+        void RotationalDensityDensityFunction(const H& hamiltonian, const iSize_t eigenvectorIndex,
+                                              std::function<kState_t(const kState_t, const int)> R60,
+                                              std::function<kState_t(const kState_t, const int)> InverseR60,
+                                              dcmplx* functionList, utilities::MpiWrapper& mpi)
+        {
+            (void)R60;  // the reference evaluates only the inverse rotation
+            Device<H> d;
+            if (!d.Load(hamiltonian, eigenvectorIndex, "RotationalDensityDensityFunction", mpi)) return;
+            const iSize_t norb = hamiltonian.m_data.m_nbrOrbitals;
+            std::vector<kState_t> inv((size_t)norb * 6);
+            for (iSize_t k = 0; k < norb; ++k)
+                for (int a = 0; a < 6; ++a) inv[(size_t)k * 6 + a] = InverseR60((kState_t)k, a);
+            double out[12];
+            if (Check(hd_obs_rotational_dd(d.basis, d.cplx, d.vec, inv.data(), out), mpi))
+                for (int m = 0; m < 6; ++m) functionList[m] = dcmplx(out[2 * m], out[2 * m + 1]);
+        }
+
+        template <class H>
+        double GetParticipationRatio(const H& hamiltonian, const iSize_t eigenvectorIndex, utilities::MpiWrapper& mpi)
+        {
+            Device<H> d;
+            if (!d.Load(hamiltonian, eigenvectorIndex, "GetParticipationRatio", mpi)) return 0.0;
+            double r = 0.0;
+            Check(hd_obs_participation_ratio(d.basis, d.cplx, d.vec, &r), mpi);
+            return r;
+        }
+
+        // states and amplitudes hold nbrStates entries per eigenvector
+        template <class H>
+        void GetMostProbable(const H& hamiltonian, const iSize_t nbrStates, fock_t* states, dcmplx* amplitudes,
+                             utilities::MpiWrapper& mpi)
+        {
+            using T = typename std::remove_const<
+                typename std::remove_reference<decltype(hamiltonian.m_eigenvectors[0])>::type>::type;
+            if (!hamiltonian.m_data.m_eigenvectorsCalculated) return;
+            const iSize_t n = (iSize_t)std::min<fock_t>(hamiltonian.m_data.m_fockSpaceDim, nbrStates);
+            std::vector<T> amp(n);
+            for (iSize_t e = 0; e < hamiltonian.m_data.m_nbrEigenvalues; ++e) {
+                Device<H> d;
+                if (!d.Load(hamiltonian, e, "GetMostProbable", mpi)) return;
+                if (!Check(hd_obs_most_probable(d.basis, d.cplx, d.vec, (int32_t)n, states + (size_t)e * n, amp.data()),
+                           mpi))
+                    return;
+                for (iSize_t i = 0; i < n; ++i) amplitudes[(size_t)e * n + i] = dcmplx(amp[i]);
+            }
+        }
+
+        private:
+        static bool Check(const int status, utilities::MpiWrapper& mpi)
+        {
+            if (utilities::detail::Ok(status, "Observables")) return true;
+            mpi.m_exitFlag = true;
+            return false;
+        }
+
+        // one eigenvector uploaded to the device, released with the helper
+        template <class H>
+        struct Device
+        {
+            const hd_basis* basis = nullptr;
+            void* vec = nullptr;
+            int32_t cplx = 0;
+            ~Device()
+            {
+                if (vec) hd_free(vec);
+            }
+            bool Load(const H& hamiltonian, const iSize_t index, const char* where, utilities::MpiWrapper& mpi)
+            {
+                if (!hamiltonian.m_data.m_eigenvectorsCalculated) {
+                    std::cerr << "\n\tERROR in " << where << ": eigenvectors were not calculated" << std::endl;
+                    std::exit(EXIT_FAILURE);  // fermion_observables.hpp:101-104
+                }
+                if (index >= hamiltonian.m_data.m_nbrEigenvalues) {
+                    std::cerr << "\n\tERROR in " << where << ": eigenvectorIndex out of range." << std::endl;
+                    return false;
+                }
+                basis = hamiltonian.m_fockBasis.Device().get();
+                if (!basis) {
+                    std::cerr << "\n\tERROR in " << where << ": no device Fock basis" << std::endl;
+                    mpi.m_exitFlag = true;
+                    return false;
+                }
+                using T = typename std::remove_const<
+                    typename std::remove_reference<decltype(hamiltonian.m_eigenvectors[0])>::type>::type;
+                cplx = utilities::detail::IsComplex<T>::value ? 1 : 0;
+                const fock_t dim = hamiltonian.GetEigenvectorNodeDim();
+                const size_t bytes = (size_t)dim * sizeof(T);
+                if (!Check(hd_malloc(&vec, std::max<size_t>(bytes, 8)), mpi)) return false;
+                return Check(hd_memcpy_h2d(vec, hamiltonian.m_eigenvectors.data() + (size_t)index * dim, bytes), mpi);
+            }
+        };
+    };
+
     // Term tables with the reference's accessors and file formats
     // (term_tables_base.hpp:44-345; sphere term_tables.cpp:36-169; flux-lattice
     // term_tables.cpp:36-221), for callers that have no table classes of their own.

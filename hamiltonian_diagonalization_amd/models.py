@@ -107,3 +107,18 @@ def ofl_problem(dim_x: int, dim_y: int, n: int, kx: int = 0, ky: int = 0, seed: 
     K = dim_x * dim_y
     basis = FockBasis(Sector.k2d(kx, ky, dim_x, dim_y), n, K)
     return basis, TermTables(LAYOUT_OFL, K, kt, vt, eps)
+
+
+def lattice_tables(dim_x: int, dim_y: int):
+    """StateAddition, StateSubtraction and InverseR60 of the flux-lattice k grid
+    (src/optical_flux_lattice/interacting_ofl_model/linear_momentum_constraints_2D.cpp:128-264,
+    one unit cell) as tables: add[k, q] = k + q, sub[k, q] = k - q (component-wise
+    mod the grid, k = kx*dim_y + ky) and inv_r60[k, alpha] = InverseR60(k, alpha)."""
+    K = dim_x * dim_y
+    kx, ky = np.divmod(np.arange(K), dim_y)
+    add = ((kx[:, None] + kx[None, :]) % dim_x) * dim_y + (ky[:, None] + ky[None, :]) % dim_y
+    sub = ((kx[:, None] - kx[None, :]) % dim_x) * dim_y + (ky[:, None] - ky[None, :]) % dim_y
+    rot = [(kx, ky), (-ky, kx + ky), (-kx - ky, kx), (-kx, -ky), (ky, -kx - ky), (kx + ky, -kx)]
+    inv = np.stack([(rx % dim_x) * dim_y + ry % dim_y for rx, ry in rot], axis=1)
+    return add.astype(np.uint16), sub.astype(np.uint16), inv.astype(np.uint16)
+

@@ -188,6 +188,36 @@ int hd_ham_device_crs(const hd_hamiltonian* h, const int64_t** d_rowptr, const i
 int hd_ham_device_coded(const hd_hamiltonian* h, const uint16_t** d_code, const void** d_diag,
                         const void** d_dict, int32_t* ndict);
 
+/* ------------------------------------------------------------ observables */
+
+/* Observables of one eigenvector (struct Observables,
+ * src/observables/fermion_observables.hpp:38-626), computed on the device from
+ * the basis handle's states and a device vector d_vec of dim entries (double,
+ * or interleaved complex when is_complex).  Deterministic reductions. */
+
+/* DensityFunction (fermion_observables.hpp:47-106): h_out[k] = <c+_k c_k>, Norb values. */
+int hd_obs_density(const hd_basis* b, int32_t is_complex, const void* d_vec, double* h_out);
+/* DensityDensityFunction (:115-198): h_out[k*Norb+q] = <n_k n_q>, Norb^2 values. */
+int hd_obs_density_density(const hd_basis* b, int32_t is_complex, const void* d_vec, double* h_out);
+/* TranslationalDensityDensityFunction (:216-317): h_out[q] =
+ * 1/N sum_{k1,k2} <c+_{k2-q} c_{k2} c+_{k1+q} c_{k1}> for q < nq, with the
+ * model's StateAddition / StateSubtraction as tables h_add[k*nq+q] = k+q and
+ * h_sub[k*nq+q] = k-q (linear_momentum_constraints_2D.cpp:128-168). */
+int hd_obs_translational_dd(const hd_basis* b, int32_t is_complex, const void* d_vec, int32_t nq,
+                            const uint16_t* h_add, const uint16_t* h_sub, double* h_out);
+/* RotationalDensityDensityFunction (:334-428): h_out[2m], h_out[2m+1] = Re, Im of
+ * R(m), m = 0..5, with InverseR60 as a table h_inv_r60[k*6+alpha]
+ * (linear_momentum_constraints_2D.cpp:221-264). */
+int hd_obs_rotational_dd(const hd_basis* b, int32_t is_complex, const void* d_vec, const uint16_t* h_inv_r60,
+                         double* h_out);
+/* GetParticipationRatio (:528-563): 1 / sum |psi_i|^4. */
+int hd_obs_participation_ratio(const hd_basis* b, int32_t is_complex, const void* d_vec, double* h_out);
+/* GetMostProbable (:437-517) for one vector: the n (<= 1024) largest |psi_i|,
+ * descending (ties by ascending state index), as states and amplitudes
+ * (double, or interleaved complex). */
+int hd_obs_most_probable(const hd_basis* b, int32_t is_complex, const void* d_vec, int32_t n, uint64_t* h_states,
+                         void* h_amplitudes);
+
 /* ---------------------------------------------------------------- mat-vec */
 
 /* MatrixVectorFunction<T,L1,L2>::Simple (src/hamiltonians/matrix_vector_routines.hpp:94-104):
