@@ -30,7 +30,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libhdiag_hip.so")
 
 HD_OK, HD_ERR_ARG, HD_ERR_HIP, HD_ERR_NOMEM, HD_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 HD_ERR_NOT_CONVERGED, HD_ERR_EMPTY = 5, 6
-SECTOR_NONE, SECTOR_LZ, SECTOR_K2D, SECTOR_K1D = 0, 1, 2, 3
+SECTOR_NONE, SECTOR_LZ, SECTOR_K2D, SECTOR_K1D, SECTOR_LZ2 = 0, 1, 2, 3, 4
 LAYOUT_SPHERE, LAYOUT_OFL = 0, 1
 F64, F32, CODED, TILED = 0, 1, 2, 3
 STORAGE_FULL, STORAGE_UPPER_REF = 0, 1
@@ -119,6 +119,7 @@ SIGNATURES = {
     "hd_sphere_ktable": (ctypes.c_int, [_I32, _P]),
     "hd_sphere_vtable": (ctypes.c_int, [_I32, _P, _I32, _P]),
     "hd_tables_create": (ctypes.c_int, [_I32, _U32, _P, _P, _I32, _P, _PP]),
+    "hd_tables_create_two_level": (ctypes.c_int, [_U32, _P, _P, _P, _P, _P, _P, _PP]),
     "hd_tables_destroy": (ctypes.c_int, [_P]),
     "hd_ham_build": (ctypes.c_int, [_P, _P, _I32, _D, _I32, _PP]),
     "hd_ham_destroy": (ctypes.c_int, [_P]),
@@ -257,6 +258,12 @@ class Sector:
     def k1d(cls, ky, dim_y):
         return cls(SECTOR_K1D, ky, dim_y)
 
+    @classmethod
+    def lz2(cls, lz_tot, n1):
+        """TestAngularMomentumSector2LL with max = m_maxLz2 = n1 + 1 over the
+        combined orbitals of n1 first-level and n1 + 2 second-level orbitals."""
+        return cls(SECTOR_LZ2, lz_tot, n1 + 1, n1)
+
 
 class FockBasis:
     """FermionFockBasis::GenerateFockSpace on the device."""
@@ -376,6 +383,31 @@ class TermTables:
         self.h = h
         self.layout, self.kmax = layout, kmax
         self.ktable, self.vtable, self.eps = kt, vt, eps
+
+    @classmethod
+    def two_level(cls, n1: int, ktable1, vtable1, ktable2, vtable2, eps1=None, eps2=None):
+        """Both Landau levels of the two-level sphere model as one table over
+        the 2 n1 + 2 combined orbitals (level 0 on orbitals n1+2.., level 1
+        on 0..n1+1)."""
+        self = cls.__new__(cls)
+        self.is_complex = False
+        k1 = np.ascontiguousarray(ktable1, dtype=np.uint16)
+        k2 = np.ascontiguousarray(ktable2, dtype=np.uint16)
+        v1 = np.ascontiguousarray(vtable1, dtype=np.float64)
+        v2 = np.ascontiguousarray(vtable2, dtype=np.float64)
+        if len(k1) != n1 ** 3 or len(v1) != n1 ** 3 or len(k2) != (n1 + 2) ** 3 or len(v2) != (n1 + 2) ** 3:
+            raise ValueError("two-level tables: n1^3 and (n1+2)^3 entries expected")
+        e1 = None if eps1 is None else np.ascontiguousarray(eps1, dtype=np.float64)
+        e2 = None if eps2 is None else np.ascontiguousarray(eps2, dtype=np.float64)
+        if (e1 is not None and len(e1) != n1) or (e2 is not None and len(e2) != n1 + 2):
+            raise ValueError("two-level energies: n1 and n1+2 entries expected")
+        h = ctypes.c_void_p()
+        _check(lib().hd_tables_create_two_level(n1, _ptr(k1), _ptr(v1), _ptr(e1), _ptr(k2), _ptr(v2), _ptr(e2),
+                                                ctypes.byref(h)))
+        self.h = h
+        self.layout, self.kmax = LAYOUT_SPHERE, 2 * n1 + 2
+        self.ktable, self.vtable, self.eps = (k1, k2), (v1, v2), (e1, e2)
+        return self
 
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib is not None:

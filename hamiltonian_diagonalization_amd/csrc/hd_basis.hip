@@ -108,6 +108,31 @@ int group_spec(const hd_sector& sec, int N, int Norb, GroupSpec& gs)
         if (gs.target > maxsum) gs.empty = true;
         return HD_OK;
     }
+    case HD_SECTOR_LZ2: {
+        // TestAngularMomentumSector2LL (two_level_angular_momentum_constraint.hpp:41-73)
+        // on the combined state (first << N2) | second: second-level orbital j
+        // has m = 2j - max, first-level orbital i has m = 2i - (max - 2), so
+        // with weights w(j) = j and w(N2 + i) = i + 1 the condition is
+        // sum m = 2 sum w - N max = lzTot.
+        const int n1 = sec.c, n2 = Norb - sec.c;
+        if (n1 <= 0 || n2 != n1 + 2)
+            return fail(HD_ERR_ARG, "two-level sector: the orbitals must be n1 first-level plus n1 + 2 second-level");
+        const long long tot = (long long)sec.a + (long long)N * sec.b;
+        gs.lz = true;
+        for (int j = 0; j < n2; ++j) gs.w[j] = j;
+        for (int i = 0; i < n1; ++i) gs.w[n2 + i] = i + 1;
+        if (tot % 2 != 0 || tot < 0) {
+            gs.empty = true;
+            return HD_OK;
+        }
+        gs.target = (int)(tot / 2);
+        std::vector<int> ws(gs.w);
+        std::sort(ws.begin(), ws.end());
+        long long maxsum = 0;
+        for (int k = 0; k < N; ++k) maxsum += ws[Norb - 1 - k];
+        if (gs.target > maxsum) gs.empty = true;
+        return HD_OK;
+    }
     case HD_SECTOR_K2D:
     case HD_SECTOR_K1D: {
         int X, Y, kx, ky;
@@ -166,7 +191,7 @@ int make_rank_table(const hd_sector& sec, int N, int Norb, RankTable& rt)
                 unsigned long long v = at(p, c, g);
                 if (c > 0) {
                     if (gs.lz) {
-                        if (g - p >= 0) v += at(p, c - 1, g - p);
+                        if (g - gs.w[p] >= 0) v += at(p, c - 1, g - gs.w[p]);
                     } else {
                         v += at(p, c - 1, gsub(gs, g, gs.w[p]));
                     }
@@ -189,8 +214,16 @@ int make_rank_table(const hd_sector& sec, int N, int Norb, RankTable& rt)
             const int i = p * (N + 1) + c;
             int lo, hi;
             if (gs.lz) {
-                lo = c * (c - 1) / 2;
-                hi = std::min(gs.target, (c <= p) ? c * (2 * p - c - 1) / 2 : -1);
+                // sums of the c smallest / largest weights among orbitals 0..p-1
+                std::vector<int> ws(gs.w.begin(), gs.w.begin() + p);
+                std::sort(ws.begin(), ws.end());
+                lo = 0;
+                hi = 0;
+                for (int k = 0; k < c && k < p; ++k) {
+                    lo += ws[k];
+                    hi += ws[p - 1 - k];
+                }
+                hi = std::min(gs.target, hi);
                 if (c > p) hi = lo - 1;
             } else {
                 lo = 0;
@@ -217,7 +250,9 @@ int make_rank_table(const hd_sector& sec, int N, int Norb, RankTable& rt)
     b.push_back((int32_t)tab.size());
     b.insert(b.end(), off.begin(), off.end());
     b.insert(b.end(), gmm.begin(), gmm.end());
-    if (!gs.lz) {
+    if (gs.lz) {
+        for (int k = 0; k < Norb; ++k) b.push_back(gs.w[k]);  // orbital weights
+    } else {
         for (int g = 0; g < gs.G; ++g)
             for (int k = 0; k < Norb; ++k) b.push_back(gsub(gs, g, gs.w[k]));
     }

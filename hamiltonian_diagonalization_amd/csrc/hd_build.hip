@@ -69,7 +69,7 @@ static double absv(const cd& v, bool cplx) { return cplx ? std::abs(v) : std::fa
 
 // Device-side derived tables for one build.
 struct Derived {
-    int K = 0, npairs = 0, cplx = 0, has_eps = 0;
+    int K = 0, npairs = 0, cplx = 0, has_eps = 0, split = 0;
     std::vector<uint8_t> kpair;  // npairs*K: d | 0x80 if trimmed; 0xFF none
     std::vector<double> pval;    // npairs*K (x2 complex)
     std::vector<double> dterm;   // K*K*2 (x2 complex): diagonal terms of ordered (k4,k3)
@@ -121,6 +121,7 @@ static int derive_tables(const hd_tables* t, const hd_basis* b, double tol, Deri
     const bool cplx = t->is_complex != 0;
     D.K = K;
     D.cplx = cplx;
+    D.split = t->split;
     D.npairs = K * (K - 1) / 2;
     const int w = cplx ? 2 : 1;
     D.kpair.assign((size_t)D.npairs * K, 0xFF);
@@ -221,7 +222,7 @@ HD_DEV float to_f32(double v) { return (float)v; }
 HD_DEV float2 to_f32(double2 v) { return make_float2((float)v.x, (float)v.y); }
 
 struct BuildArgs {
-    int N, Norb, K, upper, has_eps;
+    int N, Norb, K, upper, has_eps, split;
     double tol;
     uint64_t norb_mask;
     const uint8_t* kpair;   // global copy (kernels stage it into LDS)
@@ -305,27 +306,38 @@ k_count(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A, int32_t* 
     for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < dim; row += stride) {
         const uint64_t s = states[row];
         // diagonal: quadratic terms first (Add_CdC_Terms runs before
-        // Add_CdCdCC_Terms), then the quartic terms in (k4, k3, k2) order
+        // Add_CdCdCC_Terms), then the quartic terms in (k4, k3, k2) order.
+        // Two levels (two_level_pseudopotential_model.cpp:188-200): E terms of
+        // the first level (orbitals >= split), of the second, then the first
+        // level's quartic terms, the Trim that ends that call (|v| < tol
+        // restarts the sum at 0), then the second level's quartic terms.
         T dsum = vzero<T>();
         bool any = false;
+        const uint64_t lo_mask = A.split ? (1ull << A.split) - 1ull : ~0ull;
+        const uint64_t part[2] = {A.split ? (s & ~lo_mask) : s, A.split ? (s & lo_mask) : 0ull};
         if (A.has_eps)
-            for (uint64_t t = s; t; t &= t - 1) {
-                const int k = __builtin_ctzll(t);
-                if (s_ef[k]) {
-                    dsum = vadd(dsum, s_eps[k]);
-                    any = true;
+            for (int q = 0; q < 2; ++q)
+                for (uint64_t t = part[q]; t; t &= t - 1) {
+                    const int k = __builtin_ctzll(t);
+                    if (s_ef[k]) {
+                        dsum = vadd(dsum, s_eps[k]);
+                        any = true;
+                    }
+                }
+        for (int q = 0; q < 2; ++q) {
+            const uint64_t sq = part[q];
+            for (uint64_t t4 = sq; t4; t4 &= t4 - 1) {
+                const int k4 = __builtin_ctzll(t4);
+                for (uint64_t t3 = sq ^ (1ull << k4); t3; t3 &= t3 - 1) {
+                    const int k3 = __builtin_ctzll(t3);
+                    const int idx = k4 * K + k3;
+                    const int f = s_df[idx];
+                    if (f & 1) dsum = vadd(dsum, s_dt[2 * idx]);
+                    if (f & 2) dsum = vadd(dsum, s_dt[2 * idx + 1]);
+                    any |= (f != 0);
                 }
             }
-        for (uint64_t t4 = s; t4; t4 &= t4 - 1) {
-            const int k4 = __builtin_ctzll(t4);
-            for (uint64_t t3 = s ^ (1ull << k4); t3; t3 &= t3 - 1) {
-                const int k3 = __builtin_ctzll(t3);
-                const int idx = k4 * K + k3;
-                const int f = s_df[idx];
-                if (f & 1) dsum = vadd(dsum, s_dt[2 * idx]);
-                if (f & 2) dsum = vadd(dsum, s_dt[2 * idx + 1]);
-                any |= (f != 0);
-            }
+            if (q == 0 && A.split && any && vabs(dsum) < A.tol) dsum = vzero<T>();
         }
         const bool keep = any && !(vabs(dsum) < A.tol);
         if (any && !keep) ++trimmed;
@@ -646,6 +658,7 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
     A.K = K;
     A.upper = h->storage == HD_STORAGE_UPPER_REF;
     A.has_eps = D.has_eps;
+    A.split = D.split;
     A.tol = tol;
     A.norb_mask = b->Norb >= 64 ? ~0ull : ((1ull << b->Norb) - 1ull);
     A.kpair = d_kpair;
@@ -806,6 +819,8 @@ int build_batch_sector(const hd_basis* b, const hd_tables* const* tables, int n,
 {
     *out = nullptr;
     const hd_tables* t0 = tables[0];
+    for (int p = 0; p < n; ++p)
+        if (tables[p]->split) return fail(HD_ERR_UNSUPPORTED, "two-level tables are not batched");
     std::vector<Derived> Ds((size_t)n);
     for (int p = 0; p < n; ++p) HD_CHECK(derive_tables(tables[p], b, tol, Ds[(size_t)p]));
     const int K = t0->K;
@@ -912,6 +927,47 @@ int hd_tables_create(int32_t layout, uint32_t kmax, const uint16_t* h_ktable, co
     const size_t w = t->is_complex ? 2 : 1;
     t->vtab.assign(h_vtable, h_vtable + n * w);
     if (h_eps) t->eps.assign(h_eps, h_eps + K * w);
+    *out = t;
+    return HD_OK;
+}
+
+int hd_tables_create_two_level(uint32_t n1, const uint16_t* h_ktable1, const double* h_vtable1,
+                               const double* h_eps1, const uint16_t* h_ktable2, const double* h_vtable2,
+                               const double* h_eps2, hd_tables** out)
+{
+    if (!out || !h_ktable1 || !h_vtable1 || !h_ktable2 || !h_vtable2)
+        return fail(HD_ERR_ARG, "hd_tables_create_two_level: null argument");
+    *out = nullptr;
+    if (n1 == 0 || 2 * n1 + 2 > 64) return fail(HD_ERR_ARG, "n1 must be in 1..31");
+    const int K1 = (int)n1, K2 = (int)n1 + 2, K = K1 + K2;
+    // combined orbital of level-local label k: level 0 (first) at K2 + k,
+    // level 1 (second) at k
+    auto comb = [&](int level, int k) { return level == 0 ? K2 + k : k; };
+    hd_tables* t = new hd_tables();
+    t->layout = HD_LAYOUT_SPHERE;
+    t->K = K;
+    t->is_complex = 0;
+    t->split = K2;
+    t->ktab.assign((size_t)K * K * K, (uint16_t)0xFFFF);
+    t->vtab.assign((size_t)K * K * K, 0.0);
+    for (int level = 0; level < 2; ++level) {
+        const int L = level == 0 ? K1 : K2;
+        const uint16_t* kt = level == 0 ? h_ktable1 : h_ktable2;
+        const double* vt = level == 0 ? h_vtable1 : h_vtable2;
+        for (int k4 = 0; k4 < L; ++k4)
+            for (int k3 = 0; k3 < L; ++k3)
+                for (int k2 = 0; k2 < L; ++k2) {
+                    const size_t i = ((size_t)k4 * L + k3) * L + k2;
+                    const size_t j = ((size_t)comb(level, k4) * K + comb(level, k3)) * K + comb(level, k2);
+                    if (kt[i] < L) t->ktab[j] = (uint16_t)comb(level, kt[i]);  // GetK1: k1 >= kMax gives none
+                    t->vtab[j] = vt[i];
+                }
+    }
+    if (h_eps1 || h_eps2) {
+        t->eps.assign((size_t)K, 0.0);
+        for (int k = 0; k < K1 && h_eps1; ++k) t->eps[(size_t)comb(0, k)] = h_eps1[k];
+        for (int k = 0; k < K2 && h_eps2; ++k) t->eps[(size_t)comb(1, k)] = h_eps2[k];
+    }
     *out = t;
     return HD_OK;
 }

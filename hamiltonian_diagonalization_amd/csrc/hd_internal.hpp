@@ -113,7 +113,7 @@ void dfree(void* p);
 //   [0] N  [1] Norb  [2] G  [3] target  [4] is_lz  [5] tab_len
 //   off [(Norb+1)*(N+1)]    offset of (p,c) into tab, minus gmin
 //   gmm [(Norb+1)*(N+1)*2]  (gmin, gmax); gmax < gmin marks an empty row
-//   sub [G*Norb]            finite groups: g - weight(k)  (absent for LZ)
+//   sub [G*Norb]            finite groups: g - weight(k); LZ kinds: weight(k) [Norb]
 //   tab [tab_len]           uint32 counts
 struct RankTable {
     int N = 0, Norb = 0, G = 1, target = 0, is_lz = 0;
@@ -156,7 +156,7 @@ HD_DEV RankView rank_view(const int32_t* blob)
     v.off = blob + kRankHeader;
     v.gmm = v.off + npc;
     v.sub = v.gmm + 2 * npc;
-    v.tab = reinterpret_cast<const uint32_t*>(v.sub + (v.is_lz ? 0 : v.G * v.Norb));
+    v.tab = reinterpret_cast<const uint32_t*>(v.sub + (v.is_lz ? v.Norb : v.G * v.Norb));
     return v;
 }
 
@@ -172,7 +172,7 @@ HD_DEV uint32_t rank_lookup(const RankView& v, int p, int c, int g)
 template <bool LZ>
 HD_DEV int group_sub(const RankView& v, int g, int k)
 {
-    if (LZ) return g - k;
+    if (LZ) return g - v.sub[k];  // integer weights: sub holds w(k)
     return v.sub[g * v.Norb + k];
 }
 
@@ -247,6 +247,10 @@ struct hd_tables {
     std::vector<uint16_t> ktab;
     std::vector<double> vtab;  // interleaved when complex
     std::vector<double> eps;   // empty when no quadratic table
+    // Two-level tables: orbitals >= split form the first level, those below
+    // the second; the diagonal sums the first level's terms before the
+    // second's.  0 for one level.
+    int split = 0;
 };
 
 // Eigenpairs of the last Lanczos call on a Hamiltonian handle.

@@ -64,6 +64,29 @@ def sphere_problem(n: int, two_s: int, lz: int = 0, pseudopotentials=None):
     return basis, sphere_tables(two_s, pseudopotentials)
 
 
+def two_level_tables(n1: int, pseudopotentials1=None, pseudopotentials2=None, eps1=None, eps2=None):
+    """Tables of SphereTwoLevelPseudopotentialModel::BuildTermTables
+    (two_level_pseudopotential_model.cpp:61-87): level 0 over n1 orbitals at
+    maxLz = n1 - 1, level 1 over n1 + 2 orbitals at maxLz = n1 + 1, each from
+    its own pseudopotential list (default: Coulomb at that level's flux)."""
+    if pseudopotentials1 is None:
+        pseudopotentials1 = coulomb_list(n1 - 1) if n1 > 2 else [0.0, 1.0]
+    if pseudopotentials2 is None:
+        pseudopotentials2 = coulomb_list(n1 + 1)
+    kt1, vt1 = sphere_ktable(n1 - 1), sphere_vtable(n1 - 1, pseudopotentials1)
+    kt2, vt2 = sphere_ktable(n1 + 1), sphere_vtable(n1 + 1, pseudopotentials2)
+    return TermTables.two_level(n1, kt1, vt1, kt2, vt2, eps1, eps2)
+
+
+def two_level_problem(n: int, n1: int, lz: Optional[int] = 0, **kw):
+    """(basis, tables) for N electrons in the LLL (n1 orbitals) plus the second
+    Landau level (n1 + 2 orbitals) in the 2 L_z = lz sector (None: every
+    state), SphereTwoLevelPseudopotentialModel::BuildFockBasis (:114-165)."""
+    sector = Sector.none() if lz is None else Sector.lz2(lz, n1)
+    basis = FockBasis(sector, n, 2 * n1 + 2)
+    return basis, two_level_tables(n1, **kw)
+
+
 def ofl_tables(dim_x: int, dim_y: int, seed: int = 0, band_width: float = 0.1):
     """Synthetic single-band flux-lattice tables on a dim_x x dim_y k grid
     (k = kx*dim_y + ky):

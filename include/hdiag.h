@@ -45,8 +45,14 @@ typedef enum {
                            (angular_momentum_constraint.hpp:41-62) */
     HD_SECTOR_K2D = 2,  /* TestLinearMomentumSector2(state, a=kx, b=ky, c=dimX, d=dimY)
                            (linear_momentum_constraints_2D.cpp:38-79) */
-    HD_SECTOR_K1D = 3   /* TestLinearMomentumSector1(state, a=ky, b=dimY)
+    HD_SECTOR_K1D = 3,  /* TestLinearMomentumSector1(state, a=ky, b=dimY)
                            (linear_momentum_constraints_2D.cpp:87-110) */
+    HD_SECTOR_LZ2 = 4   /* TestAngularMomentumSector2LL(state, a=lzTot, b=max) with
+                           c = n1 first-level orbitals
+                           (two_level_angular_momentum_constraint.hpp:41-73), on the
+                           combined state (m_first << (n1+2)) | m_second: the
+                           TwoLevelFockState order is ascending combined order
+                           (two_level_fermion_fock_basis.cpp:40-45) */
 } hd_sector_kind;
 
 typedef struct {
@@ -142,6 +148,24 @@ int hd_sphere_vtable(int32_t max_lz, const double* h_pseudopotentials, int32_t n
 int hd_tables_create(int32_t layout, uint32_t kmax, const uint16_t* h_ktable,
                      const double* h_vtable, int32_t is_complex, const double* h_eps,
                      hd_tables** out);
+/* The two Landau levels of SphereTwoLevelPseudopotentialModel
+ * (two_level_pseudopotential_model.cpp:61-110, 188-200) as one table over the
+ * combined orbitals of TwoLevelFockState (m_first << (n1+2)) | m_second:
+ * level-0 sphere tables over n1 orbitals (maxLz = n1-1: m_quarticTables,
+ * m_quadraticTables), level-1 tables over n1+2 orbitals (maxLz = n1+1:
+ * m_quarticTables2LL, m_quadraticTables2LL), real values, no terms across
+ * levels (TwoLevelSpinlessFermionHamiltonian accepts only same-level terms,
+ * two_level_spinless_fermion_hamiltonian.hpp:155-164, 317-326).  A k-table label
+ * >= the level's orbital count means no k1 (GetK1, term_tables.cpp:109-124), so
+ * the FindM1 error label (0xFFFE in level 0, 0xFFFF in level 1) and
+ * hd_sphere_ktable's 0xFFFF act alike.  h_eps1 / h_eps2 (n1 and n1+2 energies,
+ * SetEnergyLevels :91-110) may be NULL.  hd_ham_build with this table and an
+ * HD_SECTOR_LZ2 or HD_SECTOR_NONE basis over 2n1+2 orbitals follows the
+ * model's call order: E terms of level 0 then 1, quartic terms of level 0, its
+ * Trim, quartic terms of level 1, final Trim.  Not accepted by hd_batch_create. */
+int hd_tables_create_two_level(uint32_t n1, const uint16_t* h_ktable1, const double* h_vtable1,
+                               const double* h_eps1, const uint16_t* h_ktable2,
+                               const double* h_vtable2, const double* h_eps2, hd_tables** out);
 int hd_tables_destroy(hd_tables* t);
 
 /* ------------------------------------------------------------------- build */

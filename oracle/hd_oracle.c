@@ -347,6 +347,17 @@ static int acc_add(oc_rowacc* a, int64_t col, double complex v)
     return 0;
 }
 
+static double complex* acc_find(oc_rowacc* a, int64_t col)
+{
+    uint64_t h = ((uint64_t)col * 0x9E3779B97F4A7C15ull) >> 49;
+    int s = (int)(h & a->mask);
+    while (a->slot[s] != -1) {
+        if (a->keys[s] == col) return &a->eval[a->slot[s]];
+        s = (s + 1) & a->mask;
+    }
+    return NULL;
+}
+
 typedef struct { int64_t col; double complex v; } oc_entry;
 static int cmp_entry(const void* x, const void* y)
 {
@@ -688,4 +699,190 @@ void oc_spmv_sym_mt(int64_t dim, const int64_t* rowptr, const int32_t* col,
         y[i] = s;
     }
     free(priv); free(th); free(jobs);
+}
+
+/* ------------------------------------------------------------- two levels */
+
+/* TestAngularMomentumSector2LL (two_level_angular_momentum_constraint.hpp:41-73):
+ * first-level orbital k has m = 2k - (max - 2), second-level orbital k has
+ * m = 2k - max. */
+int oc_test_lz2(uint64_t first, uint64_t second, int lzTot, int max)
+{
+    const int maxLLL = max - 2;
+    int lz = 0;
+    for (uint64_t t = first; t; t &= t - 1) lz += 2 * __builtin_ctzll(t) - maxLLL;
+    for (uint64_t t = second; t; t &= t - 1) lz += 2 * __builtin_ctzll(t) - max;
+    return lz == lzTot;
+}
+
+static int cmp_pair(const void* x, const void* y)
+{
+    const uint64_t* a = (const uint64_t*)x;
+    const uint64_t* b = (const uint64_t*)y;
+    if (a[0] != b[0]) return a[0] < b[0] ? -1 : 1;
+    if (a[1] != b[1]) return a[1] < b[1] ? -1 : 1;
+    return 0;
+}
+
+/* TwoLevelFermionFockBasis::GenerateFockSpace (two_level_fermion_fock_basis.cpp:214-337):
+ * for n0 = 0..N particles in the first level, Gosper over the first level's
+ * n1 orbitals (outer) and the second level's n2 (inner), filtered by the
+ * sector test when use_sector, then sorted by (first, second).  Writes up to
+ * cap pairs; returns the dimension. */
+int64_t oc_two_level_basis(int N, int n1, int n2, int use_sector, int lzTot, int max, uint64_t* first,
+                           uint64_t* second, int64_t cap)
+{
+    int64_t cnt = 0;
+    uint64_t* buf = NULL;
+    int64_t bcap = 0;
+    for (int n0 = 0; n0 <= N; ++n0) {
+        if (n0 > n1 || N - n0 > n2) continue;
+        uint64_t d1 = oc_binomial(n1, n0), d2 = oc_binomial(n2, N - n0);
+        uint64_t s1 = oc_generate_hamming(n0, 0);
+        for (uint64_t i = 0; i < d1; ++i) {
+            uint64_t s2 = oc_generate_hamming(N - n0, 0);
+            for (uint64_t j = 0; j < d2; ++j) {
+                if (!use_sector || oc_test_lz2(s1, s2, lzTot, max)) {
+                    if (cnt >= bcap) {
+                        bcap = bcap ? 2 * bcap : 1024;
+                        buf = (uint64_t*)realloc(buf, sizeof(uint64_t) * 2 * bcap);
+                    }
+                    buf[2 * cnt] = s1;
+                    buf[2 * cnt + 1] = s2;
+                    ++cnt;
+                }
+                s2 = oc_next_hamming(s2);
+            }
+            s1 = oc_next_hamming(s1);
+        }
+    }
+    if (cnt) qsort(buf, (size_t)cnt, 2 * sizeof(uint64_t), cmp_pair);
+    for (int64_t i = 0; i < cnt && i < cap; ++i) {
+        first[i] = buf[2 * i];
+        second[i] = buf[2 * i + 1];
+    }
+    free(buf);
+    return cnt;
+}
+
+/* FindFockStateIndex (two_level_fermion_fock_basis.cpp:155-185): binary search
+ * in the (first, second)-sorted list. */
+static int64_t oc_find_pair(const uint64_t* f, const uint64_t* s, int64_t dim, uint64_t a, uint64_t b)
+{
+    int64_t lo = 0, hi = dim - 1;
+    while (lo <= hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (f[mid] == a && s[mid] == b) return mid;
+        if (f[mid] < a || (f[mid] == a && s[mid] < b)) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return -1;
+}
+
+/* One row of TwoLevelSpinlessFermionHamiltonian: Add_CdC_Terms for level 0
+ * then level 1 (when eps given), Add_CdCdCC_Terms for level 0 then level 1
+ * (two_level_spinless_fermion_hamiltonian.hpp:128-262, 280-480), with the
+ * model's call order (two_level_pseudopotential_model.cpp:188-200).  Sphere
+ * tables per level: index (k4*kMax+k3)*kMax+k2, k1 >= kMax = no term. */
+static int oc_two_level_row(oc_rowacc* acc, int n1, int n2, int Ntot, const uint64_t* F, const uint64_t* S,
+                            int64_t dim, int64_t row, const uint16_t* kt[2], const double* vt[2],
+                            const double* eps[2], double tol, int full)
+{
+    const int nl[2] = {n1, n2};
+    const uint64_t highest = 1ull << (Ntot - 1);
+    for (int L = 0; L < 2; ++L) {
+        if (!eps[L]) continue;
+        const uint64_t in = L ? S[row] : F[row];
+        for (uint64_t t = in; t; t &= t - 1) {
+            const int k2 = __builtin_ctzll(t);
+            const double el = eps[L][k2];
+            if (fabs(el) > tol && acc_add(acc, row, el)) return -1;
+        }
+    }
+    for (int L = 0; L < 2; ++L) {
+        const int K = nl[L];
+        const uint64_t in = L ? S[row] : F[row];
+        const uint64_t other = L ? F[row] : S[row];
+        const uint64_t mask = (1ull << K) - 1;
+        for (uint64_t t4 = in; t4; t4 &= t4 - 1) {
+            const uint64_t b4 = t4 & (0 - t4);
+            const int k4 = __builtin_ctzll(b4);
+            uint64_t t3 = in ^ b4;
+            while (t3) {
+                const uint64_t b3 = t3 & (0 - t3);
+                const int k3 = __builtin_ctzll(b3);
+                if (k3 != k4) {
+                    uint64_t t2 = mask ^ (t3 ^ b3);
+                    while (t2) {
+                        const uint64_t b2 = t2 & (0 - t2);
+                        const int k2 = __builtin_ctzll(b2);
+                        const int64_t vi = ((int64_t)k4 * K + k3) * K + k2;
+                        const int k1 = kt[L][vi];
+                        if (k1 < K) {
+                            const uint64_t b1 = 1ull << k1;
+                            if ((!(b1 & in) || ((b1 & b3) || (b1 & b4))) &&
+                                (!(b2 & in) || ((b2 & b3) || (b2 & b4))) && !(b1 & b2)) {
+                                int sign;
+                                const uint64_t out = oc_cdcdcc(&sign, b1, b2, b3, b4, in, highest);
+                                if (sign != 0 && (full || out >= in)) {
+                                    const int64_t idx = L ? oc_find_pair(F, S, dim, other, out)
+                                                          : oc_find_pair(F, S, dim, out, other);
+                                    if (idx >= 0) {
+                                        const double el = (double)sign * vt[L][vi];
+                                        if (fabs(el) > tol && acc_add(acc, idx, el)) return -1;
+                                    }
+                                }
+                            }
+                        }
+                        t2 ^= b2;
+                    }
+                }
+                t3 ^= b3;
+            }
+        }
+        if (L == 0) {
+            /* Add_CdCdCC_Terms ends with Trim (two_level_spinless_fermion_hamiltonian.hpp:458-467):
+             * after the first level only the diagonal can still receive
+             * terms, so dropping it when |v| < tol restarts its sum at 0. */
+            double complex* d = acc_find(acc, row);
+            if (d && cabs(*d) < tol) *d = 0.0;
+        }
+    }
+    return 0;
+}
+
+/* The two-level upper-triangle (or full) CRS for rows [row0, row1), real
+ * values; Trim and column order as oc_build. */
+int64_t oc_two_level_build(int n1, int n2, const uint64_t* first, const uint64_t* second, int64_t dim,
+                           int64_t row0, int64_t row1, const uint16_t* kt1, const double* vt1, const double* eps1,
+                           const uint16_t* kt2, const double* vt2, const double* eps2, double tol, int full,
+                           int64_t* rowptr, int32_t* col, double* val, int64_t cap, int64_t* n_trimmed)
+{
+    if (row0 < 0 || row1 > dim || row0 > row1 || n1 <= 0 || n2 <= 0 || n1 + n2 > 64) return -1;
+    const uint16_t* kt[2] = {kt1, kt2};
+    const double* vt[2] = {vt1, vt2};
+    const double* eps[2] = {eps1, eps2};
+    oc_rowacc acc;
+    acc_init(&acc);
+    oc_entry* scratch = (oc_entry*)malloc(sizeof(oc_entry) * acc.cap);
+    int64_t nnz = 0;
+    if (n_trimmed) *n_trimmed = 0;
+    if (rowptr) rowptr[0] = 0;
+    for (int64_t r = row0; r < row1; ++r) {
+        acc_clear(&acc);
+        if (oc_two_level_row(&acc, n1, n2, n1 + n2, first, second, dim, r, kt, vt, eps, tol, full)) {
+            nnz = -1;
+            break;
+        }
+        int64_t k = oc_emit_row(&acc, scratch, 0, tol, rowptr ? col : NULL, val, nnz, cap, n_trimmed);
+        if (k < 0) {
+            nnz = -1;
+            break;
+        }
+        nnz += k;
+        if (rowptr) rowptr[r - row0 + 1] = nnz;
+    }
+    free(scratch);
+    acc_free(&acc);
+    return nnz;
 }

@@ -92,6 +92,16 @@ int64_t oc_build_mt(int N, int Norb, const uint64_t* basis, int64_t dim,
                     int64_t* rowptr, int32_t* col, double* val, int64_t cap,
                     int nthreads);
 
+/* Two-level (LLL + second Landau level) basis and Hamiltonian: see
+ * hd_oracle.c.  Pairs (first, second) sorted lexicographically. */
+int oc_test_lz2(uint64_t first, uint64_t second, int lzTot, int max);
+int64_t oc_two_level_basis(int N, int n1, int n2, int use_sector, int lzTot, int max, uint64_t* first,
+                           uint64_t* second, int64_t cap);
+int64_t oc_two_level_build(int n1, int n2, const uint64_t* first, const uint64_t* second, int64_t dim,
+                           int64_t row0, int64_t row1, const uint16_t* kt1, const double* vt1, const double* eps1,
+                           const uint16_t* kt2, const double* vt2, const double* eps2, double tol, int full,
+                           int64_t* rowptr, int32_t* col, double* val, int64_t cap, int64_t* n_trimmed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,6 +92,13 @@ def lib():
         L.oc_spmv_sym_block.argtypes = [i64, i64, P, P, P, i32, P, P]
         L.oc_spmv_sym_mt.restype = None
         L.oc_spmv_sym_mt.argtypes = [i64, P, P, P, i32, P, P, i32]
+        L.oc_test_lz2.restype = i32
+        L.oc_test_lz2.argtypes = [u64, u64, i32, i32]
+        L.oc_two_level_basis.restype = i64
+        L.oc_two_level_basis.argtypes = [i32, i32, i32, i32, i32, i32, P, P, i64]
+        L.oc_two_level_build.restype = i64
+        L.oc_two_level_build.argtypes = [i32, i32, P, P, i64, i64, i64, P, P, P, P, P, P, dbl, i32,
+                                         P, P, P, i64, ctypes.POINTER(i64)]
         _lib = L
     return _lib
 
@@ -191,6 +198,73 @@ def build(N, Norb, bas, layout, K, ktab, vtab, eps=None, is_complex=False,
                       ctypes.byref(ntr))
     assert nnz2 == nnz
     return CRS(rp, col[:nnz], val[:nnz * (2 if is_complex else 1)], is_complex, ntr.value)
+
+
+# ------------------------------------------------------------------ two levels
+
+def two_level_basis(N, n1, lz=None):
+    """TwoLevelFermionFockBasis::GenerateFockSpace with n2 = n1 + 2 second-level
+    orbitals; lz = 2 L_z total (None: no sector).  Returns (first, second)
+    sorted by (first, second)."""
+    L = lib()
+    n2 = n1 + 2
+    use, lzt = (0, 0) if lz is None else (1, lz)
+    dim = L.oc_two_level_basis(N, n1, n2, use, lzt, n1 + 1, None, None, 0)
+    f = np.zeros(max(dim, 1), dtype=np.uint64)
+    g = np.zeros(max(dim, 1), dtype=np.uint64)
+    L.oc_two_level_basis(N, n1, n2, use, lzt, n1 + 1, _p(f), _p(g), dim)
+    return f[:dim], g[:dim]
+
+
+def two_level_ktable(n1, level):
+    """BaseGenerate4KTable for Landau level `level` of the two-level model
+    (pseudopotential_model_base.hpp:96-139): maxLz = n1-1 (level 0) or n1+1
+    (level 1); FindM1's error value m_maxLz2+2 = n1+3 converts to the label
+    (maxLz - n1 - 3)/2 stored as uint16."""
+    maxLz = n1 - 1 if level == 0 else n1 + 1
+    K = maxLz + 1
+    kt = np.zeros(K ** 3, dtype=np.uint16)
+    err = n1 + 3
+    it = 0
+    for m4 in range(maxLz, -maxLz - 1, -2):
+        for m3 in range(maxLz, -maxLz - 1, -2):
+            for m2 in range(maxLz, -maxLz - 1, -2):
+                m1 = m3 + m4 - m2
+                if not (-maxLz <= m1 <= maxLz):
+                    m1 = err
+                kt[it] = ((-m1 + maxLz) // 2) & 0xFFFF
+                it += 1
+    return kt
+
+
+def two_level_build(n1, first, second, kt1, vt1, kt2, vt2, eps1=None, eps2=None, tol=ZERO_TOL,
+                    full=False, row0=0, row1=None):
+    """The two-level Hamiltonian's upper (or full) CRS for rows [row0, row1):
+    Add_CdC level 0, level 1, Add_CdCdCC level 0 (+Trim), level 1 (+Trim)."""
+    L = lib()
+    n2 = n1 + 2
+    dim = len(first)
+    row1 = dim if row1 is None else row1
+    f = np.ascontiguousarray(first, dtype=np.uint64)
+    g = np.ascontiguousarray(second, dtype=np.uint64)
+    k1 = np.ascontiguousarray(kt1, dtype=np.uint16)
+    k2 = np.ascontiguousarray(kt2, dtype=np.uint16)
+    v1 = np.ascontiguousarray(vt1, dtype=np.float64)
+    v2 = np.ascontiguousarray(vt2, dtype=np.float64)
+    e1 = None if eps1 is None else np.ascontiguousarray(eps1, dtype=np.float64)
+    e2 = None if eps2 is None else np.ascontiguousarray(eps2, dtype=np.float64)
+    assert len(k1) == n1 ** 3 and len(k2) == n2 ** 3 and len(v1) == n1 ** 3 and len(v2) == n2 ** 3
+    ntr = ctypes.c_int64(0)
+    args = (n1, n2, _p(f), _p(g), dim, row0, row1, _p(k1), _p(v1), _p(e1), _p(k2), _p(v2), _p(e2), tol, int(full))
+    nnz = L.oc_two_level_build(*args, None, None, None, 0, ctypes.byref(ntr))
+    if nnz < 0:
+        raise RuntimeError("oracle two-level build failed")
+    rp = np.zeros(row1 - row0 + 1, dtype=np.int64)
+    col = np.zeros(max(nnz, 1), dtype=np.int32)
+    val = np.zeros(max(nnz, 1), dtype=np.float64)
+    nnz2 = L.oc_two_level_build(*args, _p(rp), _p(col), _p(val), nnz, ctypes.byref(ntr))
+    assert nnz2 == nnz
+    return CRS(rp, col[:nnz], val[:nnz], False, ntr.value)
 
 
 def spmv_sym(crs, x, threads=1):
