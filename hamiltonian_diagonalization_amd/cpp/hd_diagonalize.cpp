@@ -131,9 +131,39 @@ namespace
     std::string ObservablesJson(SpinlessFermionHamiltonian<T>& ham, std::map<std::string, std::string>& args,
                                 utilities::MpiWrapper& mpi);
 
+    template <typename T>
+    std::string ObservablesJson(TwoLevelSpinlessFermionHamiltonian<T>&, std::map<std::string, std::string>&,
+                                utilities::MpiWrapper&)
+    {
+        std::fprintf(stderr, "--observables: the observables are defined on one-level bases\n");
+        return "";
+    }
+
+    // pseudopotential_model.cpp:246-250: quadratic terms first, then quartic
     template <typename T, class Quad, class Quart>
-    int Run(SpinlessFermionHamiltonian<T>& ham, Quad* quad, Quart* quart, bool haveQuad,
-            std::map<std::string, std::string>& args, utilities::MpiWrapper& mpi, const FermionFockBasis& basis)
+    void AddTerms(SpinlessFermionHamiltonian<T>& ham, Quad* quad, Quart* quart, bool haveQuad, utilities::MpiWrapper& mpi)
+    {
+        if (haveQuad) ham.Add_CdC_Terms(quad, 0, ham.m_data.m_fockSpaceDim, mpi);
+        ham.Add_CdCdCC_Terms(quart, 0, ham.m_data.m_fockSpaceDim, mpi);
+    }
+
+    // two_level_pseudopotential_model.cpp:188-200: quad[0], quad[1], quart[0], quart[1]
+    template <typename T, class Quad, class Quart>
+    void AddTerms(TwoLevelSpinlessFermionHamiltonian<T>& ham, Quad* quad, Quart* quart, bool haveQuad,
+                  utilities::MpiWrapper& mpi)
+    {
+        const fock_t dim = ham.m_data.m_fockSpaceDim;
+        if (haveQuad) {
+            ham.Add_CdC_Terms(&quad[0], 0, 0, 0, dim, mpi);
+            ham.Add_CdC_Terms(&quad[1], 1, 1, 0, dim, mpi);
+        }
+        ham.Add_CdCdCC_Terms(&quart[0], 0, 0, 0, 0, 0, dim, mpi);
+        ham.Add_CdCdCC_Terms(&quart[1], 1, 1, 1, 1, 0, dim, mpi);
+    }
+
+    template <typename T, class H, class Quad, class Quart, class B>
+    int Run(H& ham, Quad* quad, Quart* quart, bool haveQuad,
+            std::map<std::string, std::string>& args, utilities::MpiWrapper& mpi, const B& basis)
     {
         const iSize_t N = (iSize_t)std::stoul(args["nbr-particles"]);
         ham.SetFockBasis(basis);
@@ -147,9 +177,7 @@ namespace
         } else {
             ham.SetValueFormats(1);
         }
-        // pseudopotential_model.cpp:246-250: quadratic terms first, then quartic
-        if (haveQuad) ham.Add_CdC_Terms(quad, 0, ham.m_data.m_fockSpaceDim, mpi);
-        ham.Add_CdCdCC_Terms(quart, 0, ham.m_data.m_fockSpaceDim, mpi);
+        AddTerms(ham, quad, quart, haveQuad, mpi);
         mpi.ExitFlagTest();
 
         long int nnzUpper = -1;
@@ -306,6 +334,86 @@ namespace
     }
 }
 
+namespace
+{
+    // SphereTwoLevelPseudopotentialModel (two_level_pseudopotential_model.cpp):
+    // BuildTermTables (:61-87) or TermsFromFile, BuildFockBasis (:114-165),
+    // BuildHamiltonian (:179-207), then BaseDiagonalize.  The reference's
+    // main (fqhe_sphere_spinless_fermions_diagonalization.cpp:124-160) calls
+    // BuildHamiltonian without BuildFockBasis on this path; the driver builds
+    // the basis first, as the model's BuildHamiltonian requires.
+    int TwoLevelMain(std::map<std::string, std::string>& args, utilities::MpiWrapper& mpi, const iSize_t N,
+                     const io::fileFormat_t format, const bool store, const bool retrieve)
+    {
+        const iSize_t norb = (iSize_t)std::stoul(args["nbr-orbitals"]);
+        if (norb <= 2 || (norb & 1)) {
+            std::fprintf(stderr, "ERROR: MUST SPECIFY nbr orbitals > 2 and an even number of them.\n");
+            return 2;
+        }
+        const iSize_t n1 = (norb - 2) / 2;
+        const int maxLz = (int)n1 - 1, maxLz2 = (int)n1 + 1;
+        SphereQuarticTermTables quart[2];
+        DiagonalQuadraticTermTables<double> quad[2];
+        const char* tag[2] = {"LLL", "2LL"};
+        auto name = [&](const char* kind, int l) {
+            return OutPath(args) + "/" + kind + "_coefficient_table_" + tag[l] + "_L_" + std::to_string(norb) + ".dat";
+        };
+        bool haveQuad = false;
+        if (retrieve) {
+            for (int l = 0; l < 2; ++l) {
+                quad[l].FromFile(name("quadratic", l), format, mpi);
+                quart[l].FromFile(name("quartic", l), format, mpi);
+            }
+            haveQuad = quad[0].GetDimension() > 0;
+        } else {
+            // pseudopotential_model_data.cpp:249-266: V_m lists default to {0, 1}
+            auto pp = [&](const char* a, const char* b) {
+                return args.count(a) ? ParseList(args[a]) : args.count(b) ? ParseList(args[b]) : std::vector<double>{0, 1};
+            };
+            if (!quart[0].FromPseudopotentials(maxLz, pp("pseudopotentials", "two-body-pseudopotentials"), mpi, maxLz2) ||
+                !quart[1].FromPseudopotentials(maxLz2, pp("pseudopotentials-2ll", "two-body-pseudopotentials-2ll"), mpi,
+                                               maxLz2))
+                return 1;
+            if (args.count("energy-levels")) {
+                // SetEnergyLevels (:91-110): n1 first-level then n1 + 2 second-level energies
+                const auto e = ParseList(args["energy-levels"]);
+                if (e.size() != norb) {
+                    std::fprintf(stderr, "--energy-levels needs %u values\n", norb);
+                    return 2;
+                }
+                quad[0].Initialize((kState_t)n1, mpi);
+                quad[1].Initialize((kState_t)(n1 + 2), mpi);
+                for (iSize_t k = 0; k < n1; ++k) quad[0].SetEkk(e[k], (kState_t)k);
+                for (iSize_t k = 0; k < n1 + 2; ++k) quad[1].SetEkk(e[n1 + k], (kState_t)k);
+                haveQuad = true;
+            }
+        }
+        if (store)
+            for (int l = 0; l < 2; ++l) {
+                quad[l].ToFile(name("quadratic", l), format, mpi);
+                quart[l].ToFile(name("quartic", l), format, mpi);
+            }
+        TwoLevelFermionFockBasis basis;
+        bool empty;
+        if (args.count("lz"))
+            empty = basis.GenerateFockSpace(N, n1, n1 + 2, AngularMomentumSector2LL(std::stoi(args["lz"]), maxLz2), mpi);
+        else
+            empty = basis.GenerateFockSpace(N, n1, n1 + 2, TwoLevelNoConstraint(), mpi);
+        if (empty) {
+            std::printf("{\"dim\": 0, \"empty\": true}\n");
+            return 0;
+        }
+        if (args.count("dump-basis")) {
+            std::vector<fock_t> b1(basis.m_basis.size()), b2(basis.m_basis.size());
+            basis.GetFockBasis(b1.data(), b2.data(), b1.size(), mpi);
+            WriteRaw(args["dump-basis"] + ".first", b1.data(), b1.size());
+            WriteRaw(args["dump-basis"] + ".second", b2.data(), b2.size());
+        }
+        TwoLevelSpinlessFermionHamiltonian<double> ham;
+        return Run<double>(ham, quad, quart, haveQuad, args, mpi, basis);
+    }
+}
+
 int main(int argc, char** argv)
 {
     auto args = ParseArgs(argc, argv);
@@ -319,6 +427,7 @@ int main(int argc, char** argv)
     const bool store = Flag(args, "store-terms"), retrieve = Flag(args, "retrieve-terms");
     const bool ofl = args.count("layout") && args["layout"] == "ofl";
 
+    if (args.count("nbr-levels") && args["nbr-levels"] == "2") return TwoLevelMain(args, mpi, N, format, store, retrieve);
     if (!ofl && (args.count("pseudopotentials") || (retrieve && args.count("nbr-orbitals")))) {
         // sphere model: tables from V_m (pseudopotential_model_base.hpp:97-229) or from files
         const iSize_t norb = (iSize_t)std::stoul(args["nbr-orbitals"]);
@@ -346,7 +455,7 @@ int main(int argc, char** argv)
         }
         SpinlessFermionHamiltonian<double> ham;
         // the sphere model adds no quadratic terms unless they were read
-        return Run(ham, &quad, &quart, retrieve, args, mpi, basis);
+        return Run<double>(ham, &quad, &quart, retrieve, args, mpi, basis);
     }
 
     const iSize_t kmax = (iSize_t)std::stoul(args["kmax"]);
@@ -377,7 +486,8 @@ int main(int argc, char** argv)
             quad.ToFile(TermsFileName(args, "quadratic", kmax), format, mpi);
             quart.ToFile(TermsFileName(args, "quartic", kmax), format, mpi);
         }
-        return Run(ham, &quad, &quart, haveQuad > 0, args, mpi, basis);
+        using TT = typename std::decay_t<decltype(ham)>::ValueType;
+        return Run<TT>(ham, &quad, &quart, haveQuad > 0, args, mpi, basis);
     };
     if (cplx) {
         std::vector<dcmplx> vt, eps;

@@ -600,6 +600,210 @@ namespace diagonalization
         std::shared_ptr<hd_basis> m_device;
     };
 
+    // TwoLevelFockState (two_level_fermion_fock_basis.hpp:49-59, .cpp:36-70):
+    // occupations of the first (LLL) and second Landau level, ordered by
+    // (m_first, m_second).  On the device the pair is the combined state
+    // (m_first << (n1+2)) | m_second, whose ascending order is the same.
+    struct TwoLevelFockState
+    {
+        fock_t m_first = 0;
+        fock_t m_second = 0;
+        TwoLevelFockState() = default;
+        TwoLevelFockState(const fock_t first, const fock_t second) : m_first(first), m_second(second) {}
+        friend bool operator<(const TwoLevelFockState& a, const TwoLevelFockState& b)
+        {
+            return a.m_first < b.m_first || (a.m_first == b.m_first && a.m_second < b.m_second);
+        }
+        friend bool operator>(const TwoLevelFockState& a, const TwoLevelFockState& b) { return b < a; }
+        friend bool operator==(const TwoLevelFockState& a, const TwoLevelFockState& b)
+        {
+            return a.m_first == b.m_first && a.m_second == b.m_second;
+        }
+        friend std::ostream& operator<<(std::ostream& os, const TwoLevelFockState& s)
+        {
+            return os << s.m_first << "\t" << s.m_second;
+        }
+    };
+
+    // TestAngularMomentumSector2LL (two_level_angular_momentum_constraint.hpp:41-73)
+    // with max = m_maxLz2: first-level orbital k has m = 2k - (max - 2),
+    // second-level orbital k has m = 2k - max.
+    struct AngularMomentumSector2LL
+    {
+        int m_lzTot, m_max;
+        AngularMomentumSector2LL(const int lzTot, const int max) : m_lzTot(lzTot), m_max(max) {}
+        bool operator()(const TwoLevelFockState& state) const
+        {
+            long s = 0;
+            for (fock_t b = state.m_first; b; b &= b - 1) s += 2 * __builtin_ctzll(b) - (m_max - 2);
+            for (fock_t b = state.m_second; b; b &= b - 1) s += 2 * __builtin_ctzll(b) - m_max;
+            return s == m_lzTot;
+        }
+        hd_sector Descriptor(const iSize_t nbrOrbitals1) const
+        {
+            return hd_sector{HD_SECTOR_LZ2, m_lzTot, m_max, (int32_t)nbrOrbitals1, 0};
+        }
+    };
+
+    // [](const TwoLevelFockState& state){return true;} (two_level_pseudopotential_model.cpp:151)
+    struct TwoLevelNoConstraint
+    {
+        bool operator()(const TwoLevelFockState&) const { return true; }
+        hd_sector Descriptor(const iSize_t) const { return hd_sector{HD_SECTOR_NONE, 0, 0, 0, 0}; }
+    };
+
+    // TwoLevelFermionFockBasis (src/basis/two_level_fermion_fock_basis.hpp:64-89):
+    // the device basis over the 2 n1 + 2 combined orbitals; m_basis is its host
+    // copy split into pairs, in the reference's (m_first, m_second) order.  The
+    // second level holds n1 + 2 orbitals (the model's nbrOrbitalsLLL + 2).
+    class TwoLevelFermionFockBasis
+    {
+        public:
+        std::vector<TwoLevelFockState> m_basis;
+        bool m_basisStored = false;
+        fock_t m_firstState = 0;
+
+        void Clear()
+        {
+            m_basis.clear();
+            m_basisStored = false;
+            m_firstState = 0;
+            m_n1 = m_n2 = 0;
+            m_device.reset();
+        }
+
+        // two_level_fermion_fock_basis.cpp:140-155
+        fock_t GetFullFockSpaceDimension(const iSize_t nbrParticles, const iSize_t nbrOrbitals1,
+                                         const iSize_t nbrOrbitals2) const
+        {
+            auto binom = [](iSize_t n, iSize_t k) -> unsigned __int128 {
+                if (k > n) return 0;
+                unsigned __int128 c = 1;
+                for (iSize_t i = 0; i < k; ++i) c = c * (n - i) / (i + 1);
+                return c;
+            };
+            unsigned __int128 total = 0;
+            for (iSize_t n0 = 0; n0 <= nbrParticles; ++n0)
+                total += binom(nbrOrbitals1, n0) * binom(nbrOrbitals2, nbrParticles - n0);
+            return (fock_t)total;
+        }
+
+        fock_t GetReducedFockSpaceDimension(const iSize_t, const iSize_t, const iSize_t,
+                                            const utilities::MpiWrapper&) const
+        {
+            return m_basis.size();
+        }
+
+        void AddState(const fock_t state1, const fock_t state2)
+        {
+            m_basis.push_back(TwoLevelFockState(state1, state2));
+            m_basisStored = true;
+        }
+
+        // two_level_fermion_fock_basis.cpp:211-241: offset + position, or
+        // _SEARCH_ERROR_CODE_ when the state is not in the basis
+        void FindFockStateIndex(const fock_t offset, const TwoLevelFockState& state, fock_t& index) const
+        {
+            if (!m_basisStored) {
+                std::cerr << "ERROR in FindFockStateIndex: BASIS WAS NOT STORED " << std::endl;
+                return;
+            }
+            auto it = std::lower_bound(m_basis.begin(), m_basis.end(), state);
+            if (it == m_basis.end() || !(*it == state)) {
+                index = _SEARCH_ERROR_CODE_;
+                return;
+            }
+            index = offset + (fock_t)(it - m_basis.begin());
+        }
+
+        void GetFirstState(const fock_t firstNodeIndex, std::vector<TwoLevelFockState>::const_iterator& it_state) const
+        {
+            if (firstNodeIndex != m_firstState || m_basis.empty()) {
+                std::cerr << "\n\tERROR in GetFirstState: non-matching first state index." << std::endl;
+                std::exit(EXIT_FAILURE);
+            }
+            it_state = m_basis.begin();
+        }
+
+        void GetNextState(std::vector<TwoLevelFockState>::const_iterator& it_state) const { ++it_state; }
+
+        // two_level_fermion_fock_basis.cpp:214-337: returns true when the
+        // sector is empty.  The constraint must be one of the descriptors above.
+        bool GenerateFockSpace(const iSize_t nbrParticles, const iSize_t nbrOrbitals1, const iSize_t nbrOrbitals2,
+                               const std::function<bool(const TwoLevelFockState& state)>& Constraint,
+                               utilities::MpiWrapper& mpi)
+        {
+            Clear();
+            hd_sector s;
+            if (auto* p = Constraint.target<AngularMomentumSector2LL>()) s = p->Descriptor(nbrOrbitals1);
+            else if (Constraint.target<TwoLevelNoConstraint>()) s = TwoLevelNoConstraint().Descriptor(nbrOrbitals1);
+            else {
+                std::cerr << "\n\tERROR in GenerateFockSpace: the constraint must be AngularMomentumSector2LL or "
+                             "TwoLevelNoConstraint; a device basis cannot evaluate a host predicate." << std::endl;
+                mpi.m_exitFlag = true;
+                mpi.ExitFlagTest();
+                return true;
+            }
+            if (nbrOrbitals2 != nbrOrbitals1 + 2) {
+                std::cerr << "\n\tERROR in GenerateFockSpace: the second level holds nbrOrbitals1 + 2 orbitals"
+                          << std::endl;
+                mpi.m_exitFlag = true;
+                mpi.ExitFlagTest();
+                return true;
+            }
+            hd_basis* raw = nullptr;
+            const int st = hd_basis_create(&s, nbrParticles, nbrOrbitals1 + nbrOrbitals2, &raw);
+            if (st == HD_ERR_EMPTY) return true;
+            if (!utilities::detail::Ok(st, "GenerateFockSpace")) {
+                mpi.m_exitFlag = true;
+                mpi.ExitFlagTest();
+                return true;
+            }
+            m_device.reset(raw, utilities::detail::BasisDeleter());
+            m_n1 = nbrOrbitals1;
+            m_n2 = nbrOrbitals2;
+            int64_t dim = 0;
+            hd_basis_dim(raw, &dim);
+            std::vector<fock_t> comb((size_t)dim);
+            if (!utilities::detail::Ok(hd_basis_copy_states(raw, 0, dim, comb.data()), "GenerateFockSpace")) {
+                mpi.m_exitFlag = true;
+                mpi.ExitFlagTest();
+            }
+            m_basis.resize((size_t)dim);
+            const fock_t lo = ((fock_t)1 << m_n2) - 1;
+            for (size_t i = 0; i < comb.size(); ++i) m_basis[i] = TwoLevelFockState(comb[i] >> m_n2, comb[i] & lo);
+            m_basisStored = true;
+            m_firstState = 0;
+            return dim == 0;
+        }
+
+        fock_t GetStoredDimension() const { return m_basis.size(); }
+
+        // two_level_fermion_fock_basis.cpp:494-522 (each pair to its own slot;
+        // the reference's loop writes every pair to buffer[0])
+        void GetFockBasis(fock_t* buffer1, fock_t* buffer2, const fock_t dim, utilities::MpiWrapper& mpi) const
+        {
+            if (m_basis.size() >= dim) {
+                for (fock_t i = 0; i < dim; ++i) {
+                    buffer1[i] = m_basis[i].m_first;
+                    buffer2[i] = m_basis[i].m_second;
+                }
+            } else {
+                std::cerr << "\n\tERROR in GetFockBasis: Given copy buffer is longer than the allocated "
+                             "Fock space buffer dimension" << std::endl;
+                mpi.m_exitFlag = true;
+            }
+            mpi.ExitFlagTest();
+        }
+
+        std::shared_ptr<hd_basis> Device() const { return m_device; }
+        iSize_t NbrOrbitals1() const { return m_n1; }
+
+        private:
+        std::shared_ptr<hd_basis> m_device;
+        iSize_t m_n1 = 0, m_n2 = 0;
+    };
+
     // HamiltonianData (src/hamiltonians/hamiltonian_data.hpp:58-85)
     struct HamiltonianData
     {
@@ -629,6 +833,7 @@ namespace diagonalization
     };
 
     template <typename T, class L1, class L2> class MatrixVectorFunction;
+    template <typename T> class TwoLevelSpinlessFermionHamiltonian;
 
     // HamiltonianBase<T,B> (src/hamiltonians/hamiltonian_base.hpp)
     template <typename T, class B>
@@ -638,6 +843,7 @@ namespace diagonalization
                       "matrix elements are double or std::complex<double>");
 
         public:
+        using ValueType = T;
         HamiltonianData m_data;
         B m_fockBasis;
         std::vector<double> m_eigenvalues;
@@ -1083,29 +1289,227 @@ namespace diagonalization
         int m_valueMask = 1 | 2;
     };
 
+    // TwoLevelSpinlessFermionHamiltonian (src/hamiltonians/two_level_spinless_fermion_hamiltonian.hpp):
+    // same-level terms only.  The term tables of each call are recorded and the
+    // device matrix is rebuilt after every Add_CdCdCC_Terms call from all
+    // terms added so far, so that after the level-0 call the matrix holds the
+    // level-0 terms as the reference's does.  The model's order is required:
+    // Add_CdC_Terms (level 0, then 1) before Add_CdCdCC_Terms level 0, then 1
+    // (two_level_pseudopotential_model.cpp:188-200); the device build sums each
+    // diagonal entry in that order with the Trim between the levels.
+    template <typename T>
+    class TwoLevelSpinlessFermionHamiltonian : public HamiltonianBase<T, TwoLevelFermionFockBasis>
+    {
+        static_assert(std::is_same<T, double>::value, "the two-level Hamiltonian is real");
+
+        public:
+        // two_level_spinless_fermion_hamiltonian.hpp:66-114
+        void Initialize(const iSize_t nbrParticles, const iSize_t nbrOrbitals,
+                        const utilities::storageMethod_t storageMethod, utilities::MpiWrapper& mpi)
+        {
+            this->m_data = HamiltonianData(nbrParticles, nbrOrbitals, storageMethod);
+            m_nbrOrbitals1 = (nbrOrbitals - 2) / 2;
+            m_nbrOrbitals2 = m_nbrOrbitals1 + 2;
+            if (0 >= (int)nbrParticles || nbrParticles >= 64 || nbrParticles > nbrOrbitals) {
+                std::cerr << "\n\tERROR: invalid number of particles " << nbrParticles << std::endl;
+                mpi.m_exitFlag = true;
+            }
+            if (nbrOrbitals <= 2 || (nbrOrbitals & 1)) {
+                std::cerr << "\n\tERROR: MUST SPECIFY nbr orbitals > 2 and an even number of them." << std::endl;
+                mpi.m_exitFlag = true;
+            }
+            mpi.ExitFlagTest();
+            this->m_data.SetDataDimensions(this->m_fockBasis.GetStoredDimension(), mpi.m_nbrProcs, mpi);
+            for (int l = 0; l < 2; ++l) {
+                m_eps[l].clear();
+                m_kt[l].clear();
+                m_vt[l].clear();
+            }
+            m_quarticLevels = 0;
+        }
+
+        void Clear()
+        {
+            this->DeallocateMatrixMemory();
+            this->m_fockBasis.Clear();
+            this->m_data.Clear();
+            this->m_crsSparseMatrix.Clear();
+            this->m_eigenvalues.clear();
+            this->m_eigenvectors.clear();
+            for (int l = 0; l < 2; ++l) {
+                m_eps[l].clear();
+                m_kt[l].clear();
+                m_vt[l].clear();
+            }
+            m_quarticLevels = 0;
+        }
+
+        // two_level_spinless_fermion_hamiltonian.hpp:143-262 (diagonal E_k only,
+        // GetK1(k2) = k2 on the sphere tables)
+        template <class Tab>
+        void Add_CdC_Terms(Tab* termTables, const iSize_t level1, const iSize_t level2, const fock_t minRow,
+                           const fock_t maxRow, utilities::MpiWrapper& mpi)
+        {
+            if (!CheckLevels(level1 == level2, level1, "Add_CdC_Terms", mpi) || !CheckRange(minRow, maxRow, mpi)) return;
+            if (m_quarticLevels) {
+                std::cerr << "\n\tERROR in Add_CdC_Terms: quadratic terms must come before the quartic terms" << std::endl;
+                mpi.m_exitFlag = true;
+                return;
+            }
+            const iSize_t L = level1 ? m_nbrOrbitals2 : m_nbrOrbitals1;
+            if (termTables->GetDimension() != L) {
+                std::cerr << "\n\tERROR in Add_CdC_Terms: level " << level1 << " table has "
+                          << termTables->GetDimension() << " entries, expected " << L << std::endl;
+                mpi.m_exitFlag = true;
+                return;
+            }
+            m_eps[level1].assign(L, 0.0);
+            for (iSize_t k = 0; k < L; ++k) {
+                kState_t buf[8];
+                iSize_t n = 0;
+                termTables->GetK1(buf, n, (kState_t)k);
+                if (n > 1 || (n == 1 && buf[0] != k)) {
+                    std::cerr << "\n\tERROR in Add_CdC_Terms: only diagonal quadratic terms are supported" << std::endl;
+                    mpi.m_exitFlag = true;
+                    return;
+                }
+                m_eps[level1][k] = n ? termTables->GetEkk((kState_t)k, (kState_t)k) : 0.0;
+            }
+        }
+
+        // two_level_spinless_fermion_hamiltonian.hpp:280-480
+        template <class Tab>
+        void Add_CdCdCC_Terms(Tab* termTables, const iSize_t level1, const iSize_t level2, const iSize_t level3,
+                              const iSize_t level4, const fock_t minRow, const fock_t maxRow, utilities::MpiWrapper& mpi)
+        {
+            const bool same = level1 == level2 && level2 == level3 && level3 == level4;
+            if (!CheckLevels(same, level1, "Add_CdCdCC_Terms", mpi) || !CheckRange(minRow, maxRow, mpi)) return;
+            if ((int)level1 != m_quarticLevels) {
+                std::cerr << "\n\tERROR in Add_CdCdCC_Terms: the levels are added in the order 0, 1" << std::endl;
+                mpi.m_exitFlag = true;
+                return;
+            }
+            const iSize_t L = level1 ? m_nbrOrbitals2 : m_nbrOrbitals1;
+            if (termTables->GetKTable()->size() != (size_t)L * L * L || termTables->GetVTable()->size() != (size_t)L * L * L) {
+                std::cerr << "\n\tERROR in Add_CdCdCC_Terms: level " << level1 << " needs a sphere table with kMax "
+                          << L << std::endl;
+                mpi.m_exitFlag = true;
+                return;
+            }
+            m_kt[level1] = *termTables->GetKTable();
+            m_vt[level1] = *termTables->GetVTable();
+            ++m_quarticLevels;
+            Rebuild(mpi);
+        }
+
+        void SetValueFormats(const int mask) { m_valueMask = mask | 1; }
+
+        private:
+        bool CheckLevels(const bool same, const iSize_t level, const char* where, utilities::MpiWrapper& mpi)
+        {
+            if (level > 1) {
+                std::cerr << "\n\tERROR in " << where << ": level specification incorrect (set to 0,1 only)." << std::endl;
+                mpi.m_exitFlag = true;
+                return false;
+            }
+            if (!same) {
+                std::cerr << "\n\tERROR in " << where << ": only coded for terms in the same level so far." << std::endl;
+                mpi.m_exitFlag = true;
+                return false;
+            }
+            return true;
+        }
+
+        bool CheckRange(const fock_t minRow, const fock_t maxRow, utilities::MpiWrapper& mpi)
+        {
+            if (!this->m_fockBasis.Device() || this->m_fockBasis.NbrOrbitals1() != m_nbrOrbitals1) {
+                std::cerr << "\n\tERROR: no matching device two-level basis (GenerateFockSpace first)" << std::endl;
+                mpi.m_exitFlag = true;
+                return false;
+            }
+            if (minRow != 0 || maxRow < this->m_data.m_fockSpaceDim) {
+                std::cerr << "\n\tERROR: the device build covers all rows" << std::endl;
+                mpi.m_exitFlag = true;
+                return false;
+            }
+            return true;
+        }
+
+        void Rebuild(utilities::MpiWrapper& mpi)
+        {
+            const iSize_t L[2] = {m_nbrOrbitals1, m_nbrOrbitals2};
+            std::vector<kState_t> kt[2];
+            std::vector<double> vt[2];
+            for (int l = 0; l < 2; ++l) {
+                if (m_kt[l].empty()) {  // not added yet: no terms in this level
+                    kt[l].assign((size_t)L[l] * L[l] * L[l], (kState_t)0xFFFF);
+                    vt[l].assign((size_t)L[l] * L[l] * L[l], 0.0);
+                } else {
+                    kt[l] = m_kt[l];
+                    vt[l] = m_vt[l];
+                }
+            }
+            hd_tables* rawT = nullptr;
+            if (!utilities::detail::Ok(hd_tables_create_two_level(
+                                           m_nbrOrbitals1, kt[0].data(), vt[0].data(),
+                                           m_eps[0].empty() ? nullptr : m_eps[0].data(), kt[1].data(), vt[1].data(),
+                                           m_eps[1].empty() ? nullptr : m_eps[1].data(), &rawT),
+                                       "Add_CdCdCC_Terms")) {
+                mpi.m_exitFlag = true;
+                return;
+            }
+            std::shared_ptr<hd_tables> tables(rawT, utilities::detail::TablesDeleter());
+            hd_hamiltonian* rawH = nullptr;
+            this->m_device.reset();
+            if (!utilities::detail::Ok(hd_ham_build(this->m_fockBasis.Device().get(), tables.get(), HD_STORAGE_FULL,
+                                                    _SPARSE_ZERO_TOL_, m_valueMask, &rawH),
+                                       "Add_CdCdCC_Terms")) {
+                mpi.m_exitFlag = true;
+                return;
+            }
+            this->m_device.reset(rawH, utilities::detail::HamDeleter());
+            this->m_data.m_matrixAllocated = true;
+            this->m_data.m_storageMethod = utilities::_SPARSE_CRS_;
+            this->m_crsSparseMatrix.AdoptDevice(this->m_fockBasis.Device(), tables, _SPARSE_ZERO_TOL_,
+                                                (int)this->m_data.m_fockSpaceDim);
+        }
+
+        iSize_t m_nbrOrbitals1 = 0, m_nbrOrbitals2 = 0;
+        std::vector<double> m_eps[2];
+        std::vector<kState_t> m_kt[2];
+        std::vector<double> m_vt[2];
+        int m_quarticLevels = 0;
+        int m_valueMask = 1 | 2;
+    };
+
     // MatrixVectorFunction<T,L1,L2> (src/hamiltonians/matrix_vector_routines.hpp:40-135)
     template <typename T, class L1, class L2>
     class MatrixVectorFunction
     {
         public:
+        // the one-level and the two-level Hamiltonian constructors (:56-91)
         MatrixVectorFunction(SpinlessFermionHamiltonian<T>& hamiltonian, const utilities::MpiWrapper&)
-            : m_hamiltonian(&hamiltonian), UPLO(hamiltonian.m_arpackWrapper.GetUpperOrLower()) {}
+            : m_device([&hamiltonian]() { return hamiltonian.Device(); }),
+              UPLO(hamiltonian.m_arpackWrapper.GetUpperOrLower()) {}
+        MatrixVectorFunction(TwoLevelSpinlessFermionHamiltonian<T>& hamiltonian, const utilities::MpiWrapper&)
+            : m_device([&hamiltonian]() { return hamiltonian.Device(); }),
+              UPLO(hamiltonian.m_arpackWrapper.GetUpperOrLower()) {}
 
         // matrix_vector_routines.hpp:94-104: y = H x on host vectors of length nodeDim
         void Simple(T* x, T* y, const utilities::MpiWrapper& mpi)
         {
-            if (!utilities::detail::Ok(hd_spmv_host(m_hamiltonian->Device(), x, y), "MatrixVectorFunction::Simple"))
+            if (!utilities::detail::Ok(hd_spmv_host(m_device(), x, y), "MatrixVectorFunction::Simple"))
                 const_cast<utilities::MpiWrapper&>(mpi).m_exitFlag = true;
         }
 
         // The same product on device vectors (asynchronous on the matrix's stream).
         int SimpleDevice(const T* d_x, T* d_y, const int precision = HD_F64)
         {
-            return hd_spmv(m_hamiltonian->Device(), precision, d_x, d_y);
+            return hd_spmv(m_device(), precision, d_x, d_y);
         }
 
         private:
-        SpinlessFermionHamiltonian<T>* m_hamiltonian;
+        std::function<hd_hamiltonian*()> m_device;
         char UPLO;
     };
 
@@ -1431,15 +1835,24 @@ namespace diagonalization
     {
         public:
         // BaseGenerate4KTable + BaseTermsFromPseudopotentials
-        // (pseudopotential_model_base.hpp:97-229) on the C ABI
+        // (pseudopotential_model_base.hpp:97-229) on the C ABI.  FindM1 stores
+        // the label of m_maxLz2 + 2 where m1 is out of range; m_maxLz2 = maxLz
+        // for one level (label 0xFFFF), n1 + 1 for the two-level model's first
+        // level (maxLz = n1 - 1: label 0xFFFE).
         bool FromPseudopotentials(const int maxLz, const std::vector<double>& pseudopotentials,
-                                  const utilities::MpiWrapper& mpi)
+                                  const utilities::MpiWrapper& mpi, const int maxLz2 = -1)
         {
             Initialize((kState_t)(maxLz + 1), mpi);
-            return utilities::detail::Ok(hd_sphere_ktable(maxLz, m_kTable.data()), "QuarticTermTables") &&
-                   utilities::detail::Ok(hd_sphere_vtable(maxLz, pseudopotentials.data(),
-                                                          (int32_t)pseudopotentials.size(), m_vTable.data()),
-                                         "QuarticTermTables");
+            const bool ok = utilities::detail::Ok(hd_sphere_ktable(maxLz, m_kTable.data()), "QuarticTermTables") &&
+                            utilities::detail::Ok(hd_sphere_vtable(maxLz, pseudopotentials.data(),
+                                                                   (int32_t)pseudopotentials.size(), m_vTable.data()),
+                                                  "QuarticTermTables");
+            if (ok && maxLz2 >= 0) {
+                const kState_t err = (kState_t)((-(maxLz2 + 2) + maxLz) / 2);
+                for (auto& k : m_kTable)
+                    if (k >= m_kMax) k = err;
+            }
+            return ok;
         }
         void GetK1(kState_t* buf, iSize_t& n, const kState_t k2, const kState_t k3, const kState_t k4) const override
         {

@@ -175,3 +175,46 @@ def test_dropin_flux_lattice_complex_tables(oracle, tmp_path):
     assert np.array_equal(np.fromfile(pre + ".values", dtype=np.complex128), up.values())
     want = np.linalg.eigvalsh(up.to_dense(len(bas)))[:6]
     assert r["ok"] and np.max(np.abs(np.array(r["eigenvalues"]) - want) / np.maximum(1, np.abs(want))) < 1e-10
+
+
+@pytest.mark.gpu
+def test_dropin_two_level(oracle, tmp_path):
+    """--nbr-levels 2 (SphereTwoLevelPseudopotentialModel through the drop-in
+    TwoLevelFermionFockBasis / TwoLevelSpinlessFermionHamiltonian): basis
+    pairs, upper CRS bit-exact against the oracle, eigenvalues against dense,
+    and the stored term tables (level 0's k table carries FindM1's label of
+    m_maxLz2 + 2)."""
+    from hamiltonian_diagonalization_amd import fileio
+    n1, N = 6, 5
+    norb = 2 * n1 + 2
+    say("drop-in two-level n1", n1, "N", N, "lz 1")
+    v1 = [0.0, 0.7, 0.0, 0.2]
+    v2 = [0.0, 0.5, 0.0, 0.3, 0.0, 0.1]
+    e = np.linspace(-0.3, 0.4, norb)
+    pre = str(tmp_path / "crs")
+    out = tmp_path / "out"
+    out.mkdir()
+    p = run(["--nbr-levels", 2, "--nbr-particles", N, "--nbr-orbitals", norb, "--lz", 1,
+             "--pseudopotentials", pseudo_arg(v1), "--pseudopotentials-2ll", pseudo_arg(v2),
+             "--energy-levels", pseudo_arg(e), "--nbr-eigenvalues", 3, "--tol", 1e-12, "--dump-crs", pre,
+             "--dump-basis", str(tmp_path / "basis"), "--store-terms", 1, "--file-format", 1, "--out-path", str(out)])
+    r = last_json(p)
+    say(r)
+    f, g = oracle.two_level_basis(N, n1, 1)
+    assert r["dim"] == len(f)
+    assert np.array_equal(np.fromfile(str(tmp_path / "basis") + ".first", dtype=np.uint64), f)
+    assert np.array_equal(np.fromfile(str(tmp_path / "basis") + ".second", dtype=np.uint64), g)
+    kt1, kt2 = oracle.two_level_ktable(n1, 0), oracle.two_level_ktable(n1, 1)
+    vt1, vt2 = hd.sphere_vtable(n1 - 1, v1), hd.sphere_vtable(n1 + 1, v2)
+    up = oracle.two_level_build(n1, f, g, kt1, vt1, kt2, vt2, e[:n1], e[n1:], full=False)
+    assert np.array_equal(np.fromfile(pre + ".rowstarts", dtype=np.int32), up.rowptr.astype(np.int32))
+    assert np.array_equal(np.fromfile(pre + ".cols", dtype=np.int32), up.col)
+    assert np.array_equal(np.fromfile(pre + ".values", dtype=np.float64), up.values())
+    w = np.linalg.eigvalsh(up.to_dense(len(f), mirror=True))
+    got = np.array(r["eigenvalues"])
+    assert r["ok"] and abs(got[0] - w[0]) < 1e-9
+    assert all(np.min(np.abs(w - x)) < 1e-9 for x in got)
+    for tag, kt, vt, L in (("LLL", kt1, vt1, n1), ("2LL", kt2, vt2, n1 + 2)):
+        kmax, v, k = fileio.read_term_table(str(out / f"quartic_coefficient_table_{tag}_L_{norb}.dat"), "sphere")
+        assert kmax == L and np.array_equal(k, kt) and np.array_equal(v, vt)
+    assert 0xFFFE in kt1 and 0xFFFF not in kt1
