@@ -120,6 +120,8 @@ SIGNATURES = {
     "hd_sphere_vtable": (ctypes.c_int, [_I32, _P, _I32, _P]),
     "hd_tables_create": (ctypes.c_int, [_I32, _U32, _P, _P, _I32, _P, _PP]),
     "hd_tables_create_two_level": (ctypes.c_int, [_U32, _P, _P, _P, _P, _P, _P, _PP]),
+    "hd_ofl_ktable": (ctypes.c_int, [_I32, _I32, _P, _P, _P]),
+    "hd_ofl_vtable": (ctypes.c_int, [_I32, _I32, _I32, _P, _P, _P]),
     "hd_tables_destroy": (ctypes.c_int, [_P]),
     "hd_ham_build": (ctypes.c_int, [_P, _P, _I32, _D, _I32, _PP]),
     "hd_ham_destroy": (ctypes.c_int, [_P]),
@@ -419,6 +421,32 @@ def sphere_ktable(max_lz: int) -> np.ndarray:
     kt = np.zeros((max_lz + 1) ** 3, dtype=np.uint16)
     _check(lib().hd_sphere_ktable(max_lz, _ptr(kt)))
     return kt
+
+
+def ofl_ktable(dim_x: int, dim_y: int, with_g: bool = False):
+    """Generate4KTable of the flux-lattice model (folded layout); with_g also
+    returns the reciprocal-lattice shifts (gx, gy) of every entry."""
+    K = dim_x * dim_y
+    n = K * K * (K + 1) // 2
+    kt = np.zeros(n, dtype=np.uint16)
+    gx = np.zeros(n, dtype=np.int32)
+    gy = np.zeros(n, dtype=np.int32)
+    _check(lib().hd_ofl_ktable(dim_x, dim_y, _ptr(kt), _ptr(gx), _ptr(gy)))
+    return (kt, gx, gy) if with_g else kt
+
+
+def ofl_vtable(dim_x: int, dim_y: int, cutoff: int, bloch: np.ndarray, timing: bool = False):
+    """GenerateQuarticTerms on the device.  bloch: complex [K][(2c+1)^2][2]
+    (band 0, spin fastest).  Returns the folded complex V table (and the
+    kernel time in ms with timing=True)."""
+    K = dim_x * dim_y
+    b = np.ascontiguousarray(bloch, dtype=np.complex128)
+    if b.size != K * 2 * (2 * cutoff + 1) ** 2:
+        raise ValueError("bloch must hold K * 2 * (2c+1)^2 coefficients")
+    out = np.zeros(K * K * (K + 1) // 2, dtype=np.complex128)
+    ms = ctypes.c_double()
+    _check(lib().hd_ofl_vtable(dim_x, dim_y, cutoff, _ptr(b), _ptr(out), ctypes.byref(ms)))
+    return (out, ms.value) if timing else out
 
 
 def sphere_vtable(max_lz: int, pseudopotentials: Sequence[float]) -> np.ndarray:

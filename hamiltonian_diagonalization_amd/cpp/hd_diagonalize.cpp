@@ -22,6 +22,9 @@
 //          --observables 1   eigenvector 0's density, participation ratio, 5 most probable
 //              states (Observables, fermion_observables.hpp), plus the translational and
 //              rotational density-density functions when --dim-x/--dim-y are given
+//          --bloch FILE --cutoff C [--interaction-strength S] (with --layout ofl --complex 1
+//              --dim-x X --dim-y Y): the quartic table from the raw complex band-0 Bloch blocks
+//              (K blocks of 2 (2C+1)^2 values, spin fastest) instead of --ktable/--vtable
 //          --store-terms 1 / --retrieve-terms 1   term tables in OUT as
 //              quadratic|quartic_coefficient_table_L_<NORB>.dat (sphere) or
 //              ..._kx_<X>_ky_<Y>.dat (--layout ofl with --dim-x X --dim-y Y)
@@ -461,7 +464,10 @@ int main(int argc, char** argv)
     const iSize_t kmax = (iSize_t)std::stoul(args["kmax"]);
     const bool cplx = args.count("complex") && args["complex"] == "1";
     std::vector<kState_t> kt;
-    if (!retrieve && !ReadRaw(args["ktable"], kt)) { std::fprintf(stderr, "cannot read --ktable\n"); return 2; }
+    if (!retrieve && !args.count("bloch") && !ReadRaw(args["ktable"], kt)) {
+        std::fprintf(stderr, "cannot read --ktable\n");
+        return 2;
+    }
     FermionFockBasis basis;
     if (!MakeBasis(basis, args, mpi, N, kmax)) {
         std::printf("{\"dim\": 0, \"empty\": true}\n");
@@ -473,8 +479,29 @@ int main(int argc, char** argv)
             quart.FromFile(TermsFileName(args, "quartic", kmax), format, mpi);
             return quart.GetKMax() == kmax ? 1 : -1;
         }
-        if (!ReadRaw(args["vtable"], vt)) { std::fprintf(stderr, "cannot read --vtable\n"); return -1; }
-        if (!FillRaw(quart, kmax, kt, vt, mpi)) return -1;
+        if (args.count("bloch")) {
+            // BuildTermTables (interacting_ofl_model.cpp:112-256) from one Bloch table:
+            // V from the Bloch coefficients, then SetCoefficient(0.5 * strength)
+            if constexpr (std::is_same<std::decay_t<decltype(quart)>, OflQuarticTermTables>::value) {
+                std::vector<dcmplx> bloch;
+                if (!ReadRaw(args["bloch"], bloch)) { std::fprintf(stderr, "cannot read --bloch\n"); return -1; }
+                const int dx = std::stoi(args["dim-x"]), dy = std::stoi(args["dim-y"]);
+                const int cut = std::stoi(args.count("cutoff") ? args["cutoff"] : "5");
+                if ((size_t)dx * dy != kmax || bloch.size() != (size_t)kmax * 2 * (2 * cut + 1) * (2 * cut + 1)) {
+                    std::fprintf(stderr, "--bloch needs kmax = dim-x * dim-y blocks of 2 (2c+1)^2 values\n");
+                    return -1;
+                }
+                if (!quart.FromBlochCoefficients(dx, dy, cut, bloch.data(), mpi)) return -1;
+                const double strength = args.count("interaction-strength") ? std::stod(args["interaction-strength"]) : 1.0;
+                quart.SetCoefficient(dcmplx(0.5 * strength));
+            } else {
+                std::fprintf(stderr, "--bloch needs --layout ofl --complex 1\n");
+                return -1;
+            }
+        } else {
+            if (!ReadRaw(args["vtable"], vt)) { std::fprintf(stderr, "cannot read --vtable\n"); return -1; }
+            if (!FillRaw(quart, kmax, kt, vt, mpi)) return -1;
+        }
         quad.Initialize((kState_t)kmax, mpi);
         const bool haveQuad = args.count("eps") && ReadRaw(args["eps"], eps);
         for (iSize_t k = 0; haveQuad && k < kmax; ++k) quad.SetEkk(eps.at(k), (kState_t)k);

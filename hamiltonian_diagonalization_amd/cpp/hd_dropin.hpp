@@ -1892,6 +1892,21 @@ namespace diagonalization
     class OflQuarticTermTables : public TermTables<dcmplx>
     {
         public:
+        // Generate4KTable + GenerateQuarticTerms (interacting_ofl_model.cpp:258-296,
+        // 457-590) on the device from one Bloch table: bloch holds, per k, the
+        // band-0 block of NonInteractingOflModel::m_blochCoefficients
+        // (2 (2c+1)^2 values, spin fastest).  BuildTermTables then scales the
+        // table with SetCoefficient(0.5 * interactionStrength).
+        bool FromBlochCoefficients(const int dimX, const int dimY, const int cutoff, const dcmplx* bloch,
+                                   const utilities::MpiWrapper& mpi)
+        {
+            Initialize((kState_t)(dimX * dimY), mpi);
+            return utilities::detail::Ok(hd_ofl_ktable(dimX, dimY, m_kTable.data(), nullptr, nullptr),
+                                         "OflQuarticTermTables") &&
+                   utilities::detail::Ok(hd_ofl_vtable(dimX, dimY, cutoff, reinterpret_cast<const double*>(bloch),
+                                                        reinterpret_cast<double*>(m_vTable.data()), nullptr),
+                                         "OflQuarticTermTables");
+        }
         void GetK1(kState_t* buf, iSize_t& n, const kState_t k2, const kState_t k3, const kState_t k4) const override
         {
             n = 1;

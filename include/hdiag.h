@@ -168,6 +168,27 @@ int hd_tables_create_two_level(uint32_t n1, const uint16_t* h_ktable1, const dou
                                const double* h_vtable2, const double* h_eps2, hd_tables** out);
 int hd_tables_destroy(hd_tables* t);
 
+/* ------------------------------------------------- flux-lattice term tables */
+
+/* InteractingOflModel::Generate4KTable (interacting_ofl_model.cpp:258-296) with
+ * FindK1 (:374-407): the folded k table over k4 <= k3 (k2 fastest),
+ * (K+1)K^2/2 entries for K = dim_x*dim_y <= 64, and the reciprocal-lattice
+ * shifts (h_gx, h_gy may be NULL). */
+int hd_ofl_ktable(int32_t dim_x, int32_t dim_y, uint16_t* h_ktable, int32_t* h_gx, int32_t* h_gy);
+/* InteractingOflModel::GenerateQuarticTerms (interacting_ofl_model.cpp:457-590)
+ * on the device, bit-identical to the reference's sequential FP64 (Kahan sums
+ * of ArrayConvolution2D, array_convolution.hpp:136-230).  h_bloch holds, for
+ * each k, the band-0 block of NonInteractingOflModel::m_blochCoefficients:
+ * 2(2c+1)^2 complex values (interleaved re,im) with the spin index fastest
+ * (GetBlochCoefficients, noninteracting_ofl_model.cpp:550-582); the reference
+ * generates them with one cutoff c in both directions
+ * (NonInteractingOflModelGrid::GenerateBlochWaveFunctionTable).  Writes the
+ * (K+1)K^2/2 complex V (interleaved) in the folded order, before
+ * SetCoefficient(0.5*interactionStrength).  kernel_ms (may be NULL) receives the
+ * kernel time. */
+int hd_ofl_vtable(int32_t dim_x, int32_t dim_y, int32_t cutoff, const double* h_bloch, double* h_vtable,
+                  double* kernel_ms);
+
 /* ------------------------------------------------------------------- build */
 
 typedef struct {

@@ -886,3 +886,124 @@ int64_t oc_two_level_build(int n1, int n2, const uint64_t* first, const uint64_t
     acc_free(&acc);
     return nnz;
 }
+
+/* ------------------------------------------------ flux-lattice V from Bloch */
+
+/* FindK1 (interacting_ofl_model.cpp:374-407): k1 = k3 + k4 - k2 per grid
+ * component, folded into the zone; g = (folded - unfolded) / dim per component
+ * (exact: the difference is a multiple of dim). */
+int oc_ofl_find_k1(int dimX, int dimY, int k2, int k3, int k4, int* gx, int* gy)
+{
+    const int fx = k3 / dimY + k4 / dimY - k2 / dimY;
+    const int fy = k3 % dimY + k4 % dimY - k2 % dimY;
+    const int sx = ((fx % dimX) + dimX) % dimX, sy = ((fy % dimY) + dimY) % dimY;
+    *gx = (sx - fx) / dimX;
+    *gy = (sy - fy) / dimY;
+    return sx * dimY + sy;
+}
+
+typedef struct { double re, im; } oc_c;
+
+static inline oc_c c_mul(oc_c a, oc_c b)
+{
+    /* GCC's inline complex product without -ffast-math: (ac - bd, ad + bc),
+     * each product rounded before the sum (no contraction on x86-64 -O3; the
+     * oracle is built with -ffp-contract=off) */
+    const double ac = a.re * b.re, bd = a.im * b.im, ad = a.re * b.im, bc = a.im * b.re;
+    oc_c r = {ac - bd, ad + bc};
+    return r;
+}
+
+/* KahanAccumulation<dcmplx> += (kahan_arithmetic.hpp:67-76), componentwise */
+static inline void kahan_add(oc_c* sum, oc_c* corr, oc_c b)
+{
+    const double tv_re = b.re - corr->re, tv_im = b.im - corr->im;
+    const double ts_re = sum->re + tv_re, ts_im = sum->im + tv_im;
+    corr->re = (ts_re - sum->re) - tv_re;
+    corr->im = (ts_im - sum->im) - tv_im;
+    sum->re = ts_re;
+    sum->im = ts_im;
+}
+
+/* ArrayConvolution2D<dcmplx,dcmplx> (array_convolution.hpp:136-230): output
+ * (i, j) of the (2X-1) x (2Y-1) grid is the Kahan sum over the overlap, x
+ * outer and y inner, of left[..]*right[..] with the four index branches of the
+ * reference; outside [top, 2X-1-bottom) x [leftOff, 2Y-1-rightOff) it is 0. */
+void oc_conv2d(const double* right_, const double* left_, int X, int Y, double* out_, int top, int leftOff,
+               int bottom, int rightOff)
+{
+    const oc_c* right = (const oc_c*)right_;
+    const oc_c* left = (const oc_c*)left_;
+    oc_c* out = (oc_c*)out_;
+    const int cX = 2 * X - 1, cY = 2 * Y - 1;
+    for (int i = 0; i < cX; ++i) {
+        const int kX = i + 1 < cX - i ? i + 1 : cX - i;
+        for (int j = 0; j < cY; ++j) {
+            oc_c s = {0, 0}, c = {0, 0};
+            if (i >= top && i < cX - bottom && j >= leftOff && j < cY - rightOff) {
+                const int kY = j + 1 < cY - j ? j + 1 : cY - j;
+                for (int x = 0; x < kX; ++x)
+                    for (int y = 0; y < kY; ++y) {
+                        const int xl = X > i ? x : x + X - kX, xr = X > i ? x + X - kX : x;
+                        const int yl = Y > j ? y : y + Y - kY, yr = Y > j ? y + Y - kY : y;
+                        kahan_add(&s, &c, c_mul(left[xl * Y + yl], right[xr * Y + yr]));
+                    }
+            }
+            out[i * cY + j] = s;
+        }
+    }
+}
+
+/* GenerateQuarticTerms (interacting_ofl_model.cpp:457-590) for table entries
+ * [e0, e1) of the folded order (k4 < kMax, k3 >= k4, k2 < kMax).  bloch holds,
+ * per k, the band-0 block of NonInteractingOflModel::m_blochCoefficients:
+ * 2 (2cx+1)(2cy+1) complex values with the spin index fastest
+ * (GetBlochCoefficients, noninteracting_ofl_model.cpp:550-582).  Writes
+ * complex V (interleaved) for each entry. */
+void oc_ofl_vtable(int dimX, int dimY, int cutX, int cutY, const double* bloch_, int64_t e0, int64_t e1,
+                   double* out_)
+{
+    const int X = 2 * cutX + 1, Y = 2 * cutY + 1, n = X * Y;
+    const int kMax = dimX * dimY;
+    const int cdim = (2 * Y - 1) * (2 * X - 1);
+    const int cao = X + Y - 1; /* GetArrayConvolution1DSize(X, Y) */
+    const oc_c* bloch = (const oc_c*)bloch_;
+    oc_c* out = (oc_c*)out_;
+    oc_c* arr = (oc_c*)malloc(sizeof(oc_c) * 8 * n);
+    oc_c* conv = (oc_c*)malloc(sizeof(oc_c) * 4 * cdim);
+    oc_c *Au = arr, *Ad = arr + n, *Bu = arr + 2 * n, *Bd = arr + 3 * n, *Cu = arr + 4 * n, *Cd = arr + 5 * n,
+         *Du = arr + 6 * n, *Dd = arr + 7 * n;
+    int64_t counter = 0;
+    for (int k4 = 0; k4 < kMax; ++k4)
+        for (int k3 = k4; k3 < kMax; ++k3)
+            for (int k2 = 0; k2 < kMax; ++k2, ++counter) {
+                if (counter < e0 || counter >= e1) continue;
+                int gx, gy;
+                const int k1 = oc_ofl_find_k1(dimX, dimY, k2, k3, k4, &gx, &gy);
+                for (int i = 0; i < n; ++i) {
+                    const oc_c* b1 = bloch + (size_t)k1 * 2 * n + 2 * i;
+                    const oc_c* b2 = bloch + (size_t)k2 * 2 * n + 2 * i;
+                    const oc_c* b3 = bloch + (size_t)k3 * 2 * n + 2 * i;
+                    const oc_c* b4 = bloch + (size_t)k4 * 2 * n + 2 * i;
+                    Au[i].re = b1[0].re; Au[i].im = -b1[0].im; Ad[i].re = b1[1].re; Ad[i].im = -b1[1].im;
+                    Bu[i].re = b2[0].re; Bu[i].im = -b2[0].im; Bd[i].re = b2[1].re; Bd[i].im = -b2[1].im;
+                    Cu[i] = b3[0]; Cd[i] = b3[1];
+                    Du[i] = b4[0]; Dd[i] = b4[1];
+                }
+                const int o1 = gx < 0 ? -gx : 0, o2 = gy < 0 ? -gy : 0, o3 = gx > 0 ? gx : 0, o4 = gy > 0 ? gy : 0;
+                oc_c *c1 = conv, *c2 = conv + cdim, *c3 = conv + 2 * cdim, *c4 = conv + 3 * cdim;
+                oc_conv2d((const double*)Bd, (const double*)Du, X, Y, (double*)c1, 0, 0, 0, 0);
+                oc_conv2d((const double*)Cd, (const double*)Au, X, Y, (double*)c2, o1, o2, o3, o4);
+                oc_conv2d((const double*)Bu, (const double*)Dd, X, Y, (double*)c3, 0, 0, 0, 0);
+                oc_conv2d((const double*)Cu, (const double*)Ad, X, Y, (double*)c4, o1, o2, o3, o4);
+                oc_c s = {0, 0}, c = {0, 0};
+                const int lim = cdim - o1 * cao - o2 - o3 * cao - o4;
+                for (int i = 0; i < lim; ++i) {
+                    kahan_add(&s, &c, c_mul(c1[i + o3 * cao + o4], c2[i + o1 * cao + o2]));
+                    kahan_add(&s, &c, c_mul(c3[i + o3 * cao + o4], c4[i + o1 * cao + o2]));
+                }
+                out[counter - e0] = s;
+            }
+    free(arr);
+    free(conv);
+}

@@ -102,6 +102,13 @@ int64_t oc_two_level_build(int n1, int n2, const uint64_t* first, const uint64_t
                            const uint16_t* kt2, const double* vt2, const double* eps2, double tol, int full,
                            int64_t* rowptr, int32_t* col, double* val, int64_t cap, int64_t* n_trimmed);
 
+/* Flux-lattice V from Bloch coefficients (GenerateQuarticTerms): see hd_oracle.c. */
+int oc_ofl_find_k1(int dimX, int dimY, int k2, int k3, int k4, int* gx, int* gy);
+void oc_conv2d(const double* right, const double* left, int X, int Y, double* out, int top, int leftOff,
+               int bottom, int rightOff);
+void oc_ofl_vtable(int dimX, int dimY, int cutX, int cutY, const double* bloch, int64_t e0, int64_t e1,
+                   double* out);
+
 #ifdef __cplusplus
 }
 #endif

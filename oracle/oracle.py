@@ -92,6 +92,12 @@ def lib():
         L.oc_spmv_sym_block.argtypes = [i64, i64, P, P, P, i32, P, P]
         L.oc_spmv_sym_mt.restype = None
         L.oc_spmv_sym_mt.argtypes = [i64, P, P, P, i32, P, P, i32]
+        L.oc_ofl_find_k1.restype = i32
+        L.oc_ofl_find_k1.argtypes = [i32, i32, i32, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.oc_conv2d.restype = None
+        L.oc_conv2d.argtypes = [P, P, i32, i32, P, i32, i32, i32, i32]
+        L.oc_ofl_vtable.restype = None
+        L.oc_ofl_vtable.argtypes = [i32, i32, i32, i32, P, i64, i64, P]
         L.oc_test_lz2.restype = i32
         L.oc_test_lz2.argtypes = [u64, u64, i32, i32]
         L.oc_two_level_basis.restype = i64
@@ -265,6 +271,36 @@ def two_level_build(n1, first, second, kt1, vt1, kt2, vt2, eps1=None, eps2=None,
     nnz2 = L.oc_two_level_build(*args, _p(rp), _p(col), _p(val), nnz, ctypes.byref(ntr))
     assert nnz2 == nnz
     return CRS(rp, col[:nnz], val[:nnz], False, ntr.value)
+
+
+# ------------------------------------------------ flux-lattice V from Bloch
+
+def ofl_find_k1(dim_x, dim_y, k2, k3, k4):
+    gx, gy = ctypes.c_int(), ctypes.c_int()
+    k1 = lib().oc_ofl_find_k1(dim_x, dim_y, k2, k3, k4, ctypes.byref(gx), ctypes.byref(gy))
+    return k1, gx.value, gy.value
+
+
+def conv2d(right, left, X, Y, top=0, left_off=0, bottom=0, right_off=0):
+    """ArrayConvolution2D on (X*Y) complex arrays; returns the (2X-1)x(2Y-1) output."""
+    r = np.ascontiguousarray(right, dtype=np.complex128)
+    lf = np.ascontiguousarray(left, dtype=np.complex128)
+    out = np.zeros((2 * X - 1) * (2 * Y - 1), dtype=np.complex128)
+    lib().oc_conv2d(_p(r), _p(lf), X, Y, _p(out), top, left_off, bottom, right_off)
+    return out.reshape(2 * X - 1, 2 * Y - 1)
+
+
+def ofl_vtable(dim_x, dim_y, cut_x, cut_y, bloch, e0=0, e1=None):
+    """GenerateQuarticTerms for folded-table entries [e0, e1); bloch is
+    complex [kMax][(2cx+1)(2cy+1)][2] (spin fastest)."""
+    K = dim_x * dim_y
+    n = K * K * (K + 1) // 2
+    e1 = n if e1 is None else e1
+    b = np.ascontiguousarray(bloch, dtype=np.complex128)
+    assert b.size == K * 2 * (2 * cut_x + 1) * (2 * cut_y + 1)
+    out = np.zeros(e1 - e0, dtype=np.complex128)
+    lib().oc_ofl_vtable(dim_x, dim_y, cut_x, cut_y, _p(b), e0, e1, _p(out))
+    return out
 
 
 def spmv_sym(crs, x, threads=1):

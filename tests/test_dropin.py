@@ -218,3 +218,34 @@ def test_dropin_two_level(oracle, tmp_path):
         kmax, v, k = fileio.read_term_table(str(out / f"quartic_coefficient_table_{tag}_L_{norb}.dat"), "sphere")
         assert kmax == L and np.array_equal(k, kt) and np.array_equal(v, vt)
     assert 0xFFFE in kt1 and 0xFFFF not in kt1
+
+
+@pytest.mark.gpu
+def test_dropin_flux_lattice_from_bloch(oracle, tmp_path):
+    """--bloch: the driver generates the quartic table from Bloch coefficients
+    (Generate4KTable + GenerateQuarticTerms + SetCoefficient(0.5 * strength))
+    and builds; the upper CRS matches the oracle's on the oracle's V table."""
+    dim_x, dim_y, cut, n, strength = 3, 4, 2, 4, 1.7
+    K = dim_x * dim_y
+    say("drop-in flux lattice from Bloch coefficients 3x4 N=4 cut 2")
+    rng = np.random.default_rng(11)
+    b = rng.normal(size=(K, (2 * cut + 1) ** 2, 2)) + 1j * rng.normal(size=(K, (2 * cut + 1) ** 2, 2))
+    b /= np.linalg.norm(b.reshape(K, -1), axis=1)[:, None, None]
+    b.astype(np.complex128).tofile(tmp_path / "bloch.bin")
+    eps = np.linspace(-0.5, 0.5, K).astype(np.complex128)
+    eps.tofile(tmp_path / "e.bin")
+    pre = str(tmp_path / "crs")
+    p = run(["--nbr-particles", n, "--kmax", K, "--layout", "ofl", "--complex", 1, "--dim-x", dim_x, "--dim-y", dim_y,
+             "--bloch", tmp_path / "bloch.bin", "--cutoff", cut, "--interaction-strength", strength,
+             "--eps", tmp_path / "e.bin", "--sector", f"k2d:1,2,{dim_x},{dim_y}", "--nbr-eigenvalues", 2,
+             "--dense-below", 10, "--dump-crs", pre])
+    r = last_json(p)
+    v = oracle.ofl_vtable(dim_x, dim_y, cut, cut, b)
+    c = 0.5 * strength  # complex product with (c, 0): (a c - b 0, a 0 + b c)
+    vt = (v.real * c - v.imag * 0.0) + 1j * (v.real * 0.0 + v.imag * c)
+    kt = hd.ofl_ktable(dim_x, dim_y)
+    states = oracle.basis(oracle.sector(oracle.SECTOR_K2D, 1, 2, dim_x, dim_y), n, K)
+    up = oracle.build(n, K, states, oracle.LAYOUT_OFL, K, kt, vt, eps=eps, is_complex=True, full=False)
+    assert r["dim"] == len(states)
+    assert np.array_equal(np.fromfile(pre + ".cols", dtype=np.int32), up.col)
+    assert np.array_equal(np.fromfile(pre + ".values", dtype=np.complex128), up.values())
