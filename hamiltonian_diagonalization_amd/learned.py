@@ -140,7 +140,7 @@ def sample_params(rng: np.random.Generator, n: int) -> np.ndarray:
 @dataclass
 class Dataset:
     states: np.ndarray   # uint64
-    params: np.ndarray   # float32 (n, 3)
+    params: np.ndarray   # float32 (n, n_params)
     targets: np.ndarray  # float32 (n, 3): log|psi| - max, sign, weight
     solve_ms: float
 
@@ -164,16 +164,18 @@ def ground_state_targets(psi: np.ndarray, floor: float = -30.0):
 
 
 def make_dataset(basis: FockBasis, two_s: int, points: np.ndarray, per_point: int, seed: int = 0,
-                 tol: float = 1e-8, log=print) -> Dataset:
+                 tol: float = 1e-8, log=print, hamiltonian=None) -> Dataset:
     """Plain-solver ground states at each parameter point; per point, half the
-    samples drawn by |psi|^2 and half uniformly over the sector."""
+    samples drawn by |psi|^2 and half uniformly over the sector.  The model is
+    the sphere at flux two_s with parameters (V1, V3, V5), or
+    hamiltonian(basis, p) when given."""
     rng = np.random.default_rng(seed)
     states_all = basis.states()
     S, P, T = [], [], []
     t_solve = 0.0
     start = None
     for i, p in enumerate(points):
-        h = sphere_hamiltonian(basis, two_s, p)
+        h = hamiltonian(basis, p) if hamiltonian else sphere_hamiltonian(basis, two_s, p)
         t0 = time.perf_counter()
         r = h.lanczos(nev=1, tol=tol, seed=i, start=start)
         t_solve += time.perf_counter() - t0
@@ -189,7 +191,7 @@ def make_dataset(basis: FockBasis, two_s: int, points: np.ndarray, per_point: in
         P.append(np.repeat(np.asarray(p, dtype=np.float32)[None, :], len(idx), axis=0))
         T.append(np.stack([a[idx], s[idx], np.ones(len(idx), dtype=np.float32)], axis=1))
         if log:
-            log(f"  point {i}: V=({p[0]:.3f},{p[1]:.3f},{p[2]:.3f}) E0={r.evals[0]:.8f} "
+            log(f"  point {i}: p=({','.join('%.3f' % x for x in p)}) E0={r.evals[0]:.8f} "
                 f"matvecs={r.matvecs}")
         del h
     return Dataset(np.concatenate(S), np.concatenate(P).astype(np.float32),
@@ -202,7 +204,7 @@ def train(net: AmplitudeNet, data: Dataset, steps: int = 2000, batch: int = 4096
     into fixed device buffers."""
     rng = np.random.default_rng(seed)
     n = len(data.states)
-    b_st, b_pr, b_tg = DeviceBuffer(batch * 8), DeviceBuffer(batch * 12), DeviceBuffer(batch * 12)
+    b_st, b_pr, b_tg = DeviceBuffer(batch * 8), DeviceBuffer(batch * 4 * net.n_params), DeviceBuffer(batch * 12)
     losses = []
     for it in range(steps):
         idx = rng.integers(0, n, size=batch)

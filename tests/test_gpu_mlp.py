@@ -121,3 +121,4 @@ def test_training_and_learned_start():
     say("drop-in matvecs random start", out[0]["matvecs"], "learned start", out[1]["matvecs"])
     assert abs(out[1]["eigenvalues"][0] - ref.evals[0]) < 1e-9
     assert out[1]["matvecs"] < out[0]["matvecs"]
+
