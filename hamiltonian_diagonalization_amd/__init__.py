@@ -33,7 +33,7 @@ HD_ERR_NOT_CONVERGED, HD_ERR_EMPTY = 5, 6
 SECTOR_NONE, SECTOR_LZ, SECTOR_K2D, SECTOR_K1D, SECTOR_LZ2 = 0, 1, 2, 3, 4
 LAYOUT_SPHERE, LAYOUT_OFL = 0, 1
 F64, F32, CODED, TILED = 0, 1, 2, 3
-STORAGE_FULL, STORAGE_UPPER_REF = 0, 1
+STORAGE_FULL, STORAGE_UPPER_REF, STORAGE_MATRIX_FREE = 0, 1, 2
 START_RANDOM, START_USER = 0, 1
 ZERO_TOL = 1e-14  # _SPARSE_ZERO_TOL_ (src/hamiltonians/hamiltonian_data.hpp:54)
 
@@ -482,7 +482,7 @@ class Hamiltonian:
     def __init__(self, basis: FockBasis, tables: TermTables, storage=STORAGE_FULL,
                  zero_tol=ZERO_TOL, values=None):
         if values is None:
-            values = (F64, F32, CODED) if storage == STORAGE_FULL else (F64, F32)
+            values = {STORAGE_FULL: (F64, F32, CODED), STORAGE_MATRIX_FREE: (F64,)}.get(storage, (F64, F32))
         mask = (1 if F64 in values else 0) | (2 if F32 in values else 0) | (4 if CODED in values else 0)
         h = ctypes.c_void_p()
         _check(lib().hd_ham_build(basis.h, tables.h, storage, zero_tol, mask, ctypes.byref(h)))

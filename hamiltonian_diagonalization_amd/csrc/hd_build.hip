@@ -407,6 +407,52 @@ k_fill(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A,
     }
 }
 
+HD_DEV double vmad(double acc, double v, double x) { return acc + v * x; }
+HD_DEV double2 vmad(double2 acc, double2 v, double2 x)
+{
+    return make_double2(acc.x + (v.x * x.x - v.y * x.y), acc.y + (v.x * x.y + v.y * x.x));
+}
+
+// Matrix-free product: y[row] = diag[row] x[row] + sum over the row's hops (the
+// full-storage rows k_fill writes, trimmed hops skipped) of sign * P * x[rank(out)].
+// One thread per row; the rank table and the pair table sit in LDS.
+template <bool CPLX, bool LZ>
+__global__ void __launch_bounds__(256)
+k_spmv_free(const uint64_t* __restrict__ states, int64_t dim, int K, uint64_t norb_mask,
+            const uint8_t* __restrict__ kpair, int kp_bytes, const int32_t* __restrict__ rank_blob, int rank_words,
+            const typename VT<CPLX>::T* __restrict__ P, const typename VT<CPLX>::T* __restrict__ diag,
+            const typename VT<CPLX>::T* __restrict__ x, typename VT<CPLX>::T* __restrict__ y)
+{
+    using T = typename VT<CPLX>::T;
+    extern __shared__ int32_t lds[];
+    block_copy_words(lds, rank_blob, rank_words);
+    uint8_t* s_kp = reinterpret_cast<uint8_t*>(lds + rank_words);
+    block_copy_words(lds + rank_words, reinterpret_cast<const int32_t*>(kpair), (kp_bytes + 3) / 4);
+    __syncthreads();
+    const RankView rv = rank_view(lds);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < dim; row += stride) {
+        const uint64_t s = states[row];
+        T acc = vmad(vzero<T>(), diag[row], x[row]);
+        for_each_hop<false>(s, K, norb_mask, s_kp, [&](int a, int b, int c, int d, int ti, uint64_t out, bool tr) {
+            if (tr) return;
+            uint64_t r;
+            if (!rank_state<LZ>(rv, out, r)) return;  // hops stay in the sector (sector_weight_delta_ok)
+            T v = P[ti];
+            if (hop_parity(s, a, b, c, d)) v = vneg(v);
+            acc = vmad(acc, v, x[r]);
+        });
+        y[row] = acc;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fold_diag(int64_t dim, T* __restrict__ diag, const uint8_t* __restrict__ keep)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < dim && !keep[i]) diag[i] = vzero<T>();
+}
+
 // Per-row sort by column: one wavefront per row, bitonic network in LDS over
 // CAP slots, then the permuted values are gathered from the unsorted copy and
 // written in the final precisions.
@@ -716,6 +762,26 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
     h->max_row = (int64_t)(int)host_misc[2];
     dfree(d_counts);
     d_counts = nullptr;
+    if (h->storage == HD_STORAGE_MATRIX_FREE) {
+        // keep the pair tables and the diagonal (0 where the CRS holds none)
+        hipLaunchKernelGGL(k_fold_diag<T>, dim3((unsigned)((dim + 255) / 256)), dim3(256), 0, st, dim, d_diag, d_dkeep);
+        HD_TRYHIP(hipGetLastError());
+        HD_TRYHIP(hipStreamSynchronize(st));
+        h->d_diag = d_diag;
+        h->d_mf_kpair = d_kpair;
+        h->d_mf_pval = d_pval;
+        h->mf_K = K;
+        h->mf_kp_bytes = kp_bytes;
+        h->mf_norb_mask = A.norb_mask;
+        d_diag = nullptr;
+        d_kpair = nullptr;
+        d_pval = nullptr;
+        dfree(h->d_rowptr);
+        h->d_rowptr = nullptr;
+        h->bytes = (size_t)dim * sizeof(T) + kp_alloc + D.pval.size() * 8;
+        cleanup();
+        return HD_OK;
+    }
     if (h->max_row > 4096) {
         cleanup();
         return fail(HD_ERR_UNSUPPORTED, "row of " + std::to_string(h->max_row) +
@@ -814,6 +880,39 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
 // least one table; the code of an entry indexes the tuple of its hop value
 // under all tables (0.0 where the hop is trimmed or absent), numbered in order
 // of first appearance across the sectors of the batch.
+int matrix_free_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t stream)
+{
+    const hd_basis* b = h->basis;
+    if (h->dim == 0) return HD_OK;
+    const int rank_words = (int)b->rt.blob.size();
+    const size_t kp_alloc = ((size_t)h->mf_kp_bytes + 3) & ~(size_t)3;
+    const size_t lds = (size_t)rank_words * 4 + kp_alloc;
+    int n_cu = 256;
+    HD_CHECK(device_props(&n_cu, nullptr));
+    const int per_cu = lds > 80 * 1024 ? 1 : (lds > 40 * 1024 ? 2 : 4);
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((h->dim + 255) / 256, (int64_t)n_cu * per_cu * 4));
+    const bool lz = b->rt.is_lz != 0;
+#define HD_MF(C, L)                                                                                             \
+    do {                                                                                                        \
+        using T = typename VT<C>::T;                                                                            \
+        HD_CHECK(set_lds((const void*)k_spmv_free<C, L>, lds));                                                 \
+        hipLaunchKernelGGL((k_spmv_free<C, L>), dim3(grid), dim3(256), lds, stream, b->d_states, h->dim,        \
+                           h->mf_K, h->mf_norb_mask, h->d_mf_kpair, h->mf_kp_bytes, b->d_rank, rank_words,      \
+                           static_cast<const T*>(h->d_mf_pval), static_cast<const T*>(h->d_diag),              \
+                           static_cast<const T*>(x), static_cast<T*>(y));                                      \
+    } while (0)
+    if (h->is_complex) {
+        if (lz) HD_MF(true, true);
+        else HD_MF(true, false);
+    } else {
+        if (lz) HD_MF(false, true);
+        else HD_MF(false, false);
+    }
+#undef HD_MF
+    HD_HIP(hipGetLastError());
+    return HD_OK;
+}
+
 int build_batch_sector(const hd_basis* b, const hd_tables* const* tables, int n, double tol, BatchTables& bt,
                        hd_hamiltonian** out)
 {
@@ -983,8 +1082,10 @@ int hd_ham_build(const hd_basis* b, const hd_tables* t, int32_t storage, double 
 {
     if (!b || !t || !out) return fail(HD_ERR_ARG, "hd_ham_build: null argument");
     *out = nullptr;
-    if (storage != HD_STORAGE_FULL && storage != HD_STORAGE_UPPER_REF)
+    if (storage != HD_STORAGE_FULL && storage != HD_STORAGE_UPPER_REF && storage != HD_STORAGE_MATRIX_FREE)
         return fail(HD_ERR_ARG, "unknown storage");
+    if (storage == HD_STORAGE_MATRIX_FREE && value_mask != 1)
+        return fail(HD_ERR_ARG, "matrix-free storage computes in FP64 only (value_mask 1)");
     if (value_mask < 1 || value_mask > 7) return fail(HD_ERR_ARG, "value_mask must be in 1..7");
     if ((value_mask & 4) && storage != HD_STORAGE_FULL)
         return fail(HD_ERR_ARG, "the coded value format needs full storage");
@@ -1029,6 +1130,8 @@ int hd_ham_destroy(hd_hamiltonian* h)
     dfree(h->d_code);
     dfree(h->d_diag);
     dfree(h->d_dict);
+    dfree(h->d_mf_kpair);
+    dfree(h->d_mf_pval);
     tiled_free(h);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -1081,6 +1184,7 @@ int hd_ham_copy_crs(const hd_hamiltonian* h, int64_t row0, int64_t row1, int32_t
 {
     if (!h || row0 < 0 || row1 > h->dim || row0 > row1 || !h_rowptr)
         return fail(HD_ERR_ARG, "hd_ham_copy_crs: bad row range");
+    if (!h->d_rowptr) return fail(HD_ERR_ARG, "hd_ham_copy_crs: matrix-free storage holds no CRS");
     HD_HIP(hipMemcpy(h_rowptr, h->d_rowptr + row0, (size_t)(row1 - row0 + 1) * 8, hipMemcpyDeviceToHost));
     const int64_t base = h_rowptr[0];
     const int64_t n = h_rowptr[row1 - row0] - base;

@@ -289,6 +289,12 @@ struct hd_hamiltonian {
     uint32_t* d_tg_offs = nullptr;
     double* d_tg_diag = nullptr;
     double t_tile = 0;
+    // matrix-free storage: pair table, pair values and the full-storage
+    // diagonal (d_diag); the basis handle supplies states and ranking
+    uint8_t* d_mf_kpair = nullptr;
+    void* d_mf_pval = nullptr;  // double or double2
+    int mf_K = 0, mf_kp_bytes = 0;
+    uint64_t mf_norb_mask = 0;
     double t_count = 0, t_scan = 0, t_fill = 0, t_sort = 0;
     size_t bytes = 0;
     hd_lanczos_state* lz = nullptr;  // eigenvectors of the last Lanczos call
@@ -310,6 +316,8 @@ int build_batch_sector(const hd_basis* b, const hd_tables* const* tables, int n,
                        hd_hamiltonian** out);
 int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y,
                 hipStream_t stream);
+// Matrix-free FP64 product of a HD_STORAGE_MATRIX_FREE handle (hd_build.hip).
+int matrix_free_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t stream);
 // Tiled coded layout (hd_tiled.hip).
 int tiled_build(hd_hamiltonian* h, int rows_per_group);
 void tiled_free(hd_hamiltonian* h);

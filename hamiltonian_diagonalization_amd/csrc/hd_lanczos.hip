@@ -622,10 +622,12 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
     L.nb = (int)std::max<int64_t>(1, std::min<int64_t>((h->dim + kVB - 1) / kVB, (int64_t)n_cu * 4));
     const bool mixed = o.precision == HD_F32;
     if (mixed && !h->d_val32) return fail(HD_ERR_ARG, "FP32 Lanczos needs FP32 values (value_mask & 2)");
-    if (!h->d_val64 && !h->d_code && !h->d_tg_base)
+    const bool free_ = h->storage == HD_STORAGE_MATRIX_FREE;
+    if (!free_ && !h->d_val64 && !h->d_code && !h->d_tg_base)
         return fail(HD_ERR_ARG, "Lanczos needs FP64, coded or tiled values for the FP64 operator");
     if (o.precision == HD_TILED && !h->d_tg_base) return fail(HD_ERR_ARG, "no tiled layout (hd_ham_tile)");
-    L.prec64 = o.precision == HD_TILED ? HD_TILED
+    L.prec64 = free_ ? HD_F64
+               : o.precision == HD_TILED ? HD_TILED
                : (o.precision == HD_CODED || !h->d_val64) ? (h->d_code ? HD_CODED : HD_TILED)
                                                           : HD_F64;
     HD_CHECK(L.alloc(mixed));
@@ -791,7 +793,10 @@ int hd_lanczos(hd_hamiltonian* h, const hd_lanczos_opts* opts, double* h_evals, 
 {
     if (!h || !opts || !h_evals) return fail(HD_ERR_ARG, "hd_lanczos: null argument");
     if (opts->nev <= 0) return fail(HD_ERR_ARG, "nev must be positive");
-    if (h->storage != HD_STORAGE_FULL) return fail(HD_ERR_ARG, "Lanczos needs full storage");
+    if (h->storage != HD_STORAGE_FULL && h->storage != HD_STORAGE_MATRIX_FREE)
+        return fail(HD_ERR_ARG, "Lanczos needs full or matrix-free storage");
+    if (h->storage == HD_STORAGE_MATRIX_FREE && (opts->precision != HD_F64 || opts->block > 1))
+        return fail(HD_ERR_ARG, "matrix-free storage runs the single-vector Lanczos in FP64 (precision HD_F64)");
     if (!h->lanes_per_row) HD_CHECK(spmv_autotune(h));
     if (opts->block > 1) return run_block_lanczos(h, *opts, h_evals, stats);
     return h->is_complex ? run_lanczos<true>(h, *opts, h_evals, stats)

@@ -234,6 +234,10 @@ static int pick_default_sched(const hd_hamiltonian* h)
 
 int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y, hipStream_t stream)
 {
+    if (h->storage == HD_STORAGE_MATRIX_FREE) {
+        if (precision != HD_F64) return fail(HD_ERR_ARG, "matrix-free storage computes the product in FP64 only");
+        return matrix_free_launch(h, x, y, stream);
+    }
     if (h->storage != HD_STORAGE_FULL)
         return fail(HD_ERR_ARG, "SpMV needs full (both-triangle) storage");
     if (precision < 0 || precision > 3) return fail(HD_ERR_ARG, "unknown SpMV precision");
@@ -280,6 +284,10 @@ __global__ void __launch_bounds__(256) k_stream_copy(const int4* __restrict__ sr
 
 int spmv_autotune(hd_hamiltonian* h)
 {
+    if (h->storage == HD_STORAGE_MATRIX_FREE) {  // one kernel, nothing to choose
+        h->lanes_per_row = 1;
+        return HD_OK;
+    }
     // one schedule per resident value format, each timed on zero vectors
     const size_t es = 8 * (h->is_complex ? 2 : 1);
     void *x = nullptr, *y = nullptr;

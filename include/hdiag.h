@@ -82,9 +82,15 @@ typedef enum { HD_F64 = 0, HD_F32 = 1, HD_CODED = 2, HD_TILED = 3 } hd_precision
 
 typedef enum {
     HD_STORAGE_FULL = 0,     /* both triangles, int64 row pointers (GPU SpMV layout) */
-    HD_STORAGE_UPPER_REF = 1 /* the reference's upper-triangle CRS: M[i][j] = <j|H|i>, j >= i
+    HD_STORAGE_UPPER_REF = 1, /* the reference's upper-triangle CRS: M[i][j] = <j|H|i>, j >= i
                                 (src/hamiltonians/spinless_fermion_hamiltonian.hpp:362,
                                  src/utilities/data_structures/crs_sparse_matrix.hpp:127-133) */
+    HD_STORAGE_MATRIX_FREE = 2 /* no CRS: the pair tables and the diagonal (dim values) stay
+                                  resident and every SpMV recomputes the full-storage rows
+                                  from the basis (FP64 only; the matrix-free product the
+                                  reference declares as MatrixVectorFunction::Recalculate,
+                                  src/hamiltonians/matrix_vector_routines.hpp:106-143, and
+                                  never calls), for sectors whose CRS does not fit in HBM */
 } hd_storage;
 
 typedef struct hd_basis hd_basis;
