@@ -61,7 +61,7 @@ class _HamInfo(ctypes.Structure):
 class _TileInfo(ctypes.Structure):
     _fields_ = [("groups", ctypes.c_int64), ("entries", ctypes.c_int64), ("lines", ctypes.c_int64),
                 ("chunks", ctypes.c_int64), ("rows_per_group", ctypes.c_int64), ("spmv_bytes", ctypes.c_int64),
-                ("build_ms", ctypes.c_double)]
+                ("build_ms", ctypes.c_double), ("entry_bytes", ctypes.c_int64)]
 
 
 class _LanczosOpts(ctypes.Structure):
@@ -70,7 +70,7 @@ class _LanczosOpts(ctypes.Structure):
                 ("start", ctypes.c_int32), ("seed", ctypes.c_uint64),
                 ("d_start", ctypes.c_void_p), ("precision", ctypes.c_int32),
                 ("f32_switch_tol", ctypes.c_double), ("want_vectors", ctypes.c_int32),
-                ("block", ctypes.c_int32)]
+                ("block", ctypes.c_int32), ("full_reorth", ctypes.c_int32)]
 
 
 class _LanczosStats(ctypes.Structure):
@@ -78,7 +78,8 @@ class _LanczosStats(ctypes.Structure):
                 ("restarts", ctypes.c_int32), ("matvecs", ctypes.c_int32),
                 ("matvecs_f32", ctypes.c_int32), ("matvecs_f64", ctypes.c_int32),
                 ("residual", ctypes.c_double * 64), ("time_total_ms", ctypes.c_double),
-                ("time_spmv_ms", ctypes.c_double), ("time_ortho_ms", ctypes.c_double)]
+                ("time_spmv_ms", ctypes.c_double), ("time_ortho_ms", ctypes.c_double),
+                ("reorth_steps", ctypes.c_int32)]
 
 
 class _BatchOpts(ctypes.Structure):
@@ -129,7 +130,7 @@ SIGNATURES = {
     "hd_ham_copy_crs": (ctypes.c_int, [_P, _I64, _I64, _I32, _P, _P, _P]),
     "hd_ham_device_crs": (ctypes.c_int, [_P, _PP, _PP, _PP, _PP]),
     "hd_ham_device_coded": (ctypes.c_int, [_P, _PP, _PP, _PP, ctypes.POINTER(_I32)]),
-    "hd_ham_tile": (ctypes.c_int, [_P, _I32]),
+    "hd_ham_tile": (ctypes.c_int, [_P, _I32, _I32]),
     "hd_ham_tile_info": (ctypes.c_int, [_P, ctypes.POINTER(_TileInfo)]),
     "hd_ham_device_tiled": (ctypes.c_int, [_P, _PP, _PP, _PP, _PP, _PP]),
     "hd_obs_density": (ctypes.c_int, [_P, _I32, _P, _P]),
@@ -469,11 +470,13 @@ class LanczosResult:
         self.time_ms = stats.time_total_ms
         self.time_spmv_ms = stats.time_spmv_ms
         self.time_ortho_ms = stats.time_ortho_ms
+        self.reorth_steps = stats.reorth_steps
 
     def __repr__(self):
         return (f"LanczosResult(evals={self.evals}, matvecs={self.matvecs} "
-                f"(f32 {self.matvecs_f32}), restarts={self.restarts}, "
-                f"max_resid={self.residuals.max():.2e}, {self.time_ms:.1f} ms)")
+                f"(f32 {self.matvecs_f32}), restarts={self.restarts}, reorth={self.reorth_steps}, "
+                f"max_resid={self.residuals.max():.2e}, {self.time_ms:.1f} ms, "
+                f"spmv {self.time_spmv_ms:.1f} ms, ortho {self.time_ortho_ms:.1f} ms)")
 
 
 class Hamiltonian:
@@ -529,10 +532,11 @@ class Hamiltonian:
         _check(lib().hd_ham_device_coded(self.h, *[ctypes.byref(x) for x in p], ctypes.byref(nd)))
         return tuple(x.value or 0 for x in p) + (nd.value,)
 
-    def tile(self, rows_per_group: int = 0) -> dict:
+    def tile(self, rows_per_group: int = 0, entry_bytes: int = 0) -> dict:
         """Builds the tiled layout of the coded operator (precision TILED);
-        rows_per_group 128 or 64, 0 = 128 with a fallback to 64."""
-        _check(lib().hd_ham_tile(self.h, rows_per_group))
+        rows_per_group 128 or 64, 0 = 128 with a fallback to 64; entry_bytes
+        4 (0) or 3 (packed; dictionaries of at most 2047 values)."""
+        _check(lib().hd_ham_tile(self.h, rows_per_group, entry_bytes))
         return self.tile_info()
 
     def tile_info(self) -> dict:
@@ -605,12 +609,15 @@ class Hamiltonian:
         return lanes.value, threads.value
 
     def lanczos(self, nev=5, tol=1e-10, ncv=0, precision=F64, seed=0, start=None,
-                f32_switch_tol=0.0, max_restarts=0, raise_on_fail=True, block=1) -> LanczosResult:
-        """block > 1 selects the block Lanczos (4, 8 or 16 vectors per SpMM)."""
+                f32_switch_tol=0.0, max_restarts=0, raise_on_fail=True, block=1,
+                full_reorth=False) -> LanczosResult:
+        """block > 1 selects the block Lanczos (4, 8 or 16 vectors per SpMM);
+        full_reorth orthogonalises every new vector against the whole basis
+        (default: partial reorthogonalisation)."""
         o = _LanczosOpts()
         o.nev, o.ncv, o.max_restarts, o.tol = nev, ncv, max_restarts, tol
         o.seed, o.precision, o.f32_switch_tol, o.want_vectors = seed, precision, f32_switch_tol, 1
-        o.block = block
+        o.block, o.full_reorth = block, int(bool(full_reorth))
         dstart = None
         if start is None:
             o.start = START_RANDOM

@@ -283,6 +283,7 @@ struct hd_hamiltonian {
     // tiled coded layout (hd_tiled.hip): row groups, x lines staged in LDS
     int64_t tg_groups = 0, tg_entries = 0, tg_lines = 0, tg_offs = 0;
     int tg_rows = 0;  // rows per group (128, or 64 for denser matrices)
+    int tg_entry_bytes = 4;  // 4, or 3 for packed entries (dictionary <= 2047 values)
     int64_t* d_tg_base = nullptr;  // entry, line and offset-table bases, 3 x (groups + 1)
     uint32_t* d_tg_entries = nullptr;
     uint32_t* d_tg_lines = nullptr;
@@ -319,7 +320,7 @@ int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y,
 // Matrix-free FP64 product of a HD_STORAGE_MATRIX_FREE handle (hd_build.hip).
 int matrix_free_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t stream);
 // Tiled coded layout (hd_tiled.hip).
-int tiled_build(hd_hamiltonian* h, int rows_per_group);
+int tiled_build(hd_hamiltonian* h, int rows_per_group, int entry_bytes);
 void tiled_free(hd_hamiltonian* h);
 int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s);
 size_t tiled_bytes(const hd_hamiltonian* h);

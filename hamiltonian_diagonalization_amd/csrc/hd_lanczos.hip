@@ -28,6 +28,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <numeric>
 
 #include "hd_internal.hpp"
@@ -225,6 +226,30 @@ k_axpy_dot(int64_t dim, const typename E<C>::T* __restrict__ p, double a, const 
         const T v = sub(w[r], scale(p[r], a));
         w[r] = v;
         s = add2(s, conj_mul(q[r], v));
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kVB / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = add2(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// w -= a * p; partial[block] = sum |w|^2 (the updated w, as .x).
+template <bool C>
+__global__ void __launch_bounds__(kVB)
+k_axpy_norm(int64_t dim, const typename E<C>::T* __restrict__ p, double a, typename E<C>::T* __restrict__ w,
+            typename E<C>::T* __restrict__ partial)
+{
+    using T = typename E<C>::T;
+    __shared__ T red[kVB];
+    T s = zero<T>();
+    const int64_t stride = (int64_t)gridDim.x * kVB;
+    for (int64_t r = (int64_t)blockIdx.x * kVB + threadIdx.x; r < dim; r += stride) {
+        const T v = sub(w[r], scale(p[r], a));
+        w[r] = v;
+        s = add2(s, conj_mul(v, v));
     }
     red[threadIdx.x] = s;
     __syncthreads();
@@ -540,7 +565,8 @@ struct Lanczos {
     // basis' own orthogonality error (which a single classical Gram-Schmidt
     // pass would amplify by alpha/beta per step):
     //   w -= beta_prev * v_prev;  a = <v_cur, w>;  w -= a * v_cur.
-    int local_step(const T* v_prev, double beta_prev, const T* v_cur, T* w, double* a)
+    // With wnorm2, the second pass also returns |w|^2 of the updated w.
+    int local_step(const T* v_prev, double beta_prev, const T* v_cur, T* w, double* a, double* wnorm2 = nullptr)
     {
         hipLaunchKernelGGL(k_axpy_dot<C>, dim3(nb), dim3(kVB), 0, st, n, v_prev, beta_prev, v_cur, w, partial);
         hipLaunchKernelGGL(k_reduce<C>, dim3(1), dim3(kVB), 0, st, partial, nb, 1, coef);
@@ -549,7 +575,16 @@ struct Lanczos {
         HD_HIP(hipMemcpyAsync(&v, coef, sizeof(T), hipMemcpyDeviceToHost, st));
         HD_HIP(hipStreamSynchronize(st));
         *a = host_re(v);
-        hipLaunchKernelGGL(k_axpy_dot<C>, dim3(nb), dim3(kVB), 0, st, n, v_cur, *a, v_cur, w, partial);
+        if (wnorm2) {
+            hipLaunchKernelGGL(k_axpy_norm<C>, dim3(nb), dim3(kVB), 0, st, n, v_cur, *a, w, partial);
+            hipLaunchKernelGGL(k_reduce<C>, dim3(1), dim3(kVB), 0, st, partial, nb, 1, coef);
+            HD_HIP(hipGetLastError());
+            HD_HIP(hipMemcpyAsync(&v, coef, sizeof(T), hipMemcpyDeviceToHost, st));
+            HD_HIP(hipStreamSynchronize(st));
+            *wnorm2 = host_re(v);
+        } else {
+            hipLaunchKernelGGL(k_axpy_dot<C>, dim3(nb), dim3(kVB), 0, st, n, v_cur, *a, v_cur, w, partial);
+        }
         HD_HIP(hipGetLastError());
         return HD_OK;
     }
@@ -661,20 +696,92 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
     bool converged = false;
     std::vector<double> res_est(m, 0.0);
     double anorm = 0.0;
+    // Partial reorthogonalisation (H. D. Simon, Math. Comp. 42 (1984) 115):
+    // om_cur[i] ~ |<v_j, v_i>| and om_prev[i] ~ |<v_{j-1}, v_i>| are propagated
+    // by the recurrence the basis obeys, A V = V T + rounding, with T the arrow
+    // of the kept Ritz pairs followed by the tridiagonal of the cycle; a new
+    // vector is orthogonalised against the whole basis only when an estimate
+    // exceeds sqrt(eps), and then the next one too (its recurrence still holds
+    // the old error).  The first vector of a cycle (the arrow) and the last
+    // (which becomes the next cycle's first) are always orthogonalised, so every
+    // cycle starts from an orthogonal pair.
+    const bool pro = o.full_reorth == 0;
+    const double eps = std::numeric_limits<double>::epsilon();
+    // the reorthogonalisation coefficients left out of the recurrence are of
+    // the size of the estimates; keeping them at 100 tol keeps the Ritz
+    // vectors' residuals below tol (semi-orthogonality, sqrt(eps), at most)
+    const double om_max = std::min(std::sqrt(eps), 100.0 * (o.tol > 0 ? o.tol : 1e-10));
+    std::vector<double> om_prev(m + 1, 0.0), om_cur(m + 1, 0.0), om_next(m + 1, 0.0);
+    bool force_next = false;
+    int reorth_steps = 0;
     for (;;) {
         for (int j = k; j < m; ++j) {
             HD_CHECK(L.apply(L.vec(j), L.W, prec));
             double b;
+            const double eps_op = prec == HD_F32 ? 6e-8 : eps;  // rounding of the operator
             if (j == k) {
                 // first vector of a cycle: its coupling to the kept Ritz
                 // vectors is the arrow, removed by two full CGS passes
                 HD_CHECK(L.orthogonalize(j + 1, L.W, c, &b, true));
                 alpha[j] = Lanczos<C>::host_re(c[j]);
+                ++reorth_steps;
+                std::fill(om_cur.begin(), om_cur.end(), eps);  // v_k: orthogonal to the kept vectors
+                om_cur[j] = 1.0;
+                std::fill(om_next.begin(), om_next.end(), eps);
+                force_next = false;
             } else {
-                double a;
-                HD_CHECK(L.local_step(L.vec(j - 1), beta[j - 1], L.vec(j), L.W, &a));
-                HD_CHECK(L.orthogonalize(j + 1, L.W, c, &b, false));
-                alpha[j] = a + Lanczos<C>::host_re(c[j]);
+                double a, wn2 = 0.0;
+                HD_CHECK(L.local_step(L.vec(j - 1), beta[j - 1], L.vec(j), L.W, &a, pro ? &wn2 : nullptr));
+                bool full = !pro || force_next || j == m - 1;
+                force_next = false;
+                if (!full) {
+                    const double bj = std::sqrt(std::max(wn2, 0.0));
+                    anorm = std::max(anorm, std::fabs(a) + bj);
+                    if (!(bj > 1e-12 * std::max(anorm, 1e-300))) {
+                        full = true;  // breakdown: handled below with the orthogonalised w
+                    } else {
+                        // om_next[i] ~ <v_{j+1}, v_i>, i < j, from
+                        // b_j v_{j+1} = A v_j - a_j v_j - b_{j-1} v_{j-1}
+                        double worst = 0.0;
+                        const double noise = 2.0 * eps_op * anorm / bj;
+                        // Real: Simon's signed recurrence.  Complex: the overlaps are
+                        // complex, so their magnitudes are propagated by the triangle
+                        // inequality (an upper bound).
+                        auto f = [](double x) { return C ? std::fabs(x) : x; };
+                        for (int i = 0; i < j; ++i) {
+                            double tv;  // (A v_i)^H v_j through the projected matrix, diagonal term apart
+                            double dii;
+                            if (i < k) {
+                                dii = theta[i];
+                                tv = f(arrow[i]) * om_cur[k];
+                            } else if (i == k) {
+                                dii = alpha[k];
+                                tv = f(beta[k]) * om_cur[k + 1];
+                                for (int l = 0; l < k; ++l) tv += f(arrow[l]) * om_cur[l];
+                            } else {
+                                dii = alpha[i];
+                                tv = f(beta[i - 1]) * om_cur[i - 1] + f(beta[i]) * om_cur[i + 1];
+                            }
+                            const double v = (tv + f(dii - a) * om_cur[i] + (C ? 1.0 : -1.0) * beta[j - 1] * om_prev[i]) / bj;
+                            om_next[i] = v + (v >= 0 ? noise : -noise);
+                            worst = std::max(worst, std::fabs(om_next[i]));
+                        }
+                        om_next[j] = eps_op * anorm / bj;
+                        if (worst > om_max) {
+                            full = true;
+                            force_next = true;
+                        } else {
+                            alpha[j] = a;
+                            b = bj;
+                        }
+                    }
+                }
+                if (full) {
+                    HD_CHECK(L.orthogonalize(j + 1, L.W, c, &b, false));
+                    alpha[j] = a + Lanczos<C>::host_re(c[j]);
+                    ++reorth_steps;
+                    std::fill(om_next.begin(), om_next.end(), eps);
+                }
             }
             anorm = std::max(anorm, std::fabs(alpha[j]) + b);
             if (b <= 1e-12 * std::max(anorm, 1e-300)) {
@@ -685,10 +792,15 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
                 HD_CHECK(L.orthogonalize(j + 1, L.W, cc, &bb));
                 HD_CHECK(L.scale_into(L.W, 1.0 / bb, L.vec(j + 1)));
                 beta[j] = 0.0;
+                std::fill(om_next.begin(), om_next.end(), eps);
+                force_next = true;
             } else {
                 beta[j] = b;
                 HD_CHECK(L.scale_into(L.W, 1.0 / b, L.vec(j + 1)));
             }
+            om_next[j + 1] = 1.0;
+            std::swap(om_prev, om_cur);
+            std::swap(om_cur, om_next);
         }
         // projected matrix: diag(theta) + arrow at row k, tridiagonal below
         Tm.assign((size_t)m * m, 0.0);
@@ -776,6 +888,7 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
         stats->matvecs_f64 = L.mv64;
         stats->time_spmv_ms = L.t_spmv;
         stats->time_ortho_ms = L.t_ortho;
+        stats->reorth_steps = reorth_steps;
         stats->time_total_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     }
@@ -799,8 +912,31 @@ int hd_lanczos(hd_hamiltonian* h, const hd_lanczos_opts* opts, double* h_evals, 
         return fail(HD_ERR_ARG, "matrix-free storage runs the single-vector Lanczos in FP64 (precision HD_F64)");
     if (!h->lanes_per_row) HD_CHECK(spmv_autotune(h));
     if (opts->block > 1) return run_block_lanczos(h, *opts, h_evals, stats);
-    return h->is_complex ? run_lanczos<true>(h, *opts, h_evals, stats)
-                         : run_lanczos<false>(h, *opts, h_evals, stats);
+    auto run = [&](const hd_lanczos_opts& o, hd_lanczos_stats* s) {
+        return h->is_complex ? run_lanczos<true>(h, o, h_evals, s) : run_lanczos<false>(h, o, h_evals, s);
+    };
+    hd_lanczos_stats s1{};
+    const int rc = run(*opts, &s1);
+    if (rc != HD_OK || s1.converged || opts->full_reorth) {
+        if (stats) *stats = s1;
+        return rc;
+    }
+    // partial reorthogonalisation met the Ritz estimates but an explicit
+    // residual missed tol: solve again with full reorthogonalisation
+    hd_lanczos_opts o2 = *opts;
+    o2.full_reorth = 1;
+    hd_lanczos_stats s2{};
+    const int rc2 = run(o2, &s2);
+    s2.matvecs += s1.matvecs;
+    s2.matvecs_f32 += s1.matvecs_f32;
+    s2.matvecs_f64 += s1.matvecs_f64;
+    s2.restarts += s1.restarts;
+    s2.reorth_steps += s1.reorth_steps;
+    s2.time_total_ms += s1.time_total_ms;
+    s2.time_spmv_ms += s1.time_spmv_ms;
+    s2.time_ortho_ms += s1.time_ortho_ms;
+    if (stats) *stats = s2;
+    return rc2;
 }
 
 int hd_lanczos_vectors(const hd_hamiltonian* h, int32_t i, void* d_out, void* h_out)

@@ -13,7 +13,9 @@
 //              entries read, split into chunks of 256 lines (32 KiB of LDS)
 //   entry    = uint32: bits 24-31 line slot within the chunk, 20-23 word
 //              within the line, 0-19 value code (bit 19 sign, bits 0-18
-//              dictionary index into the coded format's FP64 dictionary)
+//              dictionary index into the coded format's FP64 dictionary);
+//              with at most 2047 dictionary values the entries are packed
+//              into 3 bytes (slot 16-23, word 12-15, sign 11, index 0-10)
 //   chunks   = 256 consecutive lines; per chunk the 64 row starts and the
 //              chunk end (the entries of (chunk c, row r) are
 //              contiguous and in the row's column order; each chunk's block
@@ -24,7 +26,7 @@
 // The SpMV block stages each chunk's lines into LDS with coalesced 16-byte
 // loads (the next chunk's lines are in flight while the current one is
 // processed), then every lane walks its row's entries of that chunk reading x
-// and the dictionary from LDS.  HBM carries 4 bytes per entry plus the line lists
+// and the dictionary from LDS.  HBM carries 3 or 4 bytes per entry plus the line lists
 // and offsets; the operator is the coded format's exact FP64 matrix.
 //
 // The layout is built on the device from the coded CRS, one block per group:
@@ -296,7 +298,7 @@ k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __re
 // of chunk c+1 are already in flight into registers (double buffering through
 // the register file); only rows with more than kTPR * kU entries in a chunk
 // issue further loads inside it.
-template <int TG>
+template <int TG, bool PK>
 __global__ void __launch_bounds__(kSpmvThreads)
 k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, const uint32_t* __restrict__ entries,
              const uint32_t* __restrict__ lines, const uint32_t* __restrict__ meta, const double* __restrict__ tdiag,
@@ -317,7 +319,15 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
     const int64_t mb = base[2 * (groups + 1) + g], me = base[2 * (groups + 1) + g + 1];
     const int nmeta = (int)min(me - mb, (int64_t)kMeta);
     const int nch = (nmeta - 1) / (TG + 2);
-    const uint32_t* ent = entries + base[g];
+    // packed: 3 bytes per entry; a group starts at a multiple of 4 entries,
+    // i.e. at dword 3 * base[g] / 4
+    const uint32_t* ent = entries + (PK ? base[g] / 4 * 3 : base[g]);
+    auto ent_at = [&](uint32_t j) -> uint32_t {
+        if (!PK) return ent[j];
+        const uint32_t b = 3u * j, k = b >> 2, sh = (b & 3u) * 8u;
+        const uint64_t w = (uint64_t)ent[k] | ((uint64_t)ent[k + 1] << 32);
+        return (uint32_t)(w >> sh) & 0xFFFFFFu;
+    };
     const int r = threadIdx.x / kTPR, sub = threadIdx.x % kTPR;
     const int64_t row = g * TG + r;
     const int nl = (int)min(le - lb, (int64_t)kMaxLines);
@@ -344,16 +354,22 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const uint32_t j = j0 + u * kTPR;
-            e[u] = j < je ? ent[j] : kEmpty;
+            e[u] = j < je ? ent_at(j) : kEmpty;
         }
     };
     auto consume = [&](const uint32_t (&b)[kU], double& a) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (b[u] == kEmpty) continue;
-            double v = sdict[b[u] & 0x7FFFFu];
-            if (b[u] & 0x80000u) v = -v;
-            a = fma(v, xs[(b[u] >> 24) * kLineWords + ((b[u] >> 20) & 15u)], a);
+            if (PK) {  // slot 16-23, word 12-15, sign 11, index 0-10
+                double v = sdict[b[u] & 0x7FFu];
+                if (b[u] & 0x800u) v = -v;
+                a = fma(v, xs[(b[u] >> 16) * kLineWords + ((b[u] >> 12) & 15u)], a);
+            } else {
+                double v = sdict[b[u] & 0x7FFFFu];
+                if (b[u] & 0x80000u) v = -v;
+                a = fma(v, xs[(b[u] >> 24) * kLineWords + ((b[u] >> 20) & 15u)], a);
+            }
         }
     };
     if (nch > 0) fetch(0);
@@ -377,7 +393,7 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
                 const uint32_t j = j0 + u * kTPR;
-                cur[u] = j < je ? ent[j] : kEmpty;
+                cur[u] = j < je ? ent_at(j) : kEmpty;
             }
             consume(cur, acc);
         }
@@ -387,12 +403,31 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
     if (row < dim && sub == 0) y[row] = fma(tdiag[row], x[row], acc);
 }
 
+// Packs 4-byte entries (slot 24-31, word 20-23, sign 19, index 0-18) into
+// 3-byte entries (slot 16-23, word 12-15, sign 11, index 0-10, little endian):
+// four entries per thread, 16 bytes in, 12 bytes out.  The index fits 11 bits
+// because the dictionary holds at most 2047 values.
+__global__ void __launch_bounds__(256) k_tile_pack(const uint4* __restrict__ in, int64_t n4, uint32_t* __restrict__ out)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        const uint4 q = in[i];
+        uint32_t p[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            p[k] = ((p[k] >> 24) << 16) | (((p[k] >> 20) & 15u) << 12) | (((p[k] >> 19) & 1u) << 11) | (p[k] & 0x7FFu);
+        out[3 * i] = p[0] | (p[1] << 24);
+        out[3 * i + 1] = (p[1] >> 8) | (p[2] << 16);
+        out[3 * i + 2] = (p[2] >> 16) | (p[3] << 8);
+    }
+}
+
 }  // namespace
 
 // Builds the layout with TG-row groups; HD_ERR_UNSUPPORTED when a group reads
 // more than kMaxLines distinct lines.
 template <int TG>
-static int tiled_build_tg(hd_hamiltonian* h)
+static int tiled_build_tg(hd_hamiltonian* h, int entry_bytes)
 {
     const hipStream_t st = h->stream;
     const int64_t G = (h->dim + TG - 1) / TG;
@@ -461,6 +496,24 @@ static int tiled_build_tg(hd_hamiltonian* h)
             rc = fail(HD_ERR_HIP, "tiled layout build found inconsistent counts (flags " + std::to_string(flags[1]) +
                                       ")");
     }
+    if (rc == HD_OK && entry_bytes == 3 && h->tg_entries > 0) {
+        // entries are a multiple of 4; two zero pad words let the SpMV read
+        // the dword after any entry
+        uint32_t* packed = nullptr;
+        const int64_t n4 = h->tg_entries / 4;
+        rc = dalloc_t(&packed, (size_t)(3 * n4 + 2), "packed tile entries");
+        if (rc == HD_OK) {
+            HD_ENQ(q, hipMemsetAsync(packed + 3 * n4, 0, 2 * sizeof(uint32_t), st));
+            const int grid = (int)std::min<int64_t>((n4 + 255) / 256, 65536);
+            hipLaunchKernelGGL(k_tile_pack, dim3((unsigned)grid), dim3(256), 0, st,
+                               reinterpret_cast<const uint4*>(h->d_tg_entries), n4, packed);
+            HD_ENQ(q, hipGetLastError());
+            HD_ENQ(q, hipStreamSynchronize(st));
+            rc = q.status();
+            dfree(h->d_tg_entries);
+            h->d_tg_entries = packed;
+        }
+    }
     h->t_tile = tm.stop(st);
     if (rc == HD_OK) rc = tm.status("tiled layout timing");
     dfree(d_nl);
@@ -472,15 +525,24 @@ static int tiled_build_tg(hd_hamiltonian* h)
         return rc;
     }
     h->tg_rows = TG;
-    h->bytes += (size_t)h->tg_entries * 4 + (size_t)h->tg_lines * 4 + (size_t)h->tg_offs * 4 + base.size() * 8 +
+    h->tg_entry_bytes = entry_bytes;
+    h->bytes += (size_t)h->tg_entries * entry_bytes + (size_t)h->tg_lines * 4 + (size_t)h->tg_offs * 4 + base.size() * 8 +
                 (size_t)h->dim * 8;
     return HD_OK;
 }
 
-int tiled_build(hd_hamiltonian* h, int rows_per_group)
+int tiled_build(hd_hamiltonian* h, int rows_per_group, int entry_bytes)
 {
     if (rows_per_group != 0 && rows_per_group != 64 && rows_per_group != 128)
         return fail(HD_ERR_ARG, "rows per group must be 0 (automatic), 64 or 128");
+    if (entry_bytes != 0 && entry_bytes != 3 && entry_bytes != 4)
+        return fail(HD_ERR_ARG, "entry bytes must be 0 (automatic), 3 or 4");
+    if (entry_bytes == 3 && h->ndict > 2047)
+        return fail(HD_ERR_UNSUPPORTED, "3-byte entries need a dictionary of at most 2047 values");
+    // 4 by default: the packed entries cut HBM bytes by a quarter but need two
+    // dword loads and a shift each, and measured slower (N=12, 2S=33: 4.86 ms
+    // against 3.53 ms per product)
+    if (entry_bytes == 0) entry_bytes = 4;
     if (!h->d_code) return fail(HD_ERR_ARG, "the tiled layout is built from the coded values (value_mask & 4)");
     if (h->is_complex) return fail(HD_ERR_UNSUPPORTED, "the tiled layout is implemented for real matrices");
     if (h->storage != HD_STORAGE_FULL) return fail(HD_ERR_ARG, "the tiled layout needs full storage");
@@ -491,11 +553,11 @@ int tiled_build(hd_hamiltonian* h, int rows_per_group)
         return fail(HD_ERR_UNSUPPORTED, "dictionary too large for LDS next to the line buffer");
     tiled_free(h);
     if (h->dim == 0) return HD_OK;
-    if (rows_per_group == 64) return tiled_build_tg<64>(h);
+    if (rows_per_group == 64) return tiled_build_tg<64>(h, entry_bytes);
     // 128-row groups stage fewer lines per entry; denser matrices fall back to 64
-    const int rc = tiled_build_tg<128>(h);
+    const int rc = tiled_build_tg<128>(h, entry_bytes);
     if (rc != HD_ERR_UNSUPPORTED || rows_per_group == 128) return rc;
-    return tiled_build_tg<64>(h);
+    return tiled_build_tg<64>(h, entry_bytes);
 }
 
 void tiled_free(hd_hamiltonian* h)
@@ -512,6 +574,7 @@ void tiled_free(hd_hamiltonian* h)
     h->d_tg_diag = nullptr;
     h->tg_groups = h->tg_entries = h->tg_lines = h->tg_offs = 0;
     h->tg_rows = 0;
+    h->tg_entry_bytes = 4;
 }
 
 int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
@@ -520,16 +583,20 @@ int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
     if (h->dim == 0) return HD_OK;
     const size_t lds = sizeof(double) * (size_t)std::max(h->ndict, 1);
     const int aligned = ((uintptr_t)x & 15) == 0 ? 1 : 0;
-    if (h->tg_rows == 128)
-        hipLaunchKernelGGL(k_spmv_tiled<128>, dim3((unsigned)h->tg_groups), dim3(kSpmvThreads), lds, s, h->dim,
-                           h->tg_groups, h->d_tg_base, h->d_tg_entries, h->d_tg_lines, h->d_tg_offs, h->d_tg_diag,
-                           static_cast<const double*>(h->d_dict), h->ndict, aligned, static_cast<const double*>(x),
-                           static_cast<double*>(y));
-    else
-        hipLaunchKernelGGL(k_spmv_tiled<64>, dim3((unsigned)h->tg_groups), dim3(kSpmvThreads), lds, s, h->dim,
-                           h->tg_groups, h->d_tg_base, h->d_tg_entries, h->d_tg_lines, h->d_tg_offs, h->d_tg_diag,
-                           static_cast<const double*>(h->d_dict), h->ndict, aligned, static_cast<const double*>(x),
-                           static_cast<double*>(y));
+#define HD_TL(TG, PK)                                                                                            \
+    hipLaunchKernelGGL((k_spmv_tiled<TG, PK>), dim3((unsigned)h->tg_groups), dim3(kSpmvThreads), lds, s, h->dim, \
+                       h->tg_groups, h->d_tg_base, h->d_tg_entries, h->d_tg_lines, h->d_tg_offs, h->d_tg_diag,    \
+                       static_cast<const double*>(h->d_dict), h->ndict, aligned, static_cast<const double*>(x),  \
+                       static_cast<double*>(y))
+    const bool pk = h->tg_entry_bytes == 3;
+    if (h->tg_rows == 128) {
+        if (pk) HD_TL(128, true);
+        else HD_TL(128, false);
+    } else {
+        if (pk) HD_TL(64, true);
+        else HD_TL(64, false);
+    }
+#undef HD_TL
     HD_HIP(hipGetLastError());
     return HD_OK;
 }
@@ -538,7 +605,7 @@ int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
 // bases, the dictionary, the diagonal, x once and y.
 size_t tiled_bytes(const hd_hamiltonian* h)
 {
-    return (size_t)h->tg_entries * 4 + (size_t)h->tg_offs * 4 + (size_t)h->tg_lines * 4 +
+    return (size_t)h->tg_entries * h->tg_entry_bytes + (size_t)h->tg_offs * 4 + (size_t)h->tg_lines * 4 +
            (size_t)(3 * (h->tg_groups + 1)) * 8 + (size_t)h->ndict * 8 + (size_t)h->dim * 8 * 3;
 }
 
@@ -546,10 +613,10 @@ size_t tiled_bytes(const hd_hamiltonian* h)
 
 using namespace hd;
 
-extern "C" int hd_ham_tile(hd_hamiltonian* h, int32_t rows_per_group)
+extern "C" int hd_ham_tile(hd_hamiltonian* h, int32_t rows_per_group, int32_t entry_bytes)
 {
     if (!h) return fail(HD_ERR_ARG, "hd_ham_tile: null handle");
-    return tiled_build(h, rows_per_group);
+    return tiled_build(h, rows_per_group, entry_bytes);
 }
 
 extern "C" int hd_ham_tile_info(const hd_hamiltonian* h, hd_tile_info_t* info)
@@ -563,6 +630,7 @@ extern "C" int hd_ham_tile_info(const hd_hamiltonian* h, hd_tile_info_t* info)
     info->rows_per_group = h->tg_rows;
     info->spmv_bytes = (int64_t)tiled_bytes(h);
     info->build_ms = h->t_tile;
+    info->entry_bytes = h->tg_entry_bytes;
     return HD_OK;
 }
 

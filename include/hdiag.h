@@ -76,7 +76,7 @@ typedef enum {
  *             separate FP64 array): the exact FP64 operator in 6 bytes per
  *             entry; FP64 vectors and accumulation.
  *   HD_TILED  the coded operator re-laid out in row groups whose x lines
- *             are staged through LDS (hd_ham_tile): 4 bytes per entry, the
+ *             are staged through LDS (hd_ham_tile): 3 or 4 bytes per entry, the
  *             same exact FP64 matrix; real matrices. */
 typedef enum { HD_F64 = 0, HD_F32 = 1, HD_CODED = 2, HD_TILED = 3 } hd_precision;
 
@@ -298,22 +298,27 @@ int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes_per_row);
  * precision HD_TILED is available to hd_spmv and hd_lanczos.  Fails with
  * HD_ERR_UNSUPPORTED for complex matrices or when a group reads more than
  * 5120 distinct 128-byte lines of x.  rows_per_group: 128 or 64, or 0 for
- * 128 with a fallback to 64. */
-int hd_ham_tile(hd_hamiltonian* h, int32_t rows_per_group);
+ * 128 with a fallback to 64.  entry_bytes: 4 (line slot, word, sign and a
+ * 19-bit dictionary index), 3 (the same with an 11-bit index, for
+ * dictionaries of at most 2047 values), or 0 for 4. */
+int hd_ham_tile(hd_hamiltonian* h, int32_t rows_per_group, int32_t entry_bytes);
 typedef struct {
     int64_t groups;         /* row groups */
-    int64_t entries;        /* stored entries (4 bytes each): the off-diagonal
+    int64_t entries;        /* stored entries (entry_bytes each): the off-diagonal
                                entries plus at most 3 padding entries per chunk */
     int64_t lines;          /* staged x lines, summed over groups */
     int64_t chunks;         /* chunks of 256 lines */
     int64_t rows_per_group; /* 128, or 64 when a 128-row group reads too many lines */
     int64_t spmv_bytes;     /* algorithmic HBM bytes of one product */
     double build_ms;
+    int64_t entry_bytes;    /* 3 or 4 */
 } hd_tile_info_t;
 int hd_ham_tile_info(const hd_hamiltonian* h, hd_tile_info_t* info);
 /* Zero-copy view of the tiled layout: group bases (3 x (groups+1) int64:
- * entries, lines, offset table), entries, line lists, offset tables and the
- * FP64 diagonal.  Owned by the handle. */
+ * entries, lines, offset table), entries (uint32 words, or with entry_bytes 3
+ * the packed little-endian 3-byte entries: slot bits 16-23, word 12-15, sign
+ * 11, index 0-10), line lists, offset tables and the FP64 diagonal.  Owned by
+ * the handle. */
 int hd_ham_device_tiled(const hd_hamiltonian* h, const int64_t** d_base, const uint32_t** d_entries,
                         const uint32_t** d_lines, const uint32_t** d_offsets, const double** d_diag);
 /* The SpMV schedule of one value format: lanes per row and threads per block
@@ -349,6 +354,9 @@ typedef struct {
     int32_t block;        /* 0 or 1: single-vector Lanczos.  4, 8 or 16: block Lanczos with
                              that many vectors per SpMM (real matrices, FP64 or coded operator;
                              ncv then counts columns, rounded up to whole blocks) */
+    int32_t full_reorth;  /* 0: partial reorthogonalisation (orthogonality estimated by the
+                             omega recurrence, the basis kept semi-orthogonal, sqrt(eps));
+                             1: every new vector against the whole basis */
 } hd_lanczos_opts;
 
 typedef struct {
@@ -362,6 +370,7 @@ typedef struct {
     double time_total_ms;
     double time_spmv_ms;
     double time_ortho_ms;
+    int32_t reorth_steps; /* steps that orthogonalised against the whole basis */
 } hd_lanczos_stats;
 
 /* HamiltonianBase::LanczosDiagonalize -> ArpackWrapper::DiagonalizeSymmetricMatrix

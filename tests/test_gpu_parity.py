@@ -211,21 +211,23 @@ def test_spmv_complex_large_dictionary(oracle):
         assert np.array_equal(h.spmv(e, precision=hd.CODED), h.spmv(e))
 
 
-@pytest.mark.parametrize("rows", [64, 128])
+@pytest.mark.parametrize("rows,nbytes", [(64, 4), (128, 0), (64, 3), (128, 3)])
 @pytest.mark.parametrize("n,two_s,lz", [(4, 11, 0), (5, 13, 3), (6, 15, 0), (6, 15, 24)])
-def test_tiled_columns_exact(oracle, n, two_s, lz, rows):
+def test_tiled_columns_exact(oracle, n, two_s, lz, rows, nbytes):
     """Every column H e_i through the tiled layout equals the oracle's matrix
     column bit for bit (one nonzero term per output), so the layout holds
-    exactly the coded operator's entries, signs and diagonal."""
-    say("tiled columns", n, two_s, lz, rows)
+    exactly the coded operator's entries, signs and diagonal, with 4-byte and
+    packed 3-byte entries."""
+    say("tiled columns", n, two_s, lz, rows, nbytes)
     vm = models.coulomb_list(two_s, {1: 1.0, 3: 0.37, 5: 0.11})
     b = hd.FockBasis(hd.Sector.lz(lz, two_s), n, two_s + 1)
     t = models.sphere_tables(two_s, vm)
     K = two_s + 1
     full = oracle.build(n, K, b.states(), oracle.LAYOUT_SPHERE, K, t.ktable, t.vtable, full=True)
     h = hd.Hamiltonian(b, t, values=(hd.CODED,))
-    ti = h.tile(rows)
+    ti = h.tile(rows, nbytes)
     assert ti["rows_per_group"] == rows and ti["groups"] == (b.dim + rows - 1) // rows
+    assert ti["entry_bytes"] == (nbytes or 4)
     offdiag = len(full.col) - int(np.sum(full.col == np.repeat(np.arange(b.dim), np.diff(full.rowptr))))
     assert offdiag <= ti["entries"] <= offdiag + 3 * ti["chunks"] and ti["entries"] % 4 == 0
     M = full.to_dense(b.dim, mirror=False)
@@ -292,6 +294,42 @@ def test_lanczos_config1(oracle):
     say(rc)
     assert rc.converged and np.all(rc.residuals <= 1e-12)
     assert np.max(np.abs(rc.evals - want) / np.maximum(1, np.abs(want))) < 1e-10
+
+
+@pytest.mark.parametrize("case", ["sphere", "flux_lattice"])
+def test_partial_vs_full_reorthogonalisation(oracle, case):
+    """Partial reorthogonalisation (the default) reaches the same eigenpairs as
+    full reorthogonalisation, with explicit FP64 residuals below tolerance and
+    fewer orthogonalisation passes than steps; no spurious copies of converged
+    eigenvalues appear (the eigenvalues equal the dense ones, which are
+    non-degenerate here)."""
+    if case == "sphere":
+        b, t = models.sphere_problem(8, 21)
+        full = oracle.build(8, 22, b.states(), oracle.LAYOUT_SPHERE, 22, t.ktable, t.vtable, full=True)
+        nev = 5
+    else:
+        kt, vt, eps = models.ofl_tables(3, 4, 11)
+        b = hd.FockBasis(hd.Sector.k2d(0, 0, 3, 4), 5, 12)
+        t = hd.TermTables(hd.LAYOUT_OFL, 12, kt, vt, eps)
+        full = oracle.build(5, 12, b.states(), oracle.LAYOUT_OFL, 12, kt, vt, eps=eps, is_complex=True, full=True)
+        nev = 6
+    say("partial vs full reorthogonalisation", case)
+    want = _dense_eigs(oracle, full, b.dim, nev)
+    h = hd.Hamiltonian(b, t)
+    for prec in (hd.F64, hd.F32):
+        tol = 1e-12 if prec == hd.F64 else 1e-10
+        rp = h.lanczos(nev=nev, tol=tol, precision=prec, seed=0)
+        rf = h.lanczos(nev=nev, tol=tol, precision=prec, seed=0, full_reorth=True)
+        say(case, prec, "partial", rp)
+        say(case, prec, "full", rf)
+        for r in (rp, rf):
+            assert r.converged and np.all(r.residuals <= tol)
+            assert np.max(np.abs(r.evals - want) / np.maximum(1, np.abs(want))) < 1e-10
+        # real matrices: Simon's signed estimate saves most passes; complex ones
+        # propagate a magnitude bound, which may ask for a pass at every step
+        assert rp.reorth_steps <= rf.reorth_steps
+        if prec == hd.F64 and case == "sphere":
+            assert rp.reorth_steps < rf.reorth_steps / 2
 
 
 def test_lanczos_laughlin_zero_mode():
