@@ -108,8 +108,10 @@ def measured_traffic(kernel, n, two_s):
         return None
     for line in open(path):
         f = line.split()
-        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1].replace(",true>", ">") == kernel:
-            return int(f[2])
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch":
+            name = f[1].replace(",true>", ">") if f[1].startswith("k_spmv_coded") else f[1]
+            if name == kernel:
+                return int(f[2])
     return None
 
 
@@ -248,7 +250,7 @@ def main():
     achieved = sp_b[key] / (spmv_live_ms * 1e-3) / 1e9
     lanes, spmv_threads = ham.schedule(prec)
     kernel = {"coded": f"k_spmv_coded<double,{lanes},{spmv_threads}>", "f64": f"k_spmv<double,{lanes},{spmv_threads}>",
-              "tiled": f"k_spmv_tiled<{tile['rows_per_group']}>"}[key]
+              "tiled": f"k_spmv_tiled<{tile['rows_per_group']},{'true' if tile['entry_bytes'] == 3 else 'false'}>"}[key]
     traffic = measured_traffic(kernel, n, two_s)
     out = {
         "metric": METRIC,

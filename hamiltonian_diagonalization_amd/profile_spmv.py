@@ -3,7 +3,8 @@
 python -m hamiltonian_diagonalization_amd.profile_spmv [--n 12 --two-s 33 --iters 3]
 
 Builds the N=12, 2S=33 L_z=0 Coulomb sphere Hamiltonian with FP64, FP32 and
-coded values and the tiled layout, then launches each SpMV format `iters` times
+coded values and the tiled layout (4-byte entries, then the packed 3-byte
+ones), then launches each SpMV format `iters` times
 (after one warm-up launch) and the plain streaming copy kernel over 1 GiB `iters` times, which
 calibrates FETCH_SIZE / WRITE_SIZE against a known byte count.  Prints the
 algorithmic bytes of every launch so that the counter totals can be compared.
@@ -33,6 +34,17 @@ def main():
         ms = ham.spmv_time_ms(p, args.iters)
         out["spmv"][name] = {"ms": ms, "alg_bytes": ham.spmv_bytes(p)}
         print(f"[profile] spmv {name}: {ms:.3f} ms, {ham.spmv_bytes(p)} algorithmic bytes", flush=True)
+    # the packed 3-byte entries (k_spmv_tiled<128, true>) when the dictionary allows them
+    if tile["entry_bytes"] == 4:
+        try:
+            t3 = ham.tile(0, 3)
+        except Exception as e:  # dictionary above 2047 values
+            print(f"[profile] packed entries not available: {e}", flush=True)
+        else:
+            ms = ham.spmv_time_ms(TILED, args.iters)
+            out["spmv"]["tiled3"] = {"ms": ms, "alg_bytes": t3["spmv_bytes"]}
+            print(f"[profile] spmv tiled (3-byte entries): {ms:.3f} ms, {t3['spmv_bytes']} algorithmic bytes",
+                  flush=True)
     src, dst = DeviceBuffer(args.copy_bytes), DeviceBuffer(args.copy_bytes)
     v = ctypes.c_double()
     _check(lib().hd_stream_copy(src.ptr, dst.ptr, args.copy_bytes, args.iters, ctypes.byref(v)))
