@@ -53,6 +53,34 @@ def test_basis_config2(oracle):
     assert np.array_equal(b.states(), want)
 
 
+def test_build_config2_rows_and_symmetry(oracle):
+    """N=12, 2S=33 (dim 7,764,392, 2.59e9 full entries): three row blocks of
+    the upper (reference) and full CRS bit-exact against the oracle, and the
+    size-independent check that the full operator is symmetric,
+    y.(Hx) = x.(Hy), through the device product."""
+    say("build N=12 2S=33: sampled rows bit-exact, symmetry")
+    b, t = models.sphere_problem(12, 33)
+    bas = b.states()
+    dim = b.dim
+    blocks = [(0, 1500), (dim // 2, dim // 2 + 1500), (dim - 1500, dim)]
+    up = hd.Hamiltonian(b, t, storage=hd.STORAGE_UPPER_REF, values=(hd.F64,))
+    for r0, r1 in blocks:
+        ref = oracle.build(12, 34, bas, oracle.LAYOUT_SPHERE, 34, t.ktable, t.vtable, full=False, row0=r0, row1=r1)
+        _assert_crs_equal(up, ref, r0, r1)
+    nnz_up = up.info()["nnz"]
+    del up
+    full = hd.Hamiltonian(b, t, storage=hd.STORAGE_FULL, values=(hd.F64,))
+    for r0, r1 in blocks:
+        ref = oracle.build(12, 34, bas, oracle.LAYOUT_SPHERE, 34, t.ktable, t.vtable, full=True, row0=r0, row1=r1)
+        _assert_crs_equal(full, ref, r0, r1)
+    info = full.info()
+    assert info["nnz"] == 2590040752 and nnz_up < info["nnz"] < 2 * nnz_up
+    rng = np.random.default_rng(7)
+    x, y = rng.normal(size=dim), rng.normal(size=dim)
+    a, c = y @ full.spmv(x), x @ full.spmv(y)
+    assert abs(a - c) <= 1e-12 * np.linalg.norm(x) * np.linalg.norm(y) * 100
+
+
 def test_empty_sector():
     say("empty sector")
     with pytest.raises(hd.HDError) as e:
