@@ -85,14 +85,15 @@ class _LanczosStats(ctypes.Structure):
 class _BatchOpts(ctypes.Structure):
     _fields_ = [("nev", ctypes.c_int32), ("ncv", ctypes.c_int32), ("max_restarts", ctypes.c_int32),
                 ("tol", ctypes.c_double), ("seed", ctypes.c_uint64), ("dense_below", ctypes.c_int32),
-                ("use_graph", ctypes.c_int32)]
+                ("use_graph", ctypes.c_int32), ("full_reorth", ctypes.c_int32)]
 
 
 class _BatchStats(ctypes.Structure):
     _fields_ = [("problems", ctypes.c_int32), ("lanczos_problems", ctypes.c_int32),
                 ("dense_problems", ctypes.c_int32), ("converged", ctypes.c_int32),
                 ("restarts", ctypes.c_int32), ("steps", ctypes.c_int32),
-                ("max_residual", ctypes.c_double), ("time_total_ms", ctypes.c_double)]
+                ("max_residual", ctypes.c_double), ("time_total_ms", ctypes.c_double),
+                ("reorth_steps", ctypes.c_int32)]
 
 
 _lib = None
@@ -700,11 +701,12 @@ class BatchProblem:
         return v.value
 
     def solve(self, nev=3, tol=1e-10, ncv=0, seed=0, dense_below=0, use_graph=True, max_restarts=0,
-              raise_on_fail=True):
+              raise_on_fail=True, full_reorth=False):
         """(evals[sector][point][nev], residuals, stats dict)."""
         o = _BatchOpts()
         o.nev, o.ncv, o.max_restarts, o.tol, o.seed = nev, ncv, max_restarts, tol, seed
         o.dense_below, o.use_graph = dense_below, 1 if use_graph else 0
+        o.full_reorth = 1 if full_reorth else 0
         S, P = len(self.bases), self.points
         ev = np.zeros((S, P, nev), dtype=np.float64)
         rs = np.zeros((S, P, nev), dtype=np.float64)

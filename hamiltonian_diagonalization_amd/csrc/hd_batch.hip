@@ -180,14 +180,17 @@ struct Chunks {
 };
 
 // partial[(c*PS + i)*Pp + p] = sum_rows V_i w for i < nv, and |w|^2 at i = nv
-// when want_norm.  Skipped entirely when *any == 0.
+// when want_norm.  Skipped entirely when *any == 0, and per chunk when its
+// sector's flag in sany is 0.
 __global__ void __launch_bounds__(kVB)
 k_bdots(int nv, const double* __restrict__ V0, int64_t vstride, const double* __restrict__ w, int want_norm,
-        Chunks ck, int Pp, int PS, double* __restrict__ partial, const int* __restrict__ any)
+        Chunks ck, int Pp, int PS, double* __restrict__ partial, const int* __restrict__ any,
+        const int* __restrict__ sany)
 {
     if (any && *any == 0) return;
-    __shared__ double red[kVB];
     const int c = blockIdx.x;
+    if (sany && sany[ck.sec[c]] == 0) return;
+    __shared__ double red[kVB];
     const int64_t r0 = ck.r0[c], r1 = ck.r1[c];
     const int p = threadIdx.x % Pp, rl = threadIdx.x / Pp, RL = kVB / Pp;
     const int nout = nv + want_norm;
@@ -221,10 +224,11 @@ k_bdots(int nv, const double* __restrict__ V0, int64_t vstride, const double* __
 // out[(s*PS + i)*Pp + p] = sum over the sector's chunks, in chunk order.
 __global__ void k_breduce(int nout, const double* __restrict__ partial, int PS, const int32_t* __restrict__ lsec,
                           const int32_t* __restrict__ sec_c0, const int32_t* __restrict__ sec_c1, int Pp,
-                          double* __restrict__ out, const int* __restrict__ any)
+                          double* __restrict__ out, const int* __restrict__ any, const int* __restrict__ sany)
 {
     if (any && *any == 0) return;
     const int s = lsec[blockIdx.x];
+    if (sany && sany[s] == 0) return;
     const int i = blockIdx.y;
     const int p = threadIdx.x;
     if (i >= nout || p >= Pp) return;
@@ -237,11 +241,12 @@ __global__ void k_breduce(int nout, const double* __restrict__ partial, int PS, 
 __global__ void __launch_bounds__(kVB)
 k_bupdate(int nv, const double* __restrict__ V0, int64_t vstride, const double* __restrict__ coef, int64_t cs,
           int64_t ci, const double* __restrict__ mask, Chunks ck, int Pp, double* __restrict__ w,
-          const int* __restrict__ any)
+          const int* __restrict__ any, const int* __restrict__ sany)
 {
     if (any && *any == 0) return;
     const int c = blockIdx.x;
     const int s = ck.sec[c];
+    if (sany && sany[s] == 0) return;
     const int64_t r0 = ck.r0[c], r1 = ck.r1[c];
     const int p = threadIdx.x % Pp, rl = threadIdx.x / Pp, RL = kVB / Pp;
     const double m = mask ? mask[(size_t)s * Pp + p] : 1.0;
@@ -393,8 +398,11 @@ __global__ void k_binv_norm(Problems pr, const double* __restrict__ nrm, int PS,
 
 // After the first pass: second pass where the norm fell below 1e-4 of its value
 // before the projection (DGKS) or on the first step of a cycle; *any = OR.
+// With `full` (partial reorthogonalisation) only problems whose first pass ran
+// are considered.
 __global__ void k_bflag(Problems pr, int first, const double* __restrict__ c1, int PS, int jn,
-                        const double* __restrict__ n1, double* __restrict__ need2, int* __restrict__ any)
+                        const double* __restrict__ n1, const double* __restrict__ full, double* __restrict__ need2,
+                        int* __restrict__ any)
 {
     __shared__ int sh;
     if (threadIdx.x == 0) sh = 0;
@@ -403,6 +411,10 @@ __global__ void k_bflag(Problems pr, int first, const double* __restrict__ c1, i
     for (int g = threadIdx.x; g < pr.nls * pr.Pp; g += blockDim.x) {
         int s, p;
         problem_of(pr, g, s, p);
+        if (full && full[(size_t)s * pr.Pp + p] == 0.0) {
+            need2[(size_t)s * pr.Pp + p] = 0.0;
+            continue;
+        }
         const double before = c1[((size_t)s * PS + jn) * pr.Pp + p];
         const double after = n1[(size_t)s * PS * pr.Pp + p];
         const int need = first || after < 1e-4 * before;
@@ -415,25 +427,116 @@ __global__ void k_bflag(Problems pr, int first, const double* __restrict__ c1, i
 }
 
 
+// Counts the steps whose full pass runs for some problem (any == null: always).
+__global__ void k_bcount(const int* __restrict__ any, int* __restrict__ counter)
+{
+    if (!any || *any) atomicAdd(counter, 1);
+}
+
+// Partial reorthogonalisation (Simon, Math. Comp. 42 (1984) 115), per problem:
+// om[j+1][i] ~ <v_{j+1}, v_i> propagated through the projected matrix (the kept
+// Ritz values alpha[i < k] with their arrow, then the cycle's tridiagonal) with
+// beta_j = sqrt(n0), the norm of the locally updated w, and a rounding term
+// 2 eps |A| / beta_j.  A problem needs the pass against the whole basis when an
+// estimate exceeds om_max, and then on its next step too (force).  full[q] = 1
+// where the pass runs; *any = OR over problems.
+__global__ void k_bomega(Problems pr, int S, int m, int j, int k, double om_max, const double* __restrict__ a,
+                         const double* __restrict__ n0, int PS, const double* __restrict__ alpha,
+                         const double* __restrict__ beta, const double* __restrict__ arrow,
+                         const double* __restrict__ hscale, const int* __restrict__ state, double* __restrict__ om,
+                         int* __restrict__ force, double* __restrict__ full, int* __restrict__ any,
+                         int* __restrict__ sany)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    int s, p;
+    if (!problem_of(pr, g, s, p)) return;
+    const int Pp = pr.Pp;
+    const size_t q = (size_t)s * Pp + p;
+    const double eps = 2.220446049250313e-16;
+    auto at = [&](int row, int i) -> double& { return om[(((size_t)row * (m + 1) + i) * S + s) * Pp + p]; };
+    auto A = [&](int i) { return alpha[((size_t)i * S + s) * Pp + p]; };
+    auto B = [&](int i) { return beta[((size_t)i * S + s) * Pp + p]; };
+    auto R = [&](int i) { return arrow[((size_t)i * S + s) * Pp + p]; };
+    if (state[q] != 0) {  // frozen or broken down: no pass
+        full[q] = 0.0;
+        return;
+    }
+    const double aj = a[(size_t)s * PS * Pp + p];
+    const double bj = sqrt(fmax(n0[(size_t)s * PS * Pp + p], 0.0));
+    const double anorm = fmax(hscale[q], fabs(aj) + bj);
+    bool need = force[q] != 0;
+    force[q] = 0;
+    if (!(bj > 1e-12 * anorm)) need = true;  // breakdown: the pass decides
+    if (!need) {
+        const double noise = 2.0 * eps * anorm / bj;
+        double worst = 0.0;
+        for (int i = 0; i < j; ++i) {
+            double tv, dii;  // (A v_i)^T v_j, diagonal term apart
+            if (i < k) {
+                dii = A(i);
+                tv = R(i) * at(j, k);
+            } else if (i == k) {
+                dii = A(k);
+                tv = B(k) * at(j, k + 1);
+                for (int l = 0; l < k; ++l) tv += R(l) * at(j, l);
+            } else {
+                dii = A(i);
+                tv = B(i - 1) * at(j, i - 1) + B(i) * at(j, i + 1);
+            }
+            const double v = (tv + (dii - aj) * at(j, i) - B(j - 1) * at(j - 1, i)) / bj;
+            const double w = v + (v >= 0 ? noise : -noise);
+            at(j + 1, i) = w;
+            worst = fmax(worst, fabs(w));
+        }
+        at(j + 1, j) = eps * anorm / bj;
+        if (worst > om_max) {
+            need = true;
+            force[q] = 1;
+        }
+    }
+    full[q] = need ? 1.0 : 0.0;
+    if (need) {
+        atomicOr(any, 1);
+        atomicOr(&sany[s], 1);
+    }
+}
+
 // Coefficients of step j: alpha_j, beta_j (layout [j][s][p]), the scale of
 // v_{j+1}, breakdown.  state: 0 alive, 1 broke down in this cycle (exact
 // invariant subspace of size nact), 2 finished (vectors frozen).
-__global__ void k_bfinish(Problems pr, int S, int j, int local, const double* __restrict__ a,
+// With partial reorthogonalisation `full` marks the problems whose pass against
+// the basis ran (elsewhere alpha is the local coefficient and the norm n0 of the
+// locally updated w), and the omega row of v_{j+1} (om, [row][i][s][p] over
+// m+1 rows of m+1) is reset to eps where the pass ran.
+__global__ void k_bfinish(Problems pr, int S, int m, int j, int k, int local, const double* __restrict__ a,
                           const double* __restrict__ c1, const double* __restrict__ c2, int PS,
                           const double* __restrict__ need2, const double* __restrict__ n1,
-                          const double* __restrict__ n2, double* __restrict__ alpha, double* __restrict__ beta,
-                          double* __restrict__ inv, double* __restrict__ hscale, int* __restrict__ state,
-                          int* __restrict__ nact)
+                          const double* __restrict__ n2, const double* __restrict__ full,
+                          const double* __restrict__ n0, double* __restrict__ om, double* __restrict__ alpha,
+                          double* __restrict__ beta, double* __restrict__ inv, double* __restrict__ hscale,
+                          int* __restrict__ state, int* __restrict__ nact)
 {
     int s, p;
     if (!problem_of(pr, blockIdx.x * blockDim.x + threadIdx.x, s, p)) return;
     const int Pp = pr.Pp;
     const size_t q = (size_t)s * Pp + p;
-    const bool two = need2[q] != 0.0;
-    double al = c1[((size_t)s * PS + j) * Pp + p];
+    const bool did = !full || full[q] != 0.0;
+    const bool two = did && need2[q] != 0.0;
+    double al = did ? c1[((size_t)s * PS + j) * Pp + p] : 0.0;
     if (local) al += a[(size_t)s * PS * Pp + p];
     if (two) al += c2[((size_t)s * PS + j) * Pp + p];
-    const double nn = two ? n2[(size_t)s * PS * Pp + p] : n1[(size_t)s * PS * Pp + p];
+    const double nn = two ? n2[(size_t)s * PS * Pp + p] : did ? n1[(size_t)s * PS * Pp + p] : n0[(size_t)s * PS * Pp + p];
+    if (om) {
+        auto at = [&](int row, int i) -> double& { return om[(((size_t)row * (m + 1) + i) * S + s) * Pp + p]; };
+        const double eps = 2.220446049250313e-16;
+        if (j == k) {  // v_k: orthogonal to the kept Ritz vectors
+            for (int i = 0; i < j; ++i) at(j, i) = eps;
+            at(j, j) = 1.0;
+        }
+        if (did)
+            for (int i = 0; i <= j; ++i) at(j + 1, i) = eps;
+        at(j + 1, j + 1) = 1.0;
+    }
     const double be = sqrt(fmax(nn, 0.0));
     const double hs = fmax(hscale[q], fabs(al) + be);
     hscale[q] = hs;
@@ -1035,6 +1138,11 @@ int hd_batch_solve(hd_batch* b, const hd_batch_opts* o, double* h_evals, double*
                *N1 = nullptr, *N2 = nullptr, *need2 = nullptr, *inv = nullptr, *hscale = nullptr, *alpha = nullptr,
                *beta = nullptr, *arrow = nullptr, *Yr = nullptr;
         int *state = nullptr, *nact = nullptr, *conv = nullptr, *d_any = nullptr, *d_unconv = nullptr;
+        // partial reorthogonalisation state
+        const bool pro = o->full_reorth == 0;
+        const double om_max = std::min(std::sqrt(2.220446049250313e-16), tol);
+        double *om = nullptr, *fullm = nullptr, *N0 = nullptr;
+        int *force = nullptr, *d_any0 = nullptr, *d_nfull = nullptr, *sany = nullptr;
         HD_CHECK(w.get(&V, (size_t)(m + 1) * vsz, "Lanczos basis"));
         HD_CHECK(w.get(&W, vsz, "Lanczos work vector"));
         HD_CHECK(w.get(&partial, (size_t)nck * PS * Pp, "dot partials"));
@@ -1045,6 +1153,15 @@ int hd_batch_solve(hd_batch* b, const hd_batch_opts* o, double* h_evals, double*
         for (int** p : {&state, &nact, &conv}) HD_CHECK(w.get(p, prob, "problem state"));
         HD_CHECK(w.get(&d_any, 1, "flag"));
         HD_CHECK(w.get(&d_unconv, 1, "counter"));
+        HD_CHECK(w.get(&d_any0, 1, "flag"));
+        HD_CHECK(w.get(&d_nfull, 1, "counter"));
+        if (pro) {
+            HD_CHECK(w.get(&om, (size_t)(m + 1) * (m + 1) * prob, "orthogonality estimates"));
+            HD_CHECK(w.get(&fullm, prob, "pass mask"));
+            HD_CHECK(w.get(&N0, prob * PS, "local norms"));
+            HD_CHECK(w.get(&force, prob, "forced passes"));
+            HD_CHECK(w.get(&sany, (size_t)S, "sectors with a pass"));
+        }
         auto fill_d = [&](double* a, size_t n, double v) {
             hipLaunchKernelGGL(k_fill_d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, (int64_t)n, v);
         };
@@ -1058,6 +1175,11 @@ int hd_batch_solve(hd_batch* b, const hd_batch_opts* o, double* h_evals, double*
         fill_d(need2, prob, 0.0);
         fill_i(nact, prob, m);
         fill_i(conv, prob, 0);
+        fill_i(d_nfull, 1, 0);
+        if (pro) {
+            fill_d(om, (size_t)(m + 1) * (m + 1) * prob, 0.0);
+            fill_i(force, prob, 0);
+        }
         {
             // padded points are finished from the start
             std::vector<int> s0(prob, 2);
@@ -1071,16 +1193,17 @@ int hd_batch_solve(hd_batch* b, const hd_batch_opts* o, double* h_evals, double*
         const int64_t vstride = (int64_t)vsz;
         auto Vj = [&](int j) { return V + (size_t)j * vsz; };
         const dim3 gp((unsigned)((nls * Pp + 255) / 256)), bp(256);
-        auto dots = [&](int nv, const double* base, const double* wv, int want_norm, double* out, const int* any) {
+        auto dots = [&](int nv, const double* base, const double* wv, int want_norm, double* out, const int* any,
+                        const int* sany = nullptr) {
             hipLaunchKernelGGL(k_bdots, dim3((unsigned)nck), dim3(kVB), 0, st, nv, base, vstride, wv, want_norm, ck,
-                               Pp, PS, partial, any);
+                               Pp, PS, partial, any, sany);
             hipLaunchKernelGGL(k_breduce, dim3((unsigned)nls, (unsigned)(nv + want_norm)), dim3(Pp), 0, st,
-                               nv + want_norm, partial, PS, d_lsec, d_sec_c0, d_sec_c1, Pp, out, any);
+                               nv + want_norm, partial, PS, d_lsec, d_sec_c0, d_sec_c1, Pp, out, any, sany);
         };
         auto update = [&](int nv, const double* base, const double* coef, int64_t cs, int64_t ci, const double* mask,
-                          const int* any) {
+                          const int* any, const int* sany = nullptr) {
             hipLaunchKernelGGL(k_bupdate, dim3((unsigned)nck), dim3(kVB), 0, st, nv, base, vstride, coef, cs, ci,
-                               mask, ck, Pp, W, any);
+                               mask, ck, Pp, W, any, sany);
         };
         // ---- start: V_0 = normalised random vector per problem
         hipLaunchKernelGGL(k_brandn, dim3((unsigned)nck), dim3(kVB), 0, st, o->seed, ck, Pp, P, V);
@@ -1100,16 +1223,33 @@ int hd_batch_solve(hd_batch* b, const hd_batch_opts* o, double* h_evals, double*
                 dots(1, Vj(j), W, 0, A, nullptr);
                 update(1, Vj(j), A, (int64_t)PS * Pp, 0, nullptr, nullptr);
             }
-            dots(j + 1, V, W, 1, C1, nullptr);
-            update(j + 1, V, C1, (int64_t)PS * Pp, Pp, nullptr, nullptr);
-            dots(0, V, W, 1, N1, nullptr);
-            hipLaunchKernelGGL(k_bflag, dim3(1), dim3(1024), 0, st, pr, j == k ? 1 : 0, C1, PS, j + 1, N1, need2,
+            // partial reorthogonalisation between the first and the last step of
+            // a cycle (both always make the full pass)
+            const bool pro_step = pro && local && j < m - 1;
+            const double* fm = nullptr;
+            const int* g1 = nullptr;
+            const int* sg = nullptr;
+            if (pro_step) {
+                dots(0, V, W, 1, N0, nullptr);
+                fill_i(d_any0, 1, 0);
+                fill_i(sany, (size_t)S, 0);
+                hipLaunchKernelGGL(k_bomega, gp, bp, 0, st, pr, S, m, j, k, om_max, A, N0, PS, alpha, beta, arrow,
+                                   hscale, state, om, force, fullm, d_any0, sany);
+                fm = fullm;
+                g1 = d_any0;
+                sg = sany;
+            }
+            hipLaunchKernelGGL(k_bcount, dim3(1), dim3(1), 0, st, g1, d_nfull);
+            dots(j + 1, V, W, 1, C1, g1, sg);
+            update(j + 1, V, C1, (int64_t)PS * Pp, Pp, fm, g1, sg);
+            dots(0, V, W, 1, N1, g1, sg);
+            hipLaunchKernelGGL(k_bflag, dim3(1), dim3(1024), 0, st, pr, j == k ? 1 : 0, C1, PS, j + 1, N1, fm, need2,
                                d_any);
             dots(j + 1, V, W, 0, C2, d_any);
             update(j + 1, V, C2, (int64_t)PS * Pp, Pp, need2, d_any);
             dots(0, V, W, 1, N2, d_any);
-            hipLaunchKernelGGL(k_bfinish, gp, bp, 0, st, pr, S, j, local, A, C1, C2, PS, need2, N1, N2, alpha, beta,
-                               inv, hscale, state, nact);
+            hipLaunchKernelGGL(k_bfinish, gp, bp, 0, st, pr, S, m, j, k, local, A, C1, C2, PS, need2, N1, N2, fm, N0,
+                               om, alpha, beta, inv, hscale, state, nact);
             hipLaunchKernelGGL(k_bscale, dim3((unsigned)nck), dim3(kVB), 0, st, W, inv, ck, Pp, Vj(j + 1));
         };
         auto cycle = [&](int k) {
@@ -1151,6 +1291,7 @@ int hd_batch_solve(hd_batch* b, const hd_batch_opts* o, double* h_evals, double*
             sx.steps += m - keep;
         }
         sx.restarts = restarts;
+        HD_HIP(hipMemcpy(&sx.reorth_steps, d_nfull, 4, hipMemcpyDeviceToHost));
         // true residuals of the nev Ritz vectors (V_0..V_{nev-1} after the last combine)
         std::vector<double> r2(prob);
         double* R2 = N2;
@@ -1159,7 +1300,7 @@ int hd_batch_solve(hd_batch* b, const hd_batch_opts* o, double* h_evals, double*
             hipLaunchKernelGGL(k_bresid, dim3((unsigned)nck), dim3(kVB), 0, st, W, Vj(i), d_ev + i,
                                (int64_t)Pp * nev, (int64_t)nev, ck, Pp, partial);
             hipLaunchKernelGGL(k_breduce, dim3((unsigned)nls, 1u), dim3(Pp), 0, st, 1, partial, 1, d_lsec, d_sec_c0,
-                               d_sec_c1, Pp, R2, nullptr);
+                               d_sec_c1, Pp, R2, nullptr, nullptr);
             HD_HIP(hipGetLastError());
             std::vector<double> tmp((size_t)S * Pp);
             HD_HIP(hipMemcpyAsync(tmp.data(), R2, tmp.size() * 8, hipMemcpyDeviceToHost, st));

@@ -707,10 +707,10 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
     // cycle starts from an orthogonal pair.
     const bool pro = o.full_reorth == 0;
     const double eps = std::numeric_limits<double>::epsilon();
-    // the reorthogonalisation coefficients left out of the recurrence are of
-    // the size of the estimates; keeping them at 100 tol keeps the Ritz
-    // vectors' residuals below tol (semi-orthogonality, sqrt(eps), at most)
-    const double om_max = std::min(std::sqrt(eps), 100.0 * (o.tol > 0 ? o.tol : 1e-10));
+    // a pass removes coefficients of the size of the estimates, which the
+    // recurrence (and so the Ritz vectors' residuals) then misses: keeping them
+    // below tol keeps the residuals at tol (semi-orthogonality, sqrt(eps), at most)
+    const double om_max = std::min(std::sqrt(eps), o.tol > 0 ? o.tol : 1e-10);
     std::vector<double> om_prev(m + 1, 0.0), om_cur(m + 1, 0.0), om_next(m + 1, 0.0);
     bool force_next = false;
     int reorth_steps = 0;

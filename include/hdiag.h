@@ -427,6 +427,9 @@ typedef struct {
     int32_t dense_below;  /* sectors below this dimension are diagonalised densely; 0 = 60
                              (BaseDiagonalize, pseudopotential_model_base.hpp:243-252); at most 65 */
     int32_t use_graph;    /* replay the restart cycle as a captured hipGraph */
+    int32_t full_reorth;  /* 0: partial reorthogonalisation (per-problem omega estimates; a
+                             pass against the whole basis only where one is needed);
+                             1: a pass at every step */
 } hd_batch_opts;
 
 typedef struct {
@@ -435,6 +438,7 @@ typedef struct {
     int32_t steps;        /* Lanczos steps (operator applications per problem) */
     double max_residual;  /* largest true residual over the Lanczos problems */
     double time_total_ms;
+    int32_t reorth_steps; /* steps whose full pass ran (for at least one problem) */
 } hd_batch_stats;
 
 /* The nev lowest eigenvalues of every problem, h_evals[(s*n_points + p)*nev + i];
