@@ -11,7 +11,7 @@
 //                  --pseudopotentials v0,v1,... [options]
 //   hd_diagonalize --nbr-particles N --kmax K --layout sphere|ofl --complex 0|1
 //                  --ktable F --vtable F [--eps F] --sector lz:a,b|k2d:kx,ky,dx,dy|k1d:ky,dy|none
-// options: --nbr-eigenvalues NEV (5)  --tol T (1e-12)  --seed S (0)
+// options: --nbr-eigenvalues NEV (5)  --tol T (1e-12)  --seed S (0)  --full-reorth 1
 //          --precision f64|coded|f32  --dense-below D (60; the flux-lattice caller uses 1000)
 //          --dump-crs PREFIX (writes PREFIX.rowstarts/.cols/.values of the upper-triangle CRS)
 //          --dump-basis FILE  --initial-vector FILE  --final-vector FILE
@@ -201,6 +201,7 @@ namespace
         iSize_t nev = (iSize_t)std::stoul(args.count("nbr-eigenvalues") ? args["nbr-eigenvalues"] : "5");
         ham.m_arpackWrapper.SetTolerance(args.count("tol") ? std::stod(args["tol"]) : 1e-12);
         ham.m_arpackWrapper.SetSeed(args.count("seed") ? std::stoull(args["seed"]) : 0);
+        ham.m_arpackWrapper.SetFullReorthogonalisation(Flag(args, "full-reorth"));
         if (args.count("initial-vector")) {
             ham.m_arpackWrapper.UseInitialVector();
             ham.m_arpackWrapper.SetInitialVectorFile(args["initial-vector"]);

@@ -393,6 +393,8 @@ namespace utilities
             // A device start vector (e.g. the learned start of hd_mlp_start_vector);
             // takes precedence over the initial-vector file.
             void SetDeviceStartVector(const void* d_start) { m_deviceStart = d_start; }
+            // Partial reorthogonalisation (default) or a full pass at every step.
+            void SetFullReorthogonalisation(const bool full) { m_fullReorth = full; }
 
             char m_which = 'S';
             bool m_provideInitial = false;
@@ -405,6 +407,7 @@ namespace utilities
             uint64_t m_seed = 0;
             int m_precision = HD_F64;
             const void* m_deviceStart = nullptr;
+            bool m_fullReorth = false;
             hd_lanczos_stats m_lastStats{};
         };
     }
@@ -945,6 +948,7 @@ namespace diagonalization
                 o.tol = m_arpackWrapper.m_tol;
                 o.seed = m_arpackWrapper.m_seed;
                 o.precision = m_arpackWrapper.m_precision;
+                o.full_reorth = m_arpackWrapper.m_fullReorth ? 1 : 0;
                 o.want_vectors = 1;
                 o.start = HD_START_RANDOM;
                 void* d_file = nullptr;
