@@ -61,7 +61,7 @@ class _HamInfo(ctypes.Structure):
 class _TileInfo(ctypes.Structure):
     _fields_ = [("groups", ctypes.c_int64), ("entries", ctypes.c_int64), ("lines", ctypes.c_int64),
                 ("chunks", ctypes.c_int64), ("rows_per_group", ctypes.c_int64), ("spmv_bytes", ctypes.c_int64),
-                ("build_ms", ctypes.c_double), ("entry_bytes", ctypes.c_int64)]
+                ("build_ms", ctypes.c_double), ("entry_bytes", ctypes.c_int64), ("block_window", ctypes.c_int64)]
 
 
 class _LanczosOpts(ctypes.Structure):
@@ -133,6 +133,7 @@ SIGNATURES = {
     "hd_ham_device_coded": (ctypes.c_int, [_P, _PP, _PP, _PP, ctypes.POINTER(_I32)]),
     "hd_ham_tile": (ctypes.c_int, [_P, _I32, _I32]),
     "hd_ham_tile_info": (ctypes.c_int, [_P, ctypes.POINTER(_TileInfo)]),
+    "hd_ham_tile_order": (ctypes.c_int, [_P, ctypes.c_int64]),
     "hd_ham_device_tiled": (ctypes.c_int, [_P, _PP, _PP, _PP, _PP, _PP]),
     "hd_obs_density": (ctypes.c_int, [_P, _I32, _P, _P]),
     "hd_obs_density_density": (ctypes.c_int, [_P, _I32, _P, _P]),
@@ -539,6 +540,13 @@ class Hamiltonian:
         4 (0) or 3 (packed; dictionaries of at most 2047 values)."""
         _check(lib().hd_ham_tile(self.h, rows_per_group, entry_bytes))
         return self.tile_info()
+
+    def tile_order(self, groups_per_xcd: int = -1) -> int:
+        """Block order of the tiled product: each window of 8*w workgroups
+        gives every XCD w consecutive row groups (0 = launch order, -1 = time
+        the candidates and keep the fastest).  Returns the order in use."""
+        _check(lib().hd_ham_tile_order(self.h, groups_per_xcd))
+        return self.tile_info()["block_window"]
 
     def tile_info(self) -> dict:
         i = _TileInfo()

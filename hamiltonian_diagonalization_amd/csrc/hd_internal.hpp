@@ -136,6 +136,28 @@ constexpr int kRankHeader = 6;
 
 #define HD_DEV __device__ __forceinline__
 
+// The hardware hands consecutive workgroup ids to the 8 XCDs in turn, and each
+// XCD has its own L2.  For a grid of n blocks this maps block b to the work
+// item it should take so that XCD k runs one contiguous range of items, in
+// order: neighbouring row groups, which read mostly the same x lines, then
+// share one L2.  The map is a bijection of [0, n).
+constexpr int kXcds = 8;
+HD_DEV int64_t xcd_block(int64_t b, int64_t n)
+{
+    const int64_t q = n / kXcds, rem = n % kXcds, k = b % kXcds;
+    return k * q + (k < rem ? k : rem) + b / kXcds;
+}
+// The same within windows of kXcds * win blocks (the last window may be
+// partial): every XCD takes `win` consecutive items of each window, which keeps
+// the items in flight across the chip close together.  win <= 0 keeps the
+// launch order; win >= ceil(n / kXcds) is one window over the whole grid.
+HD_DEV int64_t xcd_window(int64_t b, int64_t n, int64_t win)
+{
+    if (win <= 0) return b;
+    const int64_t W = kXcds * win, base = (b / W) * W;
+    return base + xcd_block(b - base, n - base < W ? n - base : W);
+}
+
 struct RankView {
     const int32_t* off;
     const int32_t* gmm;
@@ -284,6 +306,7 @@ struct hd_hamiltonian {
     int64_t tg_groups = 0, tg_entries = 0, tg_lines = 0, tg_offs = 0;
     int tg_rows = 0;  // rows per group (128, or 64 for denser matrices)
     int tg_entry_bytes = 4;  // 4, or 3 for packed entries (dictionary <= 2047 values)
+    int64_t tg_win = 0;      // block order of the tiled product (groups per XCD window)
     int64_t* d_tg_base = nullptr;  // entry, line and offset-table bases, 3 x (groups + 1)
     uint32_t* d_tg_entries = nullptr;
     uint32_t* d_tg_lines = nullptr;

@@ -302,7 +302,7 @@ template <int TG, bool PK>
 __global__ void __launch_bounds__(kSpmvThreads)
 k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, const uint32_t* __restrict__ entries,
              const uint32_t* __restrict__ lines, const uint32_t* __restrict__ meta, const double* __restrict__ tdiag,
-             const double* __restrict__ dict, int ndict, int x_aligned, const double* __restrict__ x,
+             const double* __restrict__ dict, int ndict, int x_aligned, int64_t win, const double* __restrict__ x,
              double* __restrict__ y)
 {
     constexpr int PER = (kTC * 8) / kSpmvThreads;  // 16-byte x pieces per thread per chunk
@@ -314,7 +314,7 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
     __shared__ uint32_t sm[kMeta];
     extern __shared__ double sdict[];
     for (int i = threadIdx.x; i < ndict; i += kSpmvThreads) sdict[i] = dict[i];
-    const int64_t g = blockIdx.x;
+    const int64_t g = xcd_window(blockIdx.x, groups, win);
     const int64_t lb = base[groups + 1 + g], le = base[groups + 1 + g + 1];
     const int64_t mb = base[2 * (groups + 1) + g], me = base[2 * (groups + 1) + g + 1];
     const int nmeta = (int)min(me - mb, (int64_t)kMeta);
@@ -426,6 +426,47 @@ __global__ void __launch_bounds__(256) k_tile_pack(const uint4* __restrict__ in,
 
 // Builds the layout with TG-row groups; HD_ERR_UNSUPPORTED when a group reads
 // more than kMaxLines distinct lines.
+// Chooses the block order of the tiled product (groups per XCD window) by
+// timing each candidate on zero vectors, one warm and three timed launches
+// each: the launch order, windows of 16, 64 and 256 groups per XCD, and one
+// window over the whole grid.
+static int tiled_pick_window(hd_hamiltonian* h)
+{
+    h->tg_win = 0;
+    if (h->dim == 0) return HD_OK;
+    double *x = nullptr, *y = nullptr;
+    HD_CHECK(dalloc_t(&x, (size_t)h->dim, "window autotune x"));
+    int st = dalloc_t(&y, (size_t)h->dim, "window autotune y");
+    if (st != HD_OK) {
+        dfree(x);
+        return st;
+    }
+    HipErrors q;
+    HD_ENQ(q, hipMemsetAsync(x, 0, (size_t)h->dim * sizeof(double), h->stream));
+    st = q.status();
+    const int64_t cands[] = {0, 16, 64, 256, (h->tg_groups + kXcds - 1) / kXcds};
+    float best = 1e30f;
+    int64_t best_w = 0;
+    EventTimer timer;
+    for (const int64_t w : cands) {
+        if (st != HD_OK) break;
+        h->tg_win = w;
+        if ((st = tiled_launch(h, x, y, h->stream)) != HD_OK) break;
+        timer.start(h->stream);
+        for (int r = 0; r < 3 && st == HD_OK; ++r) st = tiled_launch(h, x, y, h->stream);
+        const float ms = timer.stop(h->stream);
+        if (st == HD_OK) st = timer.status("tiled window autotune");
+        if (st == HD_OK && ms < best) {
+            best = ms;
+            best_w = w;
+        }
+    }
+    h->tg_win = best_w;
+    dfree(x);
+    dfree(y);
+    return st;
+}
+
 template <int TG>
 static int tiled_build_tg(hd_hamiltonian* h, int entry_bytes)
 {
@@ -528,7 +569,7 @@ static int tiled_build_tg(hd_hamiltonian* h, int entry_bytes)
     h->tg_entry_bytes = entry_bytes;
     h->bytes += (size_t)h->tg_entries * entry_bytes + (size_t)h->tg_lines * 4 + (size_t)h->tg_offs * 4 + base.size() * 8 +
                 (size_t)h->dim * 8;
-    return HD_OK;
+    return tiled_pick_window(h);
 }
 
 int tiled_build(hd_hamiltonian* h, int rows_per_group, int entry_bytes)
@@ -575,6 +616,7 @@ void tiled_free(hd_hamiltonian* h)
     h->tg_groups = h->tg_entries = h->tg_lines = h->tg_offs = 0;
     h->tg_rows = 0;
     h->tg_entry_bytes = 4;
+    h->tg_win = 0;
 }
 
 int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
@@ -586,8 +628,8 @@ int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
 #define HD_TL(TG, PK)                                                                                            \
     hipLaunchKernelGGL((k_spmv_tiled<TG, PK>), dim3((unsigned)h->tg_groups), dim3(kSpmvThreads), lds, s, h->dim, \
                        h->tg_groups, h->d_tg_base, h->d_tg_entries, h->d_tg_lines, h->d_tg_offs, h->d_tg_diag,    \
-                       static_cast<const double*>(h->d_dict), h->ndict, aligned, static_cast<const double*>(x),  \
-                       static_cast<double*>(y))
+                       static_cast<const double*>(h->d_dict), h->ndict, aligned, h->tg_win,                     \
+                       static_cast<const double*>(x), static_cast<double*>(y))
     const bool pk = h->tg_entry_bytes == 3;
     if (h->tg_rows == 128) {
         if (pk) HD_TL(128, true);
@@ -631,6 +673,17 @@ extern "C" int hd_ham_tile_info(const hd_hamiltonian* h, hd_tile_info_t* info)
     info->spmv_bytes = (int64_t)tiled_bytes(h);
     info->build_ms = h->t_tile;
     info->entry_bytes = h->tg_entry_bytes;
+    info->block_window = h->tg_win;
+    return HD_OK;
+}
+
+extern "C" int hd_ham_tile_order(hd_hamiltonian* h, int64_t groups_per_xcd)
+{
+    if (!h) return fail(HD_ERR_ARG, "hd_ham_tile_order: null handle");
+    if (!h->d_tg_base && h->dim > 0) return fail(HD_ERR_ARG, "no tiled layout (hd_ham_tile)");
+    if (groups_per_xcd < -1) return fail(HD_ERR_ARG, "groups per XCD must be -1 (autotune), 0 or positive");
+    if (groups_per_xcd == -1) return tiled_pick_window(h);
+    h->tg_win = groups_per_xcd;
     return HD_OK;
 }
 

@@ -1,6 +1,6 @@
 """A short fixed workload for hardware-counter runs of the SpMV kernels.
 
-python -m hamiltonian_diagonalization_amd.profile_spmv [--n 12 --two-s 33 --iters 3]
+python -m hamiltonian_diagonalization_amd.profile_spmv [--n 12 --two-s 33 --iters 3] [--orders]
 
 Builds the N=12, 2S=33 L_z=0 Coulomb sphere Hamiltonian with FP64, FP32 and
 coded values and the tiled layout (4-byte entries, then the packed 3-byte
@@ -23,21 +23,34 @@ def main():
     ap.add_argument("--two-s", type=int, default=33)
     ap.add_argument("--iters", type=int, default=3)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--rows", type=int, default=0, help="rows per group of the tiled layout (0 = automatic)")
+    ap.add_argument("--orders", action="store_true",
+                    help="also time the tiled product under each block order (groups per XCD window)")
     args = ap.parse_args()
     basis = FockBasis(Sector.lz(0, args.two_s), args.n, args.two_s + 1)
     ham = Hamiltonian(basis, models.sphere_tables(args.two_s, models.coulomb_list(args.two_s)),
                       storage=STORAGE_FULL, values=(F64, F32, CODED))
-    tile = ham.tile()
+    tile = ham.tile(args.rows)
     ham.set_lanes(0)
     out = {"dim": basis.dim, "nnz": ham.nnz, "lanes": ham.lanes(), "iters": args.iters, "tile": tile, "spmv": {}}
     for name, p in (("f64", F64), ("f32", F32), ("coded", CODED), ("tiled", TILED)):
         ms = ham.spmv_time_ms(p, args.iters)
         out["spmv"][name] = {"ms": ms, "alg_bytes": ham.spmv_bytes(p)}
         print(f"[profile] spmv {name}: {ms:.3f} ms, {ham.spmv_bytes(p)} algorithmic bytes", flush=True)
+    if args.orders:
+        chosen = tile["block_window"]
+        out["orders"] = {}
+        for w in (0, 16, 64, 256, (tile["groups"] + 7) // 8):
+            ham.tile_order(w)
+            ms = ham.spmv_time_ms(TILED, 20)
+            out["orders"][w] = ms
+            print(f"[profile] tiled block order {w} groups per XCD: {ms:.3f} ms", flush=True)
+        ham.tile_order(chosen)
+        print(f"[profile] autotuned order: {chosen}", flush=True)
     # the packed 3-byte entries (k_spmv_tiled<128, true>) when the dictionary allows them
     if tile["entry_bytes"] == 4:
         try:
-            t3 = ham.tile(0, 3)
+            t3 = ham.tile(args.rows, 3)
         except Exception as e:  # dictionary above 2047 values
             print(f"[profile] packed entries not available: {e}", flush=True)
         else:

@@ -312,8 +312,18 @@ typedef struct {
     int64_t spmv_bytes;     /* algorithmic HBM bytes of one product */
     double build_ms;
     int64_t entry_bytes;    /* 3 or 4 */
+    int64_t block_window;   /* block order of the product: groups per XCD window
+                               (0 = launch order), see hd_ham_tile_order */
 } hd_tile_info_t;
 int hd_ham_tile_info(const hd_hamiltonian* h, hd_tile_info_t* info);
+/* Block order of the tiled product (no reference counterpart; a schedule knob
+ * like hd_spmv_set_lanes).  Workgroups go to the 8 XCDs in turn and each XCD
+ * has its own L2; with groups_per_xcd = w > 0 each window of 8w blocks gives
+ * every XCD w consecutive row groups, so neighbouring groups, which read
+ * mostly the same x lines, share an L2.  0 keeps the launch order; -1 times
+ * the candidates (0, 16, 64, 256, one window over the grid) and keeps the
+ * fastest, which hd_ham_tile also does after building the layout. */
+int hd_ham_tile_order(hd_hamiltonian* h, int64_t groups_per_xcd);
 /* Zero-copy view of the tiled layout: group bases (3 x (groups+1) int64:
  * entries, lines, offset table), entries (uint32 words, or with entry_bytes 3
  * the packed little-endian 3-byte entries: slot bits 16-23, word 12-15, sign

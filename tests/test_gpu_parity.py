@@ -290,6 +290,27 @@ def test_tiled_spmv_and_lanczos_config1(oracle):
     assert np.array_equal(h.spmv(x, precision=hd.TILED), got)
 
 
+def test_tiled_block_orders_identical():
+    """The block order only changes which workgroup runs a row group, so
+    every order (launch order, windows with a partial last one, one window
+    over the grid, the autotuned choice) gives the same product bit for bit."""
+    say("tiled block orders N=8 2S=21")
+    b, t = models.sphere_problem(8, 21)
+    h = hd.Hamiltonian(b, t, values=(hd.CODED,))
+    ti = h.tile()
+    groups = ti["groups"]
+    assert groups == (b.dim + 127) // 128 and ti["block_window"] >= 0
+    x = np.random.default_rng(5).normal(size=b.dim)
+    h.tile_order(0)
+    want = h.spmv(x, precision=hd.TILED)
+    for w in (1, 3, 16, (groups + 7) // 8, groups, -1):
+        got_w = h.tile_order(w)
+        assert got_w == w or (w == -1 and got_w >= 0)
+        assert np.array_equal(h.spmv(x, precision=hd.TILED), want), w
+    with pytest.raises(hd.HDError):
+        h.tile_order(-2)
+
+
 def test_tiled_rejects_complex():
     kt, vt, eps = models.ofl_tables(3, 3, 1)
     b = hd.FockBasis(hd.Sector.k2d(0, 0, 3, 3), 3, 9)
