@@ -12,7 +12,7 @@
 //   hd_diagonalize --nbr-particles N --kmax K --layout sphere|ofl --complex 0|1
 //                  --ktable F --vtable F [--eps F] --sector lz:a,b|k2d:kx,ky,dx,dy|k1d:ky,dy|none
 // options: --nbr-eigenvalues NEV (5)  --tol T (1e-12)  --seed S (0)  --full-reorth 1
-//          --precision f64|coded|f32  --dense-below D (60; the flux-lattice caller uses 1000)
+//          --precision f64|coded|tiled|f32  --dense-below D (60; the flux-lattice caller uses 1000)
 //          --dump-crs PREFIX (writes PREFIX.rowstarts/.cols/.values of the upper-triangle CRS)
 //          --dump-basis FILE  --initial-vector FILE  --final-vector FILE
 // reference file options (general_options.hpp:57-78): --file-format 0|1 (text|binary)
@@ -174,9 +174,9 @@ namespace
                        utilities::_SPARSE_MAPPED_, mpi);
         if (args.count("precision")) {
             const std::string p = args["precision"];
-            const int prec = p == "coded" ? HD_CODED : p == "f32" ? HD_F32 : HD_F64;
+            const int prec = p == "coded" ? HD_CODED : p == "tiled" ? HD_TILED : p == "f32" ? HD_F32 : HD_F64;
             ham.m_arpackWrapper.SetPrecision(prec);
-            ham.SetValueFormats(prec == HD_CODED ? 1 | 4 : prec == HD_F32 ? 1 | 2 : 1);
+            ham.SetValueFormats(prec == HD_CODED || prec == HD_TILED ? 1 | 4 : prec == HD_F32 ? 1 | 2 : 1);
         } else {
             ham.SetValueFormats(1);
         }

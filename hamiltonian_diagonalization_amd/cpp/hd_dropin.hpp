@@ -973,6 +973,15 @@ namespace diagonalization
                         }
                     }
                 }
+                if (o.precision == HD_TILED) {
+                    // the tiled layout is built from the coded values on first use
+                    hd_tile_info_t ti;
+                    if (hd_ham_tile_info(m_device.get(), &ti) != HD_OK &&
+                        !utilities::detail::Ok(hd_ham_tile(m_device.get(), 0, 0), "LanczosDiagonalize")) {
+                        if (d_file) hd_free(d_file);
+                        return false;
+                    }
+                }
                 const int st = hd_lanczos(m_device.get(), &o, m_eigenvalues.data(), &m_arpackWrapper.m_lastStats);
                 if (d_file) hd_free(d_file);
                 if (st != HD_OK && st != HD_ERR_NOT_CONVERGED) {

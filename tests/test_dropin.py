@@ -120,6 +120,12 @@ def test_dropin_sphere_config1(oracle, tmp_path):
     assert r["ok"] and r["method"] == "lanczos"
     assert np.max(np.abs(got - want) / np.maximum(1, np.abs(want))) < 1e-10
     assert max(r["residuals"]) <= 1e-12
+    # the same solve on the tiled operator (built on first use from the coded values)
+    rt = last_json(run(["--nbr-particles", 8, "--nbr-orbitals", 22, "--lz", 0, "--pseudopotentials",
+                        pseudo_arg(vm), "--nbr-eigenvalues", 5, "--tol", 1e-12, "--precision", "tiled"]))
+    say(rt)
+    assert rt["ok"] and rt["method"] == "lanczos" and max(rt["residuals"]) <= 1e-12
+    assert np.max(np.abs(np.array(rt["eigenvalues"]) - want) / np.maximum(1, np.abs(want))) < 1e-10
 
 
 @pytest.mark.gpu
