@@ -202,11 +202,19 @@ k_combine(int64_t dim, const typename E<C>::T* __restrict__ V, int64_t ld, int m
         T acc[16];
 #pragma unroll
         for (int l = 0; l < 16; ++l) acc[l] = zero<T>();
-        for (int i = 0; i < m; ++i) {
-            const T v = V[(int64_t)i * ld + r];
+        // four basis vectors' loads in flight per thread; the sum over i keeps
+        // its order
+        for (int i0 = 0; i0 < m; i0 += 4) {
+            T v[4];
 #pragma unroll
-            for (int l = 0; l < 16; ++l)
-                if (l < nl) acc[l] = add2(acc[l], scale(v, sy[i * nl + l]));
+            for (int u = 0; u < 4; ++u) v[u] = i0 + u < m ? V[(int64_t)(i0 + u) * ld + r] : zero<T>();
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (i0 + u >= m) break;
+#pragma unroll
+                for (int l = 0; l < 16; ++l)
+                    if (l < nl) acc[l] = add2(acc[l], scale(v[u], sy[(i0 + u) * nl + l]));
+            }
         }
         for (int l = 0; l < nl; ++l) out[(int64_t)l * ld + r] = acc[l];
     }
