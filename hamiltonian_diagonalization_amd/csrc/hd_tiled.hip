@@ -27,7 +27,9 @@
 // loads (the next chunk's lines are in flight while the current one is
 // processed), then every lane walks its row's entries of that chunk reading x
 // and the dictionary from LDS.  HBM carries 3 or 4 bytes per entry plus the line lists
-// and offsets; the operator is the coded format's exact FP64 matrix.
+// and offsets; the operator is the coded format's exact FP64 matrix.  Complex
+// (Hermitian) matrices use the same layout with 8 values per line (a 3-bit word
+// field) and read their larger dictionary from L2 (k_spmv_tiled_c).
 //
 // The layout is built on the device from the coded CRS, one block per group:
 // an LDS hash set collects the group's lines, a bitonic sort orders them,
@@ -119,7 +121,8 @@ __device__ int row_of(const TilePlanLds<TG>& S, int nrows, int64_t j)
 }
 
 // Builds the plan in LDS; S.flags != 0 when the group does not fit.
-template <int TG>
+// LS: log2 of the x values per 128-byte line (4 real, 3 complex).
+template <int TG, int LS>
 __device__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const int64_t* __restrict__ rowptr,
                           const int32_t* __restrict__ col, const uint16_t* __restrict__ code, int64_t g)
 {
@@ -141,7 +144,7 @@ __device__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const int64_t* __rest
             S.dpos[row_of(S, nrows, j)] = j;
             continue;
         }
-        if (hash_insert(S.keys, (uint32_t)col[j] >> 4) < 0) atomicOr(&S.flags, 1);
+        if (hash_insert(S.keys, (uint32_t)col[j] >> LS) < 0) atomicOr(&S.flags, 1);
     }
     __syncthreads();
     for (int i = t; i < kHashSlots; i += kBuildThreads)
@@ -182,7 +185,7 @@ __device__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const int64_t* __rest
     // entries per (chunk, row)
     for (int64_t j = e0 + t; j < e1; j += kBuildThreads) {
         if (code[j] == 0xFFFFu) continue;
-        const int h = hash_find(S.keys, (uint32_t)col[j] >> 4);
+        const int h = hash_find(S.keys, (uint32_t)col[j] >> LS);
         if (h < 0) {
             atomicOr(&S.flags, 8);
             continue;
@@ -222,7 +225,7 @@ __device__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const int64_t* __rest
 }
 
 // Pass 1: per group, the distinct lines, the chunks and the padded entries.
-template <int TG>
+template <int TG, int LS>
 __global__ void __launch_bounds__(kBuildThreads)
 k_tile_count(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
              const uint16_t* __restrict__ code, int64_t* __restrict__ n_lines, int64_t* __restrict__ n_entries,
@@ -230,7 +233,7 @@ k_tile_count(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __r
 {
     __shared__ TilePlanLds<TG> S;
     const int64_t g = blockIdx.x;
-    tile_plan(S, dim, rowptr, col, code, g);
+    tile_plan<TG, LS>(S, dim, rowptr, col, code, g);
     if (threadIdx.x == 0) {
         n_lines[g] = S.n_lines;
         n_chunks[g] = S.n_chunks;
@@ -240,18 +243,20 @@ k_tile_count(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __r
 }
 
 // Pass 2: the line list, the chunk table (T per chunk, then the chunk line
-// starts), the entries and the diagonal.
-template <int TG>
+// starts), the entries and the diagonal (W doubles per value: 1 real, 2
+// complex; a complex line holds 8 values, so its word field uses 3 bits).
+template <int TG, int W>
 __global__ void __launch_bounds__(kBuildThreads)
 k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
             const uint16_t* __restrict__ code, const double* __restrict__ diag, const int64_t* __restrict__ base,
             int64_t groups, uint32_t* __restrict__ entries, uint32_t* __restrict__ lines,
             uint32_t* __restrict__ meta, double* __restrict__ tdiag, int* __restrict__ error)
 {
+    constexpr int LS = W == 2 ? 3 : 4;
     __shared__ TilePlanLds<TG> S;
     const int t = threadIdx.x;
     const int64_t g = blockIdx.x;
-    tile_plan(S, dim, rowptr, col, code, g);
+    tile_plan<TG, LS>(S, dim, rowptr, col, code, g);
     const int64_t r0 = g * TG, r1 = min(r0 + TG, dim);
     const int nrows = (int)(r1 - r0);
     const int64_t eb = base[g], en = base[g + 1] - eb;
@@ -276,7 +281,7 @@ k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __re
         if (q == 0xFFFFu) continue;
         const int r = row_of(S, nrows, j);
         const uint32_t cj = (uint32_t)col[j];
-        const int h = hash_find(S.keys, cj >> 4);
+        const int h = hash_find(S.keys, cj >> LS);
         if (h < 0) continue;  // flagged by the plan
         const int rank = S.rank_of[h];
         const int c = S.chunk_of[rank];
@@ -287,9 +292,10 @@ k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __re
             continue;
         }
         const uint32_t code20 = ((q >> 15) << 19) | (q & 0x7FFFu);
-        entries[eb + pos] = ((uint32_t)(rank - (int)S.ls[c]) << 24) | ((cj & 15u) << 20) | code20;
+        entries[eb + pos] = ((uint32_t)(rank - (int)S.ls[c]) << 24) | ((cj & ((1u << LS) - 1u)) << 20) | code20;
     }
-    for (int r = t; r < nrows; r += kBuildThreads) tdiag[r0 + r] = S.dpos[r] >= 0 ? diag[r0 + r] : 0.0;
+    for (int i = t; i < nrows * W; i += kBuildThreads)
+        tdiag[r0 * W + i] = S.dpos[i / W] >= 0 ? diag[r0 * W + i] : 0.0;
 }
 
 // y = H x on the tiled layout: one block of kSpmvThreads per group of TG rows.  The
@@ -417,6 +423,118 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
     if (row < dim && sub == 0) y[row] = fma(tdiag[row], x[row], acc);
 }
 
+// y = H x for a complex (Hermitian) matrix on the tiled layout.  The same
+// schedule as k_spmv_tiled with complex values: a 128-byte line holds 8 of
+// them (entry word field bits 20-22), the staged lines are the same 16-byte
+// pieces, and the dictionary (complex, up to 32767 values, 512 KiB) is read
+// in place from L2 instead of LDS.
+template <int TG>
+__global__ void __launch_bounds__(kSpmvThreads)
+k_spmv_tiled_c(int64_t dim, int64_t groups, const int64_t* __restrict__ base, const uint32_t* __restrict__ entries,
+               const uint32_t* __restrict__ lines, const uint32_t* __restrict__ meta, const double2* __restrict__ tdiag,
+               const double2* __restrict__ dict, int x_aligned, int64_t win, const double2* __restrict__ x2,
+               double2* __restrict__ y)
+{
+    constexpr int PER = (kTC * 8) / kSpmvThreads;
+    constexpr int kMeta = kMaxChunks * (TG + 2) + 1;
+    constexpr int kTPR = kSpmvThreads / TG;
+    __shared__ double xs[kTC * kLineWords];
+    __shared__ uint32_t ids[kMaxLines];
+    __shared__ uint32_t sm[kMeta];
+    const double* x = reinterpret_cast<const double*>(x2);
+    const int64_t dimd = 2 * dim;  // x as doubles
+    const int64_t g = xcd_window(blockIdx.x, groups, win);
+    const int64_t lb = base[groups + 1 + g], le = base[groups + 1 + g + 1];
+    const int64_t mb = base[2 * (groups + 1) + g], me = base[2 * (groups + 1) + g + 1];
+    const int nmeta = (int)min(me - mb, (int64_t)kMeta);
+    const int nch = (nmeta - 1) / (TG + 2);
+    const uint32_t* ent = entries + base[g];
+    const int r = threadIdx.x / kTPR, sub = threadIdx.x % kTPR;
+    const int64_t row = g * TG + r;
+    const int nl = (int)min(le - lb, (int64_t)kMaxLines);
+    for (int i = threadIdx.x; i < nl; i += kSpmvThreads) ids[i] = lines[lb + i];
+    for (int i = threadIdx.x; i < nmeta; i += kSpmvThreads) sm[i] = meta[mb + i];
+    __syncthreads();
+    const uint32_t* T = sm;
+    const uint32_t* LS = sm + nch * (TG + 1);
+    // one chunk ahead: the complex accumulator and values leave no registers
+    // for a second one at 8 waves per SIMD
+    double2 preA[PER];
+    uint32_t eA[kU];
+    auto fetch = [&](int c, double2 (&pre)[PER], uint32_t (&e)[kU]) {
+        const int l0 = (int)LS[c], n_here = (int)LS[c + 1] - l0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = threadIdx.x + k * kSpmvThreads;
+            pre[k] = make_double2(0.0, 0.0);
+            if (i < n_here * 8) {
+                const int64_t xi = (int64_t)ids[l0 + (i >> 3)] * kLineWords + (i & 7) * 2;
+                if (x_aligned && xi + 1 < dimd) pre[k] = *reinterpret_cast<const double2*>(x + xi);
+                else pre[k] = make_double2(xi < dimd ? x[xi] : 0.0, xi + 1 < dimd ? x[xi + 1] : 0.0);
+            }
+        }
+        const uint32_t j0 = T[c * (TG + 1) + r] + sub, je = T[c * (TG + 1) + r + 1];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint32_t j = j0 + u * kTPR;
+            e[u] = j < je ? ent[j] : kEmpty;
+        }
+    };
+    double2 acc = make_double2(0.0, 0.0);
+    auto consume = [&](const uint32_t (&b)[kU]) {
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (b[u] == kEmpty) continue;
+            double2 v = dict[b[u] & 0x7FFFFu];
+            if (b[u] & 0x80000u) v = make_double2(-v.x, -v.y);
+            const double2 xv = *reinterpret_cast<const double2*>(&xs[(b[u] >> 24) * kLineWords + ((b[u] >> 20) & 7u) * 2]);
+            acc.x = fma(v.x, xv.x, acc.x);
+            acc.x = fma(-v.y, xv.y, acc.x);
+            acc.y = fma(v.x, xv.y, acc.y);
+            acc.y = fma(v.y, xv.x, acc.y);
+        }
+    };
+    auto step = [&](int c, double2 (&pre)[PER], uint32_t (&e)[kU]) {
+        __syncthreads();  // chunk c-1 consumed
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = threadIdx.x + k * kSpmvThreads;
+            *reinterpret_cast<double2*>(&xs[(i >> 3) * kLineWords + (i & 7) * 2]) = pre[k];
+        }
+        uint32_t cur[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) cur[u] = e[u];
+        __syncthreads();
+        if (c + 1 < nch) fetch(c + 1, pre, e);
+        consume(cur);
+        const uint32_t je = T[c * (TG + 1) + r + 1];
+        for (uint32_t j0 = T[c * (TG + 1) + r] + sub + kTPR * kU; j0 < je; j0 += kTPR * kU) {
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const uint32_t j = j0 + u * kTPR;
+                cur[u] = j < je ? ent[j] : kEmpty;
+            }
+            consume(cur);
+        }
+    };
+    if (nch > 0) fetch(0, preA, eA);
+    for (int c = 0; c < nch; ++c) step(c, preA, eA);
+#pragma unroll
+    for (int m = kTPR / 2; m > 0; m >>= 1) {
+        acc.x += __shfl_xor(acc.x, m, 64);
+        acc.y += __shfl_xor(acc.y, m, 64);
+    }
+    if (row < dim && sub == 0) {
+        const double2 d = tdiag[row], xr = x2[row];
+        double2 out;
+        out.x = fma(d.x, xr.x, acc.x);
+        out.x = fma(-d.y, xr.y, out.x);
+        out.y = fma(d.x, xr.y, acc.y);
+        out.y = fma(d.y, xr.x, out.y);
+        y[row] = out;
+    }
+}
+
 // Packs 4-byte entries (slot 24-31, word 20-23, sign 19, index 0-18) into
 // 3-byte entries (slot 16-23, word 12-15, sign 11, index 0-10, little endian):
 // four entries per thread, 16 bytes in, 12 bytes out.  The index fits 11 bits
@@ -449,14 +567,15 @@ static int tiled_pick_window(hd_hamiltonian* h)
     h->tg_win = 0;
     if (h->dim == 0) return HD_OK;
     double *x = nullptr, *y = nullptr;
-    HD_CHECK(dalloc_t(&x, (size_t)h->dim, "window autotune x"));
-    int st = dalloc_t(&y, (size_t)h->dim, "window autotune y");
+    const size_t n = (size_t)h->dim * (h->is_complex ? 2 : 1);  // doubles per vector
+    HD_CHECK(dalloc_t(&x, n, "window autotune x"));
+    int st = dalloc_t(&y, n, "window autotune y");
     if (st != HD_OK) {
         dfree(x);
         return st;
     }
     HipErrors q;
-    HD_ENQ(q, hipMemsetAsync(x, 0, (size_t)h->dim * sizeof(double), h->stream));
+    HD_ENQ(q, hipMemsetAsync(x, 0, n * sizeof(double), h->stream));
     st = q.status();
     const int64_t cands[] = {0, 16, 64, 256, (h->tg_groups + kXcds - 1) / kXcds};
     float best = 1e30f;
@@ -499,8 +618,12 @@ static int tiled_build_tg(hd_hamiltonian* h, int entry_bytes)
     int flags[2] = {0, 0};
     if (rc == HD_OK) {
         HD_ENQ(q, hipMemsetAsync(d_flag, 0, 2 * sizeof(int), st));
-        hipLaunchKernelGGL(k_tile_count<TG>, dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim, h->d_rowptr,
-                           h->d_col, h->d_code, d_nl, d_ne, d_nc, d_flag);
+        if (h->is_complex)
+            hipLaunchKernelGGL((k_tile_count<TG, 3>), dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim,
+                               h->d_rowptr, h->d_col, h->d_code, d_nl, d_ne, d_nc, d_flag);
+        else
+            hipLaunchKernelGGL((k_tile_count<TG, 4>), dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim,
+                               h->d_rowptr, h->d_col, h->d_code, d_nl, d_ne, d_nc, d_flag);
         HD_ENQ(q, hipGetLastError());
         HD_ENQ(q, hipMemcpyAsync(nl.data(), d_nl, G * sizeof(int64_t), hipMemcpyDeviceToHost, st));
         HD_ENQ(q, hipMemcpyAsync(ne.data(), d_ne, G * sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -536,13 +659,18 @@ static int tiled_build_tg(hd_hamiltonian* h, int entry_bytes)
     if (rc == HD_OK) rc = dalloc_t(&h->d_tg_entries, (size_t)std::max<int64_t>(h->tg_entries, 1), "tile entries");
     if (rc == HD_OK) rc = dalloc_t(&h->d_tg_lines, (size_t)std::max<int64_t>(h->tg_lines, 1), "tile lines");
     if (rc == HD_OK) rc = dalloc_t(&h->d_tg_offs, (size_t)std::max<int64_t>(h->tg_offs, 1), "tile chunk tables");
-    if (rc == HD_OK) rc = dalloc_t(&h->d_tg_diag, (size_t)h->dim, "tile diagonal");
+    const int W = h->is_complex ? 2 : 1;  // doubles per value
+    if (rc == HD_OK) rc = dalloc_t(&h->d_tg_diag, (size_t)h->dim * W, "tile diagonal");
     if (rc == HD_OK) {
         HD_ENQ(q, hipMemcpyAsync(h->d_tg_base, base.data(), base.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
         HD_ENQ(q, hipMemsetAsync(d_flag + 1, 0, sizeof(int), st));
-        hipLaunchKernelGGL(k_tile_fill<TG>, dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim, h->d_rowptr, h->d_col,
-                           h->d_code, static_cast<const double*>(h->d_diag), h->d_tg_base, G, h->d_tg_entries,
-                           h->d_tg_lines, h->d_tg_offs, h->d_tg_diag, d_flag + 1);
+#define HD_TF(WV)                                                                                                  \
+    hipLaunchKernelGGL((k_tile_fill<TG, WV>), dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim, h->d_rowptr,    \
+                       h->d_col, h->d_code, static_cast<const double*>(h->d_diag), h->d_tg_base, G, h->d_tg_entries, \
+                       h->d_tg_lines, h->d_tg_offs, h->d_tg_diag, d_flag + 1)
+        if (W == 2) HD_TF(2);
+        else HD_TF(1);
+#undef HD_TF
         HD_ENQ(q, hipGetLastError());
         HD_ENQ(q, hipMemcpyAsync(flags + 1, d_flag + 1, sizeof(int), hipMemcpyDeviceToHost, st));
         HD_ENQ(q, hipStreamSynchronize(st));
@@ -582,7 +710,7 @@ static int tiled_build_tg(hd_hamiltonian* h, int entry_bytes)
     h->tg_rows = TG;
     h->tg_entry_bytes = entry_bytes;
     h->bytes += (size_t)h->tg_entries * entry_bytes + (size_t)h->tg_lines * 4 + (size_t)h->tg_offs * 4 + base.size() * 8 +
-                (size_t)h->dim * 8;
+                (size_t)h->dim * 8 * W;
     return tiled_pick_window(h);
 }
 
@@ -592,6 +720,8 @@ int tiled_build(hd_hamiltonian* h, int rows_per_group, int entry_bytes)
         return fail(HD_ERR_ARG, "rows per group must be 0 (automatic), 64 or 128");
     if (entry_bytes != 0 && entry_bytes != 3 && entry_bytes != 4)
         return fail(HD_ERR_ARG, "entry bytes must be 0 (automatic), 3 or 4");
+    if (entry_bytes == 3 && h->is_complex)
+        return fail(HD_ERR_UNSUPPORTED, "3-byte entries are implemented for real matrices");
     if (entry_bytes == 3 && h->ndict > 2047)
         return fail(HD_ERR_UNSUPPORTED, "3-byte entries need a dictionary of at most 2047 values");
     // 4 by default: the packed entries cut HBM bytes by a quarter but need two
@@ -599,10 +729,9 @@ int tiled_build(hd_hamiltonian* h, int rows_per_group, int entry_bytes)
     // against 3.53 ms per product)
     if (entry_bytes == 0) entry_bytes = 4;
     if (!h->d_code) return fail(HD_ERR_ARG, "the tiled layout is built from the coded values (value_mask & 4)");
-    if (h->is_complex) return fail(HD_ERR_UNSUPPORTED, "the tiled layout is implemented for real matrices");
     if (h->storage != HD_STORAGE_FULL) return fail(HD_ERR_ARG, "the tiled layout needs full storage");
     if (h->ndict > (1 << 19)) return fail(HD_ERR_UNSUPPORTED, "dictionary too large for the tiled layout");
-    if ((size_t)h->ndict * 8 + sizeof(double) * kTC * kLineWords +
+    if (!h->is_complex && (size_t)h->ndict * 8 + sizeof(double) * kTC * kLineWords +
             sizeof(uint32_t) * (kMaxLines + kMaxChunks * (128 + 2) + 1) >
         80 * 1024)
         return fail(HD_ERR_UNSUPPORTED, "dictionary too large for LDS next to the line buffer");
@@ -637,8 +766,20 @@ int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
 {
     if (!h->d_tg_base) return fail(HD_ERR_ARG, "no tiled layout (hd_ham_tile)");
     if (h->dim == 0) return HD_OK;
-    const size_t lds = sizeof(double) * (size_t)std::max(h->ndict, 1);
     const int aligned = ((uintptr_t)x & 15) == 0 ? 1 : 0;
+    if (h->is_complex) {
+#define HD_TC(TG)                                                                                                 \
+    hipLaunchKernelGGL((k_spmv_tiled_c<TG>), dim3((unsigned)h->tg_groups), dim3(kSpmvThreads), 0, s, h->dim,      \
+                       h->tg_groups, h->d_tg_base, h->d_tg_entries, h->d_tg_lines, h->d_tg_offs,                  \
+                       reinterpret_cast<const double2*>(h->d_tg_diag), static_cast<const double2*>(h->d_dict),     \
+                       aligned, h->tg_win, static_cast<const double2*>(x), static_cast<double2*>(y))
+        if (h->tg_rows == 128) HD_TC(128);
+        else HD_TC(64);
+#undef HD_TC
+        HD_HIP(hipGetLastError());
+        return HD_OK;
+    }
+    const size_t lds = sizeof(double) * (size_t)std::max(h->ndict, 1);
 #define HD_TL(TG, PK)                                                                                            \
     hipLaunchKernelGGL((k_spmv_tiled<TG, PK>), dim3((unsigned)h->tg_groups), dim3(kSpmvThreads), lds, s, h->dim, \
                        h->tg_groups, h->d_tg_base, h->d_tg_entries, h->d_tg_lines, h->d_tg_offs, h->d_tg_diag,    \
@@ -661,8 +802,9 @@ int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
 // bases, the dictionary, the diagonal, x once and y.
 size_t tiled_bytes(const hd_hamiltonian* h)
 {
+    const size_t w = h->is_complex ? 16 : 8;  // bytes per value
     return (size_t)h->tg_entries * h->tg_entry_bytes + (size_t)h->tg_offs * 4 + (size_t)h->tg_lines * 4 +
-           (size_t)(3 * (h->tg_groups + 1)) * 8 + (size_t)h->ndict * 8 + (size_t)h->dim * 8 * 3;
+           (size_t)(3 * (h->tg_groups + 1)) * 8 + (size_t)h->ndict * w + (size_t)h->dim * w * 3;
 }
 
 }  // namespace hd

@@ -6,8 +6,9 @@ Single band on a dim_x x dim_y k-grid (k = kx*dim_y + ky), N fermions in the
 total-momentum sector (kx, ky); band energies U(-0.1, 0.1) and complex Gaussian
 momentum-conserving V, antisymmetrised and Hermitised (models.ofl_tables,
 seed 0).  The Hermitian CRS is built with complex FP64, complex FP32 and coded
-values; the lowest nev eigenvalues are solved with each operator (the FP32 one
-with FP64 recurrence and FP64 refinement) and compared.  Prints one JSON line.
+values, plus the tiled layout of the coded values; the lowest nev eigenvalues
+are solved with each operator (the FP32 one with FP64 recurrence and FP64
+refinement) and compared.  Prints one JSON line.
 """
 import argparse
 import json
@@ -15,7 +16,7 @@ import time
 
 import numpy as np
 
-from . import CODED, F32, F64, Hamiltonian, models
+from . import CODED, F32, F64, TILED, Hamiltonian, models
 
 
 def log(*a):
@@ -43,12 +44,18 @@ def main():
     info = ham.info()
     log(f"build {t_build:.3f} s, nnz {info['nnz']}, max row {info['max_row']}, trimmed {info['n_trimmed']}")
     ham.set_lanes(0)
-    sp = {name: ham.spmv_time_ms(p, 10) for name, p in (("c64", F64), ("c32", F32), ("coded", CODED))}
+    t0 = time.perf_counter()
+    tile = ham.tile()
+    t_tile = time.perf_counter() - t0
+    log(f"tiled layout {t_tile:.3f} s: {tile['rows_per_group']}-row groups, block order {tile['block_window']}, "
+        f"{tile['lines'] / tile['entries']:.4f} staged lines per entry")
+    fmts = {"c64": F64, "c32": F32, "coded": CODED, "tiled": TILED}
+    sp = {name: ham.spmv_time_ms(p, 10) for name, p in fmts.items()}
     for k, v in sp.items():
-        log(f"SpMV {k}: {v:.3f} ms ({ham.spmv_bytes({'c64': F64, 'c32': F32, 'coded': CODED}[k]) / v / 1e6:.0f} GB/s)")
+        log(f"SpMV {k}: {v:.3f} ms ({ham.spmv_bytes(fmts[k]) / v / 1e6:.0f} GB/s)")
     res = {}
     ref = None
-    for name, p in (("fp64", F64), ("coded_fp64", CODED), ("c32_mixed", F32)):
+    for name, p in (("fp64", F64), ("coded_fp64", CODED), ("tiled_fp64", TILED), ("c32_mixed", F32)):
         r = ham.lanczos(nev=args.nev, tol=args.tol, precision=p, seed=0, ncv=args.ncv, raise_on_fail=False)
         if ref is None:
             ref = r.evals
@@ -61,9 +68,9 @@ def main():
                     f"{args.nev} lowest eigenvalues, residual {args.tol:g}",
         "dim": basis.dim, "nnz_full": info["nnz"], "basis_s": round(t_basis, 3), "build_s": round(t_build, 3),
         "build_ms": {k: round(info[k], 2) for k in ("t_count_ms", "t_scan_ms", "t_fill_ms", "t_sort_ms")},
+        "tile_s": round(t_tile, 3), "tile": tile,
         "spmv_ms": {k: round(v, 4) for k, v in sp.items()},
-        "spmv_gbs": {k: round(ham.spmv_bytes({'c64': F64, 'c32': F32, 'coded': CODED}[k]) / v / 1e6, 1)
-                     for k, v in sp.items()},
+        "spmv_gbs": {k: round(ham.spmv_bytes(fmts[k]) / v / 1e6, 1) for k, v in sp.items()},
         "solves": res, "eigenvalues": [float(e) for e in ref],
     }
     print(json.dumps(out), flush=True)

@@ -295,9 +295,10 @@ int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes_per_row);
  * the column gathers of SymmetricMatrixVectorMultiply,
  * src/utilities/mathematics/sparse_linear_algebra.hpp:218-312, for the
  * product path): built on the device from the coded values, after which
- * precision HD_TILED is available to hd_spmv and hd_lanczos.  Fails with
- * HD_ERR_UNSUPPORTED for complex matrices or when a group reads more than
- * 5120 distinct 128-byte lines of x.  rows_per_group: 128 or 64, or 0 for
+ * precision HD_TILED is available to hd_spmv and hd_lanczos.  Real and complex
+ * (Hermitian) matrices; a complex line holds 8 values and its dictionary is read
+ * from L2.  Fails with HD_ERR_UNSUPPORTED when a group reads more than 5120
+ * distinct 128-byte lines of x, or for 3-byte entries of a complex matrix.  rows_per_group: 128 or 64, or 0 for
  * 128 with a fallback to 64.  entry_bytes: 4 (line slot, word, sign and a
  * 19-bit dictionary index), 3 (the same with an 11-bit index, for
  * dictionaries of at most 2047 values), or 0 for 4. */
