@@ -181,6 +181,13 @@ def test_dropin_flux_lattice_complex_tables(oracle, tmp_path):
     assert np.array_equal(np.fromfile(pre + ".values", dtype=np.complex128), up.values())
     want = np.linalg.eigvalsh(up.to_dense(len(bas)))[:6]
     assert r["ok"] and np.max(np.abs(np.array(r["eigenvalues"]) - want) / np.maximum(1, np.abs(want))) < 1e-10
+    # the complex tiled operator through the drop-in
+    rt = last_json(run(["--nbr-particles", 5, "--kmax", 12, "--complex", 1, "--ktable", tmp_path / "k.bin",
+                        "--vtable", tmp_path / "v.bin", "--eps", tmp_path / "e.bin", "--sector", "k2d:0,0,3,4",
+                        "--nbr-eigenvalues", 6, "--tol", 1e-11, "--dense-below", 60, "--precision", "tiled"]))
+    say(rt)
+    assert rt["ok"] and rt["method"] == "lanczos"
+    assert np.max(np.abs(np.array(rt["eigenvalues"]) - want) / np.maximum(1, np.abs(want))) < 1e-10
 
 
 @pytest.mark.gpu
