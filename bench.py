@@ -171,8 +171,8 @@ def main():
     ap.add_argument("--skip-learned", action="store_true", help="skip the learned-start solves")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the sharded design is "
-              "described in DESIGN.md and not measured here.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); a multi-GPU run would shard the "
+              "rows across GPUs and is not measured here.", file=sys.stderr)
         sys.exit(2)
 
     import hamiltonian_diagonalization_amd as hd
