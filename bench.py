@@ -276,8 +276,9 @@ def main():
             "operator": {
                 "coded": "exact FP64 values in the coded CRS (int32 column + 16-bit dictionary code)",
                 "f64": "FP64 CRS (int64 row pointers, int32 columns, FP64 values)",
-                "tiled": "exact FP64 values, tiled coded layout (64-row groups, x lines staged in LDS, "
-                         "4-byte entries)"}[key],
+                "tiled": f"exact FP64 values, tiled coded layout ({tile['rows_per_group']}-row groups, x lines "
+                         f"staged in LDS, {tile['entry_bytes']}-byte entries, block order "
+                         f"{tile['block_window']} groups per XCD)"}[key],
         },
         "solve_s": {op_name: round(step_s, 4)},
         "matvecs": {op_name: res.matvecs},
