@@ -1,6 +1,7 @@
 """A short fixed workload for hardware-counter runs of the SpMV kernels.
 
 python -m hamiltonian_diagonalization_amd.profile_spmv [--n 12 --two-s 33 --iters 3] [--orders]
+python -m hamiltonian_diagonalization_amd.profile_spmv --ofl 6 5 10   (flux lattice dim_x dim_y N, sector (0,0))
 
 Builds the N=12, 2S=33 L_z=0 Coulomb sphere Hamiltonian with FP64, FP32 and
 coded values and the tiled layout (4-byte entries, then the packed 3-byte
@@ -24,12 +25,17 @@ def main():
     ap.add_argument("--iters", type=int, default=3)
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--rows", type=int, default=0, help="rows per group of the tiled layout (0 = automatic)")
+    ap.add_argument("--ofl", type=int, nargs=3, metavar=("DIM_X", "DIM_Y", "N"),
+                    help="the complex flux-lattice model (seed 0, sector (0,0)) instead of the sphere")
     ap.add_argument("--orders", action="store_true",
                     help="also time the tiled product under each block order (groups per XCD window)")
     args = ap.parse_args()
-    basis = FockBasis(Sector.lz(0, args.two_s), args.n, args.two_s + 1)
-    ham = Hamiltonian(basis, models.sphere_tables(args.two_s, models.coulomb_list(args.two_s)),
-                      storage=STORAGE_FULL, values=(F64, F32, CODED))
+    if args.ofl:
+        basis, tables = models.ofl_problem(args.ofl[0], args.ofl[1], args.ofl[2], 0, 0, seed=0)
+    else:
+        basis = FockBasis(Sector.lz(0, args.two_s), args.n, args.two_s + 1)
+        tables = models.sphere_tables(args.two_s, models.coulomb_list(args.two_s))
+    ham = Hamiltonian(basis, tables, storage=STORAGE_FULL, values=(F64, F32, CODED))
     tile = ham.tile(args.rows)
     ham.set_lanes(0)
     out = {"dim": basis.dim, "nnz": ham.nnz, "lanes": ham.lanes(), "iters": args.iters, "tile": tile, "spmv": {}}
@@ -48,7 +54,7 @@ def main():
         ham.tile_order(chosen)
         print(f"[profile] autotuned order: {chosen}", flush=True)
     # the packed 3-byte entries (k_spmv_tiled<128, true>) when the dictionary allows them
-    if tile["entry_bytes"] == 4:
+    if tile["entry_bytes"] == 4 and not args.ofl:
         try:
             t3 = ham.tile(args.rows, 3)
         except Exception as e:  # dictionary above 2047 values
