@@ -32,7 +32,7 @@ HD_OK, HD_ERR_ARG, HD_ERR_HIP, HD_ERR_NOMEM, HD_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 HD_ERR_NOT_CONVERGED, HD_ERR_EMPTY = 5, 6
 SECTOR_NONE, SECTOR_LZ, SECTOR_K2D, SECTOR_K1D, SECTOR_LZ2 = 0, 1, 2, 3, 4
 LAYOUT_SPHERE, LAYOUT_OFL = 0, 1
-F64, F32, CODED, TILED = 0, 1, 2, 3
+F64, F32, CODED, TILED, TILED_X32 = 0, 1, 2, 3, 4
 STORAGE_FULL, STORAGE_UPPER_REF, STORAGE_MATRIX_FREE = 0, 1, 2
 START_RANDOM, START_USER = 0, 1
 ZERO_TOL = 1e-14  # _SPARSE_ZERO_TOL_ (src/hamiltonians/hamiltonian_data.hpp:54)
@@ -69,7 +69,7 @@ class _LanczosOpts(ctypes.Structure):
                 ("max_restarts", ctypes.c_int32), ("tol", ctypes.c_double),
                 ("start", ctypes.c_int32), ("seed", ctypes.c_uint64),
                 ("d_start", ctypes.c_void_p), ("precision", ctypes.c_int32),
-                ("f32_switch_tol", ctypes.c_double), ("want_vectors", ctypes.c_int32),
+                ("f32_budget", ctypes.c_double), ("want_vectors", ctypes.c_int32),
                 ("block", ctypes.c_int32), ("full_reorth", ctypes.c_int32)]
 
 
@@ -561,19 +561,23 @@ class Hamiltonian:
         return tuple(x.value or 0 for x in p)
 
     def vec_dtype(self, precision=F64):
+        """dtype of x for a product in `precision` (y is FP64 except for F32)."""
         if precision in (F64, CODED, TILED):
             return np.complex128 if self.is_complex else np.float64
         return np.complex64 if self.is_complex else np.float32
 
+    def out_dtype(self, precision=F64):
+        return self.vec_dtype(F32 if precision == F32 else F64)
+
     def spmv(self, x: np.ndarray, precision=F64) -> np.ndarray:
         """y = H x (host arrays in, host array out; device SpMV in between)."""
-        dt = self.vec_dtype(precision)
+        dt, dto = self.vec_dtype(precision), self.out_dtype(precision)
         xx = np.ascontiguousarray(x, dtype=dt)
         dx = DeviceBuffer.from_array(xx)
-        dy = DeviceBuffer(xx.nbytes)
+        dy = DeviceBuffer(self.dim * np.dtype(dto).itemsize)
         _check(lib().hd_spmv(self.h, precision, dx.ptr, dy.ptr))
         _check(lib().hd_synchronize())
-        return dy.to_array(dt, self.dim)
+        return dy.to_array(dto, self.dim)
 
     def spmv_device(self, dx: DeviceBuffer, dy: DeviceBuffer, precision=F64):
         _check(lib().hd_spmv(self.h, precision, dx.ptr, dy.ptr))
@@ -598,6 +602,8 @@ class Hamiltonian:
         z = 2 if self.is_complex else 1
         if precision == TILED:
             return self.tile_info()["spmv_bytes"]
+        if precision == TILED_X32:  # x read in FP32 instead of FP64
+            return self.tile_info()["spmv_bytes"] - 4 * self.dim
         if precision == CODED:
             # column + 16-bit code per entry; rowptr, diagonal, x, y per row
             return self.nnz * 6 + self.dim * (8 + 3 * 8 * z)
@@ -619,14 +625,16 @@ class Hamiltonian:
         return lanes.value, threads.value
 
     def lanczos(self, nev=5, tol=1e-10, ncv=0, precision=F64, seed=0, start=None,
-                f32_switch_tol=0.0, max_restarts=0, raise_on_fail=True, block=1,
+                f32_budget=0.0, max_restarts=0, raise_on_fail=True, block=1,
                 full_reorth=False) -> LanczosResult:
-        """block > 1 selects the block Lanczos (4, 8 or 16 vectors per SpMM);
-        full_reorth orthogonalises every new vector against the whole basis
-        (default: partial reorthogonalisation)."""
+        """precision F64, CODED or TILED: that exact operator throughout; F32:
+        mixed, restart cycles in FP64 until the FP32-input operator's rounding
+        fits f32_budget * tol (0 = 1).  block > 1 selects the block Lanczos
+        (4, 8 or 16 vectors per SpMM); full_reorth orthogonalises every new
+        vector against the whole basis (default: partial reorthogonalisation)."""
         o = _LanczosOpts()
         o.nev, o.ncv, o.max_restarts, o.tol = nev, ncv, max_restarts, tol
-        o.seed, o.precision, o.f32_switch_tol, o.want_vectors = seed, precision, f32_switch_tol, 1
+        o.seed, o.precision, o.f32_budget, o.want_vectors = seed, precision, f32_budget, 1
         o.block, o.full_reorth = block, int(bool(full_reorth))
         dstart = None
         if start is None:

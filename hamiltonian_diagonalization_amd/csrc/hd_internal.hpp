@@ -345,7 +345,8 @@ int matrix_free_launch(const hd_hamiltonian* h, const void* x, void* y, hipStrea
 // Tiled coded layout (hd_tiled.hip).
 int tiled_build(hd_hamiltonian* h, int rows_per_group, int entry_bytes);
 void tiled_free(hd_hamiltonian* h);
-int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s);
+// x32: x in FP32 (HD_TILED_X32), y in FP64; real matrices.
+int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s, bool x32 = false);
 size_t tiled_bytes(const hd_hamiltonian* h);
 // Cyclic Jacobi on a symmetric row-major matrix (destroyed): ascending
 // eigenvalues w, eigenvectors as the columns of Z (hd_lanczos.hip).

@@ -16,12 +16,25 @@
 // tridiagonal-plus-arrow matrix is diagonalised by cyclic Jacobi rotations on
 // the host.
 //
-// Mixed precision (precision = HD_F32): the operator is applied with FP32
-// values and FP32 vectors (x rounded from the FP64 Lanczos vector, y widened
-// back) until every wanted Ritz residual is below f32_switch_tol; the sum of
-// those Ritz vectors then restarts the iteration with the FP64 operator, which
-// takes the residuals to tol.  All reductions are FP64 and deterministic
-// (per-block partials reduced in a fixed order).
+// Mixed precision (precision = HD_F32) chooses the operator per restart
+// cycle.  An operator error F_j in the product of basis vector v_j enters the
+// residual of a Ritz vector y = sum_j s_j v_j as sum_j s_j F_j, which the
+// recurrence never sees (Simoncini, SIAM J. Numer. Anal. 43 (2005) 1155:
+// products may be inexact where the Ritz coefficients are small).  The kept
+// Ritz vectors carry O(1) coefficients on every vector of the cycles that made
+// them, so the early cycles use the exact FP64 operator; once the wanted Ritz
+// vectors' coefficients on a cycle's new vectors are small, the FP32-input
+// operator (HD_TILED_X32, exact for x rounded to FP32, or the FP32-value CRS)
+// takes the cycle.  The rounding such a cycle adds to the residual of wanted
+// pair i is estimated by eps32 |T| |s_i,new| (eps32 the operator's relative
+// rounding, |T| the largest projected row, s_i,new the pair's coefficients on
+// the cycle's new vectors); a cycle runs in FP32 only while the sum of these
+// estimates, with the next cycle's predicted coefficients, stays within
+// f32_budget * tol * max(1, |theta_i|).  Convergence is the usual Ritz test;
+// the eigenpairs are then checked with the FP64 operator, and hd_lanczos
+// repeats the solve with the FP64 operator should a residual miss tol.  All
+// reductions are FP64 and deterministic (per-block partials reduced in a fixed
+// order).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -439,15 +452,13 @@ struct Lanczos {
 
     int grid() const { return nb; }
 
-    int alloc(bool need32)
+    int alloc(bool need_x32, bool need_y32)
     {
         HD_CHECK(dalloc_t(&V, (size_t)(m + 1) * n, "Lanczos basis"));
         HD_CHECK(dalloc_t(&W, (size_t)n, "Lanczos work vector"));
         HD_CHECK(dalloc_t(&tmp, (size_t)m * n, "restart buffer"));
-        if (need32) {
-            HD_CHECK(dalloc_t(&x32, (size_t)n, "FP32 x"));
-            HD_CHECK(dalloc_t(&y32, (size_t)n, "FP32 y"));
-        }
+        if (need_x32) HD_CHECK(dalloc_t(&x32, (size_t)n, "FP32 x"));
+        if (need_y32) HD_CHECK(dalloc_t(&y32, (size_t)n, "FP32 y"));
         HD_CHECK(dalloc_t(&partial, (size_t)(kNV + 1) * nb, "dot partials"));
         HD_CHECK(dalloc_t(&coef, kNormSlot + 8, "coefficients"));
         HD_CHECK(dalloc_t(&dpart, (size_t)nb, "norm partials"));
@@ -463,10 +474,14 @@ struct Lanczos {
     {
         timer.start(st);
         int s;
-        if (prec == HD_F32) {
+        if (prec == HD_F32 || prec == HD_TILED_X32) {
             hipLaunchKernelGGL(k_to_f32<C>, dim3(nb), dim3(kVB), 0, st, n, x, x32);
-            s = spmv_launch(h, HD_F32, x32, y32, st);
-            hipLaunchKernelGGL(k_to_f64<C>, dim3(nb), dim3(kVB), 0, st, n, y32, y);
+            if (prec == HD_F32) {
+                s = spmv_launch(h, HD_F32, x32, y32, st);
+                hipLaunchKernelGGL(k_to_f64<C>, dim3(nb), dim3(kVB), 0, st, n, y32, y);
+            } else {
+                s = spmv_launch(h, HD_TILED_X32, x32, y, st);
+            }
             ++mv32;
         } else {
             s = spmv_launch(h, prec, x, y, st);
@@ -664,20 +679,31 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
     HD_CHECK(device_props(&n_cu, nullptr));
     L.nb = (int)std::max<int64_t>(1, std::min<int64_t>((h->dim + kVB - 1) / kVB, (int64_t)n_cu * 4));
     const bool mixed = o.precision == HD_F32;
-    if (mixed && !h->d_val32) return fail(HD_ERR_ARG, "FP32 Lanczos needs FP32 values (value_mask & 2)");
+    // the FP32-input operator of the mixed solve: the tiled layout with FP32 x
+    // (exact for FP32 inputs), else the FP32-value CRS (FP32 accumulation)
+    const int prec32 = (h->d_tg_base && !h->is_complex) ? HD_TILED_X32 : HD_F32;
+    if (mixed && prec32 == HD_F32 && !h->d_val32)
+        return fail(HD_ERR_ARG, "FP32 Lanczos needs FP32 values (value_mask & 2) or a real tiled layout");
     const bool free_ = h->storage == HD_STORAGE_MATRIX_FREE;
     if (!free_ && !h->d_val64 && !h->d_code && !h->d_tg_base)
         return fail(HD_ERR_ARG, "Lanczos needs FP64, coded or tiled values for the FP64 operator");
     if (o.precision == HD_TILED && !h->d_tg_base) return fail(HD_ERR_ARG, "no tiled layout (hd_ham_tile)");
+    if (o.precision < HD_F64 || o.precision > HD_TILED) return fail(HD_ERR_ARG, "unknown Lanczos precision");
+    // the exact FP64 operator: the tiled layout when asked for (or, in a mixed
+    // solve, when built: it is the fastest exact one), else FP64 values, else coded
     L.prec64 = free_ ? HD_F64
-               : o.precision == HD_TILED ? HD_TILED
+               : (o.precision == HD_TILED || (mixed && h->d_tg_base)) ? HD_TILED
                : (o.precision == HD_CODED || !h->d_val64) ? (h->d_code ? HD_CODED : HD_TILED)
                                                           : HD_F64;
-    HD_CHECK(L.alloc(mixed));
+    HD_CHECK(L.alloc(mixed, mixed && prec32 == HD_F32));
     const double tol = o.tol > 0 ? o.tol : 1e-10;
-    const double sw_tol = o.f32_switch_tol > 0 ? o.f32_switch_tol : 1e-6;
+    const double budget = o.f32_budget > 0 ? std::min(o.f32_budget, 1.0) : 1.0;
+    // relative rounding of one FP32-input product: x rounded to FP32, plus
+    // FP32 values and accumulation over a row for the CRS form
+    const double u32 = 5.9604644775390625e-8;  // 2^-24
+    const double eps32 = prec32 == HD_TILED_X32 ? u32 : u32 * (1.0 + std::sqrt((double)std::max<int64_t>(h->max_row, 1)));
     const int max_restarts = o.max_restarts > 0 ? o.max_restarts : 2000;
-    int prec = mixed ? HD_F32 : L.prec64;
+    int prec = L.prec64;
     int kk_keep = nev + (m - nev) / 2;
     kk_keep = std::min(std::max(kk_keep, nev + 1), m - 2 >= nev ? m - 2 : nev);
     kk_keep = std::max(kk_keep, nev);
@@ -722,11 +748,20 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
     std::vector<double> om_prev(m + 1, 0.0), om_cur(m + 1, 0.0), om_next(m + 1, 0.0);
     bool force_next = false;
     int reorth_steps = 0;
+    // mixed precision: bound of the rounding each wanted pair has taken from
+    // FP32 cycles, and its coefficient norm on the previous cycle's new vectors
+    std::vector<double> err32(nev, 0.0), nb_prev(nev, 1.0);
     for (;;) {
+        const bool cyc32 = prec == prec32 && mixed;
+        // A rounding error of the FP32-input product is a vector A*delta with
+        // delta ~ eps32 |x| spread over the sector: its overlap with a basis
+        // vector is ~ eps32 |A| / sqrt(dim).  Such a cycle tolerates overlaps up
+        // to eps32 (counted in err32 below).
+        const double eps_op = cyc32 ? eps + eps32 / std::sqrt((double)std::max<int64_t>(L.n, 1)) : eps;
+        const double om_cycle = cyc32 ? std::max(om_max, eps32) : om_max;
         for (int j = k; j < m; ++j) {
             HD_CHECK(L.apply(L.vec(j), L.W, prec));
             double b;
-            const double eps_op = prec == HD_F32 ? 6e-8 : eps;  // rounding of the operator
             if (j == k) {
                 // first vector of a cycle: its coupling to the kept Ritz
                 // vectors is the arrow, removed by two full CGS passes
@@ -775,7 +810,7 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
                             worst = std::max(worst, std::fabs(om_next[i]));
                         }
                         om_next[j] = eps_op * anorm / bj;
-                        if (worst > om_max) {
+                        if (worst > om_cycle) {
                             full = true;
                             force_next = true;
                         } else {
@@ -823,30 +858,24 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
         if (!sym_eig(m, Tm, w, Y)) return fail(HD_ERR_NOT_CONVERGED, "projected eigenproblem did not converge");
         const double bm = beta[m - 1];
         bool all = true;
+        bool allow32 = mixed;
         for (int i = 0; i < nev; ++i) {
-            res_est[i] = std::fabs(bm * Y[(size_t)(m - 1) * m + i]) / std::max(1.0, std::fabs(w[i]));
-            const double lim = (prec == HD_F32) ? sw_tol : tol;
-            if (!(res_est[i] <= lim)) all = false;
+            const double scale_i = std::max(1.0, std::fabs(w[i]));
+            res_est[i] = std::fabs(bm * Y[(size_t)(m - 1) * m + i]) / scale_i;
+            if (mixed) {
+                // the wanted Ritz vector's coefficients on this cycle's new vectors
+                double nb2 = 0.0;
+                for (int j = k; j < m; ++j) nb2 += Y[(size_t)j * m + i] * Y[(size_t)j * m + i];
+                const double nb = std::sqrt(nb2);
+                if (cyc32) err32[i] += eps32 * anorm * nb;
+                // the next cycle's coefficients, predicted from this cycle's decrease
+                const double pred = nb * std::min(1.0, nb / std::max(nb_prev[i], 1e-300));
+                nb_prev[i] = nb;
+                if (err32[i] + eps32 * anorm * pred > budget * tol * scale_i) allow32 = false;
+            }
+            if (!(res_est[i] <= tol)) all = false;
         }
         ++restarts;
-        if (all && prec == HD_F32) {
-            // FP32 phase done: restart from the sum of the Ritz vectors with the FP64 operator
-            HD_CHECK(L.combine(m, Y, m, nev, L.tmp));
-            // V[0] = sum_i of the nev Ritz vectors
-            HD_HIP(hipMemcpyAsync(L.vec(0), L.tmp, sizeof(T) * L.n, hipMemcpyDeviceToDevice, L.st));
-            for (int i = 1; i < nev; ++i)
-                hipLaunchKernelGGL(k_accumulate<C>, dim3(L.nb), dim3(kVB), 0, L.st, L.n, L.tmp + (size_t)i * L.n,
-                                   L.vec(0));
-            HD_HIP(hipGetLastError());
-            HD_CHECK(L.norm2_of(L.vec(0), &nrm));
-            HD_CHECK(L.scale_into(L.vec(0), 1.0 / std::sqrt(nrm), L.vec(0)));
-            prec = L.prec64;
-            k = 0;
-            theta.clear();
-            arrow.clear();
-            if (restarts >= max_restarts) break;
-            continue;
-        }
         if (all) {
             converged = true;
             break;
@@ -861,6 +890,7 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
         arrow.resize(kk);
         for (int i = 0; i < kk; ++i) arrow[i] = bm * Y[(size_t)(m - 1) * m + i];
         k = kk;
+        if (mixed) prec = allow32 ? prec32 : L.prec64;
     }
     // Ritz vectors and FP64 residuals
     hd_lanczos_state* S = h->lz;
@@ -925,14 +955,17 @@ int hd_lanczos(hd_hamiltonian* h, const hd_lanczos_opts* opts, double* h_evals, 
     };
     hd_lanczos_stats s1{};
     const int rc = run(*opts, &s1);
-    if (rc != HD_OK || s1.converged || opts->full_reorth) {
+    const bool mixed = opts->precision == HD_F32;
+    if ((rc != HD_OK && rc != HD_ERR_NOT_CONVERGED) || s1.converged || (opts->full_reorth && !mixed)) {
         if (stats) *stats = s1;
         return rc;
     }
-    // partial reorthogonalisation met the Ritz estimates but an explicit
-    // residual missed tol: solve again with full reorthogonalisation
+    // the Ritz estimates met tol but an explicit FP64 residual did not: solve
+    // again with the FP64 operator (mixed precision) or with full
+    // reorthogonalisation (partial reorthogonalisation)
     hd_lanczos_opts o2 = *opts;
-    o2.full_reorth = 1;
+    if (mixed) o2.precision = h->d_tg_base ? HD_TILED : (h->d_val64 ? HD_F64 : HD_CODED);
+    else o2.full_reorth = 1;
     hd_lanczos_stats s2{};
     const int rc2 = run(o2, &s2);
     s2.matvecs += s1.matvecs;

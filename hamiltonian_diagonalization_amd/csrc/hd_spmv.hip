@@ -242,8 +242,8 @@ int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y, 
     }
     if (h->storage != HD_STORAGE_FULL)
         return fail(HD_ERR_ARG, "SpMV needs full (both-triangle) storage");
-    if (precision < 0 || precision > 3) return fail(HD_ERR_ARG, "unknown SpMV precision");
-    if (precision == HD_TILED) return tiled_launch(h, x, y, stream);
+    if (precision < 0 || precision > HD_TILED_X32) return fail(HD_ERR_ARG, "unknown SpMV precision");
+    if (precision == HD_TILED || precision == HD_TILED_X32) return tiled_launch(h, x, y, stream, precision == HD_TILED_X32);
     const int si = h->spmv_sched[precision] >= 0 ? h->spmv_sched[precision] : pick_default_sched(h);
     if (h->dim == 0) return HD_OK;
     if (precision == HD_CODED) {
@@ -371,15 +371,17 @@ int hd_spmv_time(hd_hamiltonian* h, int32_t precision, int32_t iters, double* av
 {
     if (!h || !avg_ms || iters <= 0) return fail(HD_ERR_ARG, "hd_spmv_time: bad argument");
     if (!h->lanes_per_row) HD_CHECK(spmv_autotune(h));
-    const size_t es = (precision == HD_F32 ? 4 : 8) * (h->is_complex ? 2 : 1);
+    const size_t z = h->is_complex ? 2 : 1;
+    const size_t ex = (precision == HD_F32 || precision == HD_TILED_X32 ? 4 : 8) * z;
+    const size_t ey = (precision == HD_F32 ? 4 : 8) * z;
     void *x = nullptr, *y = nullptr;
-    HD_CHECK(dalloc(&x, (size_t)h->dim * es, "timing x"));
-    int st = dalloc(&y, (size_t)h->dim * es, "timing y");
+    HD_CHECK(dalloc(&x, (size_t)h->dim * ex, "timing x"));
+    int st = dalloc(&y, (size_t)h->dim * ey, "timing y");
     if (st != HD_OK) {
         dfree(x);
         return st;
     }
-    if (hipMemsetAsync(x, 0, (size_t)h->dim * es, h->stream) != hipSuccess) st = fail(HD_ERR_HIP, "timing fill failed");
+    if (hipMemsetAsync(x, 0, (size_t)h->dim * ex, h->stream) != hipSuccess) st = fail(HD_ERR_HIP, "timing fill failed");
     if (st == HD_OK) st = spmv_launch(h, precision, x, y, h->stream);  // warm
     EventTimer timer;
     timer.start(h->stream);
@@ -434,8 +436,8 @@ int hd_spmv_get_lanes(const hd_hamiltonian* h, int32_t* lanes)
 
 int hd_spmv_get_schedule(const hd_hamiltonian* h, int32_t precision, int32_t* lanes, int32_t* threads)
 {
-    if (!h || !lanes || !threads || precision < 0 || precision > 3) return fail(HD_ERR_ARG, "bad argument");
-    if (precision == HD_TILED) {  // one schedule: 1024 threads per row group
+    if (!h || !lanes || !threads || precision < 0 || precision > HD_TILED_X32) return fail(HD_ERR_ARG, "bad argument");
+    if (precision == HD_TILED || precision == HD_TILED_X32) {  // one schedule: 1024 threads per row group
         *lanes = h->tg_rows ? 1024 / h->tg_rows : 0;
         *threads = 1024;
         return HD_OK;

@@ -6,8 +6,10 @@
 // (coefficients with |CG| <= 1e-12 are not stored and read as zero).  The
 // reference obtains CG coefficients from a numerical SU(N) algorithm; here each
 // coefficient is evaluated exactly with Racah's closed form in big-integer
-// arithmetic and rounded once to double, and the table is accumulated in the
-// reference's floating-point operation order.
+// arithmetic and rounded once to double (correctly rounded: the square root of
+// the exact rational CG^2 is checked against the squared midpoints of its
+// neighbouring doubles), and the table is accumulated in the reference's
+// floating-point operation order.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -127,6 +129,68 @@ long double to_ld(const Big& a, long& exp2)
     return m;
 }
 
+Big shl(const Big& a, long bits)  // a * 2^bits, bits >= 0
+{
+    if (a.zero() || bits == 0) return a;
+    Big r;
+    const size_t limbs = (size_t)(bits / 32);
+    const int sh = (int)(bits % 32);
+    r.d.assign(limbs, 0);
+    uint32_t carry = 0;
+    for (const uint32_t x : a.d) {
+        r.d.push_back(sh ? (x << sh) | carry : x);
+        carry = sh ? x >> (32 - sh) : 0;
+    }
+    if (carry) r.d.push_back(carry);
+    r.trim();
+    return r;
+}
+
+// Sign of num/den - ((a + b) / 2)^2 for positive finite doubles a, b, exactly:
+// with a = ma 2^xa, b = mb 2^xb (53-bit integer mantissas) and e0 = min(xa, xb),
+// ((a + b)/2)^2 = (ma 2^(xa-e0) + mb 2^(xb-e0))^2 2^(2 e0 - 2).
+int cmp_square_of_mid(const Big& num, const Big& den, double a, double b)
+{
+    int ea, eb;
+    const double fa = std::frexp(a, &ea), fb = std::frexp(b, &eb);
+    const long xa = ea - 53, xb = eb - 53, e0 = std::min(xa, xb);
+    const Big s = add(shl(Big((uint64_t)std::ldexp(fa, 53)), xa - e0), shl(Big((uint64_t)std::ldexp(fb, 53)), xb - e0));
+    Big rhs = mul(den, mul(s, s));
+    Big lhs = num;
+    const long sh = 2 * e0 - 2;
+    if (sh >= 0) rhs = shl(rhs, sh);
+    else lhs = shl(lhs, -sh);
+    return cmp(lhs, rhs);
+}
+
+// sqrt(num/den) rounded to the nearest double (ties to even): a long double
+// first guess, moved by whole ulps while num/den lies beyond the square of the
+// midpoint to a neighbour.
+double sqrt_ratio_rounded(const Big& num, const Big& den)
+{
+    long en, ed;
+    const long double mn = to_ld(num, en), md = to_ld(den, ed);
+    double v = (double)std::sqrt(std::ldexp(mn / md, (int)(en - ed)));
+    auto odd = [](double x) {
+        int e;
+        return (((uint64_t)std::ldexp(std::frexp(x, &e), 53)) & 1u) != 0;
+    };
+    for (;;) {
+        const double lo = std::nextafter(v, 0.0);
+        if (!(lo > 0.0)) break;
+        const int c = cmp_square_of_mid(num, den, lo, v);
+        if (c < 0 || (c == 0 && odd(v))) v = lo;
+        else break;
+    }
+    for (;;) {
+        const double hi = std::nextafter(v, HUGE_VAL);
+        const int c = cmp_square_of_mid(num, den, v, hi);
+        if (c > 0 || (c == 0 && odd(v))) v = hi;
+        else break;
+    }
+    return v;
+}
+
 void mul_fact(Big& x, int n)  // x *= n!
 {
     for (int i = 2; i <= n; ++i) x.mul_small((uint32_t)i);
@@ -184,10 +248,7 @@ double clebsch_gordan(int j1, int m1, int j2, int m2, int J, int M)
     Big den(1);
     mul_fact(den, (j1 + j2 + J) / 2 + 1);
     den = mul(den, mul(P, P));
-    long en, ed;
-    const long double mn = to_ld(num, en), md = to_ld(den, ed);
-    const long double ratio = std::ldexp(mn / md, (int)(en - ed));
-    const double v = (double)std::sqrt(ratio);
+    const double v = sqrt_ratio_rounded(num, den);
     return sgn > 0 ? v : -v;
 }
 

@@ -304,18 +304,49 @@ k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __re
 // of chunk c+1 are already in flight into registers (double buffering through
 // the register file); only rows with more than kTPR * kU entries in a chunk
 // issue further loads inside it.
-template <int TG, bool PK>
+//
+// XT = float stages x in FP32 (a line's 16 values are 64 bytes) and keeps the
+// FP64 dictionary, FP64 products and FP64 accumulation, so y is the same FP64
+// sum as with XT = double for x rounded to FP32: the product is exact for
+// FP32 inputs.
+template <typename XT> struct XPiece;
+template <> struct XPiece<double> {
+    using P = double2;
+    static constexpr int kWords = 2;
+    HD_DEV static P zero() { return make_double2(0.0, 0.0); }
+    HD_DEV static P load(const double* x, int64_t xi, int64_t dim, int aligned)
+    {
+        if (aligned && xi + 1 < dim) return *reinterpret_cast<const double2*>(x + xi);
+        return make_double2(xi < dim ? x[xi] : 0.0, xi + 1 < dim ? x[xi + 1] : 0.0);
+    }
+};
+template <> struct XPiece<float> {
+    using P = float4;
+    static constexpr int kWords = 4;
+    HD_DEV static P zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+    HD_DEV static P load(const float* x, int64_t xi, int64_t dim, int aligned)
+    {
+        if (aligned && xi + 3 < dim) return *reinterpret_cast<const float4*>(x + xi);
+        return make_float4(xi < dim ? x[xi] : 0.f, xi + 1 < dim ? x[xi + 1] : 0.f, xi + 2 < dim ? x[xi + 2] : 0.f,
+                           xi + 3 < dim ? x[xi + 3] : 0.f);
+    }
+};
+
+template <int TG, bool PK, typename XT>
 __global__ void __launch_bounds__(kSpmvThreads)
 k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, const uint32_t* __restrict__ entries,
              const uint32_t* __restrict__ lines, const uint32_t* __restrict__ meta, const double* __restrict__ tdiag,
-             const double* __restrict__ dict, int ndict, int x_aligned, int64_t win, const double* __restrict__ x,
+             const double* __restrict__ dict, int ndict, int x_aligned, int64_t win, const XT* __restrict__ x,
              double* __restrict__ y)
 {
-    constexpr int PER = (kTC * 8) / kSpmvThreads;  // 16-byte x pieces per thread per chunk
-    static_assert(PER * kSpmvThreads == kTC * 8, "pieces must divide evenly");
+    using XP = XPiece<XT>;
+    using Piece = typename XP::P;
+    constexpr int PPL = kLineWords / XP::kWords;       // 16-byte pieces per line
+    constexpr int PER = (kTC * PPL) / kSpmvThreads;     // 16-byte x pieces per thread per chunk
+    static_assert(PER * kSpmvThreads == kTC * PPL, "pieces must divide evenly");
     constexpr int kMeta = kMaxChunks * (TG + 2) + 1;
     constexpr int kTPR = kSpmvThreads / TG;
-    __shared__ double xs[kTC * kLineWords];
+    __shared__ XT xs[kTC * kLineWords];
     __shared__ uint32_t ids[kMaxLines];
     __shared__ uint32_t sm[kMeta];
     extern __shared__ double sdict[];
@@ -347,18 +378,17 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
     // registers (3.36 against 3.44 ms with one chunk ahead).  The packed
     // entries' unpacking needs the registers, so they keep one chunk ahead.
     constexpr int kAhead = PK ? 1 : 2;
-    double2 preA[PER], preB[PER];
+    Piece preA[PER], preB[PER];
     uint32_t eA[kU], eB[kU];
-    auto fetch = [&](int c, double2 (&pre)[PER], uint32_t (&e)[kU]) {
+    auto fetch = [&](int c, Piece (&pre)[PER], uint32_t (&e)[kU]) {
         const int l0 = (int)LS[c], n_here = (int)LS[c + 1] - l0;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int i = threadIdx.x + k * kSpmvThreads;
-            pre[k] = make_double2(0.0, 0.0);
-            if (i < n_here * 8) {
-                const int64_t xi = (int64_t)ids[l0 + (i >> 3)] * kLineWords + (i & 7) * 2;
-                if (x_aligned && xi + 1 < dim) pre[k] = *reinterpret_cast<const double2*>(x + xi);
-                else pre[k] = make_double2(xi < dim ? x[xi] : 0.0, xi + 1 < dim ? x[xi + 1] : 0.0);
+            pre[k] = XP::zero();
+            if (i < n_here * PPL) {
+                const int64_t xi = (int64_t)ids[l0 + i / PPL] * kLineWords + (i % PPL) * XP::kWords;
+                pre[k] = XP::load(x, xi, dim, x_aligned);
             }
         }
         const uint32_t j0 = T[c * (TG + 1) + r] + sub, je = T[c * (TG + 1) + r + 1];
@@ -375,21 +405,21 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
             if (PK) {  // slot 16-23, word 12-15, sign 11, index 0-10
                 double v = sdict[b[u] & 0x7FFu];
                 if (b[u] & 0x800u) v = -v;
-                a = fma(v, xs[(b[u] >> 16) * kLineWords + ((b[u] >> 12) & 15u)], a);
+                a = fma(v, (double)xs[(b[u] >> 16) * kLineWords + ((b[u] >> 12) & 15u)], a);
             } else {
                 double v = sdict[b[u] & 0x7FFFFu];
                 if (b[u] & 0x80000u) v = -v;
-                a = fma(v, xs[(b[u] >> 24) * kLineWords + ((b[u] >> 20) & 15u)], a);
+                a = fma(v, (double)xs[(b[u] >> 24) * kLineWords + ((b[u] >> 20) & 15u)], a);
             }
         }
     };
     double acc = 0.0;
-    auto step = [&](int c, double2 (&pre)[PER], uint32_t (&e)[kU]) {
+    auto step = [&](int c, Piece (&pre)[PER], uint32_t (&e)[kU]) {
         __syncthreads();  // chunk c-1 consumed
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int i = threadIdx.x + k * kSpmvThreads;
-            *reinterpret_cast<double2*>(&xs[(i >> 3) * kLineWords + (i & 7) * 2]) = pre[k];
+            *reinterpret_cast<Piece*>(&xs[(i / PPL) * kLineWords + (i % PPL) * XP::kWords]) = pre[k];
         }
         uint32_t cur[kU];
 #pragma unroll
@@ -420,7 +450,7 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
     }
 #pragma unroll
     for (int m = kTPR / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m, 64);
-    if (row < dim && sub == 0) y[row] = fma(tdiag[row], x[row], acc);
+    if (row < dim && sub == 0) y[row] = fma(tdiag[row], (double)x[row], acc);
 }
 
 // y = H x for a complex (Hermitian) matrix on the tiled layout.  The same
@@ -762,9 +792,10 @@ void tiled_free(hd_hamiltonian* h)
     h->tg_win = 0;
 }
 
-int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
+int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s, bool x32)
 {
     if (!h->d_tg_base) return fail(HD_ERR_ARG, "no tiled layout (hd_ham_tile)");
+    if (x32 && h->is_complex) return fail(HD_ERR_UNSUPPORTED, "the FP32-x tiled product is implemented for real matrices");
     if (h->dim == 0) return HD_OK;
     const int aligned = ((uintptr_t)x & 15) == 0 ? 1 : 0;
     if (h->is_complex) {
@@ -780,19 +811,23 @@ int tiled_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t s)
         return HD_OK;
     }
     const size_t lds = sizeof(double) * (size_t)std::max(h->ndict, 1);
-#define HD_TL(TG, PK)                                                                                            \
-    hipLaunchKernelGGL((k_spmv_tiled<TG, PK>), dim3((unsigned)h->tg_groups), dim3(kSpmvThreads), lds, s, h->dim, \
-                       h->tg_groups, h->d_tg_base, h->d_tg_entries, h->d_tg_lines, h->d_tg_offs, h->d_tg_diag,    \
-                       static_cast<const double*>(h->d_dict), h->ndict, aligned, h->tg_win,                     \
-                       static_cast<const double*>(x), static_cast<double*>(y))
+#define HD_TL(TG, PK, XT)                                                                                            \
+    hipLaunchKernelGGL((k_spmv_tiled<TG, PK, XT>), dim3((unsigned)h->tg_groups), dim3(kSpmvThreads), lds, s, h->dim, \
+                       h->tg_groups, h->d_tg_base, h->d_tg_entries, h->d_tg_lines, h->d_tg_offs, h->d_tg_diag,        \
+                       static_cast<const double*>(h->d_dict), h->ndict, aligned, h->tg_win,                         \
+                       static_cast<const XT*>(x), static_cast<double*>(y))
+#define HD_TL_X(TG, PK)         \
+    if (x32) HD_TL(TG, PK, float); \
+    else HD_TL(TG, PK, double)
     const bool pk = h->tg_entry_bytes == 3;
     if (h->tg_rows == 128) {
-        if (pk) HD_TL(128, true);
-        else HD_TL(128, false);
+        if (pk) HD_TL_X(128, true);
+        else HD_TL_X(128, false);
     } else {
-        if (pk) HD_TL(64, true);
-        else HD_TL(64, false);
+        if (pk) HD_TL_X(64, true);
+        else HD_TL_X(64, false);
     }
+#undef HD_TL_X
 #undef HD_TL
     HD_HIP(hipGetLastError());
     return HD_OK;

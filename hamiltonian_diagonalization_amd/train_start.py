@@ -11,10 +11,11 @@ For each held-out point the 5 lowest eigenpairs are solved to residual 1e-10
 from (a) the random-normal start (seed 0), (b) the network's start vector and
 (c) the network's start vector plus a random admixture, with the FP64
 operator; (d) the network start with the FP32-value mixed solve.  Results go to
-<checkpoint>.json.
+<checkpoint name>_report.json in the working directory (or --report).
 """
 import argparse
 import json
+import os
 import sys
 import time
 
@@ -94,7 +95,8 @@ def main():
     ap.add_argument("--nev", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--out", required=True, help="checkpoint to write (or read with --eval-only)")
-    ap.add_argument("--report", default=None, help="JSON report path (default: next to the checkpoint)")
+    ap.add_argument("--report", default=None,
+                    help="JSON report path (default: <checkpoint name>_report.json in the working directory)")
     args = ap.parse_args()
 
     basis = FockBasis(Sector.lz(0, args.two_s), args.n, args.two_s + 1)
@@ -147,7 +149,8 @@ def main():
         "max_abs_dE": float(max(max(r["learned"]["max_abs_dE_vs_plain"], r["ground_only"]["abs_dE"]) for r in ev)),
     }
     log("summary", report["summary"])
-    path = args.report or (args.out[:-4] + ".json" if args.out.endswith(".npz") else args.out + ".json")
+    stem = os.path.basename(args.out)
+    path = args.report or ((stem[:-4] if stem.endswith(".npz") else stem) + "_report.json")
     with open(path, "w") as f:
         json.dump(report, f, indent=1)
     log(f"wrote {path}")

@@ -77,8 +77,12 @@ typedef enum {
  *             entry; FP64 vectors and accumulation.
  *   HD_TILED  the coded operator re-laid out in row groups whose x lines
  *             are staged through LDS (hd_ham_tile): 3 or 4 bytes per entry, the
- *             same exact FP64 matrix; real matrices. */
-typedef enum { HD_F64 = 0, HD_F32 = 1, HD_CODED = 2, HD_TILED = 3 } hd_precision;
+ *             same exact FP64 matrix; real and complex matrices.
+ *   HD_TILED_X32  the tiled operator with x in FP32 (half the staged x bytes)
+ *             and y in FP64: FP64 dictionary values, FP64 products and FP64
+ *             accumulation, so y is the exact FP64 operator applied to x
+ *             (which is FP32) with the rounding of HD_TILED; real matrices. */
+typedef enum { HD_F64 = 0, HD_F32 = 1, HD_CODED = 2, HD_TILED = 3, HD_TILED_X32 = 4 } hd_precision;
 
 typedef enum {
     HD_STORAGE_FULL = 0,     /* both triangles, int64 row pointers (GPU SpMV layout) */
@@ -274,7 +278,8 @@ int hd_obs_most_probable(const hd_basis* b, int32_t is_complex, const void* d_ve
 /* MatrixVectorFunction<T,L1,L2>::Simple (src/hamiltonians/matrix_vector_routines.hpp:94-104):
  * y = H x on device vectors.  precision = HD_F64: FP64 values, x, y (complex
  * FP64 for complex tables).  HD_F32: FP32 values, x, y, FP32 accumulation.
- * Full storage only.  The call is asynchronous on the handle's stream. */
+ * HD_CODED, HD_TILED: FP64 x, y.  HD_TILED_X32: FP32 x, FP64 y.  Full storage
+ * only.  The call is asynchronous on the handle's stream. */
 int hd_spmv(const hd_hamiltonian* h, int32_t precision, const void* d_x, void* d_y);
 /* Simple() drop-in over host buffers (FP64 / complex FP64): copies in, multiplies,
  * copies out. */
@@ -358,9 +363,16 @@ typedef struct {
     int32_t start;        /* hd_start */
     uint64_t seed;
     const void* d_start;  /* HD_START_USER: dim FP64 (complex FP64) device vector */
-    int32_t precision;    /* HD_F64: FP64 operator throughout.  HD_F32: FP32 operator until
-                             the residual reaches f32_switch_tol, then FP64 refinement */
-    double f32_switch_tol;/* 0 = 1e-6 */
+    int32_t precision;    /* HD_F64, HD_CODED, HD_TILED: that exact FP64 operator throughout.
+                             HD_F32: mixed; each restart cycle uses the exact FP64 operator
+                             (tiled when built) or the FP32-input one (HD_TILED_X32 for real
+                             matrices with a tiled layout, else the FP32-value CRS); a cycle
+                             takes the FP32 operator only while the rounding it adds to every
+                             wanted residual, estimated from the Ritz coefficients on the
+                             cycle's new vectors, stays within f32_budget of tol.  The
+                             eigenpairs are checked with the FP64 operator; when a residual
+                             misses tol the solve is repeated with the FP64 operator. */
+    double f32_budget;    /* HD_F32: share of tol the FP32 cycles may take; 0 = 1 */
     int32_t want_vectors; /* keep the eigenvectors on the device (hd_lanczos_vectors) */
     int32_t block;        /* 0 or 1: single-vector Lanczos.  4, 8 or 16: block Lanczos with
                              that many vectors per SpMM (real matrices, FP64 or coded operator;
@@ -387,8 +399,17 @@ typedef struct {
 /* HamiltonianBase::LanczosDiagonalize -> ArpackWrapper::DiagonalizeSymmetricMatrix
  * (src/hamiltonians/hamiltonian_base.hpp:769-931, src/utilities/wrappers/arpack_wrapper.hpp:595-937):
  * the nev lowest eigenvalues (WHICH='SR'), ascending, by a device-resident
- * thick-restart Lanczos with full reorthogonalisation in FP64.  Eigenvalues go
- * to h_evals; with want_vectors the eigenvectors stay resident and are read with
+ * thick-restart Lanczos with FP64 recurrence and reductions.  By default
+ * (full_reorth = 0) the orthogonality of the basis is tracked by Simon's omega
+ * recurrence and a new vector is orthogonalised against the whole basis only
+ * when an estimate exceeds min(sqrt(eps), tol) (the first and last step of
+ * every restart cycle always are); full_reorth = 1 orthogonalises at every
+ * step.  Every returned eigenpair's residual is recomputed with the FP64
+ * operator.  When one misses tol, the solve is repeated once from the start,
+ * with the FP64 operator if precision was HD_F32, else with full_reorth = 1;
+ * the returned statistics then add up both runs, so such a call can cost
+ * about twice the operator applications.  Eigenvalues go to h_evals; with
+ * want_vectors the eigenvectors stay resident and are read with
  * hd_lanczos_vectors. */
 int hd_lanczos(hd_hamiltonian* h, const hd_lanczos_opts* opts, double* h_evals,
                hd_lanczos_stats* stats);

@@ -26,20 +26,24 @@ def test_sphere_ktable_matches_oracle(oracle, max_lz):
 @pytest.mark.parametrize("max_lz,vm", [(9, [0.0, 1.0]), (7, [0.3, 1.0, 0.0, 0.25]),
                                       (12, [0.0, 1.0, 0.0, 0.5, 0.0, 0.1])])
 def test_sphere_vtable_matches_exact_rationals(oracle, max_lz, vm):
+    # both sides take every CG correctly rounded from its exact square and sum
+    # in the reference's order: the tables are bit-identical
     got = hd.sphere_vtable(max_lz, vm)
     want = oracle.sphere_vtable(max_lz, vm)
-    scale = np.max(np.abs(want))
-    assert np.max(np.abs(got - want)) <= 1e-14 * scale
-    # CG from exact rationals rounded once: almost every entry is bit-identical
-    assert np.mean(got == want) > 0.99
+    assert np.array_equal(got, want)
 
 
-def test_sphere_vtable_config1_coulomb(oracle):
+@pytest.mark.parametrize("two_s", [21, 27, 33, 41])
+def test_sphere_vtable_coulomb_bit_identical(oracle, two_s):
+    """The benchmark fluxes (2S = 21, 27, 33) and one beyond them.  At 2S = 33
+    a long double square root of the 96-bit-truncated CG^2, rounded to double,
+    missed the correctly rounded CG in one ulp for entries of 32 table cells;
+    the library now checks its square roots against the exact rational."""
     from hamiltonian_diagonalization_amd import models
-    vm = models.coulomb_list(21)
-    got = hd.sphere_vtable(21, vm)
-    want = oracle.sphere_vtable(21, list(vm))
-    assert np.max(np.abs(got - want)) <= 1e-14 * np.max(np.abs(want))
+    vm = models.coulomb_list(two_s)
+    got = hd.sphere_vtable(two_s, vm)
+    want = oracle.sphere_vtable(two_s, list(vm))
+    assert np.array_equal(got, want)
 
 
 def test_cg_closed_forms(oracle):

@@ -1,0 +1,264 @@
+"""Parity at the benchmark sizes, through the C ABI:
+
+* the whole reference-layout CRS of the N=12, 2S=33 sphere sector and of the
+  6x5 flux-lattice sector against streamed oracle digests
+  (tests/golden/block_digests.json);
+* the headline operator (the tiled layout) at N=12, 2S=33: sampled columns bit
+  for bit against the oracle's rows, and a random product against the FP64 CRS
+  product and against the oracle's rows;
+* the tiled layout's multi-chunk pipeline (groups of 3 to 6 chunks) for both
+  group sizes and both entry widths at N=10, 2S=27;
+* the batched sweep of N=10, 2S=27 (91 sectors x 64 points) with sampled
+  problems against oracle CRS + LAPACK/ARPACK;
+* the learned start at N=12, 2S=33 on held-out parameter points.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import hamiltonian_diagonalization_amd as hd
+from hamiltonian_diagonalization_amd import models
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+CHECKPOINT = os.path.join(os.path.dirname(HERE), "hamiltonian_diagonalization_amd", "checkpoints",
+                          "start_n12_2s33.npz")
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _block_digest(rowptr, col, val):
+    # same definition as tests/golden/make_block_digests.py
+    h = hashlib.sha256()
+    h.update(np.diff(np.asarray(rowptr, dtype=np.int64)).astype(np.int64).tobytes())
+    h.update(np.ascontiguousarray(col, dtype=np.int32).tobytes())
+    h.update(np.ascontiguousarray(val).tobytes())
+    return h.hexdigest()
+
+
+def _streamed_digest(h, dim, block_rows):
+    out = []
+    for r0 in range(0, dim, block_rows):
+        rp, col, val = h.crs(r0, min(r0 + block_rows, dim))
+        out.append(_block_digest(rp, col, val))
+    return out
+
+
+GOLD = json.load(open(os.path.join(HERE, "golden", "block_digests.json")))
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("key", ["sphere", "ofl"])
+def test_whole_reference_crs_digest(key):
+    """Every row of the reference-layout CRS (int32 row starts as row lengths,
+    columns, FP64 values) equals the oracle's, block by block."""
+    g = GOLD[key]
+    say("whole upper CRS digest", g["name"])
+    if key == "sphere":
+        b, t = models.sphere_problem(12, 33)
+    else:
+        b, t = models.ofl_problem(6, 5, 10, 0, 0, seed=0)
+    assert b.dim == g["dim"]
+    h = hd.Hamiltonian(b, t, storage=hd.STORAGE_UPPER_REF, values=(hd.F64,))
+    info = h.info()
+    assert info["nnz"] == g["nnz"] and info["n_trimmed"] == g["n_trimmed"]
+    got = _streamed_digest(h, b.dim, g["block_rows"])
+    bad = [i for i, (x, y) in enumerate(zip(got, g["block_digests"])) if x != y]
+    assert not bad, f"blocks differing from the oracle: {bad[:10]}"
+    assert hashlib.sha256("".join(got).encode()).hexdigest() == g["digest"]
+
+
+def _oracle_rows(oracle, n, norb, bas, t, r0, r1, layout=None, cplx=False):
+    layout = oracle.LAYOUT_SPHERE if layout is None else layout
+    return oracle.build(n, norb, bas, layout, norb, t.ktable, t.vtable, eps=t.eps if cplx else None,
+                        is_complex=cplx, full=True, row0=r0, row1=r1)
+
+
+def _pack_disjoint(cands, pattern):
+    """Splits candidate columns into rounds whose row patterns are pairwise
+    disjoint: a product with the sum of a round's unit vectors then has at
+    most one nonzero term per output row."""
+    rounds, left = [], list(cands)
+    while left:
+        used, keep, rest = set(), [], []
+        for i in left:
+            p = pattern(i)
+            if used.isdisjoint(p):
+                used.update(p)
+                keep.append(i)
+            else:
+                rest.append(i)
+        rounds.append(keep)
+        left = rest
+    return rounds
+
+
+def _check_columns(h, precision, dim, cols, row_of):
+    """Columns H e_i through one product per disjoint round, bit for bit
+    against the oracle's row i (the matrix is symmetric bit for bit), and
+    exact zeros outside the rounds' patterns."""
+    checked = 0
+    for rnd in _pack_disjoint(cols, lambda i: set(row_of(i)[0].tolist())):
+        x = np.zeros(dim)
+        x[rnd] = 1.0
+        y = h.spmv(x, precision=precision)
+        touched = np.zeros(dim, dtype=bool)
+        for i in rnd:
+            c, v = row_of(i)
+            assert np.array_equal(y[c], v), i
+            touched[c] = True
+            checked += 1
+        assert not np.any(y[~touched]), "nonzero output outside the sampled columns' patterns"
+    return checked
+
+
+@pytest.mark.timeout(600)
+def test_tiled_headline_operator_config2(oracle):
+    """The solver's exact operator at N=12, 2S=33 (dim 7,764,392, 2.59e9
+    entries): 720 sampled columns through the tiled layout equal the oracle's
+    rows bit for bit, and a random product equals the FP64 CRS product and
+    the oracle rows' dot products to 1e-13."""
+    say("tiled operator N=12 2S=33: columns bit-exact, random product")
+    b, t = models.sphere_problem(12, 33)
+    dim = b.dim
+    bas = b.states()
+    h = hd.Hamiltonian(b, t, values=(hd.F64, hd.CODED))
+    ti = h.tile()
+    say(ti)
+    assert ti["rows_per_group"] == 128 and ti["chunks"] >= 5 * ti["groups"]
+    rows = {}
+    for r0 in np.linspace(0, dim - 120, 6).astype(np.int64):
+        ref = _oracle_rows(oracle, 12, 34, bas, t, int(r0), int(r0) + 120)
+        for k in range(120):
+            s, e = ref.rowptr[k], ref.rowptr[k + 1]
+            rows[int(r0) + k] = (ref.col[s:e].astype(np.int64), ref.values()[s:e])
+    n = _check_columns(h, hd.TILED, dim, sorted(rows), lambda i: rows[i])
+    assert n == 720
+    x = np.random.default_rng(11).normal(size=dim)
+    yt = h.spmv(x, precision=hd.TILED)
+    y64 = h.spmv(x, precision=hd.F64)
+    scale = np.max(np.abs(y64))
+    assert np.max(np.abs(yt - y64)) <= 1e-13 * scale
+    want = np.array([np.dot(v, x[c]) for c, v in (rows[i] for i in sorted(rows))])
+    assert np.max(np.abs(yt[sorted(rows)] - want)) <= 1e-13 * scale
+
+
+MULTI = None
+
+
+def _multichunk_case(oracle):
+    global MULTI
+    if MULTI is None:
+        b, t = models.sphere_problem(10, 27)
+        bas = b.states()
+        full = oracle.build(10, 28, bas, oracle.LAYOUT_SPHERE, 28, t.ktable, t.vtable, full=True, threads=16)
+        up = oracle.build(10, 28, bas, oracle.LAYOUT_SPHERE, 28, t.ktable, t.vtable, threads=16)
+        MULTI = (b, t, full, up)
+    return MULTI
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("rows,nbytes", [(64, 4), (128, 4), (64, 3), (128, 3)])
+def test_tiled_multichunk_pipeline(oracle, rows, nbytes):
+    """N=10, 2S=27 (dim 246,448): 64-row groups hold 3 chunks on average and
+    128-row groups 4, up to 4 and 6, so the loads of chunks two ahead (4-byte
+    entries) and one ahead (packed entries) run.  600 sampled columns equal
+    the oracle's full CRS bit for bit and a random product equals the oracle's
+    naive symmetric product to 1e-13."""
+    say("tiled multi-chunk N=10 2S=27", rows, nbytes)
+    b, t, full, up = _multichunk_case(oracle)
+    dim = b.dim
+    h = hd.Hamiltonian(b, t, values=(hd.CODED,))
+    ti = h.tile(rows, nbytes)
+    say(ti)
+    assert ti["rows_per_group"] == rows and ti["entry_bytes"] == nbytes
+    # mean chunks per group above 2 (64 rows) or 3 (128 rows): some groups hold 3 or more
+    assert ti["chunks"] > (2.5 if rows == 64 else 3.5) * ti["groups"]
+    cols = np.random.default_rng(rows + nbytes).choice(dim, 600, replace=False)
+
+    def row_of(i):
+        s, e = full.rowptr[i], full.rowptr[i + 1]
+        return full.col[s:e].astype(np.int64), full.values()[s:e]
+
+    assert _check_columns(h, hd.TILED, dim, cols.tolist(), row_of) == 600
+    x = np.random.default_rng(5).normal(size=dim)
+    want = oracle.spmv_sym(up, x, threads=16)
+    got = h.spmv(x, precision=hd.TILED)
+    assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+
+
+@pytest.mark.timeout(900)
+def test_batch_config4_full_sweep(oracle):
+    """N=10, 2S=27: all 91 L_z sectors x the 8x8 (V1, V3) grid, 3 lowest
+    eigenvalues each to residual 1e-10 in one batched solve; 12 sampled
+    problems against the oracle's CRS with LAPACK (dense) or ARPACK (eigsh)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    n, two_s = 10, 27
+    norb = two_s + 1
+    lzs = list(range(0, n * (norb - n) + 1, 2))
+    v1, v3 = np.linspace(0.5, 1.5, 8), np.linspace(0.0, 0.5, 8)
+    pts = [(a, c) for a in v1 for c in v3]
+    say("batch N=10 2S=27:", len(lzs), "sectors x", len(pts), "points")
+    bases = [hd.FockBasis(hd.Sector.lz(lz, two_s), n, norb) for lz in lzs]
+    vms = [models.coulomb_list(two_s, {1: a, 3: c}) for a, c in pts]
+    tables = [models.sphere_tables(two_s, vm) for vm in vms]
+    bp = hd.BatchProblem(bases, tables)
+    ev, rs, st = bp.solve(nev=3, tol=1e-10, seed=0)
+    say(st)
+    assert len(lzs) == 91 and st["problems"] == 91 * 64
+    assert st["converged"] == st["problems"] and st["max_residual"] <= 1e-10
+    dims = np.array([bb.dim for bb in bases])
+    picks = []
+    for target, npts in ((dims.max(), 1), (60000, 2), (12000, 3), (2000, 2), (300, 2), (20, 2)):
+        s = int(np.argmin(np.abs(dims - target)))
+        for p in np.random.default_rng(s).choice(len(pts), npts, replace=False):
+            picks.append((s, int(p)))
+    assert len(picks) == 12
+    for s, p in picks:
+        bas = bases[s].states()
+        t = tables[p]
+        ref = oracle.build(n, norb, bas, oracle.LAYOUT_SPHERE, norb, t.ktable, t.vtable, full=True, threads=16)
+        dim = len(bas)
+        M = sp.csr_matrix((ref.values(), ref.col, ref.rowptr), shape=(dim, dim))
+        k = min(3, dim)
+        if dim <= 3000:
+            want = np.linalg.eigvalsh(M.toarray())[:k]
+        else:
+            want = np.sort(sla.eigsh(M, k=3, which="SA", tol=1e-13, ncv=40, v0=np.ones(dim))[0])
+        got = ev[s, p, :k]
+        err = np.max(np.abs(got - want) / np.maximum(1, np.abs(want)))
+        say(f"lz={lzs[s]} dim={dim} point={p}: max rel err {err:.2e}")
+        assert err < 1e-10, (lzs[s], p, got, want)
+
+
+@pytest.mark.timeout(600)
+def test_learned_start_config5_heldout():
+    """The checkpoint trained at N=12, 2S=33 on 128 points of
+    [0.5,1.5] x [0,0.5] x [0,0.2]: on held-out points (seed 1) the solve from the
+    network's start vector reaches residual 1e-10 for the 5 lowest eigenpairs
+    with the eigenvalues of the random start to 1e-10."""
+    from hamiltonian_diagonalization_amd import learned
+    net = learned.AmplitudeNet.load(CHECKPOINT)
+    basis = hd.FockBasis(hd.Sector.lz(0, 33), 12, 34)
+    for v in learned.sample_params(np.random.default_rng(1), 2):
+        say("learned start N=12 2S=33 at", v)
+        vm = models.coulomb_list(33, {1: float(v[0]), 3: float(v[1]), 5: float(v[2])})
+        h = hd.Hamiltonian(basis, models.sphere_tables(33, vm), values=(hd.CODED,))
+        h.tile()
+        plain = h.lanczos(nev=5, tol=1e-10, seed=0, precision=hd.TILED)
+        start = net.start_vector(basis, v)
+        lr = h.lanczos(nev=5, tol=1e-10, start=start, precision=hd.TILED)
+        say("plain", plain)
+        say("learned", lr)
+        assert plain.converged and lr.converged
+        assert np.all(plain.residuals <= 1e-10) and np.all(lr.residuals <= 1e-10)
+        assert np.max(np.abs(lr.evals - plain.evals) / np.maximum(1, np.abs(plain.evals))) < 1e-10
+        del h
