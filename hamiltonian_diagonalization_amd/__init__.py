@@ -143,6 +143,7 @@ SIGNATURES = {
     "hd_obs_most_probable": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P]),
     "hd_spmv": (ctypes.c_int, [_P, _I32, _P, _P]),
     "hd_spmv_host": (ctypes.c_int, [_P, _P, _P]),
+    "hd_spmv_host_noalloc": (ctypes.c_int, [_P, _P, _P]),
     "hd_spmv_time": (ctypes.c_int, [_P, _I32, _I32, ctypes.POINTER(_D)]),
     "hd_spmv_get_schedule": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "hd_stream_copy": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _I32, ctypes.POINTER(_D)]),
@@ -588,6 +589,16 @@ class Hamiltonian:
         xx = np.ascontiguousarray(x, dtype=dt)
         y = np.zeros_like(xx)
         _check(lib().hd_spmv_host(self.h, _ptr(xx), _ptr(y)))
+        return y
+
+    def simple_noalloc(self, x: np.ndarray) -> np.ndarray:
+        """Simple() over host buffers staged through the library's fixed device
+        array (hd_spmv_host_noalloc): no device allocation; dim <= 4,194,304
+        real or 2,097,152 complex."""
+        dt = self.vec_dtype(F64)
+        xx = np.ascontiguousarray(x, dtype=dt)
+        y = np.zeros_like(xx)
+        _check(lib().hd_spmv_host_noalloc(self.h, _ptr(xx), _ptr(y)))
         return y
 
     def spmv_time_ms(self, precision=F64, iters=10) -> float:

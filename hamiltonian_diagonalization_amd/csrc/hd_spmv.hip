@@ -23,6 +23,8 @@
 // exact FP64 operator in 6 bytes per entry instead of 12.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <algorithm>
 
 #include "hd_internal.hpp"
@@ -329,11 +331,72 @@ int spmv_autotune(hd_hamiltonian* h)
     return st;
 }
 
+// Staging of hd_spmv_host_noalloc: one file-scope device array per GPU (each
+// device holds its own copy for the life of the process), x in the first
+// half and y in the second.  Host code never passes the array's name to a
+// kernel or a copy: the name is a host-side stand-in, and its device address
+// is looked up once per device with hipGetSymbolAddress.
+constexpr size_t kNoAllocHalf = (size_t)32 << 20;  // bytes per staged vector
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[2 * kNoAllocHalf];
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of `scratch` on the current device `dev` (looked up on the
+// first call for that device).
+static int scratch_address(int dev, unsigned char** out)
+{
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(HD_ERR_ARG, "device index beyond the no-alloc table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(HD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+    return HD_OK;
+}
+
+// The FP64 operator Simple() applies: FP64 values, else coded, else tiled.
+static int simple_precision(const hd_hamiltonian* h)
+{
+    return h->d_val64 ? HD_F64 : h->d_code ? HD_CODED : h->d_tg_base ? HD_TILED : -1;
+}
+
 }  // namespace hd
 
 using namespace hd;
 
 extern "C" {
+
+int hd_spmv_host_noalloc(const hd_hamiltonian* h, const void* h_x, void* h_y)
+{
+    if (!h || !h_x || !h_y) return fail(HD_ERR_ARG, "hd_spmv_host_noalloc: null argument");
+    const size_t bytes = (size_t)h->dim * 8 * (h->is_complex ? 2 : 1);
+    if (bytes > kNoAllocHalf)
+        return fail(HD_ERR_ARG, "hd_spmv_host_noalloc: a vector of " + std::to_string(bytes) +
+                                    " bytes exceeds the 32 MiB staging buffer (dim <= 4194304 real, "
+                                    "2097152 complex); use hd_spmv_host or hd_spmv");
+    const int prec = simple_precision(h);
+    if (prec < 0) return fail(HD_ERR_ARG, "Simple() needs FP64, coded or tiled values");
+    int dev = -1;
+    HD_HIP(hipGetDevice(&dev));
+    if (dev != h->device) return fail(HD_ERR_ARG, "hd_spmv_host_noalloc: the handle's device must be current");
+    unsigned char* base = nullptr;
+    HD_CHECK(scratch_address(dev, &base));
+    if (bytes == 0) return HD_OK;
+    void* x = base;
+    void* y = base + kNoAllocHalf;
+    HipErrors q;
+    HD_ENQ(q, hipMemcpyAsync(x, h_x, bytes, hipMemcpyHostToDevice, h->stream));
+    HD_CHECK(q.status());
+    HD_CHECK(spmv_launch(h, prec, x, y, h->stream));
+    HD_ENQ(q, hipMemcpyAsync(h_y, y, bytes, hipMemcpyDeviceToHost, h->stream));
+    HD_ENQ(q, hipStreamSynchronize(h->stream));
+    return q.status();
+}
 
 int hd_spmv(const hd_hamiltonian* h, int32_t precision, const void* d_x, void* d_y)
 {
@@ -345,7 +408,7 @@ int hd_spmv_host(const hd_hamiltonian* h, const void* h_x, void* h_y)
 {
     if (!h || !h_x || !h_y) return fail(HD_ERR_ARG, "hd_spmv_host: null argument");
     const size_t bytes = (size_t)h->dim * 8 * (h->is_complex ? 2 : 1);
-    const int prec = h->d_val64 ? HD_F64 : h->d_code ? HD_CODED : h->d_tg_base ? HD_TILED : -1;
+    const int prec = simple_precision(h);
     if (prec < 0) return fail(HD_ERR_ARG, "Simple() needs FP64, coded or tiled values");
     void *x = nullptr, *y = nullptr;
     HD_CHECK(dalloc(&x, bytes, "Simple() x"));

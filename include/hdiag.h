@@ -284,6 +284,16 @@ int hd_spmv(const hd_hamiltonian* h, int32_t precision, const void* d_x, void* d
 /* Simple() drop-in over host buffers (FP64 / complex FP64): copies in, multiplies,
  * copies out. */
 int hd_spmv_host(const hd_hamiltonian* h, const void* h_x, void* h_y);
+/* Simple() over host buffers without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, the reverse-communication loop
+ * of an ARPACK driver calling Simple() hundreds of times).  x and y are staged
+ * through a fixed 64 MiB device array of the library (32 MiB per vector, held
+ * by every GPU the library runs on until the process ends), so dim is at most
+ * 4,194,304 (real) or 2,097,152 (complex); larger handles get HD_ERR_ARG.  The
+ * handle's device must be current.  All calls on one GPU share that array:
+ * calls that overlap on one GPU race (serialise them). */
+int hd_spmv_host_noalloc(const hd_hamiltonian* h, const void* h_x, void* h_y);
 /* Measurement helper (no reference counterpart): runs `iters` SpMVs on zero
  * vectors allocated by the library and reports the mean kernel time from HIP
  * events recorded on the handle's stream. */
