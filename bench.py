@@ -1,20 +1,30 @@
 """Benchmark: seconds to the 5 lowest eigenpairs at residual 1e-10 for the
-Coulomb sphere N=12, 2S=33, L_z=0 sector (7,764,392 states), and the SpMV's
-HBM bandwidth, on one MI355X.
+Coulomb sphere N=12, 2S=33, L_z=0 sector (7,764,392 states), learned against
+plain start, and the SpMV's HBM bandwidth, on one MI355X.
 
-One step = one complete thick-restart Lanczos solve from the built CRS to 5
-eigenpairs with ||Hx - lx|| <= 1e-10 max(1,|l|) checked with the FP64
-operator, random-normal start (seed 0), FP64 operator in its faster format
-(FP64 CRS or coded).  Basis and build are timed once outside the steps.  The
-other FP64 format and the FP32-value mixed solve run once beside the steps
-for the comparison fields; the CPU oracle leg times the reference algorithm
-on a bounded sample on the host cores.
+One step = one complete thick-restart Lanczos solve from the built operator
+to 5 eigenpairs with ||Hx - lx|| <= 1e-10 max(1,|l|), checked with the exact
+FP64 operator.  Three forms are timed over the same K steps each:
+  plain_fp64     random-normal start (seed 0), exact FP64 operator in its
+                 fastest format (FP64 CRS, coded or tiled);
+  mixed_fp32     random-normal start, restart cycles moved to the FP32-input
+                 tiled operator once their rounding fits the tolerance;
+  learned_mixed  the amplitude network's start vector (the network pass over
+                 the whole sector is inside the step), then the mixed solve.
+`value` is the fastest form whose residuals reach tol and whose eigenvalues
+equal the plain form's to 1e-10 (the form is named in config.form); every
+form's time is reported.  Basis, build and tiled layout are timed once outside
+the steps.  The learned start is also evaluated on held-out points of its
+training box.  The CPU oracle leg times the reference algorithm on the host
+cores (bounded sample) and a full ARPACK solve of the N=8, 2S=21 sector.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-compare]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-compare] [--skip-learned]
 """
 import argparse
 import json
 import os
+import re
+import subprocess
 import sys
 import threading
 import time
@@ -26,39 +36,91 @@ sys.path.insert(0, ROOT)
 
 METRIC = "sec to 5 eigpairs @resid 1e-10, learned vs plain start; SpMV HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+CPU_POOL = 16  # worker threads per GPU allowed on the host
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n, two_s, bas, tables, matvecs, sample_rows, threads):
-    """The oracle (CPU restatement of the reference's algorithm) on a bounded
-    sample: full basis enumeration, the upper-CRS build and the naive
-    symmetric SpMV on the first `sample_rows` rows, extrapolated by row count."""
-    from oracle import oracle as orc
-    dim = len(bas)
-    K = two_s + 1
-    t0 = time.perf_counter()
-    b2 = orc.basis(orc.sector(orc.SECTOR_LZ, 0, two_s), n, K)
-    t_basis = time.perf_counter() - t0
-    assert len(b2) == dim
-    # threaded row-range build of the sample (ctypes releases the GIL)
-    chunks = np.linspace(0, sample_rows, threads + 1).astype(np.int64)
-    parts = [None] * threads
+def physical_cores():
+    """Physical cores of the host from lscpu (cores per socket x sockets)."""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        cps = int(re.search(r"^Core\(s\) per socket:\s*(\d+)", out, re.M).group(1))
+        sock = int(re.search(r"^Socket\(s\):\s*(\d+)", out, re.M).group(1))
+        return cps * sock
+    except Exception:
+        return os.cpu_count() or 1
 
-    def work(i):
-        parts[i] = orc.build(n, K, b2, orc.LAYOUT_SPHERE, K, tables.ktable, tables.vtable,
-                             row0=int(chunks[i]), row1=int(chunks[i + 1]))
 
-    t0 = time.perf_counter()
-    th = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+def _threaded(fn, n):
+    th = [threading.Thread(target=fn, args=(i,)) for i in range(n)]
     for t in th:
         t.start()
     for t in th:
         t.join()
+
+
+def cpu_arpack_config1(orc, models, nev, tol, threads):
+    """The reference's solve path on the host, whole: N=8, 2S=21 sector, oracle
+    upper CRS, ARPACK in non-symmetric mode (dnaupd, as ArpackWrapper drives it)
+    with NCV = 2 nev + 21 through scipy, the oracle's naive symmetric SpMV as the
+    operator (one thread: the sector is small)."""
+    import scipy.sparse.linalg as sla
+    n, two_s = 8, 21
+    K = two_s + 1
+    t0 = time.perf_counter()
+    bas = orc.basis(orc.sector(orc.SECTOR_LZ, 0, two_s), n, K)
+    t_basis = time.perf_counter() - t0
+    kt = orc.sphere_ktable(two_s)
+    vt = orc.sphere_vtable(two_s, models.coulomb_list(two_s))
+    t0 = time.perf_counter()
+    up = orc.build(n, K, bas, orc.LAYOUT_SPHERE, K, kt, vt)
     t_build = time.perf_counter() - t0
-    # naive symmetric SpMV over the sample rows, each thread its row block and y
+    count = [0]
+
+    def mv(x):
+        count[0] += 1
+        return orc.spmv_sym(up, np.ascontiguousarray(x, dtype=np.float64).ravel(), threads=threads)
+
+    dim = len(bas)
+    op = sla.LinearOperator((dim, dim), matvec=mv, dtype=np.float64)
+    t0 = time.perf_counter()
+    w = sla.eigs(op, k=nev, which="SR", ncv=2 * nev + 21, tol=tol, v0=np.ones(dim))[0]
+    t_solve = time.perf_counter() - t0
+    x = np.random.default_rng(0).normal(size=dim)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        mv(x)
+    t_spmv = (time.perf_counter() - t0) / 20
+    return {"dim": dim, "basis_s": round(t_basis, 4), "build_s": round(t_build, 3), "solve_s": round(t_solve, 3),
+            "matvecs": count[0] - 20, "spmv_s": t_spmv, "eigenvalues": sorted(float(v.real) for v in w)}
+
+
+def cpu_baseline(n, two_s, tables, gpu_matvecs, sample_rows, threads, cores, nev, tol):
+    """The oracle (CPU restatement of the reference's algorithm) on the host:
+    full basis enumeration, the upper-CRS build and the naive symmetric SpMV
+    on the first `sample_rows` rows (scaled by row count), and a full ARPACK
+    solve of N=8, 2S=21 that prices the Arnoldi work per product."""
+    from oracle import oracle as orc
+    from hamiltonian_diagonalization_amd import models
+    K = two_s + 1
+    t0 = time.perf_counter()
+    bas = orc.basis(orc.sector(orc.SECTOR_LZ, 0, two_s), n, K)
+    t_basis = time.perf_counter() - t0
+    dim = len(bas)
+    chunks = np.linspace(0, sample_rows, threads + 1).astype(np.int64)
+    parts = [None] * threads
+
+    def work(i):
+        parts[i] = orc.build(n, K, bas, orc.LAYOUT_SPHERE, K, tables.ktable, tables.vtable,
+                             row0=int(chunks[i]), row1=int(chunks[i + 1]))
+
+    t0 = time.perf_counter()
+    _threaded(work, threads)
+    t_build = time.perf_counter() - t0
     x = np.random.default_rng(0).normal(size=dim)
     ys = [np.zeros(dim) for _ in range(threads)]
     L = orc.lib()
@@ -71,30 +133,34 @@ def cpu_baseline(n, two_s, bas, tables, matvecs, sample_rows, threads):
     reps = 3
     t0 = time.perf_counter()
     for _ in range(reps):
-        th = [threading.Thread(target=spmv, args=(i,)) for i in range(threads)]
-        for t in th:
-            t.start()
-        for t in th:
-            t.join()
+        _threaded(spmv, threads)
     t_spmv = (time.perf_counter() - t0) / reps
     scale = dim / sample_rows
     build_s = t_build * scale
     spmv_s = t_spmv * scale
-    solve_s = matvecs * spmv_s
+    ap = cpu_arpack_config1(orc, models, nev, tol, 1)
+    # Arnoldi work per product at N=8 2S=21, scaled by the dimension ratio
+    arnoldi_per_mv = max(ap["solve_s"] - ap["matvecs"] * ap["spmv_s"], 0.0) / max(ap["matvecs"], 1) * dim / ap["dim"]
+    mv = gpu_matvecs
+    solve_s = mv * (spmv_s + arnoldi_per_mv)
     nnz_sample = sum(p.nnz for p in parts)
     return {
-        "value": round(t_basis + build_s + solve_s, 2),
-        "unit": "s (basis + build + solve, extrapolated)",
+        "value": round(t_basis + build_s + solve_s, 1),
+        "unit": "s (basis + build + solve; build and solve extrapolated from the sample)",
         "cores": threads,
+        "physical_cores": cores,
         "kind": "port",
-        "sample": (f"oracle on host cores: full basis enumeration ({t_basis:.1f} s, 1 thread); "
-                   f"upper-CRS build of rows [0,{sample_rows}) ({t_build:.1f} s, {nnz_sample} nnz) "
-                   f"and the naive symmetric SpMV on them ({t_spmv * 1e3:.1f} ms), both scaled by "
-                   f"dim/{sample_rows} = {scale:.1f}; solve = {matvecs} mat-vecs (the GPU FP64 "
-                   f"Lanczos count) x {spmv_s:.2f} s; the Arnoldi vector work is not counted"),
+        "sample": (f"oracle on {threads} host threads ({cores} physical cores, {CPU_POOL} allowed per GPU): "
+                   f"full basis enumeration ({t_basis:.1f} s, 1 thread); upper-CRS build of rows "
+                   f"[0,{sample_rows}) ({t_build:.1f} s, {nnz_sample} nnz) and its naive symmetric SpMV "
+                   f"({t_spmv * 1e3:.1f} ms), scaled by dim/{sample_rows} = {scale:.1f}; solve = {mv} products "
+                   f"(the GPU FP64 count) x (SpMV + Arnoldi work per product from a full ARPACK dnaupd solve of "
+                   f"N=8 2S=21, NCV = 2 nev + 21, 1 thread: {ap['solve_s']:.2f} s, {ap['matvecs']} products)"),
         "basis_s": round(t_basis, 2),
         "build_s_extrapolated": round(build_s, 1),
         "spmv_s_extrapolated": round(spmv_s, 3),
+        "solve_s_extrapolated": round(solve_s, 1),
+        "arpack_config1": ap,
     }
 
 
@@ -102,55 +168,54 @@ def measured_traffic(kernel, n, two_s):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (calibrated against a plain copy kernel),
     or None when that kernel and workload were not profiled."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"spmv_n{n}_2s{two_s}_hbm_traffic.txt")
+    path = os.path.join(ROOT, "profiles", f"spmv_n{n}_2s{two_s}_hbm_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
         f = line.split()
-        if len(f) == 3 and f[0] == "traffic_bytes_per_launch":
-            name = f[1].replace(",true>", ">") if f[1].startswith("k_spmv_coded") else f[1]
-            if name == kernel:
-                return int(f[2])
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return int(f[2])
     return None
 
 
-def learned_start(args, hd, basis, ham, prec, plain):
-    """The committed amplitude network's start vector for this Hamiltonian's
-    (V1, V3, V5), then the same solve from it: FP64 operator and FP32-mixed.
-    Times include evaluating the network over the whole sector."""
-    from hamiltonian_diagonalization_amd import learned as lmod
-    from hamiltonian_diagonalization_amd import models
-    ckpt = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hamiltonian_diagonalization_amd", "checkpoints",
-                        f"start_n{args.n}_2s{args.two_s}.npz")
-    if not os.path.exists(ckpt):
-        return None
-    net = lmod.AmplitudeNet.load(ckpt)
-    vm = models.coulomb_list(args.two_s)
-    params = [vm[1], vm[3], vm[5]]
-    out = {"checkpoint": os.path.basename(ckpt), "params_v1_v3_v5": [float(v) for v in params],
-           "note": "the Coulomb point lies outside the training box V1 in [0.5,1.5], V3 in [0,0.5], V5 in [0,0.2]"}
-    hd.lib().hd_synchronize()
-    t0 = time.perf_counter()
-    start = net.start_vector(basis, params)
-    hd.lib().hd_synchronize()
-    t_net = time.perf_counter() - t0
-    v = start.to_array(np.float64, basis.dim)
-    psi0 = ham.eigenvector(0)  # ground state of the last converged solve
-    out["overlap_with_ground_state"] = float(abs(v @ psi0) / (np.linalg.norm(v) * np.linalg.norm(psi0)))
-    del v, psi0
-    out["network_s"] = round(t_net, 4)
-    for name, p in (("fp64", prec), ("f32_mixed", hd.F32)):
-        t0 = time.perf_counter()
-        r = ham.lanczos(nev=args.nev, tol=args.tol, precision=p, seed=0, ncv=args.ncv, start=start,
-                        raise_on_fail=False)
+def checkpoint_path(n, two_s):
+    p = os.path.join(ROOT, "hamiltonian_diagonalization_amd", "checkpoints", f"start_n{n}_2s{two_s}.npz")
+    return p if os.path.exists(p) else None
+
+
+def heldout_eval(hd, models, lmod, net, basis, two_s, nev, tol, ncv, points):
+    """Learned against random start at held-out points of the training box
+    (seed 1): tiled FP64 operator and mixed form, means over the points."""
+    rows = []
+    for v in lmod.sample_params(np.random.default_rng(1), points):
+        vm = models.coulomb_list(two_s, {1: float(v[0]), 3: float(v[1]), 5: float(v[2])})
+        h = hd.Hamiltonian(basis, models.sphere_tables(two_s, vm), values=(hd.CODED,))
+        h.tile()
+        r = {"params": [float(x) for x in v]}
+        plain = h.lanczos(nev=nev, tol=tol, precision=hd.TILED, seed=0, ncv=ncv)
         hd.lib().hd_synchronize()
-        dt = time.perf_counter() - t0 + t_net
-        out[name] = {"s": round(dt, 4), "matvecs": r.matvecs, "max_residual": float(r.residuals.max()),
-                     "max_abs_dE_vs_plain": float(np.max(np.abs(r.evals - plain.evals)))}
-        log(f"learned start, {name}: {dt:.3f} s {r}")
-    out["solve_s"] = out["fp64"]["s"]
-    return out
+        t0 = time.perf_counter()
+        start = net.start_vector(basis, v)
+        hd.lib().hd_synchronize()
+        t_net = time.perf_counter() - t0
+        runs = {"plain_fp64": plain,
+                "learned_fp64": h.lanczos(nev=nev, tol=tol, precision=hd.TILED, start=start, ncv=ncv),
+                "mixed_fp32": h.lanczos(nev=nev, tol=tol, precision=hd.F32, seed=0, ncv=ncv),
+                "learned_mixed": h.lanczos(nev=nev, tol=tol, precision=hd.F32, start=start, ncv=ncv)}
+        for k, x in runs.items():
+            r[k] = {"s": round(x.time_ms / 1e3 + (t_net if k.startswith("learned") else 0.0), 4),
+                    "matvecs": x.matvecs, "max_residual": float(x.residuals.max()),
+                    "max_abs_dE": float(np.max(np.abs(x.evals - plain.evals)))}
+        log(f"held-out {r['params']}: " + ", ".join(f"{k} {r[k]['s']:.3f} s / {r[k]['matvecs']} mv"
+                                                    for k in runs))
+        rows.append(r)
+        del h
+    mean = {k: {"s": round(float(np.mean([r[k]["s"] for r in rows])), 4),
+                "matvecs": float(np.mean([r[k]["matvecs"] for r in rows]))}
+            for k in ("plain_fp64", "learned_fp64", "mixed_fp32", "learned_mixed")}
+    return {"points": len(rows), "box": "V1 in [0.5,1.5], V3 in [0,0.5], V5 in [0,0.2] (held-out, seed 1)",
+            "mean": mean, "max_abs_dE": max(max(r[k]["max_abs_dE"] for k in mean) for r in rows),
+            "max_residual": max(max(r[k]["max_residual"] for k in mean) for r in rows), "per_point": rows}
 
 
 def main():
@@ -165,10 +230,10 @@ def main():
     ap.add_argument("--ncv", type=int, default=48,
                     help="Krylov basis size (the reference's ARPACK default is 2*nev+21)")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--skip-compare", action="store_true",
-                    help="skip the FP64-CRS and FP32-mixed comparison solves")
-    ap.add_argument("--cpu-rows", type=int, default=64000)
-    ap.add_argument("--skip-learned", action="store_true", help="skip the learned-start solves")
+    ap.add_argument("--skip-compare", action="store_true", help="skip the other FP64 formats' solves")
+    ap.add_argument("--cpu-rows", type=int, default=400000)
+    ap.add_argument("--skip-learned", action="store_true", help="skip the learned-start forms")
+    ap.add_argument("--heldout-points", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); a multi-GPU run would shard the "
@@ -178,83 +243,162 @@ def main():
     import hamiltonian_diagonalization_amd as hd
     from hamiltonian_diagonalization_amd import models
 
-    n, two_s = args.n, args.two_s
+    n, two_s, nev, tol, ncv = args.n, args.two_s, args.nev, args.tol, args.ncv
     log(f"sphere N={n} 2S={two_s} L_z=0: basis")
     t0 = time.perf_counter()
     basis = hd.FockBasis(hd.Sector.lz(0, two_s), n, two_s + 1)
     t_basis = time.perf_counter() - t0
-    tables = models.sphere_tables(two_s, models.coulomb_list(two_s))
+    vm = models.coulomb_list(two_s)
+    tables = models.sphere_tables(two_s, vm)
     log(f"dim={basis.dim}; build")
     t0 = time.perf_counter()
     ham = hd.Hamiltonian(basis, tables, storage=hd.STORAGE_FULL, values=(hd.F64, hd.F32, hd.CODED))
     t_build = time.perf_counter() - t0
     info = ham.info()
+    build_dev_ms = sum(info[k] for k in ("t_count_ms", "t_scan_ms", "t_fill_ms", "t_sort_ms"))
     log(f"nnz={info['nnz']} max_row={info['max_row']} build {t_build:.2f} s "
         f"(count {info['t_count_ms']:.1f} scan {info['t_scan_ms']:.1f} fill {info['t_fill_ms']:.1f} "
         f"sort {info['t_sort_ms']:.1f} ms)")
+    fmts = {"coded": hd.CODED, "f32": hd.F32, "f64": hd.F64}
+    tile, t_tile, tile_err = None, None, None
     t0 = time.perf_counter()
-    tile = ham.tile()
-    t_tile = time.perf_counter() - t0
-    log(f"tiled layout: {t_tile:.3f} s, {tile['lines'] / max(tile['entries'], 1):.4f} staged lines per entry")
+    try:
+        tile = ham.tile()
+        t_tile = time.perf_counter() - t0
+        fmts.update({"tiled": hd.TILED, "tiled_x32": hd.TILED_X32})
+        log(f"tiled layout: {t_tile:.3f} s, {tile['lines'] / max(tile['entries'], 1):.4f} staged lines per entry")
+    except hd.HDError as e:
+        tile_err = str(e)
+        log(f"no tiled layout: {tile_err}")
     ham.set_lanes(0)
-    fmts = {"coded": hd.CODED, "f32": hd.F32, "f64": hd.F64, "tiled": hd.TILED}
     sp_ms = {k: ham.spmv_time_ms(p, 20) for k, p in fmts.items()}
     sp_b = {k: ham.spmv_bytes(p) for k, p in fmts.items()}
     for k in fmts:
         log(f"SpMV {k}: {sp_ms[k]:.3f} ms, {sp_b[k] / sp_ms[k] / 1e6:.0f} GB/s")
 
-    # timed steps: the exact FP64 operator in whichever of its three formats
-    # (FP64 CRS, coded, tiled) has the fastest SpMV, random-normal start, seed 0
+    # the exact FP64 operator in whichever of its formats has the fastest SpMV
     exact = {"f64": (hd.F64, "fp64_crs"), "coded": (hd.CODED, "coded_fp64"), "tiled": (hd.TILED, "tiled_fp64")}
+    exact = {k: v for k, v in exact.items() if k in fmts}
     key = min(exact, key=lambda k: sp_ms[k])
     prec, op_name = exact[key]
-    for w in range(args.warmup):
-        r = ham.lanczos(nev=args.nev, tol=args.tol, precision=prec, seed=0, ncv=args.ncv)
-        log(f"warmup {w}: {r}")
-    hd.lib().hd_synchronize()
-    res = None
-    live_ms, live_mv = 0.0, 0  # HIP-event time of every SpMV launched inside the timed steps
-    t_all = time.perf_counter()
-    for s in range(args.steps):
-        t0 = time.perf_counter()
-        res = ham.lanczos(nev=args.nev, tol=args.tol, precision=prec, seed=0, ncv=args.ncv)
+
+    net, ckpt = None, checkpoint_path(n, two_s)
+    if ckpt and not args.skip_learned:
+        from hamiltonian_diagonalization_amd import learned as lmod
+        net = lmod.AmplitudeNet.load(ckpt)
+    params = [float(vm[1]), float(vm[3]), float(vm[5])]
+
+    def solve(form):
+        """One step of a form: (result, seconds, network seconds)."""
         hd.lib().hd_synchronize()
-        live_ms += res.time_spmv_ms
-        live_mv += res.matvecs
-        log(f"step {s}: {time.perf_counter() - t0:.3f} s {res}")
-    total = time.perf_counter() - t_all
-    step_s = total / max(args.steps, 1)
+        t0 = time.perf_counter()
+        t_net = 0.0
+        if form == "plain_fp64":
+            r = ham.lanczos(nev=nev, tol=tol, precision=prec, seed=0, ncv=ncv)
+        elif form == "mixed_fp32":
+            r = ham.lanczos(nev=nev, tol=tol, precision=hd.F32, seed=0, ncv=ncv)
+        else:
+            start = net.start_vector(basis, params)
+            hd.lib().hd_synchronize()
+            t_net = time.perf_counter() - t0
+            p = hd.F32 if form == "learned_mixed" else prec
+            r = ham.lanczos(nev=nev, tol=tol, precision=p, start=start, ncv=ncv)
+        hd.lib().hd_synchronize()
+        return r, time.perf_counter() - t0, t_net
+
+    forms = ["plain_fp64", "mixed_fp32"] + (["learned_mixed"] if net is not None else [])
+    for w in range(args.warmup):
+        for f in forms:
+            r, dt, _ = solve(f)
+            log(f"warmup {w} {f}: {dt:.3f} s {r}")
+    steps = {f: [] for f in forms}
+    results = {}
+    live_ms, live_mv = 0.0, 0  # HIP-event time of every SpMV launched in the plain steps
+    t_net_last = None
+    for s in range(args.steps):
+        for f in forms:
+            r, dt, t_net = solve(f)
+            steps[f].append(dt)
+            results[f] = r
+            if f == "plain_fp64":
+                live_ms += r.time_spmv_ms
+                live_mv += r.matvecs
+            if f == "learned_mixed":
+                t_net_last = t_net
+            log(f"step {s} {f}: {dt:.3f} s {r}")
+    plain = results["plain_fp64"]
+    per_form = {}
+    for f in forms:
+        r = results[f]
+        per_form[f] = {"s": round(float(np.mean(steps[f])), 4), "matvecs": r.matvecs,
+                       "matvecs_f32": r.matvecs_f32, "max_residual": float(r.residuals.max()),
+                       "abs_dE_vs_plain": [float(abs(a - b)) for a, b in zip(r.evals, plain.evals)],
+                       "verified": bool(r.converged and r.residuals.max() <= tol
+                                        and np.max(np.abs(r.evals - plain.evals)) < 1e-10)}
+    if t_net_last is not None:
+        per_form["learned_mixed"]["network_s"] = round(t_net_last, 4)
+    best = min((f for f in forms if per_form[f]["verified"]), key=lambda f: per_form[f]["s"])
+    step_s = per_form[best]["s"]
+    log(f"forms: " + ", ".join(f"{f} {per_form[f]['s']:.4f} s" for f in forms) + f"; value = {best}")
 
     cmp = {}
     if not args.skip_compare:
-        others = [(nm, p) for k, (p, nm) in exact.items() if k != key]
-        for name, p in others + [("f32_mixed", hd.F32)]:
-            r = ham.lanczos(nev=args.nev, tol=args.tol, precision=p, seed=0, ncv=args.ncv)
-            log(f"{name}: {r}")
-            cmp[name] = r
-    learned = learned_start(args, hd, basis, ham, prec, res) if not args.skip_learned else None
+        for k, (p, nm) in exact.items():
+            if k == key:
+                continue
+            r = ham.lanczos(nev=nev, tol=tol, precision=p, seed=0, ncv=ncv)
+            log(f"{nm}: {r}")
+            cmp[nm] = r
     err = None
-    refs = [cmp[nm] for k, (p, nm) in exact.items() if k != key and nm in cmp]
-    if refs:
-        err = [float(max(abs(a - r.evals[i]) for r in refs)) for i, a in enumerate(res.evals)]
-        log(f"|E_steps - E_other_fp64_formats| per eigenvalue: {err}")
+    if cmp:
+        err = [float(max(abs(a - r.evals[i]) for r in cmp.values())) for i, a in enumerate(plain.evals)]
+        log(f"|E_plain - E_other_fp64_formats| per eigenvalue: {err}")
+
+    learned = None
+    if net is not None:
+        from hamiltonian_diagonalization_amd import learned as lmod
+        psi0 = ham.eigenvector(0)
+        v = net.start_vector(basis, params).to_array(np.float64, basis.dim)
+        overlap = float(abs(v @ psi0) / (np.linalg.norm(v) * np.linalg.norm(psi0)))
+        del v, psi0
+        rl, dtl, tnl = solve("learned_fp64")
+        flop = 2.0 * (37 * 256 + 2 * 256 * 256 + 256 * 2) * basis.dim
+        learned = {"checkpoint": os.path.basename(ckpt), "params_v1_v3_v5": params,
+                   "note": "the Coulomb point lies outside the training box V1 in [0.5,1.5], V3 in [0,0.5], "
+                           "V5 in [0,0.2]; heldout is inside it",
+                   "overlap_with_ground_state": overlap,
+                   "learned_fp64": {"s": round(dtl, 4), "matvecs": rl.matvecs,
+                                    "max_residual": float(rl.residuals.max()),
+                                    "max_abs_dE_vs_plain": float(np.max(np.abs(rl.evals - plain.evals)))},
+                   "network": {"s": round(tnl, 4), "states": basis.dim, "flop": flop,
+                               "tflops": round(flop / tnl / 1e12, 2),
+                               "mfma_frac": round(flop / tnl / 1e12 / BF16_PEAK_TFLOPS, 4)}}
+        if args.heldout_points > 0:
+            learned["heldout"] = heldout_eval(hd, models, lmod, net, basis, two_s, nev, tol, ncv,
+                                              args.heldout_points)
 
     cpu = None
     if not args.skip_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline on {threads} threads, {args.cpu_rows} sample rows")
-        cpu = cpu_baseline(n, two_s, basis.states(), tables, res.matvecs, args.cpu_rows, threads)
+        cores = physical_cores()
+        threads = max(1, min(cores, CPU_POOL))
+        log(f"cpu baseline on {threads} threads ({cores} physical cores), {args.cpu_rows} sample rows")
+        cpu = cpu_baseline(n, two_s, tables, plain.matvecs, args.cpu_rows, threads, cores, nev, tol)
         log(f"cpu baseline: {cpu}")
 
     spmv_live_ms = live_ms / max(live_mv, 1)
     achieved = sp_b[key] / (spmv_live_ms * 1e-3) / 1e9
     lanes, spmv_threads = ham.schedule(prec)
-    kernel = {"coded": f"k_spmv_coded<double,{lanes},{spmv_threads}>", "f64": f"k_spmv<double,{lanes},{spmv_threads}>",
-              "tiled": f"k_spmv_tiled<{tile['rows_per_group']},{'true' if tile['entry_bytes'] == 3 else 'false'}>"}[key]
+    kernel = {"coded": f"k_spmv_coded<double,{lanes},{spmv_threads}>",
+              "f64": f"k_spmv<double,{lanes},{spmv_threads}>",
+              "tiled": (f"k_spmv_tiled<{tile['rows_per_group']},"
+                        f"{'true' if tile['entry_bytes'] == 3 else 'false'},double>") if tile else None}[key]
     traffic = measured_traffic(kernel, n, two_s)
+    # build: the final CRS in the three resident value copies (int32 column,
+    # FP64, FP32, 16-bit code per entry), row pointers, diagonal, basis reads
+    build_bytes = info["nnz"] * (4 + 8 + 4 + 2) + basis.dim * (8 + 8 + 2 * 8 + 8)
     out = {
         "metric": METRIC,
-        "value": round(step_s, 4),
+        "value": step_s,
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -266,31 +410,36 @@ def main():
         "dtype": "f64",
         "data": "synthetic (Coulomb pseudopotentials from the closed form; random-normal start, seed 0)",
         "config": {
-            "workload": f"FQHE sphere N={n} 2S={two_s} L_z=0 Coulomb, {args.nev} lowest eigenpairs, "
-                        f"residual {args.tol:g}",
+            "workload": f"FQHE sphere N={n} 2S={two_s} L_z=0 Coulomb, {nev} lowest eigenpairs, residual {tol:g}",
+            "form": best,
             "dim": basis.dim,
             "nnz_full": info["nnz"],
-            "ncv": args.ncv,
+            "ncv": ncv,
             "lanes_per_row": lanes,
             "threads_per_block": spmv_threads,
             "operator": {
                 "coded": "exact FP64 values in the coded CRS (int32 column + 16-bit dictionary code)",
                 "f64": "FP64 CRS (int64 row pointers, int32 columns, FP64 values)",
-                "tiled": f"exact FP64 values, tiled coded layout ({tile['rows_per_group']}-row groups, x lines "
-                         f"staged in LDS, {tile['entry_bytes']}-byte entries, block order "
-                         f"{tile['block_window']} groups per XCD)"}[key],
+                "tiled": (f"exact FP64 values, tiled coded layout ({tile['rows_per_group']}-row groups, x lines "
+                          f"staged in LDS, {tile['entry_bytes']}-byte entries, block order "
+                          f"{tile['block_window']} groups per XCD); FP32-input variant for the mixed cycles")
+                         if tile else None}[key],
         },
-        "solve_s": {op_name: round(step_s, 4)},
-        "matvecs": {op_name: res.matvecs},
-        "learned_start_s": learned["solve_s"] if learned else None,
+        "forms": per_form,
+        "plain_s": per_form["plain_fp64"]["s"],
+        "learned_start_s": per_form.get("learned_mixed", {}).get("s"),
         "learned_start": learned,
-        "max_residual": float(np.max(res.residuals)),
-        "eigenvalues": [float(e) for e in res.evals],
+        "eigenvalues": [float(e) for e in plain.evals],
         "abs_err_between_fp64_formats": err,
+        "other_fp64_formats": {nm: {"s": round(r.time_ms / 1e3, 4), "matvecs": r.matvecs} for nm, r in cmp.items()},
         "basis_s": round(t_basis, 4),
         "build_s": round(t_build, 3),
-        "tile_s": round(t_tile, 3),
+        "tile_s": round(t_tile, 3) if t_tile is not None else None,
+        "tile_error": tile_err,
         "build_ms": {k: round(info[k], 2) for k in ("t_count_ms", "t_scan_ms", "t_fill_ms", "t_sort_ms")},
+        "build_roofline": {"bound": "hbm", "bytes": build_bytes, "device_ms": round(build_dev_ms, 2),
+                           "achieved": round(build_bytes / (build_dev_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(build_bytes / (build_dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "spmv_ms": {k: round(v, 4) for k, v in sp_ms.items()},
         "spmv_gbs": {k: round(sp_b[k] / sp_ms[k] / 1e6, 1) for k in fmts},
         "roofline": {
@@ -307,9 +456,6 @@ def main():
         },
         "cpu_baseline": cpu,
     }
-    for name, r in cmp.items():
-        out["solve_s"][name] = round(r.time_ms / 1e3, 4)
-        out["matvecs"][name] = [r.matvecs_f32, r.matvecs_f64] if name == "f32_mixed" else r.matvecs
     print(json.dumps(out), flush=True)
 
 
