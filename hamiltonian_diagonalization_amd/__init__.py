@@ -79,7 +79,7 @@ class _LanczosStats(ctypes.Structure):
                 ("matvecs_f32", ctypes.c_int32), ("matvecs_f64", ctypes.c_int32),
                 ("residual", ctypes.c_double * 64), ("time_total_ms", ctypes.c_double),
                 ("time_spmv_ms", ctypes.c_double), ("time_ortho_ms", ctypes.c_double),
-                ("reorth_steps", ctypes.c_int32)]
+                ("reorth_steps", ctypes.c_int32), ("f32_cycles_discarded", ctypes.c_int32)]
 
 
 class _BatchOpts(ctypes.Structure):
@@ -474,10 +474,12 @@ class LanczosResult:
         self.time_spmv_ms = stats.time_spmv_ms
         self.time_ortho_ms = stats.time_ortho_ms
         self.reorth_steps = stats.reorth_steps
+        self.f32_cycles_discarded = stats.f32_cycles_discarded
 
     def __repr__(self):
         return (f"LanczosResult(evals={self.evals}, matvecs={self.matvecs} "
-                f"(f32 {self.matvecs_f32}), restarts={self.restarts}, reorth={self.reorth_steps}, "
+                f"(f32 {self.matvecs_f32}, discarded {self.f32_cycles_discarded}), restarts={self.restarts}, "
+                f"reorth={self.reorth_steps}, "
                 f"max_resid={self.residuals.max():.2e}, {self.time_ms:.1f} ms, "
                 f"spmv {self.time_spmv_ms:.1f} ms, ortho {self.time_ortho_ms:.1f} ms)")
 

@@ -30,7 +30,11 @@
 // rounding, |T| the largest projected row, s_i,new the pair's coefficients on
 // the cycle's new vectors); a cycle runs in FP32 only while the sum of these
 // estimates, with the next cycle's predicted coefficients, stays within
-// f32_budget * tol * max(1, |theta_i|).  Convergence is the usual Ritz test;
+// f32_budget * tol * max(1, |theta_i|).  The prediction cannot see an
+// eigenvalue that enters the wanted window during the cycle, so the estimate
+// is checked again with the cycle's actual coefficients, and a cycle beyond
+// the budget is discarded (its starting state was kept) and run in FP64.
+// Convergence is the usual Ritz test;
 // the eigenpairs are then checked with the FP64 operator, and hd_lanczos
 // repeats the solve with the FP64 operator should a residual miss tol.  All
 // reductions are FP64 and deterministic (per-block partials reduced in a fixed
@@ -40,6 +44,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -432,6 +438,7 @@ struct Lanczos {
     T* V = nullptr;     // (m+1) x n
     T* W = nullptr;     // n (operator output / work)
     T* tmp = nullptr;   // up to m x n (restart combination)
+    T* bak = nullptr;   // mixed precision: the restart state before an FP32 cycle
     F* x32 = nullptr;
     F* y32 = nullptr;
     T* partial = nullptr;
@@ -446,7 +453,7 @@ struct Lanczos {
 
     ~Lanczos()
     {
-        dfree(V); dfree(W); dfree(tmp); dfree(x32); dfree(y32); dfree(partial); dfree(coef);
+        dfree(V); dfree(W); dfree(tmp); dfree(bak); dfree(x32); dfree(y32); dfree(partial); dfree(coef);
         dfree(dpart); dfree(dscal); dfree(dY);
     }
 
@@ -704,9 +711,13 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
     const double eps32 = prec32 == HD_TILED_X32 ? u32 : u32 * (1.0 + std::sqrt((double)std::max<int64_t>(h->max_row, 1)));
     const int max_restarts = o.max_restarts > 0 ? o.max_restarts : 2000;
     int prec = L.prec64;
+    // HD_LANCZOS_TRACE=1: one line per restart cycle on stderr
+    const char* trace_env = std::getenv("HD_LANCZOS_TRACE");
+    const bool trace = trace_env && trace_env[0] == '1';
     int kk_keep = nev + (m - nev) / 2;
     kk_keep = std::min(std::max(kk_keep, nev + 1), m - 2 >= nev ? m - 2 : nev);
     kk_keep = std::max(kk_keep, nev);
+    if (mixed) HD_CHECK(dalloc_t(&L.bak, (size_t)(kk_keep + 1) * L.n, "mixed-precision restart backup"));
 
     // start vector
     if (o.start == HD_START_USER) {
@@ -751,8 +762,22 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
     // mixed precision: bound of the rounding each wanted pair has taken from
     // FP32 cycles, and its coefficient norm on the previous cycle's new vectors
     std::vector<double> err32(nev, 0.0), nb_prev(nev, 1.0);
+    // An FP32 cycle is speculative: the restart state it starts from is kept,
+    // and when the cycle's own coefficients put a wanted pair's rounding
+    // estimate beyond the budget (a pair that entered the wanted window during
+    // the cycle carries O(1) coefficients on its vectors), the cycle is
+    // discarded and run again with the FP64 operator.
+    std::vector<double> theta_b, arrow_b, err32_b, nb_prev_b;
+    int rollbacks = 0;
     for (;;) {
         const bool cyc32 = prec == prec32 && mixed;
+        if (cyc32) {
+            HD_HIP(hipMemcpyAsync(L.bak, L.vec(0), sizeof(T) * L.n * (size_t)(k + 1), hipMemcpyDeviceToDevice, L.st));
+            theta_b = theta;
+            arrow_b = arrow;
+            err32_b = err32;
+            nb_prev_b = nb_prev;
+        }
         // A rounding error of the FP32-input product is a vector A*delta with
         // delta ~ eps32 |x| spread over the sector: its overlap with a basis
         // vector is ~ eps32 |A| / sqrt(dim).  Such a cycle tolerates overlaps up
@@ -859,6 +884,7 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
         const double bm = beta[m - 1];
         bool all = true;
         bool allow32 = mixed;
+        bool overshoot = false;
         for (int i = 0; i < nev; ++i) {
             const double scale_i = std::max(1.0, std::fabs(w[i]));
             res_est[i] = std::fabs(bm * Y[(size_t)(m - 1) * m + i]) / scale_i;
@@ -867,7 +893,10 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
                 double nb2 = 0.0;
                 for (int j = k; j < m; ++j) nb2 += Y[(size_t)j * m + i] * Y[(size_t)j * m + i];
                 const double nb = std::sqrt(nb2);
-                if (cyc32) err32[i] += eps32 * anorm * nb;
+                if (cyc32) {
+                    err32[i] += eps32 * anorm * nb;
+                    if (err32[i] > budget * tol * scale_i) overshoot = true;
+                }
                 // the next cycle's coefficients, predicted from this cycle's decrease
                 const double pred = nb * std::min(1.0, nb / std::max(nb_prev[i], 1e-300));
                 nb_prev[i] = nb;
@@ -876,6 +905,27 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
             if (!(res_est[i] <= tol)) all = false;
         }
         ++restarts;
+        if (trace) {
+            double rmax = 0.0, emax = 0.0;
+            for (int i = 0; i < nev; ++i) {
+                rmax = std::max(rmax, res_est[i]);
+                emax = std::max(emax, err32[i] / std::max(1.0, std::fabs(w[i])));
+            }
+            std::fprintf(stderr, "[hd_lanczos] cycle %d op %d k %d ritz-resid %.3e fp32-err %.3e next-fp32 %d anorm %.3e full-passes %d%s\n",
+                         restarts, prec, k, rmax, emax, (int)(mixed && allow32 && !overshoot), anorm, reorth_steps,
+                         overshoot ? " (discarded: run again in FP64)" : "");
+        }
+        if (overshoot) {
+            HD_HIP(hipMemcpyAsync(L.vec(0), L.bak, sizeof(T) * L.n * (size_t)(k + 1), hipMemcpyDeviceToDevice, L.st));
+            theta = theta_b;
+            arrow = arrow_b;
+            err32 = err32_b;
+            nb_prev = nb_prev_b;
+            prec = L.prec64;
+            ++rollbacks;
+            if (restarts >= max_restarts) break;
+            continue;
+        }
         if (all) {
             converged = true;
             break;
@@ -914,6 +964,7 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
         double r;
         HD_CHECK(L.residual_norm(L.W, xi, w[i], &r));
         r /= std::max(1.0, std::fabs(w[i]));
+        if (trace) std::fprintf(stderr, "[hd_lanczos] pair %d: theta %.15g FP64 residual %.3e (Ritz estimate %.3e)\n", i, w[i], r, res_est[i]);
         if (stats && i < 64) stats->residual[i] = r;
         if (r <= tol) ++nconv;
     }
@@ -927,6 +978,7 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
         stats->time_spmv_ms = L.t_spmv;
         stats->time_ortho_ms = L.t_ortho;
         stats->reorth_steps = reorth_steps;
+        stats->f32_cycles_discarded = rollbacks;
         stats->time_total_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     }
@@ -973,6 +1025,7 @@ int hd_lanczos(hd_hamiltonian* h, const hd_lanczos_opts* opts, double* h_evals, 
     s2.matvecs_f64 += s1.matvecs_f64;
     s2.restarts += s1.restarts;
     s2.reorth_steps += s1.reorth_steps;
+    s2.f32_cycles_discarded += s1.f32_cycles_discarded;
     s2.time_total_ms += s1.time_total_ms;
     s2.time_spmv_ms += s1.time_spmv_ms;
     s2.time_ortho_ms += s1.time_ortho_ms;

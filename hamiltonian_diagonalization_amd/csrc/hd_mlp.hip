@@ -185,29 +185,151 @@ __global__ void k_to_bf16(int rows, int cols, const float* __restrict__ w, __bf1
     wbT[(size_t)c * rows + r] = v;
 }
 
-// psi = sign * exp(log|psi| - shift) into the FP64 start vector.
-__global__ void k_to_amplitude(int64_t n, const float* __restrict__ out, float shift, double* __restrict__ psi)
+// ---------------------------------------------------------------- inference
+//
+// The whole network for kIT states per workgroup in one launch: each of the
+// kIT/32 waves owns 32 states (one MFMA row tile) and keeps their features and
+// activations in LDS (32 x 256 BF16); the weights are staged through LDS in
+// K chunks of kIKc shared by the waves.  Every output is the MFMA sequence of
+// k_gemm (same 32x32x16 tiles, K ascending in steps of 16) with the same
+// epilogues, so the outputs equal the layer-by-layer path bit for bit, while
+// HBM sees only the states and the outputs.
+constexpr int kIT = 128;            // states per workgroup
+constexpr int kIWaves = kIT / 32;   // waves per workgroup, one 32-state strip each
+constexpr int kIKc = 32;            // K chunk of the staged weights
+
+struct InferArgs {
+    int64_t n;
+    const uint64_t* states;
+    const float* params;  // n_params values, per state when per_sample
+    int per_sample, n_bits, n_params;
+    const __bf16* w[4];   // BF16 weights, padded shapes (out x in)
+    const float* b[4];
+    int n_out;            // live outputs per state (2)
+    float* out;           // n x n_out
+    float* partial_max;   // per workgroup: largest output 0 over its states
+};
+
+__global__ void __launch_bounds__(kIWaves * 64) k_mlp_infer(InferArgs g)
 {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n) return;
-    const float o0 = out[b * kOutPad], o1 = out[b * kOutPad + 1];
-    const double a = exp((double)(o0 - shift));
-    psi[b] = o1 >= 0.f ? a : -a;
+    __shared__ __bf16 act[kIT * kW];
+    __shared__ __bf16 wc[kW * kIKc];
+    __shared__ float red[kIWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 31, kh = (lane >> 5) * 8;
+    const int64_t s0 = (int64_t)blockIdx.x * kIT + wave * 32;
+    __bf16* strip = act + (size_t)wave * 32 * kW;
+    {   // features of state s0 + r: lane halves build columns 0-31 and 32-63
+        const int64_t b = s0 + r;
+        const bool live = b < g.n;
+        const uint64_t st = live ? g.states[b] : 0ull;
+        const float* p = g.params + (g.per_sample && live ? b * g.n_params : 0);
+        for (int k = (lane >> 5) * 32; k < (lane >> 5) * 32 + 32; ++k) {
+            float f = 0.f;
+            if (live) {
+                if (k < g.n_bits) f = (float)((st >> k) & 1ull);
+                else if (k < g.n_bits + g.n_params) f = p[k - g.n_bits];
+            }
+            strip[r * kW + k] = (__bf16)f;
+        }
+    }
+    // hidden layers: 64 -> 256 -> 256 -> 256, bias + SiLU
+    for (int l = 0; l < 3; ++l) {
+        const int K = l == 0 ? kIn : kW;
+        f32x16 acc[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc[t] = f32x16{};
+        for (int kc = 0; kc < K; kc += kIKc) {
+            __syncthreads();  // the previous chunk is consumed
+            for (int i = threadIdx.x; i < kW * (kIKc / 8); i += kIWaves * 64) {
+                const int nr = i / (kIKc / 8), kp = (i % (kIKc / 8)) * 8;
+                *reinterpret_cast<bf16x8*>(&wc[nr * kIKc + kp]) =
+                    *reinterpret_cast<const bf16x8*>(g.w[l] + (size_t)nr * K + kc + kp);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k16 = 0; k16 < kIKc; k16 += 16) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&strip[r * kW + kc + k16 + kh]);
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&wc[(t * 32 + r) * kIKc + k16 + kh]);
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc[t], 0, 0, 0);
+                }
+            }
+        }
+        // every read of this wave's strip has been consumed by the MFMAs above;
+        // other waves never read it
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int row = (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+                const int col = t * 32 + r;
+                strip[row * kW + col] = (__bf16)silu(acc[t][i] + g.b[l][col]);
+            }
+    }
+    // output layer: 256 -> the first 32 (padded) outputs, bias
+    f32x16 acc = f32x16{};
+    for (int kc = 0; kc < kW; kc += kIKc) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < 32 * (kIKc / 8); i += kIWaves * 64) {
+            const int nr = i / (kIKc / 8), kp = (i % (kIKc / 8)) * 8;
+            *reinterpret_cast<bf16x8*>(&wc[nr * kIKc + kp]) =
+                *reinterpret_cast<const bf16x8*>(g.w[3] + (size_t)nr * kW + kc + kp);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k16 = 0; k16 < kIKc; k16 += 16) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&strip[r * kW + kc + k16 + kh]);
+            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&wc[r * kIKc + k16 + kh]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc, 0, 0, 0);
+        }
+    }
+    float mx = -3.0e38f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int row = (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+        const int64_t b = s0 + row;
+        if (r < g.n_out && b < g.n) {
+            const float v = acc[i] + g.b[3][r];
+            g.out[b * g.n_out + r] = v;
+            if (r == 0) mx = fmaxf(mx, v);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) red[wave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = red[0];
+        for (int w = 1; w < kIWaves; ++w) m = fmaxf(m, red[w]);
+        g.partial_max[blockIdx.x] = m;
+    }
 }
 
-__global__ void k_max_logamp(int64_t n, const float* __restrict__ out, float* __restrict__ partial)
+// out[0] = max of n floats (one block, fixed order)
+__global__ void __launch_bounds__(256) k_max_floats(int64_t n, const float* __restrict__ v, float* __restrict__ out)
 {
     __shared__ float red[256];
     float m = -3.0e38f;
-    for (int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x; b < n; b += (int64_t)gridDim.x * 256)
-        m = fmaxf(m, out[b * kOutPad]);
+    for (int64_t i = threadIdx.x; i < n; i += 256) m = fmaxf(m, v[i]);
     red[threadIdx.x] = m;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// psi_b = exp(o0 - max o0) sign(o1) from outputs with n_out per state
+__global__ void k_amplitude(int64_t n, const float* __restrict__ out, int n_out, const float* __restrict__ shift,
+                            double* __restrict__ psi)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const float o0 = out[b * n_out], o1 = out[b * n_out + 1];
+    const double a = exp((double)(o0 - shift[0]));
+    psi[b] = o1 >= 0.f ? a : -a;
 }
 
 }  // namespace hd
@@ -316,6 +438,26 @@ GemmArgs ga(int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int 
     g.M = M; g.N = N; g.K = K;
     g.A = A; g.lda = lda; g.B = B; g.ldb = ldb;
     return g;
+}
+
+InferArgs infer_args(const hd_mlp* m, int64_t n, const uint64_t* states, const float* params, int per_sample,
+                     float* out, float* partial_max)
+{
+    InferArgs a{};
+    a.n = n;
+    a.states = states;
+    a.params = params;
+    a.per_sample = per_sample;
+    a.n_bits = m->n_bits;
+    a.n_params = m->n_params;
+    for (int l = 0; l < 4; ++l) {
+        a.w[l] = m->wb[l];
+        a.b[l] = m->b[l];
+    }
+    a.n_out = m->live_rows[3];
+    a.out = out;
+    a.partial_max = partial_max;
+    return a;
 }
 
 // forward over npad rows already featurized in h[0] (and hT[0] when training)
@@ -541,23 +683,25 @@ int hd_mlp_set_weights(hd_mlp* m, const float* h_in)
 int hd_mlp_forward(hd_mlp* m, const uint64_t* h_states, const float* h_params, int64_t n, float* h_out)
 {
     if (!m || !h_states || !h_params || !h_out || n <= 0) return fail(HD_ERR_ARG, "hd_mlp_forward: bad argument");
-    const int64_t npad = pad64(n);
-    HD_CHECK(mlp_work(m, npad, false));
     const uint64_t* ds;
     const float *dp, *dt;
     HD_CHECK(stage_batch(m, h_states, h_params, nullptr, n, true, &ds, &dp, &dt, 1));
-    hipLaunchKernelGGL(k_featurize, dim3((unsigned)((npad + 255) / 256)), dim3(256), 0, m->stream, n, npad, ds, dp, 1,
-                       m->n_bits, m->n_params, m->h[0], (__bf16*)nullptr);
-    mlp_forward_core(m, npad, false);
-    HD_HIP(hipGetLastError());
-    std::vector<float> o((size_t)n * kOutPad);
-    HD_HIP(hipMemcpyAsync(o.data(), m->out, o.size() * 4, hipMemcpyDeviceToHost, m->stream));
-    HD_HIP(hipStreamSynchronize(m->stream));
-    for (int64_t b = 0; b < n; ++b) {
-        h_out[2 * b] = o[(size_t)b * kOutPad];
-        h_out[2 * b + 1] = o[(size_t)b * kOutPad + 1];
+    const int64_t nblk = (n + kIT - 1) / kIT;
+    float *dout = nullptr, *dmax = nullptr;
+    HD_CHECK(dalloc_t(&dout, (size_t)n * 2, "MLP outputs"));
+    int st = dalloc_t(&dmax, (size_t)nblk, "MLP output maxima");
+    if (st == HD_OK) {
+        InferArgs a = infer_args(m, n, ds, dp, 1, dout, dmax);
+        hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)nblk), dim3(kIWaves * 64), 0, m->stream, a);
+        HipErrors q;
+        HD_ENQ(q, hipGetLastError());
+        HD_ENQ(q, hipMemcpyAsync(h_out, dout, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, m->stream));
+        HD_ENQ(q, hipStreamSynchronize(m->stream));
+        st = q.status();
     }
-    return HD_OK;
+    dfree(dout);
+    dfree(dmax);
+    return st;
 }
 
 int hd_mlp_gradients(hd_mlp* m, const uint64_t* h_states, const float* h_params, const float* h_targets, int64_t n,
@@ -592,57 +736,33 @@ int hd_mlp_start_vector(hd_mlp* m, const hd_basis* basis, const float* h_params,
 {
     if (!m || !basis || !h_params || !d_vec) return fail(HD_ERR_ARG, "hd_mlp_start_vector: bad argument");
     if (basis->Norb != m->n_bits) return fail(HD_ERR_ARG, "network input width differs from the orbital count");
-    const int64_t batch = 1 << 18;
-    HD_CHECK(mlp_work(m, batch, false));
-    float* dpar = nullptr;
-    HD_CHECK(dalloc_t(&dpar, (size_t)std::max(m->n_params, 1), "MLP parameters"));
-    if (hipMemcpy(dpar, h_params, sizeof(float) * m->n_params, hipMemcpyHostToDevice) != hipSuccess) {
-        dfree(dpar);
-        return fail(HD_ERR_HIP, "start vector: parameter upload failed");
-    }
-    HipErrors q;
-    // pass 1: network outputs for all states, kept as log-amplitudes in d_vec
-    // (as float pairs would double the memory; store psi after a shift pass)
-    float* allout = nullptr;
-    int st = dalloc_t(&allout, (size_t)basis->dim * 2, "start vector outputs");
-    if (st != HD_OK) {
-        dfree(dpar);
-        return st;
-    }
-    float gmax = -3.0e38f;
-    for (int64_t b0 = 0; b0 < basis->dim && st == HD_OK; b0 += batch) {
-        const int64_t n = std::min(batch, basis->dim - b0);
-        const int64_t npad = pad64(n);
-        hipLaunchKernelGGL(k_featurize, dim3((unsigned)((npad + 255) / 256)), dim3(256), 0, m->stream, n, npad,
-                           basis->d_states + b0, dpar, 0, m->n_bits, m->n_params, m->h[0], (__bf16*)nullptr);
-        mlp_forward_core(m, npad, false);
-        HD_ENQ(q, hipMemcpy2DAsync(allout + 2 * b0, 2 * sizeof(float), m->out, kOutPad * sizeof(float),
-                                   2 * sizeof(float), n, hipMemcpyDeviceToDevice, m->stream));
-        const int nb = 256;
-        hipLaunchKernelGGL(k_max_logamp, dim3(nb), dim3(256), 0, m->stream, n, m->out, m->partial);
-        std::vector<float> part(nb);
-        if (hipMemcpyAsync(part.data(), m->partial, sizeof(float) * nb, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
-            hipStreamSynchronize(m->stream) != hipSuccess)
-            st = fail(HD_ERR_HIP, "start vector pass failed");
-        if (st == HD_OK) st = q.status();
-        for (float p : part) gmax = std::max(gmax, p);
-    }
+    const int64_t n = basis->dim;
+    if (n == 0) return HD_OK;
+    const int64_t nblk = (n + kIT - 1) / kIT;
+    float *dpar = nullptr, *dout = nullptr, *dmax = nullptr;
+    int st = dalloc_t(&dpar, (size_t)std::max(m->n_params, 1) + 1, "MLP parameters and shift");
+    if (st == HD_OK) st = dalloc_t(&dout, (size_t)n * 2, "start vector outputs");
+    if (st == HD_OK) st = dalloc_t(&dmax, (size_t)nblk, "start vector output maxima");
     if (st == HD_OK) {
-        // outputs are stored as (o0, o1) pairs: reuse k_to_amplitude with stride 2
-        // by viewing allout as an n x 2 array
-        for (int64_t b0 = 0; b0 < basis->dim; b0 += batch) {
-            const int64_t n = std::min(batch, basis->dim - b0);
-            HD_ENQ(q, hipMemcpy2DAsync(m->out, kOutPad * sizeof(float), allout + 2 * b0, 2 * sizeof(float),
-                                       2 * sizeof(float), n, hipMemcpyDeviceToDevice, m->stream));
-            hipLaunchKernelGGL(k_to_amplitude, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, n, m->out,
-                               gmax, d_vec + b0);
-        }
-        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(m->stream) != hipSuccess)
-            st = fail(HD_ERR_HIP, "start vector assembly failed");
-        if (st == HD_OK) st = q.status();
+        // one pass of the fused network over the sector, then
+        // psi = exp(o0 - max o0) sign(o1)
+        HipErrors q;
+        HD_ENQ(q, hipMemcpyAsync(dpar, h_params, sizeof(float) * m->n_params, hipMemcpyHostToDevice, m->stream));
+        InferArgs a = infer_args(m, n, basis->d_states, dpar, 0, dout, dmax);
+        hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)nblk), dim3(kIWaves * 64), 0, m->stream, a);
+        HD_ENQ(q, hipGetLastError());
+        float* shift = dpar + std::max(m->n_params, 1);
+        hipLaunchKernelGGL(k_max_floats, dim3(1), dim3(256), 0, m->stream, nblk, dmax, shift);
+        HD_ENQ(q, hipGetLastError());
+        hipLaunchKernelGGL(k_amplitude, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, n, dout, 2, shift,
+                           d_vec);
+        HD_ENQ(q, hipGetLastError());
+        HD_ENQ(q, hipStreamSynchronize(m->stream));
+        st = q.status();
     }
-    dfree(allout);
     dfree(dpar);
+    dfree(dout);
+    dfree(dmax);
     return st;
 }
 

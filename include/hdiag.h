@@ -379,7 +379,9 @@ typedef struct {
                              matrices with a tiled layout, else the FP32-value CRS); a cycle
                              takes the FP32 operator only while the rounding it adds to every
                              wanted residual, estimated from the Ritz coefficients on the
-                             cycle's new vectors, stays within f32_budget of tol.  The
+                             cycle's new vectors, stays within f32_budget of tol: predicted
+                             before the cycle, and checked after it, when a cycle that
+                             overshoots is discarded and run again in FP64.  The
                              eigenpairs are checked with the FP64 operator; when a residual
                              misses tol the solve is repeated with the FP64 operator. */
     double f32_budget;    /* HD_F32: share of tol the FP32 cycles may take; 0 = 1 */
@@ -404,6 +406,9 @@ typedef struct {
     double time_spmv_ms;
     double time_ortho_ms;
     int32_t reorth_steps; /* steps that orthogonalised against the whole basis */
+    int32_t f32_cycles_discarded; /* HD_F32: FP32 cycles run again in FP64 (their rounding
+                                     estimate exceeded the budget once their coefficients
+                                     were known) */
 } hd_lanczos_stats;
 
 /* HamiltonianBase::LanczosDiagonalize -> ArpackWrapper::DiagonalizeSymmetricMatrix

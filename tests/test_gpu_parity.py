@@ -380,6 +380,43 @@ def test_lanczos_config1(oracle):
     assert np.max(np.abs(rc.evals - want) / np.maximum(1, np.abs(want))) < 1e-10
 
 
+def test_tiled_x32_product_exact_for_fp32_inputs(oracle):
+    """HD_TILED_X32 reads x in FP32 and sums the FP64 dictionary values in FP64:
+    its product equals the FP64 tiled product of the same (FP32-valued) x bit
+    for bit, and the oracle's naive SpMV of it to 1e-13."""
+    say("tiled x32 product N=8 2S=21")
+    b, t = models.sphere_problem(8, 21)
+    up = oracle.build(8, 22, b.states(), oracle.LAYOUT_SPHERE, 22, t.ktable, t.vtable)
+    h = hd.Hamiltonian(b, t, values=(hd.CODED,))
+    for rows, nbytes in ((128, 4), (64, 3)):
+        h.tile(rows, nbytes)
+        x32 = np.random.default_rng(rows).normal(size=b.dim).astype(np.float32)
+        got = h.spmv(x32, precision=hd.TILED_X32)
+        assert got.dtype == np.float64
+        assert np.array_equal(got, h.spmv(x32.astype(np.float64), precision=hd.TILED))
+        want = oracle.spmv_sym(up, x32.astype(np.float64))
+        assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+
+
+@pytest.mark.parametrize("tiled", [True, False])
+def test_mixed_precision_lanczos(oracle, tiled):
+    """HD_F32: the restart cycles after the coefficients have become small run
+    on the FP32-input operator (tiled, or the FP32-value CRS); the eigenpairs
+    equal the dense ones to 1e-10 with FP64 residuals below tol."""
+    say("mixed-precision Lanczos N=8 2S=21, tiled", tiled)
+    b, t = models.sphere_problem(8, 21)
+    full = oracle.build(8, 22, b.states(), oracle.LAYOUT_SPHERE, 22, t.ktable, t.vtable, full=True)
+    want = _dense_eigs(oracle, full, b.dim, 5)
+    h = hd.Hamiltonian(b, t, values=(hd.F64, hd.F32, hd.CODED))
+    if tiled:
+        h.tile()
+    r = h.lanczos(nev=5, tol=1e-10, precision=hd.F32, seed=0)
+    say(r)
+    assert r.converged and np.all(r.residuals <= 1e-10)
+    assert r.matvecs_f32 > 0 and r.matvecs_f64 > 0
+    assert np.max(np.abs(r.evals - want) / np.maximum(1, np.abs(want))) < 1e-10
+
+
 @pytest.mark.parametrize("case", ["sphere", "flux_lattice"])
 def test_partial_vs_full_reorthogonalisation(oracle, case):
     """Partial reorthogonalisation (the default) reaches the same eigenpairs as

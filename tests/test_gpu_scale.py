@@ -242,23 +242,32 @@ def test_batch_config4_full_sweep(oracle):
 @pytest.mark.timeout(600)
 def test_learned_start_config5_heldout():
     """The checkpoint trained at N=12, 2S=33 on 128 points of
-    [0.5,1.5] x [0,0.5] x [0,0.2]: on held-out points (seed 1) the solve from the
-    network's start vector reaches residual 1e-10 for the 5 lowest eigenpairs
-    with the eigenvalues of the random start to 1e-10."""
+    [0.5,1.5] x [0,0.5] x [0,0.2]: on 4 held-out points (seed 1) the solves
+    from the network's start vector, with the exact FP64 operator and in the
+    mixed form, reach residual 1e-10 for the 5 lowest eigenpairs with the
+    eigenvalues of the random start to 1e-10.  At the third point two
+    eigenvalues lie 0.0022 apart and the lower one enters the Krylov space
+    late from this start: a speculative FP32 cycle is discarded there and the
+    solve still needs no rerun."""
     from hamiltonian_diagonalization_amd import learned
     net = learned.AmplitudeNet.load(CHECKPOINT)
     basis = hd.FockBasis(hd.Sector.lz(0, 33), 12, 34)
-    for v in learned.sample_params(np.random.default_rng(1), 2):
+    for v in learned.sample_params(np.random.default_rng(1), 4):
         say("learned start N=12 2S=33 at", v)
         vm = models.coulomb_list(33, {1: float(v[0]), 3: float(v[1]), 5: float(v[2])})
         h = hd.Hamiltonian(basis, models.sphere_tables(33, vm), values=(hd.CODED,))
         h.tile()
-        plain = h.lanczos(nev=5, tol=1e-10, seed=0, precision=hd.TILED)
+        plain = h.lanczos(nev=5, tol=1e-10, seed=0, precision=hd.TILED, ncv=48)
         start = net.start_vector(basis, v)
-        lr = h.lanczos(nev=5, tol=1e-10, start=start, precision=hd.TILED)
+        runs = {"learned_fp64": h.lanczos(nev=5, tol=1e-10, start=start, precision=hd.TILED, ncv=48),
+                "learned_mixed": h.lanczos(nev=5, tol=1e-10, start=start, precision=hd.F32, ncv=48),
+                "mixed": h.lanczos(nev=5, tol=1e-10, seed=0, precision=hd.F32, ncv=48)}
         say("plain", plain)
-        say("learned", lr)
-        assert plain.converged and lr.converged
-        assert np.all(plain.residuals <= 1e-10) and np.all(lr.residuals <= 1e-10)
-        assert np.max(np.abs(lr.evals - plain.evals) / np.maximum(1, np.abs(plain.evals))) < 1e-10
+        assert plain.converged and np.all(plain.residuals <= 1e-10)
+        for name, r in runs.items():
+            say(name, r)
+            assert r.converged and np.all(r.residuals <= 1e-10)
+            assert np.max(np.abs(r.evals - plain.evals) / np.maximum(1, np.abs(plain.evals))) < 1e-10
+            # no rerun of the whole solve: at most the plain count plus discarded cycles
+            assert r.matvecs <= plain.matvecs + 22 * (r.f32_cycles_discarded + 2), name
         del h
