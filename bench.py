@@ -168,7 +168,7 @@ def measured_traffic(kernel, n, two_s):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (calibrated against a plain copy kernel),
     or None when that kernel and workload were not profiled."""
-    path = os.path.join(ROOT, "profiles", f"spmv_n{n}_2s{two_s}_hbm_traffic.txt")
+    path = os.path.join(ROOT, "profiles", f"stages_n{n}_2s{two_s}_hbm_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -179,8 +179,13 @@ def measured_traffic(kernel, n, two_s):
 
 
 def checkpoint_path(n, two_s):
-    p = os.path.join(ROOT, "hamiltonian_diagonalization_amd", "checkpoints", f"start_n{n}_2s{two_s}.npz")
-    return p if os.path.exists(p) else None
+    """The amplitude network for this sector: the one trained on the 5 lowest
+    eigenvectors when present, else the ground-state one."""
+    for stem in (f"start5_n{n}_2s{two_s}.npz", f"start_n{n}_2s{two_s}.npz"):
+        p = os.path.join(ROOT, "hamiltonian_diagonalization_amd", "checkpoints", stem)
+        if os.path.exists(p):
+            return p
+    return None
 
 
 def heldout_eval(hd, models, lmod, net, basis, two_s, nev, tol, ncv, points):
@@ -392,7 +397,8 @@ def main():
               "f64": f"k_spmv<double,{lanes},{spmv_threads}>",
               "tiled": (f"k_spmv_tiled<{tile['rows_per_group']},"
                         f"{'true' if tile['entry_bytes'] == 3 else 'false'},double>") if tile else None}[key]
-    traffic = measured_traffic(kernel, n, two_s)
+    # the profiles name kernels without spaces after commas
+    traffic = measured_traffic(kernel.replace(", ", ","), n, two_s) if kernel else None
     # build: the final CRS in the three resident value copies (int32 column,
     # FP64, FP32, 16-bit code per entry), row pointers, diagonal, basis reads
     build_bytes = info["nnz"] * (4 + 8 + 4 + 2) + basis.dim * (8 + 8 + 2 * 8 + 8)

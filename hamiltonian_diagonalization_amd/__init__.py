@@ -154,6 +154,8 @@ SIGNATURES = {
     "hd_lanczos_vectors": (ctypes.c_int, [_P, _I32, _P, _P]),
     "hd_full_diagonalize": (ctypes.c_int, [_P, _I32, _P]),
     "hd_mlp_create": (ctypes.c_int, [_I32, _I32, _U64, _PP]),
+    "hd_mlp_create_states": (ctypes.c_int, [_I32, _I32, _I32, _U64, _PP]),
+    "hd_mlp_num_states": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
     "hd_mlp_destroy": (ctypes.c_int, [_P]),
     "hd_mlp_num_weights": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
     "hd_mlp_get_weights": (ctypes.c_int, [_P, _P]),

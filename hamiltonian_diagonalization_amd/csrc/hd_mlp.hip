@@ -4,9 +4,10 @@
 // No reference counterpart: the reference's only warm-start hook is the ARPACK
 // initial-vector file (src/utilities/wrappers/arpack_wrapper.hpp:753-774,
 // --use-initial).  The network maps an occupation bitstring plus Hamiltonian
-// parameters to (log|psi|, sign logit):
+// parameters to (log|psi_s|, sign logit_s) for S amplitudes (the S lowest
+// eigenvectors it was trained on; S = 1: the ground state):
 //     features (n_bits occupations, n_params parameters, zero padded to 64)
-//     -> 256 -> 256 -> 256 (SiLU) -> 2
+//     -> 256 -> 256 -> 256 (SiLU) -> 2 S
 //
 // All layers run through one GEMM kernel, C[M x N] = A[M x K] . B[N x K]^T with
 // both operands K-contiguous in BF16, FP32 accumulation in
@@ -107,9 +108,10 @@ __global__ void k_featurize(int64_t n, int64_t npad, const uint64_t* __restrict_
     }
 }
 
-// loss and dL/dout for out = (log|psi|, sign logit) against targets
-// (log|psi|, sign, weight); lambda weights the sign term.
-__global__ void k_loss(int64_t n, int64_t npad, const float* __restrict__ out, const float* __restrict__ tgt,
+// loss and dL/dout for out = (log|psi_s|, sign logit_s), s < S, against
+// targets (log|psi_s|, sign_s, weight_s) per state (3 S floats per sample);
+// lambda weights the sign term.
+__global__ void k_loss(int64_t n, int64_t npad, int S, const float* __restrict__ out, const float* __restrict__ tgt,
                        float lambda, float inv_n, __bf16* __restrict__ d4, __bf16* __restrict__ d4T,
                        float* __restrict__ partial)
 {
@@ -117,21 +119,25 @@ __global__ void k_loss(int64_t n, int64_t npad, const float* __restrict__ out, c
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float l = 0.f;
     if (b < npad) {
-        float g0 = 0.f, g1 = 0.f;
-        if (b < n) {
-            const float o0 = out[b * kOutPad], o1 = out[b * kOutPad + 1];
-            const float a = tgt[3 * b], s = tgt[3 * b + 1], w = tgt[3 * b + 2];
-            const float e = o0 - a;
-            const float m = -s * o1;  // softplus(m) = log(1 + exp(-s o1))
-            const float sp = m > 20.f ? m : log1pf(__expf(m));
-            l = w * (e * e + lambda * sp) * inv_n;
-            g0 = 2.f * w * e * inv_n;
-            g1 = -lambda * w * s * (1.f / (1.f + __expf(-m))) * inv_n;
-        }
         for (int c = 0; c < kOutPad; ++c) {
-            const float v = c == 0 ? g0 : (c == 1 ? g1 : 0.f);
-            d4[b * kOutPad + c] = (__bf16)v;
-            d4T[(size_t)c * npad + b] = (__bf16)v;
+            const int st = c >> 1;
+            float g = 0.f;
+            if (b < n && st < S) {
+                const float o0 = out[b * kOutPad + 2 * st], o1 = out[b * kOutPad + 2 * st + 1];
+                const float* t = tgt + (b * S + st) * 3;
+                const float a = t[0], sg = t[1], w = t[2];
+                const float e = o0 - a;
+                const float m = -sg * o1;  // softplus(m) = log(1 + exp(-s o1))
+                if (c & 1) {
+                    g = -lambda * w * sg * (1.f / (1.f + __expf(-m))) * inv_n;
+                } else {
+                    const float sp = m > 20.f ? m : log1pf(__expf(m));
+                    l += w * (e * e + lambda * sp) * inv_n;
+                    g = 2.f * w * e * inv_n;
+                }
+            }
+            d4[b * kOutPad + c] = (__bf16)g;
+            d4T[(size_t)c * npad + b] = (__bf16)g;
         }
     }
     red[threadIdx.x] = l;
@@ -205,16 +211,16 @@ struct InferArgs {
     int per_sample, n_bits, n_params;
     const __bf16* w[4];   // BF16 weights, padded shapes (out x in)
     const float* b[4];
-    int n_out;            // live outputs per state (2)
+    int n_out;            // live outputs per state: 2 per amplitude (log|psi|, sign logit)
     float* out;           // n x n_out
-    float* partial_max;   // per workgroup: largest output 0 over its states
+    float* partial_max;   // per workgroup and amplitude s: largest output 2s over its states
 };
 
 __global__ void __launch_bounds__(kIWaves * 64) k_mlp_infer(InferArgs g)
 {
     __shared__ __bf16 act[kIT * kW];
     __shared__ __bf16 wc[kW * kIKc];
-    __shared__ float red[kIWaves];
+    __shared__ float red[kIWaves][32];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 31, kh = (lane >> 5) * 8;
     const int64_t s0 = (int64_t)blockIdx.x * kIT + wave * 32;
@@ -285,7 +291,7 @@ __global__ void __launch_bounds__(kIWaves * 64) k_mlp_infer(InferArgs g)
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc, 0, 0, 0);
         }
     }
-    float mx = -3.0e38f;
+    float mx = -3.0e38f;  // this lane's column (even columns are log-amplitudes)
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const int row = (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
@@ -293,43 +299,82 @@ __global__ void __launch_bounds__(kIWaves * 64) k_mlp_infer(InferArgs g)
         if (r < g.n_out && b < g.n) {
             const float v = acc[i] + g.b[3][r];
             g.out[b * g.n_out + r] = v;
-            if (r == 0) mx = fmaxf(mx, v);
+            mx = fmaxf(mx, v);
         }
     }
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-    if (lane == 0) red[wave] = mx;
+    mx = fmaxf(mx, __shfl_xor(mx, 32, 64));  // the two half-waves hold the same column
+    if (lane < 32) red[wave][lane] = mx;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        float m = red[0];
-        for (int w = 1; w < kIWaves; ++w) m = fmaxf(m, red[w]);
-        g.partial_max[blockIdx.x] = m;
+    if ((int)threadIdx.x < g.n_out / 2) {
+        const int c = 2 * threadIdx.x;
+        float m = red[0][c];
+        for (int w = 1; w < kIWaves; ++w) m = fmaxf(m, red[w][c]);
+        g.partial_max[(int64_t)blockIdx.x * (g.n_out / 2) + threadIdx.x] = m;
     }
 }
 
-// out[0] = max of n floats (one block, fixed order)
-__global__ void __launch_bounds__(256) k_max_floats(int64_t n, const float* __restrict__ v, float* __restrict__ out)
+// shift[s] = max over workgroups of partial[w * S + s] (one block per amplitude, fixed order)
+__global__ void __launch_bounds__(256) k_max_states(int64_t nblk, int S, const float* __restrict__ partial,
+                                                    float* __restrict__ shift)
 {
     __shared__ float red[256];
+    const int s = blockIdx.x;
     float m = -3.0e38f;
-    for (int64_t i = threadIdx.x; i < n; i += 256) m = fmaxf(m, v[i]);
+    for (int64_t i = threadIdx.x; i < nblk; i += 256) m = fmaxf(m, partial[i * S + s]);
     red[threadIdx.x] = m;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = red[0];
+    if (threadIdx.x == 0) shift[s] = red[0];
 }
 
-// psi_b = exp(o0 - max o0) sign(o1) from outputs with n_out per state
-__global__ void k_amplitude(int64_t n, const float* __restrict__ out, int n_out, const float* __restrict__ shift,
-                            double* __restrict__ psi)
+// part[blk * S + s] = sum over this block's states of exp(2 (o_2s - shift_s))
+__global__ void __launch_bounds__(256) k_norm2_states(int64_t n, int S, const float* __restrict__ out,
+                                                      const float* __restrict__ shift, double* __restrict__ part)
+{
+    __shared__ double red[256];
+    for (int s = 0; s < S; ++s) {
+        double acc = 0.0;
+        for (int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x; b < n; b += (int64_t)gridDim.x * 256)
+            acc += exp(2.0 * (double)(out[b * 2 * S + 2 * s] - shift[s]));
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * S + s] = red[0];
+        __syncthreads();
+    }
+}
+
+// inv[s] = 1 / sqrt(sum over blocks of part[blk * S + s]) (one thread per amplitude, block order)
+__global__ void k_inv_norm_states(int nb, int S, const double* __restrict__ part, double* __restrict__ inv)
+{
+    const int s = threadIdx.x;
+    if (s >= S) return;
+    double t = 0.0;
+    for (int b = 0; b < nb; ++b) t += part[(int64_t)b * S + s];
+    inv[s] = t > 0.0 ? 1.0 / sqrt(t) : 0.0;
+}
+
+// psi_b = sum_s exp(o_2s - shift_s) sign(o_2s+1) * scale_s; with one amplitude
+// the scale is 1 (psi = exp(o0 - max o0) sign(o1)), with several each
+// amplitude is normalised to unit length before the sum
+__global__ void k_amplitude(int64_t n, int S, const float* __restrict__ out, const float* __restrict__ shift,
+                            const double* __restrict__ inv, double* __restrict__ psi)
 {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n) return;
-    const float o0 = out[b * n_out], o1 = out[b * n_out + 1];
-    const double a = exp((double)(o0 - shift[0]));
-    psi[b] = o1 >= 0.f ? a : -a;
+    double v = 0.0;
+    for (int s = 0; s < S; ++s) {
+        const float o0 = out[b * 2 * S + 2 * s], o1 = out[b * 2 * S + 2 * s + 1];
+        const double a = exp((double)(o0 - shift[s])) * (S > 1 ? inv[s] : 1.0);
+        v += o1 >= 0.f ? a : -a;
+    }
+    psi[b] = v;
 }
 
 }  // namespace hd
@@ -340,6 +385,7 @@ using namespace hd;
 
 struct hd_mlp {
     int n_bits = 0, n_params = 0;
+    int n_states = 1;  // amplitudes the network predicts (2 outputs each)
     hipStream_t stream = nullptr;
     // layer l: out_l x in_l ; l = 0..3 with padded shapes
     int rows[4] = {kW, kW, kW, kOutPad};
@@ -485,7 +531,7 @@ int mlp_backward(hd_mlp* m, int64_t n, int64_t npad, const float* d_tgt, float l
     const hipStream_t st = m->stream;
     const int M = (int)npad;
     const int nblk = (int)((npad + 255) / 256);
-    hipLaunchKernelGGL(k_loss, dim3(nblk), dim3(256), 0, st, n, npad, m->out, d_tgt, lambda, 1.f / (float)n,
+    hipLaunchKernelGGL(k_loss, dim3(nblk), dim3(256), 0, st, n, npad, m->n_states, m->out, d_tgt, lambda, 1.f / (float)n,
                        m->d[4], m->dT[4], m->partial);
     // dW4 = d4^T h3 ; db4
     {
@@ -532,13 +578,14 @@ int stage_batch(hd_mlp* m, const uint64_t* states, const float* params, const fl
         m->st_buf = nullptr; m->pr_buf = nullptr; m->tg_buf = nullptr;
         HD_CHECK(dalloc_t(&m->st_buf, (size_t)n, "MLP staged states"));
         HD_CHECK(dalloc_t(&m->pr_buf, (size_t)n * std::max(m->n_params, 1), "MLP staged params"));
-        HD_CHECK(dalloc_t(&m->tg_buf, (size_t)n * 3, "MLP staged targets"));
+        HD_CHECK(dalloc_t(&m->tg_buf, (size_t)n * 3 * m->n_states, "MLP staged targets"));
         m->st_cap = n;
     }
     HD_HIP(hipMemcpyAsync(m->st_buf, states, sizeof(uint64_t) * n, hipMemcpyHostToDevice, m->stream));
     HD_HIP(hipMemcpyAsync(m->pr_buf, params, sizeof(float) * (per_sample ? n : 1) * m->n_params,
                           hipMemcpyHostToDevice, m->stream));
-    if (tgt) HD_HIP(hipMemcpyAsync(m->tg_buf, tgt, sizeof(float) * n * 3, hipMemcpyHostToDevice, m->stream));
+    if (tgt)
+        HD_HIP(hipMemcpyAsync(m->tg_buf, tgt, sizeof(float) * n * 3 * m->n_states, hipMemcpyHostToDevice, m->stream));
     *ds = m->st_buf;
     *dp = m->pr_buf;
     *dt = m->tg_buf;
@@ -568,12 +615,20 @@ extern "C" {
 
 int hd_mlp_create(int32_t n_bits, int32_t n_params, uint64_t seed, hd_mlp** out)
 {
+    return hd_mlp_create_states(n_bits, n_params, 1, seed, out);
+}
+
+int hd_mlp_create_states(int32_t n_bits, int32_t n_params, int32_t n_states, uint64_t seed, hd_mlp** out)
+{
     if (!out || n_bits <= 0 || n_params < 0 || n_bits + n_params > kIn)
         return fail(HD_ERR_ARG, "hd_mlp_create: need 0 < n_bits and n_bits + n_params <= 64");
+    if (n_states < 1 || n_states > 16) return fail(HD_ERR_ARG, "hd_mlp_create_states: 1 to 16 amplitudes");
     *out = nullptr;
     hd_mlp* m = new hd_mlp();
     m->n_bits = n_bits;
     m->n_params = n_params;
+    m->n_states = n_states;
+    m->live_rows[3] = 2 * n_states;
     if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
         delete m;
         return fail(HD_ERR_HIP, "stream creation failed");
@@ -688,14 +743,15 @@ int hd_mlp_forward(hd_mlp* m, const uint64_t* h_states, const float* h_params, i
     HD_CHECK(stage_batch(m, h_states, h_params, nullptr, n, true, &ds, &dp, &dt, 1));
     const int64_t nblk = (n + kIT - 1) / kIT;
     float *dout = nullptr, *dmax = nullptr;
-    HD_CHECK(dalloc_t(&dout, (size_t)n * 2, "MLP outputs"));
-    int st = dalloc_t(&dmax, (size_t)nblk, "MLP output maxima");
+    const int S = m->n_states;
+    HD_CHECK(dalloc_t(&dout, (size_t)n * 2 * S, "MLP outputs"));
+    int st = dalloc_t(&dmax, (size_t)nblk * S, "MLP output maxima");
     if (st == HD_OK) {
         InferArgs a = infer_args(m, n, ds, dp, 1, dout, dmax);
         hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)nblk), dim3(kIWaves * 64), 0, m->stream, a);
         HipErrors q;
         HD_ENQ(q, hipGetLastError());
-        HD_ENQ(q, hipMemcpyAsync(h_out, dout, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, m->stream));
+        HD_ENQ(q, hipMemcpyAsync(h_out, dout, sizeof(float) * 2 * S * n, hipMemcpyDeviceToHost, m->stream));
         HD_ENQ(q, hipStreamSynchronize(m->stream));
         st = q.status();
     }
@@ -738,24 +794,35 @@ int hd_mlp_start_vector(hd_mlp* m, const hd_basis* basis, const float* h_params,
     if (basis->Norb != m->n_bits) return fail(HD_ERR_ARG, "network input width differs from the orbital count");
     const int64_t n = basis->dim;
     if (n == 0) return HD_OK;
+    const int S = m->n_states;
     const int64_t nblk = (n + kIT - 1) / kIT;
+    const int nred = (int)std::min<int64_t>((n + 255) / 256, 1024);
     float *dpar = nullptr, *dout = nullptr, *dmax = nullptr;
-    int st = dalloc_t(&dpar, (size_t)std::max(m->n_params, 1) + 1, "MLP parameters and shift");
-    if (st == HD_OK) st = dalloc_t(&dout, (size_t)n * 2, "start vector outputs");
-    if (st == HD_OK) st = dalloc_t(&dmax, (size_t)nblk, "start vector output maxima");
+    double* dnorm = nullptr;
+    const int np = std::max(m->n_params, 1);
+    int st = dalloc_t(&dpar, (size_t)np + S, "MLP parameters and shifts");
+    if (st == HD_OK) st = dalloc_t(&dout, (size_t)n * 2 * S, "start vector outputs");
+    if (st == HD_OK) st = dalloc_t(&dmax, (size_t)nblk * S, "start vector output maxima");
+    if (st == HD_OK) st = dalloc_t(&dnorm, (size_t)(nred + 1) * S, "start vector norms");
     if (st == HD_OK) {
-        // one pass of the fused network over the sector, then
-        // psi = exp(o0 - max o0) sign(o1)
+        // one pass of the fused network over the sector, then per amplitude the
+        // largest log-amplitude (and, with several, the norm), then the sum
         HipErrors q;
         HD_ENQ(q, hipMemcpyAsync(dpar, h_params, sizeof(float) * m->n_params, hipMemcpyHostToDevice, m->stream));
         InferArgs a = infer_args(m, n, basis->d_states, dpar, 0, dout, dmax);
         hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)nblk), dim3(kIWaves * 64), 0, m->stream, a);
         HD_ENQ(q, hipGetLastError());
-        float* shift = dpar + std::max(m->n_params, 1);
-        hipLaunchKernelGGL(k_max_floats, dim3(1), dim3(256), 0, m->stream, nblk, dmax, shift);
+        float* shift = dpar + np;
+        hipLaunchKernelGGL(k_max_states, dim3(S), dim3(256), 0, m->stream, nblk, S, dmax, shift);
         HD_ENQ(q, hipGetLastError());
-        hipLaunchKernelGGL(k_amplitude, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, n, dout, 2, shift,
-                           d_vec);
+        double* inv = dnorm + (size_t)nred * S;
+        if (S > 1) {
+            hipLaunchKernelGGL(k_norm2_states, dim3(nred), dim3(256), 0, m->stream, n, S, dout, shift, dnorm);
+            hipLaunchKernelGGL(k_inv_norm_states, dim3(1), dim3(64), 0, m->stream, nred, S, dnorm, inv);
+            HD_ENQ(q, hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_amplitude, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, n, S, dout, shift,
+                           inv, d_vec);
         HD_ENQ(q, hipGetLastError());
         HD_ENQ(q, hipStreamSynchronize(m->stream));
         st = q.status();
@@ -763,7 +830,15 @@ int hd_mlp_start_vector(hd_mlp* m, const hd_basis* basis, const float* h_params,
     dfree(dpar);
     dfree(dout);
     dfree(dmax);
+    dfree(dnorm);
     return st;
+}
+
+int hd_mlp_num_states(const hd_mlp* m, int32_t* n_states)
+{
+    if (!m || !n_states) return fail(HD_ERR_ARG, "hd_mlp_num_states: null argument");
+    *n_states = m->n_states;
+    return HD_OK;
 }
 
 // Raw GEMM check: C (M x N, FP32) = A (M x K, BF16 bits) . B (N x K, BF16 bits)^T

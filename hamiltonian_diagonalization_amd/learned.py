@@ -1,10 +1,11 @@
 """Learned Lanczos start vector: the amplitude network, its training data
-from plain-solver ground states, training, and the start vector.
+from plain-solver eigenvectors, training, and the start vector.
 
 The network (HIP kernels in csrc/hd_mlp.hip) maps the occupation bitstring of
-a sector state plus the Hamiltonian parameters (V1, V3, V5) to log|psi| and a
-sign logit; evaluated over the whole sector it gives the start vector that
-replaces the reference's ARPACK initial-vector file
+a sector state plus the Hamiltonian parameters (V1, V3, V5) to log|psi_s| and a
+sign logit for each of its amplitudes (the lowest eigenvectors it is trained
+on); evaluated over the whole sector it gives the start vector that replaces
+the reference's ARPACK initial-vector file
 (src/utilities/wrappers/arpack_wrapper.hpp:753-774).
 """
 from __future__ import annotations
@@ -16,7 +17,7 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-from . import (DeviceBuffer, F64, FockBasis, Hamiltonian, Sector, _check, _ptr, lib)
+from . import (CODED, F64, TILED, DeviceBuffer, FockBasis, Hamiltonian, Sector, _check, _ptr, lib)
 from . import models
 
 LAYER_SHAPES = [(256, 64), (256, 256), (256, 256), (64, 256)]  # padded (rows, cols)
@@ -39,13 +40,14 @@ def flatten(layers) -> np.ndarray:
 
 
 class AmplitudeNet:
-    """Handle on the device network (hd_mlp_*)."""
+    """Handle on the device network (hd_mlp_*): n_states amplitudes (the
+    lowest eigenvectors it is trained on), two outputs each."""
 
-    def __init__(self, n_bits: int, n_params: int = 3, seed: int = 0):
+    def __init__(self, n_bits: int, n_params: int = 3, seed: int = 0, n_states: int = 1):
         h = ctypes.c_void_p()
-        _check(lib().hd_mlp_create(n_bits, n_params, seed, ctypes.byref(h)))
+        _check(lib().hd_mlp_create_states(n_bits, n_params, n_states, seed, ctypes.byref(h)))
         self.h = h
-        self.n_bits, self.n_params = n_bits, n_params
+        self.n_bits, self.n_params, self.n_states = n_bits, n_params, n_states
         c = ctypes.c_int64()
         _check(lib().hd_mlp_num_weights(h, ctypes.byref(c)))
         self.n_weights = c.value
@@ -63,14 +65,14 @@ class AmplitudeNet:
     def forward(self, states: np.ndarray, params: np.ndarray) -> np.ndarray:
         st = np.ascontiguousarray(states, dtype=np.uint64)
         pr = np.ascontiguousarray(params, dtype=np.float32).reshape(len(st), self.n_params)
-        out = np.zeros((len(st), 2), dtype=np.float32)
+        out = np.zeros((len(st), 2 * self.n_states), dtype=np.float32)
         _check(lib().hd_mlp_forward(self.h, _ptr(st), _ptr(pr), len(st), _ptr(out)))
         return out
 
     def gradients(self, states, params, targets, lam=1.0):
         st = np.ascontiguousarray(states, dtype=np.uint64)
         pr = np.ascontiguousarray(params, dtype=np.float32).reshape(len(st), self.n_params)
-        tg = np.ascontiguousarray(targets, dtype=np.float32).reshape(len(st), 3)
+        tg = np.ascontiguousarray(targets, dtype=np.float32).reshape(len(st), 3 * self.n_states)
         g = np.zeros(self.n_weights, dtype=np.float32)
         loss = ctypes.c_float()
         _check(lib().hd_mlp_gradients(self.h, _ptr(st), _ptr(pr), _ptr(tg), len(st), lam,
@@ -100,12 +102,14 @@ class AmplitudeNet:
         return v
 
     def save(self, path: str):
-        np.savez(path, weights=self.get_weights(), n_bits=self.n_bits, n_params=self.n_params)
+        np.savez(path, weights=self.get_weights(), n_bits=self.n_bits, n_params=self.n_params,
+                 n_states=self.n_states)
 
     @classmethod
     def load(cls, path: str) -> "AmplitudeNet":
         z = np.load(path)
-        net = cls(int(z["n_bits"]), int(z["n_params"]))
+        n_states = int(z["n_states"]) if "n_states" in z.files else 1
+        net = cls(int(z["n_bits"]), int(z["n_params"]), n_states=n_states)
         net.set_weights(z["weights"])
         return net
 
@@ -141,13 +145,19 @@ def sample_params(rng: np.random.Generator, n: int) -> np.ndarray:
 class Dataset:
     states: np.ndarray   # uint64
     params: np.ndarray   # float32 (n, n_params)
-    targets: np.ndarray  # float32 (n, 3): log|psi| - max, sign, weight
+    targets: np.ndarray  # float32 (n, 3 n_states): per amplitude log|psi| - max, sign, weight
     solve_ms: float
 
 
-def sphere_hamiltonian(basis: FockBasis, two_s: int, v135) -> Hamiltonian:
+def sphere_hamiltonian(basis: FockBasis, two_s: int, v135, tiled: bool = False) -> Hamiltonian:
+    """The sphere Hamiltonian at (V1, V3, V5): FP64 CRS, or with tiled=True the
+    coded values and the tiled layout (solve with precision TILED)."""
     vm = models.coulomb_list(two_s, {1: float(v135[0]), 3: float(v135[1]), 5: float(v135[2])})
-    return Hamiltonian(basis, models.sphere_tables(two_s, vm), values=(F64,))
+    if not tiled:
+        return Hamiltonian(basis, models.sphere_tables(two_s, vm), values=(F64,))
+    h = Hamiltonian(basis, models.sphere_tables(two_s, vm), values=(CODED,))
+    h.tile()
+    return h
 
 
 def ground_state_targets(psi: np.ndarray, floor: float = -30.0):
@@ -164,34 +174,40 @@ def ground_state_targets(psi: np.ndarray, floor: float = -30.0):
 
 
 def make_dataset(basis: FockBasis, two_s: int, points: np.ndarray, per_point: int, seed: int = 0,
-                 tol: float = 1e-8, log=print, hamiltonian=None) -> Dataset:
-    """Plain-solver ground states at each parameter point; per point, half the
-    samples drawn by |psi|^2 and half uniformly over the sector.  The model is
-    the sphere at flux two_s with parameters (V1, V3, V5), or
-    hamiltonian(basis, p) when given."""
+                 tol: float = 1e-8, log=print, hamiltonian=None, n_states: int = 1, tiled: bool = False) -> Dataset:
+    """Plain-solver eigenvectors (the n_states lowest) at each parameter point;
+    per point, half the samples drawn by the mean of their |psi_s|^2 and half
+    uniformly over the sector.  The model is the sphere at flux two_s with
+    parameters (V1, V3, V5) (on the tiled operator with tiled=True), or
+    hamiltonian(basis, p) when given.  Each point's solve starts from the sum
+    of the previous point's eigenvectors."""
     rng = np.random.default_rng(seed)
     states_all = basis.states()
     S, P, T = [], [], []
     t_solve = 0.0
     start = None
     for i, p in enumerate(points):
-        h = hamiltonian(basis, p) if hamiltonian else sphere_hamiltonian(basis, two_s, p)
+        h = hamiltonian(basis, p) if hamiltonian else sphere_hamiltonian(basis, two_s, p, tiled=tiled)
         t0 = time.perf_counter()
-        r = h.lanczos(nev=1, tol=tol, seed=i, start=start)
+        r = h.lanczos(nev=n_states, tol=tol, seed=i, start=start, precision=TILED if tiled and not hamiltonian else F64)
         t_solve += time.perf_counter() - t0
-        psi = h.eigenvector(0)
-        start = psi  # warm start the next point of the sweep
-        a, s = ground_state_targets(psi)
-        prob = psi ** 2
+        vecs = [h.eigenvector(k) for k in range(n_states)]
+        start = np.sum(vecs, axis=0)  # warm start for the next point of the sweep
+        cols, prob = [], np.zeros(basis.dim)
+        for psi in vecs:
+            a, s = ground_state_targets(psi)
+            cols.append((a, s))
+            prob += psi ** 2
         prob /= prob.sum()
         k = min(per_point // 2, basis.dim)
         idx = np.concatenate([rng.choice(basis.dim, size=k, p=prob),
                               rng.choice(basis.dim, size=per_point - k)])
         S.append(states_all[idx])
         P.append(np.repeat(np.asarray(p, dtype=np.float32)[None, :], len(idx), axis=0))
-        T.append(np.stack([a[idx], s[idx], np.ones(len(idx), dtype=np.float32)], axis=1))
+        T.append(np.concatenate([np.stack([a[idx], s[idx], np.ones(len(idx), dtype=np.float32)], axis=1)
+                                 for a, s in cols], axis=1))
         if log:
-            log(f"  point {i}: p=({','.join('%.3f' % x for x in p)}) E0={r.evals[0]:.8f} "
+            log(f"  point {i}: p=({','.join('%.3f' % x for x in p)}) E={np.round(r.evals[:n_states], 8).tolist()} "
                 f"matvecs={r.matvecs}")
         del h
     return Dataset(np.concatenate(S), np.concatenate(P).astype(np.float32),
@@ -204,7 +220,8 @@ def train(net: AmplitudeNet, data: Dataset, steps: int = 2000, batch: int = 4096
     into fixed device buffers."""
     rng = np.random.default_rng(seed)
     n = len(data.states)
-    b_st, b_pr, b_tg = DeviceBuffer(batch * 8), DeviceBuffer(batch * 4 * net.n_params), DeviceBuffer(batch * 12)
+    b_st, b_pr = DeviceBuffer(batch * 8), DeviceBuffer(batch * 4 * net.n_params)
+    b_tg = DeviceBuffer(batch * 12 * net.n_states)
     losses = []
     for it in range(steps):
         idx = rng.integers(0, n, size=batch)
@@ -222,3 +239,14 @@ def overlap(net: AmplitudeNet, basis: FockBasis, params, psi: np.ndarray) -> flo
     """|<psi_net|psi>| / (|psi_net| |psi|)."""
     v = net.start_vector(basis, params).to_array(np.float64, basis.dim)
     return float(abs(v @ psi) / (np.linalg.norm(v) * np.linalg.norm(psi)))
+
+
+def amplitude_overlaps(net: AmplitudeNet, basis: FockBasis, params, vecs) -> List[float]:
+    """Overlap of each predicted amplitude s with vecs[s] (unit vectors)."""
+    st = basis.states()
+    out = net.forward(st, np.repeat(np.asarray(params, dtype=np.float32)[None, :], len(st), axis=0))
+    res = []
+    for k, psi in enumerate(vecs):
+        a = np.exp((out[:, 2 * k] - out[:, 2 * k].max()).astype(np.float64)) * np.where(out[:, 2 * k + 1] >= 0, 1.0, -1.0)
+        res.append(float(abs(a @ psi) / (np.linalg.norm(a) * np.linalg.norm(psi))))
+    return res

@@ -21,7 +21,7 @@ import time
 
 import numpy as np
 
-from . import CODED, F32, F64, DeviceBuffer, FockBasis, Hamiltonian, Sector
+from . import CODED, F32, F64, TILED, DeviceBuffer, FockBasis, Hamiltonian, Sector, lib
 from . import learned, models
 
 
@@ -41,22 +41,31 @@ def nn_order(points: np.ndarray) -> np.ndarray:
 
 
 def hamiltonian(basis, two_s, v):
+    """Coded values and the tiled layout (the solver's fastest exact operator)."""
     vm = models.coulomb_list(two_s, {1: float(v[0]), 3: float(v[1]), 5: float(v[2])})
-    return Hamiltonian(basis, models.sphere_tables(two_s, vm), values=(F64, F32))
+    h = Hamiltonian(basis, models.sphere_tables(two_s, vm), values=(CODED,))
+    h.tile()
+    return h
 
 
 def evaluate(net, basis, two_s, v, nev, tol, mix=0.3):
     h = hamiltonian(basis, two_s, v)
     out = {"params": [float(x) for x in v]}
-    plain = h.lanczos(nev=nev, tol=tol, seed=0)
-    psi0 = h.eigenvector(0)
+    plain = h.lanczos(nev=nev, tol=tol, seed=0, precision=TILED, ncv=48)
+    vecs = [h.eigenvector(k) for k in range(nev)]
+    psi0 = vecs[0]
     out["plain"] = {"matvecs": plain.matvecs, "ms": plain.time_ms, "evals": plain.evals.tolist(),
                     "max_resid": float(plain.residuals.max())}
     t0 = time.perf_counter()
     start = net.start_vector(basis, v)
+    lib().hd_synchronize()
     t_net = (time.perf_counter() - t0) * 1e3
     vec = start.to_array(np.float64, basis.dim)
     out["overlap"] = float(abs(vec @ psi0) / (np.linalg.norm(vec) * np.linalg.norm(psi0)))
+    # overlap of the start vector with each wanted eigenvector, and of each
+    # predicted amplitude with its own eigenvector
+    out["overlaps_wanted"] = [float(abs(vec @ x) / np.linalg.norm(vec)) for x in vecs]
+    out["amplitude_overlaps"] = learned.amplitude_overlaps(net, basis, v, vecs[:net.n_states])
     out["net_eval_ms"] = t_net
     # diagnostics: |psi0|^2-weighted sign agreement (up to the global sign) and
     # the overlap of the magnitudes alone
@@ -64,18 +73,23 @@ def evaluate(net, basis, two_s, v, nev, tol, mix=0.3):
     agree = float(np.sum(w[np.sign(vec) == np.sign(psi0)]))
     out["sign_agreement"] = max(agree, 1.0 - agree)
     out["magnitude_overlap"] = float(np.abs(vec) @ np.abs(psi0) / (np.linalg.norm(vec) * np.linalg.norm(psi0)))
-    runs = {"learned": (vec, F64), "learned_mix": (None, F64), "learned_f32": (vec, F32)}
+    runs = {"learned": (vec, TILED), "learned_mix": (None, TILED), "learned_f32": (vec, F32), "mixed": ("rnd", F32)}
     rnd = np.random.default_rng(0).normal(size=basis.dim)
     for name, (s, prec) in runs.items():
         if s is None:
             s = vec / np.linalg.norm(vec) + mix * rnd / np.linalg.norm(rnd)
-        r = h.lanczos(nev=nev, tol=tol, precision=prec, start=s)
-        out[name] = {"matvecs": r.matvecs, "ms": r.time_ms + t_net, "evals": r.evals.tolist(),
+        if isinstance(s, str):
+            r = h.lanczos(nev=nev, tol=tol, precision=prec, seed=0, ncv=48)
+            extra = 0.0
+        else:
+            r = h.lanczos(nev=nev, tol=tol, precision=prec, start=s, ncv=48)
+            extra = t_net
+        out[name] = {"matvecs": r.matvecs, "ms": r.time_ms + extra, "evals": r.evals.tolist(),
                      "max_resid": float(r.residuals.max()),
                      "max_abs_dE_vs_plain": float(np.max(np.abs(r.evals - plain.evals)))}
     # the ground state alone, where a start close to psi0 can pay off
-    p1 = h.lanczos(nev=1, tol=tol, seed=0)
-    l1 = h.lanczos(nev=1, tol=tol, start=vec)
+    p1 = h.lanczos(nev=1, tol=tol, seed=0, precision=TILED)
+    l1 = h.lanczos(nev=1, tol=tol, start=vec, precision=TILED)
     out["ground_only"] = {"plain_matvecs": p1.matvecs, "plain_ms": p1.time_ms, "learned_matvecs": l1.matvecs,
                           "learned_ms": l1.time_ms + t_net, "abs_dE": float(abs(l1.evals[0] - p1.evals[0]))}
     return out
@@ -91,6 +105,8 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--eval-points", type=int, default=4)
+    ap.add_argument("--states", type=int, default=1,
+                    help="amplitudes the network predicts (trained on that many lowest eigenvectors)")
     ap.add_argument("--eval-only", action="store_true")
     ap.add_argument("--nev", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-10)
@@ -108,10 +124,11 @@ def main():
         pts = learned.sample_params(np.random.default_rng(0), args.train_points)
         pts = pts[nn_order(pts)]
         t0 = time.perf_counter()
-        data = learned.make_dataset(basis, args.two_s, pts, args.per_point, seed=0, tol=1e-8, log=log)
+        data = learned.make_dataset(basis, args.two_s, pts, args.per_point, seed=0, tol=1e-8, log=log,
+                                    n_states=args.states, tiled=True)
         t_data = time.perf_counter() - t0
         log(f"dataset: {len(data.states)} samples in {t_data:.1f} s")
-        net = learned.AmplitudeNet(args.two_s + 1, 3, seed=0)
+        net = learned.AmplitudeNet(args.two_s + 1, 3, seed=0, n_states=args.states)
         t0 = time.perf_counter()
         losses = learned.train(net, data, steps=args.steps, batch=args.batch, lr=args.lr, log=log,
                                log_every=max(200, args.steps // 20))
@@ -133,6 +150,9 @@ def main():
             f"learned {r['learned']['matvecs']} mix {r['learned_mix']['matvecs']} "
             f"f32 {r['learned_f32']['matvecs']}; ms plain {r['plain']['ms']:.0f} learned "
             f"{r['learned']['ms']:.0f} mix {r['learned_mix']['ms']:.0f} f32 {r['learned_f32']['ms']:.0f}")
+        log(f"held-out {i}: start-vector overlap with the {args.nev} wanted eigenvectors "
+            f"{np.round(r['overlaps_wanted'], 3).tolist()}, amplitude overlaps {np.round(r['amplitude_overlaps'], 3).tolist()}; "
+            f"mixed (random start) {r['mixed']['matvecs']} matvecs {r['mixed']['ms']:.0f} ms")
         report["eval"].append(r)
     ev = report["eval"]
     report["summary"] = {
@@ -146,7 +166,14 @@ def main():
         "ground_matvecs_learned_mean": float(np.mean([r["ground_only"]["learned_matvecs"] for r in ev])),
         "ground_ms_plain_mean": float(np.mean([r["ground_only"]["plain_ms"] for r in ev])),
         "ground_ms_learned_mean": float(np.mean([r["ground_only"]["learned_ms"] for r in ev])),
-        "max_abs_dE": float(max(max(r["learned"]["max_abs_dE_vs_plain"], r["ground_only"]["abs_dE"]) for r in ev)),
+        "nev_matvecs_learned_f32_mean": float(np.mean([r["learned_f32"]["matvecs"] for r in ev])),
+        "nev_ms_learned_f32_mean": float(np.mean([r["learned_f32"]["ms"] for r in ev])),
+        "nev_ms_mixed_mean": float(np.mean([r["mixed"]["ms"] for r in ev])),
+        "mean_overlaps_wanted": np.mean([r["overlaps_wanted"] for r in ev], axis=0).tolist(),
+        "mean_amplitude_overlaps": np.mean([r["amplitude_overlaps"] for r in ev], axis=0).tolist(),
+        "fewer_matvecs_points": int(sum(r["learned"]["matvecs"] < r["plain"]["matvecs"] for r in ev)),
+        "max_abs_dE": float(max(max(r["learned"]["max_abs_dE_vs_plain"], r["learned_f32"]["max_abs_dE_vs_plain"],
+                                    r["ground_only"]["abs_dE"]) for r in ev)),
     }
     log("summary", report["summary"])
     stem = os.path.basename(args.out)

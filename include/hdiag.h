@@ -505,22 +505,31 @@ int hd_batch_solve(hd_batch* b, const hd_batch_opts* opts, double* h_evals, doub
  * 256x256, 64x256) followed by its bias (rows). */
 typedef struct hd_mlp hd_mlp;
 int hd_mlp_create(int32_t n_bits, int32_t n_params, uint64_t seed, hd_mlp** out);
+/* The same network predicting n_states (1..16) amplitudes: 2 n_states outputs
+ * (log|psi_s|, sign logit_s), trained on the n_states lowest eigenvectors;
+ * hd_mlp_create is n_states = 1. */
+int hd_mlp_create_states(int32_t n_bits, int32_t n_params, int32_t n_states, uint64_t seed, hd_mlp** out);
+int hd_mlp_num_states(const hd_mlp* m, int32_t* n_states);
 int hd_mlp_destroy(hd_mlp* m);
 int hd_mlp_num_weights(const hd_mlp* m, int64_t* count);
 int hd_mlp_get_weights(hd_mlp* m, float* h_weights);
 int hd_mlp_set_weights(hd_mlp* m, const float* h_weights);
-/* Network outputs (n x 2: log|psi|, sign logit) for host states and per-sample
- * parameters (n x n_params). */
+/* Network outputs (n x 2 n_states: log|psi_s|, sign logit_s) for host states
+ * and per-sample parameters (n x n_params). */
 int hd_mlp_forward(hd_mlp* m, const uint64_t* h_states, const float* h_params, int64_t n, float* h_out);
 /* Loss and gradients (flat layout above) without an update, host inputs;
- * targets are n x 3 (log|psi|, sign +-1, weight); lambda weights the sign term. */
+ * targets are n x 3 n_states (per amplitude: log|psi|, sign +-1, weight);
+ * lambda weights the sign term. */
 int hd_mlp_gradients(hd_mlp* m, const uint64_t* h_states, const float* h_params, const float* h_targets,
                      int64_t n, float lambda, float* loss, float* h_grads);
 /* One Adam step on a device batch. */
 int hd_mlp_train_step(hd_mlp* m, const uint64_t* d_states, const float* d_params, const float* d_targets,
                       int64_t n, float lambda, float lr, float* loss);
 /* The network's amplitudes over a whole sector basis as an FP64 Lanczos start
- * vector (d_vec: dim entries; pass it as hd_lanczos_opts.d_start). */
+ * vector (d_vec: dim entries; pass it as hd_lanczos_opts.d_start):
+ * exp(o0 - max o0) sign(o1) for one amplitude; with several, the sum of the
+ * amplitudes each normalised to unit length, so the vector leans on all of
+ * the wanted eigenvectors. */
 int hd_mlp_start_vector(hd_mlp* m, const hd_basis* b, const float* h_params, double* d_vec);
 /* The MFMA GEMM on host data: C (M x N, FP32) = A (M x K) . B (N x K)^T with
  * BF16 bit patterns; M, N multiples of 64, K a multiple of 16. */

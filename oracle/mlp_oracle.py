@@ -75,19 +75,23 @@ def forward(flat, states, params, n_bits, n_params, bf=True):
 
 
 def loss_and_grads(flat, states, params, targets, n_bits, n_params, lam=1.0, bf=True):
-    """Loss = mean_b w_b [ (o0 - a_b)^2 + lam * softplus(-s_b o1) ] and its
+    """Loss = mean_b sum_s w_bs [ (o_2s - a_bs)^2 + lam * softplus(-s_bs o_2s+1) ]
+    over the S amplitudes of targets (n x 3S: a, s, w per amplitude) and its
     gradient in the flat layout."""
     layers = unflatten(np.asarray(flat, dtype=np.float64))
     r = (lambda a: bf16(a).astype(np.float64)) if bf else (lambda a: a)
     out, hs, zs = forward(flat, states, params, n_bits, n_params, bf)
     n = len(states)
-    a, s, w = targets[:, 0], targets[:, 1], targets[:, 2]
-    o0, o1 = out[:, 0], out[:, 1]
-    m = -s * o1
-    loss = np.sum(w * ((o0 - a) ** 2 + lam * np.logaddexp(0.0, m))) / n
+    tg = np.asarray(targets, dtype=np.float64).reshape(n, -1, 3)
+    loss = 0.0
     d = np.zeros_like(out)
-    d[:, 0] = 2 * w * (o0 - a) / n
-    d[:, 1] = -lam * w * s / (1.0 + np.exp(-m)) / n
+    for st in range(tg.shape[1]):
+        a, s, w = tg[:, st, 0], tg[:, st, 1], tg[:, st, 2]
+        o0, o1 = out[:, 2 * st], out[:, 2 * st + 1]
+        m = -s * o1
+        loss += np.sum(w * ((o0 - a) ** 2 + lam * np.logaddexp(0.0, m))) / n
+        d[:, 2 * st] = 2 * w * (o0 - a) / n
+        d[:, 2 * st + 1] = -lam * w * s / (1.0 + np.exp(-m)) / n
     d = r(d)
     grads = [None] * 4
     grads[3] = (d.T @ hs[3], d.sum(axis=0))

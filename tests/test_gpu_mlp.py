@@ -40,16 +40,21 @@ def _data(rng, n, n_bits):
     return states, params, targets
 
 
-def test_forward_and_gradients_match_oracle():
-    say("mlp forward/backward vs oracle")
+@pytest.mark.parametrize("n_states", [1, 3])
+def test_forward_and_gradients_match_oracle(n_states):
+    say("mlp forward/backward vs oracle, amplitudes", n_states)
     rng = np.random.default_rng(1)
     n_bits = 34
-    net = learned.AmplitudeNet(n_bits, 3, seed=5)
+    net = learned.AmplitudeNet(n_bits, 3, seed=5, n_states=n_states)
     w = net.get_weights()
     states, params, targets = _data(rng, 300, n_bits)  # 300 pads to 320 rows
+    targets = np.concatenate([targets] + [_data(rng, 300, n_bits)[2] for _ in range(n_states - 1)], axis=1)
     out = net.forward(states, params)
+    assert out.shape == (300, 2 * n_states)
     want, _, _ = mo.forward(w, states, params, n_bits, 3, bf=True)
-    assert np.max(np.abs(out - want[:, :2])) <= 2e-2 * max(1.0, np.max(np.abs(want[:, :2])))
+    live = want[:, :2 * n_states]
+    assert np.max(np.abs(out - live)) <= 2e-2 * max(1.0, np.max(np.abs(live)))
+    assert np.all(want[:, 2 * n_states:] == 0)
     loss, g = net.gradients(states, params, targets, lam=0.5)
     lref, gref = mo.loss_and_grads(w, states, params, targets, n_bits, 3, lam=0.5, bf=True)
     assert abs(loss - lref) <= 1e-3 * abs(lref)
@@ -122,3 +127,38 @@ def test_training_and_learned_start():
     assert abs(out[1]["eigenvalues"][0] - ref.evals[0]) < 1e-9
     assert out[1]["matvecs"] < out[0]["matvecs"]
 
+
+
+def test_multi_amplitude_start_vector():
+    """A network with 3 amplitudes trained on the 3 lowest eigenvectors of a
+    small sphere: each amplitude overlaps its eigenvector well, the start
+    vector (the sum of the normalised amplitudes) overlaps all three, and
+    Lanczos from it reaches the random start's eigenvalues."""
+    say("mlp with 3 amplitudes on N=6 2S=15")
+    n, two_s = 6, 15
+    basis = hd.FockBasis(hd.Sector.lz(0, two_s), n, two_s + 1)
+    pts = learned.sample_params(np.random.default_rng(3), 8)
+    data = learned.make_dataset(basis, two_s, pts, per_point=2048, seed=0, log=None, n_states=3)
+    assert data.targets.shape[1] == 9
+    net = learned.AmplitudeNet(two_s + 1, 3, seed=0, n_states=3)
+    losses = learned.train(net, data, steps=800, batch=1024, lr=2e-3, log=None)
+    say("loss", losses[0], "->", losses[-1])
+    held = learned.sample_params(np.random.default_rng(99), 1)[0]
+    h = learned.sphere_hamiltonian(basis, two_s, held)
+    ref = h.lanczos(nev=3, tol=1e-10, seed=0)
+    vecs = [h.eigenvector(k) for k in range(3)]
+    amp = learned.amplitude_overlaps(net, basis, held, vecs)
+    v = net.start_vector(basis, held).to_array(np.float64, basis.dim)
+    v /= np.linalg.norm(v)
+    over = [abs(v @ x) for x in vecs]
+    say("amplitude overlaps", amp, "start vector overlaps", over)
+    assert min(amp) > 0.5 and min(over) > 0.2
+    # the forward outputs assemble the same start vector
+    out = net.forward(basis.states(), np.repeat(np.asarray(held, dtype=np.float32)[None, :], basis.dim, axis=0))
+    want = np.zeros(basis.dim)
+    for s in range(3):
+        a = np.exp((out[:, 2 * s] - out[:, 2 * s].max()).astype(np.float64)) * np.where(out[:, 2 * s + 1] >= 0, 1.0, -1.0)
+        want += a / np.linalg.norm(a)
+    assert np.max(np.abs(v - want / np.linalg.norm(want))) < 1e-12
+    r = h.lanczos(nev=3, tol=1e-10, start=net.start_vector(basis, held))
+    assert r.converged and np.max(np.abs(r.evals - ref.evals)) < 1e-9

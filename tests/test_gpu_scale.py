@@ -26,7 +26,7 @@ from hamiltonian_diagonalization_amd import models
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 CHECKPOINT = os.path.join(os.path.dirname(HERE), "hamiltonian_diagonalization_amd", "checkpoints",
-                          "start_n12_2s33.npz")
+                          "start5_n12_2s33.npz")
 
 
 def say(*a):
@@ -241,17 +241,21 @@ def test_batch_config4_full_sweep(oracle):
 
 @pytest.mark.timeout(600)
 def test_learned_start_config5_heldout():
-    """The checkpoint trained at N=12, 2S=33 on 128 points of
-    [0.5,1.5] x [0,0.5] x [0,0.2]: on 4 held-out points (seed 1) the solves
-    from the network's start vector, with the exact FP64 operator and in the
-    mixed form, reach residual 1e-10 for the 5 lowest eigenpairs with the
-    eigenvalues of the random start to 1e-10.  At the third point two
-    eigenvalues lie 0.0022 apart and the lower one enters the Krylov space
-    late from this start: a speculative FP32 cycle is discarded there and the
-    solve still needs no rerun."""
+    """The 5-amplitude checkpoint trained at N=12, 2S=33 on the 5 lowest
+    eigenvectors at 128 points of [0.5,1.5] x [0,0.5] x [0,0.2]: on 4
+    held-out points (seed 1) the solves from the network's start vector, with
+    the exact FP64 operator and in the mixed form, reach residual 1e-10 for
+    the 5 lowest eigenpairs with the eigenvalues of the random start to
+    1e-10, and the FP64 solve from the learned start needs no more operator
+    applications than the random start at any point and fewer in total.  At
+    the third point two eigenvalues lie 0.0022 apart; should one enter the
+    Krylov space late, a speculative FP32 cycle is discarded and the solve
+    still needs no rerun."""
     from hamiltonian_diagonalization_amd import learned
     net = learned.AmplitudeNet.load(CHECKPOINT)
+    assert net.n_states == 5
     basis = hd.FockBasis(hd.Sector.lz(0, 33), 12, 34)
+    total_plain = total_learned = 0
     for v in learned.sample_params(np.random.default_rng(1), 4):
         say("learned start N=12 2S=33 at", v)
         vm = models.coulomb_list(33, {1: float(v[0]), 3: float(v[1]), 5: float(v[2])})
@@ -270,4 +274,9 @@ def test_learned_start_config5_heldout():
             assert np.max(np.abs(r.evals - plain.evals) / np.maximum(1, np.abs(plain.evals))) < 1e-10
             # no rerun of the whole solve: at most the plain count plus discarded cycles
             assert r.matvecs <= plain.matvecs + 22 * (r.f32_cycles_discarded + 2), name
+        assert runs["learned_fp64"].matvecs <= plain.matvecs
+        total_plain += plain.matvecs
+        total_learned += runs["learned_fp64"].matvecs
         del h
+    say("operator applications over the 4 points: random start", total_plain, "learned start", total_learned)
+    assert total_learned < total_plain

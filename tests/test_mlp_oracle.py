@@ -1,15 +1,16 @@
 """The NumPy network oracle against central finite differences (float64,
 no rounding): pins the backward pass the device gradients are checked with."""
 import numpy as np
+import pytest
 
 from oracle import mlp_oracle as mo
 
 
-def _random_net(rng, n_bits, n_params):
+def _random_net(rng, n_bits, n_params, n_states=1):
     layers = []
     for l, (r, c) in enumerate(mo.SHAPES):
         W = np.zeros((r, c))
-        live_r = 2 if l == 3 else r
+        live_r = 2 * n_states if l == 3 else r
         live_c = n_bits + n_params if l == 0 else c
         W[:live_r, :live_c] = rng.normal(scale=np.sqrt(2.0 / live_c), size=(live_r, live_c))
         b = np.zeros(r)
@@ -27,19 +28,22 @@ def test_bf16_rounding():
     assert np.all(np.abs(r - x) <= np.abs(x) * 2.0 ** -8)
 
 
-def test_gradients_match_finite_differences():
-    rng = np.random.default_rng(0)
+@pytest.mark.parametrize("n_states", [1, 3])
+def test_gradients_match_finite_differences(n_states):
+    """One or several amplitudes (2 outputs each; targets n x 3 n_states)."""
+    rng = np.random.default_rng(n_states - 1)
     n_bits, n_params = 10, 3
-    flat = _random_net(rng, n_bits, n_params)
+    flat = _random_net(rng, n_bits, n_params, n_states)
     n = 24
     states = np.array([int(rng.choice(1 << n_bits)) for _ in range(n)], dtype=np.uint64)
     params = rng.random((n, n_params))
-    targets = np.stack([rng.normal(size=n), rng.choice([-1.0, 1.0], size=n), rng.random(n) + 0.5], 1)
+    targets = np.concatenate([np.stack([rng.normal(size=n), rng.choice([-1.0, 1.0], size=n), rng.random(n) + 0.5], 1)
+                              for _ in range(n_states)], axis=1)
     loss, g = mo.loss_and_grads(flat, states, params, targets, n_bits, n_params, lam=0.7, bf=False)
     layers = mo.unflatten(np.arange(flat.size))
     picks = []
     for l, (W, b) in enumerate(layers):
-        live_r = 2 if l == 3 else W.shape[0]
+        live_r = 2 * n_states if l == 3 else W.shape[0]
         live_c = n_bits + n_params if l == 0 else W.shape[1]
         for _ in range(6):
             picks.append(int(W[rng.integers(live_r), rng.integers(live_c)]))
