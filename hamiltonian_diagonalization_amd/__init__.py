@@ -6,6 +6,7 @@ Python plumbing over the C ABI of ``lib/libhdiag_hip.so`` (declared in
 
   FockBasis        FermionFockBasis         src/basis/fermion_fock_basis.hpp:45-74
   TermTables       Quartic/QuadraticTermTables  src/hamiltonians/term_tables_base.hpp:44-174
+  TermTables.hash  Quartic/QuadraticTermHashTables  src/hamiltonians/term_hash_tables_base.hpp:44-276
   Hamiltonian      SpinlessFermionHamiltonian<T> + HamiltonianBase<T,B>
                    src/hamiltonians/spinless_fermion_hamiltonian.hpp:52-436,
                    src/hamiltonians/hamiltonian_base.hpp:70-931
@@ -31,7 +32,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libhdiag_hip.so")
 HD_OK, HD_ERR_ARG, HD_ERR_HIP, HD_ERR_NOMEM, HD_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 HD_ERR_NOT_CONVERGED, HD_ERR_EMPTY = 5, 6
 SECTOR_NONE, SECTOR_LZ, SECTOR_K2D, SECTOR_K1D, SECTOR_LZ2 = 0, 1, 2, 3, 4
-LAYOUT_SPHERE, LAYOUT_OFL = 0, 1
+LAYOUT_SPHERE, LAYOUT_OFL, LAYOUT_HASH = 0, 1, 2
 F64, F32, CODED, TILED, TILED_X32 = 0, 1, 2, 3, 4
 STORAGE_FULL, STORAGE_UPPER_REF, STORAGE_MATRIX_FREE = 0, 1, 2
 START_RANDOM, START_USER = 0, 1
@@ -122,6 +123,9 @@ SIGNATURES = {
     "hd_sphere_vtable": (ctypes.c_int, [_I32, _P, _I32, _P]),
     "hd_tables_create": (ctypes.c_int, [_I32, _U32, _P, _P, _I32, _P, _PP]),
     "hd_tables_create_two_level": (ctypes.c_int, [_U32, _P, _P, _P, _P, _P, _P, _PP]),
+    "hd_tables_create_hash": (ctypes.c_int, [_U32, _I32, _I32, _P, _P, _P, _I32, _P, _P, _P, _PP]),
+    "hd_ofl_wannier": (ctypes.c_int, [_I32, _I32, _D, _P, _P, _P, _P, _D, _P, _P, _P, _P, _P, _P,
+                                      ctypes.POINTER(_D)]),
     "hd_ofl_ktable": (ctypes.c_int, [_I32, _I32, _P, _P, _P]),
     "hd_ofl_vtable": (ctypes.c_int, [_I32, _I32, _I32, _P, _P, _P]),
     "hd_tables_destroy": (ctypes.c_int, [_P]),
@@ -417,10 +421,113 @@ class TermTables:
         self.ktable, self.vtable, self.eps = (k1, k2), (v1, v2), (e1, e2)
         return self
 
+    @classmethod
+    def hash(cls, kmax: int, nk1, k1, v, nq1=None, q1=None, e=None):
+        """General term tables as the lists TermHashTables' GetK1 returns
+        (term_hash_tables.cpp:51-130): for entry (k4*kmax + k3)*kmax + k2,
+        nk1[entry] labels k1[entry, :nk1] with values v[entry, :nk1]
+        (complex or real, 0 where the hash table holds no value); for each k2,
+        nq1[k2] quadratic labels q1[k2, :nq1] with values e (None: no
+        quadratic terms).  k1 / v have shape (kmax^3, max_k1)."""
+        self = cls.__new__(cls)
+        v = np.asarray(v)
+        self.is_complex = np.iscomplexobj(v) or (e is not None and np.iscomplexobj(e))
+        vdt = np.complex128 if self.is_complex else np.float64
+        n = kmax ** 3
+        nk = np.ascontiguousarray(nk1, dtype=np.uint8).reshape(n)
+        kk = np.ascontiguousarray(k1, dtype=np.uint16).reshape(n, -1)
+        vv = np.ascontiguousarray(v, dtype=vdt).reshape(n, -1)
+        if kk.shape != vv.shape:
+            raise ValueError("k1 and v lists must have the same shape")
+        max_k1 = kk.shape[1]
+        qn = qq = ee = None
+        max_q1 = 0
+        if nq1 is not None:
+            qn = np.ascontiguousarray(nq1, dtype=np.uint8).reshape(kmax)
+            qq = np.ascontiguousarray(q1, dtype=np.uint16).reshape(kmax, -1)
+            ee = np.ascontiguousarray(e, dtype=vdt).reshape(kmax, -1)
+            if qq.shape != ee.shape:
+                raise ValueError("q1 and e lists must have the same shape")
+            max_q1 = qq.shape[1]
+        h = ctypes.c_void_p()
+        _check(lib().hd_tables_create_hash(kmax, int(self.is_complex), max_k1, _ptr(nk), _ptr(kk),
+                                           _ptr(vv.view(np.float64)), max_q1, _ptr(qn), _ptr(qq),
+                                           None if ee is None else _ptr(ee.view(np.float64)), ctypes.byref(h)))
+        self.h = h
+        self.layout, self.kmax = LAYOUT_HASH, kmax
+        self.ktable, self.vtable, self.eps = (nk, kk), vv, (qn, qq, ee)
+        return self
+
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib is not None:
             _lib.hd_tables_destroy(self.h)
             self.h = None
+
+
+def hash_lists_from_array(layout: int, kmax: int, ktable, vtable, eps=None) -> dict:
+    """ConvertTableFormat (interacting_ofl_model.cpp:597-619): the array
+    tables as hash lists, one k1 per (k2,k3,k4) from GetK1 with
+    V = GetVkkkk(k1,k2,k3,k4) (SetQuarticFromArray,
+    term_hash_tables_base.hpp:231-252; the folded layout's k3 >= k4 symmetry,
+    term_tables.cpp:95-175), and k1 = k2 with E_k2 for the quadratic terms
+    (SetQuadraticFromArray :264-275).  Sphere tables list no k1 where
+    GetK1 returns none (k1 >= kMax).  Keyword arguments of TermTables.hash."""
+    K = kmax
+    kt = np.asarray(ktable, dtype=np.int64)
+    vt = np.asarray(vtable)
+    k4, k3, k2 = np.meshgrid(np.arange(K), np.arange(K), np.arange(K), indexing="ij")
+    k4, k3, k2 = k4.ravel(), k3.ravel(), k2.ravel()  # entry (k4*K + k3)*K + k2
+    if layout == LAYOUT_SPHERE:
+        i = (k4 * K + k3) * K + k2
+        k1 = kt[i]
+        nk1 = (k1 < K).astype(np.uint8)
+        v = np.where(nk1 > 0, vt[i], 0)
+    elif layout == LAYOUT_OFL:
+        t1, t2 = np.minimum(k3, k4), np.maximum(k3, k4)
+        row = (t1 * K + t2 - t1 * (t1 + 1) // 2) * K
+        k1 = kt[row + k2]
+        v = vt[row + np.where(k4 > k3, k1, k2)]
+        nk1 = np.ones(K ** 3, dtype=np.uint8)
+    else:
+        raise ValueError("array layouts only")
+    out = {"nk1": nk1, "k1": np.where(nk1 > 0, k1, 0).reshape(-1, 1), "v": np.asarray(v).reshape(-1, 1)}
+    if eps is not None:
+        ep = np.asarray(eps)
+        out.update(nq1=np.ones(K, dtype=np.uint8), q1=np.arange(K).reshape(-1, 1), e=ep.reshape(-1, 1))
+    return out
+
+
+def ofl_wannier(dim_x: int, dim_y: int, offset_x: float, ktable, vtable, eps, u, tol: float,
+                timing: bool = False):
+    """ChangeToWannierBasis (interacting_ofl_model.cpp:728-923): the hybrid
+    Wannier tables as hash lists (keyword arguments of TermTables.hash) from
+    the folded Bloch-basis tables, the band energies eps (K, or None), the
+    K x K change of basis u and the tolerance; the quartic transform runs on
+    the device (kernel time in ms with timing=True)."""
+    K = dim_x * dim_y
+    kt = np.ascontiguousarray(ktable, dtype=np.uint16)
+    vt = np.ascontiguousarray(vtable, dtype=np.complex128)
+    uu = np.ascontiguousarray(u, dtype=np.complex128)
+    if kt.size != K * K * (K + 1) // 2 or vt.size != kt.size or uu.size != K * K:
+        raise ValueError("folded tables of K^2 (K+1)/2 entries and a K x K matrix expected")
+    nk1 = np.zeros(K ** 3, dtype=np.uint8)
+    k1 = np.zeros((K ** 3, dim_x), dtype=np.uint16)
+    v = np.zeros((K ** 3, dim_x), dtype=np.complex128)
+    ep = nq1 = q1 = e = None
+    if eps is not None:
+        ep = np.ascontiguousarray(eps, dtype=np.complex128)
+        if ep.size != K:
+            raise ValueError("eps needs K values")
+        nq1 = np.zeros(K, dtype=np.uint8)
+        q1 = np.zeros((K, dim_x), dtype=np.uint16)
+        e = np.zeros((K, dim_x), dtype=np.complex128)
+    ms = ctypes.c_double()
+    _check(lib().hd_ofl_wannier(dim_x, dim_y, float(offset_x), _ptr(kt), _ptr(vt.view(np.float64)),
+                                None if ep is None else _ptr(ep.view(np.float64)), _ptr(uu.view(np.float64)),
+                                float(tol), _ptr(nk1), _ptr(k1), _ptr(v.view(np.float64)), _ptr(nq1), _ptr(q1),
+                                None if e is None else _ptr(e.view(np.float64)), ctypes.byref(ms)))
+    out = {"nk1": nk1, "k1": k1, "v": v, "nq1": nq1, "q1": q1, "e": e}
+    return (out, ms.value) if timing else out
 
 
 def sphere_ktable(max_lz: int) -> np.ndarray:
@@ -493,6 +600,8 @@ class Hamiltonian:
                  zero_tol=ZERO_TOL, values=None):
         if values is None:
             values = {STORAGE_FULL: (F64, F32, CODED), STORAGE_MATRIX_FREE: (F64,)}.get(storage, (F64, F32))
+            if tables.layout == LAYOUT_HASH:
+                values = (F64, F32)  # general builds keep no coded copy
         mask = (1 if F64 in values else 0) | (2 if F32 in values else 0) | (4 if CODED in values else 0)
         h = ctypes.c_void_p()
         _check(lib().hd_ham_build(basis.h, tables.h, storage, zero_tol, mask, ctypes.byref(h)))

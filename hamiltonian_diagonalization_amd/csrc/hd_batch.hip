@@ -869,6 +869,7 @@ int hd_batch_create(const hd_basis* const* bases, int32_t n_sectors, const hd_ta
     for (int p = 0; p < n_points; ++p) {
         const hd_tables* t = tables[p];
         if (!t) return fail(HD_ERR_ARG, "null table");
+        if (t->layout == HD_LAYOUT_HASH) return fail(HD_ERR_UNSUPPORTED, "batched solves take array term tables");
         if (t->is_complex) return fail(HD_ERR_UNSUPPORTED, "batched solves take real tables");
         if (t->layout != t0->layout || t->K != t0->K || t->ktab != t0->ktab || t->eps.empty() != t0->eps.empty())
             return fail(HD_ERR_ARG, "batch tables must share layout, kMax, k table and quadratic presence");

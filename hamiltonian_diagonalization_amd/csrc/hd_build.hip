@@ -117,6 +117,8 @@ static void build_dictionary(Derived& D)
 
 static int derive_tables(const hd_tables* t, const hd_basis* b, double tol, Derived& D)
 {
+    if (t->layout == HD_LAYOUT_HASH)
+        return fail(HD_ERR_UNSUPPORTED, "hash term tables are built by the general build only (not batched)");
     const int K = t->K;
     const bool cplx = t->is_complex != 0;
     D.K = K;
@@ -614,6 +616,42 @@ k_scan_final(const int32_t* __restrict__ counts, int64_t n, const long long* __r
     }
 }
 
+int scan_counts(const int32_t* d_counts, int64_t n, int64_t* d_rowptr, int64_t* total, int* max_count,
+                hipStream_t st)
+{
+    const int64_t per = (int64_t)kScanBlock * kScanItems;
+    const int nb = (int)std::max<int64_t>(1, (n + per - 1) / per);
+    long long* d_bsum = nullptr;
+    int* d_bmax = nullptr;
+    long long* d_res = nullptr;  // [0] total, [1] max (as int)
+    auto cleanup = [&]() { dfree(d_bsum); dfree(d_bmax); dfree(d_res); };
+    int status = dalloc_t(&d_bsum, (size_t)nb, "scan partials");
+    if (status == HD_OK) status = dalloc_t(&d_bmax, (size_t)nb, "scan maxima");
+    if (status == HD_OK) status = dalloc_t(&d_res, 2, "scan results");
+    if (status != HD_OK) {
+        cleanup();
+        return status;
+    }
+    long long host[2] = {0, 0};
+    HipErrors q;
+    HD_ENQ(q, hipMemsetAsync(d_res, 0, 2 * sizeof(long long), st));
+    HD_ENQ(q, hipMemsetAsync(d_rowptr, 0, sizeof(int64_t), st));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_scan_partial, dim3(nb), dim3(kScanBlock), 0, st, d_counts, n, d_bsum, d_bmax);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, d_bsum, d_bmax, nb, d_res,
+                           reinterpret_cast<int*>(d_res + 1));
+        hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kScanBlock), 0, st, d_counts, n, d_bsum, d_rowptr);
+        HD_ENQ(q, hipGetLastError());
+    }
+    HD_ENQ(q, hipMemcpyAsync(host, d_res, sizeof(host), hipMemcpyDeviceToHost, st));
+    HD_ENQ(q, hipStreamSynchronize(st));
+    cleanup();
+    HD_CHECK(q.status());
+    *total = (int64_t)host[0];
+    *max_count = (int)host[1];
+    return HD_OK;
+}
+
 // --------------------------------------------------------------- launchers
 
 static int set_lds(const void* fn, size_t lds)
@@ -1071,6 +1109,61 @@ int hd_tables_create_two_level(uint32_t n1, const uint16_t* h_ktable1, const dou
     return HD_OK;
 }
 
+int hd_tables_create_hash(uint32_t kmax, int32_t is_complex, int32_t max_k1, const uint8_t* h_nk1,
+                          const uint16_t* h_k1, const double* h_v, int32_t max_q1, const uint8_t* h_nq1,
+                          const uint16_t* h_q1, const double* h_e, hd_tables** out)
+{
+    if (!out || !h_nk1 || (max_k1 > 0 && (!h_k1 || !h_v)))
+        return fail(HD_ERR_ARG, "hd_tables_create_hash: null argument");
+    *out = nullptr;
+    if (kmax == 0 || kmax > 64) return fail(HD_ERR_ARG, "kMax must be in 1..64");
+    if (max_k1 < 0 || max_k1 > 255 || max_q1 < 0 || max_q1 > 255)
+        return fail(HD_ERR_ARG, "list lengths must be in 0..255");
+    if (h_nq1 && max_q1 > 0 && (!h_q1 || !h_e)) return fail(HD_ERR_ARG, "hd_tables_create_hash: null quadratic list");
+    const size_t K = kmax, ne = K * K * K, w = is_complex ? 2 : 1;
+    for (size_t e = 0; e < ne; ++e) {
+        if (h_nk1[e] > max_k1) return fail(HD_ERR_ARG, "k1 list longer than max_k1 at entry " + std::to_string(e));
+        for (int j = 0; j < h_nk1[e]; ++j)
+            if (h_k1[e * (size_t)max_k1 + j] >= kmax)
+                return fail(HD_ERR_ARG, "k1 label >= kMax at entry " + std::to_string(e));
+    }
+    if (h_nq1)
+        for (size_t k = 0; k < K; ++k) {
+            if (h_nq1[k] > max_q1) return fail(HD_ERR_ARG, "quadratic list longer than max_q1");
+            for (int j = 0; j < h_nq1[k]; ++j)
+                if (h_q1[k * (size_t)max_q1 + j] >= kmax) return fail(HD_ERR_ARG, "quadratic k1 label >= kMax");
+        }
+    hd_tables* t = new hd_tables();
+    t->layout = HD_LAYOUT_HASH;
+    t->K = (int)kmax;
+    t->is_complex = is_complex ? 1 : 0;
+    t->max_k1 = std::max(max_k1, 1);
+    t->nk1.assign(h_nk1, h_nk1 + ne);
+    t->k1.assign(ne * (size_t)t->max_k1, 0);
+    t->hv.assign(ne * (size_t)t->max_k1 * w, 0.0);
+    for (size_t e = 0; e < ne; ++e)
+        for (int j = 0; j < h_nk1[e]; ++j) {
+            const size_t i = e * (size_t)max_k1 + j, o = e * (size_t)t->max_k1 + j;
+            t->k1[o] = (uint8_t)h_k1[i];
+            for (size_t q = 0; q < w; ++q) t->hv[o * w + q] = h_v[i * w + q];
+        }
+    if (h_nq1) {
+        t->has_quad = 1;
+        t->max_q1 = std::max(max_q1, 1);
+        t->nq1.assign(h_nq1, h_nq1 + K);
+        t->q1.assign(K * (size_t)t->max_q1, 0);
+        t->he.assign(K * (size_t)t->max_q1 * w, 0.0);
+        for (size_t k = 0; k < K; ++k)
+            for (int j = 0; j < h_nq1[k]; ++j) {
+                const size_t i = k * (size_t)max_q1 + j, o = k * (size_t)t->max_q1 + j;
+                t->q1[o] = (uint8_t)h_q1[i];
+                for (size_t q = 0; q < w; ++q) t->he[o * w + q] = h_e[i * w + q];
+            }
+    }
+    *out = t;
+    return HD_OK;
+}
+
 int hd_tables_destroy(hd_tables* t)
 {
     delete t;
@@ -1093,6 +1186,31 @@ int hd_ham_build(const hd_basis* b, const hd_tables* t, int32_t storage, double 
     if (t->K != b->Norb)
         return fail(HD_ERR_UNSUPPORTED, "kMax (" + std::to_string(t->K) + ") must equal the number of orbitals (" +
                                             std::to_string(b->Norb) + ")");
+    if (t->layout == HD_LAYOUT_HASH) {
+        if (storage == HD_STORAGE_MATRIX_FREE)
+            return fail(HD_ERR_UNSUPPORTED, "hash term tables: full or upper storage only");
+        if (value_mask & 4)
+            return fail(HD_ERR_UNSUPPORTED, "hash term tables: FP64/FP32 values only (no coded copy)");
+        hd_hamiltonian* h = new hd_hamiltonian();
+        h->basis = b;
+        h->dim = b->dim;
+        h->storage = storage;
+        h->is_complex = t->is_complex;
+        if (hipGetDevice(&h->device) != hipSuccess ||
+            hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete h;
+            return fail(HD_ERR_HIP, "stream creation failed");
+        }
+        const int st = general_build(h, b, t, zero_tol, value_mask);
+        if (st != HD_OK) {
+            const std::string msg = hd_last_error();
+            hd_ham_destroy(h);
+            set_error(msg);
+            return st;
+        }
+        *out = h;
+        return HD_OK;
+    }
     Derived D;
     HD_CHECK(derive_tables(t, b, zero_tol, D));
     if ((value_mask & 4) && !D.codable)

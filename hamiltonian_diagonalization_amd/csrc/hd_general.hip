@@ -1,0 +1,499 @@
+// hd_general.hip -- the Hamiltonian build for general (hash) term tables.
+//
+// Reference: SpinlessFermionHamiltonian<T>::Add_CdC_Terms / Add_CdCdCC_Terms
+// (src/hamiltonians/spinless_fermion_hamiltonian.hpp:143-257, 281-436) reading
+// QuarticTermHashTables / QuadraticTermHashTables through GetK1 (a list of k1
+// per key), GetVkkkk and GetEkk
+// (src/optical_flux_lattice/interacting_ofl_model/term_hash_tables.cpp:51-130),
+// then MappedSparseMatrix::Trim and the map -> CRS conversion
+// (mapped_sparse_matrix.hpp:355-462, crs_sparse_matrix.hpp:193-254).
+//
+// With general tables a (k2,k3,k4) carries a list of k1, quadratic terms hop
+// (k1 != k2), and the terms that land on one column are no longer the four
+// terms of one pair hop, so the pair tables of hd_build.hip do not apply.
+// Every term is evaluated per row, and duplicates are summed afterwards:
+//   1. k_gcount: one thread per row walks the terms in the reference's order
+//      and counts the ones the reference inserts into its map (|el| > tol and,
+//      for upper storage, out >= in).
+//   2. exclusive scan -> raw row offsets.
+//   3. k_gfill: the same walk writes (column, value) in term order; an out
+//      state outside the sector (FindFockStateIndex's miss, skipped by the
+//      reference) gets a sentinel column and is dropped in 4.
+//   4. k_gmerge: one workgroup per row sorts (column << 32 | position) keys in
+//      LDS (bitonic; the position makes equal columns keep term order), sums
+//      each run of equal columns from +0.0 in term order -- the reference's
+//      map += sequence -- drops |sum| < tol (Trim, counted) and writes the
+//      kept entries packed at the row's raw offset.
+//   5. scan of the kept counts -> CRS row pointers; k_gcompact copies each
+//      row into the final FP64 / FP32 arrays.
+// Values are bit-identical to the reference's: the same terms, the same
+// products (sign * value is an exact negation or copy), the same additions in
+// the same order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "hd_internal.hpp"
+
+namespace hd {
+
+namespace {
+
+template <bool CPLX> struct GT;
+template <> struct GT<false> { using T = double; using F = float; };
+template <> struct GT<true> { using T = double2; using F = float2; };
+
+HD_DEV double gabs(double x) { return fabs(x); }
+HD_DEV double gabs(double2 x) { return hypot(x.x, x.y); }
+HD_DEV double gadd(double a, double b) { return a + b; }
+HD_DEV double2 gadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+HD_DEV double gneg(double a) { return -a; }
+HD_DEV double2 gneg(double2 a) { return make_double2(-a.x, -a.y); }
+template <typename T> HD_DEV T gzero();
+template <> HD_DEV double gzero<double>() { return 0.0; }
+template <> HD_DEV double2 gzero<double2>() { return make_double2(0.0, 0.0); }
+HD_DEV float gf32(double v) { return (float)v; }
+HD_DEV float2 gf32(double2 v) { return make_float2((float)v.x, (float)v.y); }
+
+constexpr int32_t kNoColumn = 0x7FFFFFFF;  // out state outside the sector
+constexpr int kMergeThreads = 256;
+
+struct GenArgs {
+    int K, max_k1, max_q1, has_quad;
+    double tol;
+    uint64_t mask;         // (1 << Norb) - 1: the reference's k2 candidate mask
+    const uint8_t* nk1;    // [K^3] list lengths, entry (k4*K + k3)*K + k2
+    const uint8_t* k1;     // [K^3 * max_k1]
+    const void* v;         // [K^3 * max_k1] T
+    const uint8_t* nq1;    // [K]
+    const uint8_t* q1;     // [K * max_q1]
+    const void* e;         // [K * max_q1] T
+};
+
+// The terms of one row in the reference's order, f(out, el) for each term the
+// reference inserts into its map.  Quadratic terms (Add_CdC_Terms :172-228):
+// k2 over occupied orbitals ascending, k1 over GetK1(k2); k1 == k2 is the
+// diagonal E, otherwise k1 must be empty and c+_k1 c_k2 gives the sign
+// (Evaluate_CdC, fermions.hpp:155-227).  Quartic terms (Add_CdCdCC_Terms
+// :316-398): k4 over occupied orbitals, k3 over the other occupied ones, k2
+// over mask ^ (occupied orbitals above k3 other than k4), k1 over
+// GetK1(k2,k3,k4), the occupation filter of :351-355, the sign of
+// c+_k1 c+_k2 c_k3 c_k4 (Evaluate_CdCdCC, fermions.hpp:52-144).
+template <bool CPLX, bool UPPER, typename Fn>
+HD_DEV void for_each_term(uint64_t s, const GenArgs& A, Fn&& f)
+{
+    using T = typename GT<CPLX>::T;
+    if (A.has_quad) {
+        const T* E = static_cast<const T*>(A.e);
+        for (uint64_t t2 = s; t2; t2 &= t2 - 1) {
+            const int k2 = __builtin_ctzll(t2);
+            const uint64_t b2 = 1ull << k2;
+            const int n = A.nq1[k2];
+            for (int j = 0; j < n; ++j) {
+                const int idx = k2 * A.max_q1 + j;
+                const int k1 = A.q1[idx];
+                if (k1 == k2) {
+                    const T el = E[idx];
+                    if (gabs(el) > A.tol) f(s, el);
+                    continue;
+                }
+                const uint64_t b1 = 1ull << k1;
+                if (b1 & s) continue;  // c+_k1 on an occupied orbital
+                int c = __popcll(s & (b2 - 1));
+                uint64_t out = s ^ b2;
+                c += __popcll(out & (b1 - 1));
+                out |= b1;
+                if (UPPER && out < s) continue;
+                T el = E[idx];
+                if (c & 1) el = gneg(el);
+                if (gabs(el) > A.tol) f(out, el);
+            }
+        }
+    }
+    const T* V = static_cast<const T*>(A.v);
+    const int K = A.K;
+    for (uint64_t t4 = s; t4; t4 &= t4 - 1) {
+        const int k4 = __builtin_ctzll(t4);
+        const uint64_t b4 = 1ull << k4;
+        uint64_t t3 = s ^ b4;
+        while (t3) {
+            const uint64_t b3 = t3 & (0 - t3);
+            const int k3 = __builtin_ctzll(t3);
+            const uint64_t b34 = b3 | b4;
+            const int ebase = (k4 * K + k3) * K;
+            for (uint64_t t2 = A.mask ^ (t3 ^ b3); t2; t2 &= t2 - 1) {
+                const int k2 = __builtin_ctzll(t2);
+                const uint64_t b2 = 1ull << k2;
+                if ((b2 & s) && !(b2 & b34)) continue;
+                const int ent = ebase + k2;
+                const int n = A.nk1[ent];
+                for (int j = 0; j < n; ++j) {
+                    const int idx = ent * A.max_k1 + j;
+                    const uint64_t b1 = 1ull << A.k1[idx];
+                    if ((b1 & s) && !(b1 & b34)) continue;
+                    if (b1 & b2) continue;
+                    int c = __popcll(s & (b4 - 1));
+                    uint64_t out = s ^ b4;
+                    c += __popcll(out & (b3 - 1));
+                    out ^= b3;
+                    c += __popcll(out & (b2 - 1));
+                    out |= b2;
+                    c += __popcll(out & (b1 - 1));
+                    out |= b1;
+                    if (UPPER && out < s) continue;
+                    T el = V[idx];
+                    if (c & 1) el = gneg(el);
+                    if (gabs(el) > A.tol) f(out, el);
+                }
+            }
+            t3 ^= b3;
+        }
+    }
+}
+
+template <bool CPLX, bool UPPER>
+__global__ void __launch_bounds__(256)
+k_gcount(const uint64_t* __restrict__ states, int64_t dim, GenArgs A, int32_t* __restrict__ counts)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < dim; row += stride) {
+        int cnt = 0;
+        for_each_term<CPLX, UPPER>(states[row], A, [&](uint64_t, const typename GT<CPLX>::T&) { ++cnt; });
+        counts[row] = cnt;
+    }
+}
+
+template <bool CPLX, bool UPPER, bool LZ>
+__global__ void __launch_bounds__(256)
+k_gfill(const uint64_t* __restrict__ states, int64_t dim, GenArgs A, const int32_t* __restrict__ rank_blob,
+        int rank_words, const int64_t* __restrict__ rowptr, int32_t* __restrict__ col_out,
+        typename GT<CPLX>::T* __restrict__ val_out, unsigned int* __restrict__ err)
+{
+    using T = typename GT<CPLX>::T;
+    extern __shared__ int32_t lds[];
+    block_copy_words(lds, rank_blob, rank_words);
+    __syncthreads();
+    const RankView rv = rank_view(lds);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < dim; row += stride) {
+        int64_t pos = rowptr[row];
+        const int64_t end = rowptr[row + 1];
+        for_each_term<CPLX, UPPER>(states[row], A, [&](uint64_t out, const T& el) {
+            if (pos >= end) {  // more terms than the count pass saw: never write past the row
+                atomicOr(err, 4u);
+                return;
+            }
+            uint64_t r;
+            col_out[pos] = rank_state<LZ>(rv, out, r) ? (int32_t)r : kNoColumn;
+            val_out[pos] = el;
+            ++pos;
+        });
+        if (pos != end) atomicOr(err, 2u);
+    }
+}
+
+// One workgroup per row: stable sort by column, run sums in term order, Trim,
+// packed write of the kept entries at the row's raw offset.
+template <bool CPLX, int CAP>
+__global__ void __launch_bounds__(kMergeThreads)
+k_gmerge(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col_in,
+         const typename GT<CPLX>::T* __restrict__ val_in, double tol, int32_t* __restrict__ col_out,
+         typename GT<CPLX>::T* __restrict__ val_out, int32_t* __restrict__ kept,
+         unsigned long long* __restrict__ ntrim)
+{
+    using T = typename GT<CPLX>::T;
+    __shared__ unsigned long long key[CAP];
+    __shared__ int part[kMergeThreads];
+    const int tid = threadIdx.x;
+    unsigned long long trimmed = 0;
+    for (int64_t row = blockIdx.x; row < dim; row += gridDim.x) {
+        const int64_t start = rowptr[row];
+        const int L = (int)(rowptr[row + 1] - start);
+        int n = 1;
+        while (n < L) n <<= 1;
+        for (int q = tid; q < n; q += kMergeThreads)
+            key[q] = q < L ? (((unsigned long long)(uint32_t)col_in[start + q] << 32) | (unsigned)q) : ~0ull;
+        __syncthreads();
+        for (int k = 2; k <= n; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = tid; i < n; i += kMergeThreads) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const unsigned long long ki = key[i], kl = key[l];
+                        if ((ki > kl) == ((i & k) == 0)) {
+                            key[i] = kl;
+                            key[l] = ki;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // contiguous chunk of sorted positions per thread; the head of a run
+        // of equal columns sums the whole run (which may cross chunks)
+        const int per = (L + kMergeThreads - 1) / kMergeThreads;
+        const int i0 = min(L, tid * per), i1 = min(L, i0 + per);
+        auto run_sum = [&](int i, int32_t col) {
+            T acc = gzero<T>();
+            for (int q = i; q < L && (int32_t)(key[q] >> 32) == col; ++q)
+                acc = gadd(acc, val_in[start + (int64_t)(key[q] & 0xFFFFFFFFull)]);
+            return acc;
+        };
+        auto is_head = [&](int i, int32_t col) { return i == 0 || (int32_t)(key[i - 1] >> 32) != col; };
+        int c = 0;
+        for (int i = i0; i < i1; ++i) {
+            const int32_t col = (int32_t)(key[i] >> 32);
+            if (col == kNoColumn || !is_head(i, col)) continue;
+            if (gabs(run_sum(i, col)) < tol) ++trimmed;
+            else ++c;
+        }
+        part[tid] = c;
+        __syncthreads();
+        if (tid == 0) {
+            int acc = 0;
+            for (int t = 0; t < kMergeThreads; ++t) {
+                const int v = part[t];
+                part[t] = acc;
+                acc += v;
+            }
+            kept[row] = acc;
+        }
+        __syncthreads();
+        int64_t dst = start + part[tid];
+        for (int i = i0; i < i1; ++i) {
+            const int32_t col = (int32_t)(key[i] >> 32);
+            if (col == kNoColumn || !is_head(i, col)) continue;
+            const T sum = run_sum(i, col);
+            if (gabs(sum) < tol) continue;
+            col_out[dst] = col;
+            val_out[dst] = sum;
+            ++dst;
+        }
+        __syncthreads();
+    }
+    if (trimmed) atomicAdd(ntrim, trimmed);
+}
+
+template <bool CPLX>
+__global__ void __launch_bounds__(256)
+k_gcompact(int64_t dim, const int64_t* __restrict__ raw_ptr, const int64_t* __restrict__ rowptr,
+           const int32_t* __restrict__ mcol, const typename GT<CPLX>::T* __restrict__ mval,
+           int32_t* __restrict__ col, typename GT<CPLX>::T* __restrict__ val64, typename GT<CPLX>::F* __restrict__ val32)
+{
+    for (int64_t row = blockIdx.x; row < dim; row += gridDim.x) {
+        const int64_t src = raw_ptr[row], dst = rowptr[row];
+        const int len = (int)(rowptr[row + 1] - dst);
+        for (int q = threadIdx.x; q < len; q += blockDim.x) {
+            col[dst + q] = mcol[src + q];
+            const typename GT<CPLX>::T v = mval[src + q];
+            if (val64) val64[dst + q] = v;
+            if (val32) val32[dst + q] = gf32(v);
+        }
+    }
+}
+
+template <bool CPLX, int CAP>
+void launch_merge(int grid, hipStream_t st, int64_t dim, const int64_t* raw_ptr, const int32_t* rcol,
+                  const typename GT<CPLX>::T* rval, double tol, int32_t* mcol, typename GT<CPLX>::T* mval,
+                  int32_t* kept, unsigned long long* ntrim)
+{
+    hipLaunchKernelGGL((k_gmerge<CPLX, CAP>), dim3(grid), dim3(kMergeThreads), 0, st, dim, raw_ptr, rcol, rval, tol,
+                       mcol, mval, kept, ntrim);
+}
+
+template <bool CPLX>
+int run_general(hd_hamiltonian* h, const hd_basis* b, const hd_tables* t, double tol, int value_mask)
+{
+    using T = typename GT<CPLX>::T;
+    using F = typename GT<CPLX>::F;
+    const hipStream_t st = h->stream;
+    const int64_t dim = b->dim;
+    const int K = t->K;
+    int n_cu = 256;
+    HD_CHECK(device_props(&n_cu, nullptr));
+    uint8_t *d_nk1 = nullptr, *d_k1 = nullptr, *d_nq1 = nullptr, *d_q1 = nullptr;
+    double *d_v = nullptr, *d_e = nullptr;
+    int32_t *d_counts = nullptr, *d_rcol = nullptr, *d_mcol = nullptr;
+    int64_t* d_raw = nullptr;
+    T *d_rval = nullptr, *d_mval = nullptr;
+    unsigned long long* d_misc = nullptr;  // [0] trimmed, [1] error flags
+    auto cleanup = [&]() {
+        dfree(d_nk1); dfree(d_k1); dfree(d_nq1); dfree(d_q1); dfree(d_v); dfree(d_e);
+        dfree(d_counts); dfree(d_rcol); dfree(d_mcol); dfree(d_raw); dfree(d_rval); dfree(d_mval); dfree(d_misc);
+    };
+    int status = HD_OK;
+#define HD_GTRY(x)             \
+    do {                       \
+        status = (x);          \
+        if (status != HD_OK) { \
+            cleanup();         \
+            return status;     \
+        }                      \
+    } while (0)
+    HipErrors q;
+    HD_GTRY(dalloc_t(&d_nk1, t->nk1.size(), "k1 list lengths"));
+    HD_GTRY(dalloc_t(&d_k1, t->k1.size(), "k1 lists"));
+    HD_GTRY(dalloc_t(&d_v, t->hv.size(), "quartic values"));
+    HD_GTRY(dalloc_t(&d_misc, 2, "counters"));
+    HD_ENQ(q, hipMemcpyAsync(d_nk1, t->nk1.data(), t->nk1.size(), hipMemcpyHostToDevice, st));
+    HD_ENQ(q, hipMemcpyAsync(d_k1, t->k1.data(), t->k1.size(), hipMemcpyHostToDevice, st));
+    HD_ENQ(q, hipMemcpyAsync(d_v, t->hv.data(), t->hv.size() * 8, hipMemcpyHostToDevice, st));
+    HD_ENQ(q, hipMemsetAsync(d_misc, 0, 2 * sizeof(unsigned long long), st));
+    if (t->has_quad) {
+        HD_GTRY(dalloc_t(&d_nq1, t->nq1.size(), "quadratic list lengths"));
+        HD_GTRY(dalloc_t(&d_q1, t->q1.size(), "quadratic lists"));
+        HD_GTRY(dalloc_t(&d_e, t->he.size(), "quadratic values"));
+        HD_ENQ(q, hipMemcpyAsync(d_nq1, t->nq1.data(), t->nq1.size(), hipMemcpyHostToDevice, st));
+        HD_ENQ(q, hipMemcpyAsync(d_q1, t->q1.data(), t->q1.size(), hipMemcpyHostToDevice, st));
+        HD_ENQ(q, hipMemcpyAsync(d_e, t->he.data(), t->he.size() * 8, hipMemcpyHostToDevice, st));
+    }
+    GenArgs A;
+    A.K = K;
+    A.max_k1 = t->max_k1;
+    A.max_q1 = t->max_q1;
+    A.has_quad = t->has_quad;
+    A.tol = tol;
+    A.mask = b->Norb >= 64 ? ~0ull : ((1ull << b->Norb) - 1ull);
+    A.nk1 = d_nk1;
+    A.k1 = d_k1;
+    A.v = d_v;
+    A.nq1 = d_nq1;
+    A.q1 = d_q1;
+    A.e = d_e;
+    const bool upper = h->storage == HD_STORAGE_UPPER_REF;
+    const int grid_rows = (int)std::max<int64_t>(1, std::min<int64_t>((dim + 255) / 256, (int64_t)n_cu * 8));
+    EventTimer ev;
+
+    // ---- count
+    HD_GTRY(dalloc_t(&d_counts, (size_t)dim, "row counts"));
+    ev.start(st);
+    if (upper) hipLaunchKernelGGL((k_gcount<CPLX, true>), dim3(grid_rows), dim3(256), 0, st, b->d_states, dim, A, d_counts);
+    else hipLaunchKernelGGL((k_gcount<CPLX, false>), dim3(grid_rows), dim3(256), 0, st, b->d_states, dim, A, d_counts);
+    HD_ENQ(q, hipGetLastError());
+    h->t_count = ev.stop(st);
+    HD_GTRY(q.status());
+
+    // ---- scan of the raw counts
+    HD_GTRY(dalloc_t(&d_raw, (size_t)dim + 1, "raw row offsets"));
+    int64_t raw_nnz = 0;
+    int raw_max = 0;
+    ev.start(st);
+    HD_GTRY(scan_counts(d_counts, dim, d_raw, &raw_nnz, &raw_max, st));
+    h->t_scan = ev.stop(st);
+    if (raw_max > 8192) {
+        cleanup();
+        return fail(HD_ERR_UNSUPPORTED, "a row has " + std::to_string(raw_max) +
+                                            " terms before merging; the per-row merge takes at most 8192");
+    }
+
+    // ---- fill in term order
+    HD_GTRY(dalloc_t(&d_rcol, (size_t)std::max<int64_t>(raw_nnz, 1), "raw columns"));
+    HD_GTRY(dalloc_t(&d_rval, (size_t)std::max<int64_t>(raw_nnz, 1), "raw values"));
+    {
+        const int rank_words = (int)b->rt.blob.size();
+        const size_t lds = (size_t)rank_words * 4;
+        if (lds > 160 * 1024) {
+            cleanup();
+            return fail(HD_ERR_UNSUPPORTED, "sector rank table exceeds LDS");
+        }
+        const bool lz = b->rt.is_lz != 0;
+        unsigned int* d_err = reinterpret_cast<unsigned int*>(d_misc + 1);
+#define HD_GFILL(U, LZV)                                                                                      \
+    do {                                                                                                      \
+        if (lds > 64 * 1024)                                                                                  \
+            HD_ENQ(q, hipFuncSetAttribute((const void*)k_gfill<CPLX, U, LZV>,                                 \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));             \
+        hipLaunchKernelGGL((k_gfill<CPLX, U, LZV>), dim3(grid_rows), dim3(256), lds, st, b->d_states, dim, A, \
+                           b->d_rank, rank_words, d_raw, d_rcol, d_rval, d_err);                              \
+    } while (0)
+        ev.start(st);
+        if (upper && lz) HD_GFILL(true, true);
+        else if (upper) HD_GFILL(true, false);
+        else if (lz) HD_GFILL(false, true);
+        else HD_GFILL(false, false);
+#undef HD_GFILL
+        HD_ENQ(q, hipGetLastError());
+        h->t_fill = ev.stop(st);
+        HD_GTRY(q.status());
+        unsigned long long host_misc[2] = {0, 0};
+        HD_ENQ(q, hipMemcpyAsync(host_misc, d_misc, sizeof(host_misc), hipMemcpyDeviceToHost, st));
+        HD_ENQ(q, hipStreamSynchronize(st));
+        HD_GTRY(q.status());
+        if (host_misc[1] != 0) {
+            cleanup();
+            return fail(HD_ERR_UNSUPPORTED, "general fill pass inconsistency (flags " + std::to_string(host_misc[1]) + ")");
+        }
+    }
+    dfree(d_nk1); dfree(d_k1); dfree(d_v); dfree(d_nq1); dfree(d_q1); dfree(d_e);
+    d_nk1 = d_k1 = d_nq1 = d_q1 = nullptr;
+    d_v = d_e = nullptr;
+
+    // ---- merge duplicates per row, Trim
+    HD_GTRY(dalloc_t(&d_mcol, (size_t)std::max<int64_t>(raw_nnz, 1), "merged columns"));
+    HD_GTRY(dalloc_t(&d_mval, (size_t)std::max<int64_t>(raw_nnz, 1), "merged values"));
+    int64_t nnz = 0;
+    int max_row = 0;
+    {
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(dim, (int64_t)n_cu * 8));
+        ev.start(st);
+        if (raw_max <= 256) launch_merge<CPLX, 256>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
+        else if (raw_max <= 1024) launch_merge<CPLX, 1024>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
+        else if (raw_max <= 4096) launch_merge<CPLX, 4096>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
+        else launch_merge<CPLX, 8192>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
+        HD_ENQ(q, hipGetLastError());
+        HD_GTRY(q.status());
+        dfree(d_rcol);
+        dfree(d_rval);
+        d_rcol = nullptr;
+        d_rval = nullptr;
+        HD_GTRY(dalloc_t(&h->d_rowptr, (size_t)dim + 1, "row pointers"));
+        h->bytes += ((size_t)dim + 1) * 8;
+        HD_GTRY(scan_counts(d_counts, dim, h->d_rowptr, &nnz, &max_row, st));
+        h->nnz = nnz;
+        h->max_row = max_row;
+        unsigned long long trimmed = 0;
+        HD_ENQ(q, hipMemcpyAsync(&trimmed, d_misc, sizeof(trimmed), hipMemcpyDeviceToHost, st));
+        HD_ENQ(q, hipStreamSynchronize(st));
+        HD_GTRY(q.status());
+        h->n_trimmed = (int64_t)trimmed;
+        // ---- final arrays, padded by 8 zeroed entries (the SpMV reads aligned
+        // groups of 4 that may run past the last row)
+        const size_t n = (size_t)nnz;
+        HD_GTRY(dalloc_t(&h->d_col, n + 8, "columns"));
+        HD_ENQ(q, hipMemsetAsync(h->d_col + n, 0, 8 * sizeof(int32_t), st));
+        h->bytes += n * 4;
+        if (value_mask & 1) {
+            HD_GTRY(dalloc(&h->d_val64, (n + 8) * sizeof(T), "FP64 values"));
+            HD_ENQ(q, hipMemsetAsync(static_cast<T*>(h->d_val64) + n, 0, 8 * sizeof(T), st));
+            h->bytes += n * sizeof(T);
+        }
+        if (value_mask & 2) {
+            HD_GTRY(dalloc(&h->d_val32, (n + 8) * sizeof(F), "FP32 values"));
+            HD_ENQ(q, hipMemsetAsync(static_cast<F*>(h->d_val32) + n, 0, 8 * sizeof(F), st));
+            h->bytes += n * sizeof(F);
+        }
+        hipLaunchKernelGGL((k_gcompact<CPLX>), dim3(grid), dim3(256), 0, st, dim, d_raw, h->d_rowptr, d_mcol, d_mval,
+                           h->d_col, static_cast<T*>(h->d_val64), static_cast<F*>(h->d_val32));
+        HD_ENQ(q, hipGetLastError());
+        h->t_sort = ev.stop(st);
+        HD_ENQ(q, hipStreamSynchronize(st));
+        HD_GTRY(q.status());
+    }
+    HD_GTRY(ev.status("general build timing"));
+    cleanup();
+#undef HD_GTRY
+    return HD_OK;
+}
+
+}  // namespace
+
+int general_build(hd_hamiltonian* h, const hd_basis* b, const hd_tables* t, double tol, int value_mask)
+{
+    if (t->layout != HD_LAYOUT_HASH) return fail(HD_ERR_ARG, "general_build: not a hash table");
+    if (b->dim > (int64_t)0x7FFFFFFE) return fail(HD_ERR_UNSUPPORTED, "sector too large for int32 columns");
+    return t->is_complex ? run_general<true>(h, b, t, tol, value_mask) : run_general<false>(h, b, t, tol, value_mask);
+}
+
+}  // namespace hd

@@ -273,6 +273,11 @@ struct hd_tables {
     // the second; the diagonal sums the first level's terms before the
     // second's.  0 for one level.
     int split = 0;
+    // HD_LAYOUT_HASH: k1 lists per (k2,k3,k4) at entry (k4*K + k3)*K + k2 and
+    // per k2, with one value per list slot (interleaved when complex)
+    int max_k1 = 0, max_q1 = 0, has_quad = 0;
+    std::vector<uint8_t> nk1, k1, nq1, q1;
+    std::vector<double> hv, he;
 };
 
 // Eigenpairs of the last Lanczos call on a Hamiltonian handle.
@@ -340,6 +345,13 @@ int build_batch_sector(const hd_basis* b, const hd_tables* const* tables, int n,
                        hd_hamiltonian** out);
 int spmv_launch(const hd_hamiltonian* h, int precision, const void* x, void* y,
                 hipStream_t stream);
+// Exclusive scan of n int32 counts into n+1 int64 row pointers on `stream`,
+// with the total and the largest count returned to the host (hd_build.hip).
+int scan_counts(const int32_t* d_counts, int64_t n, int64_t* d_rowptr, int64_t* total, int* max_count,
+                hipStream_t stream);
+// Build of HD_LAYOUT_HASH tables: full or upper storage, FP64/FP32 values
+// (hd_general.hip).
+int general_build(hd_hamiltonian* h, const hd_basis* b, const hd_tables* t, double tol, int value_mask);
 // Matrix-free FP64 product of a HD_STORAGE_MATRIX_FREE handle (hd_build.hip).
 int matrix_free_launch(const hd_hamiltonian* h, const void* x, void* y, hipStream_t stream);
 // Tiled coded layout (hd_tiled.hip).

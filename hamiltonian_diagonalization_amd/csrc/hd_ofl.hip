@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "hd_internal.hpp"
@@ -180,6 +182,58 @@ __global__ void __launch_bounds__(kOflLanes) k_ofl_vterms(OflArgs a)
     }
 }
 
+// Hybrid Wannier basis (ChangeToWannierBasis, interacting_ofl_model.cpp:820-905):
+// one thread per (k2, k3, k4, x1).  k1 = x1*dimY + ky1 with ky1 fixed by ky
+// momentum (FindK1 :410-432); the sum runs over k4x, k3x, k2x in the
+// reference's nesting (k2x fastest), q1 from the Bloch-basis array k table,
+// V from the folded array (GetK1 / GetVkkkk, term_tables.cpp:108-175), each
+// term V * conj(U1 * U2) * U3 * U4 multiplied left to right.
+struct WannierArgs {
+    int dimX, dimY, K;
+    double tol;
+    const uint16_t* kt;  // folded Bloch-basis k table
+    const double2* vt;   // folded Bloch-basis V
+    const double2* u;    // change of basis, K x K, row (x, ky), column (kx, ky')
+    uint8_t* nk1;        // [K^3]
+    uint8_t* k1;         // [K^3 * dimX]
+    double2* v;          // [K^3 * dimX]
+};
+
+__global__ void __launch_bounds__(256) k_wannier(WannierArgs a)
+{
+    const int K = a.K, X = a.dimX, Y = a.dimY;
+    const int64_t total = (int64_t)K * K * K * X;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        const int64_t e = idx / X;
+        const int x1 = (int)(idx - e * X);
+        const int k2 = (int)(e % K), k3 = (int)((e / K) % K), k4 = (int)(e / ((int64_t)K * K));
+        const int k2y = k2 % Y, k3y = k3 % Y, k4y = k4 % Y;
+        const int k1y = (((k3y + k4y - k2y) % Y) + Y) % Y;
+        const int k1 = x1 * Y + k1y;
+        double2 sum = make_double2(0.0, 0.0);
+        for (int k4x = 0; k4x < X; ++k4x)
+            for (int k3x = 0; k3x < X; ++k3x)
+                for (int k2x = 0; k2x < X; ++k2x) {
+                    const int q2 = k2x * Y + k2y, q3 = k3x * Y + k3y, q4 = k4x * Y + k4y;
+                    const int t1 = q4 > q3 ? q3 : q4, t2 = q4 > q3 ? q4 : q3;
+                    const int64_t row = ((int64_t)t1 * K + t2 - (int64_t)t1 * (t1 + 1) / 2) * K;
+                    const int q1 = a.kt[row + q2];
+                    const double2 V = a.vt[row + (q4 > q3 ? q1 : q2)];
+                    double2 p = cmul(a.u[(int64_t)k1 * K + q1], a.u[(int64_t)k2 * K + q2]);
+                    p.y = -p.y;
+                    double2 t = cmul(V, p);
+                    t = cmul(t, a.u[(int64_t)k3 * K + q3]);
+                    t = cmul(t, a.u[(int64_t)k4 * K + q4]);
+                    sum.x = sum.x + t.x;
+                    sum.y = sum.y + t.y;
+                }
+        a.k1[idx] = (uint8_t)k1;
+        a.v[idx] = hypot(sum.x, sum.y) > a.tol ? sum : make_double2(0.0, 0.0);
+        if (x1 == 0) a.nk1[e] = (uint8_t)X;
+    }
+}
+
 }  // namespace
 
 }  // namespace hd
@@ -257,6 +311,109 @@ int hd_ofl_vtable(int32_t dim_x, int32_t dim_y, int32_t cutoff, const double* h_
     }
     dfree(d_bloch);
     dfree(d_out);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+int hd_ofl_wannier(int32_t dim_x, int32_t dim_y, double offset_x, const uint16_t* h_ktable, const double* h_vtable,
+                   const double* h_eps, const double* h_u, double tol, uint8_t* h_nk1, uint16_t* h_k1,
+                   double* h_v, uint8_t* h_nq1, uint16_t* h_q1, double* h_e, double* kernel_ms)
+{
+    if (!h_ktable || !h_vtable || !h_u || !h_nk1 || !h_k1 || !h_v)
+        return fail(HD_ERR_ARG, "hd_ofl_wannier: null argument");
+    if (h_eps && (!h_nq1 || !h_q1 || !h_e)) return fail(HD_ERR_ARG, "hd_ofl_wannier: null quadratic output");
+    if (dim_x <= 0 || dim_y <= 0 || (int64_t)dim_x * dim_y > 64)
+        return fail(HD_ERR_ARG, "hd_ofl_wannier: dim_x * dim_y must be in 1..64");
+    const int K = dim_x * dim_y;
+    const int64_t folded = (int64_t)K * K * (K + 1) / 2;
+    for (int64_t i = 0; i < folded; ++i)
+        if (h_ktable[i] >= K) return fail(HD_ERR_ARG, "hd_ofl_wannier: k table label >= K");
+    // quadratic terms on the host: the backward DFT over kx of E/dim_x per ky
+    // (DiscreteFourierTransform1D with FFTW_BACKWARD), then the phase of
+    // |x2 - x1| (interacting_ofl_model.cpp:771-807)
+    if (h_eps) {
+        const long double two_pi = 6.283185307179586476925286766559005768L;
+        std::vector<double> out_re((size_t)K), out_im((size_t)K);
+        for (int ky = 0; ky < dim_y; ++ky)
+            for (int j = 0; j < dim_x; ++j) {
+                long double re = 0.0L, im = 0.0L;
+                for (int kx = 0; kx < dim_x; ++kx) {
+                    const double ire = h_eps[2 * (kx * dim_y + ky)] / (double)dim_x;
+                    const double iim = h_eps[2 * (kx * dim_y + ky) + 1] / (double)dim_x;
+                    const long double ang = two_pi * (long double)((kx * j) % dim_x) / (long double)dim_x;
+                    const long double c = cosl(ang), sn = sinl(ang);
+                    re += c * ire - sn * iim;
+                    im += c * iim + sn * ire;
+                }
+                out_re[(size_t)ky * dim_x + j] = (double)re;
+                out_im[(size_t)ky * dim_x + j] = (double)im;
+            }
+        const double pi = 3.1415926535897932384626433832795028841972;
+        for (int x2 = 0; x2 < dim_x; ++x2)
+            for (int ky = 0; ky < dim_y; ++ky) {
+                const int k2 = x2 * dim_y + ky;
+                h_nq1[k2] = (uint8_t)dim_x;
+                for (int x1 = 0; x1 < dim_x; ++x1) {
+                    const int d = std::abs(x2 - x1);
+                    double th = 2.0 * pi;
+                    th = th * (double)d;
+                    th = th * offset_x;
+                    th = th / (double)dim_x;
+                    const double c = std::cos(th), sn = std::sin(th);
+                    const double ore = out_re[(size_t)ky * dim_x + d], oim = out_im[(size_t)ky * dim_x + d];
+                    const double pr0 = c * ore, pr1 = sn * oim, pi0 = c * oim, pi1 = sn * ore;
+                    const size_t slot = (size_t)k2 * dim_x + x1;
+                    h_q1[slot] = (uint16_t)(x1 * dim_y + ky);
+                    h_e[2 * slot] = pr0 - pr1;
+                    h_e[2 * slot + 1] = pi0 + pi1;
+                }
+            }
+    }
+    int n_cu = 0;
+    HD_CHECK(device_props(&n_cu, nullptr));
+    const int64_t nslots = (int64_t)K * K * K * dim_x;
+    uint16_t* d_kt = nullptr;
+    double2 *d_vt = nullptr, *d_u = nullptr, *d_v = nullptr;
+    uint8_t *d_nk1 = nullptr, *d_k1 = nullptr;
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+        return fail(HD_ERR_HIP, "stream creation failed");
+    int rc = dalloc_t(&d_kt, (size_t)folded, "Bloch k table");
+    if (rc == HD_OK) rc = dalloc_t(&d_vt, (size_t)folded, "Bloch V table");
+    if (rc == HD_OK) rc = dalloc_t(&d_u, (size_t)K * K, "change of basis");
+    if (rc == HD_OK) rc = dalloc_t(&d_v, (size_t)nslots, "Wannier V");
+    if (rc == HD_OK) rc = dalloc_t(&d_nk1, (size_t)K * K * K, "Wannier list lengths");
+    if (rc == HD_OK) rc = dalloc_t(&d_k1, (size_t)nslots, "Wannier k1 lists");
+    std::vector<uint8_t> k1b;
+    if (rc == HD_OK) {
+        HipErrors q;
+        EventTimer ev;
+        HD_ENQ(q, hipMemcpyAsync(d_kt, h_ktable, (size_t)folded * 2, hipMemcpyHostToDevice, st));
+        HD_ENQ(q, hipMemcpyAsync(d_vt, h_vtable, (size_t)folded * sizeof(double2), hipMemcpyHostToDevice, st));
+        HD_ENQ(q, hipMemcpyAsync(d_u, h_u, (size_t)K * K * sizeof(double2), hipMemcpyHostToDevice, st));
+        WannierArgs a{dim_x, dim_y, K, tol, d_kt, d_vt, d_u, d_nk1, d_k1, d_v};
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((nslots + 255) / 256, (int64_t)n_cu * 16));
+        ev.start(st);
+        hipLaunchKernelGGL(k_wannier, dim3(grid), dim3(256), 0, st, a);
+        HD_ENQ(q, hipGetLastError());
+        const float ms = ev.stop(st);
+        k1b.resize((size_t)nslots);
+        HD_ENQ(q, hipMemcpyAsync(h_v, d_v, (size_t)nslots * sizeof(double2), hipMemcpyDeviceToHost, st));
+        HD_ENQ(q, hipMemcpyAsync(h_nk1, d_nk1, (size_t)K * K * K, hipMemcpyDeviceToHost, st));
+        HD_ENQ(q, hipMemcpyAsync(k1b.data(), d_k1, (size_t)nslots, hipMemcpyDeviceToHost, st));
+        HD_ENQ(q, hipStreamSynchronize(st));
+        rc = q.status();
+        if (rc == HD_OK) rc = ev.status("hd_ofl_wannier");
+        if (kernel_ms) *kernel_ms = ms;
+    }
+    if (rc == HD_OK)
+        for (int64_t i = 0; i < nslots; ++i) h_k1[i] = k1b[(size_t)i];
+    dfree(d_kt);
+    dfree(d_vt);
+    dfree(d_u);
+    dfree(d_v);
+    dfree(d_nk1);
+    dfree(d_k1);
     (void)hipStreamDestroy(st);
     return rc;
 }

@@ -145,3 +145,70 @@ def lattice_tables(dim_x: int, dim_y: int):
     inv = np.stack([(rx % dim_x) * dim_y + ry % dim_y for rx, ry in rot], axis=1)
     return add.astype(np.uint16), sub.astype(np.uint16), inv.astype(np.uint16)
 
+
+
+def random_hash_tables(kmax: int, seed: int = 0, complex_values: bool = True, max_k1: int = 3,
+                       quadratic: bool = True, fill: float = 0.5) -> dict:
+    """Synthetic general term tables in the list format of TermTables.hash,
+    drawn from numpy's PCG64 generator with `seed`: each (k2,k3,k4) entry
+    lists 0..max_k1 labels k1 (with probability `fill` per slot, repeats and
+    k1 in {k2,k3,k4} allowed) with normal (complex normal) values; with
+    `quadratic`, each k2 lists 0..max_k1 labels (k1 = k2 among them half the
+    time).  No symmetry: a stress input for the general build."""
+    rng = np.random.default_rng(seed)
+    K = kmax
+    n = K ** 3
+
+    def values(shape):
+        v = rng.normal(size=shape)
+        return v + 1j * rng.normal(size=shape) if complex_values else v
+
+    nk1 = (rng.random((n, max_k1)) < fill).sum(axis=1).astype(np.uint8)
+    k1 = rng.integers(0, K, size=(n, max_k1)).astype(np.uint16)
+    v = values((n, max_k1))
+    out = {"nk1": nk1, "k1": k1, "v": v}
+    if quadratic:
+        nq1 = (rng.random((K, max_k1)) < fill).sum(axis=1).astype(np.uint8)
+        q1 = rng.integers(0, K, size=(K, max_k1)).astype(np.uint16)
+        diag = rng.random(K) < 0.5
+        q1[diag, 0] = np.arange(K)[diag]
+        nq1[diag] = np.maximum(nq1[diag], 1)
+        out.update(nq1=nq1, q1=q1, e=values((K, max_k1)))
+    return out
+
+
+def berry_links(dim_x: int, dim_y: int, bloch) -> np.ndarray:
+    """A_x(kx, ky) of NonInteractingOflModelGrid::GenerateWannierCoefficients
+    (noninteracting_ofl_model_grid.cpp:799-814): the normalised overlap
+    <u(kx,ky)|u(kx+1,ky)> (zdotc) of the band-0 Bloch vectors, kx+1 periodic;
+    bloch has one vector per k = kx*dim_y + ky.  Returns a (dim_x, dim_y)
+    array of unit-modulus values."""
+    b = np.asarray(bloch, dtype=np.complex128).reshape(dim_x * dim_y, -1)
+    A = np.zeros((dim_x, dim_y), dtype=np.complex128)
+    for kx in range(dim_x):
+        for ky in range(dim_y):
+            a = np.vdot(b[kx * dim_y + ky], b[((kx + 1) % dim_x) * dim_y + ky])
+            A[kx, ky] = a / abs(a)
+    return A
+
+
+def wannier_matrix(dim_x: int, dim_y: int, offset_x: float, links) -> np.ndarray:
+    """The change-of-basis matrix of GenerateWannierCoefficients
+    (noninteracting_ofl_model_grid.cpp:815-925) from the links A_x(kx, ky):
+    lambda(ky) = (prod_kx A_x)^(1/dim_x) (principal branch),
+    aProd(kx, ky) = prod_{kappa < kx} A_x(kappa, ky), and
+    M[(x,ky), (kx,ky)] = exp(-2 pi i (kx + offset_x) x / dim_x) lambda(ky)^kx
+    / (sqrt(dim_x) aProd(kx, ky)), zero between different ky.  Unitary."""
+    A = np.asarray(links, dtype=np.complex128).reshape(dim_x, dim_y)
+    lam = np.prod(A, axis=0) ** (1.0 / dim_x)
+    aprod = np.ones((dim_x, dim_y), dtype=np.complex128)
+    for kx in range(1, dim_x):
+        aprod[kx] = A[kx - 1] * aprod[kx - 1]
+    K = dim_x * dim_y
+    M = np.zeros((K, K), dtype=np.complex128)
+    for x in range(dim_x):
+        for ky in range(dim_y):
+            for kx in range(dim_x):
+                M[x * dim_y + ky, kx * dim_y + ky] = (np.exp(-2j * np.pi * (kx + offset_x) * x / dim_x)
+                                                      * lam[ky] ** kx / (np.sqrt(dim_x) * aprod[kx, ky]))
+    return M

@@ -64,8 +64,10 @@ typedef struct {
 typedef enum {
     HD_LAYOUT_SPHERE = 0, /* index (k4*kMax+k3)*kMax+k2; k1 >= kMax means "no term"
                              (src/fqhe_sphere/pseudopotential_model/term_tables.cpp:109-149) */
-    HD_LAYOUT_OFL = 1     /* folded on k3>=k4: (t1*kMax+t2-t1(t1+1)/2)*kMax+t3
+    HD_LAYOUT_OFL = 1,    /* folded on k3>=k4: (t1*kMax+t2-t1(t1+1)/2)*kMax+t3
                              (src/optical_flux_lattice/interacting_ofl_model/term_tables.cpp:95-175) */
+    HD_LAYOUT_HASH = 2    /* general lists of k1 per (k2,k3,k4) and per k2 (hd_tables_create_hash;
+                             src/hamiltonians/term_hash_tables_base.hpp:44-276) */
 } hd_layout;
 
 /* Value formats of the device CRS / SpMV:
@@ -176,6 +178,28 @@ int hd_tables_create(int32_t layout, uint32_t kmax, const uint16_t* h_ktable,
 int hd_tables_create_two_level(uint32_t n1, const uint16_t* h_ktable1, const double* h_vtable1,
                                const double* h_eps1, const uint16_t* h_ktable2,
                                const double* h_vtable2, const double* h_eps2, hd_tables** out);
+/* TermHashTables (src/hamiltonians/term_hash_tables_base.hpp:44-276) as read by
+ * QuarticTermHashTables / QuadraticTermHashTables::GetK1, GetVkkkk and GetEkk
+ * (src/optical_flux_lattice/interacting_ofl_model/term_hash_tables.cpp:51-130):
+ * general term tables, given as the lists those calls return.
+ *   Quartic: for every (k2,k3,k4), entry e = (k4*kmax + k3)*kmax + k2 holds
+ *   h_nk1[e] <= max_k1 labels k1 = h_k1[e*max_k1 + j] (GetK1's buffer, in its
+ *   order) and V(k1,k2,k3,k4) = h_v[e*max_k1 + j] (interleaved re,im when
+ *   is_complex; 0.0 where the hash table holds no value, GetVkkkk's empty
+ *   return).  A (k2,k3,k4) the k table does not hold has h_nk1[e] = 0.
+ *   Quadratic: h_nq1[k2] <= max_q1 labels k1 = h_q1[k2*max_q1 + j] with
+ *   E(k1,k2) = h_e[k2*max_q1 + j]; h_nq1 = NULL when Add_CdC_Terms is not
+ *   called.  Off-diagonal quadratic terms (k1 != k2) are one-body hops.
+ * kmax <= 64; every label < kmax.  hd_ham_build with these tables runs the
+ * general device build (Add_CdC_Terms then Add_CdCdCC_Terms in the reference's
+ * loop order with these lists, out states outside the sector skipped as
+ * FindFockStateIndex's miss, duplicate columns of a row summed in term order
+ * by a per-row stable sort, then Trim): full or upper storage, FP64 and FP32
+ * value copies (no coded copy, so no tiled layout; not accepted by
+ * hd_batch_create). */
+int hd_tables_create_hash(uint32_t kmax, int32_t is_complex, int32_t max_k1, const uint8_t* h_nk1,
+                          const uint16_t* h_k1, const double* h_v, int32_t max_q1, const uint8_t* h_nq1,
+                          const uint16_t* h_q1, const double* h_e, hd_tables** out);
 int hd_tables_destroy(hd_tables* t);
 
 /* ------------------------------------------------- flux-lattice term tables */
@@ -198,6 +222,36 @@ int hd_ofl_ktable(int32_t dim_x, int32_t dim_y, uint16_t* h_ktable, int32_t* h_g
  * kernel time. */
 int hd_ofl_vtable(int32_t dim_x, int32_t dim_y, int32_t cutoff, const double* h_bloch, double* h_vtable,
                   double* kernel_ms);
+/* InteractingOflModel::ChangeToWannierBasis (interacting_ofl_model.cpp:728-923)
+ * with GenerateWannier2KTable / GenerateWannier4KTable (:300-355): the tables
+ * of the hybrid Wannier basis |x,ky> in the list format of
+ * hd_tables_create_hash (complex, max_k1 = max_q1 = dim_x), K = dim_x*dim_y.
+ * Inputs: the Bloch-basis array tables (h_ktable from hd_ofl_ktable, h_vtable
+ * the folded complex V after SetCoefficient), the complex band energies h_eps
+ * (K values, m_quadraticTables), the change-of-basis matrix h_u (K x K
+ * complex, row x*dim_y+ky, column kx*dim_y+ky',
+ * NonInteractingOflModelGrid::GenerateWannierCoefficients), offset_x and tol
+ * (matrixElementTol).
+ *   Quartic, on the device: for (k2,k3,k4) the list is k1 = x1*dim_y + ky1,
+ *   x1 ascending, ky1 = FindK1(k2y,k3y,k4y) (:410-432), and V'(k1,k2,k3,k4) is
+ *   the sum over k4x, k3x, k2x (k2x fastest) of
+ *   V(q1,q2,q3,q4) conj(U[k1,q1] U[k2,q2]) U[k3,q3] U[k4,q4] with q1 from the
+ *   array k table, products left to right, each complex product
+ *   (ac - bd, ad + bc) with every real product rounded, summed from 0 in FP64:
+ *   bit-identical to the sequential C++.  Stored where |sum| > tol, else 0.0
+ *   (no hash entry, GetVkkkk's empty return).
+ *   Quadratic, on the host: for k2 = (x2,ky) the list is (x1,ky), x1
+ *   ascending, E'(k1,k2) = exp(2 pi i |x2-x1| offset_x/dim_x) out[ky][|x2-x1|]
+ *   with out[ky] the unnormalised backward DFT (sign +1) of E(kx,ky)/dim_x
+ *   over kx.  The reference takes the DFT from FFTW
+ *   (DiscreteFourierTransform1D, fftw_wrapper.cpp:36-70); here it is the
+ *   direct sum in long double, rounded once, so it agrees with FFTW to a few
+ *   ulp, not bit for bit.
+ * Outputs (host): h_nk1[K^3], h_k1[K^3 dim_x], h_v[K^3 dim_x] complex,
+ * h_nq1[K], h_q1[K dim_x], h_e[K dim_x] complex; kernel_ms may be NULL. */
+int hd_ofl_wannier(int32_t dim_x, int32_t dim_y, double offset_x, const uint16_t* h_ktable, const double* h_vtable,
+                   const double* h_eps, const double* h_u, double tol, uint8_t* h_nk1, uint16_t* h_k1,
+                   double* h_v, uint8_t* h_nq1, uint16_t* h_q1, double* h_e, double* kernel_ms);
 
 /* ------------------------------------------------------------------- build */
 

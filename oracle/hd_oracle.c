@@ -513,6 +513,132 @@ int64_t oc_build(int N, int Norb, const uint64_t* basis, int64_t dim,
     return nnz;
 }
 
+/* ------------------------------------------------- general (hash) tables */
+
+/* The tables of TermHashTables (term_hash_tables_base.hpp:44-276) as the lists
+ * that QuarticTermHashTables::GetK1 / QuadraticTermHashTables::GetK1 return
+ * (term_hash_tables.cpp:51-120), with GetVkkkk / GetEkk per list slot. */
+typedef struct {
+    int K, cplx, max_k1, max_q1;
+    const uint8_t* nk1;  /* [(k4*K + k3)*K + k2] */
+    const uint16_t* k1;
+    const double* v;
+    const uint8_t* nq1;  /* NULL: no quadratic terms */
+    const uint16_t* q1;
+    const double* e;
+} oc_general;
+
+static double complex oc_gval(const double* a, int64_t i, int cplx)
+{
+    return cplx ? (a[2 * i] + I * a[2 * i + 1]) : a[i];
+}
+
+/* One row of Add_CdC_Terms (spinless_fermion_hamiltonian.hpp:172-228) and
+ * Add_CdCdCC_Terms (:316-398) with k1 lists: every term goes into the row's
+ * map in loop order; an out state missing from the basis is skipped. */
+static int oc_build_row_general(oc_rowacc* acc, int Norb, const uint64_t* basis, int64_t dim, int64_t row,
+                                const oc_general* G, double tol, int full)
+{
+    const uint64_t in = basis[row];
+    const uint64_t highest = 1ull << (Norb - 1);
+    if (G->nq1) {
+        uint64_t t2 = in;
+        while (t2) {
+            uint64_t b2 = t2 & (0 - t2);
+            int k2 = oc_popcount(b2 - 1);
+            for (int j = 0; j < G->nq1[k2]; ++j) {
+                int64_t slot = (int64_t)k2 * G->max_q1 + j;
+                int k1 = G->q1[slot];
+                double complex el = 0.0;
+                int64_t idx = row;
+                if (k1 == k2) {
+                    el = oc_gval(G->e, slot, G->cplx);
+                } else {
+                    uint64_t b1 = 1ull << k1;
+                    if (b1 & ~in) {
+                        int sign;
+                        uint64_t out = oc_cdc(&sign, b1, b2, in, highest);
+                        if (sign != 0 && (full || out >= in)) {
+                            el = oc_gval(G->e, slot, G->cplx) * (double)sign;
+                            idx = oc_find_index(basis, dim, out);
+                        }
+                    }
+                }
+                if (idx >= 0 && cabs(el) > tol && acc_add(acc, idx, el)) return -1;
+            }
+            t2 ^= b2;
+        }
+    }
+    const uint64_t mask = Norb >= 64 ? ~0ull : (1ull << Norb) - 1;
+    uint64_t t4 = in;
+    while (t4) {
+        uint64_t b4 = t4 & (0 - t4);
+        int k4 = oc_popcount(b4 - 1);
+        uint64_t t3 = in ^ b4;
+        while (t3) {
+            uint64_t b3 = t3 & (0 - t3);
+            int k3 = oc_popcount(b3 - 1);
+            if (k3 != k4) {
+                uint64_t t2 = mask ^ (t3 ^ b3);
+                while (t2) {
+                    uint64_t b2 = t2 & (0 - t2);
+                    int k2 = oc_popcount(b2 - 1);
+                    int64_t ent = ((int64_t)k4 * G->K + k3) * G->K + k2;
+                    for (int j = 0; j < G->nk1[ent]; ++j) {
+                        int64_t slot = ent * G->max_k1 + j;
+                        uint64_t b1 = 1ull << G->k1[slot];
+                        if ((!(b1 & in) || ((b1 & b3) || (b1 & b4))) &&
+                            (!(b2 & in) || ((b2 & b3) || (b2 & b4))) && !(b1 & b2)) {
+                            int sign;
+                            uint64_t out = oc_cdcdcc(&sign, b1, b2, b3, b4, in, highest);
+                            if (sign != 0 && (full || out >= in)) {
+                                int64_t idx = oc_find_index(basis, dim, out);
+                                if (idx >= 0) {
+                                    double complex el = (double)sign * oc_gval(G->v, slot, G->cplx);
+                                    if (cabs(el) > tol && acc_add(acc, idx, el)) return -1;
+                                }
+                            }
+                        }
+                    }
+                    t2 ^= b2;
+                }
+            }
+            t3 ^= b3;
+        }
+        t4 ^= b4;
+    }
+    return 0;
+}
+
+int64_t oc_build_general(int Norb, const uint64_t* basis, int64_t dim, int64_t row0, int64_t row1, int K,
+                         int is_complex, int max_k1, const uint8_t* nk1, const uint16_t* k1, const double* v,
+                         int max_q1, const uint8_t* nq1, const uint16_t* q1, const double* e, double tol,
+                         int full, int64_t* rowptr, int32_t* col, double* val, int64_t cap, int64_t* n_trimmed)
+{
+    if (K < Norb || K > 64 || row0 < 0 || row1 > dim || row0 > row1) return -1;
+    oc_general G = {K, is_complex, max_k1, max_q1, nk1, k1, v, nq1, q1, e};
+    oc_rowacc acc;
+    acc_init(&acc);
+    oc_entry* scratch = (oc_entry*)malloc(sizeof(oc_entry) * acc.cap);
+    int64_t nnz = 0;
+    if (n_trimmed) *n_trimmed = 0;
+    if (rowptr) rowptr[0] = 0;
+    for (int64_t r = row0; r < row1; ++r) {
+        acc_clear(&acc);
+        if (oc_build_row_general(&acc, Norb, basis, dim, r, &G, tol, full)) {
+            nnz = -1;
+            break;
+        }
+        int64_t k = oc_emit_row(&acc, scratch, is_complex, tol, rowptr ? col : NULL, val, nnz, cap, n_trimmed);
+        if (k < 0) { nnz = -1; break; }
+        nnz += k;
+        if (rowptr) rowptr[r - row0 + 1] = nnz;
+    }
+    free(scratch);
+    acc_free(&acc);
+    return nnz;
+}
+
 /* ------------------------------------------------------- multithreaded build */
 
 typedef struct {
@@ -1006,4 +1132,49 @@ void oc_ofl_vtable(int dimX, int dimY, int cutX, int cutY, const double* bloch_,
             }
     free(arr);
     free(conv);
+}
+
+/* ------------------------------------------------- hybrid Wannier tables */
+
+/* ChangeToWannierBasis quartic terms (interacting_ofl_model.cpp:820-905) for
+ * the list slots [s0, s1) of entry (k4*K + k3)*K + k2, slot x1: k1 = x1*dimY +
+ * ky1, ky1 = FindK1 along y (:410-432); V' = sum over k4x, k3x, k2x (k2x
+ * fastest) of V(q1..q4) * conj(U1 * U2) * U3 * U4 from 0, products left to
+ * right; V' is kept where |V'| > tol, else 0 (no hash entry). */
+void oc_ofl_wannier(int dimX, int dimY, const uint16_t* kt, const double* vt_, const double* u_, double tol,
+                    int64_t s0, int64_t s1, uint16_t* k1_out, double* v_out)
+{
+    const int K = dimX * dimY;
+    const oc_c* vt = (const oc_c*)vt_;
+    const oc_c* u = (const oc_c*)u_;
+    for (int64_t idx = s0; idx < s1; ++idx) {
+        const int64_t e = idx / dimX;
+        const int x1 = (int)(idx - e * dimX);
+        const int k2 = (int)(e % K), k3 = (int)((e / K) % K), k4 = (int)(e / ((int64_t)K * K));
+        const int k2y = k2 % dimY, k3y = k3 % dimY, k4y = k4 % dimY;
+        const int k1y = oc_mod(k3y + k4y - k2y, dimY);
+        const int k1 = x1 * dimY + k1y;
+        oc_c sum = {0.0, 0.0};
+        for (int k4x = 0; k4x < dimX; ++k4x)
+            for (int k3x = 0; k3x < dimX; ++k3x)
+                for (int k2x = 0; k2x < dimX; ++k2x) {
+                    const int q2 = k2x * dimY + k2y, q3 = k3x * dimY + k3y, q4 = k4x * dimY + k4y;
+                    int t1 = q4, t2 = q3;
+                    if (q4 > q3) { t1 = q3; t2 = q4; }
+                    const int64_t row = ((int64_t)t1 * K + t2 - (int64_t)t1 * (t1 + 1) / 2) * K;
+                    const int q1 = kt[row + q2];
+                    const oc_c V = vt[row + (q4 > q3 ? q1 : q2)];
+                    oc_c p = c_mul(u[(int64_t)k1 * K + q1], u[(int64_t)k2 * K + q2]);
+                    p.im = -p.im;
+                    oc_c t = c_mul(V, p);
+                    t = c_mul(t, u[(int64_t)k3 * K + q3]);
+                    t = c_mul(t, u[(int64_t)k4 * K + q4]);
+                    sum.re = sum.re + t.re;
+                    sum.im = sum.im + t.im;
+                }
+        k1_out[idx - s0] = (uint16_t)k1;
+        const int keep = hypot(sum.re, sum.im) > tol;
+        v_out[2 * (idx - s0)] = keep ? sum.re : 0.0;
+        v_out[2 * (idx - s0) + 1] = keep ? sum.im : 0.0;
+    }
 }

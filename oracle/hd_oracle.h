@@ -72,6 +72,15 @@ int64_t oc_build(int N, int Norb, const uint64_t* basis, int64_t dim,
                  int64_t* rowptr, int32_t* col, double* val, int64_t cap,
                  int64_t* n_trimmed);
 
+/* The same build with general (hash) term tables: k1 lists per (k2,k3,k4) at
+ * entry (k4*K + k3)*K + k2 (nk1 lengths, k1 labels, v values, max_k1 slots per
+ * entry) and per k2 for the quadratic terms (nq1 == NULL: none).  Arguments and
+ * return as oc_build. */
+int64_t oc_build_general(int Norb, const uint64_t* basis, int64_t dim, int64_t row0, int64_t row1, int K,
+                         int is_complex, int max_k1, const uint8_t* nk1, const uint16_t* k1, const double* v,
+                         int max_q1, const uint8_t* nq1, const uint16_t* q1, const double* e, double tol,
+                         int full, int64_t* rowptr, int32_t* col, double* val, int64_t cap, int64_t* n_trimmed);
+
 /* y = U x + (U - D)^H x for the reference's upper-triangle CRS (naive loop). */
 void oc_spmv_sym(int64_t dim, const int64_t* rowptr, const int32_t* col,
                  const double* val, int is_complex, const double* x, double* y);
@@ -108,6 +117,12 @@ void oc_conv2d(const double* right, const double* left, int X, int Y, double* ou
                int bottom, int rightOff);
 void oc_ofl_vtable(int dimX, int dimY, int cutX, int cutY, const double* bloch, int64_t e0, int64_t e1,
                    double* out);
+
+/* Hybrid Wannier quartic terms (ChangeToWannierBasis): see hd_oracle.c.
+ * kt / vt: folded Bloch-basis tables; u: K x K complex change of basis; slots
+ * [s0, s1) of the (k4*K + k3)*K + k2, x1 enumeration. */
+void oc_ofl_wannier(int dimX, int dimY, const uint16_t* kt, const double* vt, const double* u, double tol,
+                    int64_t s0, int64_t s1, uint16_t* k1_out, double* v_out);
 
 #ifdef __cplusplus
 }

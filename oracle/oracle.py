@@ -102,6 +102,11 @@ def lib():
         L.oc_test_lz2.argtypes = [u64, u64, i32, i32]
         L.oc_two_level_basis.restype = i64
         L.oc_two_level_basis.argtypes = [i32, i32, i32, i32, i32, i32, P, P, i64]
+        L.oc_build_general.restype = i64
+        L.oc_build_general.argtypes = [i32, P, i64, i64, i64, i32, i32, i32, P, P, P, i32, P, P, P, dbl, i32,
+                                       P, P, P, i64, ctypes.POINTER(i64)]
+        L.oc_ofl_wannier.restype = None
+        L.oc_ofl_wannier.argtypes = [i32, i32, P, P, P, dbl, i64, i64, P, P]
         L.oc_two_level_build.restype = i64
         L.oc_two_level_build.argtypes = [i32, i32, P, P, i64, i64, i64, P, P, P, P, P, P, dbl, i32,
                                          P, P, P, i64, ctypes.POINTER(i64)]
@@ -204,6 +209,57 @@ def build(N, Norb, bas, layout, K, ktab, vtab, eps=None, is_complex=False,
                       ctypes.byref(ntr))
     assert nnz2 == nnz
     return CRS(rp, col[:nnz], val[:nnz * (2 if is_complex else 1)], is_complex, ntr.value)
+
+
+def build_general(Norb, bas, K, nk1, k1, v, nq1=None, q1=None, e=None, tol=ZERO_TOL, full=False,
+                  row0=0, row1=None):
+    """The build with general (hash) term tables, in the list format of
+    TermTables.hash: Add_CdC_Terms + Add_CdCdCC_Terms with k1 lists, map
+    accumulation in term order, Trim, CRS for rows [row0, row1)."""
+    L = lib()
+    dim = len(bas)
+    row1 = dim if row1 is None else row1
+    v = np.asarray(v)
+    cplx = np.iscomplexobj(v) or (e is not None and np.iscomplexobj(e))
+    vdt = np.complex128 if cplx else np.float64
+    n = K ** 3
+    nk = np.ascontiguousarray(nk1, dtype=np.uint8).reshape(n)
+    kk = np.ascontiguousarray(k1, dtype=np.uint16).reshape(n, -1)
+    vv = np.ascontiguousarray(v, dtype=vdt).reshape(n, -1).view(np.float64)
+    qn = qq = ee = None
+    mq = 0
+    if nq1 is not None:
+        qn = np.ascontiguousarray(nq1, dtype=np.uint8).reshape(K)
+        qq = np.ascontiguousarray(q1, dtype=np.uint16).reshape(K, -1)
+        ee = np.ascontiguousarray(e, dtype=vdt).reshape(K, -1).view(np.float64)
+        mq = qq.shape[1]
+    bas = np.ascontiguousarray(bas, dtype=np.uint64)
+    ntr = ctypes.c_int64(0)
+    args = (Norb, _p(bas), dim, row0, row1, K, int(cplx), kk.shape[1], _p(nk), _p(kk), _p(vv), mq,
+            _p(qn), _p(qq), _p(ee), tol, int(full))
+    nnz = L.oc_build_general(*args, None, None, None, 0, ctypes.byref(ntr))
+    if nnz < 0:
+        raise RuntimeError("oracle general build failed")
+    rp = np.zeros(row1 - row0 + 1, dtype=np.int64)
+    col = np.zeros(max(nnz, 1), dtype=np.int32)
+    val = np.zeros(max(nnz, 1) * (2 if cplx else 1), dtype=np.float64)
+    nnz2 = L.oc_build_general(*args, _p(rp), _p(col), _p(val), nnz, ctypes.byref(ntr))
+    assert nnz2 == nnz
+    return CRS(rp, col[:nnz], val[:nnz * (2 if cplx else 1)], cplx, ntr.value)
+
+
+def ofl_wannier_quartic(dim_x, dim_y, ktable, vtable, u, tol, s0=0, s1=None):
+    """ChangeToWannierBasis quartic terms for list slots [s0, s1) of the
+    (k4*K + k3)*K + k2, x1 enumeration: (k1 labels, complex values)."""
+    K = dim_x * dim_y
+    s1 = K ** 3 * dim_x if s1 is None else s1
+    kt = np.ascontiguousarray(ktable, dtype=np.uint16)
+    vt = np.ascontiguousarray(vtable, dtype=np.complex128)
+    uu = np.ascontiguousarray(u, dtype=np.complex128)
+    k1 = np.zeros(s1 - s0, dtype=np.uint16)
+    v = np.zeros(s1 - s0, dtype=np.complex128)
+    lib().oc_ofl_wannier(dim_x, dim_y, _p(kt), _p(vt), _p(uu), tol, s0, s1, _p(k1), _p(v))
+    return k1, v
 
 
 # ------------------------------------------------------------------ two levels
@@ -409,6 +465,22 @@ def jw_operators(norb):
             M = np.kron(M, m)
         ops.append(M)
     return ops
+
+
+def dense_hamiltonian_jw_general(norb, states, quartic, quadratic=None):
+    """<out|H|in> restricted to `states` for H = sum V c+_k1 c+_k2 c_k3 c_k4
+    (quartic: iterable of ((k1,k2,k3,k4), V), repeats added) + sum E c+_k1 c_k2
+    (quadratic: iterable of ((k1,k2), E))."""
+    ops = jw_operators(norb)
+    terms = list(quartic) + list(quadratic or [])
+    dt = np.complex128 if any(np.iscomplexobj(np.asarray(v)) for _, v in terms) else np.float64
+    H = np.zeros((2 ** norb, 2 ** norb), dtype=dt)
+    for (k1, k2, k3, k4), v in quartic:
+        H += v * (ops[k1].T @ ops[k2].T @ ops[k3] @ ops[k4])
+    for (k1, k2), e in quadratic or []:
+        H += e * (ops[k1].T @ ops[k2])
+    idx = np.asarray(states, dtype=np.int64)
+    return H[np.ix_(idx, idx)]
 
 
 def dense_hamiltonian_jw(norb, states, quartic, eps=None):
