@@ -28,6 +28,13 @@
 //          --store-terms 1 / --retrieve-terms 1   term tables in OUT as
 //              quadratic|quartic_coefficient_table_L_<NORB>.dat (sphere) or
 //              ..._kx_<X>_ky_<Y>.dat (--layout ofl with --dim-x X --dim-y Y)
+//          --use-hash 1   (--layout ofl) the tables as hash tables (ConvertTableFormat,
+//              interacting_ofl_model.cpp:597-619), built by the general device build;
+//              stored and retrieved in the hash-table file format
+//          --basis 1 --kx-shift S --tol-vkkkk T   (--layout ofl) the hybrid Wannier basis
+//              (ChangeToWannierBasis, interacting_ofl_model.cpp:728-923) with the change of
+//              basis from the --bloch vectors (GenerateWannierCoefficients) or from
+//              --wannier-matrix FILE (kmax^2 raw complex values); use --sector k1d:ky,dim_y
 // Prints one JSON line with the dimension, the CRS size, the eigenvalues and
 // the solver statistics.
 #include "hd_dropin.hpp"
@@ -522,7 +529,66 @@ int main(int argc, char** argv)
         OflQuarticTermTables quart;
         DiagonalQuadraticTermTables<dcmplx> quad;
         SpinlessFermionHamiltonian<dcmplx> ham;
-        return finish(ham, quart, quad, tables(quart, quad, vt, eps));
+        const bool wannier = args.count("basis") && args["basis"] == "1";
+        if (!Flag(args, "use-hash") && !wannier) return finish(ham, quart, quad, tables(quart, quad, vt, eps));
+        // BuildTermTables (interacting_ofl_model.cpp:243-253): the Wannier
+        // basis changes the tables into hash tables (ChangeToWannierBasis),
+        // --use-hash converts the array tables (ConvertTableFormat); with
+        // --retrieve-terms the hash tables come from their files (TermsFromFile
+        // :656-680)
+        QuarticTermHashTables hashQuart;
+        QuadraticTermHashTables hashQuad;
+        int haveQuad = 1;
+        if (retrieve) {
+            hashQuad.FromFile(TermsFileName(args, "quadratic", kmax), format, mpi);
+            hashQuart.FromFile(TermsFileName(args, "quartic", kmax), format, mpi);
+        } else {
+            haveQuad = tables(quart, quad, vt, eps);
+            if (haveQuad < 0) return 2;
+            if (wannier) {
+                if (!args.count("dim-x") || !args.count("dim-y")) {
+                    std::fprintf(stderr, "--basis 1 needs --dim-x and --dim-y\n");
+                    return 2;
+                }
+                const int dx = std::stoi(args["dim-x"]), dy = std::stoi(args["dim-y"]);
+                const double shift = args.count("kx-shift") ? std::stod(args["kx-shift"]) : 0.0;
+                const double tolV = args.count("tol-vkkkk") ? std::stod(args["tol-vkkkk"]) : 1e-12;
+                std::vector<dcmplx> U((size_t)kmax * kmax);
+                if (args.count("wannier-matrix")) {
+                    std::vector<dcmplx> m;
+                    if (!ReadRaw(args["wannier-matrix"], m) || m.size() != U.size()) {
+                        std::fprintf(stderr, "--wannier-matrix needs kmax^2 complex values\n");
+                        return 2;
+                    }
+                    U = m;
+                } else if (args.count("bloch")) {
+                    // NonInteractingOflModelGrid::GenerateWannierCoefficients from the
+                    // band-0 Bloch vectors of the --bloch file
+                    std::vector<dcmplx> bloch;
+                    ReadRaw(args["bloch"], bloch);
+                    GenerateWannierCoefficients((iSize_t)dx, (iSize_t)dy, shift, bloch.data(),
+                                                (iSize_t)(bloch.size() / kmax), U.data());
+                } else {
+                    std::fprintf(stderr, "--basis 1 needs --bloch or --wannier-matrix\n");
+                    return 2;
+                }
+                if (!ChangeToWannierBasis(dx, dy, shift, quart, haveQuad > 0 ? &quad : nullptr, U.data(), tolV,
+                                          hashQuart, hashQuad))
+                    return 1;
+            } else {
+                hashQuart.Initialize((kState_t)kmax);
+                hashQuart.SetQuarticFromArray(&quart);
+                if (haveQuad > 0) {
+                    hashQuad.Initialize((kState_t)kmax);
+                    hashQuad.SetQuadraticFromArray(&quad);
+                }
+            }
+        }
+        if (store) {
+            hashQuad.ToFile(TermsFileName(args, "quadratic", kmax), format, mpi);
+            hashQuart.ToFile(TermsFileName(args, "quartic", kmax), format, mpi);
+        }
+        return Run<dcmplx>(ham, &hashQuad, &hashQuart, haveQuad > 0, args, mpi, basis);
     }
     std::vector<double> vt, eps;
     DiagonalQuadraticTermTables<double> quad;

@@ -50,6 +50,7 @@
 #include <functional>
 #include <iomanip>
 #include <iostream>
+#include <map>
 #include <memory>
 #include <sstream>
 #include <string>
@@ -167,6 +168,23 @@ namespace utilities
         struct HamDeleter { void operator()(hd_hamiltonian* h) const { hd_ham_destroy(h); } };
         template <typename T> struct IsComplex : std::false_type {};
         template <> struct IsComplex<dcmplx> : std::true_type {};
+        // array term tables expose their k vector (term_tables_base.hpp:105-115)
+        template <class L, class = void> struct IsArrayQuartic : std::false_type {};
+        template <class L>
+        struct IsArrayQuartic<L, typename std::enable_if<std::is_same<
+                                     decltype(std::declval<L&>().GetKTable()), std::vector<kState_t>*>::value>::type>
+            : std::true_type {};
+        template <class L, class = void> struct HasDimension : std::false_type {};
+        template <class L>
+        struct HasDimension<L, decltype((void)std::declval<const L&>().GetDimension())> : std::true_type {};
+        // entries of a quadratic table: its dimension for array tables, else
+        // the orbital count
+        template <class L>
+        iSize_t TableKMax(const L* t, const iSize_t nbrOrbitals)
+        {
+            if constexpr (HasDimension<L>::value) return t->GetDimension();
+            else return nbrOrbitals;
+        }
     }
 
     // CrsSparseMatrix<T> (src/utilities/data_structures/crs_sparse_matrix.hpp:123-816):
@@ -1184,6 +1202,7 @@ namespace diagonalization
             this->m_data.SetDataDimensions(this->m_fockBasis.GetStoredDimension(), mpi.m_nbrProcs, mpi);
             m_quadratic.clear();
             m_haveQuadratic = false;
+            m_quadDiagonal = true;
         }
 
         void Clear()
@@ -1196,75 +1215,71 @@ namespace diagonalization
             this->m_eigenvectors.clear();
             m_quadratic.clear();
             m_haveQuadratic = false;
+            m_quadDiagonal = true;
         }
 
-        // spinless_fermion_hamiltonian.hpp:143-257.  Quadratic terms are diagonal
-        // on both reference models (GetK1(k2) returns k1 = k2); their E_k enter the
-        // device build together with the quartic terms, quadratic terms first as
-        // in the reference.
+        // spinless_fermion_hamiltonian.hpp:143-257.  The k1 lists of every k2
+        // (GetK1) and their E (GetEkk) are recorded and enter the device build
+        // together with the quartic terms, quadratic terms first as in the
+        // reference.  Diagonal-only tables (k1 = k2, both reference array
+        // models) keep the pair-structure build; lists with hops (the Wannier
+        // basis, any hash table) take the general build.
         template <class L>
         void Add_CdC_Terms(L* termTables, const fock_t minRow, const fock_t maxRow, utilities::MpiWrapper& mpi)
         {
             if (!CheckRange(minRow, maxRow, "Add_CdC_Terms", mpi)) return;
-            const iSize_t kMax = termTables->GetDimension();
-            m_quadratic.assign(kMax * (utilities::detail::IsComplex<T>::value ? 2 : 1), 0.0);
+            const iSize_t kMax = utilities::detail::TableKMax(termTables, this->m_data.m_nbrOrbitals);
+            const iSize_t w = utilities::detail::IsComplex<T>::value ? 2 : 1;
+            const iSize_t maxK = std::max<iSize_t>(termTables->GetMaxKCount(), 1);
+            std::vector<kState_t> buf(maxK);
+            m_quadratic.assign(kMax * w, 0.0);
+            m_nq1.assign(kMax, 0);
+            m_q1.assign((size_t)kMax * maxK, 0);
+            m_e.assign((size_t)kMax * maxK * w, 0.0);
+            m_maxQ1 = maxK;
+            m_quadDiagonal = true;
             for (iSize_t k = 0; k < kMax; ++k) {
-                kState_t buf[8];
                 iSize_t n = 0;
-                termTables->GetK1(buf, n, (kState_t)k);
-                if (n > 1 || (n == 1 && buf[0] != k)) {
-                    std::cerr << "\n\tERROR in Add_CdC_Terms: only diagonal quadratic terms (k1 == k2) are "
-                                 "supported on the device" << std::endl;
-                    mpi.m_exitFlag = true;
-                    return;
+                termTables->GetK1(buf.data(), n, (kState_t)k);
+                m_nq1[k] = (uint8_t)n;
+                for (iSize_t j = 0; j < n; ++j) {
+                    const T e = termTables->GetEkk(buf[j], (kState_t)k);
+                    m_q1[(size_t)k * maxK + j] = buf[j];
+                    std::memcpy(&m_e[((size_t)k * maxK + j) * w], &e, sizeof(T));
+                    if (buf[j] != k) m_quadDiagonal = false;
                 }
+                if (n > 1) m_quadDiagonal = false;
                 const T e = n ? termTables->GetEkk((kState_t)k, (kState_t)k) : T(0.0);
-                std::memcpy(&m_quadratic[k * (utilities::detail::IsComplex<T>::value ? 2 : 1)], &e, sizeof(T));
+                std::memcpy(&m_quadratic[k * w], &e, sizeof(T));
             }
             m_haveQuadratic = true;
         }
 
         // spinless_fermion_hamiltonian.hpp:281-436 plus the map -> CRS conversion
-        // (mapped_sparse_matrix.hpp:423-462, crs_sparse_matrix.hpp:193-254)
+        // (mapped_sparse_matrix.hpp:423-462, crs_sparse_matrix.hpp:193-254).
+        // Array tables (GetKTable() gives the k vector) with diagonal quadratic
+        // terms use the pair-structure build; any other table class is read
+        // through GetK1 / GetVkkkk into k1 lists for the general build
+        // (QuarticTermHashTables, term_hash_tables.cpp:96-130).
         template <class L>
         void Add_CdCdCC_Terms(L* termTables, const fock_t minRow, const fock_t maxRow, utilities::MpiWrapper& mpi)
         {
             if (!CheckRange(minRow, maxRow, "Add_CdCdCC_Terms", mpi)) return;
-            std::vector<kState_t>* kt = termTables->GetKTable();
-            std::vector<T>* vt = termTables->GetVTable();
-            const size_t n = vt->size();
-            // the table's layout follows from its size: kMax^3 (sphere,
-            // term_tables.cpp:36-44) or (kMax+1) kMax^2 / 2 (flux lattice fold)
-            int layout = -1;
-            uint32_t kMax = 0;
-            for (uint32_t K = 1; K <= 64 && layout < 0; ++K) {
-                if ((size_t)K * K * K == n && kt->size() == n) { layout = HD_LAYOUT_SPHERE; kMax = K; }
-                else if ((size_t)(K + 1) * K * K / 2 == n && kt->size() == n) { layout = HD_LAYOUT_OFL; kMax = K; }
+            std::shared_ptr<hd_tables> tables;
+            if constexpr (utilities::detail::IsArrayQuartic<L>::value) {
+                if (!m_haveQuadratic || m_quadDiagonal) {
+                    tables = ArrayTables(termTables, mpi);
+                    if (!tables) return;
+                }
             }
-            if (layout < 0) {
-                std::cerr << "\n\tERROR in Add_CdCdCC_Terms: table size " << n
-                          << " matches neither the sphere nor the flux-lattice layout" << std::endl;
-                mpi.m_exitFlag = true;
-                return;
+            if (!tables) {
+                tables = ListTables(termTables, mpi);
+                if (!tables) return;
             }
-            if (m_haveQuadratic && m_quadratic.size() != kMax * (utilities::detail::IsComplex<T>::value ? 2u : 1u)) {
-                std::cerr << "\n\tERROR in Add_CdCdCC_Terms: quadratic and quartic tables disagree on kMax" << std::endl;
-                mpi.m_exitFlag = true;
-                return;
-            }
-            hd_tables* rawT = nullptr;
-            if (!utilities::detail::Ok(hd_tables_create(layout, kMax, kt->data(),
-                                                        reinterpret_cast<const double*>(vt->data()),
-                                                        utilities::detail::IsComplex<T>::value ? 1 : 0,
-                                                        m_haveQuadratic ? m_quadratic.data() : nullptr, &rawT),
-                                       "Add_CdCdCC_Terms")) {
-                mpi.m_exitFlag = true;
-                return;
-            }
-            std::shared_ptr<hd_tables> tables(rawT, utilities::detail::TablesDeleter());
             hd_hamiltonian* rawH = nullptr;
+            const int mask = tables_is_hash_ ? (m_valueMask & 3) | 1 : m_valueMask;
             if (!utilities::detail::Ok(hd_ham_build(this->m_fockBasis.Device().get(), tables.get(), HD_STORAGE_FULL,
-                                                    _SPARSE_ZERO_TOL_, m_valueMask, &rawH),
+                                                    _SPARSE_ZERO_TOL_, mask, &rawH),
                                        "Add_CdCdCC_Terms")) {
                 mpi.m_exitFlag = true;
                 return;
@@ -1297,8 +1312,93 @@ namespace diagonalization
             return true;
         }
 
+        template <class L>
+        std::shared_ptr<hd_tables> ArrayTables(L* termTables, utilities::MpiWrapper& mpi)
+        {
+            tables_is_hash_ = false;
+            std::vector<kState_t>* kt = termTables->GetKTable();
+            auto* vt = termTables->GetVTable();
+            const size_t n = vt->size();
+            // the table's layout follows from its size: kMax^3 (sphere,
+            // term_tables.cpp:36-44) or (kMax+1) kMax^2 / 2 (flux lattice fold)
+            int layout = -1;
+            uint32_t kMax = 0;
+            for (uint32_t K = 1; K <= 64 && layout < 0; ++K) {
+                if ((size_t)K * K * K == n && kt->size() == n) { layout = HD_LAYOUT_SPHERE; kMax = K; }
+                else if ((size_t)(K + 1) * K * K / 2 == n && kt->size() == n) { layout = HD_LAYOUT_OFL; kMax = K; }
+            }
+            if (layout < 0) {
+                std::cerr << "\n\tERROR in Add_CdCdCC_Terms: table size " << n
+                          << " matches neither the sphere nor the flux-lattice layout" << std::endl;
+                mpi.m_exitFlag = true;
+                return nullptr;
+            }
+            if (m_haveQuadratic && m_quadratic.size() != kMax * (utilities::detail::IsComplex<T>::value ? 2u : 1u)) {
+                std::cerr << "\n\tERROR in Add_CdCdCC_Terms: quadratic and quartic tables disagree on kMax" << std::endl;
+                mpi.m_exitFlag = true;
+                return nullptr;
+            }
+            hd_tables* rawT = nullptr;
+            if (!utilities::detail::Ok(hd_tables_create(layout, kMax, kt->data(), reinterpret_cast<const double*>(vt->data()),
+                                                        utilities::detail::IsComplex<T>::value ? 1 : 0,
+                                                        m_haveQuadratic ? m_quadratic.data() : nullptr, &rawT),
+                                       "Add_CdCdCC_Terms")) {
+                mpi.m_exitFlag = true;
+                return nullptr;
+            }
+            return std::shared_ptr<hd_tables>(rawT, utilities::detail::TablesDeleter());
+        }
+
+        // every (k2,k3,k4) of the table through GetK1 / GetVkkkk
+        template <class L>
+        std::shared_ptr<hd_tables> ListTables(L* termTables, utilities::MpiWrapper& mpi)
+        {
+            tables_is_hash_ = true;
+            const size_t K = this->m_data.m_nbrOrbitals;
+            const size_t w = utilities::detail::IsComplex<T>::value ? 2 : 1;
+            const iSize_t maxK = std::max<iSize_t>(termTables->GetMaxKCount(), 1);
+            if (maxK > 255 || (m_haveQuadratic && m_maxQ1 > 255)) {
+                std::cerr << "\n\tERROR in Add_CdCdCC_Terms: more than 255 k1 per key" << std::endl;
+                mpi.m_exitFlag = true;
+                return nullptr;
+            }
+            std::vector<kState_t> buf(maxK);
+            std::vector<uint8_t> nk1(K * K * K, 0);
+            std::vector<uint16_t> k1(K * K * K * maxK, 0);
+            std::vector<double> v(K * K * K * maxK * w, 0.0);
+            for (size_t k4 = 0; k4 < K; ++k4)
+                for (size_t k3 = 0; k3 < K; ++k3)
+                    for (size_t k2 = 0; k2 < K; ++k2) {
+                        const size_t e = (k4 * K + k3) * K + k2;
+                        iSize_t n = 0;
+                        termTables->GetK1(buf.data(), n, (kState_t)k2, (kState_t)k3, (kState_t)k4);
+                        nk1[e] = (uint8_t)n;
+                        for (iSize_t j = 0; j < n; ++j) {
+                            k1[e * maxK + j] = buf[j];
+                            const T val = termTables->GetVkkkk(buf[j], (kState_t)k2, (kState_t)k3, (kState_t)k4);
+                            std::memcpy(&v[(e * maxK + j) * w], &val, sizeof(T));
+                        }
+                    }
+            hd_tables* rawT = nullptr;
+            if (!utilities::detail::Ok(
+                    hd_tables_create_hash((uint32_t)K, utilities::detail::IsComplex<T>::value ? 1 : 0, (int32_t)maxK,
+                                          nk1.data(), k1.data(), v.data(), (int32_t)m_maxQ1,
+                                          m_haveQuadratic ? m_nq1.data() : nullptr, m_q1.data(), m_e.data(), &rawT),
+                    "Add_CdCdCC_Terms")) {
+                mpi.m_exitFlag = true;
+                return nullptr;
+            }
+            return std::shared_ptr<hd_tables>(rawT, utilities::detail::TablesDeleter());
+        }
+
         std::vector<double> m_quadratic;
+        std::vector<uint8_t> m_nq1;
+        std::vector<uint16_t> m_q1;
+        std::vector<double> m_e;
+        iSize_t m_maxQ1 = 1;
+        bool m_quadDiagonal = true;
         bool m_haveQuadratic = false;
+        bool tables_is_hash_ = false;
         int m_valueMask = 1 | 2;
     };
 
@@ -1979,6 +2079,358 @@ namespace diagonalization
         protected:
         iSize_t CalculateDim(const kState_t kMax) const override { return kMax; }
     };
+
+    // TermHashTables (src/hamiltonians/term_hash_tables_base.hpp:44-276): a
+    // multimap of k1 lists keyed by (k2,k3,k4) or k2 and a map of values keyed
+    // by the labels, with the keys of utilities::Key (Pack4x16: k1<<48 | k2<<32
+    // | k3<<16 | k4; Pack2x32: k1<<32 | k2; serialize.hpp:96-130).  The order
+    // of a k1 list does not change the Hamiltonian (each k1 of one list gives
+    // a different out state), so ordered maps stand in for the reference's
+    // hash maps and files are written in ascending key order.
+    class TermHashTablesBase
+    {
+        public:
+        TermHashTablesBase() : m_fileHeader("# File contains term table data in hash table format") {}
+        virtual ~TermHashTablesBase() = default;
+
+        void Initialize(const kState_t kMax) { m_kMax = kMax; }
+        void Clear()
+        {
+            m_k1.clear();
+            m_v.clear();
+        }
+        kState_t GetKMax() const { return m_kMax; }
+        // MultiHashMultiMap::GetMaxCount (multi_key_hash.hpp:942-952)
+        iSize_t GetMaxKCount() const
+        {
+            iSize_t m = 1;
+            for (const auto& kv : m_k1) m = std::max<iSize_t>(m, (iSize_t)kv.second.size());
+            return m;
+        }
+        // GetKTable()->Insert(k1, key) and GetVTable()->Insert(key) = value
+        void InsertK1(const kState_t k1, const uint64_t key) { m_k1[key].push_back(k1); }
+        dcmplx& InsertV(const uint64_t key) { return m_v[key]; }
+
+        static uint64_t Key4(const kState_t a, const kState_t b, const kState_t c, const kState_t d)
+        {
+            return ((uint64_t)a << 48) | ((uint64_t)b << 32) | ((uint64_t)c << 16) | (uint64_t)d;
+        }
+        static uint64_t Key2(const uint32_t a, const uint32_t b) { return ((uint64_t)a << 32) | (uint64_t)b; }
+
+        protected:
+        // MultiHashMultiMap::Value (:958-983); a key the table does not hold
+        // gives an empty list here (the reference leaves the count unset)
+        void List(kState_t* buf, iSize_t& n, const uint64_t key) const
+        {
+            n = 0;
+            auto it = m_k1.find(key);
+            if (it == m_k1.end()) return;
+            for (kState_t k : it->second) buf[n++] = k;
+        }
+        // MultiHashMap::Value (:783-797): 0 when absent (m_emptyReturn)
+        dcmplx Value(const uint64_t key) const
+        {
+            auto it = m_v.find(key);
+            return it == m_v.end() ? dcmplx(0.0) : it->second;
+        }
+
+        // TermHashTables::ToFileBase (term_hash_tables_base.hpp:139-175) with
+        // MultiHashBase::ToFileBase (multi_key_hash.hpp:480-566): the header,
+        // then the k table with nbrLabels - 1 key labels and k1 values, then the
+        // value table with nbrLabels labels.  Binary: uint32 size, uint64 keys,
+        // values.  Text: size, label count, then the unpacked keys and the value
+        // per line, tab separated.
+        void ToFileBase(const std::string& fileName, const io::fileFormat_t format, const iSize_t nbrLabels,
+                        utilities::MpiWrapper& mpi) const
+        {
+            std::ofstream f_out;
+            utilities::GenFileStream(f_out, fileName, format, mpi);
+            if (mpi.m_exitFlag) return;
+            if (format == io::_BINARY_) {
+                const iSize_t strSize = (iSize_t)m_fileHeader.size();
+                f_out.write(reinterpret_cast<const char*>(&strSize), sizeof(iSize_t));
+                f_out.write(m_fileHeader.data(), strSize);
+            } else {
+                f_out << m_fileHeader << "\n";
+            }
+            std::vector<uint64_t> kkeys;
+            std::vector<kState_t> kvals;
+            for (const auto& kv : m_k1)
+                for (kState_t k : kv.second) {
+                    kkeys.push_back(kv.first);
+                    kvals.push_back(k);
+                }
+            WriteMap(f_out, format, nbrLabels - 1, kkeys, kvals);
+            std::vector<uint64_t> vkeys;
+            std::vector<dcmplx> vvals;
+            for (const auto& kv : m_v) {
+                vkeys.push_back(kv.first);
+                vvals.push_back(kv.second);
+            }
+            WriteMap(f_out, format, nbrLabels, vkeys, vvals);
+        }
+
+        // TermHashTables::FromFileBase (:180-219) with MultiHashBase::FromFileBase
+        // (:572-669): a header mismatch or a short file sets the exit flag.
+        void FromFileBase(const std::string& fileName, const io::fileFormat_t format, utilities::MpiWrapper& mpi)
+        {
+            std::ifstream f_in;
+            utilities::GenFileStream(f_in, fileName, format, mpi);
+            if (mpi.m_exitFlag) return;
+            std::string header;
+            if (format == io::_BINARY_) {
+                iSize_t strSize = 0;
+                f_in.read(reinterpret_cast<char*>(&strSize), sizeof(iSize_t));
+                if (!f_in || strSize > 4096) strSize = 0;
+                header.resize(strSize);
+                if (strSize) f_in.read(&header[0], strSize);
+            } else {
+                std::getline(f_in, header);
+            }
+            if (header != m_fileHeader) {
+                std::cerr << "\n\tFile read in error - wrong file header " << header << std::endl;
+                mpi.m_exitFlag = true;
+                return;
+            }
+            Clear();
+            std::vector<uint64_t> keys;
+            std::vector<kState_t> kvals;
+            std::vector<dcmplx> vvals;
+            bool ok = ReadMap(f_in, format, keys, kvals);
+            for (size_t i = 0; ok && i < keys.size(); ++i) m_k1[keys[i]].push_back(kvals[i]);
+            ok = ok && ReadMap(f_in, format, keys, vvals);
+            for (size_t i = 0; ok && i < keys.size(); ++i) m_v[keys[i]] = vvals[i];
+            if (!ok) {
+                std::cerr << "\n\tFile read in error - " << fileName << " is truncated" << std::endl;
+                mpi.m_exitFlag = true;
+            }
+        }
+
+        std::map<uint64_t, std::vector<kState_t>> m_k1;
+        std::map<uint64_t, dcmplx> m_v;
+        kState_t m_kMax = 0;
+        std::string m_fileHeader;
+
+        private:
+        static std::string Labels(const uint64_t key, const iSize_t n)
+        {
+            std::stringstream ss;
+            if (n == 1) ss << key;
+            else if (n == 2) ss << (key >> 32) << "\t" << (key & 0xFFFFFFFFull);
+            else if (n == 3) ss << (key >> 48) << "\t" << ((key >> 32) & 0xFFFF) << "\t" << ((key >> 16) & 0xFFFF);
+            else ss << (key >> 48) << "\t" << ((key >> 32) & 0xFFFF) << "\t" << ((key >> 16) & 0xFFFF) << "\t"
+                    << (key & 0xFFFF);
+            return ss.str();
+        }
+        template <typename V>
+        static void WriteMap(std::ofstream& f, const io::fileFormat_t format, const iSize_t n,
+                             const std::vector<uint64_t>& keys, const std::vector<V>& vals)
+        {
+            const unsigned int size = (unsigned int)keys.size();
+            if (format == io::_BINARY_) {
+                f.write(reinterpret_cast<const char*>(&size), sizeof(unsigned int));
+                f.write(reinterpret_cast<const char*>(keys.data()), (std::streamsize)size * sizeof(uint64_t));
+                f.write(reinterpret_cast<const char*>(vals.data()), (std::streamsize)size * sizeof(V));
+            } else {
+                f << size << "\n" << n << "\n";
+                for (unsigned int i = 0; i < size; ++i) f << Labels(keys[i], n) << "\t" << utilities::ToStream(vals[i]) << "\n";
+            }
+        }
+        template <typename V>
+        static bool ReadMap(std::ifstream& f, const io::fileFormat_t format, std::vector<uint64_t>& keys,
+                            std::vector<V>& vals)
+        {
+            unsigned int size = 0;
+            if (format == io::_BINARY_) {
+                f.read(reinterpret_cast<char*>(&size), sizeof(unsigned int));
+                if (!f) return false;
+                keys.resize(size);
+                vals.resize(size);
+                f.read(reinterpret_cast<char*>(keys.data()), (std::streamsize)size * sizeof(uint64_t));
+                f.read(reinterpret_cast<char*>(vals.data()), (std::streamsize)size * sizeof(V));
+                return (bool)f;
+            }
+            uint64_t n = 0;
+            if (!(f >> size >> n)) return false;
+            keys.resize(size);
+            vals.resize(size);
+            for (unsigned int i = 0; i < size; ++i) {
+                uint64_t key = 0;
+                if (n == 1) {
+                    f >> key;
+                } else if (n == 2) {
+                    uint64_t a = 0, b = 0;
+                    f >> a >> b;
+                    key = Key2((uint32_t)a, (uint32_t)b);
+                } else {
+                    uint64_t a = 0, b = 0, c = 0, d = 0;
+                    f >> a >> b >> c;
+                    if (n == 4) f >> d;
+                    key = Key4((kState_t)a, (kState_t)b, (kState_t)c, (kState_t)d);
+                }
+                keys[i] = key;
+                utilities::FromStream(f, vals[i]);
+                if (!f) return false;
+            }
+            return true;
+        }
+    };
+
+    // QuarticTermHashTables (optical_flux_lattice term_hash_tables.cpp:96-172)
+    class QuarticTermHashTables : public TermHashTablesBase
+    {
+        public:
+        void GetK1(kState_t* buf, iSize_t& n, const kState_t k2, const kState_t k3, const kState_t k4) const
+        {
+            List(buf, n, Key4(k2, k3, k4, 0));
+        }
+        dcmplx GetVkkkk(const kState_t k1, const kState_t k2, const kState_t k3, const kState_t k4) const
+        {
+            return Value(Key4(k1, k2, k3, k4));
+        }
+        // SetQuarticFromArray (term_hash_tables_base.hpp:231-252): every
+        // (k2,k3,k4) with the array's k1 and GetVkkkk
+        template <class C>
+        void SetQuarticFromArray(C* quarticArray)
+        {
+            for (kState_t k2 = 0; k2 < m_kMax; ++k2)
+                for (kState_t k3 = 0; k3 < m_kMax; ++k3)
+                    for (kState_t k4 = 0; k4 < m_kMax; ++k4) {
+                        kState_t k1 = 0;
+                        iSize_t n = 0;
+                        quarticArray->GetK1(&k1, n, k2, k3, k4);
+                        InsertK1(k1, Key4(k2, k3, k4, 0));
+                        InsertV(Key4(k1, k2, k3, k4)) = quarticArray->GetVkkkk(k1, k2, k3, k4);
+                    }
+        }
+        void ToFile(const std::string fileName, const io::fileFormat_t format, utilities::MpiWrapper& mpi) const
+        {
+            ToFileBase(fileName, format, 4, mpi);
+        }
+        void FromFile(const std::string fileName, const io::fileFormat_t format, utilities::MpiWrapper& mpi)
+        {
+            FromFileBase(fileName, format, mpi);
+            kState_t km = 0;
+            for (const auto& kv : m_k1) km = std::max<kState_t>(km, (kState_t)((kv.first >> 48) + 1));
+            if (m_kMax == 0) m_kMax = km;
+        }
+    };
+
+    // QuadraticTermHashTables (term_hash_tables.cpp:36-94)
+    class QuadraticTermHashTables : public TermHashTablesBase
+    {
+        public:
+        void GetK1(kState_t* buf, iSize_t& n, const kState_t k2) const { List(buf, n, (uint64_t)k2); }
+        dcmplx GetEkk(const kState_t k1, const kState_t k2) const { return Value(Key2(k1, k2)); }
+        // SetQuadraticFromArray (term_hash_tables_base.hpp:264-275)
+        template <class C>
+        void SetQuadraticFromArray(C* quadraticArray)
+        {
+            for (kState_t k2 = 0; k2 < m_kMax; ++k2) {
+                kState_t k1 = 0;
+                iSize_t n = 0;
+                quadraticArray->GetK1(&k1, n, k2);
+                InsertK1(k1, (uint64_t)k2);
+                InsertV(Key2(k1, k2)) = quadraticArray->GetEkk(k1, k2);
+            }
+        }
+        void ToFile(const std::string fileName, const io::fileFormat_t format, utilities::MpiWrapper& mpi) const
+        {
+            ToFileBase(fileName, format, 2, mpi);
+        }
+        void FromFile(const std::string fileName, const io::fileFormat_t format, utilities::MpiWrapper& mpi)
+        {
+            FromFileBase(fileName, format, mpi);
+            kState_t km = 0;
+            for (const auto& kv : m_k1) km = std::max<kState_t>(km, (kState_t)(kv.first + 1));
+            if (m_kMax == 0) m_kMax = km;
+        }
+    };
+
+    // NonInteractingOflModelGrid::GenerateWannierCoefficients
+    // (noninteracting_ofl_model_grid.cpp:782-932): the change-of-basis matrix M
+    // (K x K, row x*dimY+ky, column kx*dimY+ky') from the band-0 Bloch vectors
+    // (blochDim values per k): A_x(kx,ky) = <u(kx,ky)|u(kx+1,ky)> / |.|,
+    // lambda(ky) = (prod_kx A_x)^(1/dimX), aProd(kx,ky) = prod_{kappa<kx} A_x,
+    // M = exp(-2 pi i (kx + offsetX) x / dimX) lambda^kx / (sqrt(dimX) aProd).
+    inline void GenerateWannierCoefficients(const iSize_t dimX, const iSize_t dimY, const double offsetX,
+                                            const dcmplx* bloch, const iSize_t blochDim, dcmplx* M)
+    {
+        const double pi = 3.1415926535897932384626433832795028841972;
+        const dcmplx I(0.0, 1.0);
+        std::vector<dcmplx> Ax((size_t)dimX * dimY), lambdaX(dimY, dcmplx(1.0)), aProd((size_t)dimX * dimY);
+        for (iSize_t kx = 0; kx < dimX; ++kx)
+            for (iSize_t ky = 0; ky < dimY; ++ky) {
+                const iSize_t kn = kx == dimX - 1 ? 0 : kx + 1;
+                dcmplx a(0.0);
+                for (iSize_t i = 0; i < blochDim; ++i)
+                    a += std::conj(bloch[((size_t)kx * dimY + ky) * blochDim + i]) * bloch[((size_t)kn * dimY + ky) * blochDim + i];
+                Ax[(size_t)kx * dimY + ky] = a / std::abs(a);
+            }
+        for (iSize_t kx = 0; kx < dimX; ++kx)
+            for (iSize_t ky = 0; ky < dimY; ++ky) lambdaX[ky] *= Ax[(size_t)kx * dimY + ky];
+        for (iSize_t ky = 0; ky < dimY; ++ky) lambdaX[ky] = std::pow(lambdaX[ky], 1.0 / dcmplx(dimX, 0));
+        for (iSize_t ky = 0; ky < dimY; ++ky) aProd[ky] = 1.0;
+        for (iSize_t kx = 1; kx < dimX; ++kx)
+            for (iSize_t ky = 0; ky < dimY; ++ky)
+                aProd[(size_t)kx * dimY + ky] = Ax[(size_t)(kx - 1) * dimY + ky] * aProd[(size_t)(kx - 1) * dimY + ky];
+        const iSize_t dim = dimX * dimY;
+        for (iSize_t x = 0; x < dimX; ++x)
+            for (iSize_t ky = 0; ky < dimY; ++ky)
+                for (iSize_t kx = 0; kx < dimX; ++kx)
+                    for (iSize_t ky1 = 0; ky1 < dimY; ++ky1) {
+                        const size_t i1 = (size_t)x * dimY + ky, i2 = (size_t)kx * dimY + ky1;
+                        M[i1 * dim + i2] = ky == ky1 ? std::exp(-2.0 * I * pi * ((double)kx + offsetX) * (double)x / (double)dimX) *
+                                                           std::pow(lambdaX[ky], dcmplx(kx, 0)) /
+                                                           (std::sqrt((double)dimX) * aProd[i2])
+                                                     : dcmplx(0.0);
+                    }
+    }
+
+    // InteractingOflModel::ChangeToWannierBasis (interacting_ofl_model.cpp:728-923)
+    // through hd_ofl_wannier: the hash tables of the hybrid Wannier basis from
+    // the Bloch-basis array tables (quad may be null: no quadratic terms), the
+    // change-of-basis matrix and matrixElementTol.
+    inline bool ChangeToWannierBasis(const int dimX, const int dimY, const double offsetX, OflQuarticTermTables& quart,
+                                     DiagonalQuadraticTermTables<dcmplx>* quad, const dcmplx* basisChange,
+                                     const double tol, QuarticTermHashTables& hashQuart,
+                                     QuadraticTermHashTables& hashQuad)
+    {
+        const size_t K = (size_t)dimX * dimY, slots = K * K * K * dimX;
+        std::vector<uint8_t> nk1(K * K * K), nq1(K);
+        std::vector<uint16_t> k1(slots), q1(K * dimX);
+        std::vector<dcmplx> v(slots), e(K * dimX);
+        if (!utilities::detail::Ok(
+                hd_ofl_wannier(dimX, dimY, offsetX, quart.GetKTable()->data(),
+                               reinterpret_cast<const double*>(quart.GetVTable()->data()),
+                               quad ? reinterpret_cast<const double*>(quad->GetVTable()->data()) : nullptr,
+                               reinterpret_cast<const double*>(basisChange), tol, nk1.data(), k1.data(),
+                               reinterpret_cast<double*>(v.data()), nq1.data(), q1.data(),
+                               reinterpret_cast<double*>(e.data()), nullptr),
+                "ChangeToWannierBasis"))
+            return false;
+        hashQuart.Clear();
+        hashQuad.Clear();
+        hashQuart.Initialize((kState_t)K);
+        hashQuad.Initialize((kState_t)K);
+        for (size_t ent = 0; ent < K * K * K; ++ent) {
+            const kState_t k2 = (kState_t)(ent % K), k3 = (kState_t)((ent / K) % K), k4 = (kState_t)(ent / (K * K));
+            for (int j = 0; j < nk1[ent]; ++j) {
+                const size_t s = ent * dimX + j;
+                hashQuart.InsertK1(k1[s], TermHashTablesBase::Key4(k2, k3, k4, 0));
+                // the reference inserts a value only where |sum| > tol
+                if (v[s] != dcmplx(0.0)) hashQuart.InsertV(TermHashTablesBase::Key4(k1[s], k2, k3, k4)) = v[s];
+            }
+        }
+        if (quad)
+            for (size_t k2 = 0; k2 < K; ++k2)
+                for (int j = 0; j < nq1[k2]; ++j) {
+                    const size_t s = k2 * dimX + j;
+                    hashQuad.InsertK1(q1[s], (uint64_t)k2);
+                    hashQuad.InsertV(TermHashTablesBase::Key2(q1[s], (uint32_t)k2)) = e[s];
+                }
+        return true;
+    }
 }
 
 #endif

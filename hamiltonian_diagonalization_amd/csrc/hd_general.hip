@@ -13,8 +13,8 @@
 // terms of one pair hop, so the pair tables of hd_build.hip do not apply.
 // Every term is evaluated per row, and duplicates are summed afterwards:
 //   1. k_gcount: one thread per row walks the terms in the reference's order
-//      and counts the ones the reference inserts into its map (|el| > tol and,
-//      for upper storage, out >= in).
+//      and counts the ones the reference inserts into its map (|el| > tol,
+//      out >= in).
 //   2. exclusive scan -> raw row offsets.
 //   3. k_gfill: the same walk writes (column, value) in term order; an out
 //      state outside the sector (FindFockStateIndex's miss, skipped by the
@@ -26,6 +26,15 @@
 //      kept entries packed at the row's raw offset.
 //   5. scan of the kept counts -> CRS row pointers; k_gcompact copies each
 //      row into the final FP64 / FP32 arrays.
+// Full storage is built from the upper triangle: row r holds
+// conj(U[i][r]) for the i < r with U[i][r] stored, in ascending i, then the
+// upper entries of row r.  That is the operator U + (U - D)^H the reference's
+// product applies to its upper CRS (sparse_linear_algebra.hpp:142-165), also
+// where the tables are not exactly Hermitian (the hybrid Wannier quadratic
+// terms of ChangeToWannierBasis take |x2 - x1| on both sides of the diagonal).
+// The lower entries are scattered per column with atomics (k_glower_scatter)
+// and put back in column order per row (k_gassemble), so the result does not
+// depend on the scatter order.
 // Values are bit-identical to the reference's: the same terms, the same
 // products (sign * value is an exact negation or copy), the same additions in
 // the same order.
@@ -276,6 +285,113 @@ k_gmerge(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restr
     if (trimmed) atomicAdd(ntrim, trimmed);
 }
 
+// Lower part of full storage: per-column counts of the strictly upper
+// entries (row r, column c > r) of the merged rows.
+__global__ void __launch_bounds__(256)
+k_glower_count(int64_t dim, const int64_t* __restrict__ raw_ptr, const int32_t* __restrict__ kept,
+               const int32_t* __restrict__ mcol, int32_t* __restrict__ lower)
+{
+    for (int64_t row = blockIdx.x; row < dim; row += gridDim.x) {
+        const int64_t src = raw_ptr[row];
+        for (int q = threadIdx.x; q < kept[row]; q += blockDim.x) {
+            const int32_t c = mcol[src + q];
+            if (c > row) atomicAdd(&lower[c], 1);
+        }
+    }
+}
+
+template <bool CPLX>
+__global__ void __launch_bounds__(256)
+k_glower_scatter(int64_t dim, const int64_t* __restrict__ raw_ptr, const int32_t* __restrict__ kept,
+                 const int32_t* __restrict__ mcol, const typename GT<CPLX>::T* __restrict__ mval,
+                 const int64_t* __restrict__ lptr, int32_t* __restrict__ cursor, int32_t* __restrict__ tcol,
+                 typename GT<CPLX>::T* __restrict__ tval)
+{
+    using T = typename GT<CPLX>::T;
+    for (int64_t row = blockIdx.x; row < dim; row += gridDim.x) {
+        const int64_t src = raw_ptr[row];
+        for (int q = threadIdx.x; q < kept[row]; q += blockDim.x) {
+            const int32_t c = mcol[src + q];
+            if (c <= row) continue;
+            const int64_t p = lptr[c] + atomicAdd(&cursor[c], 1);
+            tcol[p] = (int32_t)row;
+            T v = mval[src + q];
+            if (CPLX) reinterpret_cast<double*>(&v)[1] = -reinterpret_cast<double*>(&v)[1];
+            tval[p] = v;
+        }
+    }
+}
+
+// One workgroup per row of full storage: the row's lower entries sorted by
+// column (keys column << 32 | slot, columns distinct), then its upper entries.
+template <bool CPLX, int CAP>
+__global__ void __launch_bounds__(kMergeThreads)
+k_gassemble(int64_t dim, const int64_t* __restrict__ raw_ptr, const int32_t* __restrict__ kept,
+            const int32_t* __restrict__ mcol, const typename GT<CPLX>::T* __restrict__ mval,
+            const int64_t* __restrict__ lptr, const int32_t* __restrict__ tcol,
+            const typename GT<CPLX>::T* __restrict__ tval, const int64_t* __restrict__ rowptr,
+            int32_t* __restrict__ col, typename GT<CPLX>::T* __restrict__ val64, typename GT<CPLX>::F* __restrict__ val32)
+{
+    using T = typename GT<CPLX>::T;
+    __shared__ unsigned long long key[CAP];
+    const int tid = threadIdx.x;
+    for (int64_t row = blockIdx.x; row < dim; row += gridDim.x) {
+        const int64_t lp = lptr[row], dst = rowptr[row];
+        const int L = (int)(lptr[row + 1] - lp);
+        int n = 1;
+        while (n < L) n <<= 1;
+        for (int q = tid; q < n; q += kMergeThreads)
+            key[q] = q < L ? (((unsigned long long)(uint32_t)tcol[lp + q] << 32) | (unsigned)q) : ~0ull;
+        __syncthreads();
+        for (int k = 2; k <= n; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = tid; i < n; i += kMergeThreads) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const unsigned long long ki = key[i], kl = key[l];
+                        if ((ki > kl) == ((i & k) == 0)) {
+                            key[i] = kl;
+                            key[l] = ki;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int q = tid; q < L; q += kMergeThreads) {
+            const T v = tval[lp + (int64_t)(key[q] & 0xFFFFFFFFull)];
+            col[dst + q] = (int32_t)(key[q] >> 32);
+            if (val64) val64[dst + q] = v;
+            if (val32) val32[dst + q] = gf32(v);
+        }
+        const int64_t src = raw_ptr[row];
+        for (int q = tid; q < kept[row]; q += kMergeThreads) {
+            const T v = mval[src + q];
+            col[dst + L + q] = mcol[src + q];
+            if (val64) val64[dst + L + q] = v;
+            if (val32) val32[dst + L + q] = gf32(v);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gadd_counts(int64_t dim, const int32_t* __restrict__ a,
+                                                     const int32_t* __restrict__ b, int32_t* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < dim) out[i] = a[i] + b[i];
+}
+
+template <bool CPLX, int CAP>
+void launch_assemble(int grid, hipStream_t st, int64_t dim, const int64_t* raw_ptr, const int32_t* kept,
+                     const int32_t* mcol, const typename GT<CPLX>::T* mval, const int64_t* lptr, const int32_t* tcol,
+                     const typename GT<CPLX>::T* tval, const int64_t* rowptr, int32_t* col,
+                     typename GT<CPLX>::T* val64, typename GT<CPLX>::F* val32)
+{
+    hipLaunchKernelGGL((k_gassemble<CPLX, CAP>), dim3(grid), dim3(kMergeThreads), 0, st, dim, raw_ptr, kept, mcol,
+                       mval, lptr, tcol, tval, rowptr, col, val64, val32);
+}
+
 template <bool CPLX>
 __global__ void __launch_bounds__(256)
 k_gcompact(int64_t dim, const int64_t* __restrict__ raw_ptr, const int64_t* __restrict__ rowptr,
@@ -369,8 +485,9 @@ int run_general(hd_hamiltonian* h, const hd_basis* b, const hd_tables* t, double
     // ---- count
     HD_GTRY(dalloc_t(&d_counts, (size_t)dim, "row counts"));
     ev.start(st);
-    if (upper) hipLaunchKernelGGL((k_gcount<CPLX, true>), dim3(grid_rows), dim3(256), 0, st, b->d_states, dim, A, d_counts);
-    else hipLaunchKernelGGL((k_gcount<CPLX, false>), dim3(grid_rows), dim3(256), 0, st, b->d_states, dim, A, d_counts);
+    // the terms of the upper triangle (out >= in) in both storages; full
+    // storage completes it below
+    hipLaunchKernelGGL((k_gcount<CPLX, true>), dim3(grid_rows), dim3(256), 0, st, b->d_states, dim, A, d_counts);
     HD_ENQ(q, hipGetLastError());
     h->t_count = ev.stop(st);
     HD_GTRY(q.status());
@@ -382,10 +499,10 @@ int run_general(hd_hamiltonian* h, const hd_basis* b, const hd_tables* t, double
     ev.start(st);
     HD_GTRY(scan_counts(d_counts, dim, d_raw, &raw_nnz, &raw_max, st));
     h->t_scan = ev.stop(st);
-    if (raw_max > 8192) {
+    if (raw_max > 16384) {
         cleanup();
         return fail(HD_ERR_UNSUPPORTED, "a row has " + std::to_string(raw_max) +
-                                            " terms before merging; the per-row merge takes at most 8192");
+                                            " terms before merging; the per-row merge takes at most 16384");
     }
 
     // ---- fill in term order
@@ -409,10 +526,8 @@ int run_general(hd_hamiltonian* h, const hd_basis* b, const hd_tables* t, double
                            b->d_rank, rank_words, d_raw, d_rcol, d_rval, d_err);                              \
     } while (0)
         ev.start(st);
-        if (upper && lz) HD_GFILL(true, true);
-        else if (upper) HD_GFILL(true, false);
-        else if (lz) HD_GFILL(false, true);
-        else HD_GFILL(false, false);
+        if (lz) HD_GFILL(true, true);
+        else HD_GFILL(true, false);
 #undef HD_GFILL
         HD_ENQ(q, hipGetLastError());
         h->t_fill = ev.stop(st);
@@ -441,7 +556,8 @@ int run_general(hd_hamiltonian* h, const hd_basis* b, const hd_tables* t, double
         if (raw_max <= 256) launch_merge<CPLX, 256>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
         else if (raw_max <= 1024) launch_merge<CPLX, 1024>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
         else if (raw_max <= 4096) launch_merge<CPLX, 4096>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
-        else launch_merge<CPLX, 8192>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
+        else if (raw_max <= 8192) launch_merge<CPLX, 8192>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
+        else launch_merge<CPLX, 16384>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
         HD_ENQ(q, hipGetLastError());
         HD_GTRY(q.status());
         dfree(d_rcol);
@@ -450,36 +566,100 @@ int run_general(hd_hamiltonian* h, const hd_basis* b, const hd_tables* t, double
         d_rval = nullptr;
         HD_GTRY(dalloc_t(&h->d_rowptr, (size_t)dim + 1, "row pointers"));
         h->bytes += ((size_t)dim + 1) * 8;
-        HD_GTRY(scan_counts(d_counts, dim, h->d_rowptr, &nnz, &max_row, st));
+        // upper storage: the kept counts are the row lengths; full storage
+        // first needs the lower part of every row
+        int64_t* d_uptr = nullptr;
+        int32_t *d_lower = nullptr, *d_cursor = nullptr, *d_tcol = nullptr;
+        int64_t* d_lptr = nullptr;
+        T* d_tval = nullptr;
+        auto free_full = [&]() { dfree(d_uptr); dfree(d_lower); dfree(d_cursor); dfree(d_tcol); dfree(d_lptr); dfree(d_tval); };
+#define HD_FTRY(x)                 \
+    do {                           \
+        const int _s = (x);        \
+        if (_s != HD_OK) {         \
+            free_full();           \
+            cleanup();             \
+            return _s;             \
+        }                          \
+    } while (0)
+        int64_t total_lower = 0;
+        int max_lower = 0;
+        if (upper) {
+            HD_GTRY(scan_counts(d_counts, dim, h->d_rowptr, &nnz, &max_row, st));
+        } else {
+            int64_t nnz_u = 0;
+            int max_u = 0;
+            HD_FTRY(dalloc_t(&d_uptr, (size_t)dim + 1, "upper row pointers"));
+            HD_FTRY(scan_counts(d_counts, dim, d_uptr, &nnz_u, &max_u, st));
+            HD_FTRY(dalloc_t(&d_lower, (size_t)dim, "lower counts"));
+            HD_ENQ(q, hipMemsetAsync(d_lower, 0, (size_t)dim * sizeof(int32_t), st));
+            hipLaunchKernelGGL(k_glower_count, dim3(grid), dim3(256), 0, st, dim, d_raw, d_counts, d_mcol, d_lower);
+            HD_ENQ(q, hipGetLastError());
+            HD_FTRY(dalloc_t(&d_lptr, (size_t)dim + 1, "lower row offsets"));
+            HD_FTRY(scan_counts(d_lower, dim, d_lptr, &total_lower, &max_lower, st));
+            if (max_lower > 16384) {
+                free_full();
+                cleanup();
+                return fail(HD_ERR_UNSUPPORTED, "a column holds " + std::to_string(max_lower) +
+                                                    " upper entries; full storage takes at most 16384");
+            }
+            HD_FTRY(dalloc_t(&d_cursor, (size_t)dim, "lower cursors"));
+            HD_ENQ(q, hipMemsetAsync(d_cursor, 0, (size_t)dim * sizeof(int32_t), st));
+            HD_FTRY(dalloc_t(&d_tcol, (size_t)std::max<int64_t>(total_lower, 1), "lower columns"));
+            HD_FTRY(dalloc_t(&d_tval, (size_t)std::max<int64_t>(total_lower, 1), "lower values"));
+            hipLaunchKernelGGL((k_glower_scatter<CPLX>), dim3(grid), dim3(256), 0, st, dim, d_raw, d_counts, d_mcol,
+                               d_mval, d_lptr, d_cursor, d_tcol, d_tval);
+            HD_ENQ(q, hipGetLastError());
+            hipLaunchKernelGGL(k_gadd_counts, dim3((unsigned)((dim + 255) / 256)), dim3(256), 0, st, dim, d_counts,
+                               d_lower, d_cursor);  // the cursors now hold the full row lengths
+            HD_ENQ(q, hipGetLastError());
+            HD_FTRY(scan_counts(d_cursor, dim, h->d_rowptr, &nnz, &max_row, st));
+            HD_FTRY(q.status());
+        }
         h->nnz = nnz;
         h->max_row = max_row;
         unsigned long long trimmed = 0;
         HD_ENQ(q, hipMemcpyAsync(&trimmed, d_misc, sizeof(trimmed), hipMemcpyDeviceToHost, st));
         HD_ENQ(q, hipStreamSynchronize(st));
-        HD_GTRY(q.status());
+        HD_FTRY(q.status());
         h->n_trimmed = (int64_t)trimmed;
         // ---- final arrays, padded by 8 zeroed entries (the SpMV reads aligned
         // groups of 4 that may run past the last row)
         const size_t n = (size_t)nnz;
-        HD_GTRY(dalloc_t(&h->d_col, n + 8, "columns"));
+        HD_FTRY(dalloc_t(&h->d_col, n + 8, "columns"));
         HD_ENQ(q, hipMemsetAsync(h->d_col + n, 0, 8 * sizeof(int32_t), st));
         h->bytes += n * 4;
         if (value_mask & 1) {
-            HD_GTRY(dalloc(&h->d_val64, (n + 8) * sizeof(T), "FP64 values"));
+            HD_FTRY(dalloc(&h->d_val64, (n + 8) * sizeof(T), "FP64 values"));
             HD_ENQ(q, hipMemsetAsync(static_cast<T*>(h->d_val64) + n, 0, 8 * sizeof(T), st));
             h->bytes += n * sizeof(T);
         }
         if (value_mask & 2) {
-            HD_GTRY(dalloc(&h->d_val32, (n + 8) * sizeof(F), "FP32 values"));
+            HD_FTRY(dalloc(&h->d_val32, (n + 8) * sizeof(F), "FP32 values"));
             HD_ENQ(q, hipMemsetAsync(static_cast<F*>(h->d_val32) + n, 0, 8 * sizeof(F), st));
             h->bytes += n * sizeof(F);
         }
-        hipLaunchKernelGGL((k_gcompact<CPLX>), dim3(grid), dim3(256), 0, st, dim, d_raw, h->d_rowptr, d_mcol, d_mval,
-                           h->d_col, static_cast<T*>(h->d_val64), static_cast<F*>(h->d_val32));
+        T* v64 = static_cast<T*>(h->d_val64);
+        F* v32 = static_cast<F*>(h->d_val32);
+        if (upper) {
+            hipLaunchKernelGGL((k_gcompact<CPLX>), dim3(grid), dim3(256), 0, st, dim, d_raw, h->d_rowptr, d_mcol,
+                               d_mval, h->d_col, v64, v32);
+        } else {
+#define HD_ASM(C) launch_assemble<CPLX, C>(grid, st, dim, d_raw, d_counts, d_mcol, d_mval, d_lptr, d_tcol, d_tval, \
+                                            h->d_rowptr, h->d_col, v64, v32)
+            if (max_lower <= 256) HD_ASM(256);
+            else if (max_lower <= 1024) HD_ASM(1024);
+            else if (max_lower <= 4096) HD_ASM(4096);
+            else if (max_lower <= 8192) HD_ASM(8192);
+            else HD_ASM(16384);
+#undef HD_ASM
+        }
         HD_ENQ(q, hipGetLastError());
         h->t_sort = ev.stop(st);
         HD_ENQ(q, hipStreamSynchronize(st));
-        HD_GTRY(q.status());
+        HD_FTRY(q.status());
+        free_full();
+#undef HD_FTRY
     }
     HD_GTRY(ev.status("general build timing"));
     cleanup();

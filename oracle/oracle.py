@@ -248,6 +248,36 @@ def build_general(Norb, bas, K, nk1, k1, v, nq1=None, q1=None, e=None, tol=ZERO_
     return CRS(rp, col[:nnz], val[:nnz * (2 if cplx else 1)], cplx, ntr.value)
 
 
+def mirror_upper(crs):
+    """Full storage of the operator U + (U - D)^H that the reference's product
+    applies to its upper CRS (sparse_linear_algebra.hpp:142-165): row r holds
+    conj(U[i][r]) for every stored U[i][r] with i < r, ascending in i, then the
+    stored entries of row r."""
+    dim = len(crs.rowptr) - 1
+    cnt = np.diff(crs.rowptr)
+    rows = np.repeat(np.arange(dim), cnt)
+    v = crs.values()
+    low = crs.col > rows
+    # lower entries (row = col of the upper entry, col = its row), sorted by (row, col)
+    lr, lc, lv = crs.col[low].astype(np.int64), rows[low], np.conj(v[low]) if crs.is_complex else v[low]
+    order = np.lexsort((lc, lr))
+    lr, lc, lv = lr[order], lc[order], lv[order]
+    # merge: for each row, its lower entries then its upper entries
+    key_l = lr * 2
+    key_u = rows.astype(np.int64) * 2 + 1
+    allkey = np.concatenate([key_l, key_u])
+    allcol = np.concatenate([lc, crs.col.astype(np.int64)])
+    allval = np.concatenate([lv, v])
+    order = np.argsort(allkey, kind="stable")
+    col = allcol[order].astype(np.int32)
+    val = allval[order]
+    lcnt = np.bincount(lr, minlength=dim) if len(lr) else np.zeros(dim, dtype=np.int64)
+    rp = np.zeros(dim + 1, dtype=np.int64)
+    rp[1:] = np.cumsum(cnt + lcnt)
+    raw = val.view(np.float64) if crs.is_complex else val
+    return CRS(rp, col, np.ascontiguousarray(raw), crs.is_complex, crs.n_trimmed)
+
+
 def ofl_wannier_quartic(dim_x, dim_y, ktable, vtable, u, tol, s0=0, s1=None):
     """ChangeToWannierBasis quartic terms for list slots [s0, s1) of the
     (k4*K + k3)*K + k2, x1 enumeration: (k1 labels, complex values)."""

@@ -262,3 +262,176 @@ def test_dropin_flux_lattice_from_bloch(oracle, tmp_path):
     assert r["dim"] == len(states)
     assert np.array_equal(np.fromfile(pre + ".cols", dtype=np.int32), up.col)
     assert np.array_equal(np.fromfile(pre + ".values", dtype=np.complex128), up.values())
+
+
+def test_hash_table_file_format(tmp_path):
+    """QuarticTermHashTables / QuadraticTermHashTables files: the reference's
+    layout (term_hash_tables_base.hpp:139-219, multi_key_hash.hpp:480-669):
+    header, then the k1 multimap (uint32 size, uint64 keys Pack4x16(k2,k3,k4,0),
+    uint16 k1), then the value map (uint32 size, uint64 keys
+    Pack4x16(k1,k2,k3,k4), complex values); a written file reads back to the
+    same tables, in binary and in text."""
+    src = tmp_path / "t.cpp"
+    src.write_text(r'''
+#include "hd_dropin.hpp"
+using namespace diagonalization;
+int main(int argc, char** argv) {
+  utilities::MpiWrapper mpi;
+  const kState_t K = 6;
+  OflQuarticTermTables arr; DiagonalQuadraticTermTables<dcmplx> quad;
+  arr.Initialize(K, mpi); quad.Initialize(K, mpi);
+  for (kState_t k4 = 0; k4 < K; ++k4) for (kState_t k3 = k4; k3 < K; ++k3) for (kState_t k2 = 0; k2 < K; ++k2) {
+    const kState_t k1 = (kState_t)((k3 + k4 + 2 * K - k2) % K);
+    arr.SetK1(k1, k2, k3, k4);
+    arr.SetVkkkk(dcmplx(0.125 * k1 + k2, 0.5 * k3 - k4 + 0.1), k1, k2, k3, k4); }
+  for (kState_t k = 0; k < K; ++k) quad.SetEkk(dcmplx(-0.25 * k, 0.0), k);
+  QuarticTermHashTables hq; QuadraticTermHashTables hd2;
+  hq.Initialize(K); hd2.Initialize(K);
+  hq.SetQuarticFromArray(&arr); hd2.SetQuadraticFromArray(&quad);
+  const std::string dir = argv[1];
+  for (int f = 0; f < 2; ++f) {
+    const io::fileFormat_t fmt = f ? io::_BINARY_ : io::_TEXT_;
+    const std::string tag = f ? "bin" : "txt";
+    hq.ToFile(dir + "/quartic." + tag, fmt, mpi); hd2.ToFile(dir + "/quadratic." + tag, fmt, mpi);
+    QuarticTermHashTables rq; QuadraticTermHashTables rd;
+    rq.FromFile(dir + "/quartic." + tag, fmt, mpi); rd.FromFile(dir + "/quadratic." + tag, fmt, mpi);
+    rq.ToFile(dir + "/quartic2." + tag, fmt, mpi); rd.ToFile(dir + "/quadratic2." + tag, fmt, mpi);
+    kState_t a[8], b[8]; iSize_t na = 0, nb = 0;
+    for (kState_t k2 = 0; k2 < K; ++k2) for (kState_t k3 = 0; k3 < K; ++k3) for (kState_t k4 = 0; k4 < K; ++k4) {
+      hq.GetK1(a, na, k2, k3, k4); rq.GetK1(b, nb, k2, k3, k4);
+      if (na != 1 || nb != 1 || a[0] != b[0]) return 1;
+      if (std::abs(hq.GetVkkkk(a[0], k2, k3, k4) - rq.GetVkkkk(b[0], k2, k3, k4)) > 1e-13) return 2;
+      if (f && hq.GetVkkkk(a[0], k2, k3, k4) != arr.GetVkkkk(a[0], k2, k3, k4)) return 3; }
+    for (kState_t k = 0; k < K; ++k) { rd.GetK1(a, na, k); if (na != 1 || a[0] != k || rd.GetEkk(k, k) != quad.GetEkk(k, k)) return 4; }
+  }
+  return 0; }
+''')
+    exe = tmp_path / "t"
+    p = subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"), "-I" + CPP, str(src),
+                        "-o", str(exe)], capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
+    p = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True)
+    assert p.returncode == 0, (p.returncode, p.stdout, p.stderr)
+    header = b"# File contains term table data in hash table format"
+    K = 6
+    for name in ("quartic", "quadratic"):
+        assert (tmp_path / f"{name}.bin").read_bytes() == (tmp_path / f"{name}2.bin").read_bytes()
+        assert (tmp_path / f"{name}.txt").read_text() == (tmp_path / f"{name}2.txt").read_text()
+        raw = (tmp_path / f"{name}.bin").read_bytes()
+        n = int(np.frombuffer(raw[:4], dtype=np.uint32)[0])
+        assert raw[4:4 + n] == header
+        off = 4 + n
+        size = int(np.frombuffer(raw[off:off + 4], dtype=np.uint32)[0])
+        keys = np.frombuffer(raw[off + 4:off + 4 + 8 * size], dtype=np.uint64)
+        k1 = np.frombuffer(raw[off + 4 + 8 * size:off + 4 + 10 * size], dtype=np.uint16)
+        off += 4 + 10 * size
+        vsize = int(np.frombuffer(raw[off:off + 4], dtype=np.uint32)[0])
+        vkeys = np.frombuffer(raw[off + 4:off + 4 + 8 * vsize], dtype=np.uint64)
+        vals = np.frombuffer(raw[off + 4 + 8 * vsize:off + 4 + 24 * vsize], dtype=np.complex128)
+        assert off + 4 + 24 * vsize == len(raw)
+        if name == "quartic":
+            assert size == vsize == K ** 3
+            k2, k3, k4 = (keys >> 48) & 0xFFFF, (keys >> 32) & 0xFFFF, (keys >> 16) & 0xFFFF
+            assert np.all(keys & 0xFFFF == 0)
+            assert np.array_equal(k1, (k3 + k4 + 2 * K - k2) % K)
+            vk1, vk2, vk3, vk4 = (vkeys >> 48) & 0xFFFF, (vkeys >> 32) & 0xFFFF, (vkeys >> 16) & 0xFFFF, vkeys & 0xFFFF
+            assert np.array_equal(vk1, (vk3 + vk4 + 2 * K - vk2) % K)
+            # GetVkkkk of the folded array: k4 > k3 reads the (k3, k4) entry with k1 for k2
+            want = np.where(vk4 > vk3, 0.125 * vk2 + vk1 + 1j * (0.5 * vk4 - vk3 + 0.1),
+                            0.125 * vk1 + vk2 + 1j * (0.5 * vk3 - vk4 + 0.1))
+            assert np.array_equal(vals, want)
+        else:
+            assert size == vsize == K
+            assert np.array_equal(keys, np.arange(K)) and np.array_equal(k1, np.arange(K))
+            assert np.array_equal(vkeys, (np.arange(K, dtype=np.uint64) << np.uint64(32)) | np.arange(K, dtype=np.uint64))
+            assert np.array_equal(vals, -0.25 * np.arange(K) + 0j)
+
+
+def _ofl_files(tmp_path, kt, vt, eps):
+    kt.tofile(tmp_path / "k.bin")
+    vt.astype(np.complex128).tofile(tmp_path / "v.bin")
+    eps.astype(np.complex128).tofile(tmp_path / "e.bin")
+    return ["--ktable", tmp_path / "k.bin", "--vtable", tmp_path / "v.bin", "--eps", tmp_path / "e.bin"]
+
+
+@pytest.mark.gpu
+def test_dropin_use_hash_matches_array_path(oracle, tmp_path):
+    """--use-hash 1 (run_test.sh:7-9): the hash tables of ConvertTableFormat
+    build, through the general device build, the same upper CRS as the array
+    tables, bit for bit; stored and retrieved in the hash-table file format
+    (binary) they build it again."""
+    X, Y, n = 3, 4, 5
+    K = X * Y
+    kt, vt, eps = models.ofl_tables(X, Y, 11)
+    files = _ofl_files(tmp_path, kt, vt, eps)
+    out = tmp_path / "out"
+    out.mkdir()
+    base = ["--nbr-particles", n, "--kmax", K, "--layout", "ofl", "--complex", 1, "--dim-x", X, "--dim-y", Y,
+            "--sector", "k2d:0,0,3,4", "--nbr-eigenvalues", 5, "--tol", 1e-11, "--dense-below", 60,
+            "--out-path", str(out), "--file-format", 1]
+    say("drop-in --use-hash 1, 3x4 N=5")
+    bas = oracle.basis(oracle.sector(oracle.SECTOR_K2D, 0, 0, X, Y), n, K)
+    up = oracle.build(n, K, bas, oracle.LAYOUT_OFL, K, kt, vt, eps=eps, is_complex=True, full=False)
+    want = np.linalg.eigvalsh(up.to_dense(len(bas)))[:5]
+    for extra in (["--store-terms", 1], ["--retrieve-terms", 1]):
+        pre = str(tmp_path / ("crs" + extra[0]))
+        r = last_json(run(base + files + ["--use-hash", 1, "--dump-crs", pre] + extra))
+        say(extra, r)
+        assert np.array_equal(np.fromfile(pre + ".rowstarts", dtype=np.int32), up.rowptr.astype(np.int32))
+        assert np.array_equal(np.fromfile(pre + ".cols", dtype=np.int32), up.col)
+        assert np.array_equal(np.fromfile(pre + ".values", dtype=np.complex128), up.values())
+        assert r["ok"] and np.max(np.abs(np.array(r["eigenvalues"]) - want)) < 1e-10
+    assert (out / f"quartic_coefficient_table_kx_{X}_ky_{Y}.dat").exists()
+
+
+@pytest.mark.gpu
+def test_dropin_wannier_basis(oracle, tmp_path):
+    """--basis 1 (run_test.sh:19): the hybrid Wannier tables built through the
+    drop-in give the upper CRS of the oracle's general build on hd_ofl_wannier's
+    lists (--wannier-matrix), and with the change of basis generated from
+    Bloch vectors (--bloch) the lowest eigenvalues of the k_y sector are the
+    lowest of the Bloch sectors (k_x, k_y) together (offset 0, E(k_x) = E(-k_x))."""
+    X, Y, n, ky = 3, 4, 4, 1
+    K = X * Y
+    rng = np.random.default_rng(21)
+    E = rng.uniform(-0.3, 0.3, size=(X, Y))
+    E = (E + E[(-np.arange(X)) % X]) / 2
+    eps = E.reshape(K).astype(np.complex128)
+    eps.tofile(tmp_path / "e.bin")
+    # (1) raw tables and an explicit change of basis: bit-exact CRS
+    kt, vt, _ = models.ofl_tables(X, Y, 4)
+    U = models.wannier_matrix(X, Y, 0.3, np.exp(2j * np.pi * rng.random((X, Y))))
+    U.astype(np.complex128).tofile(tmp_path / "u.bin")
+    kt.tofile(tmp_path / "k.bin")
+    vt.astype(np.complex128).tofile(tmp_path / "v.bin")
+    pre = str(tmp_path / "crs")
+    say("drop-in --basis 1 with --wannier-matrix, 3x4 N=4")
+    r = last_json(run(["--nbr-particles", n, "--kmax", K, "--layout", "ofl", "--complex", 1, "--dim-x", X,
+                       "--dim-y", Y, "--ktable", tmp_path / "k.bin", "--vtable", tmp_path / "v.bin",
+                       "--eps", tmp_path / "e.bin", "--basis", 1, "--kx-shift", 0.3, "--tol-vkkkk", 1e-12,
+                       "--wannier-matrix", tmp_path / "u.bin", "--sector", f"k1d:{ky},{Y}", "--nbr-eigenvalues", 4,
+                       "--dense-below", 0, "--dump-crs", pre]))
+    say(r)
+    lists = hd.ofl_wannier(X, Y, 0.3, kt, vt, eps, U, 1e-12)
+    bas = oracle.basis(oracle.sector(oracle.SECTOR_K1D, ky, Y), n, K)
+    up = oracle.build_general(K, bas, K, full=False, **lists)
+    assert np.array_equal(np.fromfile(pre + ".rowstarts", dtype=np.int32), up.rowptr.astype(np.int32))
+    assert np.array_equal(np.fromfile(pre + ".cols", dtype=np.int32), up.col)
+    assert np.array_equal(np.fromfile(pre + ".values", dtype=np.complex128), up.values())
+    # (2) Bloch coefficients: V and the change of basis both from them
+    cut = 2
+    b = rng.normal(size=(K, (2 * cut + 1) ** 2, 2)) + 1j * rng.normal(size=(K, (2 * cut + 1) ** 2, 2))
+    b /= np.linalg.norm(b.reshape(K, -1), axis=1)[:, None, None]
+    b.astype(np.complex128).tofile(tmp_path / "bloch.bin")
+    common = ["--nbr-particles", n, "--kmax", K, "--layout", "ofl", "--complex", 1, "--dim-x", X, "--dim-y", Y,
+              "--bloch", tmp_path / "bloch.bin", "--cutoff", cut, "--eps", tmp_path / "e.bin",
+              "--nbr-eigenvalues", 4, "--tol", 1e-11, "--dense-below", 0]
+    say("drop-in --basis 1 from Bloch vectors")
+    w = last_json(run(common + ["--basis", 1, "--sector", f"k1d:{ky},{Y}"]))
+    bloch = []
+    for kx in range(X):
+        rb = last_json(run(common + ["--sector", f"k2d:{kx},{ky},{X},{Y}"]))
+        assert rb["ok"]
+        bloch.extend(rb["eigenvalues"])
+    say("wannier", w["eigenvalues"], "bloch", sorted(bloch)[:4])
+    assert w["ok"] and np.max(np.abs(np.array(w["eigenvalues"]) - np.sort(bloch)[:4])) < 1e-9

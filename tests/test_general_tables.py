@@ -222,3 +222,23 @@ def test_wannier_spectrum_is_union_of_bloch_sectors(oracle, X, Y, n, ky, seed):
     w = np.linalg.eigvalsh(M)
     assert len(w) == len(bloch)
     assert np.abs(w - bloch).max() < 1e-11
+
+
+def test_mirror_upper_is_the_reference_product(oracle):
+    """oracle.mirror_upper lays out U + (U - D)^H: its rows applied to x give
+    the reference's symmetric product on the upper CRS, for a non-Hermitian
+    table too."""
+    K, n = 7, 3
+    lists = models.random_hash_tables(K, seed=8, complex_values=True)
+    bas = oracle.basis(oracle.sector(oracle.SECTOR_NONE), n, K)
+    up = oracle.build_general(K, bas, K, full=False, **lists)
+    full = oracle.mirror_upper(up)
+    assert np.all(np.diff(full.rowptr) >= 0)
+    for r in range(len(bas)):
+        c = full.col[full.rowptr[r]:full.rowptr[r + 1]]
+        assert np.all(np.diff(c) > 0)
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=len(bas)) + 1j * rng.normal(size=len(bas))
+    M = full.to_dense(len(bas), mirror=False)
+    assert np.abs(M @ x - oracle.spmv_sym(up, x)).max() < 1e-12
+    assert np.abs(M - up.to_dense(len(bas), mirror=True)).max() == 0.0

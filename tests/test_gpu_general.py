@@ -58,6 +58,7 @@ CASES = [
     (18, 6, False, ("none",), 4, 0.7, 3),    # rows of > 1024 terms: the 4096-slot merge
     (16, 6, False, ("none",), 16, 0.9, 5),   # rows of > 4096 terms: the 8192-slot merge
     (20, 4, True, ("k1d", 0, 4), 3, 0.5, 4),
+    (20, 8, False, ("k1d", 0, 10), 16, 0.9, 6),  # > 8192 terms per full row: the 16384-slot merge
 ]
 
 
@@ -72,9 +73,17 @@ def test_general_build_bit_exact(oracle, K, n, cplx, kind, max_k1, fill, seed, s
     ham = hd.Hamiltonian(basis, tables, storage=storage)
     bas = oracle.basis(_oracle_sector(oracle, kind), n, K)
     assert np.array_equal(basis.states(), bas)
-    ref = oracle.build_general(K, bas, K, full=storage == hd.STORAGE_FULL, **lists)
+    up = oracle.build_general(K, bas, K, full=False, **lists)
+    ref = oracle.mirror_upper(up) if storage == hd.STORAGE_FULL else up
     _same_bits(ham, ref, f32=storage == hd.STORAGE_FULL)
     assert ham.info()["max_row"] == int(np.diff(ref.rowptr).max())
+    if storage == hd.STORAGE_FULL:
+        # the product is the reference's on its upper CRS, U x + (U - D)^H x
+        rng = np.random.default_rng(seed)
+        x = rng.normal(size=len(bas)) + (1j * rng.normal(size=len(bas)) if cplx else 0)
+        y = ham.spmv(x, precision=hd.F64)
+        want = oracle.spmv_sym(up, x)
+        assert np.abs(y - want).max() <= 1e-12 * max(1.0, np.abs(want).max())
 
 
 def test_general_build_sums_duplicates_in_term_order(oracle):
@@ -114,7 +123,7 @@ def test_general_build_spmv_and_lanczos_match_dense(oracle):
     ham = hd.Hamiltonian(basis, hd.TermTables.hash(K, **herm))
     say("general spmv/lanczos dim", basis.dim)
     bas = basis.states()
-    M = oracle.build_general(K, bas, K, full=True, **herm).to_dense(len(bas), mirror=False)
+    M = oracle.build_general(K, bas, K, full=False, **herm).to_dense(len(bas), mirror=True)
     assert np.abs(M - M.conj().T).max() < 1e-12
     rng = np.random.default_rng(0)
     x = rng.normal(size=basis.dim) + 1j * rng.normal(size=basis.dim)
@@ -149,16 +158,22 @@ def test_hash_lists_of_array_tables_equal_pair_build(oracle, case):
         lists = hd.hash_lists_from_array(hd.LAYOUT_SPHERE, K, kt, vt)
         lay, cplx = oracle.LAYOUT_SPHERE, False
     say("hash lists of", case, "tables: dim", basis.dim)
-    gen = hd.Hamiltonian(basis, hd.TermTables.hash(K, **lists), values=(hd.F64, hd.F32))
-    pair = hd.Hamiltonian(basis, arr, values=(hd.F64, hd.F32))
+    gen = hd.Hamiltonian(basis, hd.TermTables.hash(K, **lists), storage=hd.STORAGE_UPPER_REF)
+    pair = hd.Hamiltonian(basis, arr, storage=hd.STORAGE_UPPER_REF)
     a, b = gen.crs(), pair.crs()
     for u, v in zip(a, b):
         assert np.array_equal(u.view(np.uint8), v.view(np.uint8))
-    a32, b32 = gen.crs(precision=hd.F32)[2], pair.crs(precision=hd.F32)[2]
-    assert np.array_equal(a32.view(np.uint8), b32.view(np.uint8))
     bas = basis.states()
-    ref = oracle.build(n, K, bas, lay, K, kt, vt, eps=eps, is_complex=cplx, full=True)
-    _same_bits(gen, ref, f32=False)
+    up = oracle.build(n, K, bas, lay, K, kt, vt, eps=eps, is_complex=cplx, full=False)
+    _same_bits(gen, up, f32=False)
+    # full storage: the Hermitian completion of that upper triangle, and the
+    # same spectrum as the pair build's full storage
+    genf = hd.Hamiltonian(basis, hd.TermTables.hash(K, **lists))
+    _same_bits(genf, oracle.mirror_upper(up))
+    pairf = hd.Hamiltonian(basis, arr, values=(hd.F64,))
+    e1 = genf.lanczos(nev=4, tol=1e-10).evals
+    e2 = pairf.lanczos(nev=4, tol=1e-10).evals
+    assert np.abs(np.asarray(e1) - np.asarray(e2)).max() < 1e-9
 
 
 @pytest.mark.parametrize("X,Y,offset", [(3, 2, 0.0), (4, 3, 0.37), (5, 5, 0.0)])
@@ -205,7 +220,7 @@ def test_wannier_basis_run_5x5(oracle):
     say("general build ms", {k: round(info[k], 2) for k in ("t_count_ms", "t_scan_ms", "t_fill_ms", "t_sort_ms")},
         "nnz", info["nnz"], "max_row", info["max_row"])
     bas = basis.states()
-    ref = oracle.build_general(K, bas, K, full=True, **lists)
+    ref = oracle.mirror_upper(oracle.build_general(K, bas, K, full=False, **lists))
     _same_bits(ham, ref)
     nev = 4
     w = ham.lanczos(nev=nev, tol=1e-10, ncv=40).evals
