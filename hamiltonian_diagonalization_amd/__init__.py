@@ -31,7 +31,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libhdiag_hip.so")
 
 HD_OK, HD_ERR_ARG, HD_ERR_HIP, HD_ERR_NOMEM, HD_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 HD_ERR_NOT_CONVERGED, HD_ERR_EMPTY = 5, 6
-SECTOR_NONE, SECTOR_LZ, SECTOR_K2D, SECTOR_K1D, SECTOR_LZ2 = 0, 1, 2, 3, 4
+SECTOR_NONE, SECTOR_LZ, SECTOR_K2D, SECTOR_K1D, SECTOR_LZ2, SECTOR_LIST = 0, 1, 2, 3, 4, 5
 LAYOUT_SPHERE, LAYOUT_OFL, LAYOUT_HASH = 0, 1, 2
 F64, F32, CODED, TILED, TILED_X32 = 0, 1, 2, 3, 4
 STORAGE_FULL, STORAGE_UPPER_REF, STORAGE_MATRIX_FREE = 0, 1, 2
@@ -114,6 +114,7 @@ SIGNATURES = {
     "hd_synchronize": (ctypes.c_int, []),
     "hd_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
     "hd_basis_create": (ctypes.c_int, [ctypes.POINTER(_Sector), _U32, _U32, _PP]),
+    "hd_basis_create_from_states": (ctypes.c_int, [_U32, _U32, _P, _I64, _PP]),
     "hd_basis_destroy": (ctypes.c_int, [_P]),
     "hd_basis_dim": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
     "hd_basis_copy_states": (ctypes.c_int, [_P, _I64, _I64, _P]),
@@ -291,6 +292,24 @@ class FockBasis:
         d = ctypes.c_int64()
         _check(lib().hd_basis_dim(h, ctypes.byref(d)))
         self.dim = d.value
+        self.is_list = False
+
+    @classmethod
+    def from_states(cls, states, n_particles: int, n_orbitals: int):
+        """GenerateFockSpace with a predicate evaluated by the caller: the kept
+        states, ascending (fermion_fock_basis.cpp:266-307); the device ranks
+        them by binary search and Hamiltonians over them take the general
+        build."""
+        self = cls.__new__(cls)
+        st = np.ascontiguousarray(states, dtype=np.uint64)
+        h = ctypes.c_void_p()
+        _check(lib().hd_basis_create_from_states(n_particles, n_orbitals, _ptr(st), len(st), ctypes.byref(h)))
+        self.h = h
+        self.sector = Sector(SECTOR_LIST)
+        self.n_particles, self.n_orbitals = n_particles, n_orbitals
+        self.dim = len(st)
+        self.is_list = True
+        return self
 
     def states(self, first=0, count=None) -> np.ndarray:
         count = self.dim - first if count is None else count
@@ -600,7 +619,7 @@ class Hamiltonian:
                  zero_tol=ZERO_TOL, values=None):
         if values is None:
             values = {STORAGE_FULL: (F64, F32, CODED), STORAGE_MATRIX_FREE: (F64,)}.get(storage, (F64, F32))
-            if tables.layout == LAYOUT_HASH:
+            if tables.layout == LAYOUT_HASH or getattr(basis, "is_list", False):
                 values = (F64, F32)  # general builds keep no coded copy
         mask = (1 if F64 in values else 0) | (2 if F32 in values else 0) | (4 if CODED in values else 0)
         h = ctypes.c_void_p()

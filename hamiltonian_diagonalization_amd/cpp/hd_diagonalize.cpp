@@ -12,6 +12,9 @@
 //   hd_diagonalize --nbr-particles N --kmax K --layout sphere|ofl --complex 0|1
 //                  --ktable F --vtable F [--eps F] --sector lz:a,b|k2d:kx,ky,dx,dy|k1d:ky,dy|none
 // options: --nbr-eigenvalues NEV (5)  --tol T (1e-12)  --seed S (0)  --full-reorth 1
+//          --sector also takes bind-lz:lz,maxLz / bind-k2d:kx,ky,dx,dy (the predicate as an opaque
+//              std::function, as the reference's callers pass it) and exclude:lz,maxLz,mask (an L_z
+//              sector without the states occupying the mask: no device sector, the general build)
 //          --precision f64|coded|tiled|f32  --dense-below D (60; the flux-lattice caller uses 1000)
 //          --dump-crs PREFIX (writes PREFIX.rowstarts/.cols/.values of the upper-triangle CRS)
 //          --dump-basis FILE  --initial-vector FILE  --final-vector FILE
@@ -336,6 +339,29 @@ namespace
         } else if (sec.rfind("k1d:", 0) == 0) {
             auto v = ParseList(sec.substr(4));
             empty = basis.GenerateFockSpace(N, norb, LinearMomentumSector1D((int)v.at(0), (int)v.at(1)), mpi);
+        } else if (sec.rfind("bind-lz:", 0) == 0 || sec.rfind("bind-k2d:", 0) == 0) {
+            // the reference callers' form: a std::function the basis cannot
+            // look inside (std::bind(&TestAngularMomentumSector, _1, lzTot,
+            // maxLz), pseudopotential_model.cpp:185-187)
+            const bool lzs = sec.rfind("bind-lz:", 0) == 0;
+            auto v = ParseList(sec.substr(lzs ? 8 : 9));
+            std::function<bool(const fock_t)> f;
+            if (lzs) {
+                const AngularMomentumSector am((int)v.at(0), (int)v.at(1));
+                f = [am](const fock_t s) { return am(s); };
+            } else {
+                const LinearMomentumSector2D k2((int)v.at(0), (int)v.at(1), (int)v.at(2), (int)v.at(3));
+                f = [k2](const fock_t s) { return k2(s); };
+            }
+            empty = basis.GenerateFockSpace(N, norb, f, mpi);
+        } else if (sec.rfind("exclude:", 0) == 0) {
+            // a predicate no device sector reproduces: an L_z sector without
+            // the states that occupy any orbital of the mask
+            auto v = ParseList(sec.substr(8));
+            const AngularMomentumSector am((int)v.at(0), (int)v.at(1));
+            const fock_t mask = (fock_t)v.at(2);
+            std::function<bool(const fock_t)> f = [am, mask](const fock_t s) { return am(s) && !(s & mask); };
+            empty = basis.GenerateFockSpace(N, norb, f, mpi);
         } else {
             empty = basis.GenerateFockSpace(N, norb, NoConstraint(), mpi);
         }

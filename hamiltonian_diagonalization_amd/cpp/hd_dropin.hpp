@@ -562,15 +562,107 @@ namespace diagonalization
             else if (auto* p2 = Constraint.target<LinearMomentumSector2D>()) s = p2->Descriptor();
             else if (auto* p1 = Constraint.target<LinearMomentumSector1D>()) s = p1->Descriptor();
             else if (Constraint.target<NoConstraint>()) s = NoConstraint().Descriptor();
-            else {
-                std::cerr << "\n\tERROR in GenerateFockSpace: the constraint must be a sector descriptor "
-                             "(AngularMomentumSector, LinearMomentumSector2D, LinearMomentumSector1D, "
-                             "NoConstraint); a device basis cannot evaluate a host predicate." << std::endl;
+            else return GenerateFockSpaceOnHost(nbrParticles, nbrOrbitals, Constraint, mpi);
+            return GenerateFockSpace(nbrParticles, nbrOrbitals, s, mpi);
+        }
+
+        // An opaque predicate (the reference's callers pass std::bind of
+        // TestAngularMomentumSector / TestLinearMomentumSector2, or lambdas):
+        // every N-particle state in ascending order (Gosper's successor, as
+        // fermion_fock_basis.cpp:266-307) is tested on the host.  When the kept
+        // states are exactly one of the sectors the device generates (no
+        // constraint; L_z; k_y on a dimX x dimY grid; (k_x, k_y)), that device
+        // basis is used; otherwise the list is uploaded as it is
+        // (hd_basis_create_from_states) and the Hamiltonian takes the general
+        // build.  A hint names the path taken.
+        bool GenerateFockSpaceOnHost(const iSize_t nbrParticles, const iSize_t nbrOrbitals,
+                                     const std::function<bool(const fock_t state)>& Constraint,
+                                     utilities::MpiWrapper& mpi)
+        {
+            Clear();
+            if (nbrParticles == 0 || nbrParticles > nbrOrbitals || nbrOrbitals > 64 || nbrParticles >= 64) {
+                std::cerr << "\n\tERROR in GenerateFockSpace: invalid particle or orbital count" << std::endl;
                 mpi.m_exitFlag = true;
                 mpi.ExitFlagTest();
                 return true;
             }
-            return GenerateFockSpace(nbrParticles, nbrOrbitals, s, mpi);
+            std::vector<fock_t> kept;
+            const fock_t last = (nbrParticles == 64 ? ~(fock_t)0 : (((fock_t)1 << nbrParticles) - 1))
+                                << (nbrOrbitals - nbrParticles);
+            for (fock_t x = ((fock_t)1 << nbrParticles) - 1;;) {
+                if (Constraint(x)) kept.push_back(x);
+                if (x == last) break;
+                const fock_t c = x & (0 - x), r = x + c;
+                x = (((r ^ x) >> 2) / c) | r;
+            }
+            if (kept.empty()) return true;
+            // the sectors a device basis can have for these states
+            struct Candidate {
+                hd_sector sector;
+                std::function<bool(fock_t)> test;
+                std::string name;
+            };
+            std::vector<Candidate> cand;
+            if ((fock_t)kept.size() == GetFullFockSpaceDimension(nbrParticles, nbrOrbitals))
+                cand.push_back({NoConstraint().Descriptor(), NoConstraint(), "no constraint"});
+            int lz = 0;
+            for (iSize_t k = 0; k < nbrOrbitals; ++k)
+                if (kept[0] >> k & 1) lz += 2 * (int)k - (int)(nbrOrbitals - 1);
+            const AngularMomentumSector am(lz, (int)nbrOrbitals - 1);
+            cand.push_back({am.Descriptor(), am, "AngularMomentumSector"});
+            for (iSize_t dimY = 1; dimY <= nbrOrbitals; ++dimY) {
+                if (nbrOrbitals % dimY) continue;
+                const int dimX = (int)(nbrOrbitals / dimY);
+                int kx = 0, ky = 0;
+                for (iSize_t k = 0; k < nbrOrbitals; ++k)
+                    if (kept[0] >> k & 1) {
+                        kx += (int)(k / dimY);
+                        ky += (int)(k % dimY);
+                    }
+                const LinearMomentumSector1D s1(ky % (int)dimY, (int)dimY);
+                const LinearMomentumSector2D s2(kx % dimX, ky % (int)dimY, dimX, (int)dimY);
+                cand.push_back({s1.Descriptor(), s1, "LinearMomentumSector1D"});
+                cand.push_back({s2.Descriptor(), s2, "LinearMomentumSector2D"});
+            }
+            for (const auto& c : cand) {
+                // every kept state must lie in the sector before its device
+                // basis is generated and compared
+                bool inside = true;
+                for (fock_t x : kept)
+                    if (!c.test(x)) {
+                        inside = false;
+                        break;
+                    }
+                if (!inside) continue;
+                hd_basis* raw = nullptr;
+                if (hd_basis_create(&c.sector, nbrParticles, nbrOrbitals, &raw) != HD_OK) continue;
+                std::shared_ptr<hd_basis> dev(raw, utilities::detail::BasisDeleter());
+                int64_t dim = 0;
+                hd_basis_dim(raw, &dim);
+                if ((size_t)dim != kept.size()) continue;
+                std::vector<fock_t> states((size_t)dim);
+                if (hd_basis_copy_states(raw, 0, dim, states.data()) != HD_OK || states != kept) continue;
+                std::cerr << "\n\tHINT GenerateFockSpace: the constraint selects the states of a device sector ("
+                          << c.name << "); pass the descriptor to skip the host enumeration." << std::endl;
+                m_device = dev;
+                m_basis.swap(states);
+                m_basisStored = true;
+                return false;
+            }
+            std::cerr << "\n\tHINT GenerateFockSpace: the constraint was evaluated on the host for "
+                      << kept.size() << " states; the Hamiltonian takes the general build." << std::endl;
+            hd_basis* raw = nullptr;
+            if (!utilities::detail::Ok(hd_basis_create_from_states(nbrParticles, nbrOrbitals, kept.data(),
+                                                                   (int64_t)kept.size(), &raw),
+                                       "GenerateFockSpace")) {
+                mpi.m_exitFlag = true;
+                mpi.ExitFlagTest();
+                return true;
+            }
+            m_device.reset(raw, utilities::detail::BasisDeleter());
+            m_basis.swap(kept);
+            m_basisStored = true;
+            return false;
         }
 
         bool GenerateFockSpace(const iSize_t nbrParticles, const iSize_t nbrOrbitals, const hd_sector& sector,

@@ -453,6 +453,79 @@ int hd_basis_create(const hd_sector* sector, uint32_t N, uint32_t Norb, hd_basis
     return HD_OK;
 }
 
+int hd_basis_create_from_states(uint32_t N, uint32_t Norb, const uint64_t* h_states, int64_t dim, hd_basis** out)
+{
+    if (!out || (dim > 0 && !h_states)) return fail(HD_ERR_ARG, "hd_basis_create_from_states: null argument");
+    *out = nullptr;
+    if (N == 0 || Norb == 0) return fail(HD_ERR_ARG, "MUST SPECIFY nbr particles > 0 and nbr orbitals > 0");
+    if (N >= 64) return fail(HD_ERR_ARG, "CANNOT ADDRESS 64 PARTICLES OR MORE");
+    if (N > Norb) return fail(HD_ERR_ARG, "NBR PARTICLES > NBR ORBITALS - NOT ALLOWED");
+    if (Norb > 64) return fail(HD_ERR_ARG, "at most 64 orbitals fit a fock_t");
+    if (dim < 0) return fail(HD_ERR_ARG, "negative dimension");
+    if (dim == 0) return fail(HD_ERR_EMPTY, "FOCK BASIS IS ZERO DIMENSIONAL");
+    if (dim > (int64_t)0x7FFFFFFE) return fail(HD_ERR_UNSUPPORTED, "more states than int32 columns address");
+    for (int64_t i = 0; i < dim; ++i) {
+        const uint64_t s = h_states[i];
+        if (__builtin_popcountll(s) != (int)N || (Norb < 64 && (s >> Norb) != 0))
+            return fail(HD_ERR_ARG, "state " + std::to_string(i) + " does not have " + std::to_string(N) +
+                                        " particles in " + std::to_string(Norb) + " orbitals");
+        if (i > 0 && s <= h_states[i - 1]) return fail(HD_ERR_ARG, "states must be strictly ascending");
+    }
+    hd_basis* b = new hd_basis();
+    b->sector = hd_sector{HD_SECTOR_LIST, 0, 0, 0, 0};
+    b->N = (int)N;
+    b->Norb = (int)Norb;
+    b->dim = dim;
+    auto cleanup = [&]() {
+        dfree(b->d_rank);
+        dfree(b->d_states);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+    };
+    if (hipGetDevice(&b->device) != hipSuccess) {
+        delete b;
+        return fail(HD_ERR_HIP, "hipGetDevice failed");
+    }
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        cleanup();
+        return fail(HD_ERR_HIP, "stream creation failed");
+    }
+    int st = dalloc_t(&b->d_states, (size_t)dim, "basis states");
+    if (st != HD_OK) {
+        cleanup();
+        return st;
+    }
+    // header (kRankList), then the states' address and count; the offset,
+    // group and count sections of the other kinds are empty
+    RankTable& rt = b->rt;
+    rt.N = (int)N;
+    rt.Norb = (int)Norb;
+    rt.G = 0;
+    rt.is_lz = kRankList;
+    rt.dim = (uint64_t)dim;
+    rt.empty = false;
+    const uint64_t addr = reinterpret_cast<uint64_t>(b->d_states);
+    rt.blob = {(int32_t)N, (int32_t)Norb, 0, 0, kRankList, 0, (int32_t)(uint32_t)addr, (int32_t)(uint32_t)(addr >> 32),
+               (int32_t)(uint32_t)(uint64_t)dim, (int32_t)(uint32_t)((uint64_t)dim >> 32)};
+    // rank_view lays the sections out after the header from N and Norb: pad
+    // the blob so that every section pointer it forms stays inside it
+    const size_t npc = ((size_t)Norb + 1) * ((size_t)N + 1);
+    rt.blob.resize(std::max(rt.blob.size(), kRankHeader + 3 * npc + (size_t)Norb + 1), 0);
+    if ((st = dalloc(reinterpret_cast<void**>(&b->d_rank), rt.bytes(), "rank table")) != HD_OK) {
+        cleanup();
+        return st;
+    }
+    if (hipMemcpyAsync(b->d_states, h_states, (size_t)dim * sizeof(uint64_t), hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(b->d_rank, rt.blob.data(), rt.bytes(), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess) {
+        cleanup();
+        return fail(HD_ERR_HIP, "state list upload failed");
+    }
+    *out = b;
+    return HD_OK;
+}
+
 int hd_basis_destroy(hd_basis* b)
 {
     if (!b) return HD_OK;

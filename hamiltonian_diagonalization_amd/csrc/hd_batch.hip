@@ -876,6 +876,8 @@ int hd_batch_create(const hd_basis* const* bases, int32_t n_sectors, const hd_ta
     }
     for (int s = 0; s < n_sectors; ++s) {
         if (!bases[s]) return fail(HD_ERR_ARG, "null basis");
+        if (bases[s]->sector.kind == HD_SECTOR_LIST)
+            return fail(HD_ERR_UNSUPPORTED, "batched solves take sector bases (not state lists)");
         if (bases[s]->N != bases[0]->N || bases[s]->Norb != bases[0]->Norb || bases[s]->Norb != t0->K)
             return fail(HD_ERR_ARG, "batch sectors must share N and the orbital count (= kMax)");
     }

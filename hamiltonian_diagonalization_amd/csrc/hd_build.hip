@@ -67,6 +67,44 @@ static cd get_v(const hd_tables* t, int k1, int k2, int k3, int k4)
 
 static double absv(const cd& v, bool cplx) { return cplx ? std::abs(v) : std::fabs(v.real()); }
 
+// The array tables as one-k1 lists (ConvertTableFormat's SetQuarticFromArray /
+// SetQuadraticFromArray, term_hash_tables_base.hpp:231-275): k1 = GetK1 and
+// V = GetVkkkk for every (k2,k3,k4), none where the sphere table has no k1;
+// k1 = k2 with E_k for the quadratic terms.
+static void array_to_lists(const hd_tables* t, hd_tables& out)
+{
+    const int K = t->K;
+    const size_t w = t->is_complex ? 2 : 1, ne = (size_t)K * K * K;
+    out = hd_tables();
+    out.layout = HD_LAYOUT_HASH;
+    out.K = K;
+    out.is_complex = t->is_complex;
+    out.max_k1 = 1;
+    out.nk1.assign(ne, 0);
+    out.k1.assign(ne, 0);
+    out.hv.assign(ne * w, 0.0);
+    for (int k4 = 0; k4 < K; ++k4)
+        for (int k3 = 0; k3 < K; ++k3)
+            for (int k2 = 0; k2 < K; ++k2) {
+                const size_t e = ((size_t)k4 * K + k3) * K + k2;
+                const int k1 = get_k1(t, k2, k3, k4);
+                if (k1 < 0) continue;
+                const cd v = get_v(t, k1, k2, k3, k4);
+                out.nk1[e] = 1;
+                out.k1[e] = (uint8_t)k1;
+                out.hv[e * w] = v.real();
+                if (w == 2) out.hv[e * w + 1] = v.imag();
+            }
+    if (!t->eps.empty()) {
+        out.has_quad = 1;
+        out.max_q1 = 1;
+        out.nq1.assign((size_t)K, 1);
+        out.q1.resize((size_t)K);
+        for (int k = 0; k < K; ++k) out.q1[(size_t)k] = (uint8_t)k;
+        out.he = t->eps;
+    }
+}
+
 // Device-side derived tables for one build.
 struct Derived {
     int K = 0, npairs = 0, cplx = 0, has_eps = 0, split = 0;
@@ -1186,6 +1224,12 @@ int hd_ham_build(const hd_basis* b, const hd_tables* t, int32_t storage, double 
     if (t->K != b->Norb)
         return fail(HD_ERR_UNSUPPORTED, "kMax (" + std::to_string(t->K) + ") must equal the number of orbitals (" +
                                             std::to_string(b->Norb) + ")");
+    hd_tables lists;  // array tables read as one-k1 lists, for a basis given as a state list
+    if (b->sector.kind == HD_SECTOR_LIST && t->layout != HD_LAYOUT_HASH) {
+        if (t->split) return fail(HD_ERR_UNSUPPORTED, "two-level tables need a two-level sector basis");
+        array_to_lists(t, lists);
+        t = &lists;
+    }
     if (t->layout == HD_LAYOUT_HASH) {
         if (storage == HD_STORAGE_MATRIX_FREE)
             return fail(HD_ERR_UNSUPPORTED, "hash term tables: full or upper storage only");

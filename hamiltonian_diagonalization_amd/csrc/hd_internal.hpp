@@ -131,6 +131,10 @@ int sector_weight_delta_ok(const RankTable& rt, const hd_sector& sec, int a, int
                            int d);
 
 constexpr int kRankHeader = 6;
+// is_lz value of a basis given as a list of states (HD_SECTOR_LIST): the
+// blob is the header followed by the device address of the ascending states
+// (2 words) and their count (2 words); ranking is a binary search.
+constexpr int kRankList = 2;
 
 // ------------------------------------------------------------ device side
 
@@ -164,6 +168,8 @@ struct RankView {
     const int32_t* sub;
     const uint32_t* tab;
     int N, Norb, G, target, is_lz;
+    const uint64_t* list;  // kRankList: the states, ascending
+    int64_t list_dim;
 };
 
 HD_DEV RankView rank_view(const int32_t* blob)
@@ -179,7 +185,30 @@ HD_DEV RankView rank_view(const int32_t* blob)
     v.gmm = v.off + npc;
     v.sub = v.gmm + 2 * npc;
     v.tab = reinterpret_cast<const uint32_t*>(v.sub + (v.is_lz ? v.Norb : v.G * v.Norb));
+    v.list = nullptr;
+    v.list_dim = 0;
+    if (v.is_lz == kRankList) {
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(blob + kRankHeader);
+        v.list = reinterpret_cast<const uint64_t*>(((uint64_t)w[1] << 32) | w[0]);
+        v.list_dim = (int64_t)(((uint64_t)w[3] << 32) | w[2]);
+    }
     return v;
+}
+
+// Binary search of s in the list of a kRankList basis.
+HD_DEV bool list_rank(const RankView& v, uint64_t s, uint64_t& r_out)
+{
+    int64_t lo = 0, hi = v.list_dim;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (v.list[mid] < s) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < v.list_dim && v.list[lo] == s) {
+        r_out = (uint64_t)lo;
+        return true;
+    }
+    return false;
 }
 
 template <bool LZ>
@@ -203,6 +232,7 @@ HD_DEV int group_sub(const RankView& v, int g, int k)
 template <bool LZ>
 HD_DEV bool rank_state(const RankView& v, uint64_t s, uint64_t& r_out)
 {
+    if (v.is_lz == kRankList) return list_rank(v, s, r_out);
     if (__popcll(s) != v.N) return false;
     if (v.Norb < 64 && (s >> v.Norb) != 0) return false;
     uint64_t r = 0;

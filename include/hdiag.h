@@ -47,12 +47,14 @@ typedef enum {
                            (linear_momentum_constraints_2D.cpp:38-79) */
     HD_SECTOR_K1D = 3,  /* TestLinearMomentumSector1(state, a=ky, b=dimY)
                            (linear_momentum_constraints_2D.cpp:87-110) */
-    HD_SECTOR_LZ2 = 4   /* TestAngularMomentumSector2LL(state, a=lzTot, b=max) with
+    HD_SECTOR_LZ2 = 4,  /* TestAngularMomentumSector2LL(state, a=lzTot, b=max) with
                            c = n1 first-level orbitals
                            (two_level_angular_momentum_constraint.hpp:41-73), on the
                            combined state (m_first << (n1+2)) | m_second: the
                            TwoLevelFockState order is ascending combined order
                            (two_level_fermion_fock_basis.cpp:40-45) */
+    HD_SECTOR_LIST = 5  /* the states of any std::function predicate, enumerated by
+                           the caller (hd_basis_create_from_states) */
 } hd_sector_kind;
 
 typedef struct {
@@ -124,6 +126,16 @@ int hd_device_count(int* n);
  * zero-dimensional sector, mirroring the reference's `true` return. */
 int hd_basis_create(const hd_sector* sector, uint32_t n_particles, uint32_t n_orbitals,
                     hd_basis** out);
+/* FermionFockBasis::GenerateFockSpace with a predicate the device cannot
+ * evaluate (fermion_fock_basis.cpp:266-307 walks every N-particle state and
+ * keeps those the std::function accepts): the caller passes the kept states,
+ * ascending, each with n_particles bits below n_orbitals.  The basis ranks by
+ * binary search (FindFockStateIndex, fermion_fock_basis.cpp:134-161).
+ * hd_ham_build over such a basis runs the general build (array tables are
+ * read as one-k1 lists), full or upper storage, FP64/FP32 values; out states
+ * that are not in the list are skipped, as the reference's search miss. */
+int hd_basis_create_from_states(uint32_t n_particles, uint32_t n_orbitals, const uint64_t* h_states, int64_t dim,
+                                hd_basis** out);
 int hd_basis_destroy(hd_basis* b);
 /* FermionFockBasis::GetStoredDimension (fermion_fock_basis.cpp:222-225). */
 int hd_basis_dim(const hd_basis* b, int64_t* dim);

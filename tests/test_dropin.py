@@ -435,3 +435,39 @@ def test_dropin_wannier_basis(oracle, tmp_path):
         bloch.extend(rb["eigenvalues"])
     say("wannier", w["eigenvalues"], "bloch", sorted(bloch)[:4])
     assert w["ok"] and np.max(np.abs(np.array(w["eigenvalues"]) - np.sort(bloch)[:4])) < 1e-9
+
+
+@pytest.mark.gpu
+def test_dropin_opaque_predicates(oracle, tmp_path):
+    """GenerateFockSpace with the std::function the reference's callers pass
+    (pseudopotential_model.cpp:185-187): a bound L_z test is recognised as the
+    device sector (the same upper CRS as --lz, and a hint); a predicate no
+    sector reproduces is enumerated on the host and built by the general
+    build, bit for bit the oracle's over that state list."""
+    n, two_s = 6, 15
+    K = two_s + 1
+    vm = [0.0, 1.0, 0.0, 0.45]
+    common = ["--nbr-particles", n, "--nbr-orbitals", K, "--pseudopotentials", pseudo_arg(vm),
+              "--nbr-eigenvalues", 3, "--tol", 1e-12, "--dense-below", 10]
+    say("drop-in opaque predicates, N=6 2S=15")
+    p0 = run(common + ["--lz", 0, "--dump-crs", str(tmp_path / "a")])
+    p1 = run(common + ["--sector", f"bind-lz:0,{two_s}", "--dump-crs", str(tmp_path / "b")])
+    assert "AngularMomentumSector" in p1.stderr
+    for ext in (".rowstarts", ".cols", ".values"):
+        assert (tmp_path / ("a" + ext)).read_bytes() == (tmp_path / ("b" + ext)).read_bytes()
+    # the product's schedule is autotuned per run: equal to rounding
+    assert np.max(np.abs(np.array(last_json(p0)["eigenvalues"]) - np.array(last_json(p1)["eigenvalues"]))) < 1e-12
+    mask = 0b1001
+    p2 = run(common + ["--sector", f"exclude:0,{two_s},{mask}", "--dump-crs", str(tmp_path / "c"),
+                       "--dump-basis", str(tmp_path / "basis")])
+    assert "general build" in p2.stderr
+    r2 = last_json(p2)
+    full = oracle.basis(oracle.sector(oracle.SECTOR_LZ, 0, two_s), n, K)
+    bas = full[(full & mask) == 0]
+    assert np.array_equal(np.fromfile(str(tmp_path / "basis"), dtype=np.uint64), bas)
+    up = oracle.build(n, K, bas, oracle.LAYOUT_SPHERE, K, hd.sphere_ktable(two_s), hd.sphere_vtable(two_s, vm))
+    assert np.array_equal(np.fromfile(str(tmp_path / "c.rowstarts"), dtype=np.int32), up.rowptr.astype(np.int32))
+    assert np.array_equal(np.fromfile(str(tmp_path / "c.cols"), dtype=np.int32), up.col)
+    assert np.array_equal(np.fromfile(str(tmp_path / "c.values"), dtype=np.float64), up.values())
+    w = np.linalg.eigvalsh(up.to_dense(len(bas)))[:3]
+    assert r2["ok"] and np.max(np.abs(np.array(r2["eigenvalues"]) - w)) < 1e-10

@@ -232,3 +232,45 @@ def test_wannier_basis_run_5x5(oracle):
     bloch = np.sort(bloch)[:nev]
     say("wannier evals", np.asarray(w), "bloch", bloch)
     assert np.abs(np.asarray(w) - bloch).max() < 1e-8
+
+
+@pytest.mark.parametrize("case", ["sphere", "ofl"])
+def test_state_list_basis(oracle, case):
+    """hd_basis_create_from_states: a basis no sector descriptor reproduces
+    (the L_z or momentum sector without the states occupying two orbitals);
+    ranking by binary search, the upper CRS bit-exact against the oracle's
+    build over that list (out states outside it skipped), full storage its
+    Hermitian completion, the product and the spectrum against the oracle."""
+    if case == "sphere":
+        n, two_s = 7, 17
+        K = two_s + 1
+        full = oracle.basis(oracle.sector(oracle.SECTOR_LZ, 1, two_s), n, K)
+        kt, vt, eps, lay = hd.sphere_ktable(two_s), hd.sphere_vtable(two_s, [0.0, 1.0, 0.0, 0.3]), None, hd.LAYOUT_SPHERE
+        olay, cplx = oracle.LAYOUT_SPHERE, False
+    else:
+        X, Y, n = 4, 3, 5
+        K = X * Y
+        full = oracle.basis(oracle.sector(oracle.SECTOR_K2D, 1, 1, X, Y), n, K)
+        kt, vt, eps = models.ofl_tables(X, Y, seed=2)
+        lay, olay, cplx = hd.LAYOUT_OFL, oracle.LAYOUT_OFL, True
+    mask = (1 << 2) | (1 << 5)
+    bas = full[(full & mask) == 0]
+    say("state-list basis", case, "dim", len(bas), "of", len(full))
+    basis = hd.FockBasis.from_states(bas, n, K)
+    assert basis.dim == len(bas) and np.array_equal(basis.states(), bas)
+    probe = np.concatenate([bas[::5], full[(full & mask) != 0][:7]])
+    idx = basis.find_index(probe)
+    assert np.array_equal(idx[:len(bas[::5])], np.arange(0, len(bas), 5))
+    assert np.all(idx[len(bas[::5]):] == np.iinfo(np.uint64).max)
+    tables = hd.TermTables(lay, K, kt, vt, eps)
+    up = oracle.build(n, K, bas, olay, K, kt, vt, eps=eps, is_complex=cplx, full=False)
+    hu = hd.Hamiltonian(basis, tables, storage=hd.STORAGE_UPPER_REF)
+    _same_bits(hu, up, f32=False)
+    hf = hd.Hamiltonian(basis, tables)
+    _same_bits(hf, oracle.mirror_upper(up))
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=len(bas)) + (1j * rng.normal(size=len(bas)) if cplx else 0)
+    want = oracle.spmv_sym(up, x)
+    assert np.abs(hf.spmv(x) - want).max() <= 1e-12 * np.abs(want).max()
+    w = np.linalg.eigvalsh(up.to_dense(len(bas)))[:4]
+    assert np.abs(np.asarray(hf.lanczos(nev=4, tol=1e-10).evals) - w).max() < 1e-9
