@@ -342,6 +342,7 @@ struct hd_hamiltonian {
     int tg_rows = 0;  // rows per group (128, or 64 for denser matrices)
     int tg_entry_bytes = 4;  // 4, or 3 for packed entries (dictionary <= 2047 values)
     int64_t tg_win = 0;      // block order of the tiled product (groups per XCD window)
+    size_t tg_bytes = 0;     // HBM of the tiled layout, included in `bytes`
     int64_t* d_tg_base = nullptr;  // entry, line and offset-table bases, 3 x (groups + 1)
     uint32_t* d_tg_entries = nullptr;
     uint32_t* d_tg_lines = nullptr;

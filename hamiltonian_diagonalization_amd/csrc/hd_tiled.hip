@@ -739,8 +739,9 @@ static int tiled_build_tg(hd_hamiltonian* h, int entry_bytes)
     }
     h->tg_rows = TG;
     h->tg_entry_bytes = entry_bytes;
-    h->bytes += (size_t)h->tg_entries * entry_bytes + (size_t)h->tg_lines * 4 + (size_t)h->tg_offs * 4 + base.size() * 8 +
-                (size_t)h->dim * 8 * W;
+    h->tg_bytes = (size_t)h->tg_entries * entry_bytes + (size_t)h->tg_lines * 4 + (size_t)h->tg_offs * 4 +
+                  base.size() * 8 + (size_t)h->dim * 8 * W;
+    h->bytes += h->tg_bytes;
     return tiled_pick_window(h);
 }
 
@@ -786,6 +787,8 @@ void tiled_free(hd_hamiltonian* h)
     h->d_tg_lines = nullptr;
     h->d_tg_offs = nullptr;
     h->d_tg_diag = nullptr;
+    h->bytes -= std::min(h->bytes, h->tg_bytes);  // the layout's share of the handle's HBM
+    h->tg_bytes = 0;
     h->tg_groups = h->tg_entries = h->tg_lines = h->tg_offs = 0;
     h->tg_rows = 0;
     h->tg_entry_bytes = 4;
