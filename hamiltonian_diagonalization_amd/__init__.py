@@ -670,8 +670,9 @@ class Hamiltonian:
     def tile(self, rows_per_group: int = 0, entry_bytes: int = 0) -> dict:
         """Builds the tiled layout of the coded operator (precision TILED);
         rows_per_group 128 or 64, 0 = 128 with a fallback to 64; entry_bytes
-        4 (0) or 3 (packed; real matrices with dictionaries of at most 2047
-        values).  Real and complex matrices."""
+        3 (packed, read as 12-byte quads: real matrices with dictionaries of
+        at most 2047 values), 4, or 0 = 3 where it applies.  Real and complex
+        matrices."""
         _check(lib().hd_ham_tile(self.h, rows_per_group, entry_bytes))
         return self.tile_info()
 

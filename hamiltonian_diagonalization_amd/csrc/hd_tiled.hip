@@ -39,6 +39,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "hd_internal.hpp"
@@ -121,8 +122,10 @@ __device__ int row_of(const TilePlanLds<TG>& S, int nrows, int64_t j)
 }
 
 // Builds the plan in LDS; S.flags != 0 when the group does not fit.
-// LS: log2 of the x values per 128-byte line (4 real, 3 complex).
-template <int TG, int LS>
+// LS: log2 of the x values per 128-byte line (4 real, 3 complex).  QA: every
+// (chunk, row) block is padded to a multiple of 4 entries (the 3-byte layout,
+// read as 12-byte quads).
+template <int TG, int LS, bool QA>
 __device__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const int64_t* __restrict__ rowptr,
                           const int32_t* __restrict__ col, const uint16_t* __restrict__ code, int64_t g)
 {
@@ -198,7 +201,7 @@ __device__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const int64_t* __rest
         uint32_t run = 0;
         for (int r = 0; r < TG; ++r) {
             S.T[t * (TG + 1) + r] = run;
-            run += S.counts[t * TG + r];
+            run += QA ? (S.counts[t * TG + r] + 3u) & ~3u : S.counts[t * TG + r];
         }
         S.T[t * (TG + 1) + TG] = run;  // chunk size, made absolute below
     }
@@ -225,7 +228,7 @@ __device__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const int64_t* __rest
 }
 
 // Pass 1: per group, the distinct lines, the chunks and the padded entries.
-template <int TG, int LS>
+template <int TG, int LS, bool QA>
 __global__ void __launch_bounds__(kBuildThreads)
 k_tile_count(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
              const uint16_t* __restrict__ code, int64_t* __restrict__ n_lines, int64_t* __restrict__ n_entries,
@@ -233,7 +236,7 @@ k_tile_count(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __r
 {
     __shared__ TilePlanLds<TG> S;
     const int64_t g = blockIdx.x;
-    tile_plan<TG, LS>(S, dim, rowptr, col, code, g);
+    tile_plan<TG, LS, QA>(S, dim, rowptr, col, code, g);
     if (threadIdx.x == 0) {
         n_lines[g] = S.n_lines;
         n_chunks[g] = S.n_chunks;
@@ -245,7 +248,7 @@ k_tile_count(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __r
 // Pass 2: the line list, the chunk table (T per chunk, then the chunk line
 // starts), the entries and the diagonal (W doubles per value: 1 real, 2
 // complex; a complex line holds 8 values, so its word field uses 3 bits).
-template <int TG, int W>
+template <int TG, int W, bool QA>
 __global__ void __launch_bounds__(kBuildThreads)
 k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
             const uint16_t* __restrict__ code, const double* __restrict__ diag, const int64_t* __restrict__ base,
@@ -256,7 +259,7 @@ k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __re
     __shared__ TilePlanLds<TG> S;
     const int t = threadIdx.x;
     const int64_t g = blockIdx.x;
-    tile_plan<TG, LS>(S, dim, rowptr, col, code, g);
+    tile_plan<TG, LS, QA>(S, dim, rowptr, col, code, g);
     const int64_t r0 = g * TG, r1 = min(r0 + TG, dim);
     const int nrows = (int)(r1 - r0);
     const int64_t eb = base[g], en = base[g + 1] - eb;
@@ -270,11 +273,17 @@ k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __re
     for (int i = t; i < L; i += kBuildThreads) lines[lb + i] = S.sorted[i];
     for (int i = t; i < nch * (TG + 1); i += kBuildThreads) meta[mb + i] = S.T[i];
     for (int i = t; i <= nch; i += kBuildThreads) meta[mb + nch * (TG + 1) + i] = S.ls[i];
-    // padding entries never read by a row: kEmpty
+    // padding entries: kEmpty (never read, or read and skipped as a quad's tail)
     for (int c = 0; c < nch; ++c) {
         const uint32_t a = S.T[c * (TG + 1) + TG], b = S.cbase[c + 1];
         if (t < (int)(b - a)) entries[eb + a + t] = kEmpty;
     }
+    if (QA)
+        for (int i = t; i < nch * TG; i += kBuildThreads) {
+            const int c = i / TG, r = i % TG;
+            const uint32_t a = S.T[c * (TG + 1) + r] + S.counts[c * TG + r], b = S.T[c * (TG + 1) + r + 1];
+            for (uint32_t p = a; p < b && (int64_t)p < en; ++p) entries[eb + p] = kEmpty;
+        }
     const int64_t e0 = S.rp[0], e1 = S.rp[nrows];
     for (int64_t j = e0 + t; j < e1; j += kBuildThreads) {
         const uint32_t q = code[j];
@@ -357,13 +366,19 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
     const int nmeta = (int)min(me - mb, (int64_t)kMeta);
     const int nch = (nmeta - 1) / (TG + 2);
     // packed: 3 bytes per entry; a group starts at a multiple of 4 entries,
-    // i.e. at dword 3 * base[g] / 4
+    // i.e. at dword 3 * base[g] / 4, and every (chunk, row) block holds a
+    // multiple of 4 entries, read as one 12-byte item per 4 entries
     const uint32_t* ent = entries + (PK ? base[g] / 4 * 3 : base[g]);
-    auto ent_at = [&](uint32_t j) -> uint32_t {
-        if (!PK) return ent[j];
-        const uint32_t b = 3u * j, k = b >> 2, sh = (b & 3u) * 8u;
-        const uint64_t w = (uint64_t)ent[k] | ((uint64_t)ent[k + 1] << 32);
-        return (uint32_t)(w >> sh) & 0xFFFFFFu;
+    using Item = typename std::conditional<PK, uint3, uint32_t>::type;
+    constexpr int kPer = PK ? 4 : 1;        // entries per item
+    constexpr int kUI = kU / kPer;          // items each lane has in flight
+    auto item_at = [&](uint32_t q) -> Item {
+        if constexpr (PK) return *reinterpret_cast<const uint3*>(ent + 3u * q);
+        else return ent[q];
+    };
+    auto no_item = []() -> Item {
+        if constexpr (PK) return make_uint3(~0u, ~0u, ~0u);
+        else return kEmpty;
     };
     const int r = threadIdx.x / kTPR, sub = threadIdx.x % kTPR;
     const int64_t row = g * TG + r;
@@ -373,14 +388,13 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
     __syncthreads();
     const uint32_t* T = sm;
     const uint32_t* LS = sm + nch * (TG + 1);
-    // 4-byte entries keep two chunks in flight: while chunk c is consumed,
-    // the lines and first entries of chunks c+1 and c+2 are loading into
-    // registers (3.36 against 3.44 ms with one chunk ahead).  The packed
-    // entries' unpacking needs the registers, so they keep one chunk ahead.
-    constexpr int kAhead = PK ? 1 : 2;
+    // two chunks in flight: while chunk c is consumed, the lines and first
+    // entries of chunks c+1 and c+2 are loading into registers (3.36 against
+    // 3.44 ms with one chunk ahead, 4-byte entries)
+    constexpr int kAhead = 2;
     Piece preA[PER], preB[PER];
-    uint32_t eA[kU], eB[kU];
-    auto fetch = [&](int c, Piece (&pre)[PER], uint32_t (&e)[kU]) {
+    Item eA[kUI], eB[kUI];
+    auto fetch = [&](int c, Piece (&pre)[PER], Item (&e)[kUI]) {
         const int l0 = (int)LS[c], n_here = (int)LS[c + 1] - l0;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
@@ -391,22 +405,30 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
                 pre[k] = XP::load(x, xi, dim, x_aligned);
             }
         }
-        const uint32_t j0 = T[c * (TG + 1) + r] + sub, je = T[c * (TG + 1) + r + 1];
+        const uint32_t j0 = T[c * (TG + 1) + r] / kPer + sub, je = T[c * (TG + 1) + r + 1] / kPer;
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
+        for (int u = 0; u < kUI; ++u) {
             const uint32_t j = j0 + u * kTPR;
-            e[u] = j < je ? ent_at(j) : kEmpty;
+            e[u] = j < je ? item_at(j) : no_item();
         }
     };
-    auto consume = [&](const uint32_t (&b)[kU], double& a) {
+    auto consume = [&](const Item (&b)[kUI], double& a) {
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            if (b[u] == kEmpty) continue;
-            if (PK) {  // slot 16-23, word 12-15, sign 11, index 0-10
-                double v = sdict[b[u] & 0x7FFu];
-                if (b[u] & 0x800u) v = -v;
-                a = fma(v, (double)xs[(b[u] >> 16) * kLineWords + ((b[u] >> 12) & 15u)], a);
+        for (int u = 0; u < kUI; ++u) {
+            if constexpr (PK) {
+                // four little-endian 3-byte entries in 12 bytes: slot 16-23,
+                // word 12-15, sign 11, index 0-10; 0xFFFFFF pads a block
+                const uint32_t w[4] = {b[u].x & 0xFFFFFFu, (b[u].x >> 24) | ((b[u].y & 0xFFFFu) << 8),
+                                       (b[u].y >> 16) | ((b[u].z & 0xFFu) << 16), b[u].z >> 8};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (w[k] == 0xFFFFFFu) continue;
+                    double v = sdict[w[k] & 0x7FFu];
+                    if (w[k] & 0x800u) v = -v;
+                    a = fma(v, (double)xs[(w[k] >> 16) * kLineWords + ((w[k] >> 12) & 15u)], a);
+                }
             } else {
+                if (b[u] == kEmpty) continue;
                 double v = sdict[b[u] & 0x7FFFFu];
                 if (b[u] & 0x80000u) v = -v;
                 a = fma(v, (double)xs[(b[u] >> 24) * kLineWords + ((b[u] >> 20) & 15u)], a);
@@ -414,26 +436,26 @@ k_spmv_tiled(int64_t dim, int64_t groups, const int64_t* __restrict__ base, cons
         }
     };
     double acc = 0.0;
-    auto step = [&](int c, Piece (&pre)[PER], uint32_t (&e)[kU]) {
+    auto step = [&](int c, Piece (&pre)[PER], Item (&e)[kUI]) {
         __syncthreads();  // chunk c-1 consumed
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int i = threadIdx.x + k * kSpmvThreads;
             *reinterpret_cast<Piece*>(&xs[(i / PPL) * kLineWords + (i % PPL) * XP::kWords]) = pre[k];
         }
-        uint32_t cur[kU];
+        Item cur[kUI];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) cur[u] = e[u];
+        for (int u = 0; u < kUI; ++u) cur[u] = e[u];
         __syncthreads();
         if (c + kAhead < nch) fetch(c + kAhead, pre, e);
         consume(cur, acc);
         // rows with more than kTPR * kU entries in this chunk
-        const uint32_t je = T[c * (TG + 1) + r + 1];
-        for (uint32_t j0 = T[c * (TG + 1) + r] + sub + kTPR * kU; j0 < je; j0 += kTPR * kU) {
+        const uint32_t je = T[c * (TG + 1) + r + 1] / kPer;
+        for (uint32_t j0 = T[c * (TG + 1) + r] / kPer + sub + kTPR * kUI; j0 < je; j0 += kTPR * kUI) {
 #pragma unroll
-            for (int u = 0; u < kU; ++u) {
+            for (int u = 0; u < kUI; ++u) {
                 const uint32_t j = j0 + u * kTPR;
-                cur[u] = j < je ? ent_at(j) : kEmpty;
+                cur[u] = j < je ? item_at(j) : no_item();
             }
             consume(cur, acc);
         }
@@ -649,10 +671,13 @@ static int tiled_build_tg(hd_hamiltonian* h, int entry_bytes)
     if (rc == HD_OK) {
         HD_ENQ(q, hipMemsetAsync(d_flag, 0, 2 * sizeof(int), st));
         if (h->is_complex)
-            hipLaunchKernelGGL((k_tile_count<TG, 3>), dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim,
+            hipLaunchKernelGGL((k_tile_count<TG, 3, false>), dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim,
+                               h->d_rowptr, h->d_col, h->d_code, d_nl, d_ne, d_nc, d_flag);
+        else if (entry_bytes == 3)
+            hipLaunchKernelGGL((k_tile_count<TG, 4, true>), dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim,
                                h->d_rowptr, h->d_col, h->d_code, d_nl, d_ne, d_nc, d_flag);
         else
-            hipLaunchKernelGGL((k_tile_count<TG, 4>), dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim,
+            hipLaunchKernelGGL((k_tile_count<TG, 4, false>), dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim,
                                h->d_rowptr, h->d_col, h->d_code, d_nl, d_ne, d_nc, d_flag);
         HD_ENQ(q, hipGetLastError());
         HD_ENQ(q, hipMemcpyAsync(nl.data(), d_nl, G * sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -694,12 +719,13 @@ static int tiled_build_tg(hd_hamiltonian* h, int entry_bytes)
     if (rc == HD_OK) {
         HD_ENQ(q, hipMemcpyAsync(h->d_tg_base, base.data(), base.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
         HD_ENQ(q, hipMemsetAsync(d_flag + 1, 0, sizeof(int), st));
-#define HD_TF(WV)                                                                                                  \
-    hipLaunchKernelGGL((k_tile_fill<TG, WV>), dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim, h->d_rowptr,    \
-                       h->d_col, h->d_code, static_cast<const double*>(h->d_diag), h->d_tg_base, G, h->d_tg_entries, \
-                       h->d_tg_lines, h->d_tg_offs, h->d_tg_diag, d_flag + 1)
-        if (W == 2) HD_TF(2);
-        else HD_TF(1);
+#define HD_TF(WV, QAV)                                                                                             \
+    hipLaunchKernelGGL((k_tile_fill<TG, WV, QAV>), dim3((unsigned)G), dim3(kBuildThreads), 0, st, h->dim,            \
+                       h->d_rowptr, h->d_col, h->d_code, static_cast<const double*>(h->d_diag), h->d_tg_base, G,     \
+                       h->d_tg_entries, h->d_tg_lines, h->d_tg_offs, h->d_tg_diag, d_flag + 1)
+        if (W == 2) HD_TF(2, false);
+        else if (entry_bytes == 3) HD_TF(1, true);
+        else HD_TF(1, false);
 #undef HD_TF
         HD_ENQ(q, hipGetLastError());
         HD_ENQ(q, hipMemcpyAsync(flags + 1, d_flag + 1, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -755,10 +781,11 @@ int tiled_build(hd_hamiltonian* h, int rows_per_group, int entry_bytes)
         return fail(HD_ERR_UNSUPPORTED, "3-byte entries are implemented for real matrices");
     if (entry_bytes == 3 && h->ndict > 2047)
         return fail(HD_ERR_UNSUPPORTED, "3-byte entries need a dictionary of at most 2047 values");
-    // 4 by default: the packed entries cut HBM bytes by a quarter but need two
-    // dword loads and a shift each, and measured slower (N=12, 2S=33: 4.86 ms
-    // against 3.53 ms per product)
-    if (entry_bytes == 0) entry_bytes = 4;
+    // 3 where it applies: every (chunk, row) block is padded to 4 entries and
+    // read as one 12-byte item per 4 entries, a quarter fewer bytes than
+    // 4-byte entries (N=12, 2S=33: 3.09 against 3.42 ms per product; N=11,
+    // 2S=30: 0.414 against 0.464 ms)
+    if (entry_bytes == 0) entry_bytes = (!h->is_complex && h->ndict <= 2047) ? 3 : 4;
     if (!h->d_code) return fail(HD_ERR_ARG, "the tiled layout is built from the coded values (value_mask & 4)");
     if (h->storage != HD_STORAGE_FULL) return fail(HD_ERR_ARG, "the tiled layout needs full storage");
     if (h->ndict > (1 << 19)) return fail(HD_ERR_UNSUPPORTED, "dictionary too large for the tiled layout");
@@ -769,7 +796,10 @@ int tiled_build(hd_hamiltonian* h, int rows_per_group, int entry_bytes)
     tiled_free(h);
     if (h->dim == 0) return HD_OK;
     if (rows_per_group == 64) return tiled_build_tg<64>(h, entry_bytes);
-    // 128-row groups stage fewer lines per entry; denser matrices fall back to 64
+    // 128-row groups stage fewer lines per entry; denser matrices fall back to 64.
+    // (256-row groups stage 24% fewer lines per entry again, but measured no
+    // faster: N=11, 2S=30, 0.410 against 0.414 ms with 3-byte entries, 0.507
+    // against 0.464 ms with 4-byte entries)
     const int rc = tiled_build_tg<128>(h, entry_bytes);
     if (rc != HD_ERR_UNSUPPORTED || rows_per_group == 128) return rc;
     return tiled_build_tg<64>(h, entry_bytes);

@@ -239,7 +239,7 @@ def test_spmv_complex_large_dictionary(oracle):
         assert np.array_equal(h.spmv(e, precision=hd.CODED), h.spmv(e))
 
 
-@pytest.mark.parametrize("rows,nbytes", [(64, 4), (128, 0), (64, 3), (128, 3)])
+@pytest.mark.parametrize("rows,nbytes", [(64, 4), (128, 0), (64, 3), (128, 3), (128, 4)])
 @pytest.mark.parametrize("n,two_s,lz", [(4, 11, 0), (5, 13, 3), (6, 15, 0), (6, 15, 24)])
 def test_tiled_columns_exact(oracle, n, two_s, lz, rows, nbytes):
     """Every column H e_i through the tiled layout equals the oracle's matrix
@@ -255,9 +255,12 @@ def test_tiled_columns_exact(oracle, n, two_s, lz, rows, nbytes):
     h = hd.Hamiltonian(b, t, values=(hd.CODED,))
     ti = h.tile(rows, nbytes)
     assert ti["rows_per_group"] == rows and ti["groups"] == (b.dim + rows - 1) // rows
-    assert ti["entry_bytes"] == (nbytes or 4)
+    assert ti["entry_bytes"] == (nbytes or 3)
     offdiag = len(full.col) - int(np.sum(full.col == np.repeat(np.arange(b.dim), np.diff(full.rowptr))))
-    assert offdiag <= ti["entries"] <= offdiag + 3 * ti["chunks"] and ti["entries"] % 4 == 0
+    # 4-byte entries pad each chunk block to 4 entries, 3-byte entries each
+    # (chunk, row) block (read as 12-byte quads)
+    pad = 3 * ti["chunks"] * (rows if ti["entry_bytes"] == 3 else 1)
+    assert offdiag <= ti["entries"] <= offdiag + pad and ti["entries"] % 4 == 0
     M = full.to_dense(b.dim, mirror=False)
     cols = range(b.dim) if b.dim <= 1500 else np.random.default_rng(0).choice(b.dim, 600, replace=False)
     for i in cols:
@@ -275,7 +278,7 @@ def test_tiled_spmv_and_lanczos_config1(oracle):
     h64.tile(64)
     ti = h.tile()
     say(ti)
-    assert ti["rows_per_group"] == 128
+    assert ti["rows_per_group"] == 128 and ti["entry_bytes"] == 3
     rng = np.random.default_rng(4)
     x = rng.normal(size=b.dim)
     want = oracle.spmv_sym(up, x)
@@ -299,7 +302,7 @@ def test_tiled_block_orders_identical():
     h = hd.Hamiltonian(b, t, values=(hd.CODED,))
     ti = h.tile()
     groups = ti["groups"]
-    assert groups == (b.dim + 127) // 128 and ti["block_window"] >= 0
+    assert groups == (b.dim + ti["rows_per_group"] - 1) // ti["rows_per_group"] and ti["block_window"] >= 0
     x = np.random.default_rng(5).normal(size=b.dim)
     h.tile_order(0)
     want = h.spmv(x, precision=hd.TILED)

@@ -132,7 +132,7 @@ def test_tiled_headline_operator_config2(oracle):
     h = hd.Hamiltonian(b, t, values=(hd.F64, hd.CODED))
     ti = h.tile()
     say(ti)
-    assert ti["rows_per_group"] == 128 and ti["chunks"] >= 5 * ti["groups"]
+    assert ti["rows_per_group"] == 128 and ti["entry_bytes"] == 3 and ti["chunks"] >= 5 * ti["groups"]
     rows = {}
     for r0 in np.linspace(0, dim - 120, 6).astype(np.int64):
         ref = _oracle_rows(oracle, 12, 34, bas, t, int(r0), int(r0) + 120)
@@ -168,8 +168,8 @@ def _multichunk_case(oracle):
 @pytest.mark.parametrize("rows,nbytes", [(64, 4), (128, 4), (64, 3), (128, 3)])
 def test_tiled_multichunk_pipeline(oracle, rows, nbytes):
     """N=10, 2S=27 (dim 246,448): 64-row groups hold 3 chunks on average and
-    128-row groups 4, up to 4 and 6, so the loads of chunks two ahead (4-byte
-    entries) and one ahead (packed entries) run.  600 sampled columns equal
+    128-row groups 4, up to 4 and 6, so the loads of chunks two ahead run,
+    for 4-byte entries and for the packed entries read as 12-byte quads.  600 sampled columns equal
     the oracle's full CRS bit for bit and a random product equals the oracle's
     naive symmetric product to 1e-13."""
     say("tiled multi-chunk N=10 2S=27", rows, nbytes)
