@@ -367,7 +367,7 @@ def main():
         overlap = float(abs(v @ psi0) / (np.linalg.norm(v) * np.linalg.norm(psi0)))
         del v, psi0
         rl, dtl, tnl = solve("learned_fp64")
-        flop = 2.0 * (37 * 256 + 2 * 256 * 256 + 256 * 2) * basis.dim
+        flop = 2.0 * (37 * 256 + 2 * 256 * 256 + 256 * 2 * net.n_states) * basis.dim
         learned = {"checkpoint": os.path.basename(ckpt), "params_v1_v3_v5": params,
                    "note": "the Coulomb point lies outside the training box V1 in [0.5,1.5], V3 in [0,0.5], "
                            "V5 in [0,0.2]; heldout is inside it",

@@ -6,12 +6,12 @@ python -m hamiltonian_diagonalization_amd.profile_stages [--n 12 --two-s 33 --it
 Stages, in order, on the N=12, 2S=33, L_z=0 Coulomb sphere sector
 (dim 7,764,392): the basis (k_unrank), the full-storage build with FP64,
 FP32 and coded values (k_count, the scan, k_fill, k_sort), the tiled layout
-(k_tile_count, k_tile_fill, then its block-order autotune), `iters` tiled
-products with FP64 x and with FP32 x (one warm-up each), the amplitude
-network over the whole sector (k_gemm and its epilogues), and the plain
-streaming copy over 1 GiB `iters` times, which calibrates FETCH_SIZE /
-WRITE_SIZE against a known byte count.  Prints the algorithmic bytes (or
-flops) of every stage as one JSON line.
+(k_tile_count, k_tile_fill, k_tile_pack for 3-byte entries, then its
+block-order autotune), `iters` tiled products with FP64 x and with FP32 x (one
+warm-up each), the amplitude network over the whole sector (k_mlp_infer and
+the normalisation kernels), and the plain streaming copy over 1 GiB `iters`
+times, which calibrates FETCH_SIZE / WRITE_SIZE against a known byte count.
+Prints the algorithmic bytes (or flops) of every stage as one JSON line.
 """
 import argparse
 import ctypes
@@ -50,16 +50,22 @@ def main():
                                  "k_sort": 2 * nnz * (4 + 8 + 4 + 2) + dim * 8}
     tile = ham.tile()
     out["tile"] = tile
+    # the fill writes 4-byte entries; with 3-byte entries k_tile_pack reads
+    # them and writes 3 bytes each
     out["tile"]["alg_bytes"] = {"k_tile_count": nnz * 6 + dim * 8,
-                                "k_tile_fill": nnz * (6 + 4) + dim * 16 + 4 * (tile["lines"] + tile["chunks"] * 130)}
+                                "k_tile_fill": nnz * 6 + tile["entries"] * 4 + dim * 16 +
+                                4 * (tile["lines"] + tile["chunks"] * 130),
+                                "k_tile_pack": tile["entries"] * (4 + 3) if tile["entry_bytes"] == 3 else 0}
     out["spmv"] = {}
     for name, p in (("tiled", TILED), ("tiled_x32", TILED_X32)):
         ms = ham.spmv_time_ms(p, args.iters)
         out["spmv"][name] = {"ms": ms, "alg_bytes": ham.spmv_bytes(p)}
         print(f"[profile] spmv {name}: {ms:.3f} ms, {ham.spmv_bytes(p)} algorithmic bytes", flush=True)
-    ckpt = os.path.join(os.path.dirname(os.path.abspath(__file__)), "checkpoints",
-                        f"start_n{args.n}_2s{args.two_s}.npz")
-    if not args.skip_network and os.path.exists(ckpt):
+    ckdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "checkpoints")
+    ckpt = next((os.path.join(ckdir, f) for f in (f"start5_n{args.n}_2s{args.two_s}.npz",
+                                                  f"start_n{args.n}_2s{args.two_s}.npz")
+                 if os.path.exists(os.path.join(ckdir, f))), "")
+    if not args.skip_network and ckpt:
         from . import learned
         net = learned.AmplitudeNet.load(ckpt)
         for rep in range(2):  # the second pass is the measured one
@@ -68,9 +74,11 @@ def main():
             net.start_vector(basis, [vm[1], vm[3], vm[5]])
             _check(lib().hd_synchronize())
             dt = time.perf_counter() - t0
-        flop = 2.0 * (37 * 256 + 2 * 256 * 256 + 256 * 2) * dim
-        flop_padded = 2.0 * (64 * 256 + 2 * 256 * 256 + 256 * 64) * dim
-        out["network"] = {"s": dt, "states": dim, "flop": flop, "flop_padded": flop_padded}
+        S = net.n_states
+        flop = 2.0 * (37 * 256 + 2 * 256 * 256 + 256 * 2 * S) * dim
+        flop_padded = 2.0 * (64 * 256 + 2 * 256 * 256 + 256 * 32) * dim  # 32-wide output tile
+        out["network"] = {"s": dt, "states": dim, "amplitudes": S, "flop": flop, "flop_padded": flop_padded,
+                          "checkpoint": os.path.basename(ckpt)}
         print(f"[profile] network over the sector: {dt * 1e3:.1f} ms", flush=True)
     src, dst = DeviceBuffer(args.copy_bytes), DeviceBuffer(args.copy_bytes)
     v = ctypes.c_double()
