@@ -17,7 +17,7 @@
 //  - Vectors are stored point-fastest, X[row][p] (Pp = points padded to 16,
 //    32 or 64), so the SpMM gathers PT consecutive doubles of x per entry: one
 //    cache line serves PT problems.  The dictionary of the block's PT points
-//    sits in LDS.
+//    sits in LDS, and each XCD works on one tile of points.
 //  - Every Lanczos vector operation is one launch over all problems; inner
 //    products reduce per 1024-row chunk (chunks never straddle sectors) and
 //    then per sector in a fixed order, so results are deterministic.
@@ -52,7 +52,7 @@ struct hd_batch {
     int ndict = 0;
     double* d_diag = nullptr;  // [R][Pp]
     int spmm_pt = 16;          // points per SpMM tile (autotuned)
-    int spmm_grid = 0;         // blocks per tile (autotuned)
+    int spmm_grid = 0;         // SpMM blocks per XCD (autotuned)
     size_t bytes = 0;
 };
 
@@ -112,26 +112,36 @@ __global__ void k_batch_diag(const uint64_t* __restrict__ states, int64_t dim, i
 // ------------------------------------------------------------------ SpMM
 
 // Y[r][p] = diag[r][p] X[r][p] + sum_e sign_e dict[idx_e][p] X[col_e][p] for the
-// PT points of tile blockIdx.y.  A wavefront holds 64/PT row slots of PT lanes;
-// the row's columns and codes are uniform within a slot.
+// PT points of one tile.  A wavefront holds 64/PT row slots of PT lanes; the row's
+// columns and codes are uniform within a slot.  The grid is 8*G blocks laid out for
+// the round-robin placement of workgroups on the 8 XCDs (block b on XCD b % 8): XCD k
+// works on tile k % T only (T = Pp/PT tiles), and the 8/T XCDs of a tile take
+// contiguous row windows of G blocks each, so an XCD's L2 serves the x lines of one
+// tile for one window.  The dictionary tile in LDS carries a zero row at index ndict
+// that the diagonal code 0xFFFF maps to, so the lookup has no branch; the diagonal
+// itself is added at the end.  Entries are summed in stored order from +0.0.
 template <int PT>
 __global__ void __launch_bounds__(1024)
 k_bspmm(int64_t R, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
         const uint16_t* __restrict__ code, const double* __restrict__ dict, int ndict,
-        const double* __restrict__ diag, int Pp, const double* __restrict__ x, double* __restrict__ y)
+        const double* __restrict__ diag, int Pp, const double* __restrict__ x, double* __restrict__ y, int G)
 {
     extern __shared__ double s_dict[];
-    const int p0 = blockIdx.y * PT;
-    for (int i = threadIdx.x; i < ndict * PT; i += blockDim.x)
-        s_dict[i] = dict[(size_t)(i / PT) * Pp + p0 + (i % PT)];
+    const int T = Pp / PT, parts = 8 / T;
+    const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
+    const int p0 = (xcd % T) * PT, part = xcd / T;
+    for (int i = threadIdx.x; i < (ndict + 1) * PT; i += blockDim.x)
+        s_dict[i] = i < ndict * PT ? dict[(size_t)(i / PT) * Pp + p0 + (i % PT)] : 0.0;
     __syncthreads();
     constexpr int RS = 64 / PT;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = lane / PT, pl = lane % PT;
-    const int64_t rows_per_iter = (int64_t)(blockDim.x >> 6) * RS;
+    const int64_t rows_per_block = (int64_t)(blockDim.x >> 6) * RS;
+    const int64_t win = (int64_t)G * rows_per_block;
     const double* xp = x + p0 + pl;
-    for (int64_t r = (int64_t)blockIdx.x * rows_per_iter + wave * RS + slot; r < R;
-         r += (int64_t)gridDim.x * rows_per_iter) {
+    const double* dp = s_dict + pl;
+    const unsigned zrow = (unsigned)ndict;
+    for (int64_t r = part * win + j * rows_per_block + wave * RS + slot; r < R; r += parts * win) {
         const int64_t e0 = rowptr[r], e1 = rowptr[r + 1];
         double acc = 0.0;
         int64_t e = e0;
@@ -148,19 +158,15 @@ k_bspmm(int64_t R, const int64_t* __restrict__ rowptr, const int32_t* __restrict
             for (int u = 0; u < 4; ++u) xv[u] = xp[(int64_t)c[u] * Pp];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const bool dg = cd[u] == 0xFFFFu;  // the diagonal is added below
-                double v = s_dict[(dg ? 0u : (cd[u] & 0x7FFFu)) * PT + pl];
-                v = dg ? 0.0 : ((cd[u] & 0x8000u) ? -v : v);
-                acc = fma(v, xv[u], acc);
+                const double v = dp[min(cd[u] & 0x7FFFu, zrow) * PT];
+                acc = fma(((cd[u] & 0x8000u) && cd[u] != 0xFFFFu) ? -v : v, xv[u], acc);
             }
         }
         for (; e < e1; ++e) {
             const unsigned cd = code[e];
             const double xv = xp[(int64_t)col[e] * Pp];
-            const bool dg = cd == 0xFFFFu;
-            double v = s_dict[(dg ? 0u : (cd & 0x7FFFu)) * PT + pl];
-            v = dg ? 0.0 : ((cd & 0x8000u) ? -v : v);
-            acc = fma(v, xv, acc);
+            const double v = dp[min(cd & 0x7FFFu, zrow) * PT];
+            acc = fma(((cd & 0x8000u) && cd != 0xFFFFu) ? -v : v, xv, acc);
         }
         const size_t o = (size_t)r * Pp + p0 + pl;
         y[o] = fma(diag[o], x[o], acc);
@@ -756,16 +762,21 @@ k_bdense(const int32_t* __restrict__ dsec, const int64_t* __restrict__ row0, con
 
 // ----------------------------------------------------------------- host side
 
-static int spmm_launch(const hd_batch* b, int pt, int grid, const double* x, double* y, hipStream_t st)
+static size_t spmm_lds(const hd_batch* b, int pt)
 {
-    const int tiles = b->Pp / pt;
-    const size_t lds = (size_t)std::max(b->ndict, 1) * pt * sizeof(double);
+    return (size_t)(std::max(b->ndict, 1) + 1) * pt * sizeof(double);
+}
+
+// G = blocks per XCD (the grid is 8*G).
+static int spmm_launch(const hd_batch* b, int pt, int G, const double* x, double* y, hipStream_t st)
+{
+    const size_t lds = spmm_lds(b, pt);
     if (pt == 16)
-        hipLaunchKernelGGL(k_bspmm<16>, dim3(grid, tiles), dim3(1024), lds, st, b->R, b->d_rowptr, b->d_col,
-                           b->d_code, b->d_dict, b->ndict, b->d_diag, b->Pp, x, y);
+        hipLaunchKernelGGL(k_bspmm<16>, dim3(8 * G), dim3(1024), lds, st, b->R, b->d_rowptr, b->d_col, b->d_code,
+                           b->d_dict, b->ndict, b->d_diag, b->Pp, x, y, G);
     else
-        hipLaunchKernelGGL(k_bspmm<8>, dim3(grid, tiles), dim3(1024), lds, st, b->R, b->d_rowptr, b->d_col,
-                           b->d_code, b->d_dict, b->ndict, b->d_diag, b->Pp, x, y);
+        hipLaunchKernelGGL(k_bspmm<8>, dim3(8 * G), dim3(1024), lds, st, b->R, b->d_rowptr, b->d_col, b->d_code,
+                           b->d_dict, b->ndict, b->d_diag, b->Pp, x, y, G);
     HD_HIP(hipGetLastError());
     return HD_OK;
 }
@@ -782,35 +793,35 @@ static int set_big_lds(const void* fn, size_t lds)
     return HD_OK;
 }
 
-// Tile width and grid of the SpMM by timing on the given vectors (x is read,
-// y written).  PT = 16 when the dictionary tile fits in LDS, else 8.
+// Tile width and blocks per XCD of the SpMM by timing on the given vectors (x is
+// read, y written).  PT = 16 when the dictionary tile fits in LDS, else 8.
 static int spmm_autotune(hd_batch* b, const double* x, double* y)
 {
     int n_cu = 256;
     HD_CHECK(device_props(&n_cu, nullptr));
-    const size_t nd = (size_t)std::max(b->ndict, 1);
     float best = 1e30f;
-    int best_pt = 0, best_grid = 0;
+    int best_pt = 0, best_g = 0;
     EventTimer timer;
     for (int pt : {16, 8}) {
-        const size_t lds = nd * pt * sizeof(double);
-        if (lds > 160 * 1024 || b->Pp % pt) continue;
+        const size_t lds = spmm_lds(b, pt);
+        if (lds > 160 * 1024 || b->Pp % pt || b->Pp / pt > 8) continue;
         HD_CHECK(set_big_lds(pt == 16 ? (const void*)k_bspmm<16> : (const void*)k_bspmm<8>, lds));
-        const int tiles = b->Pp / pt;
+        const int parts = 8 / (b->Pp / pt);
         const int per_cu = lds > 80 * 1024 ? 1 : 2;
+        const int64_t rows_per_block = 16 * (64 / pt);
         for (int mult : {1, 2}) {
-            const int64_t rows_per_block = 16 * (64 / pt);
-            const int grid = (int)std::max<int64_t>(
-                1, std::min<int64_t>((int64_t)n_cu * per_cu * mult / tiles, (b->R + rows_per_block - 1) / rows_per_block));
-            HD_CHECK(spmm_launch(b, pt, grid, x, y, b->stream));  // warm
+            const int G = (int)std::max<int64_t>(
+                1, std::min<int64_t>((int64_t)std::max(n_cu / 8, 1) * per_cu * mult,
+                                     (b->R + rows_per_block * parts - 1) / (rows_per_block * parts)));
+            HD_CHECK(spmm_launch(b, pt, G, x, y, b->stream));  // warm
             timer.start(b->stream);
-            for (int r = 0; r < 3; ++r) HD_CHECK(spmm_launch(b, pt, grid, x, y, b->stream));
+            for (int r = 0; r < 3; ++r) HD_CHECK(spmm_launch(b, pt, G, x, y, b->stream));
             const float ms = timer.stop(b->stream);
             HD_CHECK(timer.status("batched SpMM autotune"));
             if (ms < best) {
                 best = ms;
                 best_pt = pt;
-                best_grid = grid;
+                best_g = G;
             }
         }
     }
@@ -818,7 +829,7 @@ static int spmm_autotune(hd_batch* b, const double* x, double* y)
         return fail(HD_ERR_UNSUPPORTED, "the value dictionary (" + std::to_string(b->ndict) +
                                             " tuples) does not fit in LDS for 8 points per tile");
     b->spmm_pt = best_pt;
-    b->spmm_grid = best_grid;
+    b->spmm_grid = best_g;
     return HD_OK;
 }
 
