@@ -31,6 +31,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--check", type=int, default=3, help="sectors spot-checked against single solves")
+    ap.add_argument("--spmm-only", action="store_true", help="stop after timing the batched SpMM (for profiling)")
     args = ap.parse_args()
     n, two_s = args.n, args.two_s
     norb = two_s + 1
@@ -52,6 +53,10 @@ def main():
     spmm_ms = bp.spmm_time_ms(5)
     log(f"batched SpMM over all {len(pts)} points: {spmm_ms:.2f} ms "
         f"({bp.nnz * len(pts) / spmm_ms / 1e6:.1f} G entry-products/s)")
+    if args.spmm_only:
+        print(json.dumps({"rows": bp.rows, "nnz": bp.nnz, "value_tuples": bp.ndict, "spmm_ms": round(spmm_ms, 3)}),
+              flush=True)
+        return
     t0 = time.perf_counter()
     ev, rs, st = bp.solve(nev=args.nev, tol=args.tol, use_graph=not args.no_graph, raise_on_fail=False)
     t_solve = time.perf_counter() - t0
