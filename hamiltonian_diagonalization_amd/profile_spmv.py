@@ -4,8 +4,10 @@ python -m hamiltonian_diagonalization_amd.profile_spmv [--n 12 --two-s 33 --iter
 python -m hamiltonian_diagonalization_amd.profile_spmv --ofl 6 5 10   (flux lattice dim_x dim_y N, sector (0,0))
 
 Builds the N=12, 2S=33 L_z=0 Coulomb sphere Hamiltonian with FP64, FP32 and
-coded values and the tiled layout (4-byte entries, then the packed 3-byte
-ones), then launches each SpMV format `iters` times
+coded values and the tiled layout (its automatic entry width: 3-byte entries
+for a real dictionary of at most 2047 values, else 4-byte; then the other
+width for comparison), then launches each SpMV format, the tiled one with FP64
+and with FP32 input vectors, `iters` times
 (after one warm-up launch) and the plain streaming copy kernel over 1 GiB `iters` times, which
 calibrates FETCH_SIZE / WRITE_SIZE against a known byte count.  Prints the
 algorithmic bytes of every launch so that the counter totals can be compared.
@@ -14,7 +16,7 @@ import argparse
 import ctypes
 import json
 
-from . import CODED, F32, F64, STORAGE_FULL, TILED, DeviceBuffer, FockBasis, Hamiltonian, Sector, _check, lib
+from . import CODED, F32, F64, STORAGE_FULL, TILED, TILED_X32, DeviceBuffer, FockBasis, Hamiltonian, Sector, _check, lib
 from . import models
 
 
@@ -39,7 +41,7 @@ def main():
     tile = ham.tile(args.rows)
     ham.set_lanes(0)
     out = {"dim": basis.dim, "nnz": ham.nnz, "lanes": ham.lanes(), "iters": args.iters, "tile": tile, "spmv": {}}
-    for name, p in (("f64", F64), ("f32", F32), ("coded", CODED), ("tiled", TILED)):
+    for name, p in (("f64", F64), ("f32", F32), ("coded", CODED), ("tiled", TILED), ("tiled_x32", TILED_X32)):
         ms = ham.spmv_time_ms(p, args.iters)
         out["spmv"][name] = {"ms": ms, "alg_bytes": ham.spmv_bytes(p)}
         print(f"[profile] spmv {name}: {ms:.3f} ms, {ham.spmv_bytes(p)} algorithmic bytes", flush=True)
@@ -53,16 +55,18 @@ def main():
             print(f"[profile] tiled block order {w} groups per XCD: {ms:.3f} ms", flush=True)
         ham.tile_order(chosen)
         print(f"[profile] autotuned order: {chosen}", flush=True)
-    # the packed 3-byte entries (k_spmv_tiled<128, true>) when the dictionary allows them
-    if tile["entry_bytes"] == 4 and not args.ofl:
+    # the other entry width of the tiled layout (4-byte entries always build; 3-byte ones need a
+    # real dictionary of at most 2047 values)
+    if not args.ofl:
+        other = 4 if tile["entry_bytes"] == 3 else 3
         try:
-            t3 = ham.tile(args.rows, 3)
-        except Exception as e:  # dictionary above 2047 values
-            print(f"[profile] packed entries not available: {e}", flush=True)
+            t2 = ham.tile(args.rows, other)
+        except Exception as e:
+            print(f"[profile] {other}-byte entries not available: {e}", flush=True)
         else:
             ms = ham.spmv_time_ms(TILED, args.iters)
-            out["spmv"]["tiled3"] = {"ms": ms, "alg_bytes": t3["spmv_bytes"]}
-            print(f"[profile] spmv tiled (3-byte entries): {ms:.3f} ms, {t3['spmv_bytes']} algorithmic bytes",
+            out["spmv"][f"tiled{other}"] = {"ms": ms, "alg_bytes": t2["spmv_bytes"]}
+            print(f"[profile] spmv tiled ({other}-byte entries): {ms:.3f} ms, {t2['spmv_bytes']} algorithmic bytes",
                   flush=True)
     src, dst = DeviceBuffer(args.copy_bytes), DeviceBuffer(args.copy_bytes)
     v = ctypes.c_double()
