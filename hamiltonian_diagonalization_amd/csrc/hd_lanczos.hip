@@ -769,8 +769,30 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
     // discarded and run again with the FP64 operator.
     std::vector<double> theta_b, arrow_b, err32_b, nb_prev_b;
     int rollbacks = 0;
+    // Projected matrix of the first n basis vectors: diag(theta) + arrow at row
+    // k, tridiagonal below.
+    auto projected = [&](int n) {
+        Tm.assign((size_t)n * n, 0.0);
+        for (int i = 0; i < k; ++i) {
+            Tm[(size_t)i * n + i] = theta[i];
+            Tm[(size_t)i * n + k] = Tm[(size_t)k * n + i] = arrow[i];
+        }
+        for (int j = k; j < n; ++j) {
+            Tm[(size_t)j * n + j] = alpha[j];
+            if (j + 1 < n) Tm[(size_t)j * n + j + 1] = Tm[(size_t)(j + 1) * n + j] = beta[j];
+        }
+    };
+    // Convergence inside a cycle: once the previous cycle ended with every
+    // wanted Ritz estimate within 1e4 tol, the Ritz pairs of the basis built so
+    // far are tested after each step (a (j+1)-square eigenproblem on the host),
+    // and the cycle ends at the first step where all wanted pairs meet tol.  An
+    // FP32 cycle ends there only if its rounding estimate stays in the budget.
+    double r_prev = std::numeric_limits<double>::infinity();
+    int mc = m;  // basis size at which the current cycle ends
     for (;;) {
         const bool cyc32 = prec == prec32 && mixed;
+        const bool check_in_cycle = r_prev <= 1e4 * tol;
+        mc = m;
         if (cyc32) {
             HD_HIP(hipMemcpyAsync(L.bak, L.vec(0), sizeof(T) * L.n * (size_t)(k + 1), hipMemcpyDeviceToDevice, L.st));
             theta_b = theta;
@@ -869,29 +891,40 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
             om_next[j + 1] = 1.0;
             std::swap(om_prev, om_cur);
             std::swap(om_cur, om_next);
+            const int n1 = j + 1;
+            if (check_in_cycle && n1 < m && n1 > k + 1 && n1 >= nev && beta[j] > 0.0) {
+                projected(n1);
+                if (sym_eig(n1, Tm, w, Y)) {
+                    bool done = true;
+                    for (int i = 0; i < nev && done; ++i) {
+                        const double scale_i = std::max(1.0, std::fabs(w[i]));
+                        if (!(std::fabs(beta[j] * Y[(size_t)(n1 - 1) * n1 + i]) / scale_i <= tol)) done = false;
+                        if (done && cyc32) {
+                            double nb2 = 0.0;
+                            for (int jj = k; jj < n1; ++jj) nb2 += Y[(size_t)jj * n1 + i] * Y[(size_t)jj * n1 + i];
+                            if (err32[i] + eps32 * anorm * std::sqrt(nb2) > budget * tol * scale_i) done = false;
+                        }
+                    }
+                    if (done) {
+                        mc = n1;
+                        break;
+                    }
+                }
+            }
         }
-        // projected matrix: diag(theta) + arrow at row k, tridiagonal below
-        Tm.assign((size_t)m * m, 0.0);
-        for (int i = 0; i < k; ++i) {
-            Tm[(size_t)i * m + i] = theta[i];
-            Tm[(size_t)i * m + k] = Tm[(size_t)k * m + i] = arrow[i];
-        }
-        for (int j = k; j < m; ++j) {
-            Tm[(size_t)j * m + j] = alpha[j];
-            if (j + 1 < m) Tm[(size_t)j * m + j + 1] = Tm[(size_t)(j + 1) * m + j] = beta[j];
-        }
-        if (!sym_eig(m, Tm, w, Y)) return fail(HD_ERR_NOT_CONVERGED, "projected eigenproblem did not converge");
-        const double bm = beta[m - 1];
+        projected(mc);
+        if (!sym_eig(mc, Tm, w, Y)) return fail(HD_ERR_NOT_CONVERGED, "projected eigenproblem did not converge");
+        const double bm = beta[mc - 1];
         bool all = true;
         bool allow32 = mixed;
         bool overshoot = false;
         for (int i = 0; i < nev; ++i) {
             const double scale_i = std::max(1.0, std::fabs(w[i]));
-            res_est[i] = std::fabs(bm * Y[(size_t)(m - 1) * m + i]) / scale_i;
+            res_est[i] = std::fabs(bm * Y[(size_t)(mc - 1) * mc + i]) / scale_i;
             if (mixed) {
                 // the wanted Ritz vector's coefficients on this cycle's new vectors
                 double nb2 = 0.0;
-                for (int j = k; j < m; ++j) nb2 += Y[(size_t)j * m + i] * Y[(size_t)j * m + i];
+                for (int j = k; j < mc; ++j) nb2 += Y[(size_t)j * mc + i] * Y[(size_t)j * mc + i];
                 const double nb = std::sqrt(nb2);
                 if (cyc32) {
                     err32[i] += eps32 * anorm * nb;
@@ -904,6 +937,8 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
             }
             if (!(res_est[i] <= tol)) all = false;
         }
+        r_prev = 0.0;
+        for (int i = 0; i < nev; ++i) r_prev = std::max(r_prev, res_est[i]);
         ++restarts;
         if (trace) {
             double rmax = 0.0, emax = 0.0;
@@ -933,12 +968,12 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
         if (restarts >= max_restarts) break;
         // thick restart: keep kk Ritz vectors, continue from the residual direction
         const int kk = kk_keep;
-        HD_CHECK(L.combine(m, Y, m, kk, L.tmp));
-        HD_HIP(hipMemcpyAsync(L.vec(kk), L.vec(m), sizeof(T) * L.n, hipMemcpyDeviceToDevice, L.st));
+        HD_CHECK(L.combine(mc, Y, mc, kk, L.tmp));
+        HD_HIP(hipMemcpyAsync(L.vec(kk), L.vec(mc), sizeof(T) * L.n, hipMemcpyDeviceToDevice, L.st));
         HD_HIP(hipMemcpyAsync(L.vec(0), L.tmp, sizeof(T) * L.n * kk, hipMemcpyDeviceToDevice, L.st));
         theta.assign(w.begin(), w.begin() + kk);
         arrow.resize(kk);
-        for (int i = 0; i < kk; ++i) arrow[i] = bm * Y[(size_t)(m - 1) * m + i];
+        for (int i = 0; i < kk; ++i) arrow[i] = bm * Y[(size_t)(mc - 1) * mc + i];
         k = kk;
         if (mixed) prec = allow32 ? prec32 : L.prec64;
     }
@@ -951,7 +986,7 @@ static int run_lanczos(hd_hamiltonian* h, const hd_lanczos_opts& o, double* eval
     S->cplx = C;
     S->nev = nev;
     HD_CHECK(dalloc(&S->d_vec, sizeof(T) * (size_t)L.n * nev, "eigenvectors"));
-    HD_CHECK(L.combine(m, Y, m, nev, static_cast<T*>(S->d_vec)));
+    HD_CHECK(L.combine(mc, Y, mc, nev, static_cast<T*>(S->d_vec)));
     S->evals.assign(w.begin(), w.begin() + nev);
     for (int i = 0; i < nev; ++i) evals[i] = w[i];
     int nconv = 0;
