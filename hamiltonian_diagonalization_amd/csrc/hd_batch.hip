@@ -338,6 +338,52 @@ k_bcombine(int m, int keep, double* __restrict__ V0, int64_t vstride, const doub
     }
 }
 
+// The same combination for the 16 points of tile blockIdx.y of chunk
+// blockIdx.x, with the chunk's Y block (m x keep x 16) staged in LDS and each
+// thread's m basis values in registers (MM >= m): one LDS read per term instead
+// of a global one.  Sums run over l in the same order, so the result equals
+// k_bcombine's.  Needs m * keep * 16 doubles of LDS.
+template <int MM>
+__global__ void __launch_bounds__(kVB)
+k_bcombine16(int m, int keep, double* __restrict__ V0, int64_t vstride, const double* __restrict__ Y, int S, Chunks ck,
+             int Pp)
+{
+    extern __shared__ double sY[];  // [(l*keep + i)*16 + point]
+    const int c = blockIdx.x, p0 = blockIdx.y * 16;
+    const int s = ck.sec[c];
+    const int n = m * keep * 16;
+    for (int idx = threadIdx.x; idx < n; idx += blockDim.x)
+        sY[idx] = Y[((size_t)(idx >> 4) * S + s) * Pp + p0 + (idx & 15)];
+    __syncthreads();
+    const int64_t r0 = ck.r0[c], r1 = ck.r1[c];
+    const int pl = threadIdx.x & 15, rl = threadIdx.x >> 4, RL = kVB / 16;
+    const int ystep = keep * 16;
+    for (int64_t r = r0 + rl; r < r1; r += RL) {
+        double* const row = V0 + (size_t)r * Pp + p0 + pl;
+        double v[MM];
+        const double* q = row;
+#pragma unroll
+        for (int l = 0; l < MM; ++l) {
+            v[l] = l < m ? *q : 0.0;
+            q += vstride;
+        }
+        const double last = row[(size_t)m * vstride];
+        double* out = row;
+        for (int i = 0; i < keep; ++i) {
+            const double* yq = sY + i * 16 + pl;
+            double a = 0.0;
+#pragma unroll
+            for (int l = 0; l < MM; ++l) {
+                if (l < m) a = fma(v[l], *yq, a);
+                yq += ystep;
+            }
+            *out = a;
+            out += vstride;
+        }
+        row[(size_t)keep * vstride] = last;
+    }
+}
+
 // ||H x - theta x||^2 partials for residual checks: partial[c*Pp + p], with
 // theta of problem (s, p) at theta[s*ts + p*tp]
 __global__ void __launch_bounds__(kVB)
@@ -1077,6 +1123,14 @@ int hd_batch_solve(hd_batch* b, const hd_batch_opts* o, double* h_evals, double*
     if (dense_below > 65) return fail(HD_ERR_ARG, "dense cutoff above 65 (the dense kernel holds 64 rows)");
     const int S = b->S, P = b->P, Pp = b->Pp;
     const hipStream_t st = b->stream;
+    // restart combination with the Y block in LDS when it fits (set up here,
+    // outside any graph capture)
+    size_t comb_lds = (size_t)m * keep * 16 * sizeof(double);
+    if (comb_lds > 160 * 1024) {
+        comb_lds = 0;
+    } else {
+        HD_CHECK(set_big_lds(m <= 32 ? (const void*)k_bcombine16<32> : (const void*)k_bcombine16<64>, comb_lds));
+    }
     hd_batch_stats sx{};
     std::vector<int32_t> lsec, dsec;
     for (int s = 0; s < S; ++s) (b->dim[(size_t)s] < dense_below ? dsec : lsec).push_back(s);
@@ -1271,7 +1325,14 @@ int hd_batch_solve(hd_batch* b, const hd_batch_opts* o, double* h_evals, double*
             for (int j = k; j < m; ++j) step(j, k);
             hipLaunchKernelGGL(k_brestart, dim3((unsigned)(nls * Pp)), dim3(64), rlds, st, pr, S, m, k, keep, nev,
                                tol, alpha, beta, arrow, state, nact, conv, d_ev, d_rs, Yr, d_unconv);
-            hipLaunchKernelGGL(k_bcombine, dim3((unsigned)nck), dim3(kVB), 0, st, m, keep, V, vstride, Yr, S, ck, Pp);
+            if (comb_lds == 0)
+                hipLaunchKernelGGL(k_bcombine, dim3((unsigned)nck), dim3(kVB), 0, st, m, keep, V, vstride, Yr, S, ck, Pp);
+            else if (m <= 32)
+                hipLaunchKernelGGL(k_bcombine16<32>, dim3((unsigned)nck, (unsigned)(Pp / 16)), dim3(kVB), comb_lds, st, m,
+                                   keep, V, vstride, Yr, S, ck, Pp);
+            else
+                hipLaunchKernelGGL(k_bcombine16<64>, dim3((unsigned)nck, (unsigned)(Pp / 16)), dim3(kVB), comb_lds, st, m,
+                                   keep, V, vstride, Yr, S, ck, Pp);
         };
         int unconv = 0;
         auto read_unconv = [&]() -> int {
