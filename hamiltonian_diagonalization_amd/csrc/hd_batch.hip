@@ -860,6 +860,12 @@ static int spmm_autotune(hd_batch* b, const double* x, double* y)
                 1, std::min<int64_t>((int64_t)std::max(n_cu / 8, 1) * per_cu * mult,
                                      (b->R + rows_per_block * parts - 1) / (rows_per_block * parts)));
             HD_CHECK(spmm_launch(b, pt, G, x, y, b->stream));  // warm
+            // one timed launch first: a candidate 25% slower than the best is dropped
+            timer.start(b->stream);
+            HD_CHECK(spmm_launch(b, pt, G, x, y, b->stream));
+            const float ms1 = timer.stop(b->stream);
+            HD_CHECK(timer.status("batched SpMM autotune"));
+            if (3.0f * ms1 > 1.25f * best) continue;  // best holds the time of 3 launches
             timer.start(b->stream);
             for (int r = 0; r < 3; ++r) HD_CHECK(spmm_launch(b, pt, G, x, y, b->stream));
             const float ms = timer.stop(b->stream);
