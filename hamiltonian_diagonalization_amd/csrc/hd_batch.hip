@@ -62,6 +62,7 @@ namespace {
 constexpr int kChunk = 1024;  // rows per reduction chunk
 constexpr int kVB = 256;      // threads of the chunked vector kernels
 constexpr int kMaxM = 64;     // Krylov basis limit: one lane per projected row
+constexpr int kGrp = 16;      // basis vectors per pass over w in the dot and update kernels
 
 // ------------------------------------------------------------------ build
 
@@ -200,14 +201,16 @@ k_bdots(int nv, const double* __restrict__ V0, int64_t vstride, const double* __
     const int64_t r0 = ck.r0[c], r1 = ck.r1[c];
     const int p = threadIdx.x % Pp, rl = threadIdx.x / Pp, RL = kVB / Pp;
     const int nout = nv + want_norm;
-    for (int i0 = 0; i0 < nout; i0 += 8) {
-        const int cnt = min(8, nout - i0);
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i0 = 0; i0 < nout; i0 += kGrp) {
+        const int cnt = min(kGrp, nout - i0);
+        double acc[kGrp];
+#pragma unroll
+        for (int u = 0; u < kGrp; ++u) acc[u] = 0.0;
         for (int64_t r = r0 + rl; r < r1; r += RL) {
             const size_t o = (size_t)r * Pp + p;
             const double wv = w[o];
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
+            for (int u = 0; u < kGrp; ++u)
                 if (u < cnt) {
                     const int i = i0 + u;
                     const double a = i < nv ? V0[(size_t)i * vstride + o] : wv;
@@ -256,16 +259,16 @@ k_bupdate(int nv, const double* __restrict__ V0, int64_t vstride, const double* 
     const int64_t r0 = ck.r0[c], r1 = ck.r1[c];
     const int p = threadIdx.x % Pp, rl = threadIdx.x / Pp, RL = kVB / Pp;
     const double m = mask ? mask[(size_t)s * Pp + p] : 1.0;
-    for (int i0 = 0; i0 < nv; i0 += 8) {
-        const int cnt = min(8, nv - i0);
-        double cf[8];
+    for (int i0 = 0; i0 < nv; i0 += kGrp) {
+        const int cnt = min(kGrp, nv - i0);
+        double cf[kGrp];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) cf[u] = u < cnt ? m * coef[(size_t)s * cs + (size_t)(i0 + u) * ci + p] : 0.0;
+        for (int u = 0; u < kGrp; ++u) cf[u] = u < cnt ? m * coef[(size_t)s * cs + (size_t)(i0 + u) * ci + p] : 0.0;
         for (int64_t r = r0 + rl; r < r1; r += RL) {
             const size_t o = (size_t)r * Pp + p;
             double v = w[o];
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
+            for (int u = 0; u < kGrp; ++u)
                 if (u < cnt) v = fma(-cf[u], V0[(size_t)(i0 + u) * vstride + o], v);
             w[o] = v;
         }
