@@ -206,21 +206,22 @@ __global__ void k_scale(int64_t dim, const typename E<C>::T* __restrict__ src, d
         dst[r] = scale(src[r], s);
 }
 
-// out_l = sum_i V_i Y[i][l] for l in [l0, l0+nl), Y row-major m x ldy in LDS.
-template <bool C>
+// out_l = sum_i V_i Y[i][l] for l in [l0, l0+nl), nl <= NL, with the m x nl
+// block of Y staged in LDS (m * nl doubles of dynamic LDS).
+template <bool C, int NL>
 __global__ void __launch_bounds__(kVB)
 k_combine(int64_t dim, const typename E<C>::T* __restrict__ V, int64_t ld, int m,
           const double* __restrict__ Y, int ldy, int l0, int nl, typename E<C>::T* __restrict__ out)
 {
     using T = typename E<C>::T;
-    __shared__ double sy[kMaxBasis * 16];
+    extern __shared__ double sy[];
     for (int i = threadIdx.x; i < m * nl; i += kVB) sy[i] = Y[(i / nl) * ldy + l0 + (i % nl)];
     __syncthreads();
     const int64_t stride = (int64_t)gridDim.x * kVB;
     for (int64_t r = (int64_t)blockIdx.x * kVB + threadIdx.x; r < dim; r += stride) {
-        T acc[16];
+        T acc[NL];
 #pragma unroll
-        for (int l = 0; l < 16; ++l) acc[l] = zero<T>();
+        for (int l = 0; l < NL; ++l) acc[l] = zero<T>();
         // four basis vectors' loads in flight per thread; the sum over i keeps
         // its order
         for (int i0 = 0; i0 < m; i0 += 4) {
@@ -231,11 +232,13 @@ k_combine(int64_t dim, const typename E<C>::T* __restrict__ V, int64_t ld, int m
             for (int u = 0; u < 4; ++u) {
                 if (i0 + u >= m) break;
 #pragma unroll
-                for (int l = 0; l < 16; ++l)
+                for (int l = 0; l < NL; ++l)
                     if (l < nl) acc[l] = add2(acc[l], scale(v[u], sy[(i0 + u) * nl + l]));
             }
         }
-        for (int l = 0; l < nl; ++l) out[(int64_t)l * ld + r] = acc[l];
+#pragma unroll
+        for (int l = 0; l < NL; ++l)
+            if (l < nl) out[(int64_t)l * ld + r] = acc[l];
     }
 }
 
@@ -638,10 +641,12 @@ struct Lanczos {
     int combine(int mm, const std::vector<double>& Y, int ldy, int kk, T* out)
     {
         HD_HIP(hipMemcpyAsync(dY, Y.data(), sizeof(double) * mm * ldy, hipMemcpyHostToDevice, st));
-        for (int l0 = 0; l0 < kk; l0 += 16) {
-            const int nl = std::min(16, kk - l0);
-            hipLaunchKernelGGL(k_combine<C>, dim3(nb), dim3(kVB), 0, st, n, V, n, mm, dY, ldy, l0, nl,
-                               out + (size_t)l0 * n);
+        // real vectors: up to 32 outputs per pass over the basis; complex: 16
+        constexpr int NL = C ? 16 : 32;
+        for (int l0 = 0; l0 < kk; l0 += NL) {
+            const int nl = std::min(NL, kk - l0);
+            hipLaunchKernelGGL((k_combine<C, NL>), dim3(nb), dim3(kVB), sizeof(double) * mm * nl, st, n, V, n, mm, dY,
+                               ldy, l0, nl, out + (size_t)l0 * n);
         }
         HD_HIP(hipGetLastError());
         HD_HIP(hipStreamSynchronize(st));

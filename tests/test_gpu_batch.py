@@ -150,3 +150,29 @@ def test_batch_partial_vs_full_reorthogonalisation():
     assert st["reorth_steps"] < stf["reorth_steps"]
     ev2, _, st2 = bp.solve(nev=3, tol=1e-10, seed=2, use_graph=False)
     assert np.array_equal(ev, ev2, equal_nan=True) and st2["steps"] == st["steps"]
+
+
+@pytest.mark.parametrize("ncv", [40, 64])
+def test_batch_solve_large_krylov_basis(oracle, ncv):
+    """Krylov bases above 32 vectors take the 64-register form of the restart
+    combination (ncv 40: its coefficient block still fits in LDS) or the form
+    that reads the coefficients from global memory (ncv 64: 64 x 33 x 16
+    doubles exceed the LDS): both give the dense eigenvalues."""
+    say(f"batch solve N=6 2S=15 ncv {ncv}: 4 sectors x 5 points, nev=4")
+    n, two_s = 6, 15
+    lzs = [0, 2, 6, 12]
+    pts = _points(5, 7)
+    bp, bases, tables = _batch(n, two_s, lzs, pts)
+    ev, rs, st = bp.solve(nev=4, tol=1e-10, seed=0, ncv=ncv)
+    say(st)
+    assert st["converged"] == st["problems"]
+    assert st["max_residual"] <= 1e-10
+    for s, b in enumerate(bases):
+        bas = b.states()
+        for p, t in enumerate(tables):
+            ref = _oracle_full(oracle, n, two_s, bas, t)
+            M = np.zeros((b.dim, b.dim))
+            for i in range(b.dim):
+                M[i, ref.col[ref.rowptr[i]:ref.rowptr[i + 1]]] = ref.values()[ref.rowptr[i]:ref.rowptr[i + 1]]
+            want = np.linalg.eigvalsh(M)[:4]
+            assert np.max(np.abs(ev[s, p] - want) / np.maximum(1, np.abs(want))) < 1e-10, (lzs[s], p)
