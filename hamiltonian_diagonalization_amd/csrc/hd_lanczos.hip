@@ -86,6 +86,9 @@ HD_DEV double2 scale(double2 a, double s) { return make_double2(a.x * s, a.y * s
 template <typename T> HD_DEV T zero();
 template <> HD_DEV double zero<double>() { return 0.0; }
 template <> HD_DEV double2 zero<double2>() { return make_double2(0.0, 0.0); }
+template <typename T> HD_DEV T real_part(double x);  // x + 0i
+template <> HD_DEV double real_part<double>(double x) { return x; }
+template <> HD_DEV double2 real_part<double2>(double x) { return make_double2(x, 0.0); }
 HD_DEV float narrow(double a) { return (float)a; }
 HD_DEV float2 narrow(double2 a) { return make_float2((float)a.x, (float)a.y); }
 HD_DEV double widen(float a) { return (double)a; }
@@ -119,9 +122,12 @@ k_dots(int64_t dim, const typename E<C>::T* __restrict__ V, int64_t ld, int j0, 
     }
     const int nout = nv + (want_norm ? 1 : 0);
     for (int q = 0; q < nout; ++q) {
-        T v;
-        if (q < nv) v = acc[q];
-        else v = zero<T>(), *reinterpret_cast<double*>(&v) = nrm;
+        // select acc[q] with static indices, so that acc stays in registers (a
+        // runtime index would put it in scratch memory)
+        T v = real_part<T>(nrm);
+#pragma unroll
+        for (int u = 0; u < kNV; ++u)
+            if (u == q && q < nv) v = acc[u];
         red[threadIdx.x] = v;
         __syncthreads();
         for (int o = kVB / 2; o > 0; o >>= 1) {
