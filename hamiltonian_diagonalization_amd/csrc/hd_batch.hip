@@ -146,19 +146,21 @@ k_bspmm(int64_t R, const int64_t* __restrict__ rowptr, const int32_t* __restrict
         const int64_t e0 = rowptr[r], e1 = rowptr[r + 1];
         double acc = 0.0;
         int64_t e = e0;
-        for (; e + 4 <= e1; e += 4) {
-            int c[4];
-            unsigned cd[4];
+        // six entries' gathers in flight per slot (four and eight were slower)
+        constexpr int U = 6;
+        for (; e + U <= e1; e += U) {
+            int c[U];
+            unsigned cd[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 c[u] = col[e + u];
                 cd[u] = code[e + u];
             }
-            double xv[4];
+            double xv[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xv[u] = xp[(int64_t)c[u] * Pp];
+            for (int u = 0; u < U; ++u) xv[u] = xp[(int64_t)c[u] * Pp];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 const double v = dp[min(cd[u] & 0x7FFFu, zrow) * PT];
                 acc = fma(((cd[u] & 0x8000u) && cd[u] != 0xFFFFu) ? -v : v, xv[u], acc);
             }
