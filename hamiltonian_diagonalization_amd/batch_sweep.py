@@ -29,6 +29,7 @@ def main():
     ap.add_argument("--grid", type=int, default=8)
     ap.add_argument("--nev", type=int, default=3)
     ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--ncv", type=int, default=0, help="Krylov basis size (0: 2 nev + 21, the reference's NCV)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--check", type=int, default=3, help="sectors spot-checked against single solves")
     ap.add_argument("--spmm-only", action="store_true", help="stop after timing the batched SpMM (for profiling)")
@@ -58,7 +59,7 @@ def main():
               flush=True)
         return
     t0 = time.perf_counter()
-    ev, rs, st = bp.solve(nev=args.nev, tol=args.tol, use_graph=not args.no_graph, raise_on_fail=False)
+    ev, rs, st = bp.solve(nev=args.nev, tol=args.tol, ncv=args.ncv, use_graph=not args.no_graph, raise_on_fail=False)
     t_solve = time.perf_counter() - t0
     log(f"solve {t_solve:.2f} s: {st}")
     checks = []
