@@ -485,7 +485,10 @@ typedef struct {
  * recurrence and a new vector is orthogonalised against the whole basis only
  * when an estimate exceeds min(sqrt(eps), tol) (the first and last step of
  * every restart cycle always are); full_reorth = 1 orthogonalises at every
- * step.  Every returned eigenpair's residual is recomputed with the FP64
+ * step.  Once a cycle starts with every wanted Ritz estimate within 1e4 tol,
+ * the Ritz pairs are tested after each step and the solve ends at the first
+ * step where all meet tol (not at the end of the cycle).  Every returned
+ * eigenpair's residual is recomputed with the FP64
  * operator.  When one misses tol, the solve is repeated once from the start,
  * with the FP64 operator if precision was HD_F32, else with full_reorth = 1;
  * the returned statistics then add up both runs, so such a call can cost
