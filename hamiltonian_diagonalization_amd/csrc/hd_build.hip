@@ -546,6 +546,104 @@ k_sort(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restric
     }
 }
 
+// The same per-row sort for rows of at most 512 entries with the keys in
+// registers: one wavefront per row, lane l holding elements 8l..8l+7 of a
+// 512-slot bitonic network; distances of 8 and more swap between lanes by
+// shuffles, shorter ones inside a lane.  A key is (column << S | position): 32
+// bits (S = 9) when every column is below 2^23, else 64 bits (S = 16).  The
+// columns of a row are distinct, so the result equals k_sort's.
+HD_DEV uint32_t shfl_xor_key(uint32_t v, int m) { return (uint32_t)__shfl_xor((int)v, m); }
+HD_DEV uint64_t shfl_xor_key(uint64_t v, int m)
+{
+    const int lo = __shfl_xor((int)(uint32_t)v, m);
+    const int hi = __shfl_xor((int)(uint32_t)(v >> 32), m);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+template <bool CPLX, typename Key>
+__global__ void __launch_bounds__(256)
+k_sort512(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col_in,
+          const typename VT<CPLX>::T* __restrict__ val_in, const uint16_t* __restrict__ code_in,
+          int32_t* __restrict__ col_out, typename VT<CPLX>::T* __restrict__ val64,
+          typename VT<CPLX>::F* __restrict__ val32, uint16_t* __restrict__ code_out)
+{
+    constexpr int S = sizeof(Key) == 4 ? 9 : 16;
+    constexpr Key kPosMask = (Key(1) << S) - 1;
+    __shared__ Key s_key[4 * 512];  // one 512-key row per wavefront of the block
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    // the next row's keys are loaded while the current row is sorted
+    auto load = [&](int64_t r, int64_t& start, int& L, Key (&key)[8]) {
+        start = rowptr[r];
+        L = (int)(rowptr[r + 1] - start);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int q = lane * 8 + t;
+            key[t] = q < L ? (Key)(((Key)(uint32_t)col_in[start + q] << S) | (Key)q) : ~Key(0);
+        }
+    };
+    int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    int64_t nstart = 0;
+    int nL = 0;
+    Key nkey[8];
+    if (row < dim) load(row, nstart, nL, nkey);
+    for (; row < dim; row += waves) {
+        const int64_t start = nstart;
+        const int L = nL;
+        Key key[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) key[t] = nkey[t];
+        if (row + waves < dim) load(row + waves, nstart, nL, nkey);
+#pragma unroll
+        for (int k = 2; k <= 512; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                if (j >= 8) {
+                    const int lj = j >> 3;
+                    const bool lower = (lane & lj) == 0;
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) {
+                        const Key other = shfl_xor_key(key[t], lj);
+                        const bool up = ((lane * 8 + t) & k) == 0;
+                        key[t] = (lower == up) ? (key[t] < other ? key[t] : other) : (key[t] > other ? key[t] : other);
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) {
+                        if (t & j) continue;
+                        const int t2 = t | j;
+                        const bool up = ((lane * 8 + t) & k) == 0;
+                        const Key a = key[t], b = key[t2];
+                        const Key lo = a < b ? a : b, hi = a < b ? b : a;
+                        key[t] = up ? lo : hi;
+                        key[t2] = up ? hi : lo;
+                    }
+                }
+            }
+        }
+        // through LDS to the striped order q = 64 t + lane, so that the stores of
+        // a wavefront are contiguous
+        Key* sk = s_key + (threadIdx.x >> 6) * 512;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) sk[lane * 8 + t] = key[t];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int q = t * 64 + lane;
+            if (q < L) {
+                const Key kq = sk[q];
+                const int idx = (int)(kq & kPosMask);
+                col_out[start + q] = (int32_t)(kq >> S);
+                const typename VT<CPLX>::T v = val_in[start + idx];
+                if (val64) val64[start + q] = v;
+                if (val32) val32[start + q] = to_f32(v);
+                if (code_out) code_out[start + q] = code_in[start + idx];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 // ------------------------------------------------------------ scan kernels
 
 constexpr int kScanBlock = 256;
@@ -932,8 +1030,13 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
         ev.start(st);
         T* v64 = reinterpret_cast<T*>(h->d_val64);
         F* v32 = reinterpret_cast<F*>(h->d_val32);
-        if (h->max_row <= 512)
-            hipLaunchKernelGGL((k_sort<CPLX, 512>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, d_codetmp, h->d_col, v64, v32, h->d_code);
+        const dim3 g512((unsigned)std::max<int64_t>(1, std::min<int64_t>((dim + 3) / 4, (int64_t)n_cu * 32)));
+        if (h->max_row <= 512 && dim < (int64_t(1) << 23))
+            hipLaunchKernelGGL((k_sort512<CPLX, uint32_t>), g512, dim3(256), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp,
+                               d_codetmp, h->d_col, v64, v32, h->d_code);
+        else if (h->max_row <= 512)
+            hipLaunchKernelGGL((k_sort512<CPLX, uint64_t>), g512, dim3(256), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp,
+                               d_codetmp, h->d_col, v64, v32, h->d_code);
         else if (h->max_row <= 1024)
             hipLaunchKernelGGL((k_sort<CPLX, 1024>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, d_codetmp, h->d_col, v64, v32, h->d_code);
         else if (h->max_row <= 2048)
