@@ -447,6 +447,144 @@ k_fill(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A,
     }
 }
 
+// The same rows as k_fill, one wavefront per row.  A row's candidates (occupied
+// a < b, unoccupied c) are walked 64 at a time; those that are hops (pair table
+// entry present, d unoccupied, not trimmed, out >= s for upper storage) are
+// packed as (a, b, c, d) bytes into a per-wave queue in LDS, and every 64
+// queued hops are ranked by the 64 lanes together and stored to 64 consecutive
+// entries of the row, so no lane idles through the ranking and the stores are
+// contiguous.  Entries come out in the order k_fill writes them.
+// LDS: rank table, pair table, then per block the list of (ia, ib) occupied-index
+// pairs (2 bytes each) and per wave a 128-hop queue and the row's occupied and
+// unoccupied orbitals (64 bytes).
+constexpr int kFillWaves = 16;
+static size_t fill_wave_lds(int rank_words, int kp_bytes, int N)
+{
+    const size_t kpw = (size_t)(kp_bytes + 3) / 4;
+    const size_t npair = (size_t)N * (N - 1) / 2;
+    return 4 * (size_t)rank_words + 4 * kpw + 4 * ((npair * 2 + 3) / 4) +
+           (size_t)kFillWaves * (128 * 4 + 64);
+}
+
+template <bool CPLX, bool UPPER, bool LZ>
+__global__ void __launch_bounds__(64 * kFillWaves)
+k_fill_wave(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A,
+            const int32_t* __restrict__ rank_blob, int rank_words, const int64_t* __restrict__ rowptr,
+            const typename VT<CPLX>::T* __restrict__ diag, const uint8_t* __restrict__ dkeep,
+            int32_t* __restrict__ col_out, typename VT<CPLX>::T* __restrict__ val_out,
+            uint16_t* __restrict__ code_out, unsigned int* __restrict__ err)
+{
+    using T = typename VT<CPLX>::T;
+    extern __shared__ int32_t lds[];
+    const int K = A.K, N = A.N;
+    const int kpw = (A.kpair_bytes + 3) / 4;
+    const int npair = N * (N - 1) / 2;
+    block_copy_words(lds, rank_blob, rank_words);
+    const uint8_t* s_kp = reinterpret_cast<const uint8_t*>(lds + rank_words);
+    block_copy_words(lds + rank_words, reinterpret_cast<const int32_t*>(A.kpair), kpw);
+    uint8_t* s_pair = reinterpret_cast<uint8_t*>(lds + rank_words + kpw);
+    for (int P = threadIdx.x; P < npair; P += blockDim.x) {
+        int ia = 0, rem = P;
+        while (rem >= N - 1 - ia) {
+            rem -= N - 1 - ia;
+            ++ia;
+        }
+        s_pair[2 * P] = (uint8_t)ia;
+        s_pair[2 * P + 1] = (uint8_t)(ia + 1 + rem);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t* queue = reinterpret_cast<uint32_t*>(lds + rank_words + kpw + (npair * 2 + 3) / 4) + wave * (128 + 16);
+    uint8_t* s_orb = reinterpret_cast<uint8_t*>(queue + 128);  // [0, N) occupied, [N, K) unoccupied
+    __syncthreads();
+    const RankView rv = rank_view(lds);
+    const T* P = reinterpret_cast<const T*>(A.pval);
+    const int nun = K - N;
+    const int ncand = npair * nun;
+    const uint32_t magic = nun > 0 ? (uint32_t)((0x100000000ull + nun - 1) / (uint64_t)nun) : 0u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave; row < dim; row += nw) {
+        const uint64_t s = states[row];
+        int64_t pos = rowptr[row];
+        const int64_t end = rowptr[row + 1];
+        if (dkeep[row] && pos < end) {
+            if (lane == 0) {
+                col_out[pos] = (int32_t)row;
+                val_out[pos] = diag[row];
+                if (code_out) code_out[pos] = 0xFFFF;
+            }
+            ++pos;
+        }
+        if (lane < K) {
+            const int nb = __popcll(s & below);
+            if ((s >> lane) & 1ull) s_orb[nb] = (uint8_t)lane;
+            else s_orb[N + lane - nb] = (uint8_t)lane;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // rank and store the first n queued hops (n <= 64), one per lane
+        auto drain = [&](int n) {
+            if (lane < n) {
+                const uint32_t h = queue[lane];
+                const int a = h & 0xFF, b = (h >> 8) & 0xFF, c = (h >> 16) & 0xFF, d = h >> 24;
+                const uint64_t out = (s ^ (1ull << a) ^ (1ull << b)) | (1ull << c) | (1ull << d);
+                const int ti = ((a * (2 * K - a - 1) / 2 - a - 1) + b) * K + c;
+                const int64_t q = pos + lane;
+                if (q < end) {
+                    uint64_t r;
+                    if (!rank_state<LZ>(rv, out, r)) {
+                        atomicOr(err, 1u);
+                        r = 0;
+                    }
+                    T v = P[ti];
+                    const int par = hop_parity(s, a, b, c, d);
+                    if (par) v = vneg(v);
+                    col_out[q] = (int32_t)r;
+                    val_out[q] = v;
+                    if (code_out) code_out[q] = (uint16_t)((uint16_t)A.remap[ti] | (par ? 0x8000u : 0u));
+                } else {  // more entries than the count pass saw: never write past the row
+                    atomicOr(err, 4u);
+                }
+            }
+            pos += n;
+        };
+        int nq = 0;
+        for (int base = 0; base < ncand; base += 64) {
+            const int qi = base + lane;
+            bool ok = false;
+            uint32_t h = 0;
+            if (qi < ncand) {
+                const int pi = (int)__umulhi((uint32_t)qi, magic);
+                const int ci = qi - pi * nun;
+                const int a = s_orb[s_pair[2 * pi]], b = s_orb[s_pair[2 * pi + 1]], c = s_orb[N + ci];
+                const int e = s_kp[((a * (2 * K - a - 1) / 2 - a - 1) + b) * K + c];
+                if (e != 0xFF && !(e & 0x80)) {
+                    const int d = e & 0x3F;
+                    if (!((s >> d) & 1ull)) {
+                        const uint64_t out = (s ^ (1ull << a) ^ (1ull << b)) | (1ull << c) | (1ull << d);
+                        ok = !(UPPER && out < s);
+                        h = (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16) | ((uint32_t)d << 24);
+                    }
+                }
+            }
+            const uint64_t m = __ballot(ok);
+            if (ok) queue[nq + __popcll(m & below)] = h;
+            nq += __popcll(m);
+            __builtin_amdgcn_wave_barrier();
+            if (nq >= 64) {
+                drain(64);
+                nq -= 64;
+                const uint32_t carry = lane < nq ? queue[64 + lane] : 0u;
+                __builtin_amdgcn_wave_barrier();
+                if (lane < nq) queue[lane] = carry;
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        if (nq > 0) drain(nq);
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0 && pos != end) atomicOr(err, 2u);
+    }
+}
+
 HD_DEV double vmad(double acc, double v, double x) { return acc + v * x; }
 HD_DEV double2 vmad(double2 acc, double2 v, double2 x)
 {
@@ -805,6 +943,18 @@ static int launch_fill(int grid, size_t lds, hipStream_t st, const hd_basis* b, 
                        const uint8_t* dkeep, int32_t* ctmp, typename VT<CPLX>::T* vtmp,
                        uint16_t* codetmp, unsigned int* err)
 {
+    const size_t wlds = fill_wave_lds(rank_words, A.kpair_bytes, A.N);
+    if (A.N >= 2 && A.K > A.N && wlds <= 160 * 1024) {
+        int n_cu = 256;
+        HD_CHECK(device_props(&n_cu, nullptr));
+        HD_CHECK(set_lds((const void*)k_fill_wave<CPLX, UPPER, LZ>, wlds));
+        const int wgrid = (int)std::max<int64_t>(
+            1, std::min<int64_t>((b->dim + kFillWaves - 1) / kFillWaves, (int64_t)n_cu * (wlds > 80 * 1024 ? 1 : 2)));
+        hipLaunchKernelGGL((k_fill_wave<CPLX, UPPER, LZ>), dim3(wgrid), dim3(64 * kFillWaves), wlds, st,
+                           b->d_states, b->dim, A, b->d_rank, rank_words, rowptr, diag, dkeep, ctmp, vtmp,
+                           codetmp, err);
+        return HD_OK;
+    }
     HD_CHECK(set_lds((const void*)k_fill<CPLX, UPPER, LZ>, lds));
     hipLaunchKernelGGL((k_fill<CPLX, UPPER, LZ>), dim3(grid), dim3(1024), lds, st, b->d_states, b->dim, A,
                        b->d_rank, rank_words, rowptr, diag, dkeep, ctmp, vtmp, codetmp, err);
