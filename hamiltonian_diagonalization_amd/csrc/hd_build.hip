@@ -457,6 +457,49 @@ k_fill(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A,
 // LDS: rank table, pair table, then per block the list of (ia, ib) occupied-index
 // pairs (2 bytes each) and per wave a 128-hop queue and the row's occupied and
 // unoccupied orbitals (64 bytes).
+// Split ranking.  The rank of a sector state is a sum over its occupied
+// orbitals p, taken in descending order, of W(p, c, g): the number of ways to
+// place the c particles left (p's included) below p with the group weight g left
+// (the table of make_rank_table).  For a state of the sector, c and g at p are
+// fixed by the orbitals above p, and also by those at or below p (the count and
+// weight that remain).  So the orbitals below m1 contribute L(low bits), those
+// from m2 up contribute H(high bits), both tabulated over all bit patterns, and
+// only the occupied orbitals of the middle [m1, m2) are stepped, from the c and g
+// that H leaves.  Valid for states of the sector only: the fill ranks hops of
+// the pair table, which keep the sector (derive_tables drops the others).
+struct SplitArgs {
+    int words;   // LDS words of the tables
+    int m1, m2;  // middle orbitals [m1, m2)
+    int N;
+    int oH, oPk, oSub, oTab;  // word offsets of H, the middle's (p, c) ranges, weights, counts
+};
+
+template <bool LZ>
+HD_DEV uint32_t split_rank(const int32_t* sb, const SplitArgs& S, uint64_t s)
+{
+    const uint32_t* L = reinterpret_cast<const uint32_t*>(sb);
+    const uint2* H = reinterpret_cast<const uint2*>(sb + S.oH);
+    const uint2* pk = reinterpret_cast<const uint2*>(sb + S.oPk);
+    const int32_t* sub = sb + S.oSub;
+    const uint32_t* tab = reinterpret_cast<const uint32_t*>(sb + S.oTab);
+    const int nmid = S.m2 - S.m1;
+    uint32_t r = L[s & ((1ull << S.m1) - 1ull)];
+    const uint2 he = H[S.m2 < 64 ? (s >> S.m2) : 0ull];
+    r += he.x;
+    int c = (int)(he.y & 0xFFu), g = (int)(he.y >> 8);
+    uint64_t t = nmid > 0 ? (s >> S.m1) & ((1ull << nmid) - 1ull) : 0ull;
+    while (t) {
+        const int q = 63 - __clzll(t);
+        t ^= 1ull << q;
+        const uint2 e = pk[q * (S.N + 1) + c];
+        const int lo = (int)(int16_t)(e.y & 0xFFFFu), hi = (int)(int16_t)(e.y >> 16);
+        if (g >= lo && g <= hi) r += tab[(int)e.x + g];
+        g = LZ ? g - sub[q] : sub[g * nmid + q];
+        c = c > 0 ? c - 1 : 0;
+    }
+    return r;
+}
+
 constexpr int kFillWaves = 16;
 static size_t fill_wave_lds(int rank_words, int kp_bytes, int N)
 {
@@ -466,9 +509,10 @@ static size_t fill_wave_lds(int rank_words, int kp_bytes, int N)
            (size_t)kFillWaves * (128 * 4 + 64);
 }
 
-template <bool CPLX, bool UPPER, bool LZ>
+// SPLIT: rank_blob holds the split tables (S) instead of the rank table.
+template <bool CPLX, bool UPPER, bool LZ, bool SPLIT>
 __global__ void __launch_bounds__(64 * kFillWaves)
-k_fill_wave(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A,
+k_fill_wave(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A, SplitArgs S,
             const int32_t* __restrict__ rank_blob, int rank_words, const int64_t* __restrict__ rowptr,
             const typename VT<CPLX>::T* __restrict__ diag, const uint8_t* __restrict__ dkeep,
             int32_t* __restrict__ col_out, typename VT<CPLX>::T* __restrict__ val_out,
@@ -531,7 +575,13 @@ k_fill_wave(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A,
                 const int64_t q = pos + lane;
                 if (q < end) {
                     uint64_t r;
-                    if (!rank_state<LZ>(rv, out, r)) {
+                    if (SPLIT) {
+                        r = split_rank<LZ>(lds, S, out);
+                        if (r >= (uint64_t)dim) {  // never a column outside the sector
+                            atomicOr(err, 1u);
+                            r = 0;
+                        }
+                    } else if (!rank_state<LZ>(rv, out, r)) {
                         atomicOr(err, 1u);
                         r = 0;
                     }
@@ -937,21 +987,158 @@ static int set_lds(const void* fn, size_t lds)
     return HD_OK;
 }
 
+// Host side of split_rank: L, H and the middle's tables from the rank table of
+// make_rank_table, with m1 and m2 chosen for the fewest middle orbitals whose
+// tables fit in `budget` bytes.  Returns false (no split) for a list basis or
+// when nothing fits.
+static bool make_split_rank(const RankTable& rt, size_t budget, SplitArgs& S, std::vector<int32_t>& out)
+{
+    if (rt.empty || (rt.is_lz != 0 && rt.is_lz != 1)) return false;
+    const int32_t* bl = rt.blob.data();
+    const int N = bl[0], K = bl[1], G = bl[2], target = bl[3];
+    const bool lz = bl[4] == 1;
+    if (K > 64 || N < 1 || N > 255) return false;
+    const int npc = (K + 1) * (N + 1);
+    const int32_t* off = bl + kRankHeader;
+    const int32_t* gmm = off + npc;
+    const int32_t* sub = gmm + 2 * npc;
+    const uint32_t* tab = reinterpret_cast<const uint32_t*>(sub + (lz ? K : G * K));
+    auto W = [&](int p, int c, int g) -> uint32_t {
+        if (c < 0 || c > N || p < 0 || p > K) return 0u;
+        const int i = p * (N + 1) + c;
+        if (g < gmm[2 * i] || g > gmm[2 * i + 1]) return 0u;
+        return tab[off[i] + g];
+    };
+    auto gsub = [&](int g, int k) { return lz ? g - sub[k] : sub[g * K + k]; };
+    // entries of the middle's counting table for orbitals [a, b)
+    auto mid_entries = [&](int a, int b) {
+        size_t n = 0;
+        for (int p = a; p < b; ++p)
+            for (int c = 0; c <= N; ++c) {
+                const int i = p * (N + 1) + c;
+                if (gmm[2 * i + 1] >= gmm[2 * i]) n += (size_t)(gmm[2 * i + 1] - gmm[2 * i] + 1);
+            }
+        return n;
+    };
+    int best_m1 = -1, best_hb = -1;
+    size_t best_bytes = 0;
+    for (int m1 = std::min(K, 16); m1 >= std::min(K, 4); --m1)
+        for (int hb = std::min(K - m1, 14); hb >= 0; --hb) {
+            const int nmid = K - m1 - hb;
+            const size_t bytes = 4 * ((size_t)1 << m1) + 8 * ((size_t)1 << hb) + 8 * (size_t)nmid * (N + 1) +
+                                 4 * (size_t)nmid * (lz ? 1 : G) + 4 * mid_entries(m1, m1 + nmid);
+            if (bytes > budget) continue;
+            const int bnmid = best_m1 < 0 ? 1 << 30 : K - best_m1 - best_hb;
+            if (nmid < bnmid || (nmid == bnmid && bytes < best_bytes)) {
+                best_m1 = m1;
+                best_hb = hb;
+                best_bytes = bytes;
+            }
+        }
+    if (best_m1 < 0) return false;
+    const int m1 = best_m1, m2 = K - best_hb, nmid = m2 - m1;
+    // group sum (the inverse of gsub) for the low table
+    std::vector<int> gadd;
+    if (!lz) {
+        gadd.assign((size_t)G * K, 0);
+        for (int g2 = 0; g2 < G; ++g2)
+            for (int k = 0; k < K; ++k) gadd[(size_t)sub[g2 * K + k] * K + k] = g2;
+    }
+    out.clear();
+    // L: ascending over the bits of l, c and g counted up to and including p
+    for (uint32_t l = 0; l < (1u << m1); ++l) {
+        uint32_t r = 0;
+        int c = 0, g = 0;
+        for (int p = 0; p < m1; ++p) {
+            if (!((l >> p) & 1u)) continue;
+            ++c;
+            g = lz ? g + sub[p] : gadd[(size_t)g * K + p];
+            r += W(p, c, g);
+        }
+        out.push_back((int32_t)r);
+    }
+    S.oH = (int)out.size();
+    // H: descending from (N, target); impossible patterns get zeros
+    for (uint32_t hpat = 0; hpat < (1u << (K - m2)); ++hpat) {
+        uint32_t r = 0;
+        int c = N, g = target;
+        bool ok = true;
+        for (int p = K - 1; p >= m2 && ok; --p) {
+            if (!((hpat >> (p - m2)) & 1u)) continue;
+            if (c <= 0) {
+                ok = false;
+                break;
+            }
+            r += W(p, c, g);
+            g = gsub(g, p);
+            --c;
+            if (lz && g < 0) ok = false;
+        }
+        if (!ok || g >= (1 << 24)) r = 0, c = 0, g = 0;
+        out.push_back((int32_t)r);
+        out.push_back((int32_t)((uint32_t)c | ((uint32_t)g << 8)));
+    }
+    // the middle: (p, c) ranges, weights, counts
+    S.oPk = (int)out.size();
+    const int pk_words = 2 * nmid * (N + 1);
+    out.resize(out.size() + (size_t)pk_words);
+    S.oSub = (int)out.size();
+    if (lz) {
+        for (int q = 0; q < nmid; ++q) out.push_back(sub[m1 + q]);
+    } else {
+        for (int g = 0; g < G; ++g)
+            for (int q = 0; q < nmid; ++q) out.push_back(sub[g * K + m1 + q]);
+    }
+    S.oTab = (int)out.size();
+    for (int q = 0; q < nmid; ++q)
+        for (int c = 0; c <= N; ++c) {
+            const int i = (m1 + q) * (N + 1) + c;
+            const int lo = gmm[2 * i], hi = gmm[2 * i + 1];
+            int32_t* e = out.data() + S.oPk + 2 * (q * (N + 1) + c);
+            if (hi < lo) {
+                e[0] = 0;
+                e[1] = (int32_t)(1u | (0u << 16));  // lo 1 > hi 0: empty
+                continue;
+            }
+            if (lo < -32768 || hi > 32767) return false;
+            const int base = (int)out.size() - S.oTab;
+            for (int g = lo; g <= hi; ++g) out.push_back((int32_t)tab[off[i] + g]);
+            e = out.data() + S.oPk + 2 * (q * (N + 1) + c);  // push_back may have moved the storage
+            e[0] = base - lo;
+            e[1] = (int32_t)(((uint32_t)(uint16_t)(int16_t)lo) | ((uint32_t)(uint16_t)(int16_t)hi << 16));
+        }
+    S.words = (int)out.size();
+    S.m1 = m1;
+    S.m2 = m2;
+    S.N = N;
+    return 4 * out.size() <= budget;
+}
+
 template <bool CPLX, bool UPPER, bool LZ>
 static int launch_fill(int grid, size_t lds, hipStream_t st, const hd_basis* b, const BuildArgs& A,
                        int rank_words, const int64_t* rowptr, const typename VT<CPLX>::T* diag,
                        const uint8_t* dkeep, int32_t* ctmp, typename VT<CPLX>::T* vtmp,
-                       uint16_t* codetmp, unsigned int* err)
+                       uint16_t* codetmp, unsigned int* err, const int32_t* d_split, const SplitArgs& S)
 {
-    const size_t wlds = fill_wave_lds(rank_words, A.kpair_bytes, A.N);
-    if (A.N >= 2 && A.K > A.N && wlds <= 160 * 1024) {
-        int n_cu = 256;
-        HD_CHECK(device_props(&n_cu, nullptr));
-        HD_CHECK(set_lds((const void*)k_fill_wave<CPLX, UPPER, LZ>, wlds));
+    int n_cu = 256;
+    HD_CHECK(device_props(&n_cu, nullptr));
+    if (d_split && A.N >= 2 && A.K > A.N) {
+        const size_t wlds = fill_wave_lds(S.words, A.kpair_bytes, A.N);
+        HD_CHECK(set_lds((const void*)k_fill_wave<CPLX, UPPER, LZ, true>, wlds));
         const int wgrid = (int)std::max<int64_t>(
             1, std::min<int64_t>((b->dim + kFillWaves - 1) / kFillWaves, (int64_t)n_cu * (wlds > 80 * 1024 ? 1 : 2)));
-        hipLaunchKernelGGL((k_fill_wave<CPLX, UPPER, LZ>), dim3(wgrid), dim3(64 * kFillWaves), wlds, st,
-                           b->d_states, b->dim, A, b->d_rank, rank_words, rowptr, diag, dkeep, ctmp, vtmp,
+        hipLaunchKernelGGL((k_fill_wave<CPLX, UPPER, LZ, true>), dim3(wgrid), dim3(64 * kFillWaves), wlds, st,
+                           b->d_states, b->dim, A, S, d_split, S.words, rowptr, diag, dkeep, ctmp, vtmp,
+                           codetmp, err);
+        return HD_OK;
+    }
+    const size_t wlds = fill_wave_lds(rank_words, A.kpair_bytes, A.N);
+    if (A.N >= 2 && A.K > A.N && wlds <= 160 * 1024) {
+        HD_CHECK(set_lds((const void*)k_fill_wave<CPLX, UPPER, LZ, false>, wlds));
+        const int wgrid = (int)std::max<int64_t>(
+            1, std::min<int64_t>((b->dim + kFillWaves - 1) / kFillWaves, (int64_t)n_cu * (wlds > 80 * 1024 ? 1 : 2)));
+        hipLaunchKernelGGL((k_fill_wave<CPLX, UPPER, LZ, false>), dim3(wgrid), dim3(64 * kFillWaves), wlds, st,
+                           b->d_states, b->dim, A, S, b->d_rank, rank_words, rowptr, diag, dkeep, ctmp, vtmp,
                            codetmp, err);
         return HD_OK;
     }
@@ -982,6 +1169,7 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
     long long* d_bsum = nullptr;
     int* d_bmax = nullptr;
     unsigned long long* d_misc = nullptr;  // [0] trimmed, [1] total, [2] max, [3] error flags
+    int32_t* d_split = nullptr;
     int status = HD_OK;
     const int kp_bytes = (int)D.kpair.size();
     const size_t kp_alloc = ((size_t)kp_bytes + 3) & ~(size_t)3;
@@ -989,6 +1177,7 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
         dfree(d_kpair); dfree(d_dflag); dfree(d_epsflag); dfree(d_dkeep); dfree(d_pval);
         dfree(d_dterm); dfree(d_eps); dfree(d_diag); dfree(d_vtmp); dfree(d_counts);
         dfree(d_ctmp); dfree(d_bsum); dfree(d_bmax); dfree(d_misc); dfree(d_codetmp); dfree(d_remap);
+        dfree(d_split);
     };
 #define HD_TRY(x)                  \
     do {                           \
@@ -1123,15 +1312,24 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
         const int per_cu = lds > 80 * 1024 ? 1 : (lds > 40 * 1024 ? 2 : 4);
         const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((dim + 1023) / 1024, (int64_t)n_cu * per_cu));
         unsigned int* d_err = reinterpret_cast<unsigned int*>(d_misc + 3);
+        // split rank tables in the LDS the wave fill leaves (its other tables at
+        // 0 words of rank table)
+        SplitArgs S{};
+        std::vector<int32_t> split;
+        const size_t wfix = fill_wave_lds(0, A.kpair_bytes, A.N);
+        if (wfix < 160 * 1024 && make_split_rank(b->rt, 160 * 1024 - wfix, S, split)) {
+            HD_TRY(dalloc_t(&d_split, split.size(), "split rank tables"));
+            HD_TRYHIP(hipMemcpyAsync(d_split, split.data(), split.size() * 4, hipMemcpyHostToDevice, st));
+        }
         ev.start(st);
         if (A.upper && lz)
-            HD_TRY((launch_fill<CPLX, true, true>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_codetmp, d_err)));
+            HD_TRY((launch_fill<CPLX, true, true>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_codetmp, d_err, d_split, S)));
         else if (A.upper)
-            HD_TRY((launch_fill<CPLX, true, false>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_codetmp, d_err)));
+            HD_TRY((launch_fill<CPLX, true, false>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_codetmp, d_err, d_split, S)));
         else if (lz)
-            HD_TRY((launch_fill<CPLX, false, true>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_codetmp, d_err)));
+            HD_TRY((launch_fill<CPLX, false, true>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_codetmp, d_err, d_split, S)));
         else
-            HD_TRY((launch_fill<CPLX, false, false>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_codetmp, d_err)));
+            HD_TRY((launch_fill<CPLX, false, false>(grid, lds, st, b, A, rank_words, h->d_rowptr, d_diag, d_dkeep, d_ctmp, d_vtmp, d_codetmp, d_err, d_split, S)));
         HD_TRYHIP(hipGetLastError());
         h->t_fill = ev.stop(st);
         HD_TRYHIP(hipMemcpyAsync(host_misc, d_misc, sizeof(host_misc), hipMemcpyDeviceToHost, st));
