@@ -56,6 +56,7 @@ constexpr int kMaxChunks = kMaxLines / kTC;
 constexpr int kU = 8;              // entries each SpMV lane has in flight
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kBuildThreads = 512;
+constexpr int kLU = 8;             // entries each build thread loads ahead
 constexpr int kSpmvThreads = 1024;  // one block per group: kSpmvThreads / TG lanes per row
 
 __device__ __forceinline__ uint32_t line_hash(uint32_t line)
@@ -69,7 +70,8 @@ __device__ __forceinline__ int hash_insert(uint32_t* keys, uint32_t line)
 {
     uint32_t h = line_hash(line);
     for (int probe = 0; probe < kHashSlots; ++probe) {
-        const uint32_t old = atomicCAS(&keys[h], kEmpty, line);
+        uint32_t old = keys[h];  // most lines are already in: no atomic for them
+        if (old == kEmpty) old = atomicCAS(&keys[h], kEmpty, line);
         if (old == kEmpty || old == line) return (int)h;
         h = (h + 1) & (kHashSlots - 1);
     }
@@ -110,7 +112,7 @@ struct TilePlanLds {
 
 // Row (within the group) of entry j: the last r with rp[r] <= j.
 template <int TG>
-__device__ int row_of(const TilePlanLds<TG>& S, int nrows, int64_t j)
+__device__ __forceinline__ int row_of(const TilePlanLds<TG>& S, int nrows, int64_t j)
 {
     int lo = 0, hi = nrows - 1;
     while (lo < hi) {
@@ -126,7 +128,7 @@ __device__ int row_of(const TilePlanLds<TG>& S, int nrows, int64_t j)
 // (chunk, row) block is padded to a multiple of 4 entries (the 3-byte layout,
 // read as 12-byte quads).
 template <int TG, int LS, bool QA>
-__device__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const int64_t* __restrict__ rowptr,
+__device__ __forceinline__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const int64_t* __restrict__ rowptr,
                           const int32_t* __restrict__ col, const uint16_t* __restrict__ code, int64_t g)
 {
     const int t = threadIdx.x;
@@ -142,16 +144,39 @@ __device__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const int64_t* __rest
     if (t == 0) S.n_lines = S.n_chunks = S.flags = 0;
     __syncthreads();
     const int64_t e0 = S.rp[0], e1 = S.rp[nrows];
-    for (int64_t j = e0 + t; j < e1; j += kBuildThreads) {
-        if (code[j] == 0xFFFFu) {
-            S.dpos[row_of(S, nrows, j)] = j;
-            continue;
+    // kLU entries per thread loaded before any is inserted: one block per CU
+    // leaves few waves to hide the load latency
+    for (int64_t jb = e0; jb < e1; jb += (int64_t)kBuildThreads * kLU) {
+        uint32_t qv[kLU], cv[kLU];
+#pragma unroll
+        for (int u = 0; u < kLU; ++u) {  // clamped, not guarded: the loads issue together
+            const int64_t j = min(jb + u * kBuildThreads + t, e1 - 1);
+            qv[u] = code[j];
+            cv[u] = (uint32_t)col[j];
         }
-        if (hash_insert(S.keys, (uint32_t)col[j] >> LS) < 0) atomicOr(&S.flags, 1);
+#pragma unroll
+        for (int u = 0; u < kLU; ++u) {
+            const int64_t j = jb + u * kBuildThreads + t;
+            if (j >= e1) continue;
+            if (qv[u] == 0xFFFFu) {
+                S.dpos[row_of(S, nrows, j)] = j;
+                continue;
+            }
+            if (hash_insert(S.keys, cv[u] >> LS) < 0) atomicOr(&S.flags, 1);
+        }
     }
     __syncthreads();
-    for (int i = t; i < kHashSlots; i += kBuildThreads)
-        if (S.keys[i] != kEmpty) S.sorted[atomicAdd(&S.n_lines, 1)] = S.keys[i];
+    {
+        const int lane = t & 63;
+        for (int i = t; i < kHashSlots; i += kBuildThreads) {
+            const uint32_t k = S.keys[i];
+            const uint64_t m = __ballot(k != kEmpty);
+            int basei = 0;
+            if (lane == 0 && m) basei = atomicAdd(&S.n_lines, __popcll(m));
+            basei = __shfl(basei, 0);
+            if (k != kEmpty) S.sorted[basei + __popcll(m & ((1ull << lane) - 1ull))] = k;
+        }
+    }
     __syncthreads();
     const int L = S.n_lines;
     if (L > kMaxLines) {
@@ -185,15 +210,40 @@ __device__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const int64_t* __rest
     for (int c = t; c <= nch; c += kBuildThreads) S.ls[c] = (uint32_t)min(c * kTC, L);
     if (t == 0) S.n_chunks = nch;
     __syncthreads();
-    // entries per (chunk, row)
-    for (int64_t j = e0 + t; j < e1; j += kBuildThreads) {
-        if (code[j] == 0xFFFFu) continue;
-        const int h = hash_find(S.keys, (uint32_t)col[j] >> LS);
-        if (h < 0) {
-            atomicOr(&S.flags, 8);
-            continue;
+    // entries per (chunk, row): one wavefront per row, so a row's counts have a
+    // single writer; the lanes of one chunk are counted together
+    {
+        const int lane = t & 63;
+        for (int r = t >> 6; r < nrows; r += kBuildThreads / 64) {
+            const int64_t ra = S.rp[r], rb = S.rp[r + 1];
+            for (int64_t jb = ra; jb < rb; jb += 64 * kLU) {
+                uint32_t qv[kLU], cv[kLU];
+#pragma unroll
+                for (int u = 0; u < kLU; ++u) {
+                    const int64_t j = min(jb + 64 * u + lane, rb - 1);
+                    qv[u] = code[j];
+                    cv[u] = (uint32_t)col[j];
+                }
+#pragma unroll
+                for (int u = 0; u < kLU; ++u) {
+                    if (jb + 64 * u >= rb) break;
+                    int c = -1;
+                    if (jb + 64 * u + lane < rb && qv[u] != 0xFFFFu) {
+                        const int h = hash_find(S.keys, cv[u] >> LS);
+                        if (h < 0) atomicOr(&S.flags, 8);
+                        else c = S.chunk_of[S.rank_of[h]];
+                    }
+                    uint64_t todo = __ballot(c >= 0);
+                    while (todo) {
+                        const int leader = __ffsll((unsigned long long)todo) - 1;
+                        const int cl = __shfl(c, leader);
+                        const uint64_t m = __ballot(c == cl);
+                        if (lane == leader) S.counts[cl * TG + r] += (uint32_t)__popcll(m);
+                        todo &= ~m;
+                    }
+                }
+            }
         }
-        atomicAdd(&S.counts[S.chunk_of[S.rank_of[h]] * TG + row_of(S, nrows, j)], 1u);
     }
     __syncthreads();
     // T: per chunk, the row starts; chunk blocks start on 4-entry boundaries
@@ -284,24 +334,37 @@ k_tile_fill(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __re
             const uint32_t a = S.T[c * (TG + 1) + r] + S.counts[c * TG + r], b = S.T[c * (TG + 1) + r + 1];
             for (uint32_t p = a; p < b && (int64_t)p < en; ++p) entries[eb + p] = kEmpty;
         }
-    const int64_t e0 = S.rp[0], e1 = S.rp[nrows];
-    for (int64_t j = e0 + t; j < e1; j += kBuildThreads) {
-        const uint32_t q = code[j];
-        if (q == 0xFFFFu) continue;
-        const int r = row_of(S, nrows, j);
-        const uint32_t cj = (uint32_t)col[j];
-        const int h = hash_find(S.keys, cj >> LS);
-        if (h < 0) continue;  // flagged by the plan
-        const int rank = S.rank_of[h];
-        const int c = S.chunk_of[rank];
-        const int64_t i_off = (j - S.rp[r]) - (S.dpos[r] >= 0 && S.dpos[r] < j ? 1 : 0);
-        const int64_t pos = (int64_t)S.T[c * (TG + 1) + r] + (i_off - S.cum[c * TG + r]);
-        if (pos < S.T[c * (TG + 1) + r] || pos >= S.T[c * (TG + 1) + r + 1] || pos >= en) {
-            atomicOr(error, 32);
-            continue;
+    // one wavefront per row, as in the plan's counts
+    for (int r = t >> 6; r < nrows; r += kBuildThreads / 64) {
+        const int64_t rb = S.rp[r + 1];
+        for (int64_t jb = S.rp[r]; jb < rb; jb += 64 * kLU) {
+            uint32_t qv[kLU], cv[kLU];
+#pragma unroll
+            for (int u = 0; u < kLU; ++u) {
+                const int64_t j = min(jb + 64 * u + (t & 63), rb - 1);
+                qv[u] = code[j];
+                cv[u] = (uint32_t)col[j];
+            }
+#pragma unroll
+            for (int u = 0; u < kLU; ++u) {
+                const int64_t j = jb + 64 * u + (t & 63);
+                const uint32_t q = qv[u];
+                if (j >= rb || q == 0xFFFFu) continue;
+                const uint32_t cj = cv[u];
+                const int h = hash_find(S.keys, cj >> LS);
+                if (h < 0) continue;  // flagged by the plan
+                const int rank = S.rank_of[h];
+                const int c = S.chunk_of[rank];
+                const int64_t i_off = (j - S.rp[r]) - (S.dpos[r] >= 0 && S.dpos[r] < j ? 1 : 0);
+                const int64_t pos = (int64_t)S.T[c * (TG + 1) + r] + (i_off - S.cum[c * TG + r]);
+                if (pos < S.T[c * (TG + 1) + r] || pos >= S.T[c * (TG + 1) + r + 1] || pos >= en) {
+                    atomicOr(error, 32);
+                    continue;
+                }
+                const uint32_t code20 = ((q >> 15) << 19) | (q & 0x7FFFu);
+                entries[eb + pos] = ((uint32_t)(rank - (int)S.ls[c]) << 24) | ((cj & ((1u << LS) - 1u)) << 20) | code20;
+            }
         }
-        const uint32_t code20 = ((q >> 15) << 19) | (q & 0x7FFFu);
-        entries[eb + pos] = ((uint32_t)(rank - (int)S.ls[c]) << 24) | ((cj & ((1u << LS) - 1u)) << 20) | code20;
     }
     for (int i = t; i < nrows * W; i += kBuildThreads)
         tdiag[r0 * W + i] = S.dpos[i / W] >= 0 ? diag[r0 * W + i] : 0.0;
