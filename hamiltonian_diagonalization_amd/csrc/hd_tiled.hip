@@ -123,6 +123,64 @@ __device__ __forceinline__ int row_of(const TilePlanLds<TG>& S, int nrows, int64
     return lo;
 }
 
+// Ascending bitonic sort of v[0, P) (P a power of two, P <= 8 * kBuildThreads)
+// by the whole block, in registers: thread t holds elements 8t .. 8t+7.  Stages
+// of distance below 8 run within a thread, those below 512 between the lanes
+// of a wavefront (shfl_xor), and only the longer ones exchange through v, with
+// a barrier.  Elements from P to 8 * kBuildThreads take part (as a separate
+// network) and must hold values that may move among themselves: kEmpty.
+__device__ __forceinline__ void bitonic_blocked(uint32_t* v, int P)
+{
+    const int t = threadIdx.x;
+    uint32_t a[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) a[e] = v[8 * t + e];
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j >= 8; j >>= 1) {
+            if (j >= 512) {
+                __syncthreads();
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[8 * t + e] = a[e];
+                __syncthreads();
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int i = 8 * t + e;
+                    const uint32_t b = v[i ^ j];
+                    const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
+                    a[e] = keep_min ? min(a[e], b) : max(a[e], b);
+                }
+            } else {
+                const int d = j >> 3;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int i = 8 * t + e;
+                    const uint32_t b = (uint32_t)__shfl_xor((int)a[e], d);
+                    const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
+                    a[e] = keep_min ? min(a[e], b) : max(a[e], b);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 4; j >= 1; j >>= 1) {
+            if (2 * j > k) continue;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                if (e & j) continue;
+                const bool asc = ((8 * t + e) & k) == 0;
+                const uint32_t lo = a[e], hi = a[e | j];
+                if ((lo > hi) == asc) {
+                    a[e] = hi;
+                    a[e | j] = lo;
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[8 * t + e] = a[e];
+    __syncthreads();
+}
+
 // Builds the plan in LDS; S.flags != 0 when the group does not fit.
 // LS: log2 of the x values per 128-byte line (4 real, 3 complex).  QA: every
 // (chunk, row) block is padded to a multiple of 4 entries (the 3-byte layout,
@@ -138,7 +196,6 @@ __device__ __forceinline__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const
         S.keys[i] = kEmpty;
         S.sorted[i] = kEmpty;
     }
-    for (int i = t; i < kMaxChunks * TG; i += kBuildThreads) S.counts[i] = 0;
     for (int i = t; i <= nrows; i += kBuildThreads) S.rp[i] = rowptr[r0 + i];
     for (int i = t; i < TG; i += kBuildThreads) S.dpos[i] = -1;
     if (t == 0) S.n_lines = S.n_chunks = S.flags = 0;
@@ -186,20 +243,24 @@ __device__ __forceinline__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const
     }
     int P = 1;
     while (P < L) P <<= 1;
-    for (int k = 2; k <= P; k <<= 1)
-        for (int jj = k >> 1; jj > 0; jj >>= 1) {
-            for (int i = t; i < P; i += kBuildThreads) {
-                const int ixj = i ^ jj;
-                if (ixj > i) {
-                    const uint32_t a = S.sorted[i], b = S.sorted[ixj];
-                    if ((a > b) == ((i & k) == 0)) {
-                        S.sorted[i] = b;
-                        S.sorted[ixj] = a;
+    if (P <= 8 * kBuildThreads) {
+        bitonic_blocked(S.sorted, P);
+    } else {
+        for (int k = 2; k <= P; k <<= 1)
+            for (int jj = k >> 1; jj > 0; jj >>= 1) {
+                for (int i = t; i < P; i += kBuildThreads) {
+                    const int ixj = i ^ jj;
+                    if (ixj > i) {
+                        const uint32_t a = S.sorted[i], b = S.sorted[ixj];
+                        if ((a > b) == ((i & k) == 0)) {
+                            S.sorted[i] = b;
+                            S.sorted[ixj] = a;
+                        }
                     }
                 }
+                __syncthreads();
             }
-            __syncthreads();
-        }
+    }
     // ranks, and the chunks: 256 consecutive lines each
     for (int i = t; i < L; i += kBuildThreads) {
         const int h = hash_find(S.keys, S.sorted[i]);
@@ -210,40 +271,34 @@ __device__ __forceinline__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const
     for (int c = t; c <= nch; c += kBuildThreads) S.ls[c] = (uint32_t)min(c * kTC, L);
     if (t == 0) S.n_chunks = nch;
     __syncthreads();
-    // entries per (chunk, row): one wavefront per row, so a row's counts have a
-    // single writer; the lanes of one chunk are counted together
-    {
-        const int lane = t & 63;
-        for (int r = t >> 6; r < nrows; r += kBuildThreads / 64) {
-            const int64_t ra = S.rp[r], rb = S.rp[r + 1];
-            for (int64_t jb = ra; jb < rb; jb += 64 * kLU) {
-                uint32_t qv[kLU], cv[kLU];
-#pragma unroll
-                for (int u = 0; u < kLU; ++u) {
-                    const int64_t j = min(jb + 64 * u + lane, rb - 1);
-                    qv[u] = code[j];
-                    cv[u] = (uint32_t)col[j];
-                }
-#pragma unroll
-                for (int u = 0; u < kLU; ++u) {
-                    if (jb + 64 * u >= rb) break;
-                    int c = -1;
-                    if (jb + 64 * u + lane < rb && qv[u] != 0xFFFFu) {
-                        const int h = hash_find(S.keys, cv[u] >> LS);
-                        if (h < 0) atomicOr(&S.flags, 8);
-                        else c = S.chunk_of[S.rank_of[h]];
-                    }
-                    uint64_t todo = __ballot(c >= 0);
-                    while (todo) {
-                        const int leader = __ffsll((unsigned long long)todo) - 1;
-                        const int cl = __shfl(c, leader);
-                        const uint64_t m = __ballot(c == cl);
-                        if (lane == leader) S.counts[cl * TG + r] += (uint32_t)__popcll(m);
-                        todo &= ~m;
-                    }
-                }
+    // entries per (chunk, row).  A row's columns are sorted and a chunk is a run
+    // of consecutive sorted lines, so the row's entries in chunks below c are
+    // those with columns below chunk c's first column: cum is a binary search
+    // of the row's columns (less its diagonal), and counts are differences.
+    for (int i = t; i < nch * TG; i += kBuildThreads) {
+        const int c = i / TG, r = i % TG;
+        uint32_t below = 0;
+        if (r < nrows && c > 0) {
+            const uint32_t bound = S.sorted[S.ls[c]] << LS;
+            int64_t lo = S.rp[r], hi = S.rp[r + 1];
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if ((uint32_t)col[mid] < bound) lo = mid + 1;
+                else hi = mid;
             }
+            below = (uint32_t)(lo - S.rp[r]) - (S.dpos[r] >= 0 && S.dpos[r] < lo ? 1u : 0u);
         }
+        S.cum[c * TG + r] = below;
+    }
+    __syncthreads();
+    for (int i = t; i < nch * TG; i += kBuildThreads) {
+        const int c = i / TG, r = i % TG;
+        uint32_t n = 0;
+        if (r < nrows) {
+            const uint32_t tot = (uint32_t)(S.rp[r + 1] - S.rp[r]) - (S.dpos[r] >= 0 ? 1u : 0u);
+            n = (c + 1 < nch ? S.cum[(c + 1) * TG + r] : tot) - S.cum[c * TG + r];
+        }
+        S.counts[c * TG + r] = n;
     }
     __syncthreads();
     // T: per chunk, the row starts; chunk blocks start on 4-entry boundaries
@@ -254,14 +309,6 @@ __device__ __forceinline__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const
             run += QA ? (S.counts[t * TG + r] + 3u) & ~3u : S.counts[t * TG + r];
         }
         S.T[t * (TG + 1) + TG] = run;  // chunk size, made absolute below
-    }
-    if (t >= 64 && t < 64 + TG) {
-        const int r = t - 64;
-        uint32_t run = 0;
-        for (int c = 0; c < nch; ++c) {
-            S.cum[c * TG + r] = run;
-            run += S.counts[c * TG + r];
-        }
     }
     __syncthreads();
     if (t == 0) {
