@@ -619,7 +619,7 @@ class Hamiltonian:
                  zero_tol=ZERO_TOL, values=None):
         if values is None:
             values = {STORAGE_FULL: (F64, F32, CODED), STORAGE_MATRIX_FREE: (F64,)}.get(storage, (F64, F32))
-            if tables.layout == LAYOUT_HASH or getattr(basis, "is_list", False):
+            if storage == STORAGE_FULL and (tables.layout == LAYOUT_HASH or getattr(basis, "is_list", False)):
                 values = (F64, F32)  # general builds keep no coded copy
         mask = (1 if F64 in values else 0) | (2 if F32 in values else 0) | (4 if CODED in values else 0)
         h = ctypes.c_void_p()

@@ -467,13 +467,6 @@ k_fill(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A,
 // only the occupied orbitals of the middle [m1, m2) are stepped, from the c and g
 // that H leaves.  Valid for states of the sector only: the fill ranks hops of
 // the pair table, which keep the sector (derive_tables drops the others).
-struct SplitArgs {
-    int words;   // LDS words of the tables
-    int m1, m2;  // middle orbitals [m1, m2)
-    int N;
-    int oH, oPk, oSub, oTab;  // word offsets of H, the middle's (p, c) ranges, weights, counts
-};
-
 template <bool LZ>
 HD_DEV uint32_t split_rank(const int32_t* sb, const SplitArgs& S, uint64_t s)
 {
@@ -671,6 +664,108 @@ k_spmv_free(const uint64_t* __restrict__ states, int64_t dim, int K, uint64_t no
             acc = vmad(acc, v, x[r]);
         });
         y[row] = acc;
+    }
+}
+
+HD_DEV double shfl_xor_v(double v, int d) { return __shfl_xor(v, d); }
+HD_DEV double2 shfl_xor_v(double2 v, int d) { return make_double2(__shfl_xor(v.x, d), __shfl_xor(v.y, d)); }
+
+// The matrix-free product with k_fill_wave's row walk: one wavefront per row,
+// candidates 64 at a time through a per-wave queue of hops, every 64 hops ranked
+// by the split tables (rank_blob) and gathered by the 64 lanes together, then a
+// wavefront sum.  y[row] = diag[row] x[row] + the row's full-storage hops; the
+// order of the sum differs from k_spmv_free's (and from the column-sorted CRS)
+// by rounding only.  LDS as in k_fill_wave.
+template <bool CPLX, bool LZ>
+__global__ void __launch_bounds__(64 * kFillWaves)
+k_spmv_free_wave(const uint64_t* __restrict__ states, int64_t dim, int N, int K, SplitArgs S,
+                 const int32_t* __restrict__ split_blob, const uint8_t* __restrict__ kpair, int kp_bytes,
+                 const typename VT<CPLX>::T* __restrict__ pval, const typename VT<CPLX>::T* __restrict__ diag,
+                 const typename VT<CPLX>::T* __restrict__ x, typename VT<CPLX>::T* __restrict__ y)
+{
+    using T = typename VT<CPLX>::T;
+    extern __shared__ int32_t lds[];
+    const int kpw = (kp_bytes + 3) / 4;
+    const int npair = N * (N - 1) / 2;
+    block_copy_words(lds, split_blob, S.words);
+    const uint8_t* s_kp = reinterpret_cast<const uint8_t*>(lds + S.words);
+    block_copy_words(lds + S.words, reinterpret_cast<const int32_t*>(kpair), kpw);
+    uint8_t* s_pair = reinterpret_cast<uint8_t*>(lds + S.words + kpw);
+    for (int P = threadIdx.x; P < npair; P += blockDim.x) {
+        int ia = 0, rem = P;
+        while (rem >= N - 1 - ia) {
+            rem -= N - 1 - ia;
+            ++ia;
+        }
+        s_pair[2 * P] = (uint8_t)ia;
+        s_pair[2 * P + 1] = (uint8_t)(ia + 1 + rem);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t* queue = reinterpret_cast<uint32_t*>(lds + S.words + kpw + (npair * 2 + 3) / 4) + wave * (128 + 16);
+    uint8_t* s_orb = reinterpret_cast<uint8_t*>(queue + 128);
+    __syncthreads();
+    const int nun = K - N;
+    const int ncand = npair * nun;
+    const uint32_t magic = nun > 0 ? (uint32_t)((0x100000000ull + nun - 1) / (uint64_t)nun) : 0u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave; row < dim; row += nw) {
+        const uint64_t s = states[row];
+        T acc = lane == 0 ? vmad(vzero<T>(), diag[row], x[row]) : vzero<T>();
+        if (lane < K) {
+            const int nb = __popcll(s & below);
+            if ((s >> lane) & 1ull) s_orb[nb] = (uint8_t)lane;
+            else s_orb[N + lane - nb] = (uint8_t)lane;
+        }
+        __builtin_amdgcn_wave_barrier();
+        auto drain = [&](int n) {
+            if (lane < n) {
+                const uint32_t h = queue[lane];
+                const int a = h & 0xFF, b = (h >> 8) & 0xFF, c = (h >> 16) & 0xFF, d = h >> 24;
+                const uint64_t out = (s ^ (1ull << a) ^ (1ull << b)) | (1ull << c) | (1ull << d);
+                const int ti = ((a * (2 * K - a - 1) / 2 - a - 1) + b) * K + c;
+                const uint32_t r = split_rank<LZ>(lds, S, out);
+                if ((int64_t)r < dim) {  // hops keep the sector
+                    T v = pval[ti];
+                    if (hop_parity(s, a, b, c, d)) v = vneg(v);
+                    acc = vmad(acc, v, x[r]);
+                }
+            }
+        };
+        int nq = 0;
+        for (int base = 0; base < ncand; base += 64) {
+            const int qi = base + lane;
+            bool ok = false;
+            uint32_t h = 0;
+            if (qi < ncand) {
+                const int pi = (int)__umulhi((uint32_t)qi, magic);
+                const int ci = qi - pi * nun;
+                const int a = s_orb[s_pair[2 * pi]], b = s_orb[s_pair[2 * pi + 1]], c = s_orb[N + ci];
+                const int e = s_kp[((a * (2 * K - a - 1) / 2 - a - 1) + b) * K + c];
+                if (e != 0xFF && !(e & 0x80)) {
+                    const int d = e & 0x3F;
+                    ok = !((s >> d) & 1ull);
+                    h = (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16) | ((uint32_t)d << 24);
+                }
+            }
+            const uint64_t m = __ballot(ok);
+            if (ok) queue[nq + __popcll(m & below)] = h;
+            nq += __popcll(m);
+            __builtin_amdgcn_wave_barrier();
+            if (nq >= 64) {
+                drain(64);
+                nq -= 64;
+                const uint32_t carry = lane < nq ? queue[64 + lane] : 0u;
+                __builtin_amdgcn_wave_barrier();
+                if (lane < nq) queue[lane] = carry;
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        if (nq > 0) drain(nq);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) acc = vadd(acc, shfl_xor_v(acc, d));
+        if (lane == 0) y[row] = acc;
     }
 }
 
@@ -1289,6 +1384,19 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
         d_diag = nullptr;
         d_kpair = nullptr;
         d_pval = nullptr;
+        {
+            // the product's ranking tables (k_spmv_free_wave), when they fit
+            SplitArgs S{};
+            std::vector<int32_t> split;
+            const size_t wfix = fill_wave_lds(0, kp_bytes, b->N);
+            if (b->N >= 2 && K > b->N && wfix < 160 * 1024 &&
+                make_split_rank(b->rt, 160 * 1024 - wfix, S, split)) {
+                HD_TRY(dalloc_t(&h->d_mf_split, split.size(), "split rank tables"));
+                HD_TRYHIP(hipMemcpyAsync(h->d_mf_split, split.data(), split.size() * 4, hipMemcpyHostToDevice, st));
+                HD_TRYHIP(hipStreamSynchronize(st));
+                h->mf_split = S;
+            }
+        }
         dfree(h->d_rowptr);
         h->d_rowptr = nullptr;
         h->bytes = (size_t)dim * sizeof(T) + kp_alloc + D.pval.size() * 8;
@@ -1411,6 +1519,34 @@ int matrix_free_launch(const hd_hamiltonian* h, const void* x, void* y, hipStrea
 {
     const hd_basis* b = h->basis;
     if (h->dim == 0) return HD_OK;
+    if (h->d_mf_split) {
+        const SplitArgs& S = h->mf_split;
+        const size_t wlds = fill_wave_lds(S.words, h->mf_kp_bytes, S.N);
+        int n_cu = 256;
+        HD_CHECK(device_props(&n_cu, nullptr));
+        const int wgrid = (int)std::max<int64_t>(
+            1, std::min<int64_t>((h->dim + kFillWaves - 1) / kFillWaves, (int64_t)n_cu * (wlds > 80 * 1024 ? 1 : 2)));
+        const bool lzw = b->rt.is_lz != 0;
+#define HD_MFW(C, L)                                                                                            \
+    do {                                                                                                        \
+        using T = typename VT<C>::T;                                                                            \
+        HD_CHECK(set_lds((const void*)k_spmv_free_wave<C, L>, wlds));                                           \
+        hipLaunchKernelGGL((k_spmv_free_wave<C, L>), dim3(wgrid), dim3(64 * kFillWaves), wlds, stream,          \
+                           b->d_states, h->dim, S.N, h->mf_K, S, h->d_mf_split, h->d_mf_kpair, h->mf_kp_bytes, \
+                           static_cast<const T*>(h->d_mf_pval), static_cast<const T*>(h->d_diag),              \
+                           static_cast<const T*>(x), static_cast<T*>(y));                                      \
+    } while (0)
+        if (h->is_complex) {
+            if (lzw) HD_MFW(true, true);
+            else HD_MFW(true, false);
+        } else {
+            if (lzw) HD_MFW(false, true);
+            else HD_MFW(false, false);
+        }
+#undef HD_MFW
+        HD_HIP(hipGetLastError());
+        return HD_OK;
+    }
     const int rank_words = (int)b->rt.blob.size();
     const size_t kp_alloc = ((size_t)h->mf_kp_bytes + 3) & ~(size_t)3;
     const size_t lds = (size_t)rank_words * 4 + kp_alloc;
@@ -1745,6 +1881,7 @@ int hd_ham_destroy(hd_hamiltonian* h)
     dfree(h->d_dict);
     dfree(h->d_mf_kpair);
     dfree(h->d_mf_pval);
+    dfree(h->d_mf_split);
     tiled_free(h);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;

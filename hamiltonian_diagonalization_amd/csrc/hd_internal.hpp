@@ -162,6 +162,14 @@ HD_DEV int64_t xcd_window(int64_t b, int64_t n, int64_t win)
     return base + xcd_block(b - base, n - base < W ? n - base : W);
 }
 
+// Layout of the split ranking tables (hd_build.hip: make_split_rank, split_rank).
+struct SplitArgs {
+    int words;   // LDS words of the tables
+    int m1, m2;  // middle orbitals [m1, m2)
+    int N;
+    int oH, oPk, oSub, oTab;  // word offsets of H, the middle's (p, c) ranges, weights, counts
+};
+
 struct RankView {
     const int32_t* off;
     const int32_t* gmm;
@@ -355,6 +363,8 @@ struct hd_hamiltonian {
     void* d_mf_pval = nullptr;  // double or double2
     int mf_K = 0, mf_kp_bytes = 0;
     uint64_t mf_norb_mask = 0;
+    int32_t* d_mf_split = nullptr;  // split ranking tables (null: rank table walk)
+    hd::SplitArgs mf_split{};
     double t_count = 0, t_scan = 0, t_fill = 0, t_sort = 0;
     size_t bytes = 0;
     hd_lanczos_state* lz = nullptr;  // eigenvectors of the last Lanczos call

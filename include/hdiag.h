@@ -98,7 +98,9 @@ typedef enum {
                                   from the basis (FP64 only; the matrix-free product the
                                   reference declares as MatrixVectorFunction::Recalculate,
                                   src/hamiltonians/matrix_vector_routines.hpp:106-143, and
-                                  never calls), for sectors whose CRS does not fit in HBM */
+                                  never calls), for sectors whose CRS does not fit in HBM;
+                                  sector bases with pair-structure tables only (a list basis
+                                  or hash tables: HD_ERR_UNSUPPORTED) */
 } hd_storage;
 
 typedef struct hd_basis hd_basis;

@@ -77,3 +77,15 @@ def test_matrix_free_config2_product_time():
     ms_crs = full.spmv_time_ms(hd.F64, 3)
     say(f"matrix-free product {ms_free:.1f} ms, stored FP64 CRS {ms_crs:.2f} ms, "
         f"device bytes {free.info()['device_bytes']} vs {full.info()['device_bytes']}")
+
+
+
+def test_matrix_free_list_basis_refused():
+    """A basis given as a list of states goes through the general build, which
+    stores its matrix: matrix-free storage is refused with HD_ERR_UNSUPPORTED."""
+    say("matrix-free on a list basis: refused")
+    b, t = models.sphere_problem(6, 15)
+    bl = hd.FockBasis.from_states(b.states(), 6, 16)
+    with pytest.raises(hd.HDError) as e:
+        hd.Hamiltonian(bl, t, storage=hd.STORAGE_MATRIX_FREE)
+    assert e.value.code == hd.HD_ERR_UNSUPPORTED and "full or upper" in str(e.value)
