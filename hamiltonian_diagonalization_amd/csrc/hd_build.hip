@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 
@@ -1082,6 +1083,21 @@ static int set_lds(const void* fn, size_t lds)
     return HD_OK;
 }
 
+// Row-walk kernel of the fill and of the matrix-free product: by default the
+// wavefront walk with split ranking where its tables fit, else the wavefront
+// walk with the rank table, else one thread per row.  HD_BUILD_FILL = "rank"
+// or "thread" forces the later choices (when they apply), so every kernel can
+// be checked against the others on the same matrices.
+enum class FillPath { kAuto, kRank, kThread };
+static FillPath fill_path()
+{
+    const char* e = std::getenv("HD_BUILD_FILL");
+    if (!e) return FillPath::kAuto;
+    if (!std::strcmp(e, "rank")) return FillPath::kRank;
+    if (!std::strcmp(e, "thread")) return FillPath::kThread;
+    return FillPath::kAuto;
+}
+
 // Host side of split_rank: L, H and the middle's tables from the rank table of
 // make_rank_table, with m1 and m2 chosen for the fewest middle orbitals whose
 // tables fit in `budget` bytes.  Returns false (no split) for a list basis or
@@ -1217,7 +1233,8 @@ static int launch_fill(int grid, size_t lds, hipStream_t st, const hd_basis* b, 
 {
     int n_cu = 256;
     HD_CHECK(device_props(&n_cu, nullptr));
-    if (d_split && A.N >= 2 && A.K > A.N) {
+    const FillPath path = fill_path();
+    if (d_split && A.N >= 2 && A.K > A.N && path == FillPath::kAuto) {
         const size_t wlds = fill_wave_lds(S.words, A.kpair_bytes, A.N);
         HD_CHECK(set_lds((const void*)k_fill_wave<CPLX, UPPER, LZ, true>, wlds));
         const int wgrid = (int)std::max<int64_t>(
@@ -1228,7 +1245,7 @@ static int launch_fill(int grid, size_t lds, hipStream_t st, const hd_basis* b, 
         return HD_OK;
     }
     const size_t wlds = fill_wave_lds(rank_words, A.kpair_bytes, A.N);
-    if (A.N >= 2 && A.K > A.N && wlds <= 160 * 1024) {
+    if (A.N >= 2 && A.K > A.N && wlds <= 160 * 1024 && path != FillPath::kThread) {
         HD_CHECK(set_lds((const void*)k_fill_wave<CPLX, UPPER, LZ, false>, wlds));
         const int wgrid = (int)std::max<int64_t>(
             1, std::min<int64_t>((b->dim + kFillWaves - 1) / kFillWaves, (int64_t)n_cu * (wlds > 80 * 1024 ? 1 : 2)));
@@ -1389,7 +1406,7 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
             SplitArgs S{};
             std::vector<int32_t> split;
             const size_t wfix = fill_wave_lds(0, kp_bytes, b->N);
-            if (b->N >= 2 && K > b->N && wfix < 160 * 1024 &&
+            if (b->N >= 2 && K > b->N && wfix < 160 * 1024 && fill_path() == FillPath::kAuto &&
                 make_split_rank(b->rt, 160 * 1024 - wfix, S, split)) {
                 HD_TRY(dalloc_t(&h->d_mf_split, split.size(), "split rank tables"));
                 HD_TRYHIP(hipMemcpyAsync(h->d_mf_split, split.data(), split.size() * 4, hipMemcpyHostToDevice, st));
