@@ -275,6 +275,19 @@ int sector_weight_delta_ok(const RankTable& rt, const hd_sector& sec, int a, int
     return lhs == rhs;
 }
 
+int sector_pair_sums(const RankTable& rt, const hd_sector& sec, std::vector<int>& sums)
+{
+    const int K = rt.Norb;
+    sums.assign((size_t)K * K, 0);
+    if (sec.kind == HD_SECTOR_NONE) return HD_OK;
+    GroupSpec gs;
+    const int st = group_spec(sec, rt.N, rt.Norb, gs);
+    if (st != HD_OK) return st;
+    for (int a = 0; a < K; ++a)
+        for (int b = 0; b < K; ++b) sums[(size_t)a * K + b] = gadd(gs, gadd(gs, 0, gs.w[a]), gs.w[b]);
+    return HD_OK;
+}
+
 // ----------------------------------------------------------------- kernels
 
 template <bool LZ>

@@ -179,6 +179,13 @@ static int derive_tables(const hd_tables* t, const hd_basis* b, double tol, Deri
                         std::to_string(k1) + "," + std::to_string(k2) + "," +
                         std::to_string(k3) + "," + std::to_string(k4) + ")");
     };
+    // which hops keep the sector: group sums of orbital pairs, computed once
+    std::vector<int> sums;
+    const bool by_sums = b->rt.Norb == K && sector_pair_sums(b->rt, b->sector, sums) == HD_OK;
+    auto keeps_sector = [&](int a, int bb, int c, int d) {
+        if (by_sums) return sums[(size_t)a * K + bb] == sums[(size_t)c * K + d];
+        return sector_weight_delta_ok(b->rt, b->sector, a, bb, c, d) != 0;
+    };
     for (int a = 0; a < K; ++a)
         for (int bb = a + 1; bb < K; ++bb) {
             const int pair = a * (2 * K - a - 1) / 2 + (bb - a - 1);
@@ -194,7 +201,7 @@ static int derive_tables(const hd_tables* t, const hd_basis* b, double tol, Deri
                 if (get_k1(t, d, bb, a) != c || get_k1(t, c, a, bb) != d || get_k1(t, d, a, bb) != c)
                     return unsupported(d, c, bb, a);
                 if (c > d) continue;  // the hop {a,b}->{c,d} is tabulated at c < d
-                if (!sector_weight_delta_ok(b->rt, b->sector, a, bb, c, d)) continue;
+                if (!keeps_sector(a, bb, c, d)) continue;
                 const cd terms[4] = {get_v(t, d, c, bb, a), -get_v(t, c, d, bb, a),
                                      -get_v(t, d, c, a, bb), get_v(t, c, d, a, bb)};
                 cd acc(0.0, 0.0);
@@ -1593,6 +1600,29 @@ int matrix_free_launch(const hd_hamiltonian* h, const void* x, void* y, hipStrea
     return HD_OK;
 }
 
+// The coded full-storage build of one batch sector from its merged tables.
+static int build_batch_merged(const hd_basis* b, const Derived& M, double tol, hd_hamiltonian** out)
+{
+    hd_hamiltonian* h = new hd_hamiltonian();
+    h->basis = b;
+    h->dim = b->dim;
+    h->storage = HD_STORAGE_FULL;
+    if (hipGetDevice(&h->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(HD_ERR_HIP, "stream creation failed");
+    }
+    const int st = run_build<false>(h, b, M, tol, 4);
+    if (st != HD_OK) {
+        const std::string msg = hd_last_error();
+        hd_ham_destroy(h);
+        set_error(msg);
+        return st;
+    }
+    *out = h;
+    return HD_OK;
+}
+
 int build_batch_sector(const hd_basis* b, const hd_tables* const* tables, int n, double tol, BatchTables& bt,
                        hd_hamiltonian** out)
 {
@@ -1600,9 +1630,16 @@ int build_batch_sector(const hd_basis* b, const hd_tables* const* tables, int n,
     const hd_tables* t0 = tables[0];
     for (int p = 0; p < n; ++p)
         if (tables[p]->split) return fail(HD_ERR_UNSUPPORTED, "two-level tables are not batched");
+    const int K = t0->K;
+    // the merged tables depend on the sector only through the hops it keeps
+    std::vector<int> sums;
+    const bool cacheable = b->rt.Norb == K && sector_pair_sums(b->rt, b->sector, sums) == HD_OK;
+    if (cacheable)
+        for (size_t i = 0; i < bt.merged_keys.size(); ++i)
+            if (bt.merged_keys[i] == sums)
+                return build_batch_merged(b, *static_cast<const Derived*>(bt.merged[i].get()), tol, out);
     std::vector<Derived> Ds((size_t)n);
     for (int p = 0; p < n; ++p) HD_CHECK(derive_tables(tables[p], b, tol, Ds[(size_t)p]));
-    const int K = t0->K;
     const size_t nk = Ds[0].kpair.size();
     Derived M = Ds[0];  // diagonal flags of table 0 only mark where the pattern holds a diagonal entry
     M.remap.assign(nk, (int16_t)-1);
@@ -1662,24 +1699,11 @@ int build_batch_sector(const hd_basis* b, const hd_tables* const* tables, int n,
                 if (D.has_eps && D.epsflag[(size_t)k]) bt.eps[(size_t)k * n + p] = D.eps[(size_t)k];
         }
     }
-    hd_hamiltonian* h = new hd_hamiltonian();
-    h->basis = b;
-    h->dim = b->dim;
-    h->storage = HD_STORAGE_FULL;
-    if (hipGetDevice(&h->device) != hipSuccess ||
-        hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete h;
-        return fail(HD_ERR_HIP, "stream creation failed");
+    if (cacheable) {
+        bt.merged_keys.push_back(sums);
+        bt.merged.push_back(std::make_shared<Derived>(M));
     }
-    const int st = run_build<false>(h, b, M, tol, 4);
-    if (st != HD_OK) {
-        const std::string msg = hd_last_error();
-        hd_ham_destroy(h);
-        set_error(msg);
-        return st;
-    }
-    *out = h;
-    return HD_OK;
+    return build_batch_merged(b, M, tol, out);
 }
 
 }  // namespace hd

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -129,6 +130,10 @@ int make_rank_table(const hd_sector& sec, int N, int Norb, RankTable& rt);
 // Group weight helpers on the host (used to test hop sector preservation).
 int sector_weight_delta_ok(const RankTable& rt, const hd_sector& sec, int a, int b, int c,
                            int d);
+// The same test for all pairs at once: sums[a*Norb + b] is the group sum of the
+// weights of orbitals a and b, so a hop {a,b} -> {c,d} keeps the sector exactly
+// when sums[a*Norb+b] == sums[c*Norb+d] (all zero without a sector constraint).
+int sector_pair_sums(const RankTable& rt, const hd_sector& sec, std::vector<int>& sums);
 
 constexpr int kRankHeader = 6;
 // is_lz value of a basis given as a list of states (HD_SECTOR_LIST): the
@@ -379,6 +384,10 @@ struct BatchTables {
     std::vector<double> dict;                    // tuple i at [i*n, (i+1)*n)
     std::vector<double> dterm;                   // [((k4*K + k3)*2 + q)*n + p], 0.0 where skipped
     std::vector<double> eps;                     // [k*n + p]
+    // merged pair tables already derived for this batch, by the sectors' pair
+    // sums (sector_pair_sums): sectors with equal sums keep the same hops
+    std::vector<std::vector<int>> merged_keys;
+    std::vector<std::shared_ptr<void>> merged;
 };
 // Coded full-storage pattern of one sector for a batch of real tables
 // (hd_build.hip); the codes index bt's tuples.
