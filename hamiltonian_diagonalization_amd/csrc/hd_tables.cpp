@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdint>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -280,11 +281,16 @@ int hd_sphere_vtable(int32_t max_lz, const double* vm, int32_t n_pseudo, double*
     if (n_pseudo > max_lz)
         return fail(HD_ERR_ARG, "TOO MANY 2-BODY PSEUDOPOTENTIAL COEFFICIENTS SPECIFIED");
     const int K = max_lz + 1;
-    // CG(maxLz, m_a; maxLz, m_b | 2 maxLz - L, m_a + m_b) cache over (a, b, L/2)
-    std::vector<double> cache((size_t)K * K * std::max(n_pseudo, 1), NAN);
+    // CG(maxLz, m_a; maxLz, m_b | 2 maxLz - L, m_a + m_b) over (a, b, L/2 < K),
+    // kept for the process per maxLz: a sweep over pseudopotentials reuses them
+    static std::mutex cache_mutex;
+    static std::map<int, std::vector<double>> caches;
+    std::lock_guard<std::mutex> lock(cache_mutex);
+    std::vector<double>& cache = caches[max_lz];
+    if (cache.empty()) cache.assign((size_t)K * K * K, NAN);
     auto cg = [&](int ma, int mb, int Lh) {
         const int ia = (max_lz - ma) / 2, ib = (max_lz - mb) / 2;
-        double& c = cache[((size_t)ia * K + ib) * n_pseudo + Lh];
+        double& c = cache[((size_t)ia * K + ib) * K + Lh];
         if (std::isnan(c)) {
             const double v = clebsch_gordan(max_lz, ma, max_lz, mb, 2 * max_lz - 2 * Lh, ma + mb);
             c = std::fabs(v) > 1e-12 ? v : 0.0;
