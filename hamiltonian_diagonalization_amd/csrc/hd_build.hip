@@ -851,6 +851,41 @@ HD_DEV uint64_t shfl_xor_key(uint64_t v, int m)
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
+// Ascending bitonic sort of a wavefront's 512 keys, lane l holding elements
+// 8l .. 8l+7 in registers: distances below 8 within a lane, the others
+// between lanes (shfl_xor).
+template <typename Key>
+HD_DEV void sort512_regs(Key (&key)[8], int lane)
+{
+#pragma unroll
+    for (int k = 2; k <= 512; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 8) {
+                const int lj = j >> 3;
+                const bool lower = (lane & lj) == 0;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const Key other = shfl_xor_key(key[t], lj);
+                    const bool up = ((lane * 8 + t) & k) == 0;
+                    key[t] = (lower == up) ? (key[t] < other ? key[t] : other) : (key[t] > other ? key[t] : other);
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    if (t & j) continue;
+                    const int t2 = t | j;
+                    const bool up = ((lane * 8 + t) & k) == 0;
+                    const Key a = key[t], b = key[t2];
+                    const Key lo = a < b ? a : b, hi = a < b ? b : a;
+                    key[t] = up ? lo : hi;
+                    key[t2] = up ? hi : lo;
+                }
+            }
+        }
+    }
+}
+
 template <bool CPLX, typename Key>
 __global__ void __launch_bounds__(256)
 k_sort512(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col_in,
@@ -885,33 +920,7 @@ k_sort512(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __rest
 #pragma unroll
         for (int t = 0; t < 8; ++t) key[t] = nkey[t];
         if (row + waves < dim) load(row + waves, nstart, nL, nkey);
-#pragma unroll
-        for (int k = 2; k <= 512; k <<= 1) {
-#pragma unroll
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                if (j >= 8) {
-                    const int lj = j >> 3;
-                    const bool lower = (lane & lj) == 0;
-#pragma unroll
-                    for (int t = 0; t < 8; ++t) {
-                        const Key other = shfl_xor_key(key[t], lj);
-                        const bool up = ((lane * 8 + t) & k) == 0;
-                        key[t] = (lower == up) ? (key[t] < other ? key[t] : other) : (key[t] > other ? key[t] : other);
-                    }
-                } else {
-#pragma unroll
-                    for (int t = 0; t < 8; ++t) {
-                        if (t & j) continue;
-                        const int t2 = t | j;
-                        const bool up = ((lane * 8 + t) & k) == 0;
-                        const Key a = key[t], b = key[t2];
-                        const Key lo = a < b ? a : b, hi = a < b ? b : a;
-                        key[t] = up ? lo : hi;
-                        key[t2] = up ? hi : lo;
-                    }
-                }
-            }
-        }
+        sort512_regs(key, lane);
         // through LDS to the striped order q = 64 t + lane, so that the stores of
         // a wavefront are contiguous
         Key* sk = s_key + (threadIdx.x >> 6) * 512;
@@ -929,6 +938,161 @@ k_sort512(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __rest
                 if (val64) val64[start + q] = v;
                 if (val32) val32[start + q] = to_f32(v);
                 if (code_out) code_out[start + q] = code_in[start + idx];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// The fill and the row sort in one pass, for rows of at most 512 entries and
+// columns below 2^23: one wavefront per row walks the candidates as k_fill_wave
+// does, but keeps all of the row's hops in LDS, ranks them 64 at a time into
+// keys (column << 9 | hop slot; the diagonal is slot n), sorts the 512 keys in
+// registers (sort512_regs) and writes the final column, FP64, FP32 and code
+// arrays in striped order.  Nothing is staged in HBM.  The values and codes
+// are those k_fill_wave stages, so the arrays equal the fill-then-sort result.
+// LDS: split tables, pair table, pair list, then per wave 512 hop words, 512
+// key words and the row's orbitals (16 words).
+constexpr int kSortedWaveWords = 512 + 512 + 16;
+static size_t fill_sorted_lds(int split_words, int kp_bytes, int N)
+{
+    const size_t kpw = (size_t)(kp_bytes + 3) / 4;
+    const size_t npair = (size_t)N * (N - 1) / 2;
+    return 4 * ((size_t)split_words + kpw + (npair * 2 + 3) / 4) + 4 * (size_t)kFillWaves * kSortedWaveWords;
+}
+
+template <bool CPLX, bool UPPER, bool LZ>
+__global__ void __launch_bounds__(64 * kFillWaves)
+k_fill_sorted(const uint64_t* __restrict__ states, int64_t dim, BuildArgs A, SplitArgs S,
+              const int32_t* __restrict__ split_blob, const int64_t* __restrict__ rowptr,
+              const typename VT<CPLX>::T* __restrict__ diag, const uint8_t* __restrict__ dkeep,
+              int32_t* __restrict__ col_out, typename VT<CPLX>::T* __restrict__ val64,
+              typename VT<CPLX>::F* __restrict__ val32, uint16_t* __restrict__ code_out,
+              unsigned int* __restrict__ err)
+{
+    using T = typename VT<CPLX>::T;
+    extern __shared__ int32_t lds[];
+    const int K = A.K, N = A.N;
+    const int kpw = (A.kpair_bytes + 3) / 4;
+    const int npair = N * (N - 1) / 2;
+    block_copy_words(lds, split_blob, S.words);
+    const uint8_t* s_kp = reinterpret_cast<const uint8_t*>(lds + S.words);
+    block_copy_words(lds + S.words, reinterpret_cast<const int32_t*>(A.kpair), kpw);
+    uint8_t* s_pair = reinterpret_cast<uint8_t*>(lds + S.words + kpw);
+    for (int P = threadIdx.x; P < npair; P += blockDim.x) {
+        int ia = 0, rem = P;
+        while (rem >= N - 1 - ia) {
+            rem -= N - 1 - ia;
+            ++ia;
+        }
+        s_pair[2 * P] = (uint8_t)ia;
+        s_pair[2 * P + 1] = (uint8_t)(ia + 1 + rem);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t* hops = reinterpret_cast<uint32_t*>(lds + S.words + kpw + (npair * 2 + 3) / 4) + wave * kSortedWaveWords;
+    uint32_t* keys = hops + 512;
+    uint8_t* s_orb = reinterpret_cast<uint8_t*>(keys + 512);
+    __syncthreads();
+    const T* P = reinterpret_cast<const T*>(A.pval);
+    const int nun = K - N;
+    const int ncand = npair * nun;
+    const uint32_t magic = nun > 0 ? (uint32_t)((0x100000000ull + nun - 1) / (uint64_t)nun) : 0u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave; row < dim; row += nw) {
+        const uint64_t s = states[row];
+        const int64_t start = rowptr[row];
+        const int L = (int)(rowptr[row + 1] - start);
+        if (lane < K) {
+            const int nb = __popcll(s & below);
+            if ((s >> lane) & 1ull) s_orb[nb] = (uint8_t)lane;
+            else s_orb[N + lane - nb] = (uint8_t)lane;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // the row's hops, in the reference's candidate order
+        int n = 0;
+        for (int base = 0; base < ncand; base += 64) {
+            const int qi = base + lane;
+            bool ok = false;
+            uint32_t h = 0;
+            if (qi < ncand) {
+                const int pi = (int)__umulhi((uint32_t)qi, magic);
+                const int ci = qi - pi * nun;
+                const int a = s_orb[s_pair[2 * pi]], b = s_orb[s_pair[2 * pi + 1]], c = s_orb[N + ci];
+                const int e = s_kp[((a * (2 * K - a - 1) / 2 - a - 1) + b) * K + c];
+                if (e != 0xFF && !(e & 0x80)) {
+                    const int d = e & 0x3F;
+                    if (!((s >> d) & 1ull)) {
+                        const uint64_t out = (s ^ (1ull << a) ^ (1ull << b)) | (1ull << c) | (1ull << d);
+                        ok = !(UPPER && out < s);
+                        h = (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16) | ((uint32_t)d << 24);
+                    }
+                }
+            }
+            const uint64_t m = __ballot(ok);
+            const int slot = n + __popcll(m & below);
+            if (ok && slot < 512) hops[slot] = h;
+            n += __popcll(m);
+        }
+        const bool has_diag = dkeep[row] != 0;
+        const int total = n + (has_diag ? 1 : 0);
+        if (total != L || total > 512) {  // the count pass saw another row: write nothing
+            if (lane == 0) atomicOr(err, total > 512 ? 4u : 2u);
+            __builtin_amdgcn_wave_barrier();
+            continue;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // keys: rank << 9 | slot; the diagonal is slot n; padding sorts last
+        for (int j = lane; j < 512; j += 64) {
+            uint32_t kj = 0xFFFFFFFFu;
+            if (j < n) {
+                const uint32_t h = hops[j];
+                const int a = h & 0xFF, b = (h >> 8) & 0xFF, c = (h >> 16) & 0xFF, d = h >> 24;
+                const uint64_t out = (s ^ (1ull << a) ^ (1ull << b)) | (1ull << c) | (1ull << d);
+                uint32_t r = split_rank<LZ>(lds, S, out);
+                if ((int64_t)r >= dim) {  // never a column outside the sector
+                    atomicOr(err, 1u);
+                    r = 0;
+                }
+                kj = (r << 9) | (uint32_t)j;
+            } else if (j == n && has_diag) {
+                kj = ((uint32_t)row << 9) | (uint32_t)j;
+            }
+            keys[j] = kj;
+        }
+        __builtin_amdgcn_wave_barrier();
+        uint32_t key[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) key[t] = keys[lane * 8 + t];
+        sort512_regs(key, lane);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int t = 0; t < 8; ++t) keys[lane * 8 + t] = key[t];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int q = t * 64 + lane;
+            if (q < total) {
+                const uint32_t kq = keys[q];
+                const int idx = (int)(kq & 511u);
+                T v;
+                uint16_t code;
+                if (idx == n) {
+                    v = diag[row];
+                    code = 0xFFFF;
+                } else {
+                    const uint32_t h = hops[idx];
+                    const int a = h & 0xFF, b = (h >> 8) & 0xFF, c = (h >> 16) & 0xFF, d = h >> 24;
+                    const int ti = ((a * (2 * K - a - 1) / 2 - a - 1) + b) * K + c;
+                    v = P[ti];
+                    const int par = hop_parity(s, a, b, c, d);
+                    if (par) v = vneg(v);
+                    code = code_out ? (uint16_t)((uint16_t)A.remap[ti] | (par ? 0x8000u : 0u)) : (uint16_t)0;
+                }
+                col_out[start + q] = (int32_t)(kq >> 9);
+                if (val64) val64[start + q] = v;
+                if (val32) val32[start + q] = to_f32(v);
+                if (code_out) code_out[start + q] = code;
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -1091,15 +1255,18 @@ static int set_lds(const void* fn, size_t lds)
 }
 
 // Row-walk kernel of the fill and of the matrix-free product: by default the
-// wavefront walk with split ranking where its tables fit, else the wavefront
-// walk with the rank table, else one thread per row.  HD_BUILD_FILL = "rank"
+// fill and sort in one pass (k_fill_sorted) where rows and tables allow, else
+// the wavefront walk with split ranking where its tables fit, else the
+// wavefront walk with the rank table, else one thread per row; the last three
+// stage their entries for k_sort512 / k_sort.  HD_BUILD_FILL = "split", "rank"
 // or "thread" forces the later choices (when they apply), so every kernel can
 // be checked against the others on the same matrices.
-enum class FillPath { kAuto, kRank, kThread };
+enum class FillPath { kAuto, kSplit, kRank, kThread };
 static FillPath fill_path()
 {
     const char* e = std::getenv("HD_BUILD_FILL");
     if (!e) return FillPath::kAuto;
+    if (!std::strcmp(e, "split")) return FillPath::kSplit;
     if (!std::strcmp(e, "rank")) return FillPath::kRank;
     if (!std::strcmp(e, "thread")) return FillPath::kThread;
     return FillPath::kAuto;
@@ -1241,7 +1408,7 @@ static int launch_fill(int grid, size_t lds, hipStream_t st, const hd_basis* b, 
     int n_cu = 256;
     HD_CHECK(device_props(&n_cu, nullptr));
     const FillPath path = fill_path();
-    if (d_split && A.N >= 2 && A.K > A.N && path == FillPath::kAuto) {
+    if (d_split && A.N >= 2 && A.K > A.N && (path == FillPath::kAuto || path == FillPath::kSplit)) {
         const size_t wlds = fill_wave_lds(S.words, A.kpair_bytes, A.N);
         HD_CHECK(set_lds((const void*)k_fill_wave<CPLX, UPPER, LZ, true>, wlds));
         const int wgrid = (int)std::max<int64_t>(
@@ -1413,7 +1580,9 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
             SplitArgs S{};
             std::vector<int32_t> split;
             const size_t wfix = fill_wave_lds(0, kp_bytes, b->N);
-            if (b->N >= 2 && K > b->N && wfix < 160 * 1024 && fill_path() == FillPath::kAuto &&
+            const FillPath mf_path = fill_path();
+            if (b->N >= 2 && K > b->N && wfix < 160 * 1024 &&
+                (mf_path == FillPath::kAuto || mf_path == FillPath::kSplit) &&
                 make_split_rank(b->rt, 160 * 1024 - wfix, S, split)) {
                 HD_TRY(dalloc_t(&h->d_mf_split, split.size(), "split rank tables"));
                 HD_TRYHIP(hipMemcpyAsync(h->d_mf_split, split.data(), split.size() * 4, hipMemcpyHostToDevice, st));
@@ -1434,6 +1603,88 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
     }
     // ---- fill
     const size_t nnz = (size_t)h->nnz;
+    // every entry array is padded by 8 zeroed entries: the SpMV reads aligned
+    // groups of 4 that may run past the last row
+    auto alloc_final = [&]() -> int {
+        int rc = dalloc_t(&h->d_col, nnz + 8, "columns");
+        if (rc != HD_OK) return rc;
+        HipErrors q;
+        HD_ENQ(q, hipMemsetAsync(h->d_col + nnz, 0, 8 * sizeof(int32_t), st));
+        h->bytes += nnz * 4;
+        if (value_mask & 1) {
+            if ((rc = dalloc(&h->d_val64, (nnz + 8) * sizeof(T), "FP64 values")) != HD_OK) return rc;
+            HD_ENQ(q, hipMemsetAsync(static_cast<T*>(h->d_val64) + nnz, 0, 8 * sizeof(T), st));
+            h->bytes += nnz * sizeof(T);
+        }
+        if (value_mask & 2) {
+            if ((rc = dalloc(&h->d_val32, (nnz + 8) * sizeof(F), "FP32 values")) != HD_OK) return rc;
+            HD_ENQ(q, hipMemsetAsync(static_cast<F*>(h->d_val32) + nnz, 0, 8 * sizeof(F), st));
+            h->bytes += nnz * sizeof(F);
+        }
+        if (coded) {
+            // padded by 8 entries so the SpMV's aligned 4-wide loads stay in bounds
+            if ((rc = dalloc_t(&h->d_code, nnz + 8, "value codes")) != HD_OK) return rc;
+            if ((rc = dalloc_t(reinterpret_cast<T**>(&h->d_dict), (size_t)std::max(D.ndict, 1), "value dictionary")) !=
+                HD_OK)
+                return rc;
+            HD_ENQ(q, hipMemcpyAsync(h->d_dict, D.dict.data(), D.dict.size() * 8, hipMemcpyHostToDevice, st));
+            HD_ENQ(q, hipMemsetAsync(h->d_code + nnz, 0xFF, 16, st));
+            h->ndict = D.ndict;
+            h->bytes += (nnz + 8) * 2 + (size_t)D.ndict * sizeof(T);
+        }
+        return q.status();
+    };
+    // fill and sort in one pass where the rows and the split tables allow
+    {
+        SplitArgs FS{};
+        std::vector<int32_t> fsplit;
+        const size_t ffix = fill_sorted_lds(0, A.kpair_bytes, A.N);
+        if (fill_path() == FillPath::kAuto && A.N >= 2 && K > A.N && h->max_row <= 512 &&
+            dim < (int64_t(1) << 23) && ffix < 160 * 1024 &&
+            make_split_rank(b->rt, 160 * 1024 - ffix, FS, fsplit)) {
+            HD_TRY(alloc_final());
+            HD_TRY(dalloc_t(&d_split, fsplit.size(), "split rank tables"));
+            HD_TRYHIP(hipMemcpyAsync(d_split, fsplit.data(), fsplit.size() * 4, hipMemcpyHostToDevice, st));
+            const size_t flds = fill_sorted_lds(FS.words, A.kpair_bytes, A.N);
+            const bool lz = b->rt.is_lz != 0;
+            const int fgrid = (int)std::max<int64_t>(
+                1, std::min<int64_t>((dim + kFillWaves - 1) / kFillWaves, (int64_t)n_cu * (flds > 80 * 1024 ? 1 : 2)));
+            unsigned int* d_err = reinterpret_cast<unsigned int*>(d_misc + 3);
+            T* v64 = reinterpret_cast<T*>(h->d_val64);
+            F* v32 = reinterpret_cast<F*>(h->d_val32);
+#define HD_FS(U, L)                                                                                            \
+    do {                                                                                                       \
+        HD_TRY(set_lds((const void*)k_fill_sorted<CPLX, U, L>, flds));                                         \
+        hipLaunchKernelGGL((k_fill_sorted<CPLX, U, L>), dim3(fgrid), dim3(64 * kFillWaves), flds, st,          \
+                           b->d_states, dim, A, FS, d_split, h->d_rowptr, d_diag, d_dkeep, h->d_col, v64, v32, \
+                           h->d_code, d_err);                                                                  \
+    } while (0)
+            ev.start(st);
+            if (A.upper && lz) HD_FS(true, true);
+            else if (A.upper) HD_FS(true, false);
+            else if (lz) HD_FS(false, true);
+            else HD_FS(false, false);
+#undef HD_FS
+            HD_TRYHIP(hipGetLastError());
+            h->t_fill = ev.stop(st);
+            h->t_sort = 0.0;
+            HD_TRYHIP(hipMemcpyAsync(host_misc, d_misc, sizeof(host_misc), hipMemcpyDeviceToHost, st));
+            HD_TRYHIP(hipStreamSynchronize(st));
+            HD_TRY(ev.status("build stage timing"));
+            if (host_misc[3] != 0) {
+                cleanup();
+                return fail(HD_ERR_UNSUPPORTED, "fill pass inconsistency (flags " +
+                                                    std::to_string(host_misc[3]) + ")");
+            }
+            if (coded) {
+                h->d_diag = d_diag;  // the coded format reads the diagonal from here
+                h->bytes += (size_t)dim * sizeof(T);
+                d_diag = nullptr;
+            }
+            cleanup();
+            return HD_OK;
+        }
+    }
     HD_TRY(dalloc_t(&d_ctmp, nnz, "unsorted columns"));
     HD_TRY(dalloc_t(&d_vtmp, nnz, "unsorted values"));
     if (coded) HD_TRY(dalloc_t(&d_codetmp, nnz, "unsorted codes"));
@@ -1481,30 +1732,7 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
     d_diag = nullptr;
     dfree(d_dkeep); d_dkeep = nullptr;
     // ---- sort into the final arrays
-    // every entry array is padded by 8 zeroed entries: the SpMV reads aligned
-    // groups of 4 that may run past the last row
-    HD_TRY(dalloc_t(&h->d_col, nnz + 8, "columns"));
-    HD_TRYHIP(hipMemsetAsync(h->d_col + nnz, 0, 8 * sizeof(int32_t), st));
-    h->bytes += nnz * 4;
-    if (value_mask & 1) {
-        HD_TRY(dalloc(&h->d_val64, (nnz + 8) * sizeof(T), "FP64 values"));
-        HD_TRYHIP(hipMemsetAsync(static_cast<T*>(h->d_val64) + nnz, 0, 8 * sizeof(T), st));
-        h->bytes += nnz * sizeof(T);
-    }
-    if (value_mask & 2) {
-        HD_TRY(dalloc(&h->d_val32, (nnz + 8) * sizeof(F), "FP32 values"));
-        HD_TRYHIP(hipMemsetAsync(static_cast<F*>(h->d_val32) + nnz, 0, 8 * sizeof(F), st));
-        h->bytes += nnz * sizeof(F);
-    }
-    if (coded) {
-        // padded by 8 entries so the SpMV's aligned 4-wide loads stay in bounds
-        HD_TRY(dalloc_t(&h->d_code, nnz + 8, "value codes"));
-        HD_TRY(dalloc_t(reinterpret_cast<T**>(&h->d_dict), (size_t)std::max(D.ndict, 1), "value dictionary"));
-        HD_TRYHIP(hipMemcpyAsync(h->d_dict, D.dict.data(), D.dict.size() * 8, hipMemcpyHostToDevice, st));
-        HD_TRYHIP(hipMemsetAsync(h->d_code + nnz, 0xFF, 16, st));
-        h->ndict = D.ndict;
-        h->bytes += (nnz + 8) * 2 + (size_t)D.ndict * sizeof(T);
-    }
+    HD_TRY(alloc_final());
     {
         const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(dim, (int64_t)n_cu * 32));
         ev.start(st);

@@ -1,9 +1,11 @@
 """Every row-walk kernel of the build and of the matrix-free product on the same
-matrices.  The default build fills rows one wavefront per row with split
-ranking; HD_BUILD_FILL=rank uses the wavefront walk with the rank table, and
-HD_BUILD_FILL=thread one thread per row (the kernels that run when the split
-or the rank table does not fit in LDS).  Each is compared with the oracle's
-CRS bit for bit; the matrix-free products with the stored CRS to 1e-13."""
+matrices.  The default build fills and sorts rows in one pass (one wavefront
+per row, split ranking, rows of up to 512 entries); HD_BUILD_FILL=split stages
+the wavefront walk's entries for the separate row sort, =rank uses the
+wavefront walk with the rank table, and =thread one thread per row (the
+kernels that run when rows are longer or the tables do not fit in LDS).  Each
+is compared with the oracle's CRS bit for bit; the matrix-free products with
+the stored CRS to 1e-13."""
 import sys
 
 import numpy as np
@@ -14,7 +16,7 @@ from hamiltonian_diagonalization_amd import models
 
 pytestmark = pytest.mark.gpu
 
-PATHS = ["auto", "rank", "thread"]
+PATHS = ["auto", "split", "rank", "thread"]
 
 
 def say(*a):
