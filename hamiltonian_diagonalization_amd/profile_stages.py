@@ -5,7 +5,7 @@ python -m hamiltonian_diagonalization_amd.profile_stages [--n 12 --two-s 33 --it
 
 Stages, in order, on the N=12, 2S=33, L_z=0 Coulomb sphere sector
 (dim 7,764,392): the basis (k_unrank), the full-storage build with FP64,
-FP32 and coded values (k_count, the scan, k_fill_wave, k_sort512), the tiled layout
+FP32 and coded values (k_count, the scan, k_fill_sorted), the tiled layout
 (k_tile_count, k_tile_fill, k_tile_pack for 3-byte entries, then its
 block-order autotune), `iters` tiled products with FP64 x and with FP32 x (one
 warm-up each), the amplitude network over the whole sector (k_mlp_infer and
@@ -44,11 +44,10 @@ def main():
     nnz = info["nnz"]
     out["build"] = {k: info[k] for k in ("nnz", "max_row", "t_count_ms", "t_scan_ms", "t_fill_ms", "t_sort_ms")}
     # per-stage algorithmic bytes: count reads the state and writes the row
-    # count, diagonal and flag; fill reads the state, row bounds, diagonal and
-    # flag and writes column, FP64 value and code per entry into staging; sort
-    # reads the staging and writes column, FP64, FP32 and code
-    out["build"]["alg_bytes"] = {"k_count": dim * (8 + 8 + 8 + 1), "k_fill": nnz * (4 + 8 + 2) + dim * (8 + 8 + 8 + 1),
-                                 "k_sort": nnz * ((4 + 8 + 2) + (4 + 8 + 4 + 2)) + dim * 8}
+    # count, diagonal and flag; the one-pass fill reads the state, row bounds,
+    # diagonal and flag and writes column, FP64, FP32 and code per entry
+    out["build"]["alg_bytes"] = {"k_count": dim * (8 + 8 + 8 + 1),
+                                 "k_fill_sorted": nnz * (4 + 8 + 4 + 2) + dim * (8 + 8 + 8 + 1)}
     tile = ham.tile()
     out["tile"] = tile
     # the fill writes 4-byte entries; with 3-byte entries k_tile_pack reads
