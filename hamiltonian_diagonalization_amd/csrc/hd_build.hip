@@ -1259,8 +1259,9 @@ static int set_lds(const void* fn, size_t lds)
 // the wavefront walk with split ranking where its tables fit, else the
 // wavefront walk with the rank table, else one thread per row; the last three
 // stage their entries for k_sort512 / k_sort.  HD_BUILD_FILL = "split", "rank"
-// or "thread" forces the later choices (when they apply), so every kernel can
-// be checked against the others on the same matrices.
+// or "thread" forces the later choices (when they apply), and with them the
+// 32-bit-key k_sort512, the 64-bit-key k_sort512 and the LDS k_sort<1024>, so
+// every kernel can be checked against the others on the same matrices.
 enum class FillPath { kAuto, kSplit, kRank, kThread };
 static FillPath fill_path()
 {
@@ -1739,7 +1740,12 @@ static int run_build(hd_hamiltonian* h, const hd_basis* b, const Derived& D, dou
         T* v64 = reinterpret_cast<T*>(h->d_val64);
         F* v32 = reinterpret_cast<F*>(h->d_val32);
         const dim3 g512((unsigned)std::max<int64_t>(1, std::min<int64_t>((dim + 3) / 4, (int64_t)n_cu * 32)));
-        if (h->max_row <= 512 && dim < (int64_t(1) << 23))
+        // HD_BUILD_FILL=rank takes the 64-bit keys and =thread the LDS sort, so
+        // that every sort kernel is checked on small matrices too
+        const FillPath sp = fill_path();
+        if (sp == FillPath::kThread && h->max_row <= 1024)
+            hipLaunchKernelGGL((k_sort<CPLX, 1024>), dim3(grid), dim3(64), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp, d_codetmp, h->d_col, v64, v32, h->d_code);
+        else if (h->max_row <= 512 && dim < (int64_t(1) << 23) && sp != FillPath::kRank)
             hipLaunchKernelGGL((k_sort512<CPLX, uint32_t>), g512, dim3(256), 0, st, dim, h->d_rowptr, d_ctmp, d_vtmp,
                                d_codetmp, h->d_col, v64, v32, h->d_code);
         else if (h->max_row <= 512)

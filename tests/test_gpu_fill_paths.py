@@ -1,11 +1,12 @@
-"""Every row-walk kernel of the build and of the matrix-free product on the same
-matrices.  The default build fills and sorts rows in one pass (one wavefront
-per row, split ranking, rows of up to 512 entries); HD_BUILD_FILL=split stages
-the wavefront walk's entries for the separate row sort, =rank uses the
-wavefront walk with the rank table, and =thread one thread per row (the
-kernels that run when rows are longer or the tables do not fit in LDS).  Each
-is compared with the oracle's CRS bit for bit; the matrix-free products with
-the stored CRS to 1e-13."""
+"""Every row-walk and row-sort kernel of the build and the matrix-free product
+kernels on the same matrices.  The default build fills and sorts rows in one
+pass (one wavefront per row, split ranking, rows of up to 512 entries);
+HD_BUILD_FILL=split stages the wavefront walk's entries for the register row
+sort with 32-bit keys, =rank uses the wavefront walk with the rank table and
+the 64-bit keys, and =thread one thread per row and the LDS row sort of up to
+1,024 entries (the kernels that run for longer rows, for dimensions from 2^23
+or when the tables do not fit in LDS).  Each is compared with the oracle's
+CRS bit for bit; the matrix-free products with the stored CRS to 1e-13."""
 import sys
 
 import numpy as np
@@ -92,3 +93,4 @@ def test_matrix_free_paths(monkeypatch, path):
         y_crs = hd.Hamiltonian(b, t, storage=hd.STORAGE_FULL, values=(hd.F64,)).spmv(x)
         y_free = hd.Hamiltonian(b, t, storage=hd.STORAGE_MATRIX_FREE).spmv(x)
         assert np.max(np.abs(y_free - y_crs)) <= 1e-13 * np.max(np.abs(y_crs))
+
