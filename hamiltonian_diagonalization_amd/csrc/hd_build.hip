@@ -843,48 +843,6 @@ k_sort(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restric
 // shuffles, shorter ones inside a lane.  A key is (column << S | position): 32
 // bits (S = 9) when every column is below 2^23, else 64 bits (S = 16).  The
 // columns of a row are distinct, so the result equals k_sort's.
-HD_DEV uint32_t shfl_xor_key(uint32_t v, int m) { return (uint32_t)__shfl_xor((int)v, m); }
-HD_DEV uint64_t shfl_xor_key(uint64_t v, int m)
-{
-    const int lo = __shfl_xor((int)(uint32_t)v, m);
-    const int hi = __shfl_xor((int)(uint32_t)(v >> 32), m);
-    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
-}
-
-// Ascending bitonic sort of a wavefront's 512 keys, lane l holding elements
-// 8l .. 8l+7 in registers: distances below 8 within a lane, the others
-// between lanes (shfl_xor).
-template <typename Key>
-HD_DEV void sort512_regs(Key (&key)[8], int lane)
-{
-#pragma unroll
-    for (int k = 2; k <= 512; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= 8) {
-                const int lj = j >> 3;
-                const bool lower = (lane & lj) == 0;
-#pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const Key other = shfl_xor_key(key[t], lj);
-                    const bool up = ((lane * 8 + t) & k) == 0;
-                    key[t] = (lower == up) ? (key[t] < other ? key[t] : other) : (key[t] > other ? key[t] : other);
-                }
-            } else {
-#pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    if (t & j) continue;
-                    const int t2 = t | j;
-                    const bool up = ((lane * 8 + t) & k) == 0;
-                    const Key a = key[t], b = key[t2];
-                    const Key lo = a < b ? a : b, hi = a < b ? b : a;
-                    key[t] = up ? lo : hi;
-                    key[t2] = up ? hi : lo;
-                }
-            }
-        }
-    }
-}
 
 template <bool CPLX, typename Key>
 __global__ void __launch_bounds__(256)

@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(kMergeThreads)
 k_gmerge(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col_in,
          const typename GT<CPLX>::T* __restrict__ val_in, double tol, int32_t* __restrict__ col_out,
          typename GT<CPLX>::T* __restrict__ val_out, int32_t* __restrict__ kept,
-         unsigned long long* __restrict__ ntrim)
+         unsigned long long* __restrict__ ntrim, int min_len)
 {
     using T = typename GT<CPLX>::T;
     __shared__ unsigned long long key[CAP];
@@ -220,6 +220,7 @@ k_gmerge(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restr
     for (int64_t row = blockIdx.x; row < dim; row += gridDim.x) {
         const int64_t start = rowptr[row];
         const int L = (int)(rowptr[row + 1] - start);
+        if (L <= min_len) continue;  // merged by k_gmerge_wave (uniform over the block)
         int n = 1;
         while (n < L) n <<= 1;
         for (int q = tid; q < n; q += kMergeThreads)
@@ -410,13 +411,93 @@ k_gcompact(int64_t dim, const int64_t* __restrict__ raw_ptr, const int64_t* __re
     }
 }
 
+// k_gmerge for the rows of at most 512 raw terms, one wavefront per row (longer
+// rows are skipped, for k_gmerge with min_len = 512): the keys
+// (column << 9 | position) sorted in registers (sort512_regs), then each lane
+// sums the runs that start among its 8 sorted keys, in term order from +0.0 as
+// k_gmerge does, and the kept sums are packed at the row's raw offset through a
+// wavefront prefix sum of the lanes' counts.
+template <bool CPLX>
+__global__ void __launch_bounds__(256)
+k_gmerge_wave(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col_in,
+              const typename GT<CPLX>::T* __restrict__ val_in, double tol, int32_t* __restrict__ col_out,
+              typename GT<CPLX>::T* __restrict__ val_out, int32_t* __restrict__ kept,
+              unsigned long long* __restrict__ ntrim)
+{
+    using T = typename GT<CPLX>::T;
+    __shared__ uint64_t s_key[4 * 512];  // one row's sorted keys per wavefront
+    const int lane = threadIdx.x & 63;
+    uint64_t* sk = s_key + (threadIdx.x >> 6) * 512;
+    unsigned long long trimmed = 0;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); row < dim; row += waves) {
+        const int64_t start = rowptr[row];
+        const int L = (int)(rowptr[row + 1] - start);
+        if (L > 512) continue;  // left to k_gmerge (uniform over the wavefront)
+        uint64_t key[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int q = lane * 8 + t;
+            key[t] = q < L ? (((uint64_t)(uint32_t)col_in[start + q] << 9) | (uint64_t)q) : ~0ull;
+        }
+        sort512_regs(key, lane);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) sk[lane * 8 + t] = key[t];
+        __builtin_amdgcn_wave_barrier();
+        T sum[8];
+        unsigned keep = 0;
+        int c = 0;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            sum[t] = gzero<T>();
+            const int i = lane * 8 + t;
+            if (i >= L) continue;
+            const int32_t col = (int32_t)(uint32_t)(key[t] >> 9);
+            if (col == kNoColumn) continue;
+            if (i > 0 && (int32_t)(uint32_t)(sk[i - 1] >> 9) == col) continue;  // not the head of its run
+            T acc = gzero<T>();
+            for (int q = i; q < L && (int32_t)(uint32_t)(sk[q] >> 9) == col; ++q)
+                acc = gadd(acc, val_in[start + (int64_t)(sk[q] & 511ull)]);
+            if (gabs(acc) < tol) {
+                ++trimmed;
+            } else {
+                sum[t] = acc;
+                keep |= 1u << t;
+                ++c;
+            }
+        }
+        int incl = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d);
+            if (lane >= d) incl += v;
+        }
+        if (lane == 63) kept[row] = incl;
+        int64_t dst = start + (incl - c);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            if (!(keep & (1u << t))) continue;
+            col_out[dst] = (int32_t)(uint32_t)(key[t] >> 9);
+            val_out[dst] = sum[t];
+            ++dst;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (trimmed) atomicAdd(ntrim, trimmed);
+}
+
+// Rows of at most 512 raw terms by k_gmerge_wave, the longer ones by
+// k_gmerge<CAP> (CAP >= the longest row).
 template <bool CPLX, int CAP>
 void launch_merge(int grid, hipStream_t st, int64_t dim, const int64_t* raw_ptr, const int32_t* rcol,
                   const typename GT<CPLX>::T* rval, double tol, int32_t* mcol, typename GT<CPLX>::T* mval,
                   int32_t* kept, unsigned long long* ntrim)
 {
-    hipLaunchKernelGGL((k_gmerge<CPLX, CAP>), dim3(grid), dim3(kMergeThreads), 0, st, dim, raw_ptr, rcol, rval, tol,
-                       mcol, mval, kept, ntrim);
+    hipLaunchKernelGGL((k_gmerge_wave<CPLX>), dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((dim + 3) / 4, 65536))),
+                       dim3(256), 0, st, dim, raw_ptr, rcol, rval, tol, mcol, mval, kept, ntrim);
+    if (CAP > 512)
+        hipLaunchKernelGGL((k_gmerge<CPLX, CAP>), dim3(grid), dim3(kMergeThreads), 0, st, dim, raw_ptr, rcol, rval,
+                           tol, mcol, mval, kept, ntrim, 512);
 }
 
 template <bool CPLX>
@@ -553,7 +634,7 @@ int run_general(hd_hamiltonian* h, const hd_basis* b, const hd_tables* t, double
     {
         const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(dim, (int64_t)n_cu * 8));
         ev.start(st);
-        if (raw_max <= 256) launch_merge<CPLX, 256>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
+        if (raw_max <= 512) launch_merge<CPLX, 512>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
         else if (raw_max <= 1024) launch_merge<CPLX, 1024>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
         else if (raw_max <= 4096) launch_merge<CPLX, 4096>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
         else if (raw_max <= 8192) launch_merge<CPLX, 8192>(grid, st, dim, d_raw, d_rcol, d_rval, tol, d_mcol, d_mval, d_counts, d_misc);
