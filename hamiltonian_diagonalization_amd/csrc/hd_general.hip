@@ -487,7 +487,9 @@ k_gmerge_wave(int64_t dim, const int64_t* __restrict__ rowptr, const int32_t* __
 }
 
 // Rows of at most 512 raw terms by k_gmerge_wave, the longer ones by
-// k_gmerge<CAP> (CAP >= the longest row).
+// k_gmerge<CAP> (CAP >= the longest row).  (A 512-thread block sorting rows of
+// up to 4,096 terms in registers was no faster than k_gmerge on them: 11.3
+// against 10.4 ms on the 6x5, N=8 hash tables.)
 template <bool CPLX, int CAP>
 void launch_merge(int grid, hipStream_t st, int64_t dim, const int64_t* raw_ptr, const int32_t* rcol,
                   const typename GT<CPLX>::T* rval, double tol, int32_t* mcol, typename GT<CPLX>::T* mval,

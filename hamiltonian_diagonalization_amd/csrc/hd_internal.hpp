@@ -323,6 +323,65 @@ HD_DEV void sort512_regs(Key (&key)[8], int lane)
     }
 }
 
+// Ascending bitonic sort of v[0, P) (P a power of two, P <= 8 * THREADS) by a
+// block of THREADS threads, in registers: thread t holds elements 8t .. 8t+7.
+// Stages of distance below 8 run within a thread, those below 512 between the
+// lanes of a wavefront (shfl_xor), and only the longer ones exchange through
+// v, with a barrier.  Elements from P to 8 * THREADS take part (as a separate
+// network) and must hold values that may move among themselves (all equal).
+template <typename Key, int THREADS>
+HD_DEV void bitonic_blocked(Key* v, int P)
+{
+    const int t = threadIdx.x;
+    Key a[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) a[e] = v[8 * t + e];
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j >= 8; j >>= 1) {
+            if (j >= 512) {
+                __syncthreads();
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[8 * t + e] = a[e];
+                __syncthreads();
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int i = 8 * t + e;
+                    const Key b = v[i ^ j];
+                    const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
+                    a[e] = keep_min ? (a[e] < b ? a[e] : b) : (a[e] < b ? b : a[e]);
+                }
+            } else {
+                const int d = j >> 3;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int i = 8 * t + e;
+                    const Key b = shfl_xor_key(a[e], d);
+                    const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
+                    a[e] = keep_min ? (a[e] < b ? a[e] : b) : (a[e] < b ? b : a[e]);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 4; j >= 1; j >>= 1) {
+            if (2 * j > k) continue;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                if (e & j) continue;
+                const bool asc = ((8 * t + e) & k) == 0;
+                const Key lo = a[e], hi = a[e | j];
+                if ((lo > hi) == asc) {
+                    a[e] = hi;
+                    a[e | j] = lo;
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[8 * t + e] = a[e];
+    __syncthreads();
+}
+
 // Cooperative copy of n 32-bit words into LDS by the whole block.
 HD_DEV void block_copy_words(int32_t* dst, const int32_t* src, int n)
 {

@@ -123,64 +123,6 @@ __device__ __forceinline__ int row_of(const TilePlanLds<TG>& S, int nrows, int64
     return lo;
 }
 
-// Ascending bitonic sort of v[0, P) (P a power of two, P <= 8 * kBuildThreads)
-// by the whole block, in registers: thread t holds elements 8t .. 8t+7.  Stages
-// of distance below 8 run within a thread, those below 512 between the lanes
-// of a wavefront (shfl_xor), and only the longer ones exchange through v, with
-// a barrier.  Elements from P to 8 * kBuildThreads take part (as a separate
-// network) and must hold values that may move among themselves: kEmpty.
-__device__ __forceinline__ void bitonic_blocked(uint32_t* v, int P)
-{
-    const int t = threadIdx.x;
-    uint32_t a[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) a[e] = v[8 * t + e];
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int j = k >> 1; j >= 8; j >>= 1) {
-            if (j >= 512) {
-                __syncthreads();
-#pragma unroll
-                for (int e = 0; e < 8; ++e) v[8 * t + e] = a[e];
-                __syncthreads();
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int i = 8 * t + e;
-                    const uint32_t b = v[i ^ j];
-                    const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
-                    a[e] = keep_min ? min(a[e], b) : max(a[e], b);
-                }
-            } else {
-                const int d = j >> 3;
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int i = 8 * t + e;
-                    const uint32_t b = (uint32_t)__shfl_xor((int)a[e], d);
-                    const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
-                    a[e] = keep_min ? min(a[e], b) : max(a[e], b);
-                }
-            }
-        }
-#pragma unroll
-        for (int j = 4; j >= 1; j >>= 1) {
-            if (2 * j > k) continue;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                if (e & j) continue;
-                const bool asc = ((8 * t + e) & k) == 0;
-                const uint32_t lo = a[e], hi = a[e | j];
-                if ((lo > hi) == asc) {
-                    a[e] = hi;
-                    a[e | j] = lo;
-                }
-            }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < 8; ++e) v[8 * t + e] = a[e];
-    __syncthreads();
-}
-
 // Builds the plan in LDS; S.flags != 0 when the group does not fit.
 // LS: log2 of the x values per 128-byte line (4 real, 3 complex).  QA: every
 // (chunk, row) block is padded to a multiple of 4 entries (the 3-byte layout,
@@ -244,7 +186,7 @@ __device__ __forceinline__ void tile_plan(TilePlanLds<TG>& S, int64_t dim, const
     int P = 1;
     while (P < L) P <<= 1;
     if (P <= 8 * kBuildThreads) {
-        bitonic_blocked(S.sorted, P);
+        bitonic_blocked<uint32_t, kBuildThreads>(S.sorted, P);
     } else {
         for (int k = 2; k <= P; k <<= 1)
             for (int jj = k >> 1; jj > 0; jj >>= 1) {
